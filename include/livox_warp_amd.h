@@ -1,0 +1,148 @@
+/*
+ * livox_warp_amd: MI355X (gfx950) scan-to-map odometry kernels behind a plain C ABI.
+ *
+ * The reference (nhaumann/livox-warp) has no C boundary on this path: its surface is the Python class
+ * livox_warp.odom.Odometry (python/livox_warp/odom.py:458-803), which launches Warp kernels. Each
+ * entry point below replaces one of those launches (or one host step) and cites it. The Python class
+ * that keeps the reference's call surface binds these entry points with ctypes.
+ *
+ * Conventions
+ *   - Every function returns 0 on success, a negative value for a HIP error (-(hipError_t)) or for a
+ *     bad argument (LW_EINVAL), and never throws.
+ *   - Point, attribute and timestamp pointers are DEVICE pointers, borrowed for the call. Matrices and
+ *     results are HOST memory.
+ *   - One context owns its map, its frame-sized work buffers and one HIP stream; all work is queued on
+ *     that stream. A context is driven by one thread at a time. Nothing is allocated after lw_create
+ *     (readbacks go through pinned buffers allocated there).
+ *   - Row-major 4x4 matrices: sensor-to-world poses, as in the reference.
+ */
+#ifndef LIVOX_WARP_AMD_H
+#define LIVOX_WARP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LW_EINVAL (-1000)
+#define LW_ENOTBOUND (-1001)
+#define LW_ACC_STRIDE 64
+
+typedef struct lw_ctx lw_ctx;
+
+/* Registration parameters: the Odometry tuning attributes of odom.py:499-525 that _register reads. */
+typedef struct lw_reg_params {
+    double damping;    /* Levenberg-Marquardt fraction of the diagonal (odom.py:512) */
+    double degen_thr;  /* degenerate-direction threshold, 0 = off (odom.py:516) */
+    double prior_info; /* pull toward the prediction, 0 = off (odom.py:517) */
+    float reg_voxel;   /* registration voxel edge, m (odom.py:501) */
+    float min_range;   /* thinning range filter, m (odom.py:507-508) */
+    float max_range;
+    float noise;       /* range noise sigma, m (odom.py:509) */
+    float alpha;       /* weight of loose directions (odom.py:511) */
+    int32_t ret_mask;  /* returns used (odom.py:510) */
+    int32_t max_iter;  /* odom.py:502 */
+    int32_t min_corr;  /* odom.py:506 */
+    int32_t wide_iters; /* iterations with the 5x5x5 search (odom.py:86) */
+    int32_t acc_mode;  /* 0: float64 accumulation; 1: compensated float32; 2: naive float32 */
+} lw_reg_params;
+
+/* Registration outcome: odom.py:797-803 plus the thinned count (odom.py:754-755). */
+typedef struct lw_reg_result {
+    double T[16];
+    double rms;
+    double cond;
+    int32_t ok;
+    int32_t iters;
+    int32_t corr;
+    int32_t n_ds;
+    int32_t degen;
+    int32_t weak;
+} lw_reg_result;
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int lw_version(void);
+
+/* Odometry.__init__ buffers (odom.py:461-497): a registration map of reg_slots slots (a power of two,
+ * odom.py:468) and frame-sized work buffers of frame_cap points. stream: a hipStream_t, or NULL for the
+ * default stream. The map starts empty (odom.py:527 -> reset). */
+int lw_create(lw_ctx **out, int64_t reg_slots, int64_t frame_cap, void *stream);
+int lw_destroy(lw_ctx *ctx);
+int lw_set_stream(lw_ctx *ctx, void *stream);
+/* Wait for everything queued on the context's stream (Odometry.synchronize, odom.py:534-537). */
+int lw_synchronize(lw_ctx *ctx);
+
+/* The caller-owned frame buffers the frame kernels fill (odom.py:477-481): f_raw and f_xyz are
+ * float32 x 3 per point, f_attr uint32, f_t float32, f_frame int32, each frame_cap long. */
+int lw_bind_frame(lw_ctx *ctx, float *f_raw, float *f_xyz, uint32_t *f_attr, float *f_t, int32_t *f_frame);
+
+/* Forget the map (odom.py:563-567). */
+int lw_map_reset(lw_ctx *ctx);
+/* f_count = 0 (odom.py:579, 589, 614, 636). */
+int lw_frame_reset(lw_ctx *ctx);
+
+/* k_frame_append (odom.py:97-123): append the staged points with k_min <= floor(max(t,0)*inv_dt) <=
+ * k_open to the open frame. Points keep their input order (Warp's serial CPU order); the count keeps
+ * counting past frame_cap, as the reference's does, and the surplus is dropped. */
+int lw_frame_append(lw_ctx *ctx, const float *sx, const uint32_t *sa, const float *st, int64_t n, float inv_dt,
+                    int32_t k_min, int32_t k_open);
+/* f_count read on the stream (odom.py:539-543 with odom.py:607, 645). */
+int lw_frame_count(lw_ctx *ctx, int32_t *out);
+
+/* k_deskew (odom.py:163-178, gpu.py:268-284): f_xyz[i] = Rot(theta s) f_raw[i] + rho s,
+ * s = clamp((f_t[i] - t_mid) * inv_dt, -1, 1); xi = (rho, theta). */
+int lw_deskew(lw_ctx *ctx, int32_t m, float t_mid, float inv_dt, const float *xi);
+
+/* Odometry._downsample (odom.py:731-737 with k_ds_insert, odom.py:126-160): keep the first point (in
+ * index order) of each thinning voxel of edge 1/inv_v in the sensor frame, after the range / tag /
+ * return filters; the kept points stay in index order. Writes the count to *n_ds (waits on the stream). */
+int lw_thin(lw_ctx *ctx, int32_t m, float inv_v, float min_range, float max_range, int32_t ret_mask,
+            int32_t *n_ds);
+/* Copy the first n thinned points (float32 x 3) to host memory. */
+int lw_thin_read(lw_ctx *ctx, float *out, int32_t n);
+/* Upload n points (float32 x 3, host) as the thinned set (tests and batch drivers). */
+int lw_thin_load(lw_ctx *ctx, const float *xyz, int32_t n);
+
+/* k_reg_insert (odom.py:186-230), Odometry._insert (odom.py:719-727): add n points posed by T to the
+ * map. xyz/attr are device pointers, or NULL for the bound frame buffers f_xyz / f_attr. A voxel stops
+ * accumulating at 4096 points; exactly its first points in index order contribute. */
+int lw_map_insert(lw_ctx *ctx, const float *xyz, const uint32_t *attr, int64_t n, const float *T, float inv_v,
+                  float voxel, float max_range, int32_t ret_mask);
+/* reg_stats (odom.py:474, read at odom.py:698): out[0] voxels claimed, out[1] points without a slot. */
+int lw_map_stats(lw_ctx *ctx, int32_t *out);
+/* The map in the reference's per-slot layout (odom.py:470-473), copied to host memory: keys[cap],
+ * rsum[cap*3], router[cap*9], rcnt[cap]. Any pointer may be NULL. */
+int lw_map_read(lw_ctx *ctx, int64_t *keys, float *rsum, float *router, int32_t *rcnt);
+/* Slot of a key, or -1 (gpu.find_slot, gpu.py:254-265), evaluated on the device for n keys. */
+int lw_map_find(lw_ctx *ctx, const int64_t *keys_host, int32_t n, int32_t *slots_host);
+
+/* One k_icp + k_reduce pass (odom.py:275-384, Odometry._accumulate odom.py:739-750) over the n thinned
+ * points at pose T (float32 4x4): out receives the 64 reduced float64 entries of the ACC layout
+ * (odom.py:72-82). best_slot (host, may be NULL) receives each point's chosen slot or -1. */
+int lw_icp_accum(lw_ctx *ctx, int32_t n, const float *T, float inv_v, float voxel, int32_t reach, float max_d2,
+                 float noise2, float tight2, float alpha, float kappa, int32_t acc_mode, double *out,
+                 int32_t *best_slot);
+
+/* Odometry._register (odom.py:752-803): thin the m deskewed frame points, then run the Gauss-Newton
+ * loop on the device -- accumulation, float64 host step (odom.py:387-455) and convergence test -- with
+ * one readback at the end. T_pred: the prediction (float64 4x4). */
+int lw_register(lw_ctx *ctx, int32_t m, const lw_reg_params *p, const double *T_pred, lw_reg_result *res);
+/* The same loop over the already-thinned points (lw_thin / lw_thin_load), n of them. */
+int lw_register_thinned(lw_ctx *ctx, int32_t n, const lw_reg_params *p, const double *T_pred, lw_reg_result *res);
+
+/* Host builds of the device's float64 step and per-voxel precision, for tests on machines without a
+ * GPU (same source as the device code). */
+int lw_host_gn_step(const double *v, const double *T, const double *T_pred, double damping, double prior_info,
+                    double degen_thr, int32_t min_corr, double *delta, double *rms, int32_t *n_degen);
+int lw_host_voxel_precision(const float *rsum, const float *router6, int32_t c, float voxel, float noise2,
+                            float tight2, float alpha, float *M6, float *Mg6, float *q3);
+int lw_host_apply_twist(double *T, const double *delta);
+int lw_host_twist_between(const double *T_from, const double *T_to, double *xi);
+double lw_host_cond(const double *H);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,17 @@
+"""livox-warp_amd: the scan-to-map odometry hot path of livox-warp, rebuilt for MI355X (gfx950).
+
+Import with importlib (the directory name carries a hyphen):
+
+    lw = importlib.import_module("livox-warp_amd")
+    odo = lw.Odometry(lw.PoseTable())
+
+Submodules: odom (the drop-in Odometry), engine (one context driven directly), pose_table (the pose sink),
+synth (seeded synthetic scenes, scans and walks), _abi (ctypes bindings of liblwamd.so).
+"""
+
+from . import _abi, engine, odom, pose_table, synth  # noqa: F401
+from .engine import Engine  # noqa: F401
+from .odom import Odometry  # noqa: F401
+from .pose_table import PoseTable  # noqa: F401
+
+__all__ = ["Odometry", "Engine", "PoseTable", "odom", "engine", "pose_table", "synth"]

@@ -1,0 +1,104 @@
+"""ctypes bindings of liblwamd.so (include/livox_warp_amd.h).
+
+The library is built in-tree (`make -C livox-warp_amd`, or `__graft_entry__.build()`); loading fails loudly
+when it is missing -- there is no other implementation of these kernels in the product.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblwamd.so")
+
+LW_EINVAL = -1000
+LW_ENOTBOUND = -1001
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_f32 = ctypes.c_float
+_f64 = ctypes.c_double
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+class RegParams(ctypes.Structure):
+    _fields_ = [("damping", _f64), ("degen_thr", _f64), ("prior_info", _f64), ("reg_voxel", _f32),
+                ("min_range", _f32), ("max_range", _f32), ("noise", _f32), ("alpha", _f32), ("ret_mask", _i32),
+                ("max_iter", _i32), ("min_corr", _i32), ("wide_iters", _i32), ("acc_mode", _i32)]
+
+
+class RegResult(ctypes.Structure):
+    _fields_ = [("T", _f64 * 16), ("rms", _f64), ("cond", _f64), ("ok", _i32), ("iters", _i32), ("corr", _i32),
+                ("n_ds", _i32), ("degen", _i32), ("weak", _i32)]
+
+
+# name -> (restype, argtypes); every symbol include/livox_warp_amd.h declares
+SIGNATURES = {
+    "lw_version": (_i32, []),
+    "lw_create": (_i32, [ctypes.POINTER(_vp), _i64, _i64, _vp]),
+    "lw_destroy": (_i32, [_vp]),
+    "lw_set_stream": (_i32, [_vp, _vp]),
+    "lw_synchronize": (_i32, [_vp]),
+    "lw_bind_frame": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "lw_map_reset": (_i32, [_vp]),
+    "lw_frame_reset": (_i32, [_vp]),
+    "lw_frame_append": (_i32, [_vp, _vp, _vp, _vp, _i64, _f32, _i32, _i32]),
+    "lw_frame_count": (_i32, [_vp, _i32p]),
+    "lw_deskew": (_i32, [_vp, _i32, _f32, _f32, _f32p]),
+    "lw_thin": (_i32, [_vp, _i32, _f32, _f32, _f32, _i32, _i32p]),
+    "lw_thin_read": (_i32, [_vp, _f32p, _i32]),
+    "lw_thin_load": (_i32, [_vp, _f32p, _i32]),
+    "lw_map_insert": (_i32, [_vp, _vp, _vp, _i64, _f32p, _f32, _f32, _f32, _i32]),
+    "lw_map_stats": (_i32, [_vp, _i32p]),
+    "lw_map_read": (_i32, [_vp, _i64p, _f32p, _f32p, _i32p]),
+    "lw_map_find": (_i32, [_vp, _i64p, _i32, _i32p]),
+    "lw_icp_accum": (_i32, [_vp, _i32, _f32p, _f32, _f32, _i32, _f32, _f32, _f32, _f32, _f32, _i32, _f64p, _i32p]),
+    "lw_register": (_i32, [_vp, _i32, ctypes.POINTER(RegParams), _f64p, ctypes.POINTER(RegResult)]),
+    "lw_register_thinned": (_i32, [_vp, _i32, ctypes.POINTER(RegParams), _f64p, ctypes.POINTER(RegResult)]),
+    "lw_host_gn_step": (_i32, [_f64p, _f64p, _f64p, _f64, _f64, _f64, _i32, _f64p, _f64p, _i32p]),
+    "lw_host_voxel_precision": (_i32, [_f32p, _f32p, _i32, _f32, _f32, _f32, _f32, _f32p, _f32p, _f32p]),
+    "lw_host_apply_twist": (_i32, [_f64p, _f64p]),
+    "lw_host_twist_between": (_i32, [_f64p, _f64p, _f64p]),
+    "lw_host_cond": (_f64, [_f64p]),
+}
+
+_lib = None
+
+
+class LwError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LwError(f"{LIB_PATH} is missing: build it with `make -C {HERE}` (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        if rc == LW_EINVAL:
+            raise LwError(f"{what}: invalid argument")
+        if rc == LW_ENOTBOUND:
+            raise LwError(f"{what}: frame buffers not bound")
+        raise LwError(f"{what}: HIP error {-rc}")
+
+
+def ptr(a, t):
+    """ctypes pointer of a C-contiguous numpy array."""
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(t)

@@ -1,0 +1,420 @@
+// The C ABI of include/livox_warp_amd.h: context lifecycle, argument checks, staging and readbacks.
+// Kernels live in lw_map.hip, lw_frame.hip and lw_icp.hip.
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/livox_warp_amd.h"
+#include "lw_internal.h"
+#include "lw_prec.h"
+
+#define LW_CK(x)                                        \
+    do {                                                \
+        hipError_t e_ = (x);                            \
+        if (e_ != hipSuccess) return -(int)e_;          \
+    } while (0)
+
+namespace {
+
+constexpr size_t PIN_BYTES = 1 << 20;
+constexpr size_t PIN_UP = 0;          // uploads (poses)
+constexpr size_t PIN_DOWN = 1 << 16;  // readbacks
+
+__global__ void k_fill_i64(int64_t *p, int64_t n, int64_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_set_tf(lw::RegState *st, const float *T) {
+    if (threadIdx.x < 16) st->Tf[threadIdx.x] = T[threadIdx.x];
+    if (threadIdx.x == 0) {
+        st->done = 0;
+        st->ticket = 0;
+    }
+}
+
+template <typename T>
+hipError_t dalloc(T **p, size_t count) {
+    return hipMalloc((void **)p, count * sizeof(T) + 64);
+}
+
+bool pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+
+void free_all(lw_ctx *c) {
+    void *ptrs[] = {c->slots, c->occ, c->vox_key, c->vox_mom, c->vox_mu, c->vox_prec, c->vox_nin, c->vox_quota,
+                    c->vox_dirty, c->dirty_list, c->touched, c->ctr, c->pt_slot, c->pt_vid, c->pt_take,
+                    c->ds_keys, c->ds_idx, c->ds_xyz, c->blk_cnt, c->blk_off, c->reg, c->partials, c->best_slot,
+                    c->dtmp};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->pin) (void)hipHostFree(c->pin);
+}
+
+char *pin_up(lw_ctx *c) { return (char *)c->pin + PIN_UP; }
+char *pin_down(lw_ctx *c) { return (char *)c->pin + PIN_DOWN; }
+
+// The per-voxel precision must match the query's parameters: refresh what the inserts queued, or every
+// voxel when the parameters changed.
+hipError_t ensure_precision(lw_ctx *c, float voxel, float noise2, float tight2, float alpha) {
+    bool same = c->prec_valid && c->prec_voxel == voxel && c->prec_noise2 == noise2 && c->prec_tight2 == tight2 &&
+                c->prec_alpha == alpha;
+    hipError_t e = lw::launch_refresh(c, !same, voxel, noise2, tight2, alpha);
+    c->prec_valid = true;
+    c->prec_voxel = voxel;
+    c->prec_noise2 = noise2;
+    c->prec_tight2 = tight2;
+    c->prec_alpha = alpha;
+    return e;
+}
+
+int read_ctr(lw_ctx *c, int idx, int32_t *out) {
+    LW_CK(hipMemcpyAsync(pin_down(c), &c->ctr[idx], sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    LW_CK(hipStreamSynchronize(c->stream));
+    std::memcpy(out, pin_down(c), sizeof(int32_t));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lw_version(void) { return 100; }
+
+int lw_create(lw_ctx **out, int64_t reg_slots, int64_t frame_cap, void *stream) {
+    if (!out || !pow2(reg_slots) || reg_slots < 64 || reg_slots > (1ll << 30) || frame_cap <= 0 ||
+        frame_cap > (1ll << 26))
+        return LW_EINVAL;
+    lw_ctx *c = new (std::nothrow) lw_ctx();
+    if (!c) return LW_EINVAL;
+    int rc = 0;
+    auto fail = [&](hipError_t e) {
+        free_all(c);
+        delete c;
+        return -(int)e;
+    };
+    hipError_t e;
+    if ((e = hipGetDevice(&c->device)) != hipSuccess) return fail(e);
+    c->stream = (hipStream_t)stream;
+    c->cap = reg_slots;
+    c->mask = (int32_t)(reg_slots - 1);
+    c->vox_cap = reg_slots;
+    c->occ_words = (reg_slots + 31) / 32;
+    c->frame_cap = frame_cap;
+    int64_t ds = 1;
+    while (ds < frame_cap) ds <<= 1;  // odom.py:485
+    c->ds_cap = ds;
+    c->max_blocks = (frame_cap + lw::BLOCK - 1) / lw::BLOCK;
+    if ((e = dalloc(&c->slots, c->cap)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->occ, c->occ_words)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->vox_key, c->vox_cap)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->vox_mom, c->vox_cap)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->vox_mu, c->vox_cap)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->vox_prec, c->vox_cap)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->vox_nin, c->vox_cap)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->vox_quota, c->vox_cap)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->vox_dirty, c->vox_cap)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->dirty_list, c->vox_cap)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->touched, frame_cap)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->ctr, 64)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->pt_slot, frame_cap)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->pt_vid, frame_cap)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->pt_take, frame_cap)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->ds_keys, c->ds_cap)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->ds_idx, c->ds_cap)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->ds_xyz, 3 * frame_cap)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->blk_cnt, c->max_blocks)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->blk_off, c->max_blocks)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->reg, 1)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->partials, (size_t)lw::ICP_MAX_GROUPS * 128)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->best_slot, frame_cap)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->dtmp, frame_cap + 64)) != hipSuccess) return fail(e);
+    if ((e = hipHostMalloc(&c->pin, PIN_BYTES, hipHostMallocDefault)) != hipSuccess) return fail(e);
+    c->pin_bytes = PIN_BYTES;
+    hipLaunchKernelGGL(k_fill_i64, dim3(1024), dim3(256), 0, c->stream, c->ds_keys, c->ds_cap, (int64_t)-1);
+    if ((e = lw::launch_fill_i32(c, c->ds_idx, c->ds_cap, INT_MAX)) != hipSuccess) return fail(e);
+    if ((e = lw::launch_fill_i32(c, c->ctr, 64, 0)) != hipSuccess) return fail(e);
+    if ((e = lw::launch_map_reset(c)) != hipSuccess) return fail(e);
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return fail(e);
+    (void)rc;
+    *out = c;
+    return 0;
+}
+
+int lw_destroy(lw_ctx *c) {
+    if (!c) return LW_EINVAL;
+    (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+    return 0;
+}
+
+int lw_set_stream(lw_ctx *c, void *stream) {
+    if (!c) return LW_EINVAL;
+    LW_CK(hipStreamSynchronize(c->stream));
+    c->stream = (hipStream_t)stream;
+    return 0;
+}
+
+int lw_synchronize(lw_ctx *c) {
+    if (!c) return LW_EINVAL;
+    LW_CK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int lw_bind_frame(lw_ctx *c, float *f_raw, float *f_xyz, uint32_t *f_attr, float *f_t, int32_t *f_frame) {
+    if (!c || !f_raw || !f_xyz || !f_attr || !f_t || !f_frame) return LW_EINVAL;
+    c->f_raw = f_raw;
+    c->f_xyz = f_xyz;
+    c->f_attr = f_attr;
+    c->f_t = f_t;
+    c->f_frame = f_frame;
+    return 0;
+}
+
+int lw_map_reset(lw_ctx *c) {
+    if (!c) return LW_EINVAL;
+    LW_CK(lw::launch_map_reset(c));
+    c->prec_valid = false;
+    return 0;
+}
+
+int lw_frame_reset(lw_ctx *c) {
+    if (!c) return LW_EINVAL;
+    LW_CK(lw::launch_fill_i32(c, &c->ctr[6], 1, 0));
+    return 0;
+}
+
+int lw_frame_append(lw_ctx *c, const float *sx, const uint32_t *sa, const float *st, int64_t n, float inv_dt,
+                    int32_t k_min, int32_t k_open) {
+    if (!c || n < 0 || (n > 0 && (!sx || !sa || !st))) return LW_EINVAL;
+    if (!c->f_raw) return LW_ENOTBOUND;
+    const int64_t chunk = c->max_blocks * (int64_t)lw::BLOCK;
+    for (int64_t a = 0; a < n; a += chunk) {  // chunks in order keep the points in input order
+        int32_t m = (int32_t)((n - a) < chunk ? (n - a) : chunk);
+        LW_CK(lw::launch_frame_append(c, sx + 3 * a, sa + a, st + a, m, inv_dt, k_min, k_open));
+    }
+    return 0;
+}
+
+int lw_frame_count(lw_ctx *c, int32_t *out) {
+    if (!c || !out) return LW_EINVAL;
+    return read_ctr(c, 6, out);
+}
+
+int lw_deskew(lw_ctx *c, int32_t m, float t_mid, float inv_dt, const float *xi) {
+    if (!c || !xi || m < 0 || m > c->frame_cap) return LW_EINVAL;
+    if (!c->f_raw) return LW_ENOTBOUND;
+    LW_CK(lw::launch_deskew(c, m, t_mid, inv_dt, xi));
+    return 0;
+}
+
+int lw_thin(lw_ctx *c, int32_t m, float inv_v, float min_range, float max_range, int32_t ret_mask, int32_t *n_ds) {
+    if (!c || !n_ds || m < 0 || m > c->frame_cap) return LW_EINVAL;
+    if (!c->f_raw) return LW_ENOTBOUND;
+    LW_CK(lw::launch_thin(c, m, inv_v, min_range, max_range, ret_mask));
+    return read_ctr(c, 5, n_ds);
+}
+
+int lw_thin_read(lw_ctx *c, float *out, int32_t n) {
+    if (!c || !out || n < 0 || n > c->frame_cap) return LW_EINVAL;
+    LW_CK(hipMemcpyAsync(out, c->ds_xyz, 12 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    LW_CK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int lw_thin_load(lw_ctx *c, const float *xyz, int32_t n) {
+    if (!c || n < 0 || n > c->frame_cap || (n > 0 && !xyz)) return LW_EINVAL;
+    LW_CK(hipMemcpyAsync(c->ds_xyz, xyz, 12 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    LW_CK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int lw_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, int64_t n, const float *T, float inv_v,
+                  float voxel, float max_range, int32_t ret_mask) {
+    if (!c || !T || n < 0) return LW_EINVAL;
+    if (!xyz && !attr) {
+        if (!c->f_xyz) return LW_ENOTBOUND;
+        if (n > c->frame_cap) return LW_EINVAL;
+        xyz = c->f_xyz;
+        attr = c->f_attr;
+    } else if (!xyz || !attr) {
+        return LW_EINVAL;
+    }
+    lw::Mat4f M;
+    std::memcpy(M.m, T, sizeof(M.m));
+    for (int64_t a = 0; a < n; a += c->frame_cap) {  // in index order, so the cap keeps the first points
+        int32_t m = (int32_t)((n - a) < c->frame_cap ? (n - a) : c->frame_cap);
+        LW_CK(lw::launch_map_insert(c, xyz + 3 * a, attr + a, m, M, inv_v, voxel, max_range, ret_mask));
+    }
+    return 0;
+}
+
+int lw_map_stats(lw_ctx *c, int32_t *out) {
+    if (!c || !out) return LW_EINVAL;
+    LW_CK(hipMemcpyAsync(pin_down(c), c->ctr, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    LW_CK(hipStreamSynchronize(c->stream));
+    std::memcpy(out, pin_down(c), 2 * sizeof(int32_t));
+    return 0;
+}
+
+int lw_map_read(lw_ctx *c, int64_t *keys, float *rsum, float *router, int32_t *rcnt) {
+    if (!c) return LW_EINVAL;
+    int32_t nvox = 0;
+    int rc = read_ctr(c, 2, &nvox);
+    if (rc) return rc;
+    if (nvox > c->vox_cap) nvox = (int32_t)c->vox_cap;
+    std::vector<lw::Slot> slots(c->cap);
+    std::vector<lw::VoxMom> mom(nvox > 0 ? nvox : 1);
+    LW_CK(hipMemcpy(slots.data(), c->slots, c->cap * sizeof(lw::Slot), hipMemcpyDeviceToHost));
+    if (nvox > 0) LW_CK(hipMemcpy(mom.data(), c->vox_mom, nvox * sizeof(lw::VoxMom), hipMemcpyDeviceToHost));
+    const int map3[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+    for (int64_t s = 0; s < c->cap; ++s) {
+        const lw::Slot &sl = slots[s];
+        bool have = sl.key != -1 && sl.vid >= 0 && sl.vid < nvox;
+        if (keys) keys[s] = sl.key;
+        const lw::VoxMom *m = have ? &mom[sl.vid] : nullptr;
+        if (rsum)
+            for (int a = 0; a < 3; ++a) rsum[3 * s + a] = m ? m->s[a] : 0.0f;
+        if (router)
+            for (int a = 0; a < 9; ++a) router[9 * s + a] = m ? m->o[map3[a]] : 0.0f;
+        if (rcnt) rcnt[s] = m ? m->cnt : 0;
+    }
+    return 0;
+}
+
+int lw_map_find(lw_ctx *c, const int64_t *keys_host, int32_t n, int32_t *slots_host) {
+    if (!c || n < 0 || n > c->frame_cap || (n > 0 && (!keys_host || !slots_host))) return LW_EINVAL;
+    if (n == 0) return 0;
+    LW_CK(hipMemcpyAsync(c->dtmp, keys_host, 8 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    LW_CK(lw::launch_map_find(c, c->dtmp, n, c->pt_slot));
+    LW_CK(hipMemcpyAsync(slots_host, c->pt_slot, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    LW_CK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int lw_icp_accum(lw_ctx *c, int32_t n, const float *T, float inv_v, float voxel, int32_t reach, float max_d2,
+                 float noise2, float tight2, float alpha, float kappa, int32_t acc_mode, double *out,
+                 int32_t *best_slot) {
+    if (!c || !T || !out || n < 0 || n > c->frame_cap || reach < 0 || reach > 8 || acc_mode < 0 || acc_mode > 2)
+        return LW_EINVAL;
+    LW_CK(ensure_precision(c, voxel, noise2, tight2, alpha));
+    std::memcpy(pin_up(c), T, 16 * sizeof(float));
+    LW_CK(hipMemcpyAsync(c->dtmp, pin_up(c), 16 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_set_tf, dim3(1), dim3(64), 0, c->stream, c->reg, (const float *)c->dtmp);
+    LW_CK(hipGetLastError());
+    lw::IcpPass p{};
+    p.n = n;
+    p.reach = reach;
+    p.inv_v = inv_v;
+    p.voxel = voxel;
+    p.max_d2 = max_d2;
+    p.kappa = kappa;
+    p.it = 0;
+    p.gn = 0;
+    p.want_slots = best_slot != nullptr;
+    LW_CK(lw::launch_icp_pass(c, p, acc_mode));
+    LW_CK(hipMemcpyAsync(pin_down(c), c->reg->acc, 64 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (best_slot && n > 0)
+        LW_CK(hipMemcpyAsync(best_slot, c->best_slot, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    LW_CK(hipStreamSynchronize(c->stream));
+    std::memcpy(out, pin_down(c), 64 * sizeof(double));
+    return 0;
+}
+
+int lw_register_thinned(lw_ctx *c, int32_t n, const lw_reg_params *p, const double *T_pred, lw_reg_result *res) {
+    if (!c || !p || !T_pred || !res || n < 0 || n > c->frame_cap || p->max_iter < 0 || p->acc_mode < 0 ||
+        p->acc_mode > 2 || !(p->reg_voxel > 0.0f))
+        return LW_EINVAL;
+    std::memset(res, 0, sizeof(*res));
+    std::memcpy(res->T, T_pred, 16 * sizeof(double));
+    res->n_ds = n;
+    res->cond = 1.0;
+    if (n < p->min_corr) return 0;  // odom.py:756-757
+    const double voxel = p->reg_voxel;
+    const float noise2 = (float)((double)p->noise * (double)p->noise);
+    const double tf = 0.6 * voxel / std::sqrt(12.0);  // TIGHT_FRACTION (odom.py:89, 761)
+    const float tight2 = (float)(tf * tf);
+    LW_CK(ensure_precision(c, (float)voxel, noise2, tight2, p->alpha));
+    std::memcpy(pin_up(c), T_pred, 16 * sizeof(double));
+    LW_CK(hipMemcpyAsync(c->dtmp, pin_up(c), 16 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    LW_CK(lw::launch_reg_begin(c, (const double *)c->dtmp));
+    lw::IcpPass q{};
+    q.n = n;
+    q.inv_v = (float)(1.0 / voxel);
+    q.voxel = (float)voxel;
+    q.gn = 1;
+    q.sp.damping = p->damping;
+    q.sp.prior_info = p->prior_info;
+    q.sp.degen_thr = p->degen_thr;
+    q.sp.min_corr = p->min_corr;
+    for (int it = 0; it < p->max_iter; ++it) {  // odom.py:764-796
+        q.it = it;
+        q.reach = it < p->wide_iters ? 2 : 1;
+        const double md = (q.reach + 0.5) * voxel;
+        q.max_d2 = (float)(md * md);
+        double kap = 6.0 * std::pow(0.7, it);
+        q.kappa = (float)(kap > 2.0 ? kap : 2.0);
+        LW_CK(lw::launch_icp_pass(c, q, p->acc_mode));
+    }
+    LW_CK(lw::launch_reg_finish(c, p->min_corr));
+    LW_CK(hipMemcpyAsync(pin_down(c), c->reg, sizeof(lw::RegState), hipMemcpyDeviceToHost, c->stream));
+    LW_CK(hipStreamSynchronize(c->stream));
+    lw::RegState st;
+    std::memcpy(&st, pin_down(c), sizeof(st));
+    std::memcpy(res->T, st.T, 16 * sizeof(double));
+    res->rms = st.rms;
+    res->cond = st.cond;
+    res->corr = st.corr;
+    res->iters = st.iters;
+    res->ok = st.corr >= p->min_corr ? 1 : 0;
+    res->degen = st.n_degen;
+    res->weak = (st.cond < 1e-3 || st.n_degen > 0) ? 1 : 0;  // WEAK_COND (odom.py:90, 801)
+    return 0;
+}
+
+int lw_register(lw_ctx *c, int32_t m, const lw_reg_params *p, const double *T_pred, lw_reg_result *res) {
+    if (!c || !p || !T_pred || !res || m < 0 || m > c->frame_cap) return LW_EINVAL;
+    if (!c->f_raw) return LW_ENOTBOUND;
+    int32_t n_ds = 0;
+    int rc = lw_thin(c, m, (float)(2.0 / (double)p->reg_voxel), p->min_range, p->max_range, p->ret_mask, &n_ds);
+    if (rc) return rc;
+    return lw_register_thinned(c, n_ds, p, T_pred, res);
+}
+
+// ---- host builds of the device math (tests) ---------------------------------------------------------
+
+int lw_host_gn_step(const double *v, const double *T, const double *T_pred, double damping, double prior_info,
+                    double degen_thr, int32_t min_corr, double *delta, double *rms, int32_t *n_degen) {
+    if (!v || !T || !T_pred || !delta || !rms || !n_degen) return LW_EINVAL;
+    lw::gn::StepParams P{damping, prior_info, degen_thr, min_corr, 0};
+    lw::gn::StepOut o;
+    int rc = lw::gn::step(v, T, T_pred, P, o);
+    std::memcpy(delta, o.delta, sizeof(o.delta));
+    *rms = o.rms;
+    *n_degen = o.n_degen;
+    return rc;
+}
+
+int lw_host_voxel_precision(const float *rsum, const float *router6, int32_t c, float voxel, float noise2,
+                            float tight2, float alpha, float *M6, float *Mg6, float *q3) {
+    if (!rsum || !router6 || !M6 || !Mg6 || !q3 || c <= 0) return LW_EINVAL;
+    return lw::voxel_precision(rsum, router6, c, voxel, noise2, tight2, alpha, M6, Mg6, q3);
+}
+
+int lw_host_apply_twist(double *T, const double *delta) {
+    if (!T || !delta) return LW_EINVAL;
+    lw::gn::apply_twist(T, delta);
+    return 0;
+}
+
+int lw_host_twist_between(const double *T_from, const double *T_to, double *xi) {
+    if (!T_from || !T_to || !xi) return LW_EINVAL;
+    lw::gn::twist_between(T_from, T_to, xi);
+    return 0;
+}
+
+double lw_host_cond(const double *H) { return H ? lw::gn::cond_of(H) : -1.0; }
+
+}  // extern "C"

@@ -1,0 +1,219 @@
+// Frame cutting, deskew and thinning: k_frame_append (python/livox_warp/odom.py:97-123), k_deskew
+// (odom.py:163-178 with gpu.py:268-284) and k_ds_insert (odom.py:126-160).
+//
+// The reference appends with an atomic counter, so on a GPU its frame order and its thinning winners
+// depend on scheduling; Warp's CPU backend runs them in index order. These kernels reproduce the index
+// order: appends and thinned points are stream-compacted (per-block count, one-block scan, write), and a
+// thinning voxel keeps the point of lowest index (CAS-claim the key, atomicMin the index). The thinning
+// table is left clean by the write pass (each claimed slot is reset by its single winner), so no per-call
+// clear of the whole table is needed.
+#include <climits>
+
+#include "lw_internal.h"
+
+namespace lw {
+
+namespace {
+
+__global__ void k_set_i32_one(int32_t *p, int32_t v) { *p = v; }
+
+// Exclusive prefix of pred across a block of BLOCK threads; *total gets the block's count.
+__device__ inline int32_t block_prefix(bool pred, int32_t *wsum, int32_t *total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned long long b = __ballot(pred);
+    int32_t lp = __popcll(b & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[wave] = __popcll(b);
+    __syncthreads();
+    int32_t off = 0, tot = 0;
+    for (int w = 0; w < BLOCK / 64; ++w) {
+        off += (w < wave) ? wsum[w] : 0;
+        tot += wsum[w];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + lp;
+}
+
+// One block: off[b] = base + sum(cnt[0..b)); counter = base + total, base = counter (accumulate) or 0.
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_blocks(const int32_t *cnt, int32_t nb, int32_t *off,
+                                                              int32_t *counter, int32_t accumulate) {
+    __shared__ int32_t part[SCAN_THREADS];
+    const int t = threadIdx.x;
+    const int32_t chunk = (nb + SCAN_THREADS - 1) / SCAN_THREADS;
+    const int32_t a = t * chunk, e = min(nb, a + chunk);
+    int32_t s = 0;
+    for (int32_t b = a; b < e; ++b) s += cnt[b];
+    part[t] = s;
+    __syncthreads();
+    for (int d = 1; d < SCAN_THREADS; d <<= 1) {  // inclusive Hillis-Steele
+        int32_t v = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    const int32_t base = accumulate ? *counter : 0;
+    int32_t run = base + (t > 0 ? part[t - 1] : 0);
+    for (int32_t b = a; b < e; ++b) {
+        off[b] = run;
+        run += cnt[b];
+    }
+    __syncthreads();
+    if (t == 0) *counter = base + part[SCAN_THREADS - 1];
+}
+
+__device__ inline bool append_pred(const float *st, int32_t i, float inv_dt, int32_t k_min, int32_t k_open) {
+    float t = st[i];
+    float tt = t > 0.0f ? t : 0.0f;  // wp.max(t, 0.0)
+    int32_t k = ifloor(tt * inv_dt);
+    return !(k > k_open || k < k_min);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_append_count(const float *st, int32_t n, float inv_dt, int32_t k_min,
+                                                        int32_t k_open, int32_t *blk_cnt) {
+    int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    bool p = i < n && append_pred(st, i, inv_dt, k_min, k_open);
+    int32_t c = __syncthreads_count(p);
+    if (threadIdx.x == 0) blk_cnt[blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_append_write(const float *sx, const uint32_t *sa, const float *st,
+                                                        int32_t n, float inv_dt, int32_t k_min, int32_t k_open,
+                                                        const int32_t *blk_off, int32_t cap, float *f_raw,
+                                                        uint32_t *f_attr, float *f_t, int32_t *f_frame) {
+    __shared__ int32_t wsum[BLOCK / 64];
+    int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    bool p = i < n && append_pred(st, i, inv_dt, k_min, k_open);
+    int32_t tot;
+    int32_t j = blk_off[blockIdx.x] + block_prefix(p, wsum, &tot);
+    if (!p || j >= cap) return;
+    f_raw[3 * (int64_t)j] = sx[3 * (int64_t)i];
+    f_raw[3 * (int64_t)j + 1] = sx[3 * (int64_t)i + 1];
+    f_raw[3 * (int64_t)j + 2] = sx[3 * (int64_t)i + 2];
+    f_attr[j] = sa[i];
+    f_t[j] = st[i];
+    f_frame[j] = k_open;
+}
+
+// gpu.py:268-284: rotate by the rotation vector theta s (unit quaternion), then add rho s.
+__global__ __launch_bounds__(BLOCK) void k_deskew(const float *raw, const float *t, int32_t m, float t_mid,
+                                                  float inv_dt, float r0, float r1, float r2, float h0, float h1,
+                                                  float h2, float *out) {
+    int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= m) return;
+    float s = (t[i] - t_mid) * inv_dt;
+    s = fminf(fmaxf(s, -1.0f), 1.0f);
+    float px = raw[3 * (int64_t)i], py = raw[3 * (int64_t)i + 1], pz = raw[3 * (int64_t)i + 2];
+    float vx = h0 * s, vy = h1 * s, vz = h2 * s;
+    float a = length3(vx, vy, vz);
+    float ox = px, oy = py, oz = pz;
+    if (!(a < 1.0e-9f)) {
+        float ax = vx / a, ay = vy / a, az = vz / a;
+        float sh, ch;
+        sincosf(a * 0.5f, &sh, &ch);
+        float qx = ax * sh, qy = ay * sh, qz = az * sh, qw = ch;
+        float c = 2.0f * qw * qw - 1.0f;
+        float d = 2.0f * (qx * px + qy * py + qz * pz);
+        float cx = qy * pz - qz * py, cy = qz * px - qx * pz, cz = qx * py - qy * px;
+        float w2 = 2.0f * qw;
+        ox = px * c + qx * d + cx * w2;
+        oy = py * c + qy * d + cy * w2;
+        oz = pz * c + qz * d + cz * w2;
+    }
+    out[3 * (int64_t)i] = ox + r0 * s;
+    out[3 * (int64_t)i + 1] = oy + r1 * s;
+    out[3 * (int64_t)i + 2] = oz + r2 * s;
+}
+
+// Thinning 1: filters, key at the thinning voxel (sensor frame), claim, lowest index wins.
+__global__ __launch_bounds__(BLOCK) void k_thin_claim(const float *xyz, const uint32_t *attr, int32_t m, float inv_v,
+                                                      float min_range, float max_range, int32_t ret_mask,
+                                                      int32_t ds_mask, int64_t *ds_keys, int32_t *ds_idx,
+                                                      int32_t *pt_slot) {
+    int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= m) return;
+    float px = xyz[3 * (int64_t)i], py = xyz[3 * (int64_t)i + 1], pz = xyz[3 * (int64_t)i + 2];
+    float r = length3(px, py, pz);
+    uint32_t a = attr[i];
+    pt_slot[i] = -1;
+    if (r < min_range || r > max_range) return;
+    if (tag_of(a) != 0 || !ret_ok(a, ret_mask)) return;
+    int64_t key = voxel_key_ijk(ifloor(px * inv_v), ifloor(py * inv_v), ifloor(pz * inv_v));
+    int32_t h = slot_hash(key) & ds_mask;
+    for (int probe = 0; probe < PROBES; ++probe) {
+        int32_t s = (h + probe) & ds_mask;
+        unsigned long long prev = atomicCAS((unsigned long long *)&ds_keys[s], 0xFFFFFFFFFFFFFFFFull,
+                                            (unsigned long long)key);
+        if ((int64_t)prev == -1 || (int64_t)prev == key) {
+            atomicMin(&ds_idx[s], i);
+            pt_slot[i] = s;
+            return;
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_thin_count(const int32_t *pt_slot, const int32_t *ds_idx, int32_t m,
+                                                      int32_t *blk_cnt) {
+    int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    int32_t s = i < m ? pt_slot[i] : -1;
+    bool p = s >= 0 && ds_idx[s] == i;
+    int32_t c = __syncthreads_count(p);
+    if (threadIdx.x == 0) blk_cnt[blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_thin_write(const float *xyz, const int32_t *pt_slot, int32_t *ds_idx,
+                                                      int64_t *ds_keys, int32_t m, const int32_t *blk_off,
+                                                      float *ds_xyz) {
+    __shared__ int32_t wsum[BLOCK / 64];
+    int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    int32_t s = i < m ? pt_slot[i] : -1;
+    bool p = s >= 0 && ds_idx[s] == i;  // only the winner of a slot resets it, after reading it
+    int32_t tot;
+    int32_t j = blk_off[blockIdx.x] + block_prefix(p, wsum, &tot);
+    if (!p) return;
+    ds_xyz[3 * (int64_t)j] = xyz[3 * (int64_t)i];
+    ds_xyz[3 * (int64_t)j + 1] = xyz[3 * (int64_t)i + 1];
+    ds_xyz[3 * (int64_t)j + 2] = xyz[3 * (int64_t)i + 2];
+    ds_keys[s] = -1;
+    ds_idx[s] = INT_MAX;
+}
+
+}  // namespace
+
+hipError_t launch_frame_append(lw_ctx *c, const float *sx, const uint32_t *sa, const float *st, int32_t n,
+                               float inv_dt, int32_t k_min, int32_t k_open) {
+    if (n <= 0) return hipSuccess;
+    int32_t nb = (n + BLOCK - 1) / BLOCK;
+    if (nb > c->max_blocks) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_append_count, dim3(nb), dim3(BLOCK), 0, c->stream, st, n, inv_dt, k_min, k_open,
+                       c->blk_cnt);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_THREADS), 0, c->stream, c->blk_cnt, nb, c->blk_off,
+                       &c->ctr[6], 1);
+    hipLaunchKernelGGL(k_append_write, dim3(nb), dim3(BLOCK), 0, c->stream, sx, sa, st, n, inv_dt, k_min, k_open,
+                       c->blk_off, (int32_t)c->frame_cap, c->f_raw, c->f_attr, c->f_t, c->f_frame);
+    return hipGetLastError();
+}
+
+hipError_t launch_deskew(lw_ctx *c, int32_t m, float t_mid, float inv_dt, const float *xi) {
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_deskew, dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, c->stream, c->f_raw, c->f_t, m,
+                       t_mid, inv_dt, xi[0], xi[1], xi[2], xi[3], xi[4], xi[5], c->f_xyz);
+    return hipGetLastError();
+}
+
+hipError_t launch_thin(lw_ctx *c, int32_t m, float inv_v, float min_range, float max_range, int32_t ret_mask) {
+    if (m <= 0) {
+        hipLaunchKernelGGL(k_set_i32_one, dim3(1), dim3(1), 0, c->stream, &c->ctr[5], 0);
+        return hipGetLastError();
+    }
+    int32_t nb = (m + BLOCK - 1) / BLOCK;
+    hipLaunchKernelGGL(k_thin_claim, dim3(nb), dim3(BLOCK), 0, c->stream, c->f_xyz, c->f_attr, m, inv_v, min_range,
+                       max_range, ret_mask, (int32_t)(c->ds_cap - 1), c->ds_keys, c->ds_idx, c->pt_slot);
+    hipLaunchKernelGGL(k_thin_count, dim3(nb), dim3(BLOCK), 0, c->stream, c->pt_slot, c->ds_idx, m, c->blk_cnt);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_THREADS), 0, c->stream, c->blk_cnt, nb, c->blk_off,
+                       &c->ctr[5], 0);
+    hipLaunchKernelGGL(k_thin_write, dim3(nb), dim3(BLOCK), 0, c->stream, c->f_xyz, c->pt_slot, c->ds_idx,
+                       c->ds_keys, m, c->blk_off, c->ds_xyz);
+    return hipGetLastError();
+}
+
+}  // namespace lw

@@ -1,0 +1,121 @@
+// Context and launch helpers shared by the kernel translation units and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lw_common.h"
+#include "lw_gn.h"
+
+namespace lw {
+
+constexpr int BLOCK = 256;           // threads per block of the streaming kernels
+constexpr int ICP_BLOCK = 64;        // one wave per accumulation workgroup
+constexpr int ICP_MAX_GROUPS = 2048; // partial rows kept for the cross-workgroup reduction
+constexpr int SCAN_THREADS = 1024;
+
+// Device-resident state of one registration (one Gauss-Newton loop).
+struct RegState {
+    double T[16];       // current pose, float64
+    double T_pred[16];  // the prediction the prior pulls toward
+    double H[36];       // H of the last accepted accumulation (for the condition number)
+    double acc[64];     // last reduced accumulator (ACC layout)
+    double rms;
+    double cond;
+    float Tf[16];       // the current pose rounded to float32, as the accumulation kernels use it
+    int32_t done;       // 1: converged or stopped, later passes return at once
+    int32_t iters;
+    int32_t corr;
+    int32_t n_degen;
+    int32_t status;     // last step status (gn::STEP_*)
+    uint32_t ticket;    // workgroups finished in the current pass
+    int32_t pad[2];
+};
+
+}  // namespace lw
+
+struct lw_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t cap = 0;  // map slots (power of two)
+    int32_t mask = 0;
+    int64_t frame_cap = 0;
+    int64_t ds_cap = 0;
+    int64_t vox_cap = 0;
+    // map
+    lw::Slot *slots = nullptr;
+    uint32_t *occ = nullptr;
+    int64_t occ_words = 0;
+    int64_t *vox_key = nullptr;
+    lw::VoxMom *vox_mom = nullptr;
+    float4 *vox_mu = nullptr;
+    lw::VoxPrec *vox_prec = nullptr;
+    int32_t *vox_nin = nullptr;
+    int32_t *vox_quota = nullptr;
+    int32_t *vox_dirty = nullptr;
+    int32_t *dirty_list = nullptr;
+    int32_t *touched = nullptr;
+    int32_t *ctr = nullptr;  // [0] voxels claimed (stats[0]) [1] points without a slot (stats[1])
+                             // [2] voxel ids handed out [3] dirty count [4] touched count [5] thinned count
+                             // [6] frame count [7] scan base
+    // per-point scratch, frame_cap long
+    int32_t *pt_slot = nullptr;
+    int32_t *pt_vid = nullptr;
+    uint8_t *pt_take = nullptr;
+    // frame buffers (bound by the caller)
+    float *f_raw = nullptr, *f_xyz = nullptr, *f_t = nullptr;
+    uint32_t *f_attr = nullptr;
+    int32_t *f_frame = nullptr;
+    // thinning
+    int64_t *ds_keys = nullptr;
+    int32_t *ds_idx = nullptr;
+    float *ds_xyz = nullptr;
+    // block scan scratch
+    int32_t *blk_cnt = nullptr;
+    int32_t *blk_off = nullptr;
+    int64_t max_blocks = 0;
+    // registration
+    lw::RegState *reg = nullptr;
+    double *partials = nullptr;
+    int32_t *best_slot = nullptr;
+    int64_t *dtmp = nullptr;  // small uploads: poses, keys (frame_cap + 8 entries)
+    // pinned host staging
+    void *pin = nullptr;
+    size_t pin_bytes = 0;
+    // parameters the cached per-voxel precision was computed with
+    bool prec_valid = false;
+    float prec_voxel = 0, prec_noise2 = 0, prec_tight2 = 0, prec_alpha = 0;
+};
+
+namespace lw {
+
+struct Mat4f {
+    float m[16];
+};
+
+hipError_t launch_map_reset(lw_ctx *c);
+hipError_t launch_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, int32_t n, const Mat4f &T, float inv_v,
+                             float voxel, float max_range, int32_t ret_mask);
+hipError_t launch_refresh(lw_ctx *c, bool all, float voxel, float noise2, float tight2, float alpha);
+hipError_t launch_map_find(lw_ctx *c, const int64_t *keys_dev, int32_t n, int32_t *out_dev);
+
+hipError_t launch_frame_append(lw_ctx *c, const float *sx, const uint32_t *sa, const float *st, int32_t n,
+                               float inv_dt, int32_t k_min, int32_t k_open);
+hipError_t launch_deskew(lw_ctx *c, int32_t m, float t_mid, float inv_dt, const float *xi);
+hipError_t launch_thin(lw_ctx *c, int32_t m, float inv_v, float min_range, float max_range, int32_t ret_mask);
+hipError_t launch_fill_i32(lw_ctx *c, int32_t *p, int64_t n, int32_t v);
+
+struct IcpPass {
+    int32_t n;
+    int32_t reach;
+    float inv_v, voxel, max_d2, kappa;
+    int32_t it;
+    int32_t gn;  // 1: run the Gauss-Newton step in the last workgroup
+    gn::StepParams sp;
+    int32_t want_slots;
+};
+hipError_t launch_icp_pass(lw_ctx *c, const IcpPass &p, int acc_mode);
+hipError_t launch_reg_begin(lw_ctx *c, const double *T_pred_host_pinned);
+hipError_t launch_reg_finish(lw_ctx *c, int32_t min_corr);
+
+}  // namespace lw

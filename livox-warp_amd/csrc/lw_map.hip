@@ -1,0 +1,321 @@
+// Registration-map build: the MI355X form of k_reg_insert (python/livox_warp/odom.py:186-230) and of the
+// map reset (odom.py:561-567).
+//
+// Layout in HBM (cap slots, V voxels):
+//   slots[cap]    16 B {int64 key, int32 voxel id}: the reference's open-addressed key table, probed
+//                 with its hash (gpu.py:247-265); a voxel id indexes the dense per-voxel arrays below;
+//   occ[cap/32]   occupancy bits (bit s set <=> key[s] != -1), 1/64 of the key table, L2-resident;
+//   vox_key[V], vox_mom[V] 40 B {sum r, count, sum r r^T (6)}: the moments, dense by voxel id;
+//   vox_mu[V] 16 B, vox_prec[V] 64 B: the per-voxel mean and residual precision the query reads.
+//
+// One insert is five short kernels:
+//   1 claim    probe and CAS-claim each point's slot (a claimer hands out the voxel id);
+//   2 count    per-voxel incoming count, and the list of voxels this insert touches;
+//   3 quota    voxels that would pass the 4096-point cap (odom.py:225) rank their incoming points in
+//              index order, so exactly the first points contribute, as in a serial insert;
+//   4 accum    float32 atomic sums into the moments;
+//   5 finish   reset the per-insert counters, queue the touched voxels for a precision refresh.
+#include <climits>
+
+#include "lw_internal.h"
+#include "lw_prec.h"
+
+namespace lw {
+
+namespace {
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_clear_slots(Slot *s, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        Slot z;
+        z.key = -1;
+        z.vid = -1;
+        z.pad = 0;
+        s[i] = z;
+    }
+}
+
+__device__ inline bool insert_point(const float *xyz, const uint32_t *attr, int32_t i, const Mat4f &T, float inv_v,
+                                    float max_range, int32_t ret_mask, int32_t &ix, int32_t &iy, int32_t &iz,
+                                    float &wx, float &wy, float &wz) {
+    float px = xyz[3 * (int64_t)i], py = xyz[3 * (int64_t)i + 1], pz = xyz[3 * (int64_t)i + 2];
+    uint32_t a = attr[i];
+    if (length3(px, py, pz) > max_range || tag_of(a) != 0 || !ret_ok(a, ret_mask)) return false;
+    transform_point(T.m, px, py, pz, wx, wy, wz);
+    ix = ifloor(wx * inv_v);
+    iy = ifloor(wy * inv_v);
+    iz = ifloor(wz * inv_v);
+    return true;
+}
+
+// 1: claim. A plain read first: a slot's key only ever goes from -1 to its final value.
+__global__ __launch_bounds__(BLOCK) void k_ins_claim(const float *xyz, const uint32_t *attr, int32_t n, Mat4f T,
+                                                     float inv_v, float max_range, int32_t ret_mask, int32_t mask,
+                                                     Slot *slots, uint32_t *occ, int64_t *vox_key, VoxMom *vox_mom,
+                                                     int32_t *ctr, int64_t vox_cap, int32_t *pt_slot) {
+    int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i == 0) ctr[4] = 0;  // touched count of this insert (read by the next kernel only)
+    if (i >= n) return;
+    int32_t ix, iy, iz;
+    float wx, wy, wz;
+    if (!insert_point(xyz, attr, i, T, inv_v, max_range, ret_mask, ix, iy, iz, wx, wy, wz)) {
+        pt_slot[i] = -1;
+        return;
+    }
+    int64_t key = voxel_key_ijk(ix, iy, iz);
+    int32_t h = slot_hash(key) & mask;
+    for (int probe = 0; probe < PROBES; ++probe) {
+        int32_t s = (h + probe) & mask;
+        int64_t k = __hip_atomic_load(&slots[s].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (k == key) {
+            pt_slot[i] = s;
+            return;
+        }
+        if (k == -1) {
+            unsigned long long prev = atomicCAS((unsigned long long *)&slots[s].key, 0xFFFFFFFFFFFFFFFFull,
+                                                (unsigned long long)key);
+            if ((int64_t)prev == -1) {
+                int32_t vid = atomicAdd(&ctr[2], 1);
+                if (vid >= vox_cap) {  // cannot happen while vox_cap == cap; keep the slot unusable
+                    slots[s].vid = -1;
+                    pt_slot[i] = -1;
+                    atomicAdd(&ctr[1], 1);
+                    return;
+                }
+                slots[s].vid = vid;
+                vox_key[vid] = key;
+                VoxMom z;
+                for (int a = 0; a < 3; ++a) z.s[a] = 0.0f;
+                for (int a = 0; a < 6; ++a) z.o[a] = 0.0f;
+                z.cnt = 0;
+                vox_mom[vid] = z;
+                atomicOr(&occ[s >> 5], 1u << (s & 31));
+                atomicAdd(&ctr[0], 1);
+                pt_slot[i] = s;
+                return;
+            }
+            if ((int64_t)prev == key) {
+                pt_slot[i] = s;
+                return;
+            }
+        }
+    }
+    atomicAdd(&ctr[1], 1);  // no slot within the probe window (odom.py:230)
+    pt_slot[i] = -1;
+}
+
+// 2: count the incoming points per voxel; the first one lists the voxel as touched.
+__global__ __launch_bounds__(BLOCK) void k_ins_count(const int32_t *pt_slot, int32_t n, const Slot *slots,
+                                                     int32_t *pt_vid, int32_t *vox_nin, int32_t *touched,
+                                                     int32_t *ctr) {
+    int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    int32_t s = pt_slot[i];
+    int32_t vid = s >= 0 ? slots[s].vid : -1;
+    pt_vid[i] = vid;
+    if (vid < 0) return;
+    if (atomicAdd(&vox_nin[vid], 1) == 0) touched[atomicAdd(&ctr[4], 1)] = vid;
+}
+
+// 3: per touched voxel, its quota under the cap; voxels that overflow rank their points in index order.
+__global__ __launch_bounds__(BLOCK) void k_ins_quota(const int32_t *touched, const int32_t *ctr,
+                                                     const int32_t *vox_nin, const VoxMom *vox_mom,
+                                                     int32_t *vox_quota, const int32_t *pt_vid, int32_t n,
+                                                     uint8_t *pt_take) {
+    __shared__ int32_t wsum[BLOCK / 64];
+    __shared__ int32_t s_vid, s_quota, s_nin;
+    int32_t nt = ctr[4];
+    for (int32_t j = blockIdx.x; j < nt; j += gridDim.x) {
+        if (threadIdx.x == 0) {
+            s_vid = touched[j];
+            s_nin = vox_nin[s_vid];
+            s_quota = REG_CAP - vox_mom[s_vid].cnt;
+        }
+        __syncthreads();
+        int32_t vid = s_vid, quota = s_quota, nin = s_nin;
+        __syncthreads();
+        if (nin <= quota) {
+            if (threadIdx.x == 0) vox_quota[vid] = INT_MAX;
+            continue;
+        }
+        if (threadIdx.x == 0) vox_quota[vid] = quota;
+        if (quota <= 0) continue;
+        // rank this voxel's points in index order
+        int32_t base = 0;
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int32_t t0 = 0; t0 < n; t0 += BLOCK) {
+            int32_t i = t0 + threadIdx.x;
+            bool match = i < n && pt_vid[i] == vid;
+            unsigned long long b = __ballot(match);
+            int32_t lp = __popcll(b & ((1ull << lane) - 1ull));
+            if (lane == 0) wsum[wave] = __popcll(b);
+            __syncthreads();
+            int32_t off = 0, tot = 0;
+            for (int w = 0; w < BLOCK / 64; ++w) {
+                off += (w < wave) ? wsum[w] : 0;
+                tot += wsum[w];
+            }
+            if (match) pt_take[i] = (base + off + lp) < quota ? 1 : 0;
+            base += tot;
+            __syncthreads();
+            if (base >= nin) break;  // every point of the voxel is ranked (uniform: base and nin are)
+        }
+    }
+}
+
+// 4: accumulate (odom.py:217, 225-228): r relative to the voxel centre, float32 atomics.
+__global__ __launch_bounds__(BLOCK) void k_ins_accum(const float *xyz, const uint32_t *attr, int32_t n, Mat4f T,
+                                                     float inv_v, float voxel, float max_range, int32_t ret_mask,
+                                                     const int32_t *pt_vid, const int32_t *vox_quota,
+                                                     const uint8_t *pt_take, VoxMom *vox_mom) {
+    int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    int32_t vid = pt_vid[i];
+    if (vid < 0) return;
+    int32_t q = vox_quota[vid];
+    if (q != INT_MAX && !(q > 0 && pt_take[i])) return;
+    int32_t ix, iy, iz;
+    float wx, wy, wz;
+    insert_point(xyz, attr, i, T, inv_v, max_range, ret_mask, ix, iy, iz, wx, wy, wz);
+    float r0 = wx - center_of(ix, voxel), r1 = wy - center_of(iy, voxel), r2 = wz - center_of(iz, voxel);
+    VoxMom *m = &vox_mom[vid];
+    unsafeAtomicAdd(&m->s[0], r0);
+    unsafeAtomicAdd(&m->s[1], r1);
+    unsafeAtomicAdd(&m->s[2], r2);
+    unsafeAtomicAdd(&m->o[0], r0 * r0);
+    unsafeAtomicAdd(&m->o[1], r0 * r1);
+    unsafeAtomicAdd(&m->o[2], r0 * r2);
+    unsafeAtomicAdd(&m->o[3], r1 * r1);
+    unsafeAtomicAdd(&m->o[4], r1 * r2);
+    unsafeAtomicAdd(&m->o[5], r2 * r2);
+    atomicAdd(&m->cnt, 1);
+}
+
+// 5: finish: clear the per-insert counters, queue the touched voxels for a refresh.
+__global__ __launch_bounds__(BLOCK) void k_ins_finish(const int32_t *touched, int32_t *ctr, int32_t *vox_nin,
+                                                      int32_t *vox_quota, int32_t *vox_dirty, int32_t *dirty_list) {
+    int32_t nt = ctr[4];
+    for (int32_t j = blockIdx.x * BLOCK + threadIdx.x; j < nt; j += gridDim.x * BLOCK) {
+        int32_t vid = touched[j];
+        vox_nin[vid] = 0;
+        vox_quota[vid] = INT_MAX;
+        if (atomicExch(&vox_dirty[vid], 1) == 0) dirty_list[atomicAdd(&ctr[3], 1)] = vid;
+    }
+}
+
+// Per-voxel mean and precision for the queued voxels (all == 1: every voxel handed out).
+__global__ __launch_bounds__(BLOCK) void k_refresh(int32_t all, int32_t *ctr, const int32_t *dirty_list,
+                                                   int32_t *vox_dirty, const int64_t *vox_key,
+                                                   const VoxMom *vox_mom, float4 *vox_mu, VoxPrec *vox_prec,
+                                                   float voxel, float noise2, float tight2, float alpha) {
+    int32_t cnt = all ? ctr[2] : ctr[3];
+    for (int32_t j = blockIdx.x * BLOCK + threadIdx.x; j < cnt; j += gridDim.x * BLOCK) {
+        int32_t vid = all ? j : dirty_list[j];
+        vox_dirty[vid] = 0;
+        VoxMom m = vox_mom[vid];
+        if (m.cnt <= 0) {
+            vox_mu[vid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            continue;
+        }
+        int32_t ix, iy, iz;
+        voxel_key_unpack(vox_key[vid], ix, iy, iz);
+        float mu[3];
+        voxel_mean(ix, iy, iz, m.s, m.cnt, voxel, mu);
+        vox_mu[vid] = make_float4(mu[0], mu[1], mu[2], (float)m.cnt);
+        VoxPrec P;
+        P.shape = (float)voxel_precision(m.s, m.o, m.cnt, voxel, noise2, tight2, alpha, P.M, P.Mg, P.q);
+        vox_prec[vid] = P;
+    }
+}
+
+__global__ void k_set_ctr(int32_t *ctr, int32_t idx, int32_t v) { ctr[idx] = v; }
+
+__global__ __launch_bounds__(BLOCK) void k_map_find(const Slot *slots, const uint32_t *occ, int32_t mask,
+                                                    const int64_t *keys, int32_t n, int32_t *out) {
+    int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    int64_t key = keys[i];
+    int32_t h = slot_hash(key) & mask, res = -1;
+    for (int probe = 0; probe < PROBES; ++probe) {
+        int32_t s = (h + probe) & mask;
+        if (!((occ[s >> 5] >> (s & 31)) & 1u)) break;
+        if (slots[s].key == key) {
+            res = s;
+            break;
+        }
+    }
+    out[i] = res;
+}
+
+inline int grid_for(int64_t n, int block = BLOCK, int64_t max_grid = 65535) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > max_grid) g = max_grid;
+    return (int)g;
+}
+
+}  // namespace
+
+hipError_t launch_fill_i32(lw_ctx *c, int32_t *p, int64_t n, int32_t v) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n, BLOCK, 4096)), dim3(BLOCK), 0, c->stream, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_map_reset(lw_ctx *c) {
+    hipLaunchKernelGGL(k_clear_slots, dim3(grid_for(c->cap, BLOCK, 8192)), dim3(BLOCK), 0, c->stream, c->slots,
+                       c->cap);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(c->occ_words, BLOCK, 4096)), dim3(BLOCK), 0, c->stream, c->occ,
+                       c->occ_words, 0u);
+    // the per-voxel bookkeeping: every voxel id gets its records initialised when it is handed out
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(c->vox_cap, BLOCK, 8192)), dim3(BLOCK), 0, c->stream, c->vox_nin,
+                       c->vox_cap, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(c->vox_cap, BLOCK, 8192)), dim3(BLOCK), 0, c->stream,
+                       c->vox_quota, c->vox_cap, INT_MAX);
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(c->vox_cap, BLOCK, 8192)), dim3(BLOCK), 0, c->stream,
+                       c->vox_dirty, c->vox_cap, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->ctr, (int64_t)5, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, int32_t n, const Mat4f &T, float inv_v,
+                             float voxel, float max_range, int32_t ret_mask) {
+    if (n <= 0) return hipSuccess;
+    int g = grid_for(n);
+    hipLaunchKernelGGL(k_ins_claim, dim3(g), dim3(BLOCK), 0, c->stream, xyz, attr, n, T, inv_v, max_range, ret_mask,
+                       c->mask, c->slots, c->occ, c->vox_key, c->vox_mom, c->ctr, c->vox_cap, c->pt_slot);
+    hipLaunchKernelGGL(k_ins_count, dim3(g), dim3(BLOCK), 0, c->stream, c->pt_slot, n, c->slots, c->pt_vid,
+                       c->vox_nin, c->touched, c->ctr);
+    hipLaunchKernelGGL(k_ins_quota, dim3(1024), dim3(BLOCK), 0, c->stream, c->touched, c->ctr, c->vox_nin,
+                       c->vox_mom, c->vox_quota, c->pt_vid, n, c->pt_take);
+    hipLaunchKernelGGL(k_ins_accum, dim3(g), dim3(BLOCK), 0, c->stream, xyz, attr, n, T, inv_v, voxel, max_range,
+                       ret_mask, c->pt_vid, c->vox_quota, c->pt_take, c->vox_mom);
+    hipLaunchKernelGGL(k_ins_finish, dim3(256), dim3(BLOCK), 0, c->stream, c->touched, c->ctr, c->vox_nin,
+                       c->vox_quota, c->vox_dirty, c->dirty_list);
+    return hipGetLastError();
+}
+
+hipError_t launch_refresh(lw_ctx *c, bool all, float voxel, float noise2, float tight2, float alpha) {
+    hipLaunchKernelGGL(k_refresh, dim3(all ? 4096 : 512), dim3(BLOCK), 0, c->stream, all ? 1 : 0, c->ctr,
+                       c->dirty_list, c->vox_dirty, c->vox_key, c->vox_mom, c->vox_mu, c->vox_prec, voxel, noise2,
+                       tight2, alpha);
+    hipLaunchKernelGGL(k_set_ctr, dim3(1), dim3(1), 0, c->stream, c->ctr, 3, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_map_find(lw_ctx *c, const int64_t *keys_dev, int32_t n, int32_t *out_dev) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_map_find, dim3(grid_for(n)), dim3(BLOCK), 0, c->stream, c->slots, c->occ, c->mask, keys_dev,
+                       n, out_dev);
+    return hipGetLastError();
+}
+
+}  // namespace lw

@@ -1,0 +1,364 @@
+"""LiDAR-only scan-to-map odometry on MI355X: the drop-in counterpart of livox_warp.odom.Odometry.
+
+Same constructor, tuning attributes, methods and read-back state as the reference class
+(python/livox_warp/odom.py:458-803); the host bookkeeping (frame cutting, warm-up, keyframes, velocity,
+re-deskew) is the reference's, line for line in behaviour. Every device step goes through the C ABI of
+liblwamd.so on the instance's stream:
+
+    push            -> lw_frame_reset / lw_frame_append / lw_frame_count      (odom.py:596-638)
+    _deskew         -> lw_deskew                                               (odom.py:707-710)
+    _register       -> lw_register: thinning + the whole Gauss-Newton loop on the device, one readback
+                       (odom.py:731-803: the reference reads the accumulator back every iteration)
+    _insert         -> lw_map_insert                                           (odom.py:719-727)
+    reset           -> lw_map_reset                                            (odom.py:561-579)
+
+Device arrays are torch tensors (ROCm): the staged batch passed to push() and the frame buffers handed to
+on_frame (f_xyz float32 (cap, 3), f_attr int32 holding the uint32 attribute bits, f_t float32, f_frame
+int32), borrowed until the next frame as in the reference.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+from scipy.spatial.transform import Rotation
+
+from . import _abi
+
+ACC_SLOTS = 64
+ACC_STRIDE = 64
+ACC_H, ACC_B, ACC_COUNT, ACC_ERR, ACC_RMS, ACC_WEIGHT, ACC_RANGE, ACC_HG = 0, 21, 27, 28, 29, 30, 31, 32
+REG_CAP = 4096
+WIDE_ITERS = 3
+KAPPA_START, KAPPA_DECAY, KAPPA_MIN = 6.0, 0.7, 2.0
+STEP_TOL = 1e-4
+TIGHT_FRACTION = 0.6
+WEAK_COND = 1e-3
+REDESKEW_ROT = 2e-3
+REDESKEW_TRANS = 5e-3
+MAX_FRAME_JUMP = 1.5
+
+ACC_MODES = {"fp64": 0, "kahan32": 1, "fp32": 2}
+
+
+def _twist_between(T_from: np.ndarray, T_to: np.ndarray) -> np.ndarray:
+    M = T_to @ np.linalg.inv(T_from)
+    return np.concatenate([M[:3, 3], Rotation.from_matrix(M[:3, :3]).as_rotvec()])
+
+
+def _apply_twist(T: np.ndarray, delta: np.ndarray) -> np.ndarray:
+    Rz = Rotation.from_rotvec(delta[3:]).as_matrix()
+    out = T.copy()
+    out[:3, :3] = Rz @ T[:3, :3]
+    out[:3, 3] = Rz @ T[:3, 3] + delta[:3]
+    return out
+
+
+def _body_twist(d: np.ndarray) -> np.ndarray:
+    return np.concatenate([d[:3, 3], Rotation.from_matrix(d[:3, :3]).as_rotvec()]).astype(np.float32)
+
+
+def _twist_matrix(xi: np.ndarray) -> np.ndarray:
+    out = np.eye(4)
+    out[:3, :3] = Rotation.from_rotvec(np.asarray(xi[3:], dtype=np.float64)).as_matrix()
+    out[:3, 3] = xi[:3]
+    return out
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+class Odometry:
+    """Frame cutting, the registration map and scan-to-map ICP on one stream (odom.py:458-803)."""
+
+    def __init__(self, pipe, device=None, reg_slots: int = 1 << 21, frame_cap: int = 1 << 18, stream=None):
+        torch = _torch()
+        self.pipe = pipe
+        if device is None:
+            device = torch.device("cuda", torch.cuda.current_device())
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise ValueError(f"Odometry runs on a GPU device, got {self.device}")
+        if self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
+        assert reg_slots & (reg_slots - 1) == 0
+        self.reg_cap = reg_slots
+        self.frame_cap = frame_cap
+        d = self.device
+        self.f_raw = torch.zeros((frame_cap, 3), dtype=torch.float32, device=d)  # as measured
+        self.f_xyz = torch.zeros((frame_cap, 3), dtype=torch.float32, device=d)  # deskewed to mid-frame
+        self.f_attr = torch.zeros(frame_cap, dtype=torch.int32, device=d)
+        self.f_t = torch.zeros(frame_cap, dtype=torch.float32, device=d)
+        self.f_frame = torch.zeros(frame_cap, dtype=torch.int32, device=d)
+        self._lib = _abi.lib()
+        self._ctx = ctypes_ctx = _abi.ctypes.c_void_p()
+        with torch.cuda.device(d):
+            _abi.check(self._lib.lw_create(_abi.ctypes.byref(ctypes_ctx), reg_slots, frame_cap,
+                                           _abi.ctypes.c_void_p(self.stream.cuda_stream)), "lw_create")
+        _abi.check(self._lib.lw_bind_frame(self._ctx, self.f_raw.data_ptr(), self.f_xyz.data_ptr(),
+                                           self.f_attr.data_ptr(), self.f_t.data_ptr(), self.f_frame.data_ptr()),
+                   "lw_bind_frame")
+        self._params = _abi.RegParams()
+        self._result = _abi.RegResult()
+        self._i32 = (_abi.ctypes.c_int32 * 2)()
+
+        # tuning (odom.py:499-525)
+        self.frame_dt = 0.1
+        self.reg_voxel = 0.15
+        self.max_iter = 16
+        self.warmup = 8
+        self.max_gap = 20
+        self.min_points = 200
+        self.min_corr = 60
+        self.min_range = 0.3
+        self.max_range = 60.0
+        self.noise = 0.02
+        self.ret_mask = 0b1111
+        self.alpha = 0.05
+        self.damping = 1.0e-3
+        self.velocity_smooth = 0.85
+        self.degen_thr = 3000.0
+        self.prior_info = 2000.0
+        self.kf_dist = 0.05
+        self.kf_angle = 1.0
+        self.deskew = True
+        self.redeskew = True
+        self.epoch = 0
+        self.acc_mode = "fp64"  # accumulation form: fp64 (reference), kahan32, fp32
+        self.wide_iters = WIDE_ITERS
+
+        self.reset(np.eye(4, dtype=np.float64))
+
+    def __del__(self):
+        ctx = getattr(self, "_ctx", None)
+        if ctx is not None and ctx.value:
+            try:
+                self._lib.lw_destroy(ctx)
+            except Exception:
+                pass
+            self._ctx = None
+
+    # ---- stream helpers --------------------------------------------------------------------
+
+    def synchronize(self):
+        _abi.check(self._lib.lw_synchronize(self._ctx), "lw_synchronize")
+
+    def _frame_count(self) -> int:
+        _abi.check(self._lib.lw_frame_count(self._ctx, self._i32), "lw_frame_count")
+        return int(self._i32[0])
+
+    def _zero_count(self):
+        _abi.check(self._lib.lw_frame_reset(self._ctx), "lw_frame_reset")
+
+    # ---- state -----------------------------------------------------------------------------
+
+    @staticmethod
+    def empty_stats() -> dict:
+        return {"frames": 0, "iters": 0, "corr": 0, "ds": 0, "rms": 0.0, "cond": 1.0, "ms": 0.0,
+                "weak": False, "map_voxels": 0, "map_full": 0.0, "map_overflow": 0, "skipped": 0, "warm": True,
+                "speed": 0.0, "turn": 0.0, "gaps": 0, "degen": 0, "keyframes": 0}
+
+    def reset(self, T0: np.ndarray):
+        """Forget the map and the trajectory; the next frame is posed at T0."""
+        _abi.check(self._lib.lw_map_reset(self._ctx), "lw_map_reset")
+        self.T = np.array(T0, dtype=np.float64)
+        self.T_prev = None
+        self.delta = np.eye(4)
+        self.xi = np.zeros(6, np.float32)
+        self.xi_s = np.zeros(6)
+        self.k_open = None
+        self.k_pred = -1
+        self.map_frames = 0
+        self.T_key = None
+        self.traj = []
+        self.stats = self.empty_stats()
+        self._zero_count()
+
+    def restart_clock(self):
+        self.k_open = None
+        self.k_pred = -1
+        self.T_prev = None
+        self.delta = np.eye(4)
+        self.xi = np.zeros(6, np.float32)
+        self.xi_s = np.zeros(6)
+        self._zero_count()
+
+    def predict(self) -> np.ndarray:
+        return self.T @ self.delta
+
+    # ---- per batch -------------------------------------------------------------------------
+
+    @staticmethod
+    def _dev_ptr(a, dtype, n: int, width: int = 1) -> int:
+        torch = _torch()
+        if not isinstance(a, torch.Tensor) or a.device.type != "cuda":
+            raise TypeError("push() takes device tensors")
+        if a.dtype not in dtype or not a.is_contiguous() or a.numel() < n * width:
+            raise ValueError(f"bad staged array: {a.dtype}, {tuple(a.shape)}")
+        return a.data_ptr()
+
+    def push(self, sx, sa, st, n: int, t_min: float, t_max: float, on_frame):
+        """Feed one staged batch (points, attributes, timestamps: device tensors). on_frame(k, T, f_xyz, f_attr,
+        f_t, f_frame, m) runs per completed frame, with the frame's points deskewed to mid-frame."""
+        if n <= 0:
+            return
+        torch = _torch()
+        p_x = self._dev_ptr(sx, (torch.float32,), n, 3)
+        p_a = self._dev_ptr(sa, (torch.int32, torch.uint32), n)
+        p_t = self._dev_ptr(st, (torch.float32,), n)
+        inv_dt = 1.0 / self.frame_dt
+        k_lo = int(math.floor(max(t_min, 0.0) * inv_dt))
+        k_hi = int(math.floor(max(t_max, 0.0) * inv_dt))
+        k_prev = self.k_pred
+        if self.k_open is not None and k_hi - self.k_open > self.max_gap:
+            if self._frame_count() > 0:
+                self._finalize(self.k_open, on_frame)
+            self.restart_clock()
+            self.stats["gaps"] += 1
+        if self.k_open is None:
+            self.k_open = k_lo if k_hi - k_lo <= self.max_gap else k_hi
+            self._zero_count()
+            self.pipe.set_pose(self.k_open, self.T, self.xi, self.epoch)
+            lo = (k_prev + 1) if 0 <= k_prev < self.k_open else k_lo
+            if lo < self.k_open:
+                self.pipe.set_pose_range(lo, self.k_open, self.T, None, self.epoch)
+            self.k_pred = self.k_open
+        for k in range(max(self.k_pred + 1, self.k_open + 1), k_hi + 1):
+            self.pipe.set_pose(k, self.predict(), self.xi, self.epoch)
+            self.k_pred = k
+        k_min = self.k_open - 1
+        while True:
+            _abi.check(self._lib.lw_frame_append(self._ctx, p_x, p_a, p_t, n, inv_dt, k_min, self.k_open),
+                       "lw_frame_append")
+            if k_hi <= self.k_open:
+                break
+            self._finalize(self.k_open, on_frame)
+            self.k_open += 1
+            k_min = self.k_open
+            self._zero_count()
+            self.pipe.set_pose(self.k_open, self.predict(), self.xi, self.epoch)
+            self.k_pred = max(self.k_pred, self.k_open)
+
+    # ---- per frame -------------------------------------------------------------------------
+
+    def _finalize(self, k: int, on_frame):
+        t0 = time.perf_counter()
+        m = min(self._frame_count(), self.frame_cap)
+        warm = self.map_frames < self.warmup
+        t_mid = (k + 0.5) * self.frame_dt
+        T, xi, ok = self._estimate(m, t_mid, warm)
+        if ok and (warm or self._is_keyframe(T)):
+            self._insert(m, T)
+        if self.T_prev is not None and not warm:
+            self._update_velocity(np.linalg.inv(self.T_prev) @ T)
+        self.T_prev = self.T = T
+        self.pipe.set_pose(k, T, xi, self.epoch)
+        self.traj.append((k, t_mid, T.copy()))
+        self._update_stats(t0)
+        on_frame(k, T, self.f_xyz, self.f_attr, self.f_t, self.f_frame, m)
+
+    def _estimate(self, m: int, t_mid: float, warm: bool):
+        xi = self.xi if (self.deskew and not warm) else np.zeros(6, np.float32)
+        if m > 0:
+            self._deskew(m, t_mid, xi)
+        T = self.T if warm else self.predict()
+        if m < self.min_points:
+            self.stats["skipped"] += 1
+            return T, xi, False
+        if warm:
+            return T, xi, True
+        T, ok = self._register(m, T)
+        if ok and self.deskew and self.redeskew and self.T_prev is not None:
+            xi_new = _body_twist(np.linalg.inv(self.T_prev) @ T)
+            if (np.linalg.norm(xi_new[3:] - xi[3:]) > REDESKEW_ROT
+                    or np.linalg.norm(xi_new[:3] - xi[:3]) > REDESKEW_TRANS):
+                xi = xi_new
+                self._deskew(m, t_mid, xi)
+                T, ok = self._register(m, T)
+        return T, xi, ok
+
+    def _update_velocity(self, d: np.ndarray):
+        if np.linalg.norm(d[:3, 3]) < MAX_FRAME_JUMP:
+            self.delta = d
+            self.xi = _body_twist(d)
+            if self.velocity_smooth > 0.0:
+                a = self.velocity_smooth
+                self.xi_s = a * self.xi_s + (1.0 - a) * self.xi.astype(np.float64)
+                self.delta = _twist_matrix(self.xi_s)
+        else:
+            self.delta = np.eye(4)
+            self.xi = np.zeros(6, np.float32)
+            self.xi_s = np.zeros(6)
+
+    def map_stats(self) -> tuple[int, int]:
+        _abi.check(self._lib.lw_map_stats(self._ctx, self._i32), "lw_map_stats")
+        return int(self._i32[0]), int(self._i32[1])
+
+    def _update_stats(self, t0: float):
+        nvox, overflow = self.map_stats()
+        self.stats["frames"] += 1
+        self.stats["warm"] = self.map_frames < self.warmup
+        self.stats["map_voxels"], self.stats["map_full"] = nvox, nvox / self.reg_cap
+        self.stats["map_overflow"] = overflow
+        self.stats["speed"] = float(np.linalg.norm(self.xi[:3]) / self.frame_dt)
+        self.stats["turn"] = float(math.degrees(np.linalg.norm(self.xi[3:])) / self.frame_dt)
+        self.stats["ms"] = (time.perf_counter() - t0) * 1e3
+
+    def _deskew(self, m: int, t_mid: float, xi: np.ndarray):
+        x = np.ascontiguousarray(xi, dtype=np.float32)
+        _abi.check(self._lib.lw_deskew(self._ctx, m, float(t_mid), 1.0 / self.frame_dt, _abi.ptr(x, _abi._f32p)),
+                   "lw_deskew")
+
+    def _is_keyframe(self, T: np.ndarray) -> bool:
+        if self.T_key is None:
+            return True
+        d = np.linalg.inv(self.T_key) @ T
+        ang = math.degrees(np.linalg.norm(Rotation.from_matrix(d[:3, :3]).as_rotvec()))
+        return np.linalg.norm(d[:3, 3]) >= self.kf_dist or ang >= self.kf_angle
+
+    def _insert(self, m: int, T: np.ndarray):
+        T32 = np.ascontiguousarray(T, dtype=np.float32)
+        _abi.check(self._lib.lw_map_insert(self._ctx, None, None, m, _abi.ptr(T32, _abi._f32p), 1.0 / self.reg_voxel,
+                                           self.reg_voxel, self.max_range, int(self.ret_mask)), "lw_map_insert")
+        self.map_frames += 1
+        self.T_key = T.copy()
+        self.stats["keyframes"] += 1
+
+    # ---- registration ----------------------------------------------------------------------
+
+    def reg_params(self) -> _abi.RegParams:
+        p = self._params
+        p.damping = self.damping
+        p.degen_thr = self.degen_thr
+        p.prior_info = self.prior_info
+        p.reg_voxel = self.reg_voxel
+        p.min_range = self.min_range
+        p.max_range = self.max_range
+        p.noise = self.noise
+        p.alpha = self.alpha
+        p.ret_mask = int(self.ret_mask)
+        p.max_iter = int(self.max_iter)
+        p.min_corr = int(self.min_corr)
+        p.wide_iters = int(self.wide_iters)
+        p.acc_mode = ACC_MODES[self.acc_mode]
+        return p
+
+    def _register(self, m: int, T_pred: np.ndarray):
+        """Scan-to-map ICP of the deskewed frame, starting from T_pred. Returns (T, ok)."""
+        Tp = np.ascontiguousarray(T_pred, dtype=np.float64)
+        r = self._result
+        _abi.check(self._lib.lw_register(self._ctx, m, _abi.ctypes.byref(self.reg_params()), _abi.ptr(Tp, _abi._f64p),
+                                         _abi.ctypes.byref(r)), "lw_register")
+        self.stats["ds"] = int(r.n_ds)
+        if r.n_ds < self.min_corr:
+            return T_pred, False
+        T = np.array(r.T, dtype=np.float64).reshape(4, 4)
+        self.stats.update(iters=int(r.iters), corr=int(r.corr), rms=float(r.rms), cond=float(r.cond),
+                          weak=bool(r.weak), degen=int(r.degen))
+        return T, bool(r.ok)

@@ -1,0 +1,420 @@
+"""Oracle for the scan-to-map odometry path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests, the smoke check and the CPU-baseline leg of the benchmark may import this module, and
+only as the checker: the product never routes through it.
+
+Two layers:
+  * `liblw_oracle.so` (lw_oracle.c): a serial C restatement of the reference's Warp kernels with
+    Warp-CPU semantics (index order, float32 device math, float64 accumulators);
+  * this file: ctypes bindings for it, a second, independent numpy int64 restatement of the key
+    and hash functions, and the reference's host-side Gauss-Newton step restated with the same
+    numpy/scipy routines the reference calls (odom.py:387-455 and 752-803).
+
+Reference files (relative to the reference repository root): python/livox_warp/odom.py,
+python/livox_warp/gpu.py, python/livox_warp/prior_session.py, python/livox_warp/localize.py.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+from scipy.spatial.transform import Rotation
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblw_oracle.so")
+
+# odom.py:72-94
+ACC_SLOTS = 64
+ACC_STRIDE = 64
+ACC_H, ACC_B, ACC_COUNT, ACC_ERR, ACC_RMS, ACC_WEIGHT, ACC_RANGE, ACC_HG = 0, 21, 27, 28, 29, 30, 31, 32
+REG_CAP = 4096
+WIDE_ITERS = 3
+KAPPA_START, KAPPA_DECAY, KAPPA_MIN = 6.0, 0.7, 2.0
+STEP_TOL = 1e-4
+TIGHT_FRACTION = 0.6
+WEAK_COND = 1e-3
+REDESKEW_ROT = 2e-3
+REDESKEW_TRANS = 5e-3
+MAX_FRAME_JUMP = 1.5
+# gpu.py:59-64
+VOXEL_KEY_BITS = 21
+VOXEL_KEY_BIAS = 1 << (VOXEL_KEY_BITS - 1)
+VOXEL_KEY_MASK = (1 << VOXEL_KEY_BITS) - 1
+HASH_MULTIPLIER = -7046029254386353131  # 0x9E3779B97F4A7C15 as int64
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_lib = None
+
+
+def lib():
+    """The compiled restatement (built by `make -C oracle`)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.lwo_voxel_key_ijk.restype = ctypes.c_int64
+        L.lwo_voxel_key_ijk.argtypes = [ctypes.c_int32] * 3
+        L.lwo_slot_hash.restype = ctypes.c_int32
+        L.lwo_slot_hash.argtypes = [ctypes.c_int64]
+        L.lwo_find_slot.restype = ctypes.c_int32
+        L.lwo_find_slot.argtypes = [ctypes.c_int64, _i64p, ctypes.c_int32]
+        L.lwo_frame_append.restype = None
+        L.lwo_frame_append.argtypes = [_f32p, _u32p, _f32p, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, _i32p, _f32p, _u32p, _f32p, _i32p]
+        L.lwo_deskew.restype = None
+        L.lwo_deskew.argtypes = [_f32p, _f32p, ctypes.c_int, ctypes.c_float, ctypes.c_float, _f32p, _f32p, _f32p]
+        L.lwo_ds_insert.restype = None
+        L.lwo_ds_insert.argtypes = [_f32p, _u32p, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_int, ctypes.c_int, _i64p, _i32p, _f32p]
+        L.lwo_reg_insert.restype = None
+        L.lwo_reg_insert.argtypes = [_f32p, _u32p, ctypes.c_int, _f32p, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_int, ctypes.c_int, _i64p, _f32p, _f32p, _i32p, _i32p]
+        L.lwo_eig3.restype = None
+        L.lwo_eig3.argtypes = [_f64p, _f64p, _f64p]
+        L.lwo_voxel_precision.restype = ctypes.c_int
+        L.lwo_voxel_precision.argtypes = [_f32p, _f32p, ctypes.c_int32, ctypes.c_float, ctypes.c_float,
+                                          ctypes.c_float, ctypes.c_float, _f32p, _f32p, _f32p]
+        L.lwo_icp.restype = None
+        L.lwo_icp.argtypes = [_f32p, ctypes.c_int, _f32p, ctypes.c_float, ctypes.c_float, ctypes.c_int, _i64p, _f32p,
+                              _f32p, _i32p, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                              ctypes.c_float, ctypes.c_float, _f64p, _i32p]
+        L.lwo_reduce.restype = None
+        L.lwo_reduce.argtypes = [_f64p, _f64p]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray, t):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(t)
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+# ---- keys and hash: an independent numpy int64 restatement (gpu.py:231-252) -----------------------
+
+
+def voxel_key_ijk_np(ix, iy, iz) -> np.ndarray:
+    """gpu.py:231-240 in numpy int64 (wrapping two's-complement arithmetic)."""
+    b, m = np.int64(VOXEL_KEY_BIAS), np.int64(VOXEL_KEY_MASK)
+    x = (np.asarray(ix, np.int64) + b) & m
+    y = (np.asarray(iy, np.int64) + b) & m
+    z = (np.asarray(iz, np.int64) + b) & m
+    return x | (y << np.int64(VOXEL_KEY_BITS)) | (z << np.int64(2 * VOXEL_KEY_BITS))
+
+
+def slot_hash_np(k) -> np.ndarray:
+    """gpu.py:247-252 in numpy int64: h = k * MUL (wrapping); h ^= h >> 29 (arithmetic); h & 0x7fffffff."""
+    with np.errstate(over="ignore"):
+        h = np.asarray(k, np.int64) * np.int64(HASH_MULTIPLIER)
+    h = h ^ (h >> np.int64(29))
+    return (h & np.int64(0x7FFFFFFF)).astype(np.int32)
+
+
+def voxel_key_np(w: np.ndarray, inv: float) -> np.ndarray:
+    """gpu.py:242-245: float32 multiply, floor, int32, then the key."""
+    w = _f32(w)
+    ijk = np.floor(w * np.float32(inv)).astype(np.int32)
+    return voxel_key_ijk_np(ijk[..., 0], ijk[..., 1], ijk[..., 2])
+
+
+def transform_points_f32(T, p: np.ndarray) -> np.ndarray:
+    """wp.transform_point in float32 with the row order ((m0 p0 + m1 p1) + m2 p2) + m3 (no fused multiply-add)."""
+    T = _f32(np.asarray(T).reshape(4, 4))
+    p = _f32(p)
+    out = np.empty_like(p)
+    for r in range(3):
+        a = T[r, 0] * p[:, 0]
+        a = a + T[r, 1] * p[:, 1]
+        a = a + T[r, 2] * p[:, 2]
+        out[:, r] = a + T[r, 3]
+    return out
+
+
+# ---- thin wrappers over the C restatement ---------------------------------------------------------
+
+
+def voxel_key_ijk(ix: int, iy: int, iz: int) -> int:
+    return int(lib().lwo_voxel_key_ijk(ix, iy, iz))
+
+
+def slot_hash(k: int) -> int:
+    return int(lib().lwo_slot_hash(k))
+
+
+class OracleMap:
+    """The registration map of odom.py:468-474 (SoA, per slot) built by the serial k_reg_insert."""
+
+    def __init__(self, slots: int):
+        assert slots & (slots - 1) == 0
+        self.cap = slots
+        self.keys = np.full(slots, -1, np.int64)
+        self.rsum = np.zeros((slots, 3), np.float32)
+        self.router = np.zeros((slots, 3, 3), np.float32)
+        self.rcnt = np.zeros(slots, np.int32)
+        self.stats = np.zeros(2, np.int32)
+
+    def insert(self, xyz, attr, T, voxel: float, max_range: float = 60.0, ret_mask: int = 0b1111):
+        """odom.py:186-230, driven as Odometry._insert drives it (odom.py:719-727)."""
+        xyz = _f32(xyz).reshape(-1, 3)
+        attr = np.ascontiguousarray(attr, np.uint32)
+        T32 = _f32(np.asarray(T, np.float64).reshape(4, 4))
+        lib().lwo_reg_insert(_p(xyz, _f32p), _p(attr, _u32p), len(xyz), _p(T32, _f32p), 1.0 / voxel, voxel,
+                             max_range, int(ret_mask), self.cap - 1, _p(self.keys, _i64p), _p(self.rsum, _f32p),
+                             _p(self.router, _f32p), _p(self.rcnt, _i32p), _p(self.stats, _i32p))
+
+    def find_slot(self, key: int) -> int:
+        return int(lib().lwo_find_slot(key, _p(self.keys, _i64p), self.cap - 1))
+
+    def occupied(self) -> np.ndarray:
+        return self.keys != -1
+
+
+def frame_append(sx, sa, st, inv_dt: float, k_min: int, k_open: int, cap: int, count: int,
+                 f_xyz, f_attr, f_t, f_frame) -> int:
+    """odom.py:97-123; returns the new count (which, as in the reference, may exceed cap)."""
+    sx, st = _f32(sx).reshape(-1, 3), _f32(st)
+    sa = np.ascontiguousarray(sa, np.uint32)
+    c = np.array([count], np.int32)
+    lib().lwo_frame_append(_p(sx, _f32p), _p(sa, _u32p), _p(st, _f32p), len(st), inv_dt, k_min, k_open, cap,
+                           _p(c, _i32p), _p(f_xyz, _f32p), _p(f_attr, _u32p), _p(f_t, _f32p), _p(f_frame, _i32p))
+    return int(c[0])
+
+
+def deskew(raw, t, t_mid: float, inv_dt: float, xi) -> np.ndarray:
+    """odom.py:163-178 in float32."""
+    raw, t = _f32(raw).reshape(-1, 3), _f32(t)
+    xi = _f32(xi)
+    rho, theta = _f32(xi[:3]), _f32(xi[3:])
+    out = np.empty_like(raw)
+    lib().lwo_deskew(_p(raw, _f32p), _p(t, _f32p), len(t), t_mid, inv_dt, _p(rho, _f32p), _p(theta, _f32p),
+                     _p(out, _f32p))
+    return out
+
+
+def deskew_fp64(p: np.ndarray, s: np.ndarray, xi) -> np.ndarray:
+    """prior_session.py:51-57, the reference's own host restatement of gpu.deskew, in float64."""
+    p = np.asarray(p, np.float64)
+    if xi is None or not np.any(xi):
+        return p
+    rv = s[:, None] * np.asarray(xi[3:], dtype=np.float64)[None, :]
+    return Rotation.from_rotvec(rv).apply(p) + s[:, None] * np.asarray(xi[:3], dtype=np.float64)[None, :]
+
+
+def ds_insert(xyz, attr, inv_v: float, min_range: float, max_range: float, ret_mask: int, ds_cap: int):
+    """odom.py:126-160 with a freshly cleared table of ds_cap slots (odom.py:731-737): the thinned points in
+    index order (the serial semantics)."""
+    xyz = _f32(xyz).reshape(-1, 3)
+    attr = np.ascontiguousarray(attr, np.uint32)
+    keys = np.full(ds_cap, -1, np.int64)
+    cnt = np.zeros(1, np.int32)
+    out = np.zeros_like(xyz)
+    lib().lwo_ds_insert(_p(xyz, _f32p), _p(attr, _u32p), len(xyz), inv_v, min_range, max_range, int(ret_mask),
+                        ds_cap - 1, _p(keys, _i64p), _p(cnt, _i32p), _p(out, _f32p))
+    return out[: int(cnt[0])].copy()
+
+
+def voxel_precision(rsum, router, c: int, voxel: float, noise2: float, tight2: float, alpha: float):
+    """(shape flag, M6, Mg6, q) of one voxel: the eigen-weighting of odom.py:336-368."""
+    rs, ro = _f32(rsum), _f32(router)
+    M6, Mg6, q = np.zeros(6, np.float32), np.zeros(6, np.float32), np.zeros(3, np.float32)
+    shape = lib().lwo_voxel_precision(_p(rs, _f32p), _p(ro, _f32p), int(c), voxel, noise2, tight2, alpha,
+                                      _p(M6, _f32p), _p(Mg6, _f32p), _p(q, _f32p))
+    return int(shape), M6, Mg6, q
+
+
+def eig3(A) -> tuple[np.ndarray, np.ndarray]:
+    A = np.ascontiguousarray(A, np.float64).reshape(3, 3)
+    ev, Q = np.zeros(3), np.zeros((3, 3))
+    lib().lwo_eig3(_p(A, _f64p), _p(ev, _f64p), _p(Q, _f64p))
+    return ev, Q
+
+
+def icp_accumulate(omap: OracleMap, ds, T, voxel: float, reach: int, max_d2: float, noise2: float, tight2: float,
+                   alpha: float, kappa: float, with_slots: bool = False):
+    """One k_icp pass + k_reduce (odom.py:275-384): the 64 reduced float64 entries of the ACC layout."""
+    ds = _f32(ds).reshape(-1, 3)
+    T32 = _f32(np.asarray(T, np.float64).reshape(4, 4))
+    acc = np.zeros(ACC_SLOTS * ACC_STRIDE, np.float64)
+    best = np.zeros(len(ds), np.int32) if with_slots else None
+    lib().lwo_icp(_p(ds, _f32p), len(ds), _p(T32, _f32p), 1.0 / voxel, voxel, omap.cap - 1, _p(omap.keys, _i64p),
+                  _p(omap.rsum, _f32p), _p(omap.router, _f32p), _p(omap.rcnt, _i32p), reach, max_d2, noise2, tight2,
+                  alpha, kappa, _p(acc, _f64p), _p(best, _i32p) if best is not None else None)
+    out = np.zeros(ACC_STRIDE, np.float64)
+    lib().lwo_reduce(_p(acc, _f64p), _p(out, _f64p))
+    return (out, best) if with_slots else out
+
+
+# ---- host step (odom.py:387-455), restated with the routines the reference calls --------------------
+
+_TRIU6 = np.triu_indices(6)
+
+
+def sym6(upper: np.ndarray) -> np.ndarray:
+    H = np.zeros((6, 6))
+    H[_TRIU6] = upper
+    return H + np.triu(H, 1).T
+
+
+def twist_between(T_from: np.ndarray, T_to: np.ndarray) -> np.ndarray:
+    M = T_to @ np.linalg.inv(T_from)
+    theta = Rotation.from_matrix(M[:3, :3]).as_rotvec()
+    return np.concatenate([M[:3, 3], theta])
+
+
+def apply_twist(T: np.ndarray, delta: np.ndarray) -> np.ndarray:
+    Rz = Rotation.from_rotvec(delta[3:]).as_matrix()
+    out = T.copy()
+    out[:3, :3] = Rz @ T[:3, :3]
+    out[:3, 3] = Rz @ T[:3, 3] + delta[:3]
+    return out
+
+
+def body_twist(d: np.ndarray) -> np.ndarray:
+    return np.concatenate([d[:3, 3], Rotation.from_matrix(d[:3, :3]).as_rotvec()]).astype(np.float32)
+
+
+def twist_matrix(xi: np.ndarray) -> np.ndarray:
+    out = np.eye(4)
+    out[:3, :3] = Rotation.from_rotvec(np.asarray(xi[3:], dtype=np.float64)).as_matrix()
+    out[:3, 3] = xi[:3]
+    return out
+
+
+def sensor_basis(T: np.ndarray, L: float) -> np.ndarray:
+    c = T[:3, 3]
+    J = np.eye(6)
+    J[:3, 3:] = np.array([[0.0, -c[2], c[1]], [c[2], 0.0, -c[0]], [-c[1], c[0], 0.0]])
+    B = np.zeros((6, 6))
+    B[:3, :3] = B[3:, 3:] = T[:3, :3]
+    return J @ B @ np.diag([1.0, 1.0, 1.0, 1.0 / L, 1.0 / L, 1.0 / L])
+
+
+def constrained_step(A, rhs, Hg, T, L, thr, to_pred):
+    J = sensor_basis(T, L)
+    lam, V = np.linalg.eigh(J.T @ Hg @ J)
+    good = lam >= thr
+    if good.all():
+        return np.linalg.solve(A, rhs), (lam, V, good)
+    Vd = V[:, ~good]
+    d0 = Vd @ (Vd.T @ np.linalg.solve(J, to_pred))
+    if good.any():
+        Vg = V[:, good]
+        As = J.T @ A @ J
+        d0 = d0 + Vg @ np.linalg.solve(Vg.T @ As @ Vg, Vg.T @ (J.T @ rhs - As @ d0))
+    return J @ d0, (lam, V, good)
+
+
+@dataclass
+class Params:
+    """The Odometry tuning attributes of odom.py:499-525 that registration reads."""
+
+    reg_voxel: float = 0.15
+    max_iter: int = 16
+    min_corr: int = 60
+    min_range: float = 0.3
+    max_range: float = 60.0
+    noise: float = 0.02
+    ret_mask: int = 0b1111
+    alpha: float = 0.05
+    damping: float = 1.0e-3
+    degen_thr: float = 3000.0
+    prior_info: float = 2000.0
+    wide_iters: int = WIDE_ITERS
+
+
+@dataclass
+class RegResult:
+    T: np.ndarray
+    ok: bool
+    iters: int = 0
+    corr: int = 0
+    rms: float = 0.0
+    cond: float = 1.0
+    weak: bool = False
+    degen: int = 0
+    n_ds: int = 0
+    accs: list = field(default_factory=list)
+
+
+def gn_step(v: np.ndarray, T: np.ndarray, T_pred: np.ndarray, P: Params):
+    """The body of one iteration of odom.py:768-793 after the accumulation, given the reduced accumulator v.
+    Returns (delta or None on LinAlgError, n_degen, rms, H)."""
+    corr = int(v[ACC_COUNT])
+    H, b, Hg = sym6(v[ACC_H:ACC_H + 21]), v[ACC_B:ACC_B + 6], sym6(v[ACC_HG:ACC_HG + 21])
+    L = max(v[ACC_RANGE] / corr, 1.0)
+    rms = math.sqrt(max(v[ACC_RMS], 0.0) / max(v[ACC_WEIGHT], 1e-9))
+    dg = np.diag(H)
+    A = H + np.diag(1e-9 * dg.mean() + P.damping * dg)
+    rhs = -b
+    xi = twist_between(T, T_pred)
+    if P.prior_info > 0.0:
+        Ji = np.linalg.inv(sensor_basis(T, L))
+        Pm = P.prior_info * (Ji.T @ Ji)
+        A = A + Pm
+        rhs = rhs + Pm @ xi
+    n_degen = 0
+    try:
+        if P.degen_thr > 0.0:
+            delta, (_, _, kept) = constrained_step(A, rhs, Hg, T, L, P.degen_thr, xi)
+            n_degen = int((~kept).sum())
+        else:
+            delta = np.linalg.solve(A, rhs)
+    except np.linalg.LinAlgError:
+        return None, n_degen, rms, H
+    return delta, n_degen, rms, H
+
+
+def register(omap: OracleMap, ds: np.ndarray, T_pred: np.ndarray, P: Params, keep_accs: bool = False) -> RegResult:
+    """odom.py:752-803 from the thinned points onward (the thinning itself is ds_insert)."""
+    n_ds = len(ds)
+    if n_ds < P.min_corr:
+        return RegResult(T=T_pred.copy(), ok=False, n_ds=n_ds)
+    T = T_pred.copy()
+    voxel = P.reg_voxel
+    noise2 = np.float32(P.noise * P.noise)
+    tight2 = np.float32((TIGHT_FRACTION * voxel / math.sqrt(12.0)) ** 2)
+    iters, corr, rms, cond, n_degen = 0, 0, 0.0, 1.0, 0
+    H = np.eye(6)
+    accs = []
+    for it in range(P.max_iter):
+        reach = 2 if it < P.wide_iters else 1
+        max_d2 = ((reach + 0.5) * voxel) ** 2
+        kappa = max(KAPPA_MIN, KAPPA_START * (KAPPA_DECAY ** it))
+        v = icp_accumulate(omap, ds, T, voxel, reach, max_d2, float(noise2), float(tight2), P.alpha, kappa)
+        if keep_accs:
+            accs.append((T.copy(), reach, max_d2, kappa, v.copy()))
+        corr = int(v[ACC_COUNT])
+        if corr < P.min_corr:
+            break
+        delta, n_degen, rms, H = gn_step(v, T, T_pred, P)
+        if delta is None:
+            break
+        T = apply_twist(T, delta)
+        iters = it + 1
+        if np.linalg.norm(delta[:3]) < STEP_TOL and np.linalg.norm(delta[3:]) < STEP_TOL:
+            break
+    if corr >= P.min_corr:
+        dg = np.sqrt(np.maximum(np.diag(H), 1e-12))
+        ev = np.linalg.eigvalsh(H / np.outer(dg, dg))
+        cond = float(max(ev[0], 0.0) / max(ev[-1], 1e-12))
+    return RegResult(T=T, ok=corr >= P.min_corr, iters=iters, corr=corr, rms=rms, cond=cond,
+                     weak=cond < WEAK_COND or n_degen > 0, degen=n_degen, n_ds=n_ds, accs=accs)
+
+
+def pose_error(A: np.ndarray, B: np.ndarray) -> tuple[float, float]:
+    """(translation m, rotation deg) between two 4x4 poses."""
+    d = np.linalg.inv(A) @ B
+    ang = math.degrees(math.acos(np.clip((np.trace(d[:3, :3]) - 1) / 2, -1, 1)))
+    return float(np.linalg.norm(A[:3, 3] - B[:3, 3])), ang
