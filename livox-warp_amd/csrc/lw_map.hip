@@ -216,8 +216,10 @@ __global__ __launch_bounds__(BLOCK) void k_ins_finish(const int32_t *touched, in
 __global__ __launch_bounds__(BLOCK) void k_refresh(int32_t all, int32_t *ctr, const int32_t *dirty_list,
                                                    int32_t *vox_dirty, const int64_t *vox_key,
                                                    const VoxMom *vox_mom, float4 *vox_mu, VoxPrec *vox_prec,
-                                                   float voxel, float noise2, float tight2, float alpha) {
+                                                   float voxel, float noise2, float tight2, float alpha,
+                                                   int64_t vox_cap) {
     int32_t cnt = all ? ctr[2] : ctr[3];
+    if (cnt > vox_cap) cnt = (int32_t)vox_cap;
     for (int32_t j = blockIdx.x * BLOCK + threadIdx.x; j < cnt; j += gridDim.x * BLOCK) {
         int32_t vid = all ? j : dirty_list[j];
         vox_dirty[vid] = 0;
@@ -306,7 +308,7 @@ hipError_t launch_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, 
 hipError_t launch_refresh(lw_ctx *c, bool all, float voxel, float noise2, float tight2, float alpha) {
     hipLaunchKernelGGL(k_refresh, dim3(all ? 4096 : 512), dim3(BLOCK), 0, c->stream, all ? 1 : 0, c->ctr,
                        c->dirty_list, c->vox_dirty, c->vox_key, c->vox_mom, c->vox_mu, c->vox_prec, voxel, noise2,
-                       tight2, alpha);
+                       tight2, alpha, c->vox_cap);
     hipLaunchKernelGGL(k_set_ctr, dim3(1), dim3(1), 0, c->stream, c->ctr, 3, 0);
     return hipGetLastError();
 }
