@@ -131,6 +131,15 @@ int lw_register(lw_ctx *ctx, int32_t m, const lw_reg_params *p, const double *T_
 /* The same loop over the already-thinned points (lw_thin / lw_thin_load), n of them. */
 int lw_register_thinned(lw_ctx *ctx, int32_t n, const lw_reg_params *p, const double *T_pred, lw_reg_result *res);
 
+/* Live measurement of the accumulation launches (for throughput and roofline reports): lw_prof_enable
+ * turns on HIP events around each accumulation launch and device counters; lw_prof_read returns, since
+ * lw_prof_reset: out[0] launches timed, out[1] their total duration (ms), out[2] point-iterations, out[3]
+ * neighbour lookups, out[4] occupied neighbours hit, out[5] matched points, out[6] launches that did work,
+ * out[7] reserved. No counterpart in the reference (its timing is host wall time, odom.py:644, 705). */
+int lw_prof_enable(lw_ctx *ctx, int32_t on);
+int lw_prof_reset(lw_ctx *ctx);
+int lw_prof_read(lw_ctx *ctx, double *out);
+
 /* Host builds of the device's float64 step and per-voxel precision, for tests on machines without a
  * GPU (same source as the device code). */
 int lw_host_gn_step(const double *v, const double *T, const double *T_pred, double damping, double prior_info,

@@ -49,7 +49,25 @@ void free_all(lw_ctx *c) {
                     c->dtmp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    if (c->prof_dev) (void)hipFree(c->prof_dev);
+    if (c->prof_ev) {
+        for (int i = 0; i < 2 * c->prof_cap; ++i) (void)hipEventDestroy(c->prof_ev[i]);
+        delete[] c->prof_ev;
+    }
     if (c->pin) (void)hipHostFree(c->pin);
+}
+
+// Add the elapsed times of the recorded launch pairs (call after the stream has been synchronised).
+hipError_t prof_harvest(lw_ctx *c) {
+    for (int i = 0; i < c->prof_used; ++i) {
+        float ms = 0.0f;
+        hipError_t e = hipEventElapsedTime(&ms, c->prof_ev[2 * i], c->prof_ev[2 * i + 1]);
+        if (e != hipSuccess) return e;
+        c->prof_ms += ms;
+        c->prof_launches++;
+    }
+    c->prof_used = 0;
+    return hipSuccess;
 }
 
 char *pin_up(lw_ctx *c) { return (char *)c->pin + PIN_UP; }
@@ -319,6 +337,7 @@ int lw_icp_accum(lw_ctx *c, int32_t n, const float *T, float inv_v, float voxel,
     if (best_slot && n > 0)
         LW_CK(hipMemcpyAsync(best_slot, c->best_slot, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
     LW_CK(hipStreamSynchronize(c->stream));
+    LW_CK(prof_harvest(c));
     std::memcpy(out, pin_down(c), 64 * sizeof(double));
     return 0;
 }
@@ -361,6 +380,7 @@ int lw_register_thinned(lw_ctx *c, int32_t n, const lw_reg_params *p, const doub
     LW_CK(lw::launch_reg_finish(c, p->min_corr));
     LW_CK(hipMemcpyAsync(pin_down(c), c->reg, sizeof(lw::RegState), hipMemcpyDeviceToHost, c->stream));
     LW_CK(hipStreamSynchronize(c->stream));
+    LW_CK(prof_harvest(c));
     lw::RegState st;
     std::memcpy(&st, pin_down(c), sizeof(st));
     std::memcpy(res->T, st.T, 16 * sizeof(double));
@@ -381,6 +401,56 @@ int lw_register(lw_ctx *c, int32_t m, const lw_reg_params *p, const double *T_pr
     int rc = lw_thin(c, m, (float)(2.0 / (double)p->reg_voxel), p->min_range, p->max_range, p->ret_mask, &n_ds);
     if (rc) return rc;
     return lw_register_thinned(c, n_ds, p, T_pred, res);
+}
+
+int lw_prof_enable(lw_ctx *c, int32_t on) {
+    if (!c) return LW_EINVAL;
+    if (on && !c->prof_ev) {
+        const int cap = 512;
+        c->prof_ev = new (std::nothrow) hipEvent_t[2 * cap];
+        if (!c->prof_ev) return LW_EINVAL;
+        for (int i = 0; i < 2 * cap; ++i) {
+            hipError_t e = hipEventCreate(&c->prof_ev[i]);
+            if (e != hipSuccess) {
+                for (int j = 0; j < i; ++j) (void)hipEventDestroy(c->prof_ev[j]);
+                delete[] c->prof_ev;
+                c->prof_ev = nullptr;
+                return -(int)e;
+            }
+        }
+        c->prof_cap = cap;
+        LW_CK(hipMalloc((void **)&c->prof_dev, 16 * sizeof(unsigned long long)));
+        LW_CK(hipMemsetAsync(c->prof_dev, 0, 16 * sizeof(unsigned long long), c->stream));
+        LW_CK(hipStreamSynchronize(c->stream));
+    }
+    c->prof_on = on != 0;
+    return 0;
+}
+
+int lw_prof_reset(lw_ctx *c) {
+    if (!c) return LW_EINVAL;
+    LW_CK(hipStreamSynchronize(c->stream));
+    c->prof_used = 0;
+    c->prof_ms = 0.0;
+    c->prof_launches = 0;
+    if (c->prof_dev) {
+        LW_CK(hipMemsetAsync(c->prof_dev, 0, 16 * sizeof(unsigned long long), c->stream));
+        LW_CK(hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+int lw_prof_read(lw_ctx *c, double *out) {
+    if (!c || !out) return LW_EINVAL;
+    LW_CK(hipStreamSynchronize(c->stream));
+    LW_CK(prof_harvest(c));
+    unsigned long long v[16] = {0};
+    if (c->prof_dev) LW_CK(hipMemcpy(v, c->prof_dev, sizeof(v), hipMemcpyDeviceToHost));
+    out[0] = (double)c->prof_launches;
+    out[1] = c->prof_ms;
+    for (int i = 0; i < 5; ++i) out[2 + i] = (double)v[i];
+    out[7] = 0.0;
+    return 0;
 }
 
 // ---- host builds of the device math (tests) ---------------------------------------------------------

@@ -99,6 +99,7 @@ struct IcpArgs {
     RegState *st;
     double *partials;  // [G][128]: sums, compensations
     int32_t *best_slot;
+    unsigned long long *prof;  // measurement counters (see lw_ctx), or null
     int32_t n;
     int32_t mask;
     float inv_v, voxel, max_d2, kappa;
@@ -115,7 +116,7 @@ struct IcpArgs {
 
 // Closest voxel mean within reach (odom.py:308-334); returns the voxel id or -1, and its slot.
 __device__ inline int32_t closest_mean(const IcpArgs &a, float wx, float wy, float wz, float &mx, float &my, float &mz,
-                                       int32_t &best_s) {
+                                       int32_t &best_s, int32_t &hits) {
     const int32_t bx = ifloor(wx * a.inv_v), by = ifloor(wy * a.inv_v), bz = ifloor(wz * a.inv_v);
     float best_d2 = a.max_d2;
     int32_t best_vid = -1;
@@ -139,6 +140,7 @@ __device__ inline int32_t closest_mean(const IcpArgs &a, float wx, float wy, flo
                     }
                 }
                 if (vid < 0) continue;
+                ++hits;
                 float4 mu = a.vox_mu[vid];
                 if (!(mu.w > 0.0f)) continue;  // count 0 (odom.py:324)
                 float d0 = wx - mu.x, d1 = wy - mu.y, d2 = wz - mu.z;
@@ -158,15 +160,17 @@ __device__ inline int32_t closest_mean(const IcpArgs &a, float wx, float wy, flo
 }
 
 template <int MODE>
-__device__ inline void point_terms(const IcpArgs &a, const float *Tf, int32_t i, Acc<MODE> &acc) {
+__device__ inline void point_terms(const IcpArgs &a, const float *Tf, int32_t i, Acc<MODE> &acc, int32_t &hits,
+                                   int32_t &matched) {
     const float px = a.ds[3 * (int64_t)i], py = a.ds[3 * (int64_t)i + 1], pz = a.ds[3 * (int64_t)i + 2];
     float wx, wy, wz;
     transform_point(Tf, px, py, pz, wx, wy, wz);
     float mx = 0.0f, my = 0.0f, mz = 0.0f;
     int32_t bs;
-    int32_t vid = closest_mean(a, wx, wy, wz, mx, my, mz, bs);
+    int32_t vid = closest_mean(a, wx, wy, wz, mx, my, mz, bs, hits);
     if (a.best_slot) a.best_slot[i] = bs;
     if (vid < 0) return;
+    ++matched;
     const float r0 = wx - mx, r1 = wy - my, r2 = wz - mz;
     const VoxPrec P = a.vox_prec[vid];
     const float M[3][3] = {{P.M[0], P.M[1], P.M[2]}, {P.M[1], P.M[3], P.M[4]}, {P.M[2], P.M[4], P.M[5]}};
@@ -249,8 +253,19 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_icp(IcpArgs a) {
     __syncthreads();
     Acc<MODE> acc;
     acc.zero();
+    int32_t hits = 0, matched = 0;
     for (int32_t i = blockIdx.x * ICP_BLOCK + threadIdx.x; i < a.n; i += gridDim.x * ICP_BLOCK)
-        point_terms<MODE>(a, Tf, i, acc);
+        point_terms<MODE>(a, Tf, i, acc, hits, matched);
+    if (a.prof) {
+        LW_UNROLL for (int off = 32; off > 0; off >>= 1) {
+            hits += __shfl_xor(hits, off, 64);
+            matched += __shfl_xor(matched, off, 64);
+        }
+        if (threadIdx.x == 0) {
+            atomicAdd(&a.prof[2], (unsigned long long)hits);
+            atomicAdd(&a.prof[3], (unsigned long long)matched);
+        }
+    }
     acc.wave_reduce();
     double *row = a.partials + (size_t)blockIdx.x * 128;
     // lane k stores entry k (lanes 53..63 idle); the reduced values are uniform across the wave
@@ -289,6 +304,12 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_icp(IcpArgs a) {
     if (k < 64) st->acc[k] = v[k];
     if (k == 0) {
         st->ticket = 0;
+        if (a.prof) {
+            const unsigned long long K = (unsigned long long)((2 * a.reach + 1) * (2 * a.reach + 1) * (2 * a.reach + 1));
+            atomicAdd(&a.prof[0], (unsigned long long)a.n);
+            atomicAdd(&a.prof[1], (unsigned long long)a.n * K);
+            atomicAdd(&a.prof[4], 1ull);
+        }
         if (a.gn) gauss_newton(a, v);
     }
 }
@@ -338,6 +359,7 @@ hipError_t launch_icp_pass(lw_ctx *c, const IcpPass &p, int acc_mode) {
     a.st = c->reg;
     a.partials = c->partials;
     a.best_slot = p.want_slots ? c->best_slot : nullptr;
+    a.prof = c->prof_on ? c->prof_dev : nullptr;
     a.n = p.n;
     a.mask = c->mask;
     a.inv_v = p.inv_v;
@@ -351,13 +373,24 @@ hipError_t launch_icp_pass(lw_ctx *c, const IcpPass &p, int acc_mode) {
     int g = (p.n + ICP_BLOCK - 1) / ICP_BLOCK;
     if (g < 1) g = 1;
     if (g > ICP_MAX_GROUPS) g = ICP_MAX_GROUPS;
+    const bool timed = c->prof_on && c->prof_used < c->prof_cap;
+    if (timed) {
+        hipError_t e = hipEventRecord(c->prof_ev[2 * c->prof_used], c->stream);
+        if (e != hipSuccess) return e;
+    }
     switch (acc_mode) {
     case 0: hipLaunchKernelGGL(k_icp<0>, dim3(g), dim3(ICP_BLOCK), 0, c->stream, a); break;
     case 1: hipLaunchKernelGGL(k_icp<1>, dim3(g), dim3(ICP_BLOCK), 0, c->stream, a); break;
     case 2: hipLaunchKernelGGL(k_icp<2>, dim3(g), dim3(ICP_BLOCK), 0, c->stream, a); break;
     default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (timed) {
+        e = hipEventRecord(c->prof_ev[2 * c->prof_used + 1], c->stream);
+        c->prof_used++;
+    }
+    return e;
 }
 
 }  // namespace lw

@@ -82,6 +82,15 @@ struct lw_ctx {
     // pinned host staging
     void *pin = nullptr;
     size_t pin_bytes = 0;
+    // live measurement of the accumulation launches: HIP event pairs around each launch, and device
+    // counters [0] point-iterations [1] neighbour lookups [2] occupied neighbours hit [3] matched points
+    // [4] launches that did work
+    bool prof_on = false;
+    hipEvent_t *prof_ev = nullptr;
+    int32_t prof_cap = 0, prof_used = 0;
+    double prof_ms = 0.0;
+    int64_t prof_launches = 0;
+    unsigned long long *prof_dev = nullptr;
     // parameters the cached per-voxel precision was computed with
     bool prec_valid = false;
     float prec_voxel = 0, prec_noise2 = 0, prec_tight2 = 0, prec_alpha = 0;

@@ -205,8 +205,9 @@ class Walk:
     """A seeded smooth handheld walk around an elliptical loop: speed 0.5-1.2 m/s, yaw following the heading
     plus a look-around swing (|yaw rate| <= 60 deg/s), and smooth Gaussian jitter in every axis."""
 
-    def __init__(self, seed: int = 4, centre=(10.0, 7.5), radii=(6.0, 4.2), height: float = 1.4):
+    def __init__(self, seed: int = 4, centre=(10.0, 7.5), radii=(6.0, 4.2), height: float = 1.4, hold: float = 1.5):
         rng = np.random.default_rng(seed)
+        self.hold = hold
         self.c = np.asarray(centre, float)
         self.ab = np.asarray(radii, float)
         self.h = height
@@ -230,11 +231,17 @@ class Walk:
         self._su = np.concatenate([[0.0], np.cumsum(0.5 * (du[1:] + du[:-1]) * np.diff(u))])
         self.perimeter = self._su[-1]
 
+    def ramp(self, t):
+        """0 while the sensor holds still (the odometry's warm-up frames assume it does, as the reference's
+        simulator holds for 1.5 s, sources.py HOLD), then a smooth 1 s ramp up to 1."""
+        u = np.clip((np.asarray(t, float) - self.hold) / 1.0, 0.0, 1.0)
+        return u * u * (3.0 - 2.0 * u)
+
     def speed(self, t):
         t = np.asarray(t, float)
         s = 0.85 + 0.22 * np.sin(2 * math.pi * self.v_f[0] * t + self.v_ph[0]) \
             + 0.12 * np.sin(2 * math.pi * self.v_f[1] * t + self.v_ph[1])
-        return np.clip(s, 0.5, 1.2)
+        return np.clip(s, 0.5, 1.2) * self.ramp(t)
 
     def _param(self, t):
         s = np.interp(t, self._tg, self._s)
@@ -242,8 +249,9 @@ class Walk:
         return np.interp(rem, self._su, self._u) + 2 * math.pi * lap + self.phase0
 
     def _jit(self, t, k):
+        g = self.ramp(t)
         t = np.asarray(t, float)[..., None]
-        return np.sum(self.j_amp[k] * np.sin(2 * math.pi * self.j_f[k] * t + self.j_ph[k]), axis=-1)
+        return g * np.sum(self.j_amp[k] * np.sin(2 * math.pi * self.j_f[k] * t + self.j_ph[k]), axis=-1)
 
     def state(self, t):
         """(position (..., 3), yaw, pitch, roll) at times t."""
@@ -252,7 +260,7 @@ class Walk:
         x = self.c[0] + self.ab[0] * np.cos(u)
         y = self.c[1] + self.ab[1] * np.sin(u)
         heading = np.arctan2(self.ab[1] * np.cos(u), -self.ab[0] * np.sin(u))
-        yaw = heading + self.look_a * np.sin(2 * math.pi * self.look_f * t) + self._jit(t, 3)
+        yaw = heading + self.ramp(t) * self.look_a * np.sin(2 * math.pi * self.look_f * t) + self._jit(t, 3)
         p = np.stack([x + self._jit(t, 0), y + self._jit(t, 1), self.h + self._jit(t, 2)], axis=-1)
         return p, yaw, self._jit(t, 4), self._jit(t, 5)
 

@@ -418,3 +418,185 @@ def pose_error(A: np.ndarray, B: np.ndarray) -> tuple[float, float]:
     d = np.linalg.inv(A) @ B
     ang = math.degrees(math.acos(np.clip((np.trace(d[:3, :3]) - 1) / 2, -1, 1)))
     return float(np.linalg.norm(A[:3, 3] - B[:3, 3])), ang
+
+
+class OracleOdometry:
+    """odom.py:458-803 over host arrays: frame cutting, warm-up, deskew, thinning, registration with the numpy
+    host step every iteration, keyframe inserts, constant-velocity prediction. Poses go to `pipe` like the
+    reference's (set_pose / set_pose_range)."""
+
+    def __init__(self, pipe, reg_slots: int = 1 << 21, frame_cap: int = 1 << 18):
+        self.pipe = pipe
+        self.reg_cap = reg_slots
+        self.frame_cap = frame_cap
+        self.ds_cap = 1 << (frame_cap - 1).bit_length()
+        self.f_raw = np.zeros((frame_cap, 3), np.float32)
+        self.f_xyz = np.zeros((frame_cap, 3), np.float32)
+        self.f_attr = np.zeros(frame_cap, np.uint32)
+        self.f_t = np.zeros(frame_cap, np.float32)
+        self.f_frame = np.zeros(frame_cap, np.int32)
+        self.f_count = 0
+        self.frame_dt = 0.1
+        self.reg_voxel = 0.15
+        self.max_iter = 16
+        self.warmup = 8
+        self.max_gap = 20
+        self.min_points = 200
+        self.min_corr = 60
+        self.min_range = 0.3
+        self.max_range = 60.0
+        self.noise = 0.02
+        self.ret_mask = 0b1111
+        self.alpha = 0.05
+        self.damping = 1.0e-3
+        self.velocity_smooth = 0.85
+        self.degen_thr = 3000.0
+        self.prior_info = 2000.0
+        self.kf_dist = 0.05
+        self.kf_angle = 1.0
+        self.deskew = True
+        self.redeskew = True
+        self.epoch = 0
+        self.reset(np.eye(4))
+
+    def reset(self, T0):
+        self.map = OracleMap(self.reg_cap)
+        self.T = np.array(T0, dtype=np.float64)
+        self.T_prev = None
+        self.delta = np.eye(4)
+        self.xi = np.zeros(6, np.float32)
+        self.xi_s = np.zeros(6)
+        self.k_open = None
+        self.k_pred = -1
+        self.map_frames = 0
+        self.T_key = None
+        self.traj = []
+        self.stats = {"frames": 0, "iters": 0, "corr": 0, "ds": 0, "rms": 0.0, "cond": 1.0, "weak": False,
+                      "degen": 0, "skipped": 0, "gaps": 0, "keyframes": 0, "map_voxels": 0}
+        self.f_count = 0
+
+    def restart_clock(self):
+        self.k_open = None
+        self.k_pred = -1
+        self.T_prev = None
+        self.delta = np.eye(4)
+        self.xi = np.zeros(6, np.float32)
+        self.xi_s = np.zeros(6)
+        self.f_count = 0
+
+    def predict(self):
+        return self.T @ self.delta
+
+    def params(self) -> Params:
+        return Params(reg_voxel=self.reg_voxel, max_iter=self.max_iter, min_corr=self.min_corr,
+                      min_range=self.min_range, max_range=self.max_range, noise=self.noise, ret_mask=self.ret_mask,
+                      alpha=self.alpha, damping=self.damping, degen_thr=self.degen_thr, prior_info=self.prior_info)
+
+    def push(self, sx, sa, st, n, t_min, t_max, on_frame=None):
+        if n <= 0:
+            return
+        inv_dt = 1.0 / self.frame_dt
+        k_lo = int(math.floor(max(t_min, 0.0) * inv_dt))
+        k_hi = int(math.floor(max(t_max, 0.0) * inv_dt))
+        k_prev = self.k_pred
+        if self.k_open is not None and k_hi - self.k_open > self.max_gap:
+            if self.f_count > 0:
+                self._finalize(self.k_open, on_frame)
+            self.restart_clock()
+            self.stats["gaps"] += 1
+        if self.k_open is None:
+            self.k_open = k_lo if k_hi - k_lo <= self.max_gap else k_hi
+            self.f_count = 0
+            self.pipe.set_pose(self.k_open, self.T, self.xi, self.epoch)
+            lo = (k_prev + 1) if 0 <= k_prev < self.k_open else k_lo
+            if lo < self.k_open:
+                self.pipe.set_pose_range(lo, self.k_open, self.T, None, self.epoch)
+            self.k_pred = self.k_open
+        for k in range(max(self.k_pred + 1, self.k_open + 1), k_hi + 1):
+            self.pipe.set_pose(k, self.predict(), self.xi, self.epoch)
+            self.k_pred = k
+        k_min = self.k_open - 1
+        while True:
+            self.f_count = frame_append(sx, sa, st, inv_dt, k_min, self.k_open, self.frame_cap, self.f_count,
+                                        self.f_raw, self.f_attr, self.f_t, self.f_frame)
+            if k_hi <= self.k_open:
+                break
+            self._finalize(self.k_open, on_frame)
+            self.k_open += 1
+            k_min = self.k_open
+            self.f_count = 0
+            self.pipe.set_pose(self.k_open, self.predict(), self.xi, self.epoch)
+            self.k_pred = max(self.k_pred, self.k_open)
+
+    def _finalize(self, k, on_frame):
+        m = min(self.f_count, self.frame_cap)
+        warm = self.map_frames < self.warmup
+        t_mid = (k + 0.5) * self.frame_dt
+        T, xi, ok = self._estimate(m, t_mid, warm)
+        if ok and (warm or self._is_keyframe(T)):
+            self._insert(m, T)
+        if self.T_prev is not None and not warm:
+            self._update_velocity(np.linalg.inv(self.T_prev) @ T)
+        self.T_prev = self.T = T
+        self.pipe.set_pose(k, T, xi, self.epoch)
+        self.traj.append((k, t_mid, T.copy()))
+        self.stats["frames"] += 1
+        self.stats["map_voxels"] = int(self.map.stats[0])
+        if on_frame is not None:
+            on_frame(k, T, self.f_xyz, self.f_attr, self.f_t, self.f_frame, m)
+
+    def _estimate(self, m, t_mid, warm):
+        xi = self.xi if (self.deskew and not warm) else np.zeros(6, np.float32)
+        if m > 0:
+            self.f_xyz[:m] = deskew(self.f_raw[:m], self.f_t[:m], t_mid, 1.0 / self.frame_dt, xi)
+        T = self.T if warm else self.predict()
+        if m < self.min_points:
+            self.stats["skipped"] += 1
+            return T, xi, False
+        if warm:
+            return T, xi, True
+        T, ok = self._register(m, T)
+        if ok and self.deskew and self.redeskew and self.T_prev is not None:
+            xi_new = body_twist(np.linalg.inv(self.T_prev) @ T)
+            if (np.linalg.norm(xi_new[3:] - xi[3:]) > REDESKEW_ROT
+                    or np.linalg.norm(xi_new[:3] - xi[:3]) > REDESKEW_TRANS):
+                xi = xi_new
+                self.f_xyz[:m] = deskew(self.f_raw[:m], self.f_t[:m], t_mid, 1.0 / self.frame_dt, xi)
+                T, ok = self._register(m, T)
+        return T, xi, ok
+
+    def _update_velocity(self, d):
+        if np.linalg.norm(d[:3, 3]) < MAX_FRAME_JUMP:
+            self.delta = d
+            self.xi = body_twist(d)
+            if self.velocity_smooth > 0.0:
+                a = self.velocity_smooth
+                self.xi_s = a * self.xi_s + (1.0 - a) * self.xi.astype(np.float64)
+                self.delta = twist_matrix(self.xi_s)
+        else:
+            self.delta = np.eye(4)
+            self.xi = np.zeros(6, np.float32)
+            self.xi_s = np.zeros(6)
+
+    def _is_keyframe(self, T):
+        if self.T_key is None:
+            return True
+        d = np.linalg.inv(self.T_key) @ T
+        ang = math.degrees(np.linalg.norm(Rotation.from_matrix(d[:3, :3]).as_rotvec()))
+        return np.linalg.norm(d[:3, 3]) >= self.kf_dist or ang >= self.kf_angle
+
+    def _insert(self, m, T):
+        self.map.insert(self.f_xyz[:m], self.f_attr[:m], T, self.reg_voxel, self.max_range, self.ret_mask)
+        self.map_frames += 1
+        self.T_key = T.copy()
+        self.stats["keyframes"] += 1
+
+    def _register(self, m, T_pred):
+        ds = ds_insert(self.f_xyz[:m], self.f_attr[:m], 2.0 / self.reg_voxel, self.min_range, self.max_range,
+                       self.ret_mask, self.ds_cap)
+        self.stats["ds"] = len(ds)
+        r = register(self.map, ds, T_pred, self.params())
+        if len(ds) < self.min_corr:
+            return T_pred, False
+        self.stats.update(iters=r.iters, corr=r.corr, rms=r.rms, cond=r.cond, weak=r.weak, degen=r.degen)
+        return r.T, r.ok
