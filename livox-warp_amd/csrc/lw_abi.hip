@@ -43,7 +43,7 @@ hipError_t dalloc(T **p, size_t count) {
 bool pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
 
 void free_all(lw_ctx *c) {
-    void *ptrs[] = {c->slots, c->occ, c->vox_key, c->vox_mom, c->vox_mu, c->vox_prec, c->vox_nin, c->vox_quota,
+    void *ptrs[] = {c->slots, c->occ, c->vox_key, c->vox_mom, c->vox_slot, c->vox_prec, c->vox_nin, c->vox_quota,
                     c->vox_dirty, c->dirty_list, c->touched, c->ctr, c->pt_slot, c->pt_vid, c->pt_take,
                     c->ds_keys, c->ds_idx, c->ds_xyz, c->blk_cnt, c->blk_off, c->reg, c->partials, c->best_slot,
                     c->dtmp};
@@ -128,7 +128,7 @@ int lw_create(lw_ctx **out, int64_t reg_slots, int64_t frame_cap, void *stream) 
     if ((e = dalloc(&c->occ, c->occ_words)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->vox_key, c->vox_cap)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->vox_mom, c->vox_cap)) != hipSuccess) return fail(e);
-    if ((e = dalloc(&c->vox_mu, c->vox_cap)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->vox_slot, c->vox_cap)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->vox_prec, c->vox_cap)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->vox_nin, c->vox_cap)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->vox_quota, c->vox_cap)) != hipSuccess) return fail(e);
@@ -145,7 +145,8 @@ int lw_create(lw_ctx **out, int64_t reg_slots, int64_t frame_cap, void *stream) 
     if ((e = dalloc(&c->blk_cnt, c->max_blocks)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->blk_off, c->max_blocks)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->reg, 1)) != hipSuccess) return fail(e);
-    if ((e = dalloc(&c->partials, (size_t)lw::ICP_MAX_GROUPS * 128)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->partials, (size_t)((frame_cap + lw::ICP_BLOCK - 1) / lw::ICP_BLOCK) * 128)) != hipSuccess)
+        return fail(e);
     if ((e = dalloc(&c->best_slot, frame_cap)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->dtmp, frame_cap + 64)) != hipSuccess) return fail(e);
     if ((e = hipHostMalloc(&c->pin, PIN_BYTES, hipHostMallocDefault)) != hipSuccess) return fail(e);

@@ -22,11 +22,15 @@ constexpr int ACC_H = 0, ACC_B = 21, ACC_COUNT = 27, ACC_ERR = 28, ACC_RMS = 29,
               ACC_RANGE = 31, ACC_HG = 32;            // odom.py:73-81
 constexpr int ACC_USED = 53;                          // 21 + 6 + 5 + 21
 
-// One hash-table slot: the reference's int64 key plus the dense voxel id of its record.
-struct __align__(16) Slot {
+// One hash-table slot (32 B): the reference's int64 key, the dense voxel id of its records, and a copy of
+// the voxel's mean and count (float), so the closest-mean search reads one slot per neighbour and nothing
+// else. Mean and count are written by the precision refresh (count 0 until then: the voxel is skipped).
+struct __align__(32) Slot {
     int64_t key;
     int32_t vid;
-    int32_t pad;
+    float cnt;
+    float mu[3];
+    float pad;
 };
 
 // Moments of one voxel, relative to the voxel centre (odom.py:198-200): sum r, count, and the six
@@ -91,21 +95,6 @@ __device__ inline void transform_point(const float *T, float px, float py, float
     a = T[0] * px; a = a + T[1] * py; a = a + T[2] * pz; wx = a + T[3];
     a = T[4] * px; a = a + T[5] * py; a = a + T[6] * pz; wy = a + T[7];
     a = T[8] * px; a = a + T[9] * py; a = a + T[10] * pz; wz = a + T[11];
-}
-
-// Occupancy bitmap probe: equivalent to gpu.py:254-265 because keys[s] == -1 exactly when bit s is clear;
-// the bitmap (cap / 8 bytes) stays in L2, so a missing neighbour costs no access to the key array.
-__device__ inline int32_t find_vid(int64_t key, const Slot *__restrict__ slots, const uint32_t *__restrict__ occ,
-                                   int32_t mask) {
-    int32_t h = slot_hash(key) & mask;
-    for (int probe = 0; probe < PROBES; ++probe) {
-        int32_t s = (h + probe) & mask;
-        uint32_t w = occ[s >> 5];
-        if (!((w >> (s & 31)) & 1u)) return -1;
-        Slot sl = slots[s];
-        if (sl.key == key) return sl.vid;
-    }
-    return -1;
 }
 
 // ---- float32 accumulation terms in the reference's order, structural zeros of J skipped ----------

@@ -335,6 +335,26 @@ struct StepOut {
 
 enum { STEP_OK = 0, STEP_FEW = 1, STEP_SINGULAR = 2 };
 
+// True when A - shift I is positive definite (a Cholesky factorisation succeeds): every eigenvalue of the
+// symmetric A exceeds shift.
+LW_HD inline bool pd_above(const double *A, double shift) {
+    double L[36];
+    bool pd = true;
+    LW_UNROLL for (int j = 0; j < 6; ++j) {
+        double s = A[6 * j + j] - shift;
+        LW_UNROLL for (int k = 0; k < j; ++k) s -= L[6 * j + k] * L[6 * j + k];
+        if (!(s > 0.0)) pd = false;
+        double d = sqrt(s > 0.0 ? s : 1.0);
+        L[6 * j + j] = d;
+        LW_UNROLL for (int i = j + 1; i < 6; ++i) {
+            double t = A[6 * i + j];
+            LW_UNROLL for (int k = 0; k < j; ++k) t -= L[6 * i + k] * L[6 * j + k];
+            L[6 * i + j] = t / d;
+        }
+    }
+    return pd;
+}
+
 // odom.py:436-455
 LW_HD inline bool constrained_step(const double *A, const double *rhs, const double *Hg, const double *T, double L,
                                    double thr, const double *to_pred, double *delta, int32_t *n_degen) {
@@ -342,6 +362,11 @@ LW_HD inline bool constrained_step(const double *A, const double *rhs, const dou
     sensor_basis(T, L, J, Ji);
     mat6_mul(Hg, J, tmp);
     mat6_tmul(J, tmp, JHJ);
+    if (pd_above(JHJ, thr)) {  // every direction constrained: the plain solve (odom.py:447-448), no eigen sweeps
+        *n_degen = 0;
+        const bool all[6] = {true, true, true, true, true, true};
+        return solve6(A, rhs, all, delta);
+    }
     double lam[6], V[36];
     sym_eig<6>(JHJ, lam, V);
     bool good[6];

@@ -48,7 +48,7 @@ struct lw_ctx {
     int64_t occ_words = 0;
     int64_t *vox_key = nullptr;
     lw::VoxMom *vox_mom = nullptr;
-    float4 *vox_mu = nullptr;
+    int32_t *vox_slot = nullptr;  // slot of each voxel id
     lw::VoxPrec *vox_prec = nullptr;
     int32_t *vox_nin = nullptr;
     int32_t *vox_quota = nullptr;

@@ -2,11 +2,12 @@
 // map reset (odom.py:561-567).
 //
 // Layout in HBM (cap slots, V voxels):
-//   slots[cap]    16 B {int64 key, int32 voxel id}: the reference's open-addressed key table, probed
-//                 with its hash (gpu.py:247-265); a voxel id indexes the dense per-voxel arrays below;
-//   occ[cap/32]   occupancy bits (bit s set <=> key[s] != -1), 1/64 of the key table, L2-resident;
-//   vox_key[V], vox_mom[V] 40 B {sum r, count, sum r r^T (6)}: the moments, dense by voxel id;
-//   vox_mu[V] 16 B, vox_prec[V] 64 B: the per-voxel mean and residual precision the query reads.
+//   slots[cap]    32 B {int64 key, int32 voxel id, float count, float mean[3]}: the reference's
+//                 open-addressed key table, probed with its hash (gpu.py:247-265), carrying the voxel's
+//                 mean so the closest-mean search reads nothing else per neighbour;
+//   occ[cap/32]   occupancy bits (bit s set <=> key[s] != -1), 1/256 of the slot table, L2-resident;
+//   vox_key[V], vox_slot[V], vox_mom[V] 40 B {sum r, count, sum r r^T (6)}: the moments, dense by voxel id;
+//   vox_prec[V] 64 B: the residual precision of each voxel, read once per matched point.
 //
 // One insert is five short kernels:
 //   1 claim    probe and CAS-claim each point's slot (a claimer hands out the voxel id);
@@ -39,7 +40,9 @@ __global__ void k_clear_slots(Slot *s, int64_t n) {
         Slot z;
         z.key = -1;
         z.vid = -1;
-        z.pad = 0;
+        z.cnt = 0.0f;
+        z.mu[0] = z.mu[1] = z.mu[2] = 0.0f;
+        z.pad = 0.0f;
         s[i] = z;
     }
 }
@@ -60,8 +63,9 @@ __device__ inline bool insert_point(const float *xyz, const uint32_t *attr, int3
 // 1: claim. A plain read first: a slot's key only ever goes from -1 to its final value.
 __global__ __launch_bounds__(BLOCK) void k_ins_claim(const float *xyz, const uint32_t *attr, int32_t n, Mat4f T,
                                                      float inv_v, float max_range, int32_t ret_mask, int32_t mask,
-                                                     Slot *slots, uint32_t *occ, int64_t *vox_key, VoxMom *vox_mom,
-                                                     int32_t *ctr, int64_t vox_cap, int32_t *pt_slot) {
+                                                     Slot *slots, uint32_t *occ, int64_t *vox_key, int32_t *vox_slot,
+                                                     VoxMom *vox_mom, int32_t *ctr, int64_t vox_cap,
+                                                     int32_t *pt_slot) {
     int32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i == 0) ctr[4] = 0;  // touched count of this insert (read by the next kernel only)
     if (i >= n) return;
@@ -93,6 +97,7 @@ __global__ __launch_bounds__(BLOCK) void k_ins_claim(const float *xyz, const uin
                 }
                 slots[s].vid = vid;
                 vox_key[vid] = key;
+                vox_slot[vid] = s;
                 VoxMom z;
                 for (int a = 0; a < 3; ++a) z.s[a] = 0.0f;
                 for (int a = 0; a < 6; ++a) z.o[a] = 0.0f;
@@ -215,24 +220,28 @@ __global__ __launch_bounds__(BLOCK) void k_ins_finish(const int32_t *touched, in
 // Per-voxel mean and precision for the queued voxels (all == 1: every voxel handed out).
 __global__ __launch_bounds__(BLOCK) void k_refresh(int32_t all, int32_t *ctr, const int32_t *dirty_list,
                                                    int32_t *vox_dirty, const int64_t *vox_key,
-                                                   const VoxMom *vox_mom, float4 *vox_mu, VoxPrec *vox_prec,
-                                                   float voxel, float noise2, float tight2, float alpha,
-                                                   int64_t vox_cap) {
+                                                   const int32_t *vox_slot, const VoxMom *vox_mom, Slot *slots,
+                                                   VoxPrec *vox_prec, float voxel, float noise2, float tight2,
+                                                   float alpha, int64_t vox_cap) {
     int32_t cnt = all ? ctr[2] : ctr[3];
     if (cnt > vox_cap) cnt = (int32_t)vox_cap;
     for (int32_t j = blockIdx.x * BLOCK + threadIdx.x; j < cnt; j += gridDim.x * BLOCK) {
         int32_t vid = all ? j : dirty_list[j];
         vox_dirty[vid] = 0;
         VoxMom m = vox_mom[vid];
+        Slot *sl = &slots[vox_slot[vid]];
         if (m.cnt <= 0) {
-            vox_mu[vid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            sl->cnt = 0.0f;
             continue;
         }
         int32_t ix, iy, iz;
         voxel_key_unpack(vox_key[vid], ix, iy, iz);
         float mu[3];
         voxel_mean(ix, iy, iz, m.s, m.cnt, voxel, mu);
-        vox_mu[vid] = make_float4(mu[0], mu[1], mu[2], (float)m.cnt);
+        sl->mu[0] = mu[0];
+        sl->mu[1] = mu[1];
+        sl->mu[2] = mu[2];
+        sl->cnt = (float)m.cnt;
         VoxPrec P;
         P.shape = (float)voxel_precision(m.s, m.o, m.cnt, voxel, noise2, tight2, alpha, P.M, P.Mg, P.q);
         vox_prec[vid] = P;
@@ -293,7 +302,7 @@ hipError_t launch_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, 
     if (n <= 0) return hipSuccess;
     int g = grid_for(n);
     hipLaunchKernelGGL(k_ins_claim, dim3(g), dim3(BLOCK), 0, c->stream, xyz, attr, n, T, inv_v, max_range, ret_mask,
-                       c->mask, c->slots, c->occ, c->vox_key, c->vox_mom, c->ctr, c->vox_cap, c->pt_slot);
+                       c->mask, c->slots, c->occ, c->vox_key, c->vox_slot, c->vox_mom, c->ctr, c->vox_cap, c->pt_slot);
     hipLaunchKernelGGL(k_ins_count, dim3(g), dim3(BLOCK), 0, c->stream, c->pt_slot, n, c->slots, c->pt_vid,
                        c->vox_nin, c->touched, c->ctr);
     hipLaunchKernelGGL(k_ins_quota, dim3(1024), dim3(BLOCK), 0, c->stream, c->touched, c->ctr, c->vox_nin,
@@ -307,7 +316,7 @@ hipError_t launch_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, 
 
 hipError_t launch_refresh(lw_ctx *c, bool all, float voxel, float noise2, float tight2, float alpha) {
     hipLaunchKernelGGL(k_refresh, dim3(all ? 4096 : 512), dim3(BLOCK), 0, c->stream, all ? 1 : 0, c->ctr,
-                       c->dirty_list, c->vox_dirty, c->vox_key, c->vox_mom, c->vox_mu, c->vox_prec, voxel, noise2,
+                       c->dirty_list, c->vox_dirty, c->vox_key, c->vox_slot, c->vox_mom, c->slots, c->vox_prec, voxel, noise2,
                        tight2, alpha, c->vox_cap);
     hipLaunchKernelGGL(k_set_ctr, dim3(1), dim3(1), 0, c->stream, c->ctr, 3, 0);
     return hipGetLastError();
