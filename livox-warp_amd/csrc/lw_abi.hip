@@ -145,8 +145,7 @@ int lw_create(lw_ctx **out, int64_t reg_slots, int64_t frame_cap, void *stream) 
     if ((e = dalloc(&c->blk_cnt, c->max_blocks)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->blk_off, c->max_blocks)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->reg, 1)) != hipSuccess) return fail(e);
-    if ((e = dalloc(&c->partials, (size_t)((frame_cap + lw::ICP_BLOCK - 1) / lw::ICP_BLOCK) * 128)) != hipSuccess)
-        return fail(e);
+    if ((e = dalloc(&c->partials, (size_t)lw::icp_max_groups(frame_cap) * 128)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->best_slot, frame_cap)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->dtmp, frame_cap + 64)) != hipSuccess) return fail(e);
     if ((e = hipHostMalloc(&c->pin, PIN_BYTES, hipHostMallocDefault)) != hipSuccess) return fail(e);
@@ -378,12 +377,13 @@ int lw_register_thinned(lw_ctx *c, int32_t n, const lw_reg_params *p, const doub
         q.kappa = (float)(kap > 2.0 ? kap : 2.0);
         LW_CK(lw::launch_icp_pass(c, q, p->acc_mode));
     }
-    LW_CK(lw::launch_reg_finish(c, p->min_corr));
     LW_CK(hipMemcpyAsync(pin_down(c), c->reg, sizeof(lw::RegState), hipMemcpyDeviceToHost, c->stream));
     LW_CK(hipStreamSynchronize(c->stream));
     LW_CK(prof_harvest(c));
     lw::RegState st;
     std::memcpy(&st, pin_down(c), sizeof(st));
+    // odom.py:797-800, on the host copy of the last accepted H (a 6x6 eigen problem: microseconds here)
+    st.cond = st.corr >= p->min_corr ? lw::gn::cond_of(st.H) : 1.0;
     std::memcpy(res->T, st.T, 16 * sizeof(double));
     res->rms = st.rms;
     res->cond = st.cond;

@@ -1,16 +1,19 @@
 // Scan-to-map accumulation and the Gauss-Newton loop on the device: k_icp + k_reduce
 // (python/livox_warp/odom.py:275-384) and the host step of Odometry._register (odom.py:752-803).
 //
-// One pass = one launch of ceil(n / 64) one-wave workgroups, one thinned point per lane. A lane poses its
-// point with the float32 pose and searches the (2 reach + 1)^3 voxels around it for the closest voxel mean,
-// in the reference's order (dx outer, dz inner, strict <; odom.py:315-332). The search runs in chunks: the
+// One pass = one launch of one-wave workgroups. Each thinned point gets S lanes (S = 2 for the 27-voxel
+// search, 4 for the 125-voxel one); the lanes split the point's neighbours, in the reference's order
+// (dx outer, dz inner; odom.py:315-332), into contiguous parts. A lane searches its part in chunks: the
 // bitmap words of a whole chunk are loaded before any is tested, then the chunk's slots (32 B records that
-// carry key, voxel id, count and mean), so a chunk costs two dependent memory round trips. The lane then
-// forms the 53 float32 terms of the ACC layout exactly as the reference rounds them (odom.py:246-259,
-// 269-271, 372-375); the wave sums them with cross-lane shuffles and writes one partial row. The last
-// workgroup to finish (atomic ticket, no waiting) sums the rows in a fixed order into the reduced
-// accumulator. In a registration a one-wave kernel then runs the float64 step (lw_gn.h) and updates the
-// pose in device memory; passes after convergence return at once, so a registration is queued in full
+// carry key, voxel id, count and mean), so a chunk costs two dependent memory round trips and a lane one
+// or two chunks. The lanes then merge their closest means by (distance, position in the order), which is
+// exactly the serial loop's strict-< choice. The first lane of the point forms the 53 float32 terms of
+// the ACC layout as the reference rounds them (odom.py:246-259, 269-271, 372-375).
+//
+// Reduction: the wave's terms are transposed through LDS and summed per entry in lane order; each
+// workgroup writes one partial row; the last workgroup to finish (atomic ticket, no waiting) sums the rows
+// in workgroup order and, in a registration, runs the float64 step (lw_gn.h) in one lane and updates the
+// pose in device memory. Passes after convergence return at once, so a registration is queued in full
 // with one readback at the end.
 //
 // Accumulation forms (acc_mode): 0 float64 (the reference), 1 compensated float32 (Neumaier sums, two-sum
@@ -21,51 +24,26 @@ namespace lw {
 
 namespace {
 
-template <int MODE>
-struct Acc;
-
-template <>
-struct Acc<0> {
-    double s[ACC_USED];
-    __device__ void zero() {
-        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) s[k] = 0.0;
-    }
-    __device__ void add(int k, float x) { s[k] += (double)x; }
-    __device__ void wave_reduce() {
-        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) {
-            double v = s[k];
-            LW_UNROLL for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-            s[k] = v;
-        }
-    }
-    __device__ void store(double *row, int k) const { row[k] = s[k]; row[64 + k] = 0.0; }
-};
-
-template <>
-struct Acc<2> {
-    float s[ACC_USED];
-    __device__ void zero() {
-        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) s[k] = 0.0f;
-    }
-    __device__ void add(int k, float x) { s[k] += x; }
-    __device__ void wave_reduce() {
-        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) {
-            float v = s[k];
-            LW_UNROLL for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-            s[k] = v;
-        }
-    }
-    __device__ void store(double *row, int k) const { row[k] = (double)s[k]; row[64 + k] = 0.0; }
-};
-
 __device__ inline void two_sum(float a, float b, float &s, float &e) {
     s = a + b;
     float bb = s - a;
     e = (a - (s - bb)) + (b - bb);
 }
 
+template <int MODE>
+struct Acc;
+
 template <>
-struct Acc<1> {
+struct Acc<0> {  // float64
+    double s[ACC_USED];
+    __device__ void zero() {
+        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) s[k] = 0.0;
+    }
+    __device__ void add(int k, float x) { s[k] += (double)x; }
+};
+
+template <>
+struct Acc<1> {  // compensated float32
     float s[ACC_USED], c[ACC_USED];
     __device__ void zero() {
         LW_UNROLL for (int k = 0; k < ACC_USED; ++k) s[k] = c[k] = 0.0f;
@@ -76,22 +54,66 @@ struct Acc<1> {
         s[k] = t;
         c[k] += e;
     }
-    __device__ void wave_reduce() {
+};
+
+template <>
+struct Acc<2> {  // naive float32
+    float s[ACC_USED];
+    __device__ void zero() {
+        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) s[k] = 0.0f;
+    }
+    __device__ void add(int k, float x) { s[k] += x; }
+};
+
+constexpr int RED_STRIDE = ACC_USED + 1;  // LDS row stride (odd, so column reads do not share banks)
+constexpr int RED_BYTES = ICP_BLOCK * RED_STRIDE * 8;
+
+// Sum the wave's per-lane accumulators per entry, in lane order, through LDS. Lane k < 53 returns entry k
+// (sum, and compensation for the compensated form).
+template <int MODE, int S>
+__device__ inline void wave_sum(const Acc<MODE> &acc, unsigned char *lds, double &sum, double &comp) {
+    const int lane = threadIdx.x;
+    sum = 0.0;
+    comp = 0.0;
+    if constexpr (MODE == 0) {
+        double *red = reinterpret_cast<double *>(lds);
+        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) red[lane * RED_STRIDE + k] = acc.s[k];
+        __syncthreads();
+        if (lane < ACC_USED) {
+            double t = 0.0;
+            for (int r = 0; r < ICP_BLOCK; r += S) t += red[r * RED_STRIDE + lane];
+            sum = t;
+        }
+    } else if constexpr (MODE == 2) {
+        float *red = reinterpret_cast<float *>(lds);
+        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) red[lane * RED_STRIDE + k] = acc.s[k];
+        __syncthreads();
+        if (lane < ACC_USED) {
+            float t = 0.0f;
+            for (int r = 0; r < ICP_BLOCK; r += S) t += red[r * RED_STRIDE + lane];
+            sum = (double)t;
+        }
+    } else {
+        float *rs = reinterpret_cast<float *>(lds);
+        float *rc = rs + ICP_BLOCK * RED_STRIDE;
         LW_UNROLL for (int k = 0; k < ACC_USED; ++k) {
-            float v = s[k], w = c[k];
-            LW_UNROLL for (int off = 32; off > 0; off >>= 1) {
-                float ov = __shfl_xor(v, off, 64), ow = __shfl_xor(w, off, 64);
+            rs[lane * RED_STRIDE + k] = acc.s[k];
+            rc[lane * RED_STRIDE + k] = acc.c[k];
+        }
+        __syncthreads();
+        if (lane < ACC_USED) {
+            float s = 0.0f, c = 0.0f;
+            for (int r = 0; r < ICP_BLOCK; r += S) {
                 float t, e;
-                two_sum(v, ov, t, e);
-                v = t;
-                w = w + ow + e;
+                two_sum(s, rs[r * RED_STRIDE + lane], t, e);
+                s = t;
+                c = c + rc[r * RED_STRIDE + lane] + e;
             }
-            s[k] = v;
-            c[k] = w;
+            sum = (double)s;
+            comp = (double)c;
         }
     }
-    __device__ void store(double *row, int k) const { row[k] = (double)s[k]; row[64 + k] = (double)c[k]; }
-};
+}
 
 struct IcpArgs {
     const float *ds;
@@ -105,6 +127,9 @@ struct IcpArgs {
     int32_t n;
     int32_t mask;
     float inv_v, voxel, max_d2, kappa;
+    int32_t it;
+    int32_t gn;
+    gn::StepParams sp;
 };
 
 #define LW_H_PAIRS(X) \
@@ -135,15 +160,33 @@ __device__ __noinline__ int32_t probe_on(const Slot *slots, const uint32_t *occ,
     return -1;
 }
 
-// Closest voxel mean within reach R (odom.py:308-334): chunks of C neighbours in the reference's order.
+// Lanes per point and chunk size per reach: 27 neighbours as 14 + 13 on two lanes; 125 as 32+32+32+29 on
+// four lanes, each in two chunks of 16.
+template <int R>
+struct Split;
+template <>
+struct Split<1> {
+    static constexpr int S = 2, C = 14;
+};
+template <>
+struct Split<2> {
+    static constexpr int S = 4, C = 16;
+};
+
+struct Cand {
+    float d2;
+    int32_t s;  // slot, -1 for none
+    int32_t j;  // position in the reference's neighbour order
+    float mx, my, mz;
+};
+
+// The closest mean over neighbours [j_lo, j_hi) of the reference's order (odom.py:308-334).
 template <int R, int C>
-__device__ inline int32_t closest_mean(const IcpArgs &a, float wx, float wy, float wz, float &mx, float &my,
-                                       float &mz, int32_t &hits) {
-    constexpr int D = 2 * R + 1, K = D * D * D;
+__device__ inline void search_part(const IcpArgs &a, float wx, float wy, float wz, int j_lo, int j_hi, Cand &best,
+                                   int32_t &hits) {
+    constexpr int D = 2 * R + 1;
     const int32_t bx = ifloor(wx * a.inv_v), by = ifloor(wy * a.inv_v), bz = ifloor(wz * a.inv_v);
-    float best_d2 = a.max_d2;
-    int32_t best_s = -1;
-    for (int j0 = 0; j0 < K; j0 += C) {
+    for (int j0 = j_lo; j0 < j_hi; j0 += C) {
         int64_t key[C];
         int32_t h[C];
         uint32_t w[C];
@@ -153,7 +196,7 @@ __device__ inline int32_t closest_mean(const IcpArgs &a, float wx, float wy, flo
             key[c] = voxel_key_ijk(bx + dx, by + dy, bz + dz);
             h[c] = slot_hash(key[c]) & a.mask;
         }
-        LW_UNROLL for (int c = 0; c < C; ++c) w[c] = (j0 + c < K) ? a.occ[h[c] >> 5] : 0u;
+        LW_UNROLL for (int c = 0; c < C; ++c) w[c] = (j0 + c < j_hi) ? a.occ[h[c] >> 5] : 0u;
         int4 lo[C];
         float4 hi[C];
         LW_UNROLL for (int c = 0; c < C; ++c) {
@@ -175,31 +218,24 @@ __device__ inline int32_t closest_mean(const IcpArgs &a, float wx, float wy, flo
             float dd = d0 * d0;
             dd = dd + d1 * d1;
             dd = dd + d2 * d2;
-            if (dd < best_d2) {
-                best_d2 = dd;
-                best_s = s;
-                mx = hi[c].x;
-                my = hi[c].y;
-                mz = hi[c].z;
+            if (dd < best.d2) {
+                best.d2 = dd;
+                best.s = s;
+                best.j = j0 + c;
+                best.mx = hi[c].x;
+                best.my = hi[c].y;
+                best.mz = hi[c].z;
             }
         }
     }
-    return best_s;
 }
 
-template <int MODE, int R, int C>
-__device__ inline void point_terms(const IcpArgs &a, const float *Tf, int32_t i, Acc<MODE> &acc, int32_t &hits,
-                                   int32_t &matched) {
-    const float px = a.ds[3 * (int64_t)i], py = a.ds[3 * (int64_t)i + 1], pz = a.ds[3 * (int64_t)i + 2];
-    float wx, wy, wz;
-    transform_point(Tf, px, py, pz, wx, wy, wz);
-    float mx = 0.0f, my = 0.0f, mz = 0.0f;
-    const int32_t s = closest_mean<R, C>(a, wx, wy, wz, mx, my, mz, hits);
-    if (a.best_slot) a.best_slot[i] = s;
-    if (s < 0) return;
-    ++matched;
-    const int32_t vid = a.slots[s].vid;
-    const float r0 = wx - mx, r1 = wy - my, r2 = wz - mz;
+// The reference's terms of one matched point (odom.py:335-375) into acc.
+template <int MODE>
+__device__ inline void point_terms(const IcpArgs &a, float px, float py, float pz, float wx, float wy, float wz,
+                                   const Cand &b, Acc<MODE> &acc) {
+    const int32_t vid = a.slots[b.s].vid;
+    const float r0 = wx - b.mx, r1 = wy - b.my, r2 = wz - b.mz;
     const VoxPrec P = a.vox_prec[vid];
     const float M[3][3] = {{P.M[0], P.M[1], P.M[2]}, {P.M[1], P.M[3], P.M[4]}, {P.M[2], P.M[4], P.M[5]}};
     const float G[3][3] = {{P.Mg[0], P.Mg[1], P.Mg[2]}, {P.Mg[1], P.Mg[3], P.Mg[4]}, {P.Mg[2], P.Mg[4], P.Mg[5]}};
@@ -241,97 +277,10 @@ __device__ inline void point_terms(const IcpArgs &a, const float *Tf, int32_t i,
     acc.add(ACC_RANGE, length3(px, py, pz));
 }
 
-// Chunk sizes: 27 neighbours in two chunks, 125 in eight.
-template <int R>
-struct Chunk;
-template <>
-struct Chunk<1> {
-    static constexpr int C = 14;
-};
-template <>
-struct Chunk<2> {
-    static constexpr int C = 16;
-};
-
-template <int MODE, int R>
-__global__ __launch_bounds__(ICP_BLOCK) void k_icp(IcpArgs a) {
-    RegState *st = a.st;
-    if (st->done) return;
-    __shared__ float Tf[16];
-    __shared__ int32_t s_last;
-    if (threadIdx.x < 16) Tf[threadIdx.x] = st->Tf[threadIdx.x];
-    __syncthreads();
-    const int32_t i = blockIdx.x * ICP_BLOCK + threadIdx.x;
-    int32_t hits = 0, matched = 0;
-    double *row = a.partials + (size_t)blockIdx.x * 128;
-    {
-        Acc<MODE> acc;
-        acc.zero();
-        if (i < a.n) point_terms<MODE, R, Chunk<R>::C>(a, Tf, i, acc, hits, matched);
-        acc.wave_reduce();
-        // lane k stores entry k (lanes 53..63 idle); the reduced values are uniform across the wave
-        LW_UNROLL for (int k = 0; k < ACC_USED; ++k)
-            if (threadIdx.x == k) acc.store(row, k);
-    }
-    if (a.prof) {
-        LW_UNROLL for (int off = 32; off > 0; off >>= 1) {
-            hits += __shfl_xor(hits, off, 64);
-            matched += __shfl_xor(matched, off, 64);
-        }
-        if (threadIdx.x == 0) {
-            atomicAdd(&a.prof[2], (unsigned long long)hits);
-            atomicAdd(&a.prof[3], (unsigned long long)matched);
-        }
-    }
-    __threadfence();
-    if (threadIdx.x == 0) s_last = (atomicAdd(&st->ticket, 1u) == gridDim.x - 1);
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    const int k = threadIdx.x;
-    double v = 0.0;
-    if (k < ACC_USED) {
-        if (MODE == 0) {
-            double sum = 0.0;
-            for (int b = 0; b < (int)gridDim.x; ++b)
-                sum += __hip_atomic_load(&a.partials[(size_t)b * 128 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            v = sum;
-        } else if (MODE == 2) {
-            float sum = 0.0f;
-            for (int b = 0; b < (int)gridDim.x; ++b)
-                sum += (float)__hip_atomic_load(&a.partials[(size_t)b * 128 + k], __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT);
-            v = (double)sum;
-        } else {
-            float s = 0.0f, c = 0.0f;
-            for (int b = 0; b < (int)gridDim.x; ++b) {
-                float t, e;
-                two_sum(s, (float)__hip_atomic_load(&a.partials[(size_t)b * 128 + k], __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT), t, e);
-                s = t;
-                c = c + (float)__hip_atomic_load(&a.partials[(size_t)b * 128 + 64 + k], __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT) + e;
-            }
-            v = (double)s + (double)c;
-        }
-    }
-    st->acc[k] = v;
-    if (k == 0) {
-        st->ticket = 0;
-        if (a.prof) {
-            constexpr unsigned long long K = (2 * R + 1) * (2 * R + 1) * (2 * R + 1);
-            atomicAdd(&a.prof[0], (unsigned long long)a.n);
-            atomicAdd(&a.prof[1], (unsigned long long)a.n * K);
-            atomicAdd(&a.prof[4], 1ull);
-        }
-    }
-}
-
 // The float64 step after a pass (odom.py:769-796), in one lane.
-__global__ __launch_bounds__(64) void k_gn(RegState *st, gn::StepParams sp, int32_t it) {
-    if (threadIdx.x != 0 || st->done) return;
+__device__ void gauss_newton(RegState *st, const double *v, const gn::StepParams &sp, int32_t it) {
     gn::StepOut o;
-    int rc = gn::step(st->acc, st->T, st->T_pred, sp, o);
+    int rc = gn::step(v, st->T, st->T_pred, sp, o);
     st->corr = o.corr;
     if (rc == gn::STEP_FEW) {  // corr < min_corr: stop (odom.py:770-771)
         st->done = 1;
@@ -357,6 +306,124 @@ __global__ __launch_bounds__(64) void k_gn(RegState *st, gn::StepParams sp, int3
     }
 }
 
+template <int MODE, int R>
+__global__ __launch_bounds__(ICP_BLOCK) void k_icp(IcpArgs a) {
+    constexpr int S = Split<R>::S, C = Split<R>::C;
+    constexpr int D = 2 * R + 1, K = D * D * D, KQ = (K + S - 1) / S;
+    RegState *st = a.st;
+    if (st->done) return;
+    __shared__ float Tf[16];
+    __shared__ int32_t s_last;
+    __shared__ __align__(16) unsigned char lds[RED_BYTES];
+    const int lane = threadIdx.x;
+    if (lane < 16) Tf[lane] = st->Tf[lane];
+    __syncthreads();
+    const int sub = lane % S;
+    const int32_t i = (blockIdx.x * ICP_BLOCK + lane) / S;
+    int32_t hits = 0, matched = 0;
+    double sum, comp;
+    {
+        float px = 0.0f, py = 0.0f, pz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
+        Cand best{a.max_d2, -1, K, 0.0f, 0.0f, 0.0f};
+        if (i < a.n) {
+            px = a.ds[3 * (int64_t)i];
+            py = a.ds[3 * (int64_t)i + 1];
+            pz = a.ds[3 * (int64_t)i + 2];
+            transform_point(Tf, px, py, pz, wx, wy, wz);
+            search_part<R, C>(a, wx, wy, wz, sub * KQ, min(K, (sub + 1) * KQ), best, hits);
+        }
+        // merge the S parts: smaller distance wins, a tie goes to the earlier neighbour (the serial strict <)
+        LW_UNROLL for (int off = 1; off < S; off <<= 1) {
+            Cand o;
+            o.d2 = __shfl_xor(best.d2, off, 64);
+            o.s = __shfl_xor(best.s, off, 64);
+            o.j = __shfl_xor(best.j, off, 64);
+            o.mx = __shfl_xor(best.mx, off, 64);
+            o.my = __shfl_xor(best.my, off, 64);
+            o.mz = __shfl_xor(best.mz, off, 64);
+            if (o.s >= 0 && (best.s < 0 || o.d2 < best.d2 || (o.d2 == best.d2 && o.j < best.j))) best = o;
+        }
+        Acc<MODE> acc;
+        acc.zero();
+        if (i < a.n && sub == 0) {
+            if (a.best_slot) a.best_slot[i] = best.s;
+            if (best.s >= 0) {
+                ++matched;
+                point_terms<MODE>(a, px, py, pz, wx, wy, wz, best, acc);
+            }
+        }
+        wave_sum<MODE, S>(acc, lds, sum, comp);
+    }
+    double *row = a.partials + (size_t)blockIdx.x * 128;
+    if (lane < ACC_USED) {
+        row[lane] = sum;
+        row[64 + lane] = comp;
+    }
+    if (a.prof) {
+        LW_UNROLL for (int off = 32; off > 0; off >>= 1) {
+            hits += __shfl_xor(hits, off, 64);
+            matched += __shfl_xor(matched, off, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(&a.prof[2], (unsigned long long)hits);
+            atomicAdd(&a.prof[3], (unsigned long long)matched);
+        }
+    }
+    __threadfence();
+    if (lane == 0) s_last = (atomicAdd(&st->ticket, 1u) == gridDim.x - 1);
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();  // acquire: the other workgroups' rows are visible from here on
+    const int G = (int)gridDim.x;
+    double v = 0.0;
+    if (lane < ACC_USED) {
+        const double *col = a.partials + lane;
+        if constexpr (MODE == 0) {
+            double t = 0.0;
+            int b = 0;
+            for (; b + 8 <= G; b += 8) {
+                double x[8];
+                LW_UNROLL for (int u = 0; u < 8; ++u) x[u] = col[(size_t)(b + u) * 128];
+                LW_UNROLL for (int u = 0; u < 8; ++u) t += x[u];
+            }
+            for (; b < G; ++b) t += col[(size_t)b * 128];
+            v = t;
+        } else if constexpr (MODE == 2) {
+            float t = 0.0f;
+            int b = 0;
+            for (; b + 8 <= G; b += 8) {
+                double x[8];
+                LW_UNROLL for (int u = 0; u < 8; ++u) x[u] = col[(size_t)(b + u) * 128];
+                LW_UNROLL for (int u = 0; u < 8; ++u) t += (float)x[u];
+            }
+            for (; b < G; ++b) t += (float)col[(size_t)b * 128];
+            v = (double)t;
+        } else {
+            float s = 0.0f, c = 0.0f;
+            for (int b = 0; b < G; ++b) {
+                float t, e;
+                two_sum(s, (float)col[(size_t)b * 128], t, e);
+                s = t;
+                c = c + (float)col[(size_t)b * 128 + 64] + e;
+            }
+            v = (double)s + (double)c;
+        }
+    }
+    __shared__ double vs[64];
+    vs[lane] = v;
+    st->acc[lane] = v;
+    __syncthreads();
+    if (lane == 0) {
+        st->ticket = 0;
+        if (a.prof) {
+            atomicAdd(&a.prof[0], (unsigned long long)a.n);
+            atomicAdd(&a.prof[1], (unsigned long long)a.n * (unsigned long long)K);
+            atomicAdd(&a.prof[4], 1ull);
+        }
+        if (a.gn) gauss_newton(st, vs, a.sp, a.it);
+    }
+}
+
 __global__ void k_reg_begin(RegState *st, const double *Tp) {
     if (threadIdx.x != 0) return;
     for (int k = 0; k < 16; ++k) {
@@ -375,19 +442,17 @@ __global__ void k_reg_begin(RegState *st, const double *Tp) {
     st->ticket = 0;
 }
 
-__global__ void k_reg_finish(RegState *st, int32_t min_corr) {
-    if (threadIdx.x != 0) return;
-    st->cond = st->corr >= min_corr ? gn::cond_of(st->H) : 1.0;  // odom.py:797-800
-}
-
 template <int MODE>
-hipError_t launch_mode(lw_ctx *c, const IcpArgs &a, int reach, int g) {
-    if (reach == 1)
-        hipLaunchKernelGGL((k_icp<MODE, 1>), dim3(g), dim3(ICP_BLOCK), 0, c->stream, a);
-    else if (reach == 2)
-        hipLaunchKernelGGL((k_icp<MODE, 2>), dim3(g), dim3(ICP_BLOCK), 0, c->stream, a);
-    else
+hipError_t launch_mode(lw_ctx *c, const IcpArgs &a, int reach) {
+    if (reach == 1) {
+        const int g = (int)(((int64_t)a.n * Split<1>::S + ICP_BLOCK - 1) / ICP_BLOCK);
+        hipLaunchKernelGGL((k_icp<MODE, 1>), dim3(g > 0 ? g : 1), dim3(ICP_BLOCK), 0, c->stream, a);
+    } else if (reach == 2) {
+        const int g = (int)(((int64_t)a.n * Split<2>::S + ICP_BLOCK - 1) / ICP_BLOCK);
+        hipLaunchKernelGGL((k_icp<MODE, 2>), dim3(g > 0 ? g : 1), dim3(ICP_BLOCK), 0, c->stream, a);
+    } else {
         return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -398,10 +463,7 @@ hipError_t launch_reg_begin(lw_ctx *c, const double *T_pred_dev) {
     return hipGetLastError();
 }
 
-hipError_t launch_reg_finish(lw_ctx *c, int32_t min_corr) {
-    hipLaunchKernelGGL(k_reg_finish, dim3(1), dim3(64), 0, c->stream, c->reg, min_corr);
-    return hipGetLastError();
-}
+int32_t icp_max_groups(int64_t frame_cap) { return (int32_t)((frame_cap * 4 + ICP_BLOCK - 1) / ICP_BLOCK); }
 
 hipError_t launch_icp_pass(lw_ctx *c, const IcpPass &p, int acc_mode) {
     if (p.n > c->frame_cap) return hipErrorInvalidValue;
@@ -420,8 +482,9 @@ hipError_t launch_icp_pass(lw_ctx *c, const IcpPass &p, int acc_mode) {
     a.voxel = p.voxel;
     a.max_d2 = p.max_d2;
     a.kappa = p.kappa;
-    int g = (p.n + ICP_BLOCK - 1) / ICP_BLOCK;
-    if (g < 1) g = 1;
+    a.it = p.it;
+    a.gn = p.gn;
+    a.sp = p.sp;
     const bool timed = c->prof_on && c->prof_used < c->prof_cap;
     if (timed) {
         hipError_t e = hipEventRecord(c->prof_ev[2 * c->prof_used], c->stream);
@@ -429,19 +492,15 @@ hipError_t launch_icp_pass(lw_ctx *c, const IcpPass &p, int acc_mode) {
     }
     hipError_t e;
     switch (acc_mode) {
-    case 0: e = launch_mode<0>(c, a, p.reach, g); break;
-    case 1: e = launch_mode<1>(c, a, p.reach, g); break;
-    case 2: e = launch_mode<2>(c, a, p.reach, g); break;
+    case 0: e = launch_mode<0>(c, a, p.reach); break;
+    case 1: e = launch_mode<1>(c, a, p.reach); break;
+    case 2: e = launch_mode<2>(c, a, p.reach); break;
     default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
     if (timed) {
         e = hipEventRecord(c->prof_ev[2 * c->prof_used + 1], c->stream);
         c->prof_used++;
-    }
-    if (e == hipSuccess && p.gn) {
-        hipLaunchKernelGGL(k_gn, dim3(1), dim3(64), 0, c->stream, c->reg, p.sp, p.it);
-        e = hipGetLastError();
     }
     return e;
 }

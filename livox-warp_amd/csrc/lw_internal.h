@@ -124,7 +124,7 @@ struct IcpPass {
     int32_t want_slots;
 };
 hipError_t launch_icp_pass(lw_ctx *c, const IcpPass &p, int acc_mode);
-hipError_t launch_reg_begin(lw_ctx *c, const double *T_pred_host_pinned);
-hipError_t launch_reg_finish(lw_ctx *c, int32_t min_corr);
+int32_t icp_max_groups(int64_t frame_cap);  // partial rows one pass over frame_cap points can write
+hipError_t launch_reg_begin(lw_ctx *c, const double *T_pred_dev);
 
 }  // namespace lw
