@@ -31,6 +31,22 @@
 #include <stdint.h>
 #include <string.h>
 
+/* Device-side arithmetic type: float32 (the reference's kernels), or float64 with -DLWO_FP64 (the
+ * all-float64 oracle of the precision study, built as liblw_oracle64.so from this same file). */
+#ifdef LWO_FP64
+typedef double real;
+#define LWO_FLOOR floor
+#define LWO_SQRT sqrt
+#define LWO_SIN sin
+#define LWO_COS cos
+#else
+typedef float real;
+#define LWO_FLOOR floorf
+#define LWO_SQRT sqrtf
+#define LWO_SIN sinf
+#define LWO_COS cosf
+#endif
+
 #define LWO_KEY_BITS 21
 #define LWO_KEY_BIAS (1ULL << 20)
 #define LWO_KEY_MASK ((1ULL << 21) - 1ULL)
@@ -72,9 +88,9 @@ int32_t lwo_find_slot(int64_t key, const int64_t *keys, int32_t mask)
     return -1;
 }
 
-static inline int32_t ifloor(float x) { return (int32_t)floorf(x); }
+static inline int32_t ifloor(real x) { return (int32_t)LWO_FLOOR(x); }
 
-int64_t lwo_voxel_key(const float *w, float inv)
+int64_t lwo_voxel_key(const real *w, real inv)
 {
     return lwo_voxel_key_ijk(ifloor(w[0] * inv), ifloor(w[1] * inv), ifloor(w[2] * inv));
 }
@@ -87,12 +103,12 @@ static inline int ret_ok(uint32_t a, int ret_mask)
     return r < 31 && ((1 << r) & ret_mask) != 0;
 }
 
-static inline float length3(const float *p) { return sqrtf(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]); }
+static inline real length3(const real *p) { return LWO_SQRT(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]); }
 
-void lwo_transform_point(const float *T, const float *p, float *w)
+void lwo_transform_point(const real *T, const real *p, real *w)
 {
     for (int r = 0; r < 3; ++r) {
-        float a = T[r * 4 + 0] * p[0];
+        real a = T[r * 4 + 0] * p[0];
         a = a + T[r * 4 + 1] * p[1];
         a = a + T[r * 4 + 2] * p[2];
         w[r] = a + T[r * 4 + 3];
@@ -101,17 +117,17 @@ void lwo_transform_point(const float *T, const float *p, float *w)
 
 /* ---- frame cutting (odom.py:97-123) ---------------------------------------------------------- */
 
-void lwo_frame_append(const float *sx, const uint32_t *sa, const float *st, int n, float inv_dt, int k_min,
-                      int k_open, int cap, int32_t *count, float *f_xyz, uint32_t *f_attr, float *f_t,
+void lwo_frame_append(const real *sx, const uint32_t *sa, const real *st, int n, real inv_dt, int k_min,
+                      int k_open, int cap, int32_t *count, real *f_xyz, uint32_t *f_attr, real *f_t,
                       int32_t *f_frame)
 {
     for (int i = 0; i < n; ++i) {
-        float tt = st[i] > 0.0f ? st[i] : 0.0f;
+        real tt = st[i] > ((real)0.0) ? st[i] : ((real)0.0);
         int k = ifloor(tt * inv_dt);
         if (k > k_open || k < k_min) continue;
         int j = (*count)++;
         if (j >= cap) continue;
-        memcpy(f_xyz + 3 * (size_t)j, sx + 3 * (size_t)i, 12);
+        memcpy(f_xyz + 3 * (size_t)j, sx + 3 * (size_t)i, 3 * sizeof(real));
         f_attr[j] = sa[i];
         f_t[j] = st[i];
         f_frame[j] = k_open;
@@ -120,36 +136,36 @@ void lwo_frame_append(const float *sx, const uint32_t *sa, const float *st, int 
 
 /* ---- deskew (odom.py:163-178, gpu.py:268-284) ------------------------------------------------ */
 
-static void rotvec_rotate(const float *r, const float *p, float *out)
+static void rotvec_rotate(const real *r, const real *p, real *out)
 {
-    float a = length3(r);
-    if (a < 1.0e-9f) {
+    real a = length3(r);
+    if (a < ((real)1.0e-9)) {
         out[0] = p[0]; out[1] = p[1]; out[2] = p[2];
         return;
     }
-    float ax[3] = {r[0] / a, r[1] / a, r[2] / a};
-    float sh = sinf(a * 0.5f), ch = cosf(a * 0.5f);
-    float q[3] = {ax[0] * sh, ax[1] * sh, ax[2] * sh};
-    float w = ch;
-    float c = 2.0f * w * w - 1.0f;
-    float d = 2.0f * (q[0] * p[0] + q[1] * p[1] + q[2] * p[2]);
-    float cx = q[1] * p[2] - q[2] * p[1];
-    float cy = q[2] * p[0] - q[0] * p[2];
-    float cz = q[0] * p[1] - q[1] * p[0];
-    float w2 = 2.0f * w;
+    real ax[3] = {r[0] / a, r[1] / a, r[2] / a};
+    real sh = LWO_SIN(a * ((real)0.5)), ch = LWO_COS(a * ((real)0.5));
+    real q[3] = {ax[0] * sh, ax[1] * sh, ax[2] * sh};
+    real w = ch;
+    real c = ((real)2.0) * w * w - ((real)1.0);
+    real d = ((real)2.0) * (q[0] * p[0] + q[1] * p[1] + q[2] * p[2]);
+    real cx = q[1] * p[2] - q[2] * p[1];
+    real cy = q[2] * p[0] - q[0] * p[2];
+    real cz = q[0] * p[1] - q[1] * p[0];
+    real w2 = ((real)2.0) * w;
     out[0] = p[0] * c + q[0] * d + cx * w2;
     out[1] = p[1] * c + q[1] * d + cy * w2;
     out[2] = p[2] * c + q[2] * d + cz * w2;
 }
 
-void lwo_deskew(const float *raw, const float *t, int m, float t_mid, float inv_dt, const float *rho,
-                const float *theta, float *out)
+void lwo_deskew(const real *raw, const real *t, int m, real t_mid, real inv_dt, const real *rho,
+                const real *theta, real *out)
 {
     for (int i = 0; i < m; ++i) {
-        float s = (t[i] - t_mid) * inv_dt;
-        s = s < -1.0f ? -1.0f : (s > 1.0f ? 1.0f : s);
-        float r[3] = {theta[0] * s, theta[1] * s, theta[2] * s};
-        float q[3];
+        real s = (t[i] - t_mid) * inv_dt;
+        s = s < -((real)1.0) ? -((real)1.0) : (s > ((real)1.0) ? ((real)1.0) : s);
+        real r[3] = {theta[0] * s, theta[1] * s, theta[2] * s};
+        real q[3];
         rotvec_rotate(r, raw + 3 * (size_t)i, q);
         out[3 * (size_t)i + 0] = q[0] + rho[0] * s;
         out[3 * (size_t)i + 1] = q[1] + rho[1] * s;
@@ -159,12 +175,12 @@ void lwo_deskew(const float *raw, const float *t, int m, float t_mid, float inv_
 
 /* ---- thinning (odom.py:126-160) -------------------------------------------------------------- */
 
-void lwo_ds_insert(const float *xyz, const uint32_t *attr, int n, float inv_v, float min_range, float max_range,
-                   int ret_mask, int mask, int64_t *keys, int32_t *count, float *out)
+void lwo_ds_insert(const real *xyz, const uint32_t *attr, int n, real inv_v, real min_range, real max_range,
+                   int ret_mask, int mask, int64_t *keys, int32_t *count, real *out)
 {
     for (int i = 0; i < n; ++i) {
-        const float *p = xyz + 3 * (size_t)i;
-        float r = length3(p);
+        const real *p = xyz + 3 * (size_t)i;
+        real r = length3(p);
         if (r < min_range || r > max_range) continue;
         if (tag_of(attr[i]) != 0 || !ret_ok(attr[i], ret_mask)) continue;
         int64_t key = lwo_voxel_key(p, inv_v);
@@ -173,7 +189,7 @@ void lwo_ds_insert(const float *xyz, const uint32_t *attr, int n, float inv_v, f
             int32_t s = (h + probe) & mask;
             if (keys[s] == -1) {
                 keys[s] = key;
-                memcpy(out + 3 * (size_t)(*count), p, 12);
+                memcpy(out + 3 * (size_t)(*count), p, 3 * sizeof(real));
                 (*count)++;
                 break;
             }
@@ -184,20 +200,20 @@ void lwo_ds_insert(const float *xyz, const uint32_t *attr, int n, float inv_v, f
 
 /* ---- map build (odom.py:181-230) ------------------------------------------------------------- */
 
-static inline float center_of(int32_t i, float voxel) { return ((float)i + 0.5f) * voxel; }
+static inline real center_of(int32_t i, real voxel) { return ((real)i + ((real)0.5)) * voxel; }
 
-void lwo_reg_insert(const float *xyz, const uint32_t *attr, int n, const float *T, float inv_v, float voxel,
-                    float max_range, int ret_mask, int mask, int64_t *keys, float *rsum, float *router,
+void lwo_reg_insert(const real *xyz, const uint32_t *attr, int n, const real *T, real inv_v, real voxel,
+                    real max_range, int ret_mask, int mask, int64_t *keys, real *rsum, real *router,
                     int32_t *rcnt, int32_t *stats)
 {
     for (int i = 0; i < n; ++i) {
-        const float *p = xyz + 3 * (size_t)i;
+        const real *p = xyz + 3 * (size_t)i;
         if (length3(p) > max_range || tag_of(attr[i]) != 0 || !ret_ok(attr[i], ret_mask)) continue;
-        float w[3];
+        real w[3];
         lwo_transform_point(T, p, w);
         int32_t ix = ifloor(w[0] * inv_v), iy = ifloor(w[1] * inv_v), iz = ifloor(w[2] * inv_v);
         int64_t key = lwo_voxel_key_ijk(ix, iy, iz);
-        float r[3] = {w[0] - center_of(ix, voxel), w[1] - center_of(iy, voxel), w[2] - center_of(iz, voxel)};
+        real r[3] = {w[0] - center_of(ix, voxel), w[1] - center_of(iy, voxel), w[2] - center_of(iz, voxel)};
         int32_t h = lwo_slot_hash(key) & mask;
         int done = 0;
         for (int probe = 0; probe < LWO_PROBES && !done; ++probe) {
@@ -271,11 +287,11 @@ void lwo_eig3(const double *A_in, double *ev, double *Q)
  * xx xy xz yy yz zz), its tight part Mg, the tightest direction q and whether the voxel is a shape
  * (1: rn = r.q) or the weak isotropic branch (0: rn = |r|). Returns the shape flag.
  */
-int lwo_voxel_precision(const float *rsum, const float *router, int32_t c, float voxel, float noise2, float tight2,
-                        float alpha, float *M6, float *Mg6, float *q3)
+int lwo_voxel_precision(const real *rsum, const real *router, int32_t c, real voxel, real noise2, real tight2,
+                        real alpha, real *M6, real *Mg6, real *q3)
 {
-    float cf = (float)c;
-    float mrel[3] = {rsum[0] / cf, rsum[1] / cf, rsum[2] / cf};
+    real cf = (real)c;
+    real mrel[3] = {rsum[0] / cf, rsum[1] / cf, rsum[2] / cf};
     double cov[9];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) cov[3 * i + j] = (double)(router[3 * i + j] / cf - mrel[i] * mrel[j]);
@@ -300,33 +316,33 @@ int lwo_voxel_precision(const float *rsum, const float *router, int32_t c, float
                     if (tight) Mg[3 * i + j] += wa * q[i] * q[j];
                 }
             if (a == kmin)
-                for (int i = 0; i < 3; ++i) q3[i] = (float)q[i];
+                for (int i = 0; i < 3; ++i) q3[i] = (real)q[i];
         }
         const int iu[6] = {0, 1, 2, 4, 5, 8};
         for (int k = 0; k < 6; ++k) {
-            M6[k] = (float)M[iu[k]];
-            Mg6[k] = (float)Mg[iu[k]];
+            M6[k] = (real)M[iu[k]];
+            Mg6[k] = (real)Mg[iu[k]];
         }
         return 1;
     }
-    float wa = alpha / (0.25f * voxel * voxel + noise2);
-    M6[0] = wa; M6[1] = 0.0f; M6[2] = 0.0f; M6[3] = wa; M6[4] = 0.0f; M6[5] = wa;
-    for (int k = 0; k < 6; ++k) Mg6[k] = 0.0f;
-    q3[0] = q3[1] = q3[2] = 0.0f;
+    real wa = alpha / (((real)0.25) * voxel * voxel + noise2);
+    M6[0] = wa; M6[1] = ((real)0.0); M6[2] = ((real)0.0); M6[3] = wa; M6[4] = ((real)0.0); M6[5] = wa;
+    for (int k = 0; k < 6; ++k) Mg6[k] = ((real)0.0);
+    q3[0] = q3[1] = q3[2] = ((real)0.0);
     return 0;
 }
 
 /* ---- accumulation (odom.py:233-272) ---------------------------------------------------------- */
 
-static void accumulate_h(double *acc, const float J[3][6], const float M[3][3], float wgt)
+static void accumulate_h(double *acc, const real J[3][6], const real M[3][3], real wgt)
 {
     int idx = 0;
     for (int p = 0; p < 6; ++p)
         for (int q = p; q < 6; ++q) {
             /* odom.py:250-252: v = row 0; v += row 1; v += row 2, each row summed left to right */
-            float v = 0.0f;
+            real v = ((real)0.0);
             for (int i = 0; i < 3; ++i) {
-                float row = M[i][0] * J[i][p] * J[0][q];
+                real row = M[i][0] * J[i][p] * J[0][q];
                 row = row + M[i][1] * J[i][p] * J[1][q];
                 row = row + M[i][2] * J[i][p] * J[2][q];
                 v = (i == 0) ? row : v + row;
@@ -335,9 +351,9 @@ static void accumulate_h(double *acc, const float J[3][6], const float M[3][3], 
         }
 }
 
-static void accumulate(double *acc, const float J[3][6], const float M[3][3], const float *r, float wgt)
+static void accumulate(double *acc, const real J[3][6], const real M[3][3], const real *r, real wgt)
 {
-    float Mr[3];
+    real Mr[3];
     for (int i = 0; i < 3; ++i) Mr[i] = M[i][0] * r[0] + M[i][1] * r[1] + M[i][2] * r[2];
     accumulate_h(acc + ACC_H, J, M, wgt);
     for (int p = 0; p < 6; ++p) acc[ACC_B + p] += (double)(wgt * (Mr[0] * J[0][p] + Mr[1] * J[1][p] + Mr[2] * J[2][p]));
@@ -347,20 +363,20 @@ static void accumulate(double *acc, const float J[3][6], const float M[3][3], co
 }
 
 /* One point of k_icp (odom.py:299-375). Returns the slot of the chosen voxel, or -1. */
-static int32_t icp_point(const float *p, int i, const float *T, float inv_v, float voxel, int mask,
-                         const int64_t *keys, const float *rsum, const float *router, const int32_t *rcnt, int reach,
-                         float max_d2, float noise2, float tight2, float alpha, float kappa, double *acc)
+static int32_t icp_point(const real *p, int i, const real *T, real inv_v, real voxel, int mask,
+                         const int64_t *keys, const real *rsum, const real *router, const int32_t *rcnt, int reach,
+                         real max_d2, real noise2, real tight2, real alpha, real kappa, double *acc)
 {
-    float w[3];
+    real w[3];
     lwo_transform_point(T, p, w);
     double *slot = acc + (size_t)(i & (LWO_ACC_SLOTS - 1)) * LWO_ACC_STRIDE;
-    float J[3][6] = {{1.0f, 0.0f, 0.0f, 0.0f, w[2], -w[1]},
-                     {0.0f, 1.0f, 0.0f, -w[2], 0.0f, w[0]},
-                     {0.0f, 0.0f, 1.0f, w[1], -w[0], 0.0f}};
+    real J[3][6] = {{((real)1.0), ((real)0.0), ((real)0.0), ((real)0.0), w[2], -w[1]},
+                     {((real)0.0), ((real)1.0), ((real)0.0), -w[2], ((real)0.0), w[0]},
+                     {((real)0.0), ((real)0.0), ((real)1.0), w[1], -w[0], ((real)0.0)}};
     int32_t bx = ifloor(w[0] * inv_v), by = ifloor(w[1] * inv_v), bz = ifloor(w[2] * inv_v);
-    float best_d2 = max_d2;
+    real best_d2 = max_d2;
     int32_t best_s = -1, best_c = 0;
-    float best_mu[3] = {0, 0, 0};
+    real best_mu[3] = {0, 0, 0};
     for (int dx = -reach; dx <= reach; ++dx)
         for (int dy = -reach; dy <= reach; ++dy)
             for (int dz = -reach; dz <= reach; ++dz) {
@@ -369,31 +385,31 @@ static int32_t icp_point(const float *p, int i, const float *T, float inv_v, flo
                 if (s < 0) continue;
                 int32_t c = rcnt[s];
                 if (c <= 0) continue;
-                float cf = (float)c;
-                float mu[3] = {center_of(jx, voxel) + rsum[3 * (size_t)s + 0] / cf,
+                real cf = (real)c;
+                real mu[3] = {center_of(jx, voxel) + rsum[3 * (size_t)s + 0] / cf,
                                center_of(jy, voxel) + rsum[3 * (size_t)s + 1] / cf,
                                center_of(jz, voxel) + rsum[3 * (size_t)s + 2] / cf};
-                float dd[3] = {w[0] - mu[0], w[1] - mu[1], w[2] - mu[2]};
-                float d2 = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
+                real dd[3] = {w[0] - mu[0], w[1] - mu[1], w[2] - mu[2]};
+                real d2 = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
                 if (d2 < best_d2) {
                     best_d2 = d2;
                     best_s = s;
                     best_c = c;
-                    memcpy(best_mu, mu, 12);
+                    memcpy(best_mu, mu, 3 * sizeof(real));
                 }
             }
     if (best_s < 0) return -1;
-    float r[3] = {w[0] - best_mu[0], w[1] - best_mu[1], w[2] - best_mu[2]};
-    float M6[6], Mg6[6], q[3];
+    real r[3] = {w[0] - best_mu[0], w[1] - best_mu[1], w[2] - best_mu[2]};
+    real M6[6], Mg6[6], q[3];
     int shape = lwo_voxel_precision(rsum + 3 * (size_t)best_s, router + 9 * (size_t)best_s, best_c, voxel, noise2,
                                     tight2, alpha, M6, Mg6, q);
-    float M[3][3] = {{M6[0], M6[1], M6[2]}, {M6[1], M6[3], M6[4]}, {M6[2], M6[4], M6[5]}};
-    float Mg[3][3] = {{Mg6[0], Mg6[1], Mg6[2]}, {Mg6[1], Mg6[3], Mg6[4]}, {Mg6[2], Mg6[4], Mg6[5]}};
-    float rn = shape ? (r[0] * q[0] + r[1] * q[1] + r[2] * q[2]) : length3(r);
-    float Mr[3];
+    real M[3][3] = {{M6[0], M6[1], M6[2]}, {M6[1], M6[3], M6[4]}, {M6[2], M6[4], M6[5]}};
+    real Mg[3][3] = {{Mg6[0], Mg6[1], Mg6[2]}, {Mg6[1], Mg6[3], Mg6[4]}, {Mg6[2], Mg6[4], Mg6[5]}};
+    real rn = shape ? (r[0] * q[0] + r[1] * q[1] + r[2] * q[2]) : length3(r);
+    real Mr[3];
     for (int a = 0; a < 3; ++a) Mr[a] = M[a][0] * r[0] + M[a][1] * r[1] + M[a][2] * r[2];
-    float d2 = r[0] * Mr[0] + r[1] * Mr[1] + r[2] * Mr[2];
-    float wgt = 1.0f / (1.0f + d2 / (kappa * kappa));
+    real d2 = r[0] * Mr[0] + r[1] * Mr[1] + r[2] * Mr[2];
+    real wgt = ((real)1.0) / (((real)1.0) + d2 / (kappa * kappa));
     wgt = wgt * wgt;
     slot[ACC_RMS] += (double)(wgt * rn * rn);
     accumulate(slot, J, M, r, wgt);
@@ -404,9 +420,9 @@ static int32_t icp_point(const float *p, int i, const float *T, float inv_v, flo
 
 /* k_icp over m points (odom.py:275-375) into acc[ACC_SLOTS * ACC_STRIDE]; best_slot (optional) gets
  * each point's chosen slot or -1. */
-void lwo_icp(const float *ds, int m, const float *T, float inv_v, float voxel, int mask, const int64_t *keys,
-             const float *rsum, const float *router, const int32_t *rcnt, int reach, float max_d2, float noise2,
-             float tight2, float alpha, float kappa, double *acc, int32_t *best_slot)
+void lwo_icp(const real *ds, int m, const real *T, real inv_v, real voxel, int mask, const int64_t *keys,
+             const real *rsum, const real *router, const int32_t *rcnt, int reach, real max_d2, real noise2,
+             real tight2, real alpha, real kappa, double *acc, int32_t *best_slot)
 {
     for (int i = 0; i < m; ++i) {
         int32_t s = icp_point(ds + 3 * (size_t)i, i, T, inv_v, voxel, mask, keys, rsum, router, rcnt, reach, max_d2,

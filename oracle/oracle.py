@@ -51,46 +51,43 @@ _f64p = ctypes.POINTER(ctypes.c_double)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _i64p = ctypes.POINTER(ctypes.c_int64)
-_lib = None
+_libs = {}
+LIB64_PATH = os.path.join(HERE, "liblw_oracle64.so")
+
+# signatures; "R" is the device-side real (float32, or float64 in the all-float64 build), "RP" a pointer to it
+_I, _L = ctypes.c_int, ctypes.c_int64
+_SIGS = {
+    "lwo_voxel_key_ijk": (ctypes.c_int64, [ctypes.c_int32] * 3),
+    "lwo_slot_hash": (ctypes.c_int32, [ctypes.c_int64]),
+    "lwo_find_slot": (ctypes.c_int32, [ctypes.c_int64, _i64p, ctypes.c_int32]),
+    "lwo_frame_append": (None, ["RP", _u32p, "RP", _I, "R", _I, _I, _I, _i32p, "RP", _u32p, "RP", _i32p]),
+    "lwo_deskew": (None, ["RP", "RP", _I, "R", "R", "RP", "RP", "RP"]),
+    "lwo_ds_insert": (None, ["RP", _u32p, _I, "R", "R", "R", _I, _I, _i64p, _i32p, "RP"]),
+    "lwo_reg_insert": (None, ["RP", _u32p, _I, "RP", "R", "R", "R", _I, _I, _i64p, "RP", "RP", _i32p, _i32p]),
+    "lwo_eig3": (None, [_f64p, _f64p, _f64p]),
+    "lwo_voxel_precision": (ctypes.c_int, ["RP", "RP", ctypes.c_int32, "R", "R", "R", "R", "RP", "RP", "RP"]),
+    "lwo_icp": (None, ["RP", _I, "RP", "R", "R", _I, _i64p, "RP", "RP", _i32p, _I, "R", "R", "R", "R", "R", _f64p,
+                       _i32p]),
+    "lwo_reduce": (None, [_f64p, _f64p]),
+}
 
 
-def lib():
-    """The compiled restatement (built by `make -C oracle`)."""
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
-        L = ctypes.CDLL(LIB_PATH)
-        L.lwo_voxel_key_ijk.restype = ctypes.c_int64
-        L.lwo_voxel_key_ijk.argtypes = [ctypes.c_int32] * 3
-        L.lwo_slot_hash.restype = ctypes.c_int32
-        L.lwo_slot_hash.argtypes = [ctypes.c_int64]
-        L.lwo_find_slot.restype = ctypes.c_int32
-        L.lwo_find_slot.argtypes = [ctypes.c_int64, _i64p, ctypes.c_int32]
-        L.lwo_frame_append.restype = None
-        L.lwo_frame_append.argtypes = [_f32p, _u32p, _f32p, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int,
-                                       ctypes.c_int, _i32p, _f32p, _u32p, _f32p, _i32p]
-        L.lwo_deskew.restype = None
-        L.lwo_deskew.argtypes = [_f32p, _f32p, ctypes.c_int, ctypes.c_float, ctypes.c_float, _f32p, _f32p, _f32p]
-        L.lwo_ds_insert.restype = None
-        L.lwo_ds_insert.argtypes = [_f32p, _u32p, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
-                                    ctypes.c_int, ctypes.c_int, _i64p, _i32p, _f32p]
-        L.lwo_reg_insert.restype = None
-        L.lwo_reg_insert.argtypes = [_f32p, _u32p, ctypes.c_int, _f32p, ctypes.c_float, ctypes.c_float, ctypes.c_float,
-                                     ctypes.c_int, ctypes.c_int, _i64p, _f32p, _f32p, _i32p, _i32p]
-        L.lwo_eig3.restype = None
-        L.lwo_eig3.argtypes = [_f64p, _f64p, _f64p]
-        L.lwo_voxel_precision.restype = ctypes.c_int
-        L.lwo_voxel_precision.argtypes = [_f32p, _f32p, ctypes.c_int32, ctypes.c_float, ctypes.c_float,
-                                          ctypes.c_float, ctypes.c_float, _f32p, _f32p, _f32p]
-        L.lwo_icp.restype = None
-        L.lwo_icp.argtypes = [_f32p, ctypes.c_int, _f32p, ctypes.c_float, ctypes.c_float, ctypes.c_int, _i64p, _f32p,
-                              _f32p, _i32p, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
-                              ctypes.c_float, ctypes.c_float, _f64p, _i32p]
-        L.lwo_reduce.restype = None
-        L.lwo_reduce.argtypes = [_f64p, _f64p]
-        _lib = L
-    return _lib
+def lib(fp64: bool = False):
+    """The compiled restatement (built by `make -C oracle`): the float32 one (the reference's arithmetic), or
+    with fp64=True the same code with every device-side step in float64."""
+    if fp64 not in _libs:
+        path = LIB64_PATH if fp64 else LIB_PATH
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run `make -C oracle`)")
+        L = ctypes.CDLL(path)
+        R = ctypes.c_double if fp64 else ctypes.c_float
+        RP = _f64p if fp64 else _f32p
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = [R if a == "R" else RP if a == "RP" else a for a in args]
+        _libs[fp64] = L
+    return _libs[fp64]
 
 
 def _p(a: np.ndarray, t):
@@ -100,6 +97,14 @@ def _p(a: np.ndarray, t):
 
 def _f32(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _real(a, fp64: bool) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64 if fp64 else np.float32)
+
+
+def _rp(fp64: bool):
+    return _f64p if fp64 else _f32p
 
 
 # ---- keys and hash: an independent numpy int64 restatement (gpu.py:231-252) -----------------------
@@ -154,25 +159,30 @@ def slot_hash(k: int) -> int:
 
 
 class OracleMap:
-    """The registration map of odom.py:468-474 (SoA, per slot) built by the serial k_reg_insert."""
+    """The registration map of odom.py:468-474 (SoA, per slot) built by the serial k_reg_insert. fp64=True keeps
+    the moments in float64 (the all-float64 oracle)."""
 
-    def __init__(self, slots: int):
+    def __init__(self, slots: int, fp64: bool = False):
         assert slots & (slots - 1) == 0
         self.cap = slots
+        self.fp64 = fp64
+        rt = np.float64 if fp64 else np.float32
         self.keys = np.full(slots, -1, np.int64)
-        self.rsum = np.zeros((slots, 3), np.float32)
-        self.router = np.zeros((slots, 3, 3), np.float32)
+        self.rsum = np.zeros((slots, 3), rt)
+        self.router = np.zeros((slots, 3, 3), rt)
         self.rcnt = np.zeros(slots, np.int32)
         self.stats = np.zeros(2, np.int32)
 
     def insert(self, xyz, attr, T, voxel: float, max_range: float = 60.0, ret_mask: int = 0b1111):
         """odom.py:186-230, driven as Odometry._insert drives it (odom.py:719-727)."""
-        xyz = _f32(xyz).reshape(-1, 3)
+        fp = self.fp64
+        xyz = _real(xyz, fp).reshape(-1, 3)
         attr = np.ascontiguousarray(attr, np.uint32)
-        T32 = _f32(np.asarray(T, np.float64).reshape(4, 4))
-        lib().lwo_reg_insert(_p(xyz, _f32p), _p(attr, _u32p), len(xyz), _p(T32, _f32p), 1.0 / voxel, voxel,
-                             max_range, int(ret_mask), self.cap - 1, _p(self.keys, _i64p), _p(self.rsum, _f32p),
-                             _p(self.router, _f32p), _p(self.rcnt, _i32p), _p(self.stats, _i32p))
+        Tr = _real(np.asarray(T, np.float64).reshape(4, 4), fp)
+        rp = _rp(fp)
+        lib(fp).lwo_reg_insert(_p(xyz, rp), _p(attr, _u32p), len(xyz), _p(Tr, rp), 1.0 / voxel, voxel, max_range,
+                               int(ret_mask), self.cap - 1, _p(self.keys, _i64p), _p(self.rsum, rp),
+                               _p(self.router, rp), _p(self.rcnt, _i32p), _p(self.stats, _i32p))
 
     def find_slot(self, key: int) -> int:
         return int(lib().lwo_find_slot(key, _p(self.keys, _i64p), self.cap - 1))
@@ -182,24 +192,25 @@ class OracleMap:
 
 
 def frame_append(sx, sa, st, inv_dt: float, k_min: int, k_open: int, cap: int, count: int,
-                 f_xyz, f_attr, f_t, f_frame) -> int:
+                 f_xyz, f_attr, f_t, f_frame, fp64: bool = False) -> int:
     """odom.py:97-123; returns the new count (which, as in the reference, may exceed cap)."""
-    sx, st = _f32(sx).reshape(-1, 3), _f32(st)
+    sx, st = _real(sx, fp64).reshape(-1, 3), _real(st, fp64)
     sa = np.ascontiguousarray(sa, np.uint32)
     c = np.array([count], np.int32)
-    lib().lwo_frame_append(_p(sx, _f32p), _p(sa, _u32p), _p(st, _f32p), len(st), inv_dt, k_min, k_open, cap,
-                           _p(c, _i32p), _p(f_xyz, _f32p), _p(f_attr, _u32p), _p(f_t, _f32p), _p(f_frame, _i32p))
+    rp = _rp(fp64)
+    lib(fp64).lwo_frame_append(_p(sx, rp), _p(sa, _u32p), _p(st, rp), len(st), inv_dt, k_min, k_open, cap,
+                               _p(c, _i32p), _p(f_xyz, rp), _p(f_attr, _u32p), _p(f_t, rp), _p(f_frame, _i32p))
     return int(c[0])
 
 
-def deskew(raw, t, t_mid: float, inv_dt: float, xi) -> np.ndarray:
-    """odom.py:163-178 in float32."""
-    raw, t = _f32(raw).reshape(-1, 3), _f32(t)
-    xi = _f32(xi)
-    rho, theta = _f32(xi[:3]), _f32(xi[3:])
+def deskew(raw, t, t_mid: float, inv_dt: float, xi, fp64: bool = False) -> np.ndarray:
+    """odom.py:163-178 in float32 (or float64)."""
+    raw, t = _real(raw, fp64).reshape(-1, 3), _real(t, fp64)
+    xi = _real(xi, fp64)
+    rho, theta = _real(xi[:3], fp64), _real(xi[3:], fp64)
     out = np.empty_like(raw)
-    lib().lwo_deskew(_p(raw, _f32p), _p(t, _f32p), len(t), t_mid, inv_dt, _p(rho, _f32p), _p(theta, _f32p),
-                     _p(out, _f32p))
+    rp = _rp(fp64)
+    lib(fp64).lwo_deskew(_p(raw, rp), _p(t, rp), len(t), t_mid, inv_dt, _p(rho, rp), _p(theta, rp), _p(out, rp))
     return out
 
 
@@ -212,16 +223,18 @@ def deskew_fp64(p: np.ndarray, s: np.ndarray, xi) -> np.ndarray:
     return Rotation.from_rotvec(rv).apply(p) + s[:, None] * np.asarray(xi[:3], dtype=np.float64)[None, :]
 
 
-def ds_insert(xyz, attr, inv_v: float, min_range: float, max_range: float, ret_mask: int, ds_cap: int):
+def ds_insert(xyz, attr, inv_v: float, min_range: float, max_range: float, ret_mask: int, ds_cap: int,
+              fp64: bool = False):
     """odom.py:126-160 with a freshly cleared table of ds_cap slots (odom.py:731-737): the thinned points in
     index order (the serial semantics)."""
-    xyz = _f32(xyz).reshape(-1, 3)
+    xyz = _real(xyz, fp64).reshape(-1, 3)
     attr = np.ascontiguousarray(attr, np.uint32)
     keys = np.full(ds_cap, -1, np.int64)
     cnt = np.zeros(1, np.int32)
     out = np.zeros_like(xyz)
-    lib().lwo_ds_insert(_p(xyz, _f32p), _p(attr, _u32p), len(xyz), inv_v, min_range, max_range, int(ret_mask),
-                        ds_cap - 1, _p(keys, _i64p), _p(cnt, _i32p), _p(out, _f32p))
+    rp = _rp(fp64)
+    lib(fp64).lwo_ds_insert(_p(xyz, rp), _p(attr, _u32p), len(xyz), inv_v, min_range, max_range, int(ret_mask),
+                            ds_cap - 1, _p(keys, _i64p), _p(cnt, _i32p), _p(out, rp))
     return out[: int(cnt[0])].copy()
 
 
@@ -243,16 +256,19 @@ def eig3(A) -> tuple[np.ndarray, np.ndarray]:
 
 def icp_accumulate(omap: OracleMap, ds, T, voxel: float, reach: int, max_d2: float, noise2: float, tight2: float,
                    alpha: float, kappa: float, with_slots: bool = False):
-    """One k_icp pass + k_reduce (odom.py:275-384): the 64 reduced float64 entries of the ACC layout."""
-    ds = _f32(ds).reshape(-1, 3)
-    T32 = _f32(np.asarray(T, np.float64).reshape(4, 4))
+    """One k_icp pass + k_reduce (odom.py:275-384): the 64 reduced float64 entries of the ACC layout (device math
+    in the map's arithmetic: float32, or float64 for an fp64 map)."""
+    fp = omap.fp64
+    rp = _rp(fp)
+    ds = _real(ds, fp).reshape(-1, 3)
+    Tr = _real(np.asarray(T, np.float64).reshape(4, 4), fp)
     acc = np.zeros(ACC_SLOTS * ACC_STRIDE, np.float64)
     best = np.zeros(len(ds), np.int32) if with_slots else None
-    lib().lwo_icp(_p(ds, _f32p), len(ds), _p(T32, _f32p), 1.0 / voxel, voxel, omap.cap - 1, _p(omap.keys, _i64p),
-                  _p(omap.rsum, _f32p), _p(omap.router, _f32p), _p(omap.rcnt, _i32p), reach, max_d2, noise2, tight2,
-                  alpha, kappa, _p(acc, _f64p), _p(best, _i32p) if best is not None else None)
+    lib(fp).lwo_icp(_p(ds, rp), len(ds), _p(Tr, rp), 1.0 / voxel, voxel, omap.cap - 1, _p(omap.keys, _i64p),
+                    _p(omap.rsum, rp), _p(omap.router, rp), _p(omap.rcnt, _i32p), reach, max_d2, noise2, tight2,
+                    alpha, kappa, _p(acc, _f64p), _p(best, _i32p) if best is not None else None)
     out = np.zeros(ACC_STRIDE, np.float64)
-    lib().lwo_reduce(_p(acc, _f64p), _p(out, _f64p))
+    lib(fp).lwo_reduce(_p(acc, _f64p), _p(out, _f64p))
     return (out, best) if with_slots else out
 
 
@@ -383,8 +399,9 @@ def register(omap: OracleMap, ds: np.ndarray, T_pred: np.ndarray, P: Params, kee
         return RegResult(T=T_pred.copy(), ok=False, n_ds=n_ds)
     T = T_pred.copy()
     voxel = P.reg_voxel
-    noise2 = np.float32(P.noise * P.noise)
-    tight2 = np.float32((TIGHT_FRACTION * voxel / math.sqrt(12.0)) ** 2)
+    rt = np.float64 if omap.fp64 else np.float32  # the kernel arguments' type (odom.py:744-746)
+    noise2 = rt(P.noise * P.noise)
+    tight2 = rt((TIGHT_FRACTION * voxel / math.sqrt(12.0)) ** 2)
     iters, corr, rms, cond, n_degen = 0, 0, 0.0, 1.0, 0
     H = np.eye(6)
     accs = []
@@ -425,15 +442,17 @@ class OracleOdometry:
     host step every iteration, keyframe inserts, constant-velocity prediction. Poses go to `pipe` like the
     reference's (set_pose / set_pose_range)."""
 
-    def __init__(self, pipe, reg_slots: int = 1 << 21, frame_cap: int = 1 << 18):
+    def __init__(self, pipe, reg_slots: int = 1 << 21, frame_cap: int = 1 << 18, fp64: bool = False):
         self.pipe = pipe
+        self.fp64 = fp64
+        rt = np.float64 if fp64 else np.float32
         self.reg_cap = reg_slots
         self.frame_cap = frame_cap
         self.ds_cap = 1 << (frame_cap - 1).bit_length()
-        self.f_raw = np.zeros((frame_cap, 3), np.float32)
-        self.f_xyz = np.zeros((frame_cap, 3), np.float32)
+        self.f_raw = np.zeros((frame_cap, 3), rt)
+        self.f_xyz = np.zeros((frame_cap, 3), rt)
         self.f_attr = np.zeros(frame_cap, np.uint32)
-        self.f_t = np.zeros(frame_cap, np.float32)
+        self.f_t = np.zeros(frame_cap, rt)
         self.f_frame = np.zeros(frame_cap, np.int32)
         self.f_count = 0
         self.frame_dt = 0.1
@@ -460,7 +479,7 @@ class OracleOdometry:
         self.reset(np.eye(4))
 
     def reset(self, T0):
-        self.map = OracleMap(self.reg_cap)
+        self.map = OracleMap(self.reg_cap, fp64=self.fp64)
         self.T = np.array(T0, dtype=np.float64)
         self.T_prev = None
         self.delta = np.eye(4)
@@ -518,7 +537,7 @@ class OracleOdometry:
         k_min = self.k_open - 1
         while True:
             self.f_count = frame_append(sx, sa, st, inv_dt, k_min, self.k_open, self.frame_cap, self.f_count,
-                                        self.f_raw, self.f_attr, self.f_t, self.f_frame)
+                                        self.f_raw, self.f_attr, self.f_t, self.f_frame, fp64=self.fp64)
             if k_hi <= self.k_open:
                 break
             self._finalize(self.k_open, on_frame)
@@ -548,7 +567,7 @@ class OracleOdometry:
     def _estimate(self, m, t_mid, warm):
         xi = self.xi if (self.deskew and not warm) else np.zeros(6, np.float32)
         if m > 0:
-            self.f_xyz[:m] = deskew(self.f_raw[:m], self.f_t[:m], t_mid, 1.0 / self.frame_dt, xi)
+            self.f_xyz[:m] = deskew(self.f_raw[:m], self.f_t[:m], t_mid, 1.0 / self.frame_dt, xi, self.fp64)
         T = self.T if warm else self.predict()
         if m < self.min_points:
             self.stats["skipped"] += 1
@@ -561,7 +580,7 @@ class OracleOdometry:
             if (np.linalg.norm(xi_new[3:] - xi[3:]) > REDESKEW_ROT
                     or np.linalg.norm(xi_new[:3] - xi[:3]) > REDESKEW_TRANS):
                 xi = xi_new
-                self.f_xyz[:m] = deskew(self.f_raw[:m], self.f_t[:m], t_mid, 1.0 / self.frame_dt, xi)
+                self.f_xyz[:m] = deskew(self.f_raw[:m], self.f_t[:m], t_mid, 1.0 / self.frame_dt, xi, self.fp64)
                 T, ok = self._register(m, T)
         return T, xi, ok
 
@@ -593,7 +612,7 @@ class OracleOdometry:
 
     def _register(self, m, T_pred):
         ds = ds_insert(self.f_xyz[:m], self.f_attr[:m], 2.0 / self.reg_voxel, self.min_range, self.max_range,
-                       self.ret_mask, self.ds_cap)
+                       self.ret_mask, self.ds_cap, self.fp64)
         self.stats["ds"] = len(ds)
         r = register(self.map, ds, T_pred, self.params())
         if len(ds) < self.min_corr:
