@@ -27,6 +27,7 @@ extern "C" {
 
 #define LW_EINVAL (-1000)
 #define LW_ENOTBOUND (-1001)
+#define LW_ELOOP (-1002) /* the persistent registration loop stopped waiting (a bounded wait ran out) */
 #define LW_ACC_STRIDE 64
 
 typedef struct lw_ctx lw_ctx;
@@ -130,6 +131,10 @@ int lw_icp_accum(lw_ctx *ctx, int32_t n, const float *T, float inv_v, float voxe
 int lw_register(lw_ctx *ctx, int32_t m, const lw_reg_params *p, const double *T_pred, lw_reg_result *res);
 /* The same loop over the already-thinned points (lw_thin / lw_thin_load), n of them. */
 int lw_register_thinned(lw_ctx *ctx, int32_t n, const lw_reg_params *p, const double *T_pred, lw_reg_result *res);
+/* on = 1 (default): a registration is one cooperative launch that runs every iteration; on = 0: one launch
+ * per iteration (the path taken automatically where cooperative launches are unavailable). Same results up
+ * to the order of the float64 partial sums. No counterpart in the reference. */
+int lw_set_persistent(lw_ctx *ctx, int32_t on);
 
 /* Live measurement of the accumulation launches (for throughput and roofline reports): lw_prof_enable
  * turns on HIP events around each accumulation launch and device counters; lw_prof_read returns, since

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "liblwamd.so")
 
 LW_EINVAL = -1000
 LW_ENOTBOUND = -1001
+LW_ELOOP = -1002
 
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -60,6 +61,7 @@ SIGNATURES = {
     "lw_icp_accum": (_i32, [_vp, _i32, _f32p, _f32, _f32, _i32, _f32, _f32, _f32, _f32, _f32, _i32, _f64p, _i32p]),
     "lw_register": (_i32, [_vp, _i32, ctypes.POINTER(RegParams), _f64p, ctypes.POINTER(RegResult)]),
     "lw_register_thinned": (_i32, [_vp, _i32, ctypes.POINTER(RegParams), _f64p, ctypes.POINTER(RegResult)]),
+    "lw_set_persistent": (_i32, [_vp, _i32]),
     "lw_prof_enable": (_i32, [_vp, _i32]),
     "lw_prof_reset": (_i32, [_vp]),
     "lw_prof_read": (_i32, [_vp, _f64p]),

@@ -342,46 +342,90 @@ int lw_icp_accum(lw_ctx *c, int32_t n, const float *T, float inv_v, float voxel,
     return 0;
 }
 
-int lw_register_thinned(lw_ctx *c, int32_t n, const lw_reg_params *p, const double *T_pred, lw_reg_result *res) {
-    if (!c || !p || !T_pred || !res || n < 0 || n > c->frame_cap || p->max_iter < 0 || p->acc_mode < 0 ||
-        p->acc_mode > 2 || !(p->reg_voxel > 0.0f))
-        return LW_EINVAL;
+}  // extern "C"
+
+namespace {
+
+// Odometry._register from the thinned points on (odom.py:756-803). The thinned count is on the device
+// (n_dev) and n_bound bounds it on the host; n_known >= 0 when the host knows it exactly.
+int run_registration(lw_ctx *c, const int32_t *n_dev, int32_t n_bound, int32_t n_known, const lw_reg_params *p,
+                     const double *T_pred, lw_reg_result *res) {
     std::memset(res, 0, sizeof(*res));
     std::memcpy(res->T, T_pred, 16 * sizeof(double));
-    res->n_ds = n;
     res->cond = 1.0;
-    if (n < p->min_corr) return 0;  // odom.py:756-757
+    if (n_known >= 0 && n_known < p->min_corr) {  // odom.py:756-757
+        res->n_ds = n_known;
+        return 0;
+    }
     const double voxel = p->reg_voxel;
     const float noise2 = (float)((double)p->noise * (double)p->noise);
     const double tf = 0.6 * voxel / std::sqrt(12.0);  // TIGHT_FRACTION (odom.py:89, 761)
     const float tight2 = (float)(tf * tf);
     LW_CK(ensure_precision(c, (float)voxel, noise2, tight2, p->alpha));
+    // the schedule (odom.py:765-767), computed in float64 and passed as float32 like the reference's launch
+    lw::RegLoop L{};
+    L.max_iter = p->max_iter;
+    L.wide_iters = p->wide_iters;
+    L.min_corr = p->min_corr;
+    L.inv_v = (float)(1.0 / voxel);
+    L.voxel = (float)voxel;
+    L.max_d2_r1 = (float)((1.5 * voxel) * (1.5 * voxel));
+    L.max_d2_r2 = (float)((2.5 * voxel) * (2.5 * voxel));
+    for (int it = 0; it < 64; ++it) {
+        const double kap = 6.0 * std::pow(0.7, it);
+        L.kappa[it] = (float)(kap > 2.0 ? kap : 2.0);
+    }
+    L.sp.damping = p->damping;
+    L.sp.prior_info = p->prior_info;
+    L.sp.degen_thr = p->degen_thr;
+    L.sp.min_corr = p->min_corr;
+    L.sp.pad = 0;
     std::memcpy(pin_up(c), T_pred, 16 * sizeof(double));
-    LW_CK(hipMemcpyAsync(c->dtmp, pin_up(c), 16 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    std::memcpy(pin_up(c) + 16 * sizeof(double), L.kappa, 64 * sizeof(float));
+    LW_CK(hipMemcpyAsync(c->dtmp, pin_up(c), 16 * sizeof(double) + 64 * sizeof(float), hipMemcpyHostToDevice,
+                         c->stream));
     LW_CK(lw::launch_reg_begin(c, (const double *)c->dtmp));
-    lw::IcpPass q{};
-    q.n = n;
-    q.inv_v = (float)(1.0 / voxel);
-    q.voxel = (float)voxel;
-    q.gn = 1;
-    q.sp.damping = p->damping;
-    q.sp.prior_info = p->prior_info;
-    q.sp.degen_thr = p->degen_thr;
-    q.sp.min_corr = p->min_corr;
-    for (int it = 0; it < p->max_iter; ++it) {  // odom.py:764-796
-        q.it = it;
-        q.reach = it < p->wide_iters ? 2 : 1;
-        const double md = (q.reach + 0.5) * voxel;
-        q.max_d2 = (float)(md * md);
-        double kap = 6.0 * std::pow(0.7, it);
-        q.kappa = (float)(kap > 2.0 ? kap : 2.0);
-        LW_CK(lw::launch_icp_pass(c, q, p->acc_mode));
+    hipError_t e = hipErrorNotSupported;
+    if (c->loop_ok) {
+        e = lw::launch_register_loop(c, n_dev, n_bound, L, p->acc_mode);
+        if (e != hipSuccess) {
+            c->loop_ok = false;     // cooperative launches unavailable here: one launch per pass from now on
+            (void)hipGetLastError();
+        }
+    }
+    int32_t n = n_known;
+    if (e != hipSuccess) {
+        if (n < 0) {
+            int rc = read_ctr(c, 5, &n);
+            if (rc) return rc;
+        }
+        if (n < p->min_corr) {
+            res->n_ds = n;
+            return 0;
+        }
+        lw::IcpPass q{};
+        q.n = n;
+        q.inv_v = L.inv_v;
+        q.voxel = L.voxel;
+        q.gn = 1;
+        q.sp = L.sp;
+        for (int it = 0; it < p->max_iter; ++it) {  // odom.py:764-796
+            q.it = it;
+            q.reach = it < p->wide_iters ? 2 : 1;
+            q.max_d2 = q.reach == 2 ? L.max_d2_r2 : L.max_d2_r1;
+            q.kappa = L.kappa[it < 64 ? it : 63];
+            LW_CK(lw::launch_icp_pass(c, q, p->acc_mode));
+        }
     }
     LW_CK(hipMemcpyAsync(pin_down(c), c->reg, sizeof(lw::RegState), hipMemcpyDeviceToHost, c->stream));
     LW_CK(hipStreamSynchronize(c->stream));
     LW_CK(prof_harvest(c));
     lw::RegState st;
     std::memcpy(&st, pin_down(c), sizeof(st));
+    if (st.status < 0) return LW_ELOOP;
+    if (n < 0) n = st.n;
+    res->n_ds = n;
+    if (n < p->min_corr) return 0;
     // odom.py:797-800, on the host copy of the last accepted H (a 6x6 eigen problem: microseconds here)
     st.cond = st.corr >= p->min_corr ? lw::gn::cond_of(st.H) : 1.0;
     std::memcpy(res->T, st.T, 16 * sizeof(double));
@@ -395,13 +439,31 @@ int lw_register_thinned(lw_ctx *c, int32_t n, const lw_reg_params *p, const doub
     return 0;
 }
 
+bool bad_params(const lw_reg_params *p) {
+    return !p || p->max_iter < 0 || p->acc_mode < 0 || p->acc_mode > 2 || !(p->reg_voxel > 0.0f);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lw_register_thinned(lw_ctx *c, int32_t n, const lw_reg_params *p, const double *T_pred, lw_reg_result *res) {
+    if (!c || bad_params(p) || !T_pred || !res || n < 0 || n > c->frame_cap) return LW_EINVAL;
+    LW_CK(lw::launch_fill_i32(c, &c->ctr[8], 1, n));
+    return run_registration(c, &c->ctr[8], n, n, p, T_pred, res);
+}
+
+int lw_set_persistent(lw_ctx *c, int32_t on) {
+    if (!c) return LW_EINVAL;
+    c->loop_ok = on != 0;
+    return 0;
+}
+
 int lw_register(lw_ctx *c, int32_t m, const lw_reg_params *p, const double *T_pred, lw_reg_result *res) {
-    if (!c || !p || !T_pred || !res || m < 0 || m > c->frame_cap) return LW_EINVAL;
+    if (!c || bad_params(p) || !T_pred || !res || m < 0 || m > c->frame_cap) return LW_EINVAL;
     if (!c->f_raw) return LW_ENOTBOUND;
-    int32_t n_ds = 0;
-    int rc = lw_thin(c, m, (float)(2.0 / (double)p->reg_voxel), p->min_range, p->max_range, p->ret_mask, &n_ds);
-    if (rc) return rc;
-    return lw_register_thinned(c, n_ds, p, T_pred, res);
+    LW_CK(lw::launch_thin(c, m, (float)(2.0 / (double)p->reg_voxel), p->min_range, p->max_range, p->ret_mask));
+    return run_registration(c, &c->ctr[5], m, -1, p, T_pred, res);  // the thinned count stays on the device
 }
 
 int lw_prof_enable(lw_ctx *c, int32_t on) {

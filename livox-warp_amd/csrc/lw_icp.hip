@@ -1,20 +1,25 @@
 // Scan-to-map accumulation and the Gauss-Newton loop on the device: k_icp + k_reduce
 // (python/livox_warp/odom.py:275-384) and the host step of Odometry._register (odom.py:752-803).
 //
-// One pass = one launch of one-wave workgroups. Each thinned point gets S lanes (S = 2 for the 27-voxel
-// search, 4 for the 125-voxel one); the lanes split the point's neighbours, in the reference's order
-// (dx outer, dz inner; odom.py:315-332), into contiguous parts. A lane searches its part in chunks: the
-// bitmap words of a whole chunk are loaded before any is tested, then the chunk's slots (32 B records that
-// carry key, voxel id, count and mean), so a chunk costs two dependent memory round trips and a lane one
-// or two chunks. The lanes then merge their closest means by (distance, position in the order), which is
-// exactly the serial loop's strict-< choice. The first lane of the point forms the 53 float32 terms of
-// the ACC layout as the reference rounds them (odom.py:246-259, 269-271, 372-375).
+// One pass: each thinned point gets S lanes (S = 2 for the 27-voxel search, 4 for the 125-voxel one); the
+// lanes split the point's neighbours, in the reference's order (dx outer, dz inner; odom.py:315-332), into
+// contiguous parts. A lane searches its part in chunks: the bitmap words of a whole chunk are loaded before
+// any is tested, then the chunk's slots (32 B records that carry key, voxel id, count and mean), so a chunk
+// costs two dependent memory round trips and a lane one or two chunks. The lanes then merge their closest
+// means by (distance, position in the order), which is exactly the serial loop's strict-< choice. The
+// first lane of the point forms the 53 float32 terms of the ACC layout as the reference rounds them
+// (odom.py:246-259, 269-271, 372-375).
 //
 // Reduction: the wave's terms are transposed through LDS and summed per entry in lane order; each
-// workgroup writes one partial row; the last workgroup to finish (atomic ticket, no waiting) sums the rows
-// in workgroup order and, in a registration, runs the float64 step (lw_gn.h) in one lane and updates the
-// pose in device memory. Passes after convergence return at once, so a registration is queued in full
-// with one readback at the end.
+// workgroup writes one partial row; the last workgroup to finish (atomic ticket) sums the rows in
+// workgroup order and, in a registration, runs the float64 step (lw_gn.h) in one lane.
+//
+// Two drivers:
+//   k_icp       one pass per launch (lw_icp_accum, and the registration fallback);
+//   k_register  the whole loop in one cooperative launch: after each pass the last workgroup updates the
+//               pose and bumps a generation counter, the others wait for it (bounded spin; co-residency
+//               is what the cooperative launch guarantees), so a registration is one launch with no idle
+//               passes after convergence and reads the thinned count on the device.
 //
 // Accumulation forms (acc_mode): 0 float64 (the reference), 1 compensated float32 (Neumaier sums, two-sum
 // merges), 2 naive float32.
@@ -68,17 +73,56 @@ struct Acc<2> {  // naive float32
 constexpr int RED_STRIDE = ACC_USED + 1;  // LDS row stride (odd, so column reads do not share banks)
 constexpr int RED_BYTES = ICP_BLOCK * RED_STRIDE * 8;
 
-// Sum the wave's per-lane accumulators per entry, in lane order, through LDS. Lane k < 53 returns entry k
-// (sum, and compensation for the compensated form).
+// Each lane owns one LDS row of 53 running sums (and compensations for the compensated form): a point's
+// terms are added to its first lane's row as soon as they are formed, so no accumulator stays in registers
+// across the neighbour search.
+template <int MODE>
+__device__ inline void row_zero(unsigned char *lds) {
+    const int lane = threadIdx.x;
+    if constexpr (MODE == 0) {
+        double *red = reinterpret_cast<double *>(lds);
+        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) red[lane * RED_STRIDE + k] = 0.0;
+    } else {
+        float *rs = reinterpret_cast<float *>(lds);
+        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) rs[lane * RED_STRIDE + k] = 0.0f;
+        if constexpr (MODE == 1) {
+            float *rc = rs + ICP_BLOCK * RED_STRIDE;
+            LW_UNROLL for (int k = 0; k < ACC_USED; ++k) rc[lane * RED_STRIDE + k] = 0.0f;
+        }
+    }
+}
+
+template <int MODE>
+__device__ inline void row_add(unsigned char *lds, const Acc<MODE> &acc) {
+    const int lane = threadIdx.x;
+    if constexpr (MODE == 0) {
+        double *red = reinterpret_cast<double *>(lds);
+        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) red[lane * RED_STRIDE + k] += acc.s[k];
+    } else if constexpr (MODE == 2) {
+        float *rs = reinterpret_cast<float *>(lds);
+        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) rs[lane * RED_STRIDE + k] += acc.s[k];
+    } else {
+        float *rs = reinterpret_cast<float *>(lds);
+        float *rc = rs + ICP_BLOCK * RED_STRIDE;
+        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) {
+            float t, e;
+            two_sum(rs[lane * RED_STRIDE + k], acc.s[k], t, e);
+            rs[lane * RED_STRIDE + k] = t;
+            rc[lane * RED_STRIDE + k] += acc.c[k] + e;
+        }
+    }
+}
+
+// Sum the wave's rows per entry, in lane order. Lane k < 53 returns entry k (sum, and compensation for the
+// compensated form).
 template <int MODE, int S>
-__device__ inline void wave_sum(const Acc<MODE> &acc, unsigned char *lds, double &sum, double &comp) {
+__device__ inline void wave_sum(unsigned char *lds, double &sum, double &comp) {
     const int lane = threadIdx.x;
     sum = 0.0;
     comp = 0.0;
+    __syncthreads();
     if constexpr (MODE == 0) {
         double *red = reinterpret_cast<double *>(lds);
-        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) red[lane * RED_STRIDE + k] = acc.s[k];
-        __syncthreads();
         if (lane < ACC_USED) {
             double t = 0.0;
             for (int r = 0; r < ICP_BLOCK; r += S) t += red[r * RED_STRIDE + lane];
@@ -86,8 +130,6 @@ __device__ inline void wave_sum(const Acc<MODE> &acc, unsigned char *lds, double
         }
     } else if constexpr (MODE == 2) {
         float *red = reinterpret_cast<float *>(lds);
-        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) red[lane * RED_STRIDE + k] = acc.s[k];
-        __syncthreads();
         if (lane < ACC_USED) {
             float t = 0.0f;
             for (int r = 0; r < ICP_BLOCK; r += S) t += red[r * RED_STRIDE + lane];
@@ -96,11 +138,6 @@ __device__ inline void wave_sum(const Acc<MODE> &acc, unsigned char *lds, double
     } else {
         float *rs = reinterpret_cast<float *>(lds);
         float *rc = rs + ICP_BLOCK * RED_STRIDE;
-        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) {
-            rs[lane * RED_STRIDE + k] = acc.s[k];
-            rc[lane * RED_STRIDE + k] = acc.c[k];
-        }
-        __syncthreads();
         if (lane < ACC_USED) {
             float s = 0.0f, c = 0.0f;
             for (int r = 0; r < ICP_BLOCK; r += S) {
@@ -113,6 +150,7 @@ __device__ inline void wave_sum(const Acc<MODE> &acc, unsigned char *lds, double
             comp = (double)c;
         }
     }
+    __syncthreads();  // the LDS rows may be rewritten by the next pass
 }
 
 struct IcpArgs {
@@ -130,6 +168,13 @@ struct IcpArgs {
     int32_t it;
     int32_t gn;
     gn::StepParams sp;
+};
+
+// What changes from pass to pass (kept out of IcpArgs so the persistent loop never copies the arguments).
+struct Pass {
+    int32_t n;
+    float max_d2, kappa;
+    int32_t it;
 };
 
 #define LW_H_PAIRS(X) \
@@ -232,8 +277,8 @@ __device__ inline void search_part(const IcpArgs &a, float wx, float wy, float w
 
 // The reference's terms of one matched point (odom.py:335-375) into acc.
 template <int MODE>
-__device__ inline void point_terms(const IcpArgs &a, float px, float py, float pz, float wx, float wy, float wz,
-                                   const Cand &b, Acc<MODE> &acc) {
+__device__ inline void point_terms(const IcpArgs &a, float kappa, float px, float py, float pz, float wx, float wy,
+                                   float wz, const Cand &b, Acc<MODE> &acc) {
     const int32_t vid = a.slots[b.s].vid;
     const float r0 = wx - b.mx, r1 = wy - b.my, r2 = wz - b.mz;
     const VoxPrec P = a.vox_prec[vid];
@@ -256,7 +301,7 @@ __device__ inline void point_terms(const IcpArgs &a, float px, float py, float p
     float d2 = r0 * Mr[0];
     d2 = d2 + r1 * Mr[1];
     d2 = d2 + r2 * Mr[2];
-    float wgt = 1.0f / (1.0f + d2 / (a.kappa * a.kappa));
+    float wgt = 1.0f / (1.0f + d2 / (kappa * kappa));
     wgt = wgt * wgt;  // Geman-McClure (odom.py:370-371)
     acc.add(ACC_RMS, wgt * rn * rn);
 #define LW_ADD_H(p, q, k) acc.add(ACC_H + k, wgt * hval<p, q>(M, wx, wy, wz));
@@ -277,8 +322,94 @@ __device__ inline void point_terms(const IcpArgs &a, float px, float py, float p
     acc.add(ACC_RANGE, length3(px, py, pz));
 }
 
-// The float64 step after a pass (odom.py:769-796), in one lane.
-__device__ void gauss_newton(RegState *st, const double *v, const gn::StepParams &sp, int32_t it) {
+// One workgroup's share of a pass: the points of lane groups blockIdx.x, blockIdx.x + G, ... (one group
+// per workgroup whenever the grid is large enough). Lane k < 53 returns the workgroup's entry k.
+template <int MODE, int R>
+__device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf, int G, unsigned char *lds, double &sum,
+                                 double &comp, int32_t &hits, int32_t &matched) {
+    constexpr int S = Split<R>::S, C = Split<R>::C;
+    constexpr int D = 2 * R + 1, K = D * D * D, KQ = (K + S - 1) / S;
+    const int lane = threadIdx.x;
+    const int sub = lane % S;
+    row_zero<MODE>(lds);
+    for (int64_t base = (int64_t)blockIdx.x * ICP_BLOCK; base < (int64_t)p.n * S; base += (int64_t)G * ICP_BLOCK) {
+        const int32_t i = (int32_t)((base + lane) / S);
+        float px = 0.0f, py = 0.0f, pz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
+        Cand best{p.max_d2, -1, K, 0.0f, 0.0f, 0.0f};
+        if (i < p.n) {
+            px = a.ds[3 * (int64_t)i];
+            py = a.ds[3 * (int64_t)i + 1];
+            pz = a.ds[3 * (int64_t)i + 2];
+            transform_point(Tf, px, py, pz, wx, wy, wz);
+            search_part<R, C>(a, wx, wy, wz, sub * KQ, min(K, (sub + 1) * KQ), best, hits);
+        }
+        // merge the S parts: smaller distance wins, a tie goes to the earlier neighbour (the serial strict <)
+        LW_UNROLL for (int off = 1; off < S; off <<= 1) {
+            Cand o;
+            o.d2 = __shfl_xor(best.d2, off, 64);
+            o.s = __shfl_xor(best.s, off, 64);
+            o.j = __shfl_xor(best.j, off, 64);
+            o.mx = __shfl_xor(best.mx, off, 64);
+            o.my = __shfl_xor(best.my, off, 64);
+            o.mz = __shfl_xor(best.mz, off, 64);
+            if (o.s >= 0 && (best.s < 0 || o.d2 < best.d2 || (o.d2 == best.d2 && o.j < best.j))) best = o;
+        }
+        if (i < p.n && sub == 0) {
+            if (a.best_slot) a.best_slot[i] = best.s;
+            if (best.s >= 0) {
+                ++matched;
+                Acc<MODE> acc;
+                acc.zero();
+                point_terms<MODE>(a, p.kappa, px, py, pz, wx, wy, wz, best, acc);
+                row_add<MODE>(lds, acc);
+            }
+        }
+    }
+    wave_sum<MODE, S>(lds, sum, comp);
+}
+
+// Sum the G partial rows in workgroup order (the last workgroup, after its acquire fence). Lane k < 53
+// returns entry k of the reduced accumulator.
+template <int MODE>
+__device__ inline double reduce_rows(const double *partials, int G) {
+    const int lane = threadIdx.x;
+    if (lane >= ACC_USED) return 0.0;
+    const double *col = partials + lane;
+    if constexpr (MODE == 0) {
+        double t = 0.0;
+        int b = 0;
+        for (; b + 8 <= G; b += 8) {
+            double x[8];
+            LW_UNROLL for (int u = 0; u < 8; ++u) x[u] = col[(size_t)(b + u) * 128];
+            LW_UNROLL for (int u = 0; u < 8; ++u) t += x[u];
+        }
+        for (; b < G; ++b) t += col[(size_t)b * 128];
+        return t;
+    } else if constexpr (MODE == 2) {
+        float t = 0.0f;
+        int b = 0;
+        for (; b + 8 <= G; b += 8) {
+            double x[8];
+            LW_UNROLL for (int u = 0; u < 8; ++u) x[u] = col[(size_t)(b + u) * 128];
+            LW_UNROLL for (int u = 0; u < 8; ++u) t += (float)x[u];
+        }
+        for (; b < G; ++b) t += (float)col[(size_t)b * 128];
+        return (double)t;
+    } else {
+        float s = 0.0f, c = 0.0f;
+        for (int b = 0; b < G; ++b) {
+            float t, e;
+            two_sum(s, (float)col[(size_t)b * 128], t, e);
+            s = t;
+            c = c + (float)col[(size_t)b * 128 + 64] + e;
+        }
+        return (double)s + (double)c;
+    }
+}
+
+// The float64 step after a pass (odom.py:769-796), in one lane. Not inlined: its 6x6 temporaries get their
+// own register budget instead of adding to the accumulation's.
+__device__ __noinline__ void gauss_newton(RegState *st, const double *v, const gn::StepParams &sp, int32_t it) {
     gn::StepOut o;
     int rc = gn::step(v, st->T, st->T_pred, sp, o);
     st->corr = o.corr;
@@ -306,54 +437,12 @@ __device__ void gauss_newton(RegState *st, const double *v, const gn::StepParams
     }
 }
 
-template <int MODE, int R>
-__global__ __launch_bounds__(ICP_BLOCK) void k_icp(IcpArgs a) {
-    constexpr int S = Split<R>::S, C = Split<R>::C;
-    constexpr int D = 2 * R + 1, K = D * D * D, KQ = (K + S - 1) / S;
-    RegState *st = a.st;
-    if (st->done) return;
-    __shared__ float Tf[16];
-    __shared__ int32_t s_last;
-    __shared__ __align__(16) unsigned char lds[RED_BYTES];
+// Write this workgroup's row and the measurement counters, then take a ticket; true for the last
+// workgroup of the pass (which then sees every row).
+template <int MODE>
+__device__ inline bool row_and_ticket(const IcpArgs &a, double sum, double comp, int32_t hits, int32_t matched,
+                                      int32_t *s_last) {
     const int lane = threadIdx.x;
-    if (lane < 16) Tf[lane] = st->Tf[lane];
-    __syncthreads();
-    const int sub = lane % S;
-    const int32_t i = (blockIdx.x * ICP_BLOCK + lane) / S;
-    int32_t hits = 0, matched = 0;
-    double sum, comp;
-    {
-        float px = 0.0f, py = 0.0f, pz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
-        Cand best{a.max_d2, -1, K, 0.0f, 0.0f, 0.0f};
-        if (i < a.n) {
-            px = a.ds[3 * (int64_t)i];
-            py = a.ds[3 * (int64_t)i + 1];
-            pz = a.ds[3 * (int64_t)i + 2];
-            transform_point(Tf, px, py, pz, wx, wy, wz);
-            search_part<R, C>(a, wx, wy, wz, sub * KQ, min(K, (sub + 1) * KQ), best, hits);
-        }
-        // merge the S parts: smaller distance wins, a tie goes to the earlier neighbour (the serial strict <)
-        LW_UNROLL for (int off = 1; off < S; off <<= 1) {
-            Cand o;
-            o.d2 = __shfl_xor(best.d2, off, 64);
-            o.s = __shfl_xor(best.s, off, 64);
-            o.j = __shfl_xor(best.j, off, 64);
-            o.mx = __shfl_xor(best.mx, off, 64);
-            o.my = __shfl_xor(best.my, off, 64);
-            o.mz = __shfl_xor(best.mz, off, 64);
-            if (o.s >= 0 && (best.s < 0 || o.d2 < best.d2 || (o.d2 == best.d2 && o.j < best.j))) best = o;
-        }
-        Acc<MODE> acc;
-        acc.zero();
-        if (i < a.n && sub == 0) {
-            if (a.best_slot) a.best_slot[i] = best.s;
-            if (best.s >= 0) {
-                ++matched;
-                point_terms<MODE>(a, px, py, pz, wx, wy, wz, best, acc);
-            }
-        }
-        wave_sum<MODE, S>(acc, lds, sum, comp);
-    }
     double *row = a.partials + (size_t)blockIdx.x * 128;
     if (lane < ACC_USED) {
         row[lane] = sum;
@@ -370,61 +459,112 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_icp(IcpArgs a) {
         }
     }
     __threadfence();
-    if (lane == 0) s_last = (atomicAdd(&st->ticket, 1u) == gridDim.x - 1);
+    if (lane == 0) *s_last = (atomicAdd(&a.st->ticket, 1u) == gridDim.x - 1);
     __syncthreads();
-    if (!s_last) return;
-    __threadfence();  // acquire: the other workgroups' rows are visible from here on
-    const int G = (int)gridDim.x;
-    double v = 0.0;
-    if (lane < ACC_USED) {
-        const double *col = a.partials + lane;
-        if constexpr (MODE == 0) {
-            double t = 0.0;
-            int b = 0;
-            for (; b + 8 <= G; b += 8) {
-                double x[8];
-                LW_UNROLL for (int u = 0; u < 8; ++u) x[u] = col[(size_t)(b + u) * 128];
-                LW_UNROLL for (int u = 0; u < 8; ++u) t += x[u];
-            }
-            for (; b < G; ++b) t += col[(size_t)b * 128];
-            v = t;
-        } else if constexpr (MODE == 2) {
-            float t = 0.0f;
-            int b = 0;
-            for (; b + 8 <= G; b += 8) {
-                double x[8];
-                LW_UNROLL for (int u = 0; u < 8; ++u) x[u] = col[(size_t)(b + u) * 128];
-                LW_UNROLL for (int u = 0; u < 8; ++u) t += (float)x[u];
-            }
-            for (; b < G; ++b) t += (float)col[(size_t)b * 128];
-            v = (double)t;
-        } else {
-            float s = 0.0f, c = 0.0f;
-            for (int b = 0; b < G; ++b) {
-                float t, e;
-                two_sum(s, (float)col[(size_t)b * 128], t, e);
-                s = t;
-                c = c + (float)col[(size_t)b * 128 + 64] + e;
-            }
-            v = (double)s + (double)c;
-        }
-    }
-    __shared__ double vs[64];
+    const bool last = *s_last != 0;
+    __syncthreads();
+    if (last) __threadfence();  // acquire: the other workgroups' rows are visible from here on
+    return last;
+}
+
+template <int MODE, int R>
+__device__ inline void finish_pass(const IcpArgs &a, const Pass p, double *vs) {
+    constexpr int D = 2 * R + 1, K = D * D * D;
+    const int lane = threadIdx.x;
+    const double v = reduce_rows<MODE>(a.partials, (int)gridDim.x);
     vs[lane] = v;
-    st->acc[lane] = v;
+    a.st->acc[lane] = v;
     __syncthreads();
     if (lane == 0) {
-        st->ticket = 0;
+        a.st->ticket = 0;
         if (a.prof) {
-            atomicAdd(&a.prof[0], (unsigned long long)a.n);
-            atomicAdd(&a.prof[1], (unsigned long long)a.n * (unsigned long long)K);
+            atomicAdd(&a.prof[0], (unsigned long long)p.n);
+            atomicAdd(&a.prof[1], (unsigned long long)p.n * (unsigned long long)K);
             atomicAdd(&a.prof[4], 1ull);
         }
-        if (a.gn) gauss_newton(st, vs, a.sp, a.it);
+        if (a.gn) gauss_newton(a.st, vs, a.sp, p.it);
     }
 }
 
+template <int MODE, int R>
+__global__ __launch_bounds__(ICP_BLOCK) void k_icp(IcpArgs a) {
+    RegState *st = a.st;
+    if (st->done) return;
+    __shared__ float Tf[16];
+    __shared__ int32_t s_last;
+    __shared__ double vs[64];
+    __shared__ __align__(16) unsigned char lds[RED_BYTES];
+    if (threadIdx.x < 16) Tf[threadIdx.x] = st->Tf[threadIdx.x];
+    __syncthreads();
+    double sum, comp;
+    int32_t hits = 0, matched = 0;
+    const Pass p{a.n, a.max_d2, a.kappa, a.it};
+    wave_pass<MODE, R>(a, p, Tf, (int)gridDim.x, lds, sum, comp, hits, matched);
+    if (!row_and_ticket<MODE>(a, sum, comp, hits, matched, &s_last)) return;
+    finish_pass<MODE, R>(a, p, vs);
+}
+
+// ---- the persistent registration ------------------------------------------------------------------
+
+struct LoopArgs {
+    IcpArgs a;                // pointers and the fixed scalars; n, max_d2, kappa, it are set per pass
+    const int32_t *n_dev;     // thinned count on the device
+    int32_t max_iter, wide_iters, min_corr;
+    float max_d2_r1, max_d2_r2;
+};
+
+constexpr uint32_t SPIN_LIMIT = 1u << 24;  // ~ seconds of polling: a wait that long means the loop is broken
+constexpr int32_t STATUS_SPIN_TIMEOUT = -1;
+
+template <int MODE>
+__global__ __launch_bounds__(ICP_BLOCK) void k_register(LoopArgs L) {
+    RegState *st = L.a.st;
+    const int lane = threadIdx.x;
+    __shared__ float Tf[16];
+    __shared__ int32_t s_last;
+    __shared__ double vs[64];
+    __shared__ __align__(16) unsigned char lds[RED_BYTES];
+    const int32_t n = *L.n_dev;
+    if (blockIdx.x == 0 && lane == 0) st->n = n;
+    if (n < L.min_corr) return;  // odom.py:756-757: the host sees n and keeps the prediction
+    const IcpArgs &a = L.a;
+    for (int it = 0; it < L.max_iter; ++it) {
+        if (lane < 16) Tf[lane] = __hip_atomic_load(&st->Tf[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int reach = it < L.wide_iters ? 2 : 1;
+        const Pass p{n, reach == 2 ? L.max_d2_r2 : L.max_d2_r1, st->kappa[it], it};
+        double sum, comp;
+        int32_t hits = 0, matched = 0;
+        if (reach == 2)
+            wave_pass<MODE, 2>(a, p, Tf, (int)gridDim.x, lds, sum, comp, hits, matched);
+        else
+            wave_pass<MODE, 1>(a, p, Tf, (int)gridDim.x, lds, sum, comp, hits, matched);
+        if (row_and_ticket<MODE>(a, sum, comp, hits, matched, &s_last)) {
+            if (reach == 2)
+                finish_pass<MODE, 2>(a, p, vs);
+            else
+                finish_pass<MODE, 1>(a, p, vs);
+            __threadfence();  // release: pose, done flag and ticket before the generation
+            if (lane == 0) __hip_atomic_store(&st->gen, (uint32_t)(it + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            uint32_t spins = 0;
+            while (__hip_atomic_load(&st->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < (uint32_t)(it + 1)) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > SPIN_LIMIT) {
+                    if (lane == 0) st->status = STATUS_SPIN_TIMEOUT;
+                    return;
+                }
+            }
+            __threadfence();  // acquire
+        }
+        __syncthreads();
+        if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    }
+}
+
+// Tp: 16 doubles (the prediction) followed by 64 floats (the per-iteration robust-kernel scale)
 __global__ void k_reg_begin(RegState *st, const double *Tp) {
+    if (threadIdx.x < 64) st->kappa[threadIdx.x] = reinterpret_cast<const float *>(Tp + 16)[threadIdx.x];
     if (threadIdx.x != 0) return;
     for (int k = 0; k < 16; ++k) {
         st->T[k] = Tp[k];
@@ -440,6 +580,8 @@ __global__ void k_reg_begin(RegState *st, const double *Tp) {
     st->n_degen = 0;
     st->status = 0;
     st->ticket = 0;
+    st->gen = 0;
+    st->n = 0;
 }
 
 template <int MODE>
@@ -456,6 +598,25 @@ hipError_t launch_mode(lw_ctx *c, const IcpArgs &a, int reach) {
     return hipGetLastError();
 }
 
+IcpArgs base_args(lw_ctx *c) {
+    IcpArgs a{};
+    a.ds = c->ds_xyz;
+    a.slots = c->slots;
+    a.occ = c->occ;
+    a.vox_prec = c->vox_prec;
+    a.st = c->reg;
+    a.partials = c->partials;
+    a.best_slot = nullptr;
+    a.prof = c->prof_on ? c->prof_dev : nullptr;
+    a.mask = c->mask;
+    return a;
+}
+
+template <int MODE>
+const void *loop_kernel() {
+    return reinterpret_cast<const void *>(&k_register<MODE>);
+}
+
 }  // namespace
 
 hipError_t launch_reg_begin(lw_ctx *c, const double *T_pred_dev) {
@@ -467,17 +628,9 @@ int32_t icp_max_groups(int64_t frame_cap) { return (int32_t)((frame_cap * 4 + IC
 
 hipError_t launch_icp_pass(lw_ctx *c, const IcpPass &p, int acc_mode) {
     if (p.n > c->frame_cap) return hipErrorInvalidValue;
-    IcpArgs a;
-    a.ds = c->ds_xyz;
-    a.slots = c->slots;
-    a.occ = c->occ;
-    a.vox_prec = c->vox_prec;
-    a.st = c->reg;
-    a.partials = c->partials;
+    IcpArgs a = base_args(c);
     a.best_slot = p.want_slots ? c->best_slot : nullptr;
-    a.prof = c->prof_on ? c->prof_dev : nullptr;
     a.n = p.n;
-    a.mask = c->mask;
     a.inv_v = p.inv_v;
     a.voxel = p.voxel;
     a.max_d2 = p.max_d2;
@@ -497,6 +650,51 @@ hipError_t launch_icp_pass(lw_ctx *c, const IcpPass &p, int acc_mode) {
     case 2: e = launch_mode<2>(c, a, p.reach); break;
     default: return hipErrorInvalidValue;
     }
+    if (e != hipSuccess) return e;
+    if (timed) {
+        e = hipEventRecord(c->prof_ev[2 * c->prof_used + 1], c->stream);
+        c->prof_used++;
+    }
+    return e;
+}
+
+hipError_t launch_register_loop(lw_ctx *c, const int32_t *n_dev, int32_t n_bound, const RegLoop &R, int acc_mode) {
+    if (acc_mode < 0 || acc_mode > 2 || R.max_iter > 64 || n_bound > c->frame_cap) return hipErrorNotSupported;
+    const void *fn = acc_mode == 0 ? loop_kernel<0>() : acc_mode == 1 ? loop_kernel<1>() : loop_kernel<2>();
+    if (c->loop_max_groups[acc_mode] == 0) {  // co-resident workgroups, once per context
+        int per_cu = 0, cus = 0, coop = 0;
+        if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c->device) != hipSuccess || !coop)
+            return hipErrorNotSupported;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, ICP_BLOCK, 0) != hipSuccess || per_cu <= 0)
+            return hipErrorNotSupported;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
+            return hipErrorNotSupported;
+        c->loop_max_groups[acc_mode] = per_cu * cus;
+    }
+    LoopArgs L;
+    L.a = base_args(c);
+    L.a.inv_v = R.inv_v;
+    L.a.voxel = R.voxel;
+    L.a.gn = 1;
+    L.a.sp = R.sp;
+    L.n_dev = n_dev;
+    L.max_iter = R.max_iter;
+    L.wide_iters = R.wide_iters;
+    L.min_corr = R.min_corr;
+    L.max_d2_r1 = R.max_d2_r1;
+    L.max_d2_r2 = R.max_d2_r2;
+    // one lane group per workgroup for the widest pass, up to what can be co-resident
+    int64_t g = ((int64_t)(n_bound > 0 ? n_bound : 1) * Split<2>::S + ICP_BLOCK - 1) / ICP_BLOCK;
+    if (g > c->loop_max_groups[acc_mode]) g = c->loop_max_groups[acc_mode];
+    if (g > icp_max_groups(c->frame_cap)) g = icp_max_groups(c->frame_cap);
+    if (g < 1) g = 1;
+    void *args[] = {&L};
+    const bool timed = c->prof_on && c->prof_used < c->prof_cap;
+    if (timed) {
+        hipError_t e = hipEventRecord(c->prof_ev[2 * c->prof_used], c->stream);
+        if (e != hipSuccess) return e;
+    }
+    hipError_t e = hipLaunchCooperativeKernel(fn, dim3((unsigned)g), dim3(ICP_BLOCK), args, 0, c->stream);
     if (e != hipSuccess) return e;
     if (timed) {
         e = hipEventRecord(c->prof_ev[2 * c->prof_used + 1], c->stream);

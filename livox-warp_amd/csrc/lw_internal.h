@@ -29,7 +29,9 @@ struct RegState {
     int32_t n_degen;
     int32_t status;     // last step status (gn::STEP_*)
     uint32_t ticket;    // workgroups finished in the current pass
-    int32_t pad[2];
+    uint32_t gen;       // passes completed (the persistent registration's release counter)
+    int32_t n;          // thinned points the registration ran on
+    float kappa[64];    // robust-kernel scale per iteration (odom.py:767)
 };
 
 }  // namespace lw
@@ -91,6 +93,10 @@ struct lw_ctx {
     double prof_ms = 0.0;
     int64_t prof_launches = 0;
     unsigned long long *prof_dev = nullptr;
+    // the persistent registration: co-resident workgroups per accumulation form (0 = not yet queried), and
+    // whether cooperative launches work on this device
+    int32_t loop_max_groups[3] = {0, 0, 0};
+    bool loop_ok = true;
     // parameters the cached per-voxel precision was computed with
     bool prec_valid = false;
     float prec_voxel = 0, prec_noise2 = 0, prec_tight2 = 0, prec_alpha = 0;
@@ -124,6 +130,17 @@ struct IcpPass {
     int32_t want_slots;
 };
 hipError_t launch_icp_pass(lw_ctx *c, const IcpPass &p, int acc_mode);
+
+// The whole Gauss-Newton loop in one cooperative launch (see lw_icp.hip); n_dev: device count of thinned
+// points, n_bound: an upper bound known on the host (sizes the grid). Returns hipErrorNotSupported when the
+// device cannot run it (the caller then queues one launch per pass).
+struct RegLoop {
+    int32_t max_iter, wide_iters, min_corr;
+    float inv_v, voxel, max_d2_r1, max_d2_r2;
+    float kappa[64];
+    gn::StepParams sp;
+};
+hipError_t launch_register_loop(lw_ctx *c, const int32_t *n_dev, int32_t n_bound, const RegLoop &L, int acc_mode);
 int32_t icp_max_groups(int64_t frame_cap);  // partial rows one pass over frame_cap points can write
 hipError_t launch_reg_begin(lw_ctx *c, const double *T_pred_dev);
 

@@ -32,6 +32,25 @@ def test_single_scan_registration(lw, orc, seed, prior, degen, gpu_ok):
     assert abs(rg.rms - ro.rms) <= 1e-6 + 1e-4 * ro.rms
 
 
+def test_persistent_loop_matches_per_pass(lw, orc, gpu_ok):
+    w = lw.synth.single_scan(seed=18)
+    voxel = 0.25
+    eng = lw.Engine(1 << 21, 1 << 18)
+    eng.insert(w.map_xyz, None, np.eye(4), voxel, max_range=1e30)
+    ds = orc.ds_insert(w.scan_xyz, w.scan_attr, 2.0 / voxel, 0.3, 60.0, 15, 1 << 15)
+    eng.thin_load(ds)
+    out = {}
+    for on in (1, 0):
+        lw._abi.check(eng.lib.lw_set_persistent(eng.ctx, on), "lw_set_persistent")
+        for mode in ("fp64", "kahan32"):
+            p = eng.params(reg_voxel=voxel, max_iter=20, acc_mode=mode)
+            out[(on, mode)] = eng.register_thinned(len(ds), p, w.T_init)
+    for mode in ("fp64", "kahan32"):
+        a, b = out[(1, mode)], out[(0, mode)]
+        assert a.iters == b.iters and a.corr == b.corr and a.degen == b.degen and a.ok == b.ok
+        assert np.abs(np.array(a.T) - np.array(b.T)).max() < 1e-9
+
+
 def test_register_from_frame_buffers(lw, orc, gpu_ok):
     w = lw.synth.single_scan(seed=16)
     voxel = 0.25
