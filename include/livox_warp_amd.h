@@ -136,6 +136,31 @@ int lw_register_thinned(lw_ctx *ctx, int32_t n, const lw_reg_params *p, const do
  * to the order of the float64 partial sums. No counterpart in the reference. */
 int lw_set_persistent(lw_ctx *ctx, int32_t on);
 
+/* Batched global localisation parameters. */
+typedef struct lw_loc_params {
+    double damping;    /* 1e-3: fraction of the diagonal added (localize.py:538) */
+    float reg_voxel;   /* the prior map's voxel edge, m */
+    float noise;       /* range noise sigma for the voxel precision, m (odom.py:509) */
+    float alpha;       /* weight of loose directions (odom.py:511) */
+    int32_t iters;     /* Gauss-Newton iterations per hypothesis */
+    int32_t min_count; /* matches below which a step is skipped (30, localize.py:535) */
+    int32_t block;     /* tile: threads per workgroup, 64 / 128 / 256 / 512 */
+    int32_t grid;      /* persistent workgroups (0: four per compute unit) */
+    int32_t use_graph; /* capture the launch in a hipGraph and replay it while the arguments stay the same
+                          (needs a context stream other than the null stream) */
+} lw_loc_params;
+
+/* GlobalLocalizer._icp (localize.py:517-553, kernels :126-179), batched: n_hyp hypotheses for each of n_queries
+ * scans, registered against the context's map in one launch. Correspondence and residual are odom.py's
+ * (closest voxel mean in the 27 neighbours, point-to-distribution; odom.py:275-375), the step localize.py's.
+ * scans: float32 x 3, concatenated; scan_off: n_queries + 1 offsets; order: the queries in processing order
+ * (longest first balances the work); hyp_T / out_T: n_queries * n_hyp row-major 4x4 float64 (query-major);
+ * out_score: sum of robust weights of the last pass; out_count: its matched points; out_iters: steps taken.
+ * Every array is a DEVICE pointer. Queued on the context's stream; it does not wait. */
+int lw_loc_batch(lw_ctx *ctx, const float *scans, const int32_t *scan_off, const int32_t *order, int32_t n_queries,
+                 const double *hyp_T, int32_t n_hyp, const lw_loc_params *p, double *out_T, double *out_score,
+                 int32_t *out_count, int32_t *out_iters);
+
 /* Live measurement of the accumulation launches (for throughput and roofline reports): lw_prof_enable
  * turns on HIP events around each accumulation launch and device counters; lw_prof_read returns, since
  * lw_prof_reset: out[0] launches timed, out[1] their total duration (ms), out[2] point-iterations, out[3]

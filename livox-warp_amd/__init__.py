@@ -9,7 +9,7 @@ Submodules: odom (the drop-in Odometry), engine (one context driven directly), p
 synth (seeded synthetic scenes, scans and walks), _abi (ctypes bindings of liblwamd.so).
 """
 
-from . import _abi, engine, odom, pose_table, synth  # noqa: F401
+from . import _abi, engine, loc, odom, pose_table, synth  # noqa: F401
 from .engine import Engine  # noqa: F401
 from .odom import Odometry  # noqa: F401
 from .pose_table import PoseTable  # noqa: F401

@@ -38,6 +38,11 @@ class RegResult(ctypes.Structure):
                 ("n_ds", _i32), ("degen", _i32), ("weak", _i32)]
 
 
+class LocParams(ctypes.Structure):
+    _fields_ = [("damping", _f64), ("reg_voxel", _f32), ("noise", _f32), ("alpha", _f32), ("iters", _i32),
+                ("min_count", _i32), ("block", _i32), ("grid", _i32), ("use_graph", _i32)]
+
+
 # name -> (restype, argtypes); every symbol include/livox_warp_amd.h declares
 SIGNATURES = {
     "lw_version": (_i32, []),
@@ -62,6 +67,7 @@ SIGNATURES = {
     "lw_register": (_i32, [_vp, _i32, ctypes.POINTER(RegParams), _f64p, ctypes.POINTER(RegResult)]),
     "lw_register_thinned": (_i32, [_vp, _i32, ctypes.POINTER(RegParams), _f64p, ctypes.POINTER(RegResult)]),
     "lw_set_persistent": (_i32, [_vp, _i32]),
+    "lw_loc_batch": (_i32, [_vp, _vp, _vp, _vp, _i32, _vp, _i32, ctypes.POINTER(LocParams), _vp, _vp, _vp, _vp]),
     "lw_prof_enable": (_i32, [_vp, _i32]),
     "lw_prof_reset": (_i32, [_vp]),
     "lw_prof_read": (_i32, [_vp, _f64p]),

@@ -8,6 +8,7 @@
 
 #include "../../include/livox_warp_amd.h"
 #include "lw_internal.h"
+#include "lw_loc.h"
 #include "lw_prec.h"
 
 #define LW_CK(x)                                        \
@@ -50,6 +51,9 @@ void free_all(lw_ctx *c) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->prof_dev) (void)hipFree(c->prof_dev);
+    if (c->loc_queue) (void)hipFree(c->loc_queue);
+    if (c->loc_exec) (void)hipGraphExecDestroy(c->loc_exec);
+    if (c->loc_graph) (void)hipGraphDestroy(c->loc_graph);
     if (c->prof_ev) {
         for (int i = 0; i < 2 * c->prof_cap; ++i) (void)hipEventDestroy(c->prof_ev[i]);
         delete[] c->prof_ev;
@@ -148,6 +152,7 @@ int lw_create(lw_ctx **out, int64_t reg_slots, int64_t frame_cap, void *stream) 
     if ((e = dalloc(&c->partials, (size_t)lw::icp_max_groups(frame_cap) * 128)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->best_slot, frame_cap)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->dtmp, frame_cap + 64)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->loc_queue, 16)) != hipSuccess) return fail(e);
     if ((e = hipHostMalloc(&c->pin, PIN_BYTES, hipHostMallocDefault)) != hipSuccess) return fail(e);
     c->pin_bytes = PIN_BYTES;
     hipLaunchKernelGGL(k_fill_i64, dim3(1024), dim3(256), 0, c->stream, c->ds_keys, c->ds_cap, (int64_t)-1);
@@ -451,6 +456,74 @@ int lw_register_thinned(lw_ctx *c, int32_t n, const lw_reg_params *p, const doub
     if (!c || bad_params(p) || !T_pred || !res || n < 0 || n > c->frame_cap) return LW_EINVAL;
     LW_CK(lw::launch_fill_i32(c, &c->ctr[8], 1, n));
     return run_registration(c, &c->ctr[8], n, n, p, T_pred, res);
+}
+
+int lw_loc_batch(lw_ctx *c, const float *scans, const int32_t *scan_off, const int32_t *order, int32_t n_queries,
+                 const double *hyp_T, int32_t n_hyp, const lw_loc_params *p, double *out_T, double *out_score,
+                 int32_t *out_count, int32_t *out_iters) {
+    if (!c || !p || !scans || !scan_off || !order || !hyp_T || !out_T || !out_score || !out_count || !out_iters ||
+        n_queries <= 0 || n_hyp <= 0 || (int64_t)n_queries * n_hyp > (1ll << 30) || p->iters < 0 || p->iters > 64 ||
+        !(p->reg_voxel > 0.0f) || (p->block != 64 && p->block != 128 && p->block != 256 && p->block != 512))
+        return LW_EINVAL;
+    const double voxel = p->reg_voxel;
+    const float noise2 = (float)((double)p->noise * (double)p->noise);
+    const double tf = 0.6 * voxel / std::sqrt(12.0);
+    LW_CK(ensure_precision(c, (float)voxel, noise2, (float)(tf * tf), p->alpha));
+    lw::LocLaunch L{};
+    L.scans = scans;
+    L.scan_off = scan_off;
+    L.order = order;
+    L.hyp_T = hyp_T;
+    L.out_T = out_T;
+    L.out_score = out_score;
+    L.out_count = out_count;
+    L.out_iters = out_iters;
+    L.Q = n_queries;
+    L.H = n_hyp;
+    L.iters = p->iters;
+    L.min_count = p->min_count;
+    L.block = p->block;
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
+    L.grid = p->grid > 0 ? p->grid : 4 * cus;
+    L.inv_v = (float)(1.0 / voxel);
+    L.max_d2 = (float)((1.5 * voxel) * (1.5 * voxel));  // reach 1 (odom.py:766)
+    L.damping = p->damping;
+    for (int it = 0; it < 16; ++it) {  // the odometry's robust-kernel schedule (odom.py:767)
+        const double kap = 6.0 * std::pow(0.7, it);
+        L.kappa[it] = (float)(kap > 2.0 ? kap : 2.0);
+    }
+    const bool timed = c->prof_on && c->prof_used < c->prof_cap;
+    if (timed) LW_CK(hipEventRecord(c->prof_ev[2 * c->prof_used], c->stream));
+    bool launched = false;
+    if (p->use_graph && c->stream != nullptr) {  // a captured launch needs a stream other than the null stream
+        unsigned char key[512] = {0};
+        static_assert(sizeof(lw::LocLaunch) <= sizeof(key), "key");
+        std::memcpy(key, &L, sizeof(L));
+        if (!c->loc_exec || std::memcmp(key, c->loc_key, sizeof(key)) != 0) {
+            if (c->loc_exec) (void)hipGraphExecDestroy(c->loc_exec);
+            if (c->loc_graph) (void)hipGraphDestroy(c->loc_graph);
+            c->loc_exec = nullptr;
+            c->loc_graph = nullptr;
+            LW_CK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = lw::launch_loc(c, L);
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+            if (e != hipSuccess) return -(int)e;
+            if (e2 != hipSuccess) return -(int)e2;
+            c->loc_graph = g;
+            LW_CK(hipGraphInstantiate(&c->loc_exec, g, nullptr, nullptr, 0));
+            std::memcpy(c->loc_key, key, sizeof(key));
+        }
+        LW_CK(hipGraphLaunch(c->loc_exec, c->stream));
+        launched = true;
+    }
+    if (!launched) LW_CK(lw::launch_loc(c, L));
+    if (timed) {
+        LW_CK(hipEventRecord(c->prof_ev[2 * c->prof_used + 1], c->stream));
+        c->prof_used++;
+    }
+    return 0;
 }
 
 int lw_set_persistent(lw_ctx *c, int32_t on) {

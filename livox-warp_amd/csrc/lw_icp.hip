@@ -441,7 +441,7 @@ __device__ __noinline__ void gauss_newton(RegState *st, const double *v, const g
 // workgroup of the pass (which then sees every row).
 template <int MODE>
 __device__ inline bool row_and_ticket(const IcpArgs &a, double sum, double comp, int32_t hits, int32_t matched,
-                                      int32_t *s_last) {
+                                      int G, int32_t *s_last) {
     const int lane = threadIdx.x;
     double *row = a.partials + (size_t)blockIdx.x * 128;
     if (lane < ACC_USED) {
@@ -459,7 +459,7 @@ __device__ inline bool row_and_ticket(const IcpArgs &a, double sum, double comp,
         }
     }
     __threadfence();
-    if (lane == 0) *s_last = (atomicAdd(&a.st->ticket, 1u) == gridDim.x - 1);
+    if (lane == 0) *s_last = (atomicAdd(&a.st->ticket, 1u) == (uint32_t)G - 1);
     __syncthreads();
     const bool last = *s_last != 0;
     __syncthreads();
@@ -468,10 +468,10 @@ __device__ inline bool row_and_ticket(const IcpArgs &a, double sum, double comp,
 }
 
 template <int MODE, int R>
-__device__ inline void finish_pass(const IcpArgs &a, const Pass p, double *vs) {
+__device__ inline void finish_pass(const IcpArgs &a, const Pass p, int G, double *vs) {
     constexpr int D = 2 * R + 1, K = D * D * D;
     const int lane = threadIdx.x;
-    const double v = reduce_rows<MODE>(a.partials, (int)gridDim.x);
+    const double v = reduce_rows<MODE>(a.partials, G);
     vs[lane] = v;
     a.st->acc[lane] = v;
     __syncthreads();
@@ -499,9 +499,10 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_icp(IcpArgs a) {
     double sum, comp;
     int32_t hits = 0, matched = 0;
     const Pass p{a.n, a.max_d2, a.kappa, a.it};
-    wave_pass<MODE, R>(a, p, Tf, (int)gridDim.x, lds, sum, comp, hits, matched);
-    if (!row_and_ticket<MODE>(a, sum, comp, hits, matched, &s_last)) return;
-    finish_pass<MODE, R>(a, p, vs);
+    const int G = (int)gridDim.x;
+    wave_pass<MODE, R>(a, p, Tf, G, lds, sum, comp, hits, matched);
+    if (!row_and_ticket<MODE>(a, sum, comp, hits, matched, G, &s_last)) return;
+    finish_pass<MODE, R>(a, p, G, vs);
 }
 
 // ---- the persistent registration ------------------------------------------------------------------
@@ -527,35 +528,48 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_register(LoopArgs L) {
     const int32_t n = *L.n_dev;
     if (blockIdx.x == 0 && lane == 0) st->n = n;
     if (n < L.min_corr) return;  // odom.py:756-757: the host sees n and keeps the prediction
+    // workgroups that no pass can use (the widest pass needs ceil(n * 4 / 64)) leave now
+    if ((int64_t)blockIdx.x * ICP_BLOCK >= (int64_t)n * Split<2>::S) return;
     const IcpArgs &a = L.a;
     for (int it = 0; it < L.max_iter; ++it) {
         if (lane < 16) Tf[lane] = __hip_atomic_load(&st->Tf[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         const int reach = it < L.wide_iters ? 2 : 1;
         const Pass p{n, reach == 2 ? L.max_d2_r2 : L.max_d2_r1, st->kappa[it], it};
-        double sum, comp;
-        int32_t hits = 0, matched = 0;
-        if (reach == 2)
-            wave_pass<MODE, 2>(a, p, Tf, (int)gridDim.x, lds, sum, comp, hits, matched);
-        else
-            wave_pass<MODE, 1>(a, p, Tf, (int)gridDim.x, lds, sum, comp, hits, matched);
-        if (row_and_ticket<MODE>(a, sum, comp, hits, matched, &s_last)) {
+        // the workgroups that hold points in this pass (one lane group each while the grid is large
+        // enough): only they write rows and take tickets, so the reduction reads no empty rows and the
+        // sum runs over the same rows, in the same order, as a one-pass launch of that many workgroups
+        const int S = reach == 2 ? Split<2>::S : Split<1>::S;
+        const int G = (int)min((int64_t)gridDim.x, ((int64_t)n * S + ICP_BLOCK - 1) / ICP_BLOCK);
+        bool last = false;
+        if ((int)blockIdx.x < G) {
+            double sum, comp;
+            int32_t hits = 0, matched = 0;
             if (reach == 2)
-                finish_pass<MODE, 2>(a, p, vs);
+                wave_pass<MODE, 2>(a, p, Tf, G, lds, sum, comp, hits, matched);
             else
-                finish_pass<MODE, 1>(a, p, vs);
+                wave_pass<MODE, 1>(a, p, Tf, G, lds, sum, comp, hits, matched);
+            last = row_and_ticket<MODE>(a, sum, comp, hits, matched, G, &s_last);
+        }
+        if (last) {
+            if (reach == 2)
+                finish_pass<MODE, 2>(a, p, G, vs);
+            else
+                finish_pass<MODE, 1>(a, p, G, vs);
             __threadfence();  // release: pose, done flag and ticket before the generation
             if (lane == 0) __hip_atomic_store(&st->gen, (uint32_t)(it + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
+            // poll with relaxed loads (they read through to L2 without invalidating this CU's caches, which
+            // the waves still searching share), then one acquire fence
             uint32_t spins = 0;
-            while (__hip_atomic_load(&st->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < (uint32_t)(it + 1)) {
-                __builtin_amdgcn_s_sleep(2);
+            while (__hip_atomic_load(&st->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < (uint32_t)(it + 1)) {
+                __builtin_amdgcn_s_sleep(8);
                 if (++spins > SPIN_LIMIT) {
                     if (lane == 0) st->status = STATUS_SPIN_TIMEOUT;
                     return;
                 }
             }
-            __threadfence();  // acquire
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the pose and done flag written before gen
         }
         __syncthreads();
         if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
@@ -669,7 +683,10 @@ hipError_t launch_register_loop(lw_ctx *c, const int32_t *n_dev, int32_t n_bound
             return hipErrorNotSupported;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
             return hipErrorNotSupported;
-        c->loop_max_groups[acc_mode] = per_cu * cus;
+        // co-resident limit, and at most two workgroups per CU: the loop never needs more than one lane
+        // group per workgroup (8192 thinned points at the 5x5x5 reach), and every extra workgroup
+        // only polls the generation counter
+        c->loop_max_groups[acc_mode] = per_cu < 2 ? per_cu * cus : 2 * cus;
     }
     LoopArgs L;
     L.a = base_args(c);

@@ -93,6 +93,12 @@ struct lw_ctx {
     double prof_ms = 0.0;
     int64_t prof_launches = 0;
     unsigned long long *prof_dev = nullptr;
+    // batched localisation: the work-queue counter, and the last captured launch (replayed while the
+    // arguments stay the same)
+    uint32_t *loc_queue = nullptr;
+    hipGraph_t loc_graph = nullptr;
+    hipGraphExec_t loc_exec = nullptr;
+    unsigned char loc_key[512] = {0};
     // the persistent registration: co-resident workgroups per accumulation form (0 = not yet queried), and
     // whether cooperative launches work on this device
     int32_t loop_max_groups[3] = {0, 0, 0};

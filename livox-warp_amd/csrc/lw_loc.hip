@@ -1,0 +1,288 @@
+// Batched global localisation: many pose hypotheses x query scans registered against one prior map in one
+// persistent launch. The reference's counterpart is GlobalLocalizer._icp (python/livox_warp/localize.py:517-553,
+// kernels k_icp_batch / k_reduce_batch :126-179), which registers up to 16 hypotheses with one host solve per
+// hypothesis and iteration; here the correspondence and accumulation are odom.py's (closest voxel mean in the
+// 27 neighbours, point-to-distribution residual, odom.py:275-375) and the step is localize.py's
+// (A = H + diag(1e-3 diag H + 1e-9), solve, left twist; skipped below 30 matches; localize.py:531-547).
+//
+// Work: (query, hypothesis) units, queries longest first; a workgroup takes a unit from an atomic queue and
+// runs all its iterations alone: every lane takes points of the query scan (grid-stride), searches the 27
+// voxels around each (bitmap words of a chunk loaded together, then its slots), forms the float32 terms of
+// H and b as the reference rounds them and adds them, widened, to float64 running sums; the workgroup
+// reduces (wave shuffles, then LDS) and one lane solves the 6x6 step in float64. Units never wait on each
+// other, so the grid needs no co-residency; the grid size and block size are tuning parameters.
+#include "lw_internal.h"
+#include "lw_loc.h"
+
+namespace lw {
+
+namespace {
+
+constexpr int LOC_C = 14;  // 27 neighbours in two chunks
+constexpr int LOC_ACC = 29;  // H (21), b (6), count, weight
+
+struct LocArgs {
+    const float *scans;
+    const int32_t *scan_off;
+    const int32_t *order;
+    const double *hyp_T;
+    double *out_T;
+    double *out_score;
+    int32_t *out_count;
+    int32_t *out_iters;
+    const Slot *slots;
+    const uint32_t *occ;
+    const VoxPrec *vox_prec;
+    uint32_t *queue;
+    unsigned long long *prof;
+    int32_t Q, H, mask, iters, min_count;
+    float inv_v, max_d2;
+    double damping;
+    float kappa[16];
+};
+
+// Closest voxel mean in the 27 neighbours (odom.py:308-334), chunked loads.
+__device__ inline int32_t loc_search(const LocArgs &a, float wx, float wy, float wz, float &mx, float &my, float &mz,
+                                     int32_t &hits) {
+    const int32_t bx = ifloor(wx * a.inv_v), by = ifloor(wy * a.inv_v), bz = ifloor(wz * a.inv_v);
+    float best_d2 = a.max_d2;
+    int32_t best_s = -1;
+    for (int j0 = 0; j0 < 27; j0 += LOC_C) {
+        int64_t key[LOC_C];
+        int32_t h[LOC_C];
+        uint32_t w[LOC_C];
+        LW_UNROLL for (int c = 0; c < LOC_C; ++c) {
+            const int j = j0 + c;
+            key[c] = voxel_key_ijk(bx + j / 9 - 1, by + (j / 3) % 3 - 1, bz + j % 3 - 1);
+            h[c] = slot_hash(key[c]) & a.mask;
+        }
+        LW_UNROLL for (int c = 0; c < LOC_C; ++c) w[c] = (j0 + c < 27) ? a.occ[h[c] >> 5] : 0u;
+        int4 lo[LOC_C];
+        float4 hi[LOC_C];
+        LW_UNROLL for (int c = 0; c < LOC_C; ++c) {
+            lo[c] = make_int4(-1, -1, -1, 0);
+            hi[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if ((w[c] >> (h[c] & 31)) & 1u) {
+                const int4 *p = reinterpret_cast<const int4 *>(a.slots + h[c]);
+                lo[c] = p[0];
+                hi[c] = reinterpret_cast<const float4 *>(p)[1];
+            }
+        }
+        LW_UNROLL for (int c = 0; c < LOC_C; ++c) {
+            if (!((w[c] >> (h[c] & 31)) & 1u)) continue;
+            int32_t s = h[c];
+            const int64_t k = (int64_t)(((uint64_t)(uint32_t)lo[c].y << 32) | (uint64_t)(uint32_t)lo[c].x);
+            if (k != key[c]) {  // the rest of the linear probe (gpu.py:254-265)
+                s = -1;
+                for (int probe = 1; probe < PROBES; ++probe) {
+                    int32_t t = (h[c] + probe) & a.mask;
+                    if (!((a.occ[t >> 5] >> (t & 31)) & 1u)) break;
+                    if (a.slots[t].key == key[c]) {
+                        s = t;
+                        break;
+                    }
+                }
+                if (s < 0) continue;
+                const int4 *p = reinterpret_cast<const int4 *>(a.slots + s);
+                lo[c] = p[0];
+                hi[c] = reinterpret_cast<const float4 *>(p)[1];
+            }
+            ++hits;
+            if (!(__int_as_float(lo[c].w) > 0.0f)) continue;
+            float d0 = wx - hi[c].x, d1 = wy - hi[c].y, d2 = wz - hi[c].z;
+            float dd = d0 * d0;
+            dd = dd + d1 * d1;
+            dd = dd + d2 * d2;
+            if (dd < best_d2) {
+                best_d2 = dd;
+                best_s = s;
+                mx = hi[c].x;
+                my = hi[c].y;
+                mz = hi[c].z;
+            }
+        }
+    }
+    return best_s;
+}
+
+#define LW_H_PAIRS(X) \
+    X(0, 0, 0) X(0, 1, 1) X(0, 2, 2) X(0, 3, 3) X(0, 4, 4) X(0, 5, 5) X(1, 1, 6) X(1, 2, 7) X(1, 3, 8) X(1, 4, 9) \
+    X(1, 5, 10) X(2, 2, 11) X(2, 3, 12) X(2, 4, 13) X(2, 5, 14) X(3, 3, 15) X(3, 4, 16) X(3, 5, 17) X(4, 4, 18)   \
+    X(4, 5, 19) X(5, 5, 20)
+
+// localize.py:531-547 for one hypothesis: returns whether the pose moved by less than the batch tolerance.
+__device__ __noinline__ bool loc_step(const double *v, double damping, int32_t min_count, double *T, double *moved) {
+    *moved = 0.0;
+    if (v[21 + 6] < (double)min_count) return false;  // too few matches: keep the pose (localize.py:535-536)
+    double Hm[36], A[36], rhs[6], d[6];
+    gn::sym6(v, Hm);
+    LW_UNROLL for (int i = 0; i < 36; ++i) A[i] = Hm[i];
+    LW_UNROLL for (int i = 0; i < 6; ++i) {
+        A[7 * i] += damping * Hm[7 * i] + 1e-9;
+        rhs[i] = -v[21 + i];
+    }
+    const bool all[6] = {true, true, true, true, true, true};
+    if (!gn::solve6(A, rhs, all, d)) return false;  // LinAlgError: keep the pose (localize.py:541-543)
+    gn::apply_twist(T, d);
+    const double nr = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double nt = sqrt(d[3] * d[3] + d[4] * d[4] + d[5] * d[5]);
+    *moved = nr > 5.0 * nt ? nr : 5.0 * nt;  // localize.py:545
+    return true;
+}
+
+template <int B>
+__global__ __launch_bounds__(B) void k_loc(LocArgs a) {
+    constexpr int W = B / 64;
+    __shared__ double red[W][LOC_ACC + 1];
+    __shared__ double vsum[32];
+    __shared__ double Td[16];
+    __shared__ float Tf[16];
+    __shared__ int32_t s_unit, s_stop;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int32_t hits = 0, matched_total = 0;
+    unsigned long long pt_iters = 0;
+    const int32_t total = a.Q * a.H;
+    for (;;) {
+        if (threadIdx.x == 0) s_unit = (int32_t)atomicAdd(a.queue, 1u);
+        __syncthreads();
+        const int32_t u = s_unit;
+        __syncthreads();
+        if (u >= total) break;  // every workgroup leaves once the queue is drained
+        const int32_t q = a.order[u / a.H];
+        const int32_t h = u % a.H;
+        const int32_t unit = q * a.H + h;
+        const int32_t p0 = a.scan_off[q], n = a.scan_off[q + 1] - p0;
+        if (threadIdx.x < 16) Td[threadIdx.x] = a.hyp_T[(size_t)unit * 16 + threadIdx.x];
+        if (threadIdx.x == 0) s_stop = 0;
+        __syncthreads();
+        double score = 0.0, count = 0.0;
+        int32_t iters = 0;
+        for (int it = 0; it < a.iters; ++it) {
+            if (threadIdx.x < 16) Tf[threadIdx.x] = (float)Td[threadIdx.x];
+            __syncthreads();
+            const float kappa = a.kappa[it < 16 ? it : 15];
+            double acc[LOC_ACC];
+            LW_UNROLL for (int k = 0; k < LOC_ACC; ++k) acc[k] = 0.0;
+            for (int32_t i = threadIdx.x; i < n; i += B) {
+                const float *pp = a.scans + 3 * (int64_t)(p0 + i);
+                float wx, wy, wz, mx = 0.0f, my = 0.0f, mz = 0.0f;
+                transform_point(Tf, pp[0], pp[1], pp[2], wx, wy, wz);
+                const int32_t s = loc_search(a, wx, wy, wz, mx, my, mz, hits);
+                if (s < 0) continue;
+                ++matched_total;
+                const VoxPrec P = a.vox_prec[a.slots[s].vid];
+                const float M[3][3] = {{P.M[0], P.M[1], P.M[2]}, {P.M[1], P.M[3], P.M[4]}, {P.M[2], P.M[4], P.M[5]}};
+                const float r0 = wx - mx, r1 = wy - my, r2 = wz - mz;
+                float Mr[3];
+                LW_UNROLL for (int k = 0; k < 3; ++k) {
+                    float t = M[k][0] * r0;
+                    t = t + M[k][1] * r1;
+                    Mr[k] = t + M[k][2] * r2;
+                }
+                float d2 = r0 * Mr[0];
+                d2 = d2 + r1 * Mr[1];
+                d2 = d2 + r2 * Mr[2];
+                float wgt = 1.0f / (1.0f + d2 / (kappa * kappa));
+                wgt = wgt * wgt;
+#define LW_LOC_H(p, q, k) acc[k] += (double)(wgt * hval<p, q>(M, wx, wy, wz));
+                LW_H_PAIRS(LW_LOC_H)
+#undef LW_LOC_H
+                acc[21] += (double)(wgt * bval<0>(Mr, wx, wy, wz));
+                acc[22] += (double)(wgt * bval<1>(Mr, wx, wy, wz));
+                acc[23] += (double)(wgt * bval<2>(Mr, wx, wy, wz));
+                acc[24] += (double)(wgt * bval<3>(Mr, wx, wy, wz));
+                acc[25] += (double)(wgt * bval<4>(Mr, wx, wy, wz));
+                acc[26] += (double)(wgt * bval<5>(Mr, wx, wy, wz));
+                acc[27] += 1.0;
+                acc[28] += (double)wgt;
+            }
+            pt_iters += (threadIdx.x == 0) ? (unsigned long long)n : 0ull;
+            // reduce: wave shuffles, then one LDS row per wave summed in wave order
+            LW_UNROLL for (int k = 0; k < LOC_ACC; ++k) {
+                double x = acc[k];
+                LW_UNROLL for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+                if (lane == 0) red[wave][k] = x;
+            }
+            __syncthreads();
+            if (threadIdx.x < LOC_ACC) {
+                double t = 0.0;
+                LW_UNROLL for (int wv = 0; wv < W; ++wv) t += red[wv][threadIdx.x];
+                vsum[threadIdx.x] = t;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                score = vsum[28];
+                count = vsum[27];
+                double moved;
+                const bool stepped = loc_step(vsum, a.damping, a.min_count, Td, &moved);
+                if (stepped) iters = it + 1;
+                s_stop = stepped && moved < 2e-4 ? 1 : 0;  // converged (localize.py:548-549, per hypothesis)
+            }
+            __syncthreads();
+            if (s_stop) break;
+        }
+        if (threadIdx.x < 16) a.out_T[(size_t)unit * 16 + threadIdx.x] = Td[threadIdx.x];
+        if (threadIdx.x == 0) {
+            a.out_score[unit] = score;
+            a.out_count[unit] = (int32_t)count;
+            a.out_iters[unit] = iters;
+        }
+        __syncthreads();
+    }
+    if (a.prof) {
+        LW_UNROLL for (int off = 32; off > 0; off >>= 1) {
+            hits += __shfl_xor(hits, off, 64);
+            matched_total += __shfl_xor(matched_total, off, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(&a.prof[2], (unsigned long long)hits);
+            atomicAdd(&a.prof[3], (unsigned long long)matched_total);
+        }
+        if (threadIdx.x == 0) {
+            atomicAdd(&a.prof[0], pt_iters);
+            atomicAdd(&a.prof[1], pt_iters * 27ull);
+        }
+    }
+}
+
+__global__ void k_queue_reset(uint32_t *q) { *q = 0u; }
+
+}  // namespace
+
+hipError_t launch_loc(lw_ctx *c, const LocLaunch &L) {
+    LocArgs a{};
+    a.scans = L.scans;
+    a.scan_off = L.scan_off;
+    a.order = L.order;
+    a.hyp_T = L.hyp_T;
+    a.out_T = L.out_T;
+    a.out_score = L.out_score;
+    a.out_count = L.out_count;
+    a.out_iters = L.out_iters;
+    a.slots = c->slots;
+    a.occ = c->occ;
+    a.vox_prec = c->vox_prec;
+    a.queue = c->loc_queue;
+    a.prof = c->prof_on ? c->prof_dev : nullptr;
+    a.Q = L.Q;
+    a.H = L.H;
+    a.mask = c->mask;
+    a.iters = L.iters;
+    a.min_count = L.min_count;
+    a.inv_v = L.inv_v;
+    a.max_d2 = L.max_d2;
+    a.damping = L.damping;
+    for (int k = 0; k < 16; ++k) a.kappa[k] = L.kappa[k];
+    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(1), 0, c->stream, c->loc_queue);
+    const dim3 grid((unsigned)(L.grid > 0 ? L.grid : 1));
+    switch (L.block) {
+    case 64: hipLaunchKernelGGL(k_loc<64>, grid, dim3(64), 0, c->stream, a); break;
+    case 128: hipLaunchKernelGGL(k_loc<128>, grid, dim3(128), 0, c->stream, a); break;
+    case 256: hipLaunchKernelGGL(k_loc<256>, grid, dim3(256), 0, c->stream, a); break;
+    case 512: hipLaunchKernelGGL(k_loc<512>, grid, dim3(512), 0, c->stream, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace lw

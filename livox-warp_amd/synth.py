@@ -318,9 +318,10 @@ def scan(S: Scene, rosette: Rosette, pose_fn, t0: float, n: int, seed: int, nois
     return xyz, attr, t.astype(np.float32)
 
 
-def sample_surfaces(S: Scene, spacing: float, seed: int = 7) -> np.ndarray:
+def sample_surfaces(S: Scene, spacing: float, seed: int = 7, prune: bool = True) -> np.ndarray:
     """Points on every visible surface of the scene (room inside faces, box faces except the bottom) on a
-    uniform grid of the given spacing with a seeded sub-spacing offset; world float32 (n, 3)."""
+    uniform grid of the given spacing with a seeded sub-spacing offset; world float32 (n, 3). prune=False keeps
+    the few points hidden inside other boxes (faster for very large scenes)."""
     rng = np.random.default_rng(seed)
     out = []
 
@@ -346,11 +347,47 @@ def sample_surfaces(S: Scene, spacing: float, seed: int = 7) -> np.ndarray:
     for lo, hi in zip(S.box_lo, S.box_hi):
         boxfaces(lo, hi, lo[2] > S.room_lo[2] + 1e-6)
     pts = np.concatenate(out)
+    if not prune:
+        return pts.astype(np.float32)
     # drop points hidden inside a box (faces that touch another box or the floor)
     inside = np.zeros(len(pts), bool)
     for lo, hi in zip(S.box_lo, S.box_hi):
         inside |= np.all((pts > lo + 1e-6) & (pts < hi - 1e-6), axis=1)
     return pts[~inside].astype(np.float32)
+
+
+@dataclass
+class LocWorkload:
+    """Batched global localisation: a multi-room floor prior map and query scans from seeded true poses."""
+
+    map_xyz: np.ndarray
+    scans: list
+    T_true: np.ndarray  # (Q, 4, 4)
+    scene: Scene
+
+
+def loc_workload(seed: int = 31, n_queries: int = 4, spacing: float = 0.02, min_pts: int = 5000,
+                 max_pts: int = 30000) -> LocWorkload:
+    """The 60 x 40 x 4 m multi-room floor sampled at `spacing` (about 20M points at 20 mm), and n_queries
+    rosette scans of log-uniform 5k-30k points from seeded poses: sensor at 1.4 m, away from walls, random yaw,
+    level."""
+    rng = np.random.default_rng(seed)
+    S = multi_room_floor(seed)
+    mp = sample_surfaces(S, spacing, seed + 1, prune=False)
+    walls_x = (12.0, 24.0, 36.0, 48.0)
+    walls_y = (13.0, 26.0)
+    scans, poses = [], []
+    ros = Rosette(seed + 2)
+    while len(scans) < n_queries:
+        x, y = rng.uniform(1.0, 59.0), rng.uniform(1.0, 39.0)
+        if min(abs(x - w) for w in walls_x) < 0.6 or min(abs(y - w) for w in walls_y) < 0.6:
+            continue
+        T = pose_matrix(np.array([x, y, 1.4]), rng.uniform(0, 2 * math.pi), 0.0, 0.0)
+        n = int(round(math.exp(rng.uniform(math.log(min_pts), math.log(max_pts)))))
+        xyz, _, _ = scan(S, ros, lambda tt, T=T: T, 0.0, n, seed * 1000 + len(scans))
+        scans.append(xyz)
+        poses.append(T)
+    return LocWorkload(mp, scans, np.stack(poses), S)
 
 
 # ---- the workloads ------------------------------------------------------------------------------

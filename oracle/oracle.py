@@ -619,3 +619,33 @@ class OracleOdometry:
             return T_pred, False
         self.stats.update(iters=r.iters, corr=r.corr, rms=r.rms, cond=r.cond, weak=r.weak, degen=r.degen)
         return r.T, r.ok
+
+
+def loc_register(omap: OracleMap, scan, T0, voxel: float, iters: int = 10, min_count: int = 30,
+                 damping: float = 1e-3, noise: float = 0.02, alpha: float = 0.05):
+    """One hypothesis of the batched localisation: odom.py's accumulation at reach 1 (odom.py:275-384) with
+    localize.py's step (localize.py:531-549: A = H + diag(damping diag H + 1e-9), solve, left twist; skipped
+    below min_count matches; stop once the move max(|d_rho|, 5 |d_theta|) < 2e-4, here per hypothesis).
+    Returns (T, score = robust weight sum of the last pass, matched count of the last pass, steps taken)."""
+    T = np.array(T0, dtype=np.float64)
+    noise2 = float(np.float32(noise * noise))
+    tight2 = float(np.float32((TIGHT_FRACTION * voxel / math.sqrt(12.0)) ** 2))
+    max_d2 = float(np.float32((1.5 * voxel) ** 2))
+    score, count, steps = 0.0, 0, 0
+    for it in range(iters):
+        kappa = float(np.float32(max(KAPPA_MIN, KAPPA_START * (KAPPA_DECAY ** min(it, 15)))))
+        v = icp_accumulate(omap, scan, T, voxel, 1, max_d2, noise2, tight2, alpha, kappa)
+        score, count = float(v[ACC_WEIGHT]), int(v[ACC_COUNT])
+        if v[ACC_COUNT] < min_count:
+            continue
+        H = sym6(v[ACC_H:ACC_H + 21])
+        A = H + np.diag(damping * np.diag(H) + 1e-9)
+        try:
+            delta = np.linalg.solve(A, -v[ACC_B:ACC_B + 6])
+        except np.linalg.LinAlgError:
+            continue
+        T = apply_twist(T, delta)
+        steps = it + 1
+        if max(float(np.linalg.norm(delta[:3])), float(np.linalg.norm(delta[3:])) * 5.0) < 2e-4:
+            break
+    return T, score, count, steps

@@ -1,0 +1,80 @@
+"""Batched localisation (lw_loc_batch) against the oracle's loc_register: every (query, hypothesis) unit of one
+persistent launch, at every tile, eager and replayed from the captured graph."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+VOXEL = 0.25
+SLOTS = 1 << 20
+
+
+@pytest.fixture(scope="module")
+def problem(lw, orc):
+    ss = lw.synth.single_scan(seed=5, n=6000, map_points=150_000)
+    scans = [ss.scan_xyz[:1500], ss.scan_xyz[1500:6000], ss.scan_xyz[:700]]  # uneven sizes, out of order
+    rng = np.random.default_rng(3)
+    hyps = np.zeros((len(scans), 6, 4, 4))
+    for q in range(len(scans)):
+        hyps[q, 0] = ss.T_true
+        for h in range(1, 5):
+            d = np.concatenate([rng.uniform(-0.12, 0.12, 3), np.radians(rng.uniform(-4, 4, 3))])
+            hyps[q, h] = orc.apply_twist(ss.T_true, d)
+        hyps[q, 5] = ss.T_true.copy()
+        hyps[q, 5, :3, 3] += (30.0, 0.0, 0.0)  # no matches: every step skipped
+    omap = orc.OracleMap(SLOTS)
+    omap.insert(ss.map_xyz, np.zeros(len(ss.map_xyz), np.uint32), np.eye(4), VOXEL, max_range=1e30)
+    want = [[orc.loc_register(omap, scans[q], hyps[q, h], VOXEL, iters=10) for h in range(hyps.shape[1])]
+            for q in range(len(scans))]
+    loc = lw.loc.BatchLocalizer(ss.map_xyz, voxel=VOXEL, reg_slots=SLOTS, chunk=1 << 18)
+    assert loc.map_stats() == tuple(int(x) for x in omap.stats)
+    return loc, scans, hyps, want
+
+
+def _check(got, want, ang_deg=1e-3, pos_m=1e-4):
+    T, S, C, I = (x.cpu().numpy() for x in got)
+    for q, row in enumerate(want):
+        for h, (Tw, sw, cw, iw) in enumerate(row):
+            d = np.linalg.inv(Tw) @ T[q, h]
+            ang = math.degrees(math.acos(max(-1.0, min(1.0, (np.trace(d[:3, :3]) - 1) / 2))))
+            assert np.linalg.norm(d[:3, 3]) < pos_m and ang < ang_deg, (q, h)
+            assert abs(int(C[q, h]) - cw) <= 2, (q, h, C[q, h], cw)
+            assert abs(S[q, h] - sw) <= 1e-3 * max(1.0, abs(sw)), (q, h, S[q, h], sw)
+            assert I[q, h] == iw, (q, h, I[q, h], iw)
+
+
+@pytest.mark.parametrize("tile", [(64, 2), (128, 4), (256, 8), (512, 1)])
+def test_batch_matches_oracle(problem, tile, gpu_ok):
+    loc, scans, hyps, want = problem
+    print(f"lw_loc_batch tile {tile}, queries {[len(s) for s in scans]}, hypotheses {hyps.shape[1]}", flush=True)
+    got = loc.run(scans, hyps, tile=tile)
+    loc.stream.synchronize()
+    _check(got, want)
+
+
+def test_graph_replay_matches_eager(problem, gpu_ok):
+    loc, scans, hyps, want = problem
+    staged = loc._stage(scans, hyps)
+    eager = [x.clone() for x in loc.run(scans, hyps, tile=(128, 4), staged=staged)]
+    loc.stream.synchronize()
+    for rep in range(3):
+        print(f"graph replay {rep}", flush=True)
+        got = loc.run(scans, hyps, tile=(128, 4), use_graph=True, staged=staged)
+        loc.stream.synchronize()
+        for a, b in zip(got, eager):
+            assert torch.equal(a, b)
+        torch.cuda.synchronize()
+    _check(got, want)
+
+
+def test_truth_hypothesis_scores_highest(problem, gpu_ok):
+    """The hypothesis at the true pose stays there and outscores the one with no overlap."""
+    loc, scans, hyps, want = problem
+    T, S, C, _ = (x.cpu().numpy() for x in loc.run(scans, hyps, tile=(128, 4)))
+    for q in range(len(scans)):
+        assert np.linalg.norm(T[q, 0, :3, 3] - hyps[q, 0, :3, 3]) < 0.05
+        assert S[q, 0] > S[q, 5] and C[q, 5] < 30

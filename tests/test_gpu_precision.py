@@ -1,6 +1,22 @@
 """Precision study: the office sequence translated by (1000, -750, 30) m, run with the three accumulation forms
-(float64, compensated float32, naive float32; residuals float32 in all three) and compared frame by frame with
-the all-float64 oracle (every device-side step in float64, the reference's numpy host step)."""
+(float64, compensated float32, naive float32; residuals and Jacobians float32 in all three, as odom.py:275-375
+computes them). Every registration is compared with the oracle's registration on identical inputs -- the
+device's map (read back in the reference's layout), its thinned points and the same prediction -- so the
+comparison isolates the arithmetic of one registration.
+
+A whole trajectory is not a usable comparison this far from the origin: the reference algorithm's sequence is
+chaotic there (in the oracle itself, a 1 um change of the start pose moves later poses by up to 17 cm), so any
+difference of rounding, however small, grows to centimetres within a few dozen frames.
+
+Two oracles:
+  - the reference's own arithmetic (float32 residuals, float64 J^T J / J^T r and float64 host step): the float64
+    form must agree with it to the parity bar (1 mm / 0.01 deg), the compensated form to 1 cm / 0.1 deg; naive
+    float32 accumulation is the form the study is there to show failing: reported, not bounded;
+  - the all-float64 build (every device-side step in float64, the same map moments widened): what float32
+    residuals cost at 1250 m from the origin, per registration. Reported, with a loose bound as a guard.
+"""
+
+import ctypes
 
 import numpy as np
 import pytest
@@ -10,6 +26,7 @@ pytestmark = pytest.mark.gpu
 
 OFFSET = (1000.0, -750.0, 30.0)
 FRAMES = 60
+SLOTS = 1 << 19
 
 
 def _pose_err(A, B):
@@ -19,31 +36,84 @@ def _pose_err(A, B):
 
 
 @pytest.fixture(scope="module")
-def translated(lw, orc):
+def translated(lw):
     seq = lw.synth.Sequence(seed=21, seconds=FRAMES / 10 + 1, offset=OFFSET)
-    frames = list(seq.frames(FRAMES))
-    od = orc.OracleOdometry(lw.PoseTable(), reg_slots=1 << 20, frame_cap=1 << 15, fp64=True)
-    od.reg_voxel = 0.2
-    od.max_iter = 20
-    od.reset(seq.pose(0.05))
-    for x, a, t in frames:
-        od.push(x, a, t, len(t), float(t[0]), float(t[-1]))
-    return seq, frames, {k: T for k, _, T in od.traj}
+    return seq, list(seq.frames(FRAMES))
 
 
-@pytest.mark.parametrize("mode,bar_m,bar_deg", [("fp64", 0.01, 0.1), ("kahan32", 0.01, 0.1), ("fp32", 1.0, 10.0)])
-def test_forms_against_fp64_oracle(lw, translated, mode, bar_m, bar_deg, gpu_ok):
-    seq, frames, ref = translated
-    odo = lw.Odometry(lw.PoseTable(), "cuda:0", reg_slots=1 << 20, frame_cap=1 << 15)
+def _oracle_map(orc, maps, fp64):
+    keys, rsum, router, rcnt = maps
+    om = orc.OracleMap(SLOTS, fp64=fp64)
+    om.keys[:] = keys
+    om.rsum[:] = rsum
+    om.router[:] = router
+    om.rcnt[:] = rcnt
+    return om
+
+
+_runs = {}
+
+
+def _per_registration(lw, orc, translated, mode):
+    """Run the device odometry with one accumulation form; for every registration, the oracle's registrations
+    (reference arithmetic, all-float64) from the same inputs. Returns (errors vs reference arithmetic, errors vs
+    all-float64), each (registrations, 2) in metres / degrees."""
+    if mode in _runs:
+        return _runs[mode]
+    seq, frames = translated
+    odo = lw.Odometry(lw.PoseTable(), "cuda:0", reg_slots=SLOTS, frame_cap=1 << 15)
     odo.reg_voxel = 0.2
     odo.max_iter = 20
     odo.acc_mode = mode
+    L, ctx = odo._lib, odo._ctx
+    P = orc.Params(reg_voxel=odo.reg_voxel, max_iter=odo.max_iter, min_corr=odo.min_corr, min_range=odo.min_range,
+                   max_range=odo.max_range, noise=odo.noise, ret_mask=odo.ret_mask, alpha=odo.alpha,
+                   damping=odo.damping, degen_thr=odo.degen_thr, prior_info=odo.prior_info)
+    e32, e64 = [], []
+    inner = odo._register
+
+    def wrapped(m, T_pred):
+        T, ok = inner(m, T_pred)
+        n = odo.stats["ds"]
+        if n < odo.min_corr:
+            return T, ok
+        ds = np.empty((n, 3), np.float32)
+        assert L.lw_thin_read(ctx, ds.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n) == 0
+        maps = (np.empty(SLOTS, np.int64), np.empty((SLOTS, 3), np.float32), np.empty((SLOTS, 3, 3), np.float32),
+                np.empty(SLOTS, np.int32))
+        types = (ctypes.c_int64, ctypes.c_float, ctypes.c_float, ctypes.c_int32)
+        assert L.lw_map_read(ctx, *(a.ctypes.data_as(ctypes.POINTER(t)) for a, t in zip(maps, types))) == 0
+        r32 = orc.register(_oracle_map(orc, maps, False), ds, np.array(T_pred, np.float64), P)
+        r64 = orc.register(_oracle_map(orc, maps, True), ds, np.array(T_pred, np.float64), P)
+        e32.append(_pose_err(T, r32.T))
+        e64.append(_pose_err(T, r64.T))
+        return T, ok
+
+    odo._register = wrapped
     odo.reset(seq.pose(0.05))
     for x, a, t in frames:
         odo.push(torch.from_numpy(x).cuda(), torch.from_numpy(a.view(np.int32)).cuda(), torch.from_numpy(t).cuda(),
                  len(t), float(t[0]), float(t[-1]), lambda *_: None)
-    errs = np.array([_pose_err(T, ref[k]) for k, _, T in odo.traj if k in ref])
-    print(f"{mode}: vs float64 oracle, per frame max {errs[:, 0].max() * 1e3:.3f} mm / {errs[:, 1].max():.4f} deg, "
-          f"rms {np.sqrt(np.mean(errs[:, 0] ** 2)) * 1e3:.3f} mm")
-    assert len(errs) >= FRAMES - 2
-    assert errs[:, 0].max() < bar_m and errs[:, 1].max() < bar_deg
+    _runs[mode] = (np.array(e32), np.array(e64))
+    return _runs[mode]
+
+
+def _report(mode, errs, what):
+    print(f"{mode} vs {what}: {len(errs)} registrations, max {errs[:, 0].max() * 1e3:.4f} mm / "
+          f"{errs[:, 1].max():.5f} deg, rms {np.sqrt(np.mean(errs[:, 0] ** 2)) * 1e3:.4f} mm", flush=True)
+    assert len(errs) >= FRAMES - 10
+    assert np.all(np.isfinite(errs))
+
+
+@pytest.mark.parametrize("mode,bar_m,bar_deg", [("fp64", 1e-3, 0.01), ("kahan32", 0.01, 0.1), ("fp32", None, None)])
+def test_forms_against_reference_arithmetic(lw, orc, translated, mode, bar_m, bar_deg, gpu_ok):
+    errs, _ = _per_registration(lw, orc, translated, mode)
+    _report(mode, errs, "oracle (float32 residuals, float64 accumulation)")
+    if bar_m is not None:
+        assert errs[:, 0].max() < bar_m and errs[:, 1].max() < bar_deg
+
+
+def test_float32_residual_cost(lw, orc, translated, gpu_ok):
+    _, errs = _per_registration(lw, orc, translated, "fp64")
+    _report("fp64", errs, "all-float64 oracle")
+    assert errs[:, 0].max() < 0.1 and errs[:, 1].max() < 1.0
