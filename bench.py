@@ -3,24 +3,25 @@
 Workload (one "step" = one odometry frame): the plain odometry sequence -- 600 frames (60 s at 10 Hz) of a
 seeded handheld walk through a 20 x 15 x 3 m office, 10,000 rose-curve rosette points per frame with
 per-point timestamps (deskewed against the constant-velocity model), 0.2 m registration voxels, a 2^22-slot
-map, up to 20 Gauss-Newton iterations per registration. All 600 frames are generated on the host and uploaded
+map, up to 20 Gauss-Newton iterations per registration. All frames are generated on the host and uploaded
 to HBM before the timed region; the timed region pushes frames from HBM through Odometry.push exactly as a
 caller would (frame cutting, deskew, thinning, registration, keyframe inserts, one readback per
 registration).
 
 Printed: one JSON line with frames/s (value), points/s, drift against the ground truth, the roofline of the
-dominant kernel (the accumulation pass: algorithmic bytes from its counters, duration from HIP events on its
-stream) and the CPU baseline (the oracle's serial restatement on a bounded sample of the same sequence, one
-host thread).
+dominant kernel (the persistent registration kernel: algorithmic bytes from its counters, duration from HIP
+events on its stream), the CPU baseline (the oracle's serial restatement on a bounded sample of the same
+sequence, one host thread) and, unless --no-loc, the batched global localisation (hypotheses/s, map build,
+top-1 recovery, with its own roofline and CPU baseline).
 """
 
 from __future__ import annotations
 
 import argparse
-import ctypes
 import importlib
 import json
 import os
+import platform
 import sys
 import time
 
@@ -32,14 +33,27 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "ICP frames/s @10k pts/frame, % HBM roofline; global-loc hypotheses/s"
 SEED = 21
+LOC_SEED = 31
+LOC_VOXEL = 0.1
 
 
 def accumulation_bytes(prof) -> float:
-    """Algorithmic bytes of the accumulation passes: per point-iteration the point (12 B) and one 8 B key per
+    """Algorithmic bytes of accumulation passes: per point-iteration the point (12 B) and one 8 B key per
     neighbour looked up, 16 B per occupied neighbour hit (mean + count) and 36 B per matched point (the chosen
     voxel's second moments)."""
     pt_iters, lookups, hits, matched = prof[2], prof[3], prof[4], prof[5]
     return 12.0 * pt_iters + 8.0 * lookups + 16.0 * hits + 36.0 * matched
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
 
 
 def drift(seq, traj):
@@ -57,12 +71,17 @@ def drift(seq, traj):
             "distance_m": dist}
 
 
-def cpu_baseline(frames, seq, warmup: int, n_frames: int):
-    """The oracle (serial C restatement of the kernels + the reference's numpy host step) on n_frames of the same
-    sequence after the same untimed warm-up, one host thread."""
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
+    return oracle
+
+
+def cpu_baseline(frames, seq, warmup: int, n_frames: int):
+    """The oracle (serial C restatement of the kernels + the reference's numpy host step) on n_frames of the same
+    sequence after the same untimed warm-up, one host thread."""
+    oracle = _oracle()
     lw = importlib.import_module("livox-warp_amd")
     od = oracle.OracleOdometry(lw.PoseTable(), reg_slots=1 << 22, frame_cap=1 << 15)
     od.reg_voxel = 0.2
@@ -78,7 +97,113 @@ def cpu_baseline(frames, seq, warmup: int, n_frames: int):
     done = len(od.traj) - f0
     return {"value": round(done / el, 3), "unit": "frames/s", "cores": 1, "kind": "port",
             "sample": f"{done} frames of the same sequence after the same {warmup}-frame warm-up; oracle serial C "
-                      f"kernels + the reference's numpy host step", "seconds": round(el, 3)}
+                      f"kernels + the reference's numpy host step (reference-semantics CPU restatement)",
+            "seconds": round(el, 3), "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}
+
+
+def _pose_err(A, B):
+    d = np.linalg.inv(A) @ B
+    ang = float(np.degrees(np.arccos(np.clip((np.trace(d[:3, :3]) - 1) / 2, -1, 1))))
+    return float(np.linalg.norm(d[:3, 3])), ang
+
+
+def localisation(args, torch, lw, dev):
+    """Batched global localisation: the 60 x 40 x 4 m floor at 20 mm (about 20M points) voxelised at 0.1 m,
+    args.loc_queries scans of 5k-30k points, args.loc_hyps hypotheses each (xy uniform over the floor, yaw
+    uniform, z / roll / pitch at truth), 10 Gauss-Newton iterations per hypothesis, all in one launch."""
+    t_gen = time.perf_counter()
+    W = lw.synth.loc_workload(seed=LOC_SEED, n_queries=args.loc_queries)
+    rng = np.random.default_rng(LOC_SEED + 7)
+    H = args.loc_hyps
+    hyps = np.stack([lw.loc.hypotheses(rng, H, W.T_true[q], (0.5, 0.5), (59.5, 39.5))
+                     for q in range(len(W.scans))])
+    t_gen = time.perf_counter() - t_gen
+    X = torch.from_numpy(W.map_xyz).to(dev)
+    torch.cuda.synchronize()
+    loc = lw.loc.BatchLocalizer(X, voxel=LOC_VOXEL, reg_slots=1 << 22, chunk=1 << 22, device=dev)
+    # map build, timed on its own: clear, then the bulk insert of every map point (in index order)
+    loc.eng.sync()
+    t0 = time.perf_counter()
+    loc.eng.reset()
+    with torch.cuda.stream(loc.stream):
+        loc.eng.insert(X, None, np.eye(4), LOC_VOXEL, max_range=1e30)
+    loc.eng.sync()
+    t_map = time.perf_counter() - t0
+    n_map = int(X.shape[0])
+    V = loc.map_stats()[0]
+    map_bytes = 16.0 * n_map + 60.0 * V  # xyz + attr per point, one record written per new voxel
+    # tile autotuner on a slice of the workload, then the timed launches (graph-replayed)
+    tile, table = loc.autotune([W.scans[0]], hyps[:1, :256])
+    staged = loc._stage(W.scans, hyps)
+    L = loc.eng.lib
+    ctx = loc.eng.ctx
+    loc.run(W.scans, hyps, tile=tile, use_graph=True, staged=staged)
+    loc.stream.synchronize()
+    lw._abi.check(L.lw_prof_enable(ctx, 1), "lw_prof_enable")
+    lw._abi.check(L.lw_prof_reset(ctx), "lw_prof_reset")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.loc_reps):
+        out = loc.run(W.scans, hyps, tile=tile, use_graph=True, staged=staged)
+    loc.stream.synchronize()
+    el = time.perf_counter() - t0
+    prof = np.zeros(8)
+    lw._abi.check(L.lw_prof_read(ctx, lw._abi.ptr(prof, lw._abi._f64p)), "lw_prof_read")
+    lw._abi.check(L.lw_prof_enable(ctx, 0), "lw_prof_enable")
+    n_units = len(W.scans) * H
+    T, S, C, I = (x.cpu().numpy() for x in out)
+    best = [_pose_err(W.T_true[q], T[q, int(np.argmax(S[q]))]) for q in range(len(W.scans))]
+    closest = [min(_pose_err(W.T_true[q], T[q, h])[0] for h in range(H)) for q in range(len(W.scans))]
+    launch_s = prof[1] * 1e-3 / max(prof[0], 1.0)
+    bytes_per_launch = accumulation_bytes(prof) / max(prof[0], 1.0)
+    achieved = bytes_per_launch / launch_s / 1e9 if launch_s > 0 else 0.0
+    res = {
+        "value": round(n_units * args.loc_reps / el, 1), "unit": "hypotheses/s",
+        "workload": f"{len(W.scans)} queries x {H} hypotheses, scans {[len(s) for s in W.scans]} points, map "
+                    f"{n_map} points at 20 mm -> {V} voxels of 0.1 m, 10 GN iterations at reach 1",
+        "seed": LOC_SEED, "tile": {"block": tile[0], "workgroups_per_cu": tile[1]}, "autotune_ms": table,
+        "launch_ms": round(launch_s * 1e3, 3), "graph": True,
+        "top1_recovered": sum(1 for e in best if e[0] < 0.3 and e[1] < 3.0), "queries": len(W.scans),
+        "top1_error": [[round(e[0], 3), round(e[1], 2)] for e in best],
+        "closest_final_m": [round(c, 3) for c in closest],
+        "mean_iters": round(float(I.mean()), 2), "point_iterations": int(prof[2]),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "k_loc",
+                     "bytes_per_launch": round(bytes_per_launch, 1)},
+        "map_build": {"points": n_map, "voxels": V, "seconds": round(t_map, 4),
+                      "points_per_s": round(n_map / t_map, 1), "algorithmic_GBps": round(map_bytes / t_map / 1e9, 2)},
+        "gen_seconds": round(t_gen, 2),
+    }
+    if args.cpu_frames > 0:
+        res["cpu_baseline"] = loc_cpu_baseline(W, hyps, args.loc_cpu_hyps)
+    return res
+
+
+def loc_cpu_baseline(W, hyps, n: int):
+    """The oracle's loc_register (serial) on n hypotheses spread over the queries, extrapolated linearly
+    (hypotheses are independent)."""
+    oracle = _oracle()
+    om = oracle.OracleMap(1 << 22)
+    om.insert(W.map_xyz, np.zeros(len(W.map_xyz), np.uint32), np.eye(4), LOC_VOXEL, max_range=1e30)
+    Q = len(W.scans)
+    t0 = time.perf_counter()
+    for j in range(n):
+        q = j % Q
+        oracle.loc_register(om, W.scans[q], hyps[q, j // Q], LOC_VOXEL, iters=10)
+    el = time.perf_counter() - t0
+    return {"value": round(n / el, 3), "unit": "hypotheses/s", "cores": 1, "kind": "port",
+            "sample": f"{n} hypotheses ({n // Q} per query) through the oracle's serial loc_register, "
+                      f"extrapolated linearly", "seconds": round(el, 3)}
+
+
+def copy_probe(torch, dev, mib: int):
+    """A plain device-to-device copy of known size, for calibrating hardware byte counters against."""
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
 
 
 def main():
@@ -88,10 +213,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=29, help="untimed frames before the timed region")
     ap.add_argument("--cpu-frames", type=int, default=150, help="frames of the CPU baseline sample (0: skip)")
     ap.add_argument("--acc", default="fp64", choices=["fp64", "kahan32", "fp32"])
+    ap.add_argument("--no-loc", action="store_true", help="skip the batched localisation leg")
+    ap.add_argument("--loc-queries", type=int, default=4)
+    ap.add_argument("--loc-hyps", type=int, default=4096)
+    ap.add_argument("--loc-reps", type=int, default=3)
+    ap.add_argument("--loc-cpu-hyps", type=int, default=64)
+    ap.add_argument("--copy-probe", type=int, default=0, help="MiB of a calibration copy at the end (0: none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the odometry is a sequential "
-              f"per-frame pipeline; see DESIGN.md for how independent sequences would shard)", file=sys.stderr)
+              f"per-frame pipeline; independent sequences and hypothesis batches would shard one per GPU)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -140,6 +272,7 @@ def main():
     timed = len(odo.traj) - f0
     prof = np.zeros(8)
     lw._abi.check(L.lw_prof_read(odo._ctx, lw._abi.ptr(prof, lw._abi._f64p)), "lw_prof_read")
+    lw._abi.check(L.lw_prof_enable(odo._ctx, 0), "lw_prof_enable")
     fps = timed / el
     launch_s = prof[1] * 1e-3 / max(prof[0], 1.0)
     bytes_per_launch = accumulation_bytes(prof) / max(prof[0], 1.0)
@@ -150,9 +283,9 @@ def main():
         "value": round(fps, 2),
         "unit": "frames/s",
         "n_gpus": 1,
-        "steps": timed,
+        "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(el * 1e3 / max(timed, 1), 4),
+        "ms_per_step": round(el * 1e3 / max(args.steps, 1), 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -160,23 +293,29 @@ def main():
         "data": "synthetic",
         "config": {"workload": "plain odometry sequence: 600 frames x 10k rose-curve rosette points, office "
                                "20x15x3 m, 0.2 m voxels, 2^22-slot map, <=20 GN iterations",
-                   "seed": SEED, "points_per_frame": per, "frames": n_frames, "acc_mode": args.acc},
+                   "seed": SEED, "points_per_frame": per, "frames_generated": n_frames, "frames_timed": timed,
+                   "acc_mode": args.acc},
         "points_per_s": round(fps * per, 1),
         "drift": dr,
         "map_voxels": odo.stats["map_voxels"],
-        "accumulation": {"launches": int(prof[0]), "working_launches": int(prof[6]),
+        "registration": {"launches": int(prof[0]), "passes": int(prof[6]),
                          "avg_launch_us": round(launch_s * 1e6, 3), "point_iterations": int(prof[2]),
                          "lookups": int(prof[3]), "hits": int(prof[4]), "matched": int(prof[5]),
                          "time_share": round(prof[1] * 1e-3 / el, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-                     "kernel": "k_icp<0> (accumulation pass)", "bytes_per_launch": round(bytes_per_launch, 1)},
+                     "kernel": "k_register<0> (one launch per registration: every accumulation pass and step)",
+                     "bytes_per_launch": round(bytes_per_launch, 1)},
         "gen_seconds": round(t_gen, 2),
     }
+    if not args.no_loc:
+        out["localisation"] = localisation(args, torch, lw, dev)
     if args.cpu_frames > 0:
         out["cpu_baseline"] = cpu_baseline(frames, seq, args.warmup, args.cpu_frames)
     else:
         out["cpu_baseline"] = None
+    if args.copy_probe > 0:
+        copy_probe(torch, dev, args.copy_probe)
     print(json.dumps(out), flush=True)
 
 

@@ -478,6 +478,7 @@ int lw_loc_batch(lw_ctx *c, const float *scans, const int32_t *scan_off, const i
     L.out_score = out_score;
     L.out_count = out_count;
     L.out_iters = out_iters;
+    L.prof = c->prof_on ? c->prof_dev : nullptr;
     L.Q = n_queries;
     L.H = n_hyp;
     L.iters = p->iters;

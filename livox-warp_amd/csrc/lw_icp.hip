@@ -3,10 +3,11 @@
 //
 // One pass: each thinned point gets S lanes (S = 2 for the 27-voxel search, 4 for the 125-voxel one); the
 // lanes split the point's neighbours, in the reference's order (dx outer, dz inner; odom.py:315-332), into
-// contiguous parts. A lane searches its part in chunks: the bitmap words of a whole chunk are loaded before
-// any is tested, then the chunk's slots (32 B records that carry key, voxel id, count and mean), so a chunk
-// costs two dependent memory round trips and a lane one or two chunks. The lanes then merge their closest
-// means by (distance, position in the order), which is exactly the serial loop's strict-< choice. The
+// contiguous parts. A lane searches its part in chunks (lw_search.h): the bitmap words of a whole chunk are
+// loaded before any is tested, then the chunk's slots (32 B records that carry key, voxel id, count and
+// mean), and the keys whose home slot holds another key continue their probes together, so a chunk costs two
+// dependent memory round trips plus two per probe depth, and a lane one or two chunks. The lanes then merge
+// their closest means by (distance, position in the order), which is exactly the serial loop's choice. The
 // first lane of the point forms the 53 float32 terms of the ACC layout as the reference rounds them
 // (odom.py:246-259, 269-271, 372-375).
 //
@@ -24,6 +25,7 @@
 // Accumulation forms (acc_mode): 0 float64 (the reference), 1 compensated float32 (Neumaier sums, two-sum
 // merges), 2 naive float32.
 #include "lw_internal.h"
+#include "lw_search.h"
 
 namespace lw {
 
@@ -182,29 +184,6 @@ struct Pass {
     X(1, 5, 10) X(2, 2, 11) X(2, 3, 12) X(2, 4, 13) X(2, 5, 14) X(3, 3, 15) X(3, 4, 16) X(3, 5, 17) X(4, 4, 18)   \
     X(4, 5, 19) X(5, 5, 20)
 
-// A slot as two 16-byte loads: lo = {key lo, key hi, voxel id, count bits}, hi = {mean x, y, z, pad}.
-__device__ inline void load_slot(const Slot *slots, int32_t s, int4 &lo, float4 &hi) {
-    const int4 *p = reinterpret_cast<const int4 *>(slots + s);
-    lo = p[0];
-    hi = reinterpret_cast<const float4 *>(p)[1];
-}
-
-__device__ inline int64_t key_of(const int4 &lo) {
-    return (int64_t)(((uint64_t)(uint32_t)lo.y << 32) | (uint64_t)(uint32_t)lo.x);
-}
-
-// The rest of the linear probe after a first slot that holds another key (rare at the loads the map runs
-// at): gpu.py:254-265 from probe 1 on. Returns the slot or -1.
-__device__ __noinline__ int32_t probe_on(const Slot *slots, const uint32_t *occ, int32_t mask, int32_t h,
-                                         int64_t key) {
-    for (int probe = 1; probe < PROBES; ++probe) {
-        int32_t t = (h + probe) & mask;
-        if (!((occ[t >> 5] >> (t & 31)) & 1u)) return -1;
-        if (slots[t].key == key) return t;
-    }
-    return -1;
-}
-
 // Lanes per point and chunk size per reach: 27 neighbours as 14 + 13 on two lanes; 125 as 32+32+32+29 on
 // four lanes, each in two chunks of 16.
 template <int R>
@@ -217,63 +196,6 @@ template <>
 struct Split<2> {
     static constexpr int S = 4, C = 16;
 };
-
-struct Cand {
-    float d2;
-    int32_t s;  // slot, -1 for none
-    int32_t j;  // position in the reference's neighbour order
-    float mx, my, mz;
-};
-
-// The closest mean over neighbours [j_lo, j_hi) of the reference's order (odom.py:308-334).
-template <int R, int C>
-__device__ inline void search_part(const IcpArgs &a, float wx, float wy, float wz, int j_lo, int j_hi, Cand &best,
-                                   int32_t &hits) {
-    constexpr int D = 2 * R + 1;
-    const int32_t bx = ifloor(wx * a.inv_v), by = ifloor(wy * a.inv_v), bz = ifloor(wz * a.inv_v);
-    for (int j0 = j_lo; j0 < j_hi; j0 += C) {
-        int64_t key[C];
-        int32_t h[C];
-        uint32_t w[C];
-        LW_UNROLL for (int c = 0; c < C; ++c) {
-            const int j = j0 + c;
-            const int dx = j / (D * D) - R, dy = (j / D) % D - R, dz = j % D - R;
-            key[c] = voxel_key_ijk(bx + dx, by + dy, bz + dz);
-            h[c] = slot_hash(key[c]) & a.mask;
-        }
-        LW_UNROLL for (int c = 0; c < C; ++c) w[c] = (j0 + c < j_hi) ? a.occ[h[c] >> 5] : 0u;
-        int4 lo[C];
-        float4 hi[C];
-        LW_UNROLL for (int c = 0; c < C; ++c) {
-            lo[c] = make_int4(-1, -1, -1, 0);
-            hi[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if ((w[c] >> (h[c] & 31)) & 1u) load_slot(a.slots, h[c], lo[c], hi[c]);
-        }
-        LW_UNROLL for (int c = 0; c < C; ++c) {
-            if (!((w[c] >> (h[c] & 31)) & 1u)) continue;
-            int32_t s = h[c];
-            if (key_of(lo[c]) != key[c]) {
-                s = probe_on(a.slots, a.occ, a.mask, h[c], key[c]);
-                if (s < 0) continue;
-                load_slot(a.slots, s, lo[c], hi[c]);
-            }
-            ++hits;
-            if (!(__int_as_float(lo[c].w) > 0.0f)) continue;  // count 0 (odom.py:324)
-            float d0 = wx - hi[c].x, d1 = wy - hi[c].y, d2 = wz - hi[c].z;
-            float dd = d0 * d0;
-            dd = dd + d1 * d1;
-            dd = dd + d2 * d2;
-            if (dd < best.d2) {
-                best.d2 = dd;
-                best.s = s;
-                best.j = j0 + c;
-                best.mx = hi[c].x;
-                best.my = hi[c].y;
-                best.mz = hi[c].z;
-            }
-        }
-    }
-}
 
 // The reference's terms of one matched point (odom.py:335-375) into acc.
 template <int MODE>
@@ -341,7 +263,8 @@ __device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf
             py = a.ds[3 * (int64_t)i + 1];
             pz = a.ds[3 * (int64_t)i + 2];
             transform_point(Tf, px, py, pz, wx, wy, wz);
-            search_part<R, C>(a, wx, wy, wz, sub * KQ, min(K, (sub + 1) * KQ), best, hits);
+            search_part<R, C>(Table{a.slots, a.occ, a.mask}, a.inv_v, wx, wy, wz, sub * KQ, min(K, (sub + 1) * KQ),
+                               best, hits);
         }
         // merge the S parts: smaller distance wins, a tie goes to the earlier neighbour (the serial strict <)
         LW_UNROLL for (int off = 1; off < S; off <<= 1) {

@@ -17,6 +17,7 @@ struct LocLaunch {
     double *out_score;        // Q * H: sum of robust weights of the last pass (device)
     int32_t *out_count;       // Q * H: matched points of the last pass (device)
     int32_t *out_iters;       // Q * H: steps taken (device)
+    unsigned long long *prof; // measurement counters, or null (part of a captured launch's identity)
     int32_t Q, H, iters, min_count, block, grid;
     float inv_v, max_d2;
     double damping;

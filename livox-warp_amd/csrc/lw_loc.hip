@@ -13,6 +13,7 @@
 // other, so the grid needs no co-residency; the grid size and block size are tuning parameters.
 #include "lw_internal.h"
 #include "lw_loc.h"
+#include "lw_search.h"
 
 namespace lw {
 
@@ -40,70 +41,6 @@ struct LocArgs {
     double damping;
     float kappa[16];
 };
-
-// Closest voxel mean in the 27 neighbours (odom.py:308-334), chunked loads.
-__device__ inline int32_t loc_search(const LocArgs &a, float wx, float wy, float wz, float &mx, float &my, float &mz,
-                                     int32_t &hits) {
-    const int32_t bx = ifloor(wx * a.inv_v), by = ifloor(wy * a.inv_v), bz = ifloor(wz * a.inv_v);
-    float best_d2 = a.max_d2;
-    int32_t best_s = -1;
-    for (int j0 = 0; j0 < 27; j0 += LOC_C) {
-        int64_t key[LOC_C];
-        int32_t h[LOC_C];
-        uint32_t w[LOC_C];
-        LW_UNROLL for (int c = 0; c < LOC_C; ++c) {
-            const int j = j0 + c;
-            key[c] = voxel_key_ijk(bx + j / 9 - 1, by + (j / 3) % 3 - 1, bz + j % 3 - 1);
-            h[c] = slot_hash(key[c]) & a.mask;
-        }
-        LW_UNROLL for (int c = 0; c < LOC_C; ++c) w[c] = (j0 + c < 27) ? a.occ[h[c] >> 5] : 0u;
-        int4 lo[LOC_C];
-        float4 hi[LOC_C];
-        LW_UNROLL for (int c = 0; c < LOC_C; ++c) {
-            lo[c] = make_int4(-1, -1, -1, 0);
-            hi[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if ((w[c] >> (h[c] & 31)) & 1u) {
-                const int4 *p = reinterpret_cast<const int4 *>(a.slots + h[c]);
-                lo[c] = p[0];
-                hi[c] = reinterpret_cast<const float4 *>(p)[1];
-            }
-        }
-        LW_UNROLL for (int c = 0; c < LOC_C; ++c) {
-            if (!((w[c] >> (h[c] & 31)) & 1u)) continue;
-            int32_t s = h[c];
-            const int64_t k = (int64_t)(((uint64_t)(uint32_t)lo[c].y << 32) | (uint64_t)(uint32_t)lo[c].x);
-            if (k != key[c]) {  // the rest of the linear probe (gpu.py:254-265)
-                s = -1;
-                for (int probe = 1; probe < PROBES; ++probe) {
-                    int32_t t = (h[c] + probe) & a.mask;
-                    if (!((a.occ[t >> 5] >> (t & 31)) & 1u)) break;
-                    if (a.slots[t].key == key[c]) {
-                        s = t;
-                        break;
-                    }
-                }
-                if (s < 0) continue;
-                const int4 *p = reinterpret_cast<const int4 *>(a.slots + s);
-                lo[c] = p[0];
-                hi[c] = reinterpret_cast<const float4 *>(p)[1];
-            }
-            ++hits;
-            if (!(__int_as_float(lo[c].w) > 0.0f)) continue;
-            float d0 = wx - hi[c].x, d1 = wy - hi[c].y, d2 = wz - hi[c].z;
-            float dd = d0 * d0;
-            dd = dd + d1 * d1;
-            dd = dd + d2 * d2;
-            if (dd < best_d2) {
-                best_d2 = dd;
-                best_s = s;
-                mx = hi[c].x;
-                my = hi[c].y;
-                mz = hi[c].z;
-            }
-        }
-    }
-    return best_s;
-}
 
 #define LW_H_PAIRS(X) \
     X(0, 0, 0) X(0, 1, 1) X(0, 2, 2) X(0, 3, 3) X(0, 4, 4) X(0, 5, 5) X(1, 1, 6) X(1, 2, 7) X(1, 3, 8) X(1, 4, 9) \
@@ -165,10 +102,13 @@ __global__ __launch_bounds__(B) void k_loc(LocArgs a) {
             LW_UNROLL for (int k = 0; k < LOC_ACC; ++k) acc[k] = 0.0;
             for (int32_t i = threadIdx.x; i < n; i += B) {
                 const float *pp = a.scans + 3 * (int64_t)(p0 + i);
-                float wx, wy, wz, mx = 0.0f, my = 0.0f, mz = 0.0f;
+                float wx, wy, wz;
                 transform_point(Tf, pp[0], pp[1], pp[2], wx, wy, wz);
-                const int32_t s = loc_search(a, wx, wy, wz, mx, my, mz, hits);
+                Cand best{a.max_d2, -1, 27, 0.0f, 0.0f, 0.0f};
+                search_part<1, LOC_C>(Table{a.slots, a.occ, a.mask}, a.inv_v, wx, wy, wz, 0, 27, best, hits);
+                const int32_t s = best.s;
                 if (s < 0) continue;
+                const float mx = best.mx, my = best.my, mz = best.mz;
                 ++matched_total;
                 const VoxPrec P = a.vox_prec[a.slots[s].vid];
                 const float M[3][3] = {{P.M[0], P.M[1], P.M[2]}, {P.M[1], P.M[3], P.M[4]}, {P.M[2], P.M[4], P.M[5]}};
@@ -263,7 +203,7 @@ hipError_t launch_loc(lw_ctx *c, const LocLaunch &L) {
     a.occ = c->occ;
     a.vox_prec = c->vox_prec;
     a.queue = c->loc_queue;
-    a.prof = c->prof_on ? c->prof_dev : nullptr;
+    a.prof = L.prof;
     a.Q = L.Q;
     a.H = L.H;
     a.mask = c->mask;

@@ -1,0 +1,143 @@
+// The closest-mean neighbour search shared by the odometry accumulation (lw_icp.hip) and the batched
+// localisation (lw_loc.hip): odom.py:308-334 with find_slot (gpu.py:254-265).
+//
+// A lane resolves a chunk of C neighbour keys together. Round 0 loads the bitmap words of every home slot
+// (slot_hash & mask) and then every occupied home slot's 32 B record; a record holding the key is a hit. Keys
+// whose home slot holds another key continue the linear probe, all of them together: each probe depth is one
+// round of bitmap loads and one of key loads for every key still unresolved, so a wave pays one pair of round
+// trips per depth, not one per key (with 64 lanes, some lane almost always needs a probe somewhere in a
+// chunk). A key ends at its slot (hit) or at the first empty slot (miss), within 64 probes.
+//
+// The closest mean is chosen by (squared distance, position in the reference's neighbour order), so the
+// order in which candidates are evaluated does not matter: the result is the serial loop's strict-< choice.
+#pragma once
+
+#include "lw_common.h"
+#include "lw_gn.h"
+
+namespace lw {
+
+struct Cand {
+    float d2;
+    int32_t s;  // slot, -1 for none
+    int32_t j;  // position in the reference's neighbour order
+    float mx, my, mz;
+};
+
+struct Table {
+    const Slot *slots;
+    const uint32_t *occ;  // one bit per slot
+    int32_t mask;
+};
+
+// A slot as two 16-byte loads: lo = {key lo, key hi, voxel id, count bits}, hi = {mean x, y, z, pad}.
+__device__ inline void load_slot(const Slot *slots, int32_t s, int4 &lo, float4 &hi) {
+    const int4 *p = reinterpret_cast<const int4 *>(slots + s);
+    lo = p[0];
+    hi = reinterpret_cast<const float4 *>(p)[1];
+}
+
+__device__ inline int64_t key_of(const int4 &lo) {
+    return (int64_t)(((uint64_t)(uint32_t)lo.y << 32) | (uint64_t)(uint32_t)lo.x);
+}
+
+__device__ inline bool bit_of(uint32_t word, int32_t s) { return ((word >> (s & 31)) & 1u) != 0u; }
+
+// One candidate: skip count 0 (odom.py:324), then keep it if it is closer, or as close and earlier.
+__device__ inline void consider(float wx, float wy, float wz, int32_t s, int32_t j, const int4 &lo, const float4 &hi,
+                                Cand &best) {
+    if (!(__int_as_float(lo.w) > 0.0f)) return;
+    float d0 = wx - hi.x, d1 = wy - hi.y, d2 = wz - hi.z;
+    float dd = d0 * d0;
+    dd = dd + d1 * d1;
+    dd = dd + d2 * d2;
+    // the serial loop starts from best_d2 = max_d2 and takes strictly smaller distances (odom.py:326-330)
+    const bool take = best.s < 0 ? dd < best.d2 : (dd < best.d2 || (dd == best.d2 && j < best.j));
+    if (take) {
+        best.d2 = dd;
+        best.s = s;
+        best.j = j;
+        best.mx = hi.x;
+        best.my = hi.y;
+        best.mz = hi.z;
+    }
+}
+
+// The closest mean over neighbours [j_lo, j_hi) of the (2R+1)^3 block around the point's voxel (dx outer, dz
+// inner), in chunks of C. hits counts the occupied neighbours found.
+template <int R, int C>
+__device__ inline void search_part(const Table &t, float inv_v, float wx, float wy, float wz, int j_lo, int j_hi,
+                                   Cand &best, int32_t &hits) {
+    constexpr int D = 2 * R + 1;
+    static_assert(C <= 32, "one bit per key of a chunk");
+    const int32_t bx = ifloor(wx * inv_v), by = ifloor(wy * inv_v), bz = ifloor(wz * inv_v);
+    for (int j0 = j_lo; j0 < j_hi; j0 += C) {
+        int64_t key[C];
+        int32_t h[C];
+        uint32_t w[C];
+        LW_UNROLL for (int c = 0; c < C; ++c) {
+            const int j = j0 + c;
+            const int dx = j / (D * D) - R, dy = (j / D) % D - R, dz = j % D - R;
+            key[c] = voxel_key_ijk(bx + dx, by + dy, bz + dz);
+            h[c] = slot_hash(key[c]) & t.mask;
+        }
+        LW_UNROLL for (int c = 0; c < C; ++c) w[c] = (j0 + c < j_hi) ? t.occ[h[c] >> 5] : 0u;
+        // round 0: the home slots
+        uint32_t need = 0u;
+        {
+            int4 lo[C];
+            float4 hi[C];
+            LW_UNROLL for (int c = 0; c < C; ++c) {
+                lo[c] = make_int4(-1, -1, -1, 0);
+                hi[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (bit_of(w[c], h[c])) load_slot(t.slots, h[c], lo[c], hi[c]);
+            }
+            LW_UNROLL for (int c = 0; c < C; ++c) {
+                if (!bit_of(w[c], h[c])) continue;
+                if (key_of(lo[c]) != key[c]) {
+                    need |= 1u << c;
+                    continue;
+                }
+                ++hits;
+                consider(wx, wy, wz, h[c], j0 + c, lo[c], hi[c], best);
+            }
+        }
+        if (need == 0u) continue;
+        // the rest of the linear probes, every unresolved key of the chunk at once
+        uint32_t found = 0u;
+        for (int probe = 1; need != 0u && probe < PROBES; ++probe) {
+            uint32_t wv[C];
+            LW_UNROLL for (int c = 0; c < C; ++c) wv[c] = ((need >> c) & 1u) ? t.occ[((h[c] + probe) & t.mask) >> 5] : 0u;
+            int64_t kk[C];
+            LW_UNROLL for (int c = 0; c < C; ++c) {
+                kk[c] = -1;
+                if (!((need >> c) & 1u)) continue;
+                const int32_t s = (h[c] + probe) & t.mask;
+                if (bit_of(wv[c], s))
+                    kk[c] = t.slots[s].key;
+                else
+                    need &= ~(1u << c);  // an empty slot ends the probe: not in the map
+            }
+            LW_UNROLL for (int c = 0; c < C; ++c) {
+                if (((need >> c) & 1u) && kk[c] == key[c]) {
+                    need &= ~(1u << c);
+                    found |= 1u << c;
+                    h[c] = (h[c] + probe) & t.mask;
+                }
+            }
+        }
+        if (found == 0u) continue;
+        int4 lo[C];
+        float4 hi[C];
+        LW_UNROLL for (int c = 0; c < C; ++c) {
+            if ((found >> c) & 1u) load_slot(t.slots, h[c], lo[c], hi[c]);
+        }
+        LW_UNROLL for (int c = 0; c < C; ++c) {
+            if (!((found >> c) & 1u)) continue;
+            ++hits;
+            consider(wx, wy, wz, h[c], j0 + c, lo[c], hi[c], best);
+        }
+    }
+}
+
+}  // namespace lw

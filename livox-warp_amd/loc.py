@@ -29,7 +29,7 @@ def _torch():
 LocParams = _abi.LocParams
 
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "livox-warp_amd", "loc_tiles.json")
-TILE_CANDIDATES = [(b, g) for b in (64, 128, 256) for g in (2, 4, 8)]  # (threads, workgroups per CU)
+TILE_CANDIDATES = [(b, g) for b in (64, 128, 256, 512) for g in (1, 2, 4, 8)]  # (threads, workgroups per CU)
 
 
 class BatchLocalizer:
