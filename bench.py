@@ -45,6 +45,16 @@ def accumulation_bytes(prof) -> float:
     return 12.0 * pt_iters + 8.0 * lookups + 16.0 * hits + 36.0 * matched
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of a kernel, from the committed hardware-counter summary (None if not profiled)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            e = json.load(f).get(kernel)
+        return None if e is None else float(e["fetch_bytes"] + e["write_bytes"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -120,7 +130,7 @@ def localisation(args, torch, lw, dev):
     t_gen = time.perf_counter() - t_gen
     X = torch.from_numpy(W.map_xyz).to(dev)
     torch.cuda.synchronize()
-    loc = lw.loc.BatchLocalizer(X, voxel=LOC_VOXEL, reg_slots=1 << 22, chunk=1 << 22, device=dev)
+    loc = lw.loc.BatchLocalizer(X, voxel=LOC_VOXEL, reg_slots=1 << args.loc_slots, chunk=1 << 22, device=dev)
     # map build, timed on its own: clear, then the bulk insert of every map point (in index order)
     loc.eng.sync()
     t0 = time.perf_counter()
@@ -160,7 +170,7 @@ def localisation(args, torch, lw, dev):
     res = {
         "value": round(n_units * args.loc_reps / el, 1), "unit": "hypotheses/s",
         "workload": f"{len(W.scans)} queries x {H} hypotheses, scans {[len(s) for s in W.scans]} points, map "
-                    f"{n_map} points at 20 mm -> {V} voxels of 0.1 m, 10 GN iterations at reach 1",
+                    f"{n_map} points at 20 mm -> {V} voxels of 0.1 m in 2^{args.loc_slots} slots, 10 GN iterations at reach 1",
         "seed": LOC_SEED, "tile": {"block": tile[0], "workgroups_per_cu": tile[1]}, "autotune_ms": table,
         "launch_ms": round(launch_s * 1e3, 3), "graph": True,
         "top1_recovered": sum(1 for e in best if e[0] < 0.3 and e[1] < 3.0), "queries": len(W.scans),
@@ -218,6 +228,7 @@ def main():
     ap.add_argument("--loc-hyps", type=int, default=4096)
     ap.add_argument("--loc-reps", type=int, default=3)
     ap.add_argument("--loc-cpu-hyps", type=int, default=64)
+    ap.add_argument("--loc-slots", type=int, default=24, help="log2 of the localisation map's hash slots")
     ap.add_argument("--copy-probe", type=int, default=0, help="MiB of a calibration copy at the end (0: none)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -303,7 +314,7 @@ def main():
                          "lookups": int(prof[3]), "hits": int(prof[4]), "matched": int(prof[5]),
                          "time_share": round(prof[1] * 1e-3 / el, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic("k_register<0>"),
                      "kernel": "k_register<0> (one launch per registration: every accumulation pass and step)",
                      "bytes_per_launch": round(bytes_per_launch, 1)},
         "gen_seconds": round(t_gen, 2),

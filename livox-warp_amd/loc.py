@@ -36,7 +36,7 @@ class BatchLocalizer:
     """A prior map (built once, in index order, like prior_map.voxel_downsample's bulk insert) and batched
     hypothesis registration against it."""
 
-    def __init__(self, map_xyz, voxel: float = 0.1, reg_slots: int = 1 << 23, chunk: int = 1 << 22, device=None,
+    def __init__(self, map_xyz, voxel: float = 0.1, reg_slots: int = 1 << 24, chunk: int = 1 << 22, device=None,
                  stream=None, iters: int = 10, noise: float = 0.02, alpha: float = 0.05):
         torch = _torch()
         self.voxel = voxel
