@@ -60,6 +60,8 @@ typedef struct lw_reg_result {
     int32_t n_ds;
     int32_t degen;
     int32_t weak;
+    double rms0; /* the residual of the first pass, at the prediction (no counterpart in the reference: the
+                    learned prior's fallback test) */
 } lw_reg_result;
 
 /* Library version (major * 10000 + minor * 100 + patch). */
@@ -160,6 +162,38 @@ typedef struct lw_loc_params {
 int lw_loc_batch(lw_ctx *ctx, const float *scans, const int32_t *scan_off, const int32_t *order, int32_t n_queries,
                  const double *hyp_T, int32_t n_hyp, const lw_loc_params *p, double *out_T, double *out_score,
                  int32_t *out_count, int32_t *out_iters);
+
+/* ---- Learned pose-increment prior. No counterpart in the reference: it replaces the constant-velocity guess
+ * of Odometry.predict (odom.py:591-592, used by _estimate at odom.py:666) when enabled, and plain ICP from that
+ * guess stays the fallback. A 152 -> 256 -> 256 -> 256 -> 6 MLP with GELU, BF16 weights on the matrix cores
+ * with FP32 accumulation, trained by the forward / loss / backward / Adam entry points below. Its parameter
+ * vector is lw_mlp_param_count() float32s: W1 160x256, b1 256, W2 256x256, b2 256, W3 256x256, b3 256,
+ * W4 256x16, b4 16 (row-major [in][out]; W1 rows 152..159 and W4 columns 6..15 stay zero). */
+typedef struct lw_mlp lw_mlp;
+
+int lw_mlp_param_count(void);
+/* Device buffers for batches of up to batch_cap rows; work is queued on stream (NULL: the null stream). */
+int lw_mlp_create(lw_mlp **out, int32_t batch_cap, void *stream);
+int lw_mlp_destroy(lw_mlp *m);
+int lw_mlp_set_stream(lw_mlp *m, void *stream);
+/* Parameters from / to host memory (float32, the layout above); setting them clears the Adam moments. */
+int lw_mlp_set_params(lw_mlp *m, const float *host);
+int lw_mlp_get_params(lw_mlp *m, float *host);
+/* The gradient of the last backward pass (host, the parameter layout). */
+int lw_mlp_get_grads(lw_mlp *m, float *host);
+/* y (n x 6 float32, device) = MLP(x (n x 152 float32, device, row stride ldx)); keeps the activations. */
+int lw_mlp_forward(lw_mlp *m, const float *x, int64_t ldx, int32_t n, float *y);
+/* *loss (device float) = mean of (y - target)^2 over the n x 6 entries; keeps d loss / d y for the backward. */
+int lw_mlp_loss(lw_mlp *m, const float *y, const float *target, int32_t n, float *loss);
+/* Gradients of the loss with respect to every parameter, through the last forward (same n). */
+int lw_mlp_backward(lw_mlp *m, int32_t n);
+/* One Adam step (bias-corrected; step counts from 1) on the float32 masters, refreshing the BF16 copy. */
+int lw_mlp_adam(lw_mlp *m, float lr, float beta1, float beta2, float eps, int32_t step);
+
+/* The 152 inputs of the open frame's m points (the bound f_raw buffer, sensor frame) into out (device):
+ * motion[24] as given, then 64 range bins over [0, r_max) and 64 elevation bins over [-e_max, e_max), each
+ * count * 64 / m. */
+int lw_frame_features(lw_ctx *ctx, int32_t m, float r_max, float e_max, const float *motion, float *out);
 
 /* Live measurement of the accumulation launches (for throughput and roofline reports): lw_prof_enable
  * turns on HIP events around each accumulation launch and device counters; lw_prof_read returns, since

@@ -6,12 +6,13 @@ Import with importlib (the directory name carries a hyphen):
     odo = lw.Odometry(lw.PoseTable())
 
 Submodules: odom (the drop-in Odometry), engine (one context driven directly), pose_table (the pose sink),
-synth (seeded synthetic scenes, scans and walks), _abi (ctypes bindings of liblwamd.so).
+loc (batched global localisation), prior (the learned pose-increment prior), synth (seeded synthetic scenes,
+scans and walks), _abi (ctypes bindings of liblwamd.so).
 """
 
-from . import _abi, engine, loc, odom, pose_table, synth  # noqa: F401
+from . import _abi, engine, loc, odom, pose_table, prior, synth  # noqa: F401
 from .engine import Engine  # noqa: F401
 from .odom import Odometry  # noqa: F401
 from .pose_table import PoseTable  # noqa: F401
 
-__all__ = ["Odometry", "Engine", "PoseTable", "odom", "engine", "pose_table", "synth"]
+__all__ = ["Odometry", "Engine", "PoseTable", "odom", "engine", "loc", "pose_table", "prior", "synth"]

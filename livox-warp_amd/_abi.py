@@ -35,7 +35,7 @@ class RegParams(ctypes.Structure):
 
 class RegResult(ctypes.Structure):
     _fields_ = [("T", _f64 * 16), ("rms", _f64), ("cond", _f64), ("ok", _i32), ("iters", _i32), ("corr", _i32),
-                ("n_ds", _i32), ("degen", _i32), ("weak", _i32)]
+                ("n_ds", _i32), ("degen", _i32), ("weak", _i32), ("rms0", _f64)]
 
 
 class LocParams(ctypes.Structure):
@@ -68,6 +68,18 @@ SIGNATURES = {
     "lw_register_thinned": (_i32, [_vp, _i32, ctypes.POINTER(RegParams), _f64p, ctypes.POINTER(RegResult)]),
     "lw_set_persistent": (_i32, [_vp, _i32]),
     "lw_loc_batch": (_i32, [_vp, _vp, _vp, _vp, _i32, _vp, _i32, ctypes.POINTER(LocParams), _vp, _vp, _vp, _vp]),
+    "lw_mlp_param_count": (_i32, []),
+    "lw_mlp_create": (_i32, [ctypes.POINTER(_vp), _i32, _vp]),
+    "lw_mlp_destroy": (_i32, [_vp]),
+    "lw_mlp_set_stream": (_i32, [_vp, _vp]),
+    "lw_mlp_set_params": (_i32, [_vp, _f32p]),
+    "lw_mlp_get_params": (_i32, [_vp, _f32p]),
+    "lw_mlp_get_grads": (_i32, [_vp, _f32p]),
+    "lw_mlp_forward": (_i32, [_vp, _vp, _i64, _i32, _vp]),
+    "lw_mlp_loss": (_i32, [_vp, _vp, _vp, _i32, _vp]),
+    "lw_mlp_backward": (_i32, [_vp, _i32]),
+    "lw_mlp_adam": (_i32, [_vp, _f32, _f32, _f32, _f32, _i32]),
+    "lw_frame_features": (_i32, [_vp, _i32, _f32, _f32, _f32p, _vp]),
     "lw_prof_enable": (_i32, [_vp, _i32]),
     "lw_prof_reset": (_i32, [_vp]),
     "lw_prof_read": (_i32, [_vp, _f64p]),

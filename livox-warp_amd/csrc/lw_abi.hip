@@ -9,6 +9,7 @@
 #include "../../include/livox_warp_amd.h"
 #include "lw_internal.h"
 #include "lw_loc.h"
+#include "lw_mlp.h"
 #include "lw_prec.h"
 
 #define LW_CK(x)                                        \
@@ -435,6 +436,7 @@ int run_registration(lw_ctx *c, const int32_t *n_dev, int32_t n_bound, int32_t n
     st.cond = st.corr >= p->min_corr ? lw::gn::cond_of(st.H) : 1.0;
     std::memcpy(res->T, st.T, 16 * sizeof(double));
     res->rms = st.rms;
+    res->rms0 = st.rms0;
     res->cond = st.cond;
     res->corr = st.corr;
     res->iters = st.iters;
@@ -524,6 +526,17 @@ int lw_loc_batch(lw_ctx *c, const float *scans, const int32_t *scan_off, const i
         LW_CK(hipEventRecord(c->prof_ev[2 * c->prof_used + 1], c->stream));
         c->prof_used++;
     }
+    return 0;
+}
+
+int lw_frame_features(lw_ctx *c, int32_t m, float r_max, float e_max, const float *motion, float *out) {
+    if (!c || !motion || !out || m < 0 || m > c->frame_cap || !(r_max > 0.0f) || !(e_max > 0.0f)) return LW_EINVAL;
+    if (!c->f_raw) return LW_ENOTBOUND;
+    lw::FeatArgs f{};
+    f.r_max = r_max;
+    f.e_max = e_max;
+    std::memcpy(f.motion, motion, sizeof(f.motion));
+    LW_CK(lw::launch_features(c->stream, c->f_raw, m, f, out));
     return 0;
 }
 

@@ -342,6 +342,7 @@ __device__ __noinline__ void gauss_newton(RegState *st, const double *v, const g
         return;
     }
     st->rms = o.rms;
+    if (it == 0) st->rms0 = o.rms;
     for (int k = 0; k < 36; ++k) st->H[k] = o.H[k];
     if (rc == gn::STEP_SINGULAR) {  // LinAlgError: stop (odom.py:791-792)
         st->done = 1;
@@ -510,6 +511,7 @@ __global__ void k_reg_begin(RegState *st, const double *Tp) {
     }
     for (int k = 0; k < 36; ++k) st->H[k] = (k % 7 == 0) ? 1.0 : 0.0;  // np.eye(6) (odom.py:763)
     st->rms = 0.0;
+    st->rms0 = 0.0;
     st->cond = 1.0;
     st->done = 0;
     st->iters = 0;

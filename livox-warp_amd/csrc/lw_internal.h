@@ -21,6 +21,7 @@ struct RegState {
     double H[36];       // H of the last accepted accumulation (for the condition number)
     double acc[64];     // last reduced accumulator (ACC layout)
     double rms;
+    double rms0;        // the residual of the first pass (at the prediction)
     double cond;
     float Tf[16];       // the current pose rounded to float32, as the accumulation kernels use it
     int32_t done;       // 1: converged or stopped, later passes return at once

@@ -132,6 +132,10 @@ class Odometry:
         self.epoch = 0
         self.acc_mode = "fp64"  # accumulation form: fp64 (reference), kahan32, fp32
         self.wide_iters = WIDE_ITERS
+        # learned pose-increment prior (prior.LearnedPrior) in place of the constant-velocity guess; None: off
+        self.prior = None
+        # running totals beside the reference's per-frame stats: registrations, Gauss-Newton steps, prior use
+        self.totals = {"registrations": 0, "gn_iters": 0, "prior_used": 0, "prior_fallbacks": 0}
 
         self.reset(np.eye(4, dtype=np.float64))
 
@@ -257,6 +261,8 @@ class Odometry:
             self._insert(m, T)
         if self.T_prev is not None and not warm:
             self._update_velocity(np.linalg.inv(self.T_prev) @ T)
+            if self.prior is not None:
+                self.prior.observe(self.T_prev, T)
         self.T_prev = self.T = T
         self.pipe.set_pose(k, T, xi, self.epoch)
         self.traj.append((k, t_mid, T.copy()))
@@ -264,16 +270,31 @@ class Odometry:
         on_frame(k, T, self.f_xyz, self.f_attr, self.f_t, self.f_frame, m)
 
     def _estimate(self, m: int, t_mid: float, warm: bool):
-        xi = self.xi if (self.deskew and not warm) else np.zeros(6, np.float32)
+        guess = None
+        if self.prior is not None and not warm and m >= self.min_points:
+            guess = self.prior.guess(self, m)  # (T, deskew twist) from the learned increment, or None
+        if guess is not None:
+            T, xi = guess[0], (guess[1] if self.deskew else np.zeros(6, np.float32))
+        else:
+            xi = self.xi if (self.deskew and not warm) else np.zeros(6, np.float32)
+            T = self.T if warm else self.predict()
         if m > 0:
             self._deskew(m, t_mid, xi)
-        T = self.T if warm else self.predict()
         if m < self.min_points:
             self.stats["skipped"] += 1
             return T, xi, False
         if warm:
             return T, xi, True
         T, ok = self._register(m, T)
+        if guess is not None:
+            self.totals["prior_used"] += 1
+            if not self.prior.accept(float(self._result.rms0)):
+                # the learned start left the first residual above threshold: plain ICP from the constant-
+                # velocity guess decides the frame
+                self.totals["prior_fallbacks"] += 1
+                xi = self.xi if self.deskew else np.zeros(6, np.float32)
+                self._deskew(m, t_mid, xi)
+                T, ok = self._register(m, self.predict())
         if ok and self.deskew and self.redeskew and self.T_prev is not None:
             xi_new = _body_twist(np.linalg.inv(self.T_prev) @ T)
             if (np.linalg.norm(xi_new[3:] - xi[3:]) > REDESKEW_ROT
@@ -356,8 +377,10 @@ class Odometry:
         _abi.check(self._lib.lw_register(self._ctx, m, _abi.ctypes.byref(self.reg_params()), _abi.ptr(Tp, _abi._f64p),
                                          _abi.ctypes.byref(r)), "lw_register")
         self.stats["ds"] = int(r.n_ds)
+        self.totals["registrations"] += 1
         if r.n_ds < self.min_corr:
             return T_pred, False
+        self.totals["gn_iters"] += int(r.iters)
         T = np.array(r.T, dtype=np.float64).reshape(4, 4)
         self.stats.update(iters=int(r.iters), corr=int(r.corr), rms=float(r.rms), cond=float(r.cond),
                           weak=bool(r.weak), degen=int(r.degen))

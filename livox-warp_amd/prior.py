@@ -1,0 +1,330 @@
+"""The learned pose-increment prior: a BF16 MLP on the matrix cores (lw_mlp_*) that predicts each frame's pose
+increment from the last four increments and the frame's range / elevation histograms, in place of the
+constant-velocity guess of Odometry.predict (python/livox_warp/odom.py:591-592, used at odom.py:666).
+
+Plain ICP stays the judge: the registration starts from the learned guess, and when the first pass's residual
+at that start exceeds `fallback` times the running median of those residuals, the frame is registered again
+from the constant-velocity guess (Odometry._estimate).
+
+The network predicts the change of the increment: d_k = d_{k-1} + (y * sd_t + mu_t), with d the body twist
+(translation, rotation vector) between consecutive frame poses, as Odometry._update_velocity forms it. Its
+labels are the poses of the plain (constant-velocity) GPU odometry on synthetic sequences (`collect`), and it
+is trained by the library's own forward / backward / Adam kernels (`fit`).
+"""
+
+from __future__ import annotations
+
+import collections
+import ctypes
+import math
+
+import numpy as np
+
+from . import _abi
+
+MOTION, IN, OUT = 24, 152, 6
+R_MAX = 20.0                # range histogram over [0, 20) m
+E_MAX = math.radians(20.0)  # elevation histogram over [-20, 20) degrees
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def body_twist(d: np.ndarray) -> np.ndarray:
+    """(translation, rotation vector) of a 4x4 increment, float64."""
+    from scipy.spatial.transform import Rotation
+
+    return np.concatenate([d[:3, 3], Rotation.from_matrix(d[:3, :3]).as_rotvec()])
+
+
+def twist_matrix(xi: np.ndarray) -> np.ndarray:
+    from scipy.spatial.transform import Rotation
+
+    out = np.eye(4)
+    out[:3, :3] = Rotation.from_rotvec(np.asarray(xi[3:], np.float64)).as_matrix()
+    out[:3, 3] = xi[:3]
+    return out
+
+
+def init_params(rng, out_scale: float = 0.1) -> np.ndarray:
+    """He-normal hidden weights, zero biases, a small last layer; the padded rows / columns stay zero."""
+    P = _abi.lib().lw_mlp_param_count()
+    p = np.zeros(P, np.float32)
+    o = 0
+    for rows, cols, real_rows, real_cols, fan_in, scale in ((160, 256, IN, 256, IN, 2.0), (256, 256, 256, 256, 256, 2.0),
+                                                           (256, 256, 256, 256, 256, 2.0),
+                                                           (256, 16, 256, OUT, 256, out_scale ** 2)):
+        W = p[o:o + rows * cols].reshape(rows, cols)
+        W[:real_rows, :real_cols] = rng.normal(0.0, math.sqrt(scale / fan_in), (real_rows, real_cols))
+        o += rows * cols + cols  # weights, then the zero biases
+    assert o == P
+    return p
+
+
+class MLP:
+    """The network's device state (parameters, Adam moments, activations for up to batch_cap rows)."""
+
+    def __init__(self, batch_cap: int = 4096, device=None, stream=None):
+        torch = _torch()
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
+        self.lib = _abi.lib()
+        self.P = self.lib.lw_mlp_param_count()
+        self.cap = batch_cap
+        self.h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _abi.check(self.lib.lw_mlp_create(ctypes.byref(self.h), batch_cap, ctypes.c_void_p(self.stream.cuda_stream)),
+                       "lw_mlp_create")
+        self._y = torch.empty((batch_cap, OUT), dtype=torch.float32, device=self.device)
+        self._loss = torch.zeros(1, dtype=torch.float32, device=self.device)
+        self.steps = 0
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                self.lib.lw_mlp_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+    def set_stream(self, stream):
+        self.stream = stream
+        _abi.check(self.lib.lw_mlp_set_stream(self.h, ctypes.c_void_p(stream.cuda_stream)), "lw_mlp_set_stream")
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.shape == (self.P,)
+        _abi.check(self.lib.lw_mlp_set_params(self.h, _abi.ptr(p, _abi._f32p)), "lw_mlp_set_params")
+        self.steps = 0
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.P, np.float32)
+        _abi.check(self.lib.lw_mlp_get_params(self.h, _abi.ptr(out, _abi._f32p)), "lw_mlp_get_params")
+        return out
+
+    def grads(self) -> np.ndarray:
+        out = np.empty(self.P, np.float32)
+        _abi.check(self.lib.lw_mlp_get_grads(self.h, _abi.ptr(out, _abi._f32p)), "lw_mlp_get_grads")
+        return out
+
+    def forward(self, x):
+        """x: (n, 152) float32 device tensor (rows contiguous). Returns a view (n, 6) of a reused buffer."""
+        n = int(x.shape[0])
+        assert x.dtype == _torch().float32 and x.stride(1) == 1 and 0 < n <= self.cap
+        _abi.check(self.lib.lw_mlp_forward(self.h, x.data_ptr(), x.stride(0), n, self._y.data_ptr()), "lw_mlp_forward")
+        return self._y[:n]
+
+    def loss(self, y, t):
+        n = int(y.shape[0])
+        assert t.shape == (n, OUT) and t.is_contiguous()
+        _abi.check(self.lib.lw_mlp_loss(self.h, y.data_ptr(), t.data_ptr(), n, self._loss.data_ptr()), "lw_mlp_loss")
+        return self._loss
+
+    def backward(self, n: int):
+        _abi.check(self.lib.lw_mlp_backward(self.h, n), "lw_mlp_backward")
+
+    def adam(self, lr: float, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8):
+        self.steps += 1
+        _abi.check(self.lib.lw_mlp_adam(self.h, lr, beta1, beta2, eps, self.steps), "lw_mlp_adam")
+
+    def train_step(self, x, t, lr: float):
+        y = self.forward(x)
+        loss = self.loss(y, t)
+        self.backward(int(x.shape[0]))
+        self.adam(lr)
+        return loss
+
+
+class Normalizer:
+    """Scales of the motion inputs (per twist component, shared by the four lags) and of the targets."""
+
+    def __init__(self, mu_d, sd_d, mu_t, sd_t):
+        self.mu_d, self.sd_d = np.asarray(mu_d, np.float64), np.asarray(sd_d, np.float64)
+        self.mu_t, self.sd_t = np.asarray(mu_t, np.float64), np.asarray(sd_t, np.float64)
+
+    def motion(self, hist) -> np.ndarray:
+        """hist: the last increments, most recent first (up to 4; missing lags read as zero motion)."""
+        out = np.zeros(MOTION, np.float32)
+        for j, d in enumerate(list(hist)[:4]):
+            out[6 * j:6 * j + 6] = (np.asarray(d) - self.mu_d) / self.sd_d
+        for j in range(len(hist), 4):
+            out[6 * j:6 * j + 6] = -self.mu_d / self.sd_d
+        return out
+
+    def to_dict(self):
+        return {"mu_d": self.mu_d, "sd_d": self.sd_d, "mu_t": self.mu_t, "sd_t": self.sd_t}
+
+
+class LearnedPrior:
+    """Odometry.prior: the learned guess of each frame's pose, with the residual test that hands a frame back to
+    the constant-velocity guess."""
+
+    def __init__(self, params: np.ndarray, norm: Normalizer, odo, fallback: float = 2.0, window: int = 51,
+                 min_history: int = 10):
+        torch = _torch()
+        self.mlp = MLP(batch_cap=16, device=odo.device, stream=odo.stream)
+        self.mlp.set_params(params)
+        self.norm = norm
+        self.fallback = fallback
+        self.min_history = min_history
+        self.hist = collections.deque(maxlen=4)   # increments, most recent first
+        self.res = collections.deque(maxlen=window)
+        self.feat = torch.zeros((1, IN), dtype=torch.float32, device=odo.device)
+        self.last = None
+
+    def guess(self, odo, m: int):
+        if len(self.hist) < 4:
+            return None
+        mot = self.norm.motion(self.hist)
+        _abi.check(odo._lib.lw_frame_features(odo._ctx, m, R_MAX, E_MAX, _abi.ptr(mot, _abi._f32p),
+                                              self.feat.data_ptr()), "lw_frame_features")
+        y = self.mlp.forward(self.feat)
+        out = y.cpu().numpy()[0] if odo.stream == _torch().cuda.current_stream(odo.device) else self._read(odo, y)
+        d = self.hist[0] + out.astype(np.float64) * self.norm.sd_t + self.norm.mu_t
+        self.last = d
+        return odo.T @ twist_matrix(d), d.astype(np.float32)
+
+    @staticmethod
+    def _read(odo, y):
+        odo.synchronize()
+        return y.cpu().numpy()[0]
+
+    def accept(self, rms0: float) -> bool:
+        ok = len(self.res) < self.min_history or rms0 <= self.fallback * float(np.median(self.res))
+        self.res.append(rms0)
+        return ok
+
+    def observe(self, T_prev: np.ndarray, T: np.ndarray):
+        self.hist.appendleft(body_twist(np.linalg.inv(T_prev) @ T))
+
+
+class Recorder:
+    """Odometry.prior for collecting training data: computes each registered frame's inputs (on the device, as
+    LearnedPrior does) from the raw motion history, and leaves the guess to the constant-velocity model."""
+
+    def __init__(self, odo, capacity: int):
+        torch = _torch()
+        self.X = torch.zeros((capacity, IN), dtype=torch.float32, device=odo.device)
+        self.frames = []   # for each row: the index of the frame in odo.traj it belongs to
+        self.hist = collections.deque(maxlen=4)
+        self.unit = Normalizer(np.zeros(6), np.ones(6), np.zeros(6), np.ones(6))
+        self.odo = odo
+
+    def guess(self, odo, m: int):
+        if len(self.hist) == 4 and len(self.frames) < len(self.X):
+            mot = self.unit.motion(self.hist)  # raw increments; scaled after collection
+            row = self.X[len(self.frames)]
+            _abi.check(odo._lib.lw_frame_features(odo._ctx, m, R_MAX, E_MAX, _abi.ptr(mot, _abi._f32p),
+                                                  row.data_ptr()), "lw_frame_features")
+            self.frames.append(len(odo.traj))  # this frame's index once it is finalised
+        return None
+
+    def accept(self, rms0: float) -> bool:
+        return True
+
+    def observe(self, T_prev, T):
+        self.hist.appendleft(body_twist(np.linalg.inv(T_prev) @ T))
+
+
+def run_sequence(lw, seq, n_frames: int, device, prior_factory=None, reg_slots: int = 1 << 22, max_iter: int = 20,
+                 frames=None):
+    """The plain odometry over n_frames of seq (frames generated on the device unless given); prior_factory(odo)
+    optionally attaches a prior. Returns (odometry, seconds in the push loop)."""
+    import time
+
+    torch = _torch()
+    X, A, Tt = frames if frames is not None else seq.frames_device(n_frames, device)
+    odo = lw.Odometry(lw.PoseTable(), device, reg_slots=reg_slots, frame_cap=1 << 15)
+    odo.reg_voxel = 0.2
+    odo.max_iter = max_iter
+    odo.reset(seq.pose(0.05))
+    if prior_factory is not None:
+        odo.prior = prior_factory(odo)
+    per = X.shape[1]
+    spans = Tt[:, [0, -1]].cpu().numpy()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(n_frames):
+        odo.push(X[k], A[k], Tt[k], per, float(spans[k, 0]), float(spans[k, 1]), lambda *_: None)
+    odo.synchronize()
+    return odo, time.perf_counter() - t0
+
+
+def collect(lw, seeds, n_frames: int, device, seconds_margin: float = 1.0):
+    """Training rows from the plain GPU odometry on the sequences `seeds`: inputs (raw motion, histograms) and
+    targets d_k - d_{k-1} from the odometry's own poses. Returns (X (N, 152), Y (N, 6)) device tensors."""
+    torch = _torch()
+    xs, ys = [], []
+    for s in seeds:
+        seq = lw.synth.Sequence(seed=int(s), seconds=n_frames * 0.1 + seconds_margin)
+        rec = {}
+        odo, _ = run_sequence(lw, seq, n_frames, device, prior_factory=lambda o: rec.setdefault("r", Recorder(o, n_frames)))
+        r = rec["r"]
+        traj = [T for _, _, T in odo.traj]
+        rows, targets = [], []
+        for row, f in enumerate(r.frames):
+            if f < 2 or f >= len(traj):
+                continue
+            d_k = body_twist(np.linalg.inv(traj[f - 1]) @ traj[f])
+            d_p = body_twist(np.linalg.inv(traj[f - 2]) @ traj[f - 1])
+            rows.append(row)
+            targets.append(d_k - d_p)
+        if rows:
+            xs.append(r.X[torch.as_tensor(rows, device=r.X.device)])
+            ys.append(torch.as_tensor(np.array(targets), dtype=torch.float64, device=r.X.device))
+    return torch.cat(xs), torch.cat(ys)
+
+
+def make_normalizer(X, Y) -> Normalizer:
+    d = X[:, :MOTION].double().view(-1, 4, 6).reshape(-1, 6)
+    mu_d, sd_d = d.mean(0).cpu().numpy(), d.std(0).clamp_min(1e-6).cpu().numpy()
+    mu_t, sd_t = Y.mean(0).cpu().numpy(), Y.std(0).clamp_min(1e-9).cpu().numpy()
+    return Normalizer(mu_d, sd_d, mu_t, sd_t)
+
+
+def normalise(X, Y, norm: Normalizer):
+    torch = _torch()
+    Xn = X.clone()
+    mu = torch.as_tensor(np.tile(norm.mu_d, 4), dtype=torch.float32, device=X.device)
+    sd = torch.as_tensor(np.tile(norm.sd_d, 4), dtype=torch.float32, device=X.device)
+    Xn[:, :MOTION] = (X[:, :MOTION] - mu) / sd
+    Yn = ((Y - torch.as_tensor(norm.mu_t, device=Y.device)) / torch.as_tensor(norm.sd_t, device=Y.device))
+    return Xn.contiguous(), Yn.to(torch.float32).contiguous()
+
+
+def fit(Xn, Yn, epochs: int = 30, batch: int = 1024, lr: float = 1e-3, seed: int = 0, params=None, device=None):
+    """Train on (Xn, Yn) with the library's kernels (minibatch Adam, shuffled each epoch); returns (params,
+    per-epoch mean loss)."""
+    torch = _torch()
+    mlp = MLP(batch_cap=batch, device=device or Xn.device)
+    rng = np.random.default_rng(seed)
+    mlp.set_params(params if params is not None else init_params(rng))
+    n = int(Xn.shape[0])
+    hist = []
+    gen = torch.Generator(device=Xn.device)
+    gen.manual_seed(seed)
+    for ep in range(epochs):
+        perm = torch.randperm(n, device=Xn.device, generator=gen)
+        tot = torch.zeros(1, dtype=torch.float64, device=Xn.device)
+        nb = 0
+        for a in range(0, n - batch + 1, batch):
+            idx = perm[a:a + batch]
+            xb, yb = Xn[idx].contiguous(), Yn[idx].contiguous()
+            decay = 0.5 * (1.0 + math.cos(math.pi * ep / max(epochs, 1)))
+            tot += mlp.train_step(xb, yb, lr * (0.1 + 0.9 * decay)).double()
+            nb += 1
+        hist.append(float(tot.item()) / max(nb, 1))
+    return mlp.params(), hist
+
+
+def save(path: str, params: np.ndarray, norm: Normalizer, meta: dict | None = None):
+    np.savez_compressed(path, params=params, **norm.to_dict(), meta=np.array(repr(meta or {})))
+
+
+def load(path: str):
+    z = np.load(path, allow_pickle=False)
+    return z["params"], Normalizer(z["mu_d"], z["sd_d"], z["mu_t"], z["sd_t"])

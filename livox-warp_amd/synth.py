@@ -174,6 +174,41 @@ def cast(o: np.ndarray, d: np.ndarray, S: Scene, chunk: int = 8192):
     return t_out, r_out
 
 
+def cast_torch(o, d, S: Scene, chunk: int = 1 << 18):
+    """cast() on torch float64 tensors (any device): the same slab tests, for bulk synthetic data."""
+    import torch
+
+    dev = o.device
+    lo_r = torch.as_tensor(S.room_lo, dtype=torch.float64, device=dev)
+    hi_r = torch.as_tensor(S.room_hi, dtype=torch.float64, device=dev)
+    blo = torch.as_tensor(S.box_lo, dtype=torch.float64, device=dev)
+    bhi = torch.as_tensor(S.box_hi, dtype=torch.float64, device=dev)
+    brefl = torch.as_tensor(S.refl, dtype=torch.float64, device=dev)
+    t_out = torch.empty(len(d), dtype=torch.float64, device=dev)
+    r_out = torch.empty(len(d), dtype=torch.float64, device=dev)
+    for a in range(0, len(d), chunk):
+        oo, dd = o[a:a + chunk], d[a:a + chunk]
+        inv = 1.0 / torch.where(dd.abs() < 1e-12, torch.full_like(dd, 1e-12), dd)
+        ta, tb = (lo_r - oo) * inv, (hi_r - oo) * inv
+        far = torch.maximum(ta, tb)
+        best, wall = far.min(dim=1)
+        refl = 20.0 + 8.0 * wall.to(torch.float64)
+        if len(S.box_lo):
+            t1 = (blo[None] - oo[:, None]) * inv[:, None]
+            t2 = (bhi[None] - oo[:, None]) * inv[:, None]
+            tmin = torch.minimum(t1, t2).amax(dim=2)
+            tmax = torch.maximum(t1, t2).amin(dim=2)
+            hit = (tmax >= tmin.clamp_min(0.0)) & (tmin > 0.0)
+            tb_ = torch.where(hit, tmin, torch.full_like(tmin, float("inf")))
+            tj, j = tb_.min(dim=1)
+            closer = tj < best
+            best = torch.where(closer, tj, best)
+            refl = torch.where(closer, brefl[j], refl)
+        t_out[a:a + chunk] = best
+        r_out[a:a + chunk] = refl
+    return t_out, r_out
+
+
 # ---- poses ------------------------------------------------------------------------------------
 
 
@@ -453,6 +488,46 @@ class Sequence:
 
     def batch(self, t0: float, n: int, index: int):
         return scan(self.scene, self.rosette, self.pose, t0, n, self.seed * 100_003 + index, rate=self.rate)
+
+    def frames_device(self, n_frames: int, device, seed: int | None = None):
+        """All n_frames at once, generated on a torch device: (xyz (F, P, 3) float32, attr (F, P) int32, t (F, P)
+        float32). Same scene, walk and rosette as frames(); per-point poses interpolated from a 1 kHz sampling of
+        the walk, ray casting on the device, noise from a seeded torch generator (so the points differ from
+        frames()'s, not their distribution)."""
+        import torch
+
+        per = int(round(self.rate * self.frame_dt))
+        n = n_frames * per
+        dev = torch.device(device)
+        step = 1e-3
+        tg = np.arange(0.0, n / self.rate + 3 * step, step)
+        p, yaw, pitch, roll = self.walk.state(tg)
+        g = np.column_stack([p + self.offset, np.unwrap(yaw), pitch, roll])
+        G = torch.from_numpy(g).to(dev)
+        t = torch.arange(n, dtype=torch.float64, device=dev) / self.rate
+        u = t / step
+        i0 = u.floor().long()
+        f = (u - i0.to(torch.float64))[:, None]
+        st = G[i0] * (1.0 - f) + G[i0 + 1] * f
+        cz, sz = torch.cos(st[:, 3]), torch.sin(st[:, 3])
+        cy, sy = torch.cos(st[:, 4]), torch.sin(st[:, 4])
+        cx, sx = torch.cos(st[:, 5]), torch.sin(st[:, 5])
+        R = torch.stack([cz * cy, cz * sy * sx - sz * cx, cz * sy * cx + sz * sx,
+                         sz * cy, sz * sy * sx + cz * cx, sz * sy * cx - cz * sx,
+                         -sy, cy * sx, cy * cx], dim=1).view(n, 3, 3)
+        th = self.rosette.omega * t
+        phi = HALF_FOV * torch.sin(self.rosette.k * th).abs()
+        az = th + self.rosette.prec * t
+        ds = torch.stack([torch.cos(phi), torch.sin(phi) * torch.cos(az), torch.sin(phi) * torch.sin(az)], dim=1)
+        dw = torch.bmm(R, ds[:, :, None])[:, :, 0]
+        rng_true, refl = cast_torch(st[:, :3].contiguous(), dw, self.scene)
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(self.seed * 1_000_003 + 17 if seed is None else seed)
+        r = rng_true + 0.02 * torch.randn(n, dtype=torch.float64, device=dev, generator=gen)
+        xyz = (ds * r[:, None]).to(torch.float32).view(n_frames, per, 3)
+        attr = (refl + 3.0 * torch.randn(n, dtype=torch.float64, device=dev, generator=gen)).clamp(0, 255)
+        attr = attr.to(torch.int32).view(n_frames, per)
+        return xyz.contiguous(), attr.contiguous(), t.to(torch.float32).view(n_frames, per).contiguous()
 
     def frames(self, n_frames: int | None = None):
         """Yield (xyz, attr, t) per frame_dt batch, n_frames of them (default: the whole sequence)."""
