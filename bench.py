@@ -206,6 +206,45 @@ def loc_cpu_baseline(W, hyps, n: int):
                       f"extrapolated linearly", "seconds": round(el, 3)}
 
 
+MLP_FLOPS_PER_ROW = 2.0 * (152 * 256 + 256 * 256 + 256 * 256 + 256 * 6)  # forward multiply-adds x 2
+
+
+def learned_prior(args, torch, lw, dev):
+    """The learned pose-increment prior (shipped weights, trained on 200 x 600 frames of the plain GPU
+    odometry) against the constant-velocity guess on held-out office sequences, and the training step's speed
+    on the matrix cores."""
+    params, norm = lw.prior.load(os.path.join(ROOT, "livox-warp_amd", "prior_weights.npz"))
+    rows = lw.prior.evaluate(lw, params, norm, range(5000, 5000 + args.prior_seqs), args.prior_frames, dev)
+    res = lw.prior.summarise(rows)
+    res["workload"] = (f"{args.prior_seqs} held-out office walks (seeds 5000..) x {args.prior_frames} frames, "
+                       f"10k points/frame, both guesses on the same frames")
+    # training step at batch B: forward, loss, backward (2x the forward's products), Adam
+    B = 4096
+    net = lw.prior.MLP(batch_cap=B, device=dev)
+    net.set_params(lw.prior.init_params(np.random.default_rng(0)))
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    X = torch.randn((B, 152), generator=g, device=dev)
+    Y = torch.randn((B, 6), generator=g, device=dev)
+    for _ in range(3):
+        net.train_step(X, Y, 1e-4)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    steps = 50
+    e0.record(net.stream)
+    for _ in range(steps):
+        net.train_step(X, Y, 1e-4)
+    e1.record(net.stream)
+    torch.cuda.synchronize()
+    step_s = e0.elapsed_time(e1) * 1e-3 / steps
+    tflops = 3.0 * MLP_FLOPS_PER_ROW * B / step_s / 1e12
+    res["training_step"] = {"batch": B, "ms": round(step_s * 1e3, 4), "samples_per_s": round(B / step_s, 1),
+                            "roofline": {"bound": "mfma", "achieved": round(tflops, 3), "peak": 2500.0,
+                                         "unit": "TFLOP/s", "frac": round(tflops / 2500.0, 6),
+                                         "kernel": "k_gemm (bf16 16x16x32 MFMA, fp32 accumulate)"}}
+    return res
+
+
 def copy_probe(torch, dev, mib: int):
     """A plain device-to-device copy of known size, for calibrating hardware byte counters against."""
     n = mib * (1 << 20) // 4
@@ -230,6 +269,9 @@ def main():
     ap.add_argument("--loc-cpu-hyps", type=int, default=64)
     ap.add_argument("--loc-slots", type=int, default=24, help="log2 of the localisation map's hash slots")
     ap.add_argument("--copy-probe", type=int, default=0, help="MiB of a calibration copy at the end (0: none)")
+    ap.add_argument("--no-prior", action="store_true", help="skip the learned-prior leg")
+    ap.add_argument("--prior-seqs", type=int, default=4)
+    ap.add_argument("--prior-frames", type=int, default=600)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the odometry is a sequential "
@@ -321,6 +363,8 @@ def main():
     }
     if not args.no_loc:
         out["localisation"] = localisation(args, torch, lw, dev)
+    if not args.no_prior:
+        out["learned_prior"] = learned_prior(args, torch, lw, dev)
     if args.cpu_frames > 0:
         out["cpu_baseline"] = cpu_baseline(frames, seq, args.warmup, args.cpu_frames)
     else:

@@ -33,18 +33,37 @@ def _torch():
     return torch
 
 
+def rotvec_of(R: np.ndarray) -> np.ndarray:
+    """Rotation vector of a rotation matrix: angle atan2(|v| / 2, (tr R - 1) / 2) along v = vee(R - R^T)
+    (accurate at small angles; the frame increments here are far from pi)."""
+    v = np.array([R[2, 1] - R[1, 2], R[0, 2] - R[2, 0], R[1, 0] - R[0, 1]])
+    s = 0.5 * float(np.sqrt(v @ v))
+    c = 0.5 * (float(R[0, 0] + R[1, 1] + R[2, 2]) - 1.0)
+    th = math.atan2(s, c)
+    if th > 3.0:  # near pi: v loses the axis
+        from scipy.spatial.transform import Rotation
+
+        return Rotation.from_matrix(R).as_rotvec()
+    return v * (0.5 if s < 1e-12 else th / (2.0 * s))
+
+
+def rotmat_of(w: np.ndarray) -> np.ndarray:
+    """Rodrigues: exp of the skew matrix of w."""
+    th = float(np.sqrt(w @ w))
+    K = np.array([[0.0, -w[2], w[1]], [w[2], 0.0, -w[0]], [-w[1], w[0], 0.0]])
+    if th < 1e-8:
+        return np.eye(3) + K + 0.5 * (K @ K)
+    return np.eye(3) + (math.sin(th) / th) * K + ((1.0 - math.cos(th)) / (th * th)) * (K @ K)
+
+
 def body_twist(d: np.ndarray) -> np.ndarray:
     """(translation, rotation vector) of a 4x4 increment, float64."""
-    from scipy.spatial.transform import Rotation
-
-    return np.concatenate([d[:3, 3], Rotation.from_matrix(d[:3, :3]).as_rotvec()])
+    return np.concatenate([d[:3, 3], rotvec_of(d[:3, :3])])
 
 
 def twist_matrix(xi: np.ndarray) -> np.ndarray:
-    from scipy.spatial.transform import Rotation
-
     out = np.eye(4)
-    out[:3, :3] = Rotation.from_rotvec(np.asarray(xi[3:], np.float64)).as_matrix()
+    out[:3, :3] = rotmat_of(np.asarray(xi[3:], np.float64))
     out[:3, 3] = xi[:3]
     return out
 
@@ -175,6 +194,7 @@ class LearnedPrior:
         self.res = collections.deque(maxlen=window)
         self.feat = torch.zeros((1, IN), dtype=torch.float32, device=odo.device)
         self.last = None
+        self.guess_err = {"learned": [], "cv": []}
 
     def guess(self, odo, m: int):
         if len(self.hist) < 4:
@@ -185,7 +205,7 @@ class LearnedPrior:
         y = self.mlp.forward(self.feat)
         out = y.cpu().numpy()[0] if odo.stream == _torch().cuda.current_stream(odo.device) else self._read(odo, y)
         d = self.hist[0] + out.astype(np.float64) * self.norm.sd_t + self.norm.mu_t
-        self.last = d
+        self.last = (d, body_twist(odo.delta))  # the learned and the constant-velocity increments
         return odo.T @ twist_matrix(d), d.astype(np.float32)
 
     @staticmethod
@@ -199,7 +219,12 @@ class LearnedPrior:
         return ok
 
     def observe(self, T_prev: np.ndarray, T: np.ndarray):
-        self.hist.appendleft(body_twist(np.linalg.inv(T_prev) @ T))
+        d = body_twist(np.linalg.inv(T_prev) @ T)
+        if self.last is not None:  # how far each guess was from the registered increment
+            for key, g in zip(("learned", "cv"), self.last):
+                self.guess_err[key].append((float(np.linalg.norm(g[:3] - d[:3])), float(np.linalg.norm(g[3:] - d[3:]))))
+            self.last = None
+        self.hist.appendleft(d)
 
 
 class Recorder:
@@ -254,7 +279,7 @@ def run_sequence(lw, seq, n_frames: int, device, prior_factory=None, reg_slots: 
     return odo, time.perf_counter() - t0
 
 
-def collect(lw, seeds, n_frames: int, device, seconds_margin: float = 1.0):
+def collect(lw, seeds, n_frames: int, device, seconds_margin: float = 1.0, progress=None):
     """Training rows from the plain GPU odometry on the sequences `seeds`: inputs (raw motion, histograms) and
     targets d_k - d_{k-1} from the odometry's own poses. Returns (X (N, 152), Y (N, 6)) device tensors."""
     torch = _torch()
@@ -276,6 +301,8 @@ def collect(lw, seeds, n_frames: int, device, seconds_margin: float = 1.0):
         if rows:
             xs.append(r.X[torch.as_tensor(rows, device=r.X.device)])
             ys.append(torch.as_tensor(np.array(targets), dtype=torch.float64, device=r.X.device))
+        if progress is not None:
+            progress(f"sequence {s}: {len(rows)} rows")
     return torch.cat(xs), torch.cat(ys)
 
 
@@ -296,29 +323,56 @@ def normalise(X, Y, norm: Normalizer):
     return Xn.contiguous(), Yn.to(torch.float32).contiguous()
 
 
-def fit(Xn, Yn, epochs: int = 30, batch: int = 1024, lr: float = 1e-3, seed: int = 0, params=None, device=None):
-    """Train on (Xn, Yn) with the library's kernels (minibatch Adam, shuffled each epoch); returns (params,
-    per-epoch mean loss)."""
+def fit(Xn, Yn, epochs: int = 30, batch: int = 1024, lr: float = 1e-3, seed: int = 0, params=None, device=None,
+        val=None):
+    """Train on (Xn, Yn) with the library's kernels (minibatch Adam, cosine-decayed rate, shuffled each epoch).
+    With val = (Xv, Yv), the validation loss is measured after every epoch and the best epoch's parameters are
+    returned. Returns (params, per-epoch training loss, per-epoch validation loss or [])."""
     torch = _torch()
     mlp = MLP(batch_cap=batch, device=device or Xn.device)
     rng = np.random.default_rng(seed)
     mlp.set_params(params if params is not None else init_params(rng))
     n = int(Xn.shape[0])
-    hist = []
+    hist, vhist = [], []
+    best, best_params = float("inf"), None
     gen = torch.Generator(device=Xn.device)
     gen.manual_seed(seed)
     for ep in range(epochs):
         perm = torch.randperm(n, device=Xn.device, generator=gen)
         tot = torch.zeros(1, dtype=torch.float64, device=Xn.device)
         nb = 0
+        decay = 0.5 * (1.0 + math.cos(math.pi * ep / max(epochs, 1)))
         for a in range(0, n - batch + 1, batch):
             idx = perm[a:a + batch]
-            xb, yb = Xn[idx].contiguous(), Yn[idx].contiguous()
-            decay = 0.5 * (1.0 + math.cos(math.pi * ep / max(epochs, 1)))
-            tot += mlp.train_step(xb, yb, lr * (0.1 + 0.9 * decay)).double()
+            tot += mlp.train_step(Xn[idx].contiguous(), Yn[idx].contiguous(), lr * (0.1 + 0.9 * decay)).double()
             nb += 1
         hist.append(float(tot.item()) / max(nb, 1))
-    return mlp.params(), hist
+        if val is not None:
+            v = evaluate_loss(mlp, *val)
+            vhist.append(v)
+            if v < best:
+                best, best_params = v, mlp.params()
+    return (best_params if best_params is not None else mlp.params()), hist, vhist
+
+
+def evaluate_loss(mlp: MLP, X, Y) -> float:
+    """Mean squared error of the network over (X, Y), in chunks of the network's batch capacity."""
+    tot, n = 0.0, int(X.shape[0])
+    for a in range(0, n, mlp.cap):
+        b = min(n, a + mlp.cap)
+        tot += float(mlp.loss(mlp.forward(X[a:b].contiguous()), Y[a:b].contiguous()).item()) * (b - a)
+    return tot / max(n, 1)
+
+
+def split(Xn, Yn, fraction: float = 0.1, seed: int = 0):
+    """A random (train, validation) split of the rows."""
+    torch = _torch()
+    g = torch.Generator(device=Xn.device)
+    g.manual_seed(seed)
+    perm = torch.randperm(int(Xn.shape[0]), device=Xn.device, generator=g)
+    k = max(int(len(perm) * fraction), 1)
+    v, t = perm[:k], perm[k:]
+    return (Xn[t].contiguous(), Yn[t].contiguous()), (Xn[v].contiguous(), Yn[v].contiguous())
 
 
 def save(path: str, params: np.ndarray, norm: Normalizer, meta: dict | None = None):
@@ -328,3 +382,56 @@ def save(path: str, params: np.ndarray, norm: Normalizer, meta: dict | None = No
 def load(path: str):
     z = np.load(path, allow_pickle=False)
     return z["params"], Normalizer(z["mu_d"], z["sd_d"], z["mu_t"], z["sd_t"])
+
+
+def drift(seq, traj) -> float:
+    """RMS position error of the trajectory against the ground truth, both relative to their first pose."""
+    T0 = seq.pose(traj[0][1])
+    err = [np.linalg.norm((np.linalg.inv(traj[0][2]) @ T)[:3, 3] - (np.linalg.inv(T0) @ seq.pose(tm))[:3, 3])
+           for _, tm, T in traj]
+    return float(np.sqrt(np.mean(np.square(err))))
+
+
+def evaluate(lw, params, norm: Normalizer, seeds, n_frames: int, device, fallback: float = 2.0, progress=None):
+    """Each held-out sequence twice on the same device-generated frames: the constant-velocity guess, then the
+    learned prior. Per sequence: frames/s, Gauss-Newton steps and registrations per frame, drift, fallbacks,
+    and the mean error of both guesses against the registered increment (learned run)."""
+    torch = _torch()
+    rows = []
+    for s in seeds:
+        seq = lw.synth.Sequence(seed=int(s), seconds=n_frames * 0.1 + 1.0)
+        frames = seq.frames_device(n_frames, device)
+        torch.cuda.synchronize()
+        out = {"seed": int(s)}
+        for name, factory in (("cv", None), ("learned", lambda o: LearnedPrior(params, norm, o, fallback=fallback))):
+            odo, el = run_sequence(lw, seq, n_frames, device, prior_factory=factory, frames=frames)
+            n = len(odo.traj)
+            tot = odo.totals
+            r = {"fps": n / el, "gn_per_frame": tot["gn_iters"] / n, "reg_per_frame": tot["registrations"] / n,
+                 "drift_m": drift(seq, odo.traj)}
+            if factory is not None:
+                ge = odo.prior.guess_err
+                r["fallbacks"] = tot["prior_fallbacks"]
+                r["used"] = tot["prior_used"]
+                for key in ("learned", "cv"):
+                    e = np.array(ge[key]) if ge[key] else np.zeros((1, 2))
+                    r[f"guess_{key}_mm"] = float(e[:, 0].mean() * 1e3)
+                    r[f"guess_{key}_mdeg"] = float(np.degrees(e[:, 1].mean()) * 1e3)
+            out[name] = r
+        rows.append(out)
+        if progress is not None:
+            progress(f"held-out {s}: cv {out['cv']['fps']:.0f} fps {out['cv']['gn_per_frame']:.2f} GN/frame, learned "
+                     f"{out['learned']['fps']:.0f} fps {out['learned']['gn_per_frame']:.2f} GN/frame")
+    return rows
+
+
+def summarise(rows) -> dict:
+    def mean(k, key):
+        return float(np.mean([r[k][key] for r in rows]))
+
+    res = {}
+    for k in ("cv", "learned"):
+        res[k] = {key: round(mean(k, key), 4) for key in rows[0][k]}
+    res["speedup"] = round(res["learned"]["fps"] / res["cv"]["fps"], 3)
+    res["gn_reduction"] = round(1.0 - res["learned"]["gn_per_frame"] / res["cv"]["gn_per_frame"], 3)
+    return res

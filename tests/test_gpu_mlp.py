@@ -132,6 +132,6 @@ def test_training_learns_a_mapping(lw, om, gpu_ok):
     A = rng.normal(0, 0.5, (om.IN, om.OUT))
     Y = np.tanh(X @ A / math.sqrt(om.IN) * 3).astype(np.float32)
     Xd, Yd = torch.from_numpy(X).cuda(), torch.from_numpy(Y).cuda()
-    params, hist = lw.prior.fit(Xd, Yd, epochs=75, batch=1024, lr=2e-3)
+    params, hist, _ = lw.prior.fit(Xd, Yd, epochs=75, batch=1024, lr=2e-3)
     print("loss per epoch", [round(h, 5) for h in hist[::10]], flush=True)
     assert hist[-1] < hist[0] / 20
