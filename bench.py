@@ -325,6 +325,8 @@ def main():
     timed = len(odo.traj) - f0
     prof = np.zeros(8)
     lw._abi.check(L.lw_prof_read(odo._ctx, lw._abi.ptr(prof, lw._abi._f64p)), "lw_prof_read")
+    phases = np.zeros(8)
+    lw._abi.check(L.lw_prof_phases(odo._ctx, lw._abi.ptr(phases, lw._abi._f64p)), "lw_prof_phases")
     lw._abi.check(L.lw_prof_enable(odo._ctx, 0), "lw_prof_enable")
     fps = timed / el
     launch_s = prof[1] * 1e-3 / max(prof[0], 1.0)
@@ -354,7 +356,14 @@ def main():
         "registration": {"launches": int(prof[0]), "passes": int(prof[6]),
                          "avg_launch_us": round(launch_s * 1e6, 3), "point_iterations": int(prof[2]),
                          "lookups": int(prof[3]), "hits": int(prof[4]), "matched": int(prof[5]),
-                         "time_share": round(prof[1] * 1e-3 / el, 4)},
+                         "time_share": round(prof[1] * 1e-3 / el, 4),
+                         "us_per_pass": {"search": round(phases[0] / max(phases[3], 1), 2),
+                                         "reduce": round(phases[1] / max(phases[3], 1), 2),
+                                         "step": round(phases[2] / max(phases[3], 1), 2),
+                                         "launch": round(launch_s * 1e6 * prof[0] / max(phases[3], 1), 2),
+                                         "block0_wake": round(phases[4] / max(phases[7], 1), 2),
+                                         "block0_search": round(phases[5] / max(phases[7], 1), 2),
+                                         "block0_row_ticket": round(phases[6] / max(phases[7], 1), 2)}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic("k_register<0>"),
                      "kernel": "k_register<0> (one launch per registration: every accumulation pass and step)",

@@ -203,6 +203,13 @@ int lw_frame_features(lw_ctx *ctx, int32_t m, float r_max, float e_max, const fl
 int lw_prof_enable(lw_ctx *ctx, int32_t on);
 int lw_prof_reset(lw_ctx *ctx);
 int lw_prof_read(lw_ctx *ctx, double *out);
+/* Where the registration passes spend their time, since lw_prof_reset, from device wall-clock stamps of each
+ * pass's last workgroup (microseconds, summed over passes): out[0] from the pass's start to the last
+ * workgroup's ticket (search, terms, partial rows), out[1] the cross-workgroup reduction, out[2] the float64
+ * step; out[3] the passes timed. Block 0's view of the persistent loop's passes: out[4] from the release of
+ * the previous pass to its wake-up, out[5] its search and terms, out[6] its row write and ticket; out[7] the
+ * passes it timed. No counterpart in the reference. */
+int lw_prof_phases(lw_ctx *ctx, double *out);
 
 /* Host builds of the device's float64 step and per-voxel precision, for tests on machines without a
  * GPU (same source as the device code). */

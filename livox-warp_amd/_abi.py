@@ -83,6 +83,7 @@ SIGNATURES = {
     "lw_prof_enable": (_i32, [_vp, _i32]),
     "lw_prof_reset": (_i32, [_vp]),
     "lw_prof_read": (_i32, [_vp, _f64p]),
+    "lw_prof_phases": (_i32, [_vp, _f64p]),
     "lw_host_gn_step": (_i32, [_f64p, _f64p, _f64p, _f64, _f64, _f64, _i32, _f64p, _f64p, _i32p]),
     "lw_host_voxel_precision": (_i32, [_f32p, _f32p, _i32, _f32, _f32, _f32, _f32, _f32p, _f32p, _f32p]),
     "lw_host_apply_twist": (_i32, [_f64p, _f64p]),

@@ -603,6 +603,21 @@ int lw_prof_read(lw_ctx *c, double *out) {
     return 0;
 }
 
+int lw_prof_phases(lw_ctx *c, double *out) {
+    if (!c || !out) return LW_EINVAL;
+    LW_CK(hipStreamSynchronize(c->stream));
+    unsigned long long v[16] = {0};
+    if (c->prof_dev) LW_CK(hipMemcpy(v, c->prof_dev, sizeof(v), hipMemcpyDeviceToHost));
+    int khz = 0;
+    LW_CK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+    const double us_per_tick = khz > 0 ? 1000.0 / (double)khz : 0.0;
+    for (int i = 0; i < 3; ++i) out[i] = (double)v[8 + i] * us_per_tick;
+    out[3] = (double)v[11];
+    for (int i = 0; i < 3; ++i) out[4 + i] = (double)v[12 + i] * us_per_tick;
+    out[7] = (double)v[15];
+    return 0;
+}
+
 // ---- host builds of the device math (tests) ---------------------------------------------------------
 
 int lw_host_gn_step(const double *v, const double *T, const double *T_pred, double damping, double prior_info,

@@ -25,12 +25,14 @@ constexpr int ACC_USED = 53;                          // 21 + 6 + 5 + 21
 // One hash-table slot (32 B): the reference's int64 key, the dense voxel id of its records, and a copy of
 // the voxel's mean and count (float), so the closest-mean search reads one slot per neighbour and nothing
 // else. Mean and count are written by the precision refresh (count 0 until then: the voxel is skipped).
+// disp is the largest probe distance of any key whose home (slot_hash & mask) is this slot, 0 when none
+// was displaced: a lookup whose home slot holds another key needs to look no further than that.
 struct __align__(32) Slot {
     int64_t key;
     int32_t vid;
     float cnt;
     float mu[3];
-    float pad;
+    int32_t disp;
 };
 
 // Moments of one voxel, relative to the voxel centre (odom.py:198-200): sum r, count, and the six

@@ -291,42 +291,57 @@ __device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf
     wave_sum<MODE, S>(lds, sum, comp);
 }
 
-// Sum the G partial rows in workgroup order (the last workgroup, after its acquire fence). Lane k < 53
-// returns entry k of the reduced accumulator.
+// Sum the G partial rows in workgroup order (the last workgroup, after its acquire fence), 32 rows in flight
+// per lane. Lane k < RED_COLS returns entry k of the reduced row: the 53 ACC entries, then the measurement
+// counts (hits, matched points) in 53 and 54.
+constexpr int RED_COLS = ACC_USED + 2;
+constexpr int RED_BATCH = 32;
+
 template <int MODE>
 __device__ inline double reduce_rows(const double *partials, int G) {
     const int lane = threadIdx.x;
-    if (lane >= ACC_USED) return 0.0;
+    if (lane >= RED_COLS) return 0.0;
     const double *col = partials + lane;
     if constexpr (MODE == 0) {
         double t = 0.0;
-        int b = 0;
-        for (; b + 8 <= G; b += 8) {
-            double x[8];
-            LW_UNROLL for (int u = 0; u < 8; ++u) x[u] = col[(size_t)(b + u) * 128];
-            LW_UNROLL for (int u = 0; u < 8; ++u) t += x[u];
+        for (int b = 0; b < G; b += RED_BATCH) {
+            double x[RED_BATCH];
+            LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) x[u] = b + u < G ? col[(size_t)(b + u) * 128] : 0.0;
+            LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) if (b + u < G) t += x[u];
         }
-        for (; b < G; ++b) t += col[(size_t)b * 128];
         return t;
     } else if constexpr (MODE == 2) {
         float t = 0.0f;
-        int b = 0;
-        for (; b + 8 <= G; b += 8) {
-            double x[8];
-            LW_UNROLL for (int u = 0; u < 8; ++u) x[u] = col[(size_t)(b + u) * 128];
-            LW_UNROLL for (int u = 0; u < 8; ++u) t += (float)x[u];
+        double e = 0.0;  // the counts (integers: summed exactly)
+        for (int b = 0; b < G; b += RED_BATCH) {
+            double x[RED_BATCH];
+            LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) x[u] = b + u < G ? col[(size_t)(b + u) * 128] : 0.0;
+            LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) {
+                if (b + u >= G) continue;
+                t += (float)x[u];
+                e += x[u];
+            }
         }
-        for (; b < G; ++b) t += (float)col[(size_t)b * 128];
-        return (double)t;
+        return lane < ACC_USED ? (double)t : e;
     } else {
         float s = 0.0f, c = 0.0f;
-        for (int b = 0; b < G; ++b) {
-            float t, e;
-            two_sum(s, (float)col[(size_t)b * 128], t, e);
-            s = t;
-            c = c + (float)col[(size_t)b * 128 + 64] + e;
+        double e = 0.0;  // the counts
+        for (int b = 0; b < G; b += RED_BATCH) {
+            double x[RED_BATCH], y[RED_BATCH];
+            LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) {
+                x[u] = b + u < G ? col[(size_t)(b + u) * 128] : 0.0;
+                y[u] = b + u < G && lane < ACC_USED ? col[(size_t)(b + u) * 128 + 64] : 0.0;
+            }
+            LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) {
+                if (b + u >= G) continue;
+                float t, er;
+                two_sum(s, (float)x[u], t, er);
+                s = t;
+                c = c + (float)y[u] + er;
+                e += x[u];
+            }
         }
-        return (double)s + (double)c;
+        return lane < ACC_USED ? (double)s + (double)c : e;
     }
 }
 
@@ -361,8 +376,8 @@ __device__ __noinline__ void gauss_newton(RegState *st, const double *v, const g
     }
 }
 
-// Write this workgroup's row and the measurement counters, then take a ticket; true for the last
-// workgroup of the pass (which then sees every row).
+// Write this workgroup's row (its 53 sums and compensations, and its measurement counts in the spare
+// entries 53 and 54), then take a ticket; true for the last workgroup of the pass (which then sees every row).
 template <int MODE>
 __device__ inline bool row_and_ticket(const IcpArgs &a, double sum, double comp, int32_t hits, int32_t matched,
                                       int G, int32_t *s_last) {
@@ -372,15 +387,13 @@ __device__ inline bool row_and_ticket(const IcpArgs &a, double sum, double comp,
         row[lane] = sum;
         row[64 + lane] = comp;
     }
-    if (a.prof) {
-        LW_UNROLL for (int off = 32; off > 0; off >>= 1) {
-            hits += __shfl_xor(hits, off, 64);
-            matched += __shfl_xor(matched, off, 64);
-        }
-        if (lane == 0) {
-            atomicAdd(&a.prof[2], (unsigned long long)hits);
-            atomicAdd(&a.prof[3], (unsigned long long)matched);
-        }
+    LW_UNROLL for (int off = 32; off > 0; off >>= 1) {
+        hits += __shfl_xor(hits, off, 64);
+        matched += __shfl_xor(matched, off, 64);
+    }
+    if (lane == 0) {
+        row[ACC_USED] = (double)hits;
+        row[ACC_USED + 1] = (double)matched;
     }
     __threadfence();
     if (lane == 0) *s_last = (atomicAdd(&a.st->ticket, 1u) == (uint32_t)G - 1);
@@ -391,22 +404,34 @@ __device__ inline bool row_and_ticket(const IcpArgs &a, double sum, double comp,
     return last;
 }
 
+// The last workgroup's part of a pass: reduce the rows, then (registration) the float64 step in lane 0. With
+// measurement on, lane 0 adds the pass's phases to the counters, in device wall-clock ticks: from the
+// pass's start (st->t_rel) to this workgroup's ticket (t_ticket), the reduction, and the step.
 template <int MODE, int R>
-__device__ inline void finish_pass(const IcpArgs &a, const Pass p, int G, double *vs) {
+__device__ inline void finish_pass(const IcpArgs &a, const Pass p, int G, double *vs, unsigned long long t_ticket) {
     constexpr int D = 2 * R + 1, K = D * D * D;
     const int lane = threadIdx.x;
     const double v = reduce_rows<MODE>(a.partials, G);
     vs[lane] = v;
-    a.st->acc[lane] = v;
+    a.st->acc[lane] = lane < ACC_USED ? v : 0.0;
     __syncthreads();
     if (lane == 0) {
         a.st->ticket = 0;
+        const unsigned long long t_red = a.prof ? wall_clock64() : 0ull;
+        if (a.gn) gauss_newton(a.st, vs, a.sp, p.it);
         if (a.prof) {
+            const unsigned long long t_step = wall_clock64();
             atomicAdd(&a.prof[0], (unsigned long long)p.n);
             atomicAdd(&a.prof[1], (unsigned long long)p.n * (unsigned long long)K);
+            atomicAdd(&a.prof[2], (unsigned long long)vs[ACC_USED]);
+            atomicAdd(&a.prof[3], (unsigned long long)vs[ACC_USED + 1]);
             atomicAdd(&a.prof[4], 1ull);
+            atomicAdd(&a.prof[8], t_ticket - a.st->t_rel);
+            atomicAdd(&a.prof[9], t_red - t_ticket);
+            atomicAdd(&a.prof[10], t_step - t_red);
+            atomicAdd(&a.prof[11], 1ull);
+            a.st->t_rel = wall_clock64();  // the next pass starts when this one is released
         }
-        if (a.gn) gauss_newton(a.st, vs, a.sp, p.it);
     }
 }
 
@@ -424,9 +449,10 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_icp(IcpArgs a) {
     int32_t hits = 0, matched = 0;
     const Pass p{a.n, a.max_d2, a.kappa, a.it};
     const int G = (int)gridDim.x;
+    if (a.prof && blockIdx.x == 0 && threadIdx.x == 0) a.st->t_rel = wall_clock64();
     wave_pass<MODE, R>(a, p, Tf, G, lds, sum, comp, hits, matched);
     if (!row_and_ticket<MODE>(a, sum, comp, hits, matched, G, &s_last)) return;
-    finish_pass<MODE, R>(a, p, G, vs);
+    finish_pass<MODE, R>(a, p, G, vs, a.prof ? wall_clock64() : 0ull);
 }
 
 // ---- the persistent registration ------------------------------------------------------------------
@@ -455,9 +481,15 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_register(LoopArgs L) {
     // workgroups that no pass can use (the widest pass needs ceil(n * 4 / 64)) leave now
     if ((int64_t)blockIdx.x * ICP_BLOCK >= (int64_t)n * Split<2>::S) return;
     const IcpArgs &a = L.a;
+    if (a.prof && blockIdx.x == 0 && lane == 0) st->t_rel = wall_clock64();  // block 0 takes part in pass 0
     for (int it = 0; it < L.max_iter; ++it) {
         if (lane < 16) Tf[lane] = __hip_atomic_load(&st->Tf[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
+        // measurement: block 0's view of the pass (wake-up after the release, search, row and ticket)
+        const bool stamp = a.prof && blockIdx.x == 0;
+        const unsigned long long t_wake = stamp ? wall_clock64() : 0ull;
+        const unsigned long long t_pass = stamp ? __hip_atomic_load(&st->t_rel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                                : 0ull;
         const int reach = it < L.wide_iters ? 2 : 1;
         const Pass p{n, reach == 2 ? L.max_d2_r2 : L.max_d2_r1, st->kappa[it], it};
         // the workgroups that hold points in this pass (one lane group each while the grid is large
@@ -473,13 +505,21 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_register(LoopArgs L) {
                 wave_pass<MODE, 2>(a, p, Tf, G, lds, sum, comp, hits, matched);
             else
                 wave_pass<MODE, 1>(a, p, Tf, G, lds, sum, comp, hits, matched);
+            const unsigned long long t_found = stamp ? wall_clock64() : 0ull;
             last = row_and_ticket<MODE>(a, sum, comp, hits, matched, G, &s_last);
+            if (stamp && lane == 0) {
+                atomicAdd(&a.prof[12], t_wake - t_pass);
+                atomicAdd(&a.prof[13], t_found - t_wake);
+                atomicAdd(&a.prof[14], wall_clock64() - t_found);
+                atomicAdd(&a.prof[15], 1ull);
+            }
         }
         if (last) {
+            const unsigned long long t_ticket = a.prof ? wall_clock64() : 0ull;
             if (reach == 2)
-                finish_pass<MODE, 2>(a, p, G, vs);
+                finish_pass<MODE, 2>(a, p, G, vs, t_ticket);
             else
-                finish_pass<MODE, 1>(a, p, G, vs);
+                finish_pass<MODE, 1>(a, p, G, vs, t_ticket);
             __threadfence();  // release: pose, done flag and ticket before the generation
             if (lane == 0) __hip_atomic_store(&st->gen, (uint32_t)(it + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {

@@ -32,6 +32,7 @@ struct RegState {
     uint32_t ticket;    // workgroups finished in the current pass
     uint32_t gen;       // passes completed (the persistent registration's release counter)
     int32_t n;          // thinned points the registration ran on
+    unsigned long long t_rel;  // device wall clock at the current pass's start (measurement only)
     float kappa[64];    // robust-kernel scale per iteration (odom.py:767)
 };
 

@@ -42,7 +42,7 @@ __global__ void k_clear_slots(Slot *s, int64_t n) {
         z.vid = -1;
         z.cnt = 0.0f;
         z.mu[0] = z.mu[1] = z.mu[2] = 0.0f;
-        z.pad = 0.0f;
+        z.disp = 0;
         s[i] = z;
     }
 }
@@ -104,6 +104,7 @@ __global__ __launch_bounds__(BLOCK) void k_ins_claim(const float *xyz, const uin
                 z.cnt = 0;
                 vox_mom[vid] = z;
                 atomicOr(&occ[s >> 5], 1u << (s & 31));
+                if (probe > 0) atomicMax(&slots[h].disp, probe);  // the home slot bounds this key's probe
                 atomicAdd(&ctr[0], 1);
                 pt_slot[i] = s;
                 return;

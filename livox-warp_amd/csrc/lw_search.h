@@ -2,11 +2,12 @@
 // localisation (lw_loc.hip): odom.py:308-334 with find_slot (gpu.py:254-265).
 //
 // A lane resolves a chunk of C neighbour keys together. Round 0 loads the bitmap words of every home slot
-// (slot_hash & mask) and then every occupied home slot's 32 B record; a record holding the key is a hit. Keys
-// whose home slot holds another key continue the linear probe, all of them together: each probe depth is one
-// round of bitmap loads and one of key loads for every key still unresolved, so a wave pays one pair of round
-// trips per depth, not one per key (with 64 lanes, some lane almost always needs a probe somewhere in a
-// chunk). A key ends at its slot (hit) or at the first empty slot (miss), within 64 probes.
+// (slot_hash & mask) and then every occupied home slot's 32 B record; a record holding the key is a hit. A
+// key whose home slot holds another key can only lie within the home's displacement bound (Slot::disp, the
+// largest probe distance of the keys homed there, kept by the inserts): a bound of 0 (the usual case) ends the
+// lookup as a miss at once, otherwise the slots up to the bound are read, the records of every key of the
+// chunk still unresolved at a depth loaded together. The slot found is the one find_slot's linear probe
+// reaches (gpu.py:254-265): keys are never removed, so a key homed at h sits within h .. h + disp(h).
 //
 // The closest mean is chosen by (squared distance, position in the reference's neighbour order), so the
 // order in which candidates are evaluated does not matter: the result is the serial loop's strict-< choice.
@@ -30,7 +31,7 @@ struct Table {
     int32_t mask;
 };
 
-// A slot as two 16-byte loads: lo = {key lo, key hi, voxel id, count bits}, hi = {mean x, y, z, pad}.
+// A slot as two 16-byte loads: lo = {key lo, key hi, voxel id, count bits}, hi = {mean x, y, z, disp bits}.
 __device__ inline void load_slot(const Slot *slots, int32_t s, int4 &lo, float4 &hi) {
     const int4 *p = reinterpret_cast<const int4 *>(slots + s);
     lo = p[0];
@@ -84,6 +85,7 @@ __device__ inline void search_part(const Table &t, float inv_v, float wx, float 
         LW_UNROLL for (int c = 0; c < C; ++c) w[c] = (j0 + c < j_hi) ? t.occ[h[c] >> 5] : 0u;
         // round 0: the home slots
         uint32_t need = 0u;
+        int32_t dmax[C];
         {
             int4 lo[C];
             float4 hi[C];
@@ -93,49 +95,36 @@ __device__ inline void search_part(const Table &t, float inv_v, float wx, float 
                 if (bit_of(w[c], h[c])) load_slot(t.slots, h[c], lo[c], hi[c]);
             }
             LW_UNROLL for (int c = 0; c < C; ++c) {
+                dmax[c] = 0;
                 if (!bit_of(w[c], h[c])) continue;
                 if (key_of(lo[c]) != key[c]) {
-                    need |= 1u << c;
+                    dmax[c] = __float_as_int(hi[c].w);  // the home's displacement bound
+                    if (dmax[c] > 0) need |= 1u << c;
                     continue;
                 }
                 ++hits;
                 consider(wx, wy, wz, h[c], j0 + c, lo[c], hi[c], best);
             }
         }
-        if (need == 0u) continue;
-        // the rest of the linear probes, every unresolved key of the chunk at once
-        uint32_t found = 0u;
+        // the displaced keys: every unresolved key of the chunk one slot further per round
         for (int probe = 1; need != 0u && probe < PROBES; ++probe) {
-            uint32_t wv[C];
-            LW_UNROLL for (int c = 0; c < C; ++c) wv[c] = ((need >> c) & 1u) ? t.occ[((h[c] + probe) & t.mask) >> 5] : 0u;
-            int64_t kk[C];
+            int4 lo[C];
+            float4 hi[C];
             LW_UNROLL for (int c = 0; c < C; ++c) {
-                kk[c] = -1;
-                if (!((need >> c) & 1u)) continue;
-                const int32_t s = (h[c] + probe) & t.mask;
-                if (bit_of(wv[c], s))
-                    kk[c] = t.slots[s].key;
-                else
-                    need &= ~(1u << c);  // an empty slot ends the probe: not in the map
+                lo[c] = make_int4(-1, -1, -1, 0);
+                hi[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if ((need >> c) & 1u) load_slot(t.slots, (h[c] + probe) & t.mask, lo[c], hi[c]);
             }
             LW_UNROLL for (int c = 0; c < C; ++c) {
-                if (((need >> c) & 1u) && kk[c] == key[c]) {
+                if (!((need >> c) & 1u)) continue;
+                if (key_of(lo[c]) == key[c]) {
                     need &= ~(1u << c);
-                    found |= 1u << c;
-                    h[c] = (h[c] + probe) & t.mask;
+                    ++hits;
+                    consider(wx, wy, wz, (h[c] + probe) & t.mask, j0 + c, lo[c], hi[c], best);
+                } else if (probe >= dmax[c]) {
+                    need &= ~(1u << c);  // past the bound: not in the map
                 }
             }
-        }
-        if (found == 0u) continue;
-        int4 lo[C];
-        float4 hi[C];
-        LW_UNROLL for (int c = 0; c < C; ++c) {
-            if ((found >> c) & 1u) load_slot(t.slots, h[c], lo[c], hi[c]);
-        }
-        LW_UNROLL for (int c = 0; c < C; ++c) {
-            if (!((found >> c) & 1u)) continue;
-            ++hits;
-            consider(wx, wy, wz, h[c], j0 + c, lo[c], hi[c], best);
         }
     }
 }
