@@ -325,7 +325,7 @@ def main():
     timed = len(odo.traj) - f0
     prof = np.zeros(8)
     lw._abi.check(L.lw_prof_read(odo._ctx, lw._abi.ptr(prof, lw._abi._f64p)), "lw_prof_read")
-    phases = np.zeros(8)
+    phases = np.zeros(10)
     lw._abi.check(L.lw_prof_phases(odo._ctx, lw._abi.ptr(phases, lw._abi._f64p)), "lw_prof_phases")
     lw._abi.check(L.lw_prof_enable(odo._ctx, 0), "lw_prof_enable")
     fps = timed / el
@@ -363,7 +363,8 @@ def main():
                                          "launch": round(launch_s * 1e6 * prof[0] / max(phases[3], 1), 2),
                                          "block0_wake": round(phases[4] / max(phases[7], 1), 2),
                                          "block0_search": round(phases[5] / max(phases[7], 1), 2),
-                                         "block0_row_ticket": round(phases[6] / max(phases[7], 1), 2)}},
+                                         "block0_row_ticket": round(phases[6] / max(phases[7], 1), 2)},
+                         "steps_on_wave": int(phases[8]), "steps_one_lane": int(phases[9])},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic("k_register<0>"),
                      "kernel": "k_register<0> (one launch per registration: every accumulation pass and step)",

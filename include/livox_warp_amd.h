@@ -208,7 +208,8 @@ int lw_prof_read(lw_ctx *ctx, double *out);
  * workgroup's ticket (search, terms, partial rows), out[1] the cross-workgroup reduction, out[2] the float64
  * step; out[3] the passes timed. Block 0's view of the persistent loop's passes: out[4] from the release of
  * the previous pass to its wake-up, out[5] its search and terms, out[6] its row write and ticket; out[7] the
- * passes it timed. No counterpart in the reference. */
+ * passes it timed. out[8] steps taken on the whole wave, out[9] steps taken by the general one-lane routine
+ * (a degenerate direction, an indefinite system). No counterpart in the reference. */
 int lw_prof_phases(lw_ctx *ctx, double *out);
 
 /* Host builds of the device's float64 step and per-voxel precision, for tests on machines without a

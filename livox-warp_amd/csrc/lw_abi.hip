@@ -615,6 +615,8 @@ int lw_prof_phases(lw_ctx *c, double *out) {
     out[3] = (double)v[11];
     for (int i = 0; i < 3; ++i) out[4 + i] = (double)v[12 + i] * us_per_tick;
     out[7] = (double)v[15];
+    out[8] = (double)v[5];
+    out[9] = (double)v[6];
     return 0;
 }
 

@@ -1,0 +1,192 @@
+// The float64 Gauss-Newton step (lw_gn.h's gn::step) spread over the 64 lanes of one wave, for the common
+// case: every direction constrained (J^T Hg J - thr I positive definite) and the damped system positive
+// definite. Matrices live in LDS; each lane forms matrix entries with the scalar routine's exact operation
+// order (the same products summed in the same order, the same divisions and square roots), so the result
+// equals gn::step's bit for bit; the Cholesky factorisations go column by column with the column's entries
+// in parallel. Anything else (a degenerate direction to hold, or an indefinite system needing the LU
+// fallback) reports "not handled" and the caller runs gn::step in one lane.
+//
+// The scalar routine compiles to tens of kilobytes of straight-line code executed by one lane; this one is
+// a few hundred instructions per lane.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lw_gn.h"
+
+namespace lw {
+namespace gnw {
+
+// LDS layout (doubles) of the step's scratch
+constexpr int W_H = 0, W_HG = 36, W_A = 72, W_J = 108, W_JI = 144, W_M1 = 180, W_M2 = 216, W_L = 252, W_RHS = 288,
+              W_XI = 294, W_Y = 300, W_OUT = 306, W_T = 312, W_DOUBLES = 328;
+
+// Index of (p, q), p <= q, in a row-major upper triangle of a 6x6 matrix (odom.py:387, np.triu_indices(6)).
+__device__ inline int triu(int p, int q) { return p * 6 - (p * (p - 1)) / 2 + (q - p); }
+
+// Entry l (< 36) of C = A B, summed over k in order (gn::mat6_mul).
+__device__ inline double mul_entry(const double *A, const double *B, int l) {
+    const int i = l / 6, j = l % 6;
+    double s = 0.0;
+    for (int k = 0; k < 6; ++k) s += A[6 * i + k] * B[6 * k + j];
+    return s;
+}
+
+// Entry l (< 36) of C = A^T B (gn::mat6_tmul).
+__device__ inline double tmul_entry(const double *A, const double *B, int l) {
+    const int i = l / 6, j = l % 6;
+    double s = 0.0;
+    for (int k = 0; k < 6; ++k) s += A[6 * k + i] * B[6 * k + j];
+    return s;
+}
+
+// Cholesky factor of A - shift I into L (lower, LDS), column by column: lane j computes the diagonal, lanes
+// i > j the column below it (gn::pd_above / gn::solve6 order). Every lane returns whether every pivot was
+// positive.
+__device__ inline bool wave_cholesky(const double *A, double shift, double *L) {
+    const int lane = threadIdx.x;
+    bool pd = true;
+    for (int j = 0; j < 6; ++j) {
+        double s = A[6 * j + j] - shift;
+        for (int k = 0; k < j; ++k) s -= L[6 * j + k] * L[6 * j + k];
+        if (!(s > 0.0)) pd = false;
+        const double d = sqrt(s > 0.0 ? s : 1.0);
+        if (lane > j && lane < 6) {
+            double t = A[6 * lane + j];
+            for (int k = 0; k < j; ++k) t -= L[6 * lane + k] * L[6 * j + k];
+            L[6 * lane + j] = t / d;
+        }
+        if (lane == j) L[6 * j + j] = d;
+        __syncthreads();
+    }
+    return pd;
+}
+
+// Entry (r, c) of K = [t]x for the pose T (row-major 4x4 in LDS), as gn::sensor_basis lays it out:
+// {0, -t2, t1; t2, 0, -t0; -t1, t0, 0}. Selects, not a private array: a lane-dependent index into a private
+// array would put it in scratch memory.
+__device__ inline double skew_entry(const double *Ts, int r, int c) {
+    const int e = 3 * r + c;
+    double v = 0.0;
+    v = e == 1 ? -Ts[11] : v;
+    v = e == 2 ? Ts[7] : v;
+    v = e == 3 ? Ts[11] : v;
+    v = e == 5 ? -Ts[3] : v;
+    v = e == 6 ? -Ts[7] : v;
+    v = e == 7 ? Ts[3] : v;
+    return v;
+}
+
+// gn::sensor_basis entry by entry (same products in the same order): J = [[R, [c]x R / L], [0, R / L]],
+// J^-1 = [[R^T, -R^T [c]x], [0, L R^T]], with R = T[0:3, 0:3] and c = T[0:3, 3] (Ts: T in LDS).
+__device__ inline void basis_entry(const double *Ts, double L, int l, double &j_out, double &ji_out) {
+    const int i = l / 6, j = l % 6;
+    j_out = 0.0;
+    ji_out = 0.0;
+    if (i < 3 && j < 3) {
+        j_out = Ts[4 * i + j];
+        ji_out = Ts[4 * j + i];
+    } else if (i < 3) {
+        const int jj = j - 3;
+        const double kr = skew_entry(Ts, i, 0) * Ts[jj] + skew_entry(Ts, i, 1) * Ts[4 + jj] +
+                          skew_entry(Ts, i, 2) * Ts[8 + jj];  // (K R)[i][jj], mat3_mul order
+        j_out = kr / L;
+        ji_out = -(Ts[i] * skew_entry(Ts, 0, jj) + Ts[4 + i] * skew_entry(Ts, 1, jj) +
+                   Ts[8 + i] * skew_entry(Ts, 2, jj));  // -(R^T K)[i][jj]
+    } else if (j >= 3) {
+        const int ii = i - 3, jj = j - 3;
+        j_out = Ts[4 * ii + jj] / L;
+        ji_out = L * Ts[4 * jj + ii];
+    }
+}
+
+struct WaveOut {
+    double delta[6];
+    double rms;
+    int32_t corr;
+    int32_t n_degen;
+};
+
+// The step on all 64 lanes of the calling wave (a one-wave workgroup). sh: W_DOUBLES doubles of LDS. Returns
+// 1 when handled (lane 0's o holds delta, and H_out, the registration's copy of H, has been written), 0 when
+// the caller must run gn::step instead, or -1 for gn::STEP_FEW (corr < min_corr). T, T_pred: the registration
+// state (global memory).
+__device__ inline int wave_step(const double *v, const double *T, const double *T_pred, const gn::StepParams &P,
+                                double *sh, WaveOut &o, double *H_out) {
+    const int lane = threadIdx.x;
+    o.corr = (int32_t)v[27];
+    o.n_degen = 0;
+    if (o.corr < P.min_corr) return -1;
+    double *H = sh + W_H, *Hg = sh + W_HG, *A = sh + W_A, *J = sh + W_J, *Ji = sh + W_JI, *M1 = sh + W_M1,
+           *M2 = sh + W_M2, *L = sh + W_L, *rhs = sh + W_RHS, *xi = sh + W_XI, *y = sh + W_Y, *out = sh + W_OUT,
+           *Ts = sh + W_T;
+    if (lane < 16) Ts[lane] = T[lane];
+    __syncthreads();
+    double Lr = v[31] / (double)o.corr;
+    if (!(Lr > 1.0)) Lr = 1.0;
+    const double rr = v[29] > 0.0 ? v[29] : 0.0;
+    const double ww = v[30] > 1e-9 ? v[30] : 1e-9;
+    o.rms = sqrt(rr / ww);
+    if (lane < 36) {
+        const int i = lane / 6, j = lane % 6;
+        const int u = i <= j ? triu(i, j) : triu(j, i);
+        H[lane] = v[u];
+        Hg[lane] = v[32 + u];
+        double bj, bji;
+        basis_entry(Ts, Lr, lane, bj, bji);
+        J[lane] = bj;
+        Ji[lane] = bji;
+    }
+    if (lane == 0) {
+        double x[6];
+        gn::twist_between(T, T_pred, x);
+        LW_UNROLL for (int k = 0; k < 6; ++k) xi[k] = x[k];
+    }
+    __syncthreads();
+    double mean = 0.0;
+    for (int i = 0; i < 6; ++i) mean += H[7 * i];
+    mean /= 6.0;
+    if (lane < 36) {
+        double a = H[lane];
+        if (lane % 7 == 0) a += 1e-9 * mean + P.damping * H[lane];
+        if (P.prior_info > 0.0) M1[lane] = tmul_entry(Ji, Ji, lane) * P.prior_info;  // P = prior_info J^-T J^-1
+        A[lane] = a;
+    }
+    if (lane < 6) rhs[lane] = -v[21 + lane];
+    __syncthreads();
+    if (P.prior_info > 0.0) {
+        if (lane < 36) A[lane] += M1[lane];
+        if (lane < 6) {
+            double s = 0.0;
+            for (int k = 0; k < 6; ++k) s += M1[6 * lane + k] * xi[k];
+            rhs[lane] += s;
+        }
+        __syncthreads();
+    }
+    if (P.degen_thr > 0.0) {  // the degeneracy test: J^T Hg J - thr I positive definite
+        if (lane < 36) M2[lane] = mul_entry(Hg, J, lane);
+        __syncthreads();
+        if (lane < 36) M1[lane] = tmul_entry(J, M2, lane);
+        __syncthreads();
+        if (!wave_cholesky(M1, P.degen_thr, L)) return 0;  // a direction to hold: the scalar routine does it
+    }
+    if (!wave_cholesky(A, 0.0, L)) return 0;  // indefinite: the scalar routine's LU fallback
+    if (lane == 0) {
+        for (int i = 0; i < 6; ++i) {
+            double t = rhs[i];
+            for (int k = 0; k < i; ++k) t -= L[6 * i + k] * y[k];
+            y[i] = t / L[6 * i + i];
+        }
+        for (int i = 5; i >= 0; --i) {
+            double t = y[i];
+            for (int k = i + 1; k < 6; ++k) t -= L[6 * k + i] * out[k];
+            out[i] = t / L[6 * i + i];
+        }
+        LW_UNROLL for (int i = 0; i < 6; ++i) o.delta[i] = out[i];
+    }
+    if (lane < 36) H_out[lane] = H[lane];
+    return 1;
+}
+
+}  // namespace gnw
+}  // namespace lw

@@ -24,6 +24,7 @@
 //
 // Accumulation forms (acc_mode): 0 float64 (the reference), 1 compensated float32 (Neumaier sums, two-sum
 // merges), 2 naive float32.
+#include "lw_gn_wave.h"
 #include "lw_internal.h"
 #include "lw_search.h"
 
@@ -172,6 +173,25 @@ struct IcpArgs {
     gn::StepParams sp;
 };
 
+// What one workgroup hands to another within a launch -- partial rows, the step, the done flag, the ticket --
+// moves through device-coherent accesses (agent scope: they bypass this XCD's non-coherent L2 copy), and a
+// writer releases before it raises the flag (ticket, generation) the reader waits on. No pass invalidates the
+// L2, so the map the search reads stays cached there.
+template <class T>
+__device__ inline void put(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class T>
+__device__ inline T get(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A writer's release before it raises a flag: the agent-scope release writes back this XCD's dirty L2 lines
+// (few: the shared data goes through device-coherent stores) and waits for the stores. Readers need no
+// acquire: they read the shared data with device-coherent loads, so nothing is invalidated.
+__device__ inline void stores_done() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+
 // What changes from pass to pass (kept out of IcpArgs so the persistent loop never copies the arguments).
 struct Pass {
     int32_t n;
@@ -291,7 +311,7 @@ __device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf
     wave_sum<MODE, S>(lds, sum, comp);
 }
 
-// Sum the G partial rows in workgroup order (the last workgroup, after its acquire fence), 32 rows in flight
+// Sum the G partial rows in workgroup order (the last workgroup, after its ticket; rows read device-coherently), 32 rows in flight
 // per lane. Lane k < RED_COLS returns entry k of the reduced row: the 53 ACC entries, then the measurement
 // counts (hits, matched points) in 53 and 54.
 constexpr int RED_COLS = ACC_USED + 2;
@@ -306,7 +326,7 @@ __device__ inline double reduce_rows(const double *partials, int G) {
         double t = 0.0;
         for (int b = 0; b < G; b += RED_BATCH) {
             double x[RED_BATCH];
-            LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) x[u] = b + u < G ? col[(size_t)(b + u) * 128] : 0.0;
+            LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) x[u] = b + u < G ? get(col + (size_t)(b + u) * 128) : 0.0;
             LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) if (b + u < G) t += x[u];
         }
         return t;
@@ -315,7 +335,7 @@ __device__ inline double reduce_rows(const double *partials, int G) {
         double e = 0.0;  // the counts (integers: summed exactly)
         for (int b = 0; b < G; b += RED_BATCH) {
             double x[RED_BATCH];
-            LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) x[u] = b + u < G ? col[(size_t)(b + u) * 128] : 0.0;
+            LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) x[u] = b + u < G ? get(col + (size_t)(b + u) * 128) : 0.0;
             LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) {
                 if (b + u >= G) continue;
                 t += (float)x[u];
@@ -329,8 +349,8 @@ __device__ inline double reduce_rows(const double *partials, int G) {
         for (int b = 0; b < G; b += RED_BATCH) {
             double x[RED_BATCH], y[RED_BATCH];
             LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) {
-                x[u] = b + u < G ? col[(size_t)(b + u) * 128] : 0.0;
-                y[u] = b + u < G && lane < ACC_USED ? col[(size_t)(b + u) * 128 + 64] : 0.0;
+                x[u] = b + u < G ? get(col + (size_t)(b + u) * 128) : 0.0;
+                y[u] = b + u < G && lane < ACC_USED ? get(col + (size_t)(b + u) * 128 + 64) : 0.0;
             }
             LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) {
                 if (b + u >= G) continue;
@@ -345,35 +365,75 @@ __device__ inline double reduce_rows(const double *partials, int G) {
     }
 }
 
-// The float64 step after a pass (odom.py:769-796), in one lane. Not inlined: its 6x6 temporaries get their
-// own register budget instead of adding to the accumulation's.
-__device__ __noinline__ void gauss_newton(RegState *st, const double *v, const gn::StepParams &sp, int32_t it) {
-    gn::StepOut o;
-    int rc = gn::step(v, st->T, st->T_pred, sp, o);
-    st->corr = o.corr;
-    if (rc == gn::STEP_FEW) {  // corr < min_corr: stop (odom.py:770-771)
-        st->done = 1;
-        st->status = rc;
-        return;
+// After an accepted step (odom.py:794-796): the pose T (the caller's copy: LDS in the persistent loop, the
+// registration state otherwise) moves by delta; the increment is published for the persistent loop's other
+// workgroups, the pose mirrored into the registration state for the host, and the convergence test set.
+__device__ inline void step_tail(RegState *st, double *T, const double *delta, double rms, int32_t n_degen,
+                                 int32_t it) {
+    st->rms = rms;
+    if (it == 0) st->rms0 = rms;
+    st->n_degen = n_degen;
+    gn::apply_twist(T, delta);
+    LW_UNROLL for (int k = 0; k < 6; ++k) put(&st->delta[k], delta[k]);
+    LW_UNROLL for (int k = 0; k < 16; ++k) {
+        if (T != st->T) st->T[k] = T[k];
+        st->Tf[k] = (float)T[k];
     }
-    st->rms = o.rms;
-    if (it == 0) st->rms0 = o.rms;
-    for (int k = 0; k < 36; ++k) st->H[k] = o.H[k];
-    if (rc == gn::STEP_SINGULAR) {  // LinAlgError: stop (odom.py:791-792)
-        st->done = 1;
-        st->status = rc;
-        return;
-    }
-    st->n_degen = o.n_degen;
-    gn::apply_twist(st->T, o.delta);
-    for (int k = 0; k < 16; ++k) st->Tf[k] = (float)st->T[k];
     st->iters = it + 1;
-    double nr = sqrt(o.delta[0] * o.delta[0] + o.delta[1] * o.delta[1] + o.delta[2] * o.delta[2]);
-    double nt = sqrt(o.delta[3] * o.delta[3] + o.delta[4] * o.delta[4] + o.delta[5] * o.delta[5]);
+    double nr = sqrt(delta[0] * delta[0] + delta[1] * delta[1] + delta[2] * delta[2]);
+    double nt = sqrt(delta[3] * delta[3] + delta[4] * delta[4] + delta[5] * delta[5]);
     if (nr < 1e-4 && nt < 1e-4) {  // STEP_TOL (odom.py:88, 795-796)
-        st->done = 1;
+        put(&st->done, 1);
         st->status = 3;
     }
+}
+
+// The float64 step after a pass (odom.py:769-796), in one lane: the general routine, for the steps the wave
+// form does not take (a degenerate direction to hold, or an indefinite system). Not inlined: its 6x6
+// temporaries get their own register budget instead of adding to the accumulation's. The parameters come by
+// value: a reference into the kernel's arguments would make the compiler copy the whole argument block to
+// private memory, and every pointer in it would then be read back from there as a generic (flat) pointer.
+__device__ __noinline__ void gauss_newton(RegState *st, const double *v, const gn::StepParams sp, int32_t it,
+                                          double *T, const double *Tp) {
+    gn::StepOut o;
+    int rc = gn::step(v, T, Tp, sp, o);
+    st->corr = o.corr;
+    if (rc == gn::STEP_FEW) {  // corr < min_corr: stop (odom.py:770-771)
+        put(&st->done, 1);
+        st->status = rc;
+        return;
+    }
+    for (int k = 0; k < 36; ++k) st->H[k] = o.H[k];
+    if (rc == gn::STEP_SINGULAR) {  // LinAlgError: stop (odom.py:791-792)
+        st->rms = o.rms;
+        if (it == 0) st->rms0 = o.rms;
+        put(&st->done, 1);
+        st->status = rc;
+        return;
+    }
+    step_tail(st, T, o.delta, o.rms, o.n_degen, it);
+}
+
+// The float64 step on the whole wave (lw_gn_wave.h) when it applies, otherwise gauss_newton in lane 0. Same
+// results either way. Inlined: a call would save and restore much of the register file around it; the
+// general routine, the rare path, stays a call.
+__device__ inline void wave_gauss_newton(RegState *st, const double *v, const gn::StepParams &sp, int32_t it,
+                                         double *sh, double *T, const double *Tp, unsigned long long *prof) {
+    gnw::WaveOut o;
+    const int r = gnw::wave_step(v, T, Tp, sp, sh, o, st->H);
+    if (threadIdx.x != 0) return;
+    if (prof) atomicAdd(&prof[r == 0 ? 6 : 5], 1ull);  // steps taken on the wave / in one lane
+    if (r == 0) {
+        gauss_newton(st, v, sp, it, T, Tp);
+        return;
+    }
+    st->corr = o.corr;
+    if (r < 0) {  // corr < min_corr: stop (odom.py:770-771)
+        put(&st->done, 1);
+        st->status = gn::STEP_FEW;
+        return;
+    }
+    step_tail(st, T, o.delta, o.rms, 0, it);
 }
 
 // Write this workgroup's row (its 53 sums and compensations, and its measurement counts in the spare
@@ -384,23 +444,22 @@ __device__ inline bool row_and_ticket(const IcpArgs &a, double sum, double comp,
     const int lane = threadIdx.x;
     double *row = a.partials + (size_t)blockIdx.x * 128;
     if (lane < ACC_USED) {
-        row[lane] = sum;
-        row[64 + lane] = comp;
+        put(&row[lane], sum);
+        put(&row[64 + lane], comp);
     }
     LW_UNROLL for (int off = 32; off > 0; off >>= 1) {
         hits += __shfl_xor(hits, off, 64);
         matched += __shfl_xor(matched, off, 64);
     }
     if (lane == 0) {
-        row[ACC_USED] = (double)hits;
-        row[ACC_USED + 1] = (double)matched;
+        put(&row[ACC_USED], (double)hits);
+        put(&row[ACC_USED + 1], (double)matched);
     }
-    __threadfence();
+    stores_done();  // the row is device-visible before the ticket says so
     if (lane == 0) *s_last = (atomicAdd(&a.st->ticket, 1u) == (uint32_t)G - 1);
     __syncthreads();
     const bool last = *s_last != 0;
     __syncthreads();
-    if (last) __threadfence();  // acquire: the other workgroups' rows are visible from here on
     return last;
 }
 
@@ -408,17 +467,18 @@ __device__ inline bool row_and_ticket(const IcpArgs &a, double sum, double comp,
 // measurement on, lane 0 adds the pass's phases to the counters, in device wall-clock ticks: from the
 // pass's start (st->t_rel) to this workgroup's ticket (t_ticket), the reduction, and the step.
 template <int MODE, int R>
-__device__ inline void finish_pass(const IcpArgs &a, const Pass p, int G, double *vs, unsigned long long t_ticket) {
+__device__ inline void finish_pass(const IcpArgs &a, const Pass p, int G, double *vs, unsigned long long t_ticket,
+                                   double *sh, double *T, const double *Tp) {
     constexpr int D = 2 * R + 1, K = D * D * D;
     const int lane = threadIdx.x;
     const double v = reduce_rows<MODE>(a.partials, G);
     vs[lane] = v;
     a.st->acc[lane] = lane < ACC_USED ? v : 0.0;
     __syncthreads();
+    const unsigned long long t_red = a.prof ? wall_clock64() : 0ull;
+    if (a.gn) wave_gauss_newton(a.st, vs, a.sp, p.it, sh, T, Tp, a.prof);
     if (lane == 0) {
-        a.st->ticket = 0;
-        const unsigned long long t_red = a.prof ? wall_clock64() : 0ull;
-        if (a.gn) gauss_newton(a.st, vs, a.sp, p.it);
+        put(&a.st->ticket, 0u);
         if (a.prof) {
             const unsigned long long t_step = wall_clock64();
             atomicAdd(&a.prof[0], (unsigned long long)p.n);
@@ -430,7 +490,7 @@ __device__ inline void finish_pass(const IcpArgs &a, const Pass p, int G, double
             atomicAdd(&a.prof[9], t_red - t_ticket);
             atomicAdd(&a.prof[10], t_step - t_red);
             atomicAdd(&a.prof[11], 1ull);
-            a.st->t_rel = wall_clock64();  // the next pass starts when this one is released
+            put(&a.st->t_rel, (unsigned long long)wall_clock64());  // the next pass starts when this one is released
         }
     }
 }
@@ -452,7 +512,8 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_icp(IcpArgs a) {
     if (a.prof && blockIdx.x == 0 && threadIdx.x == 0) a.st->t_rel = wall_clock64();
     wave_pass<MODE, R>(a, p, Tf, G, lds, sum, comp, hits, matched);
     if (!row_and_ticket<MODE>(a, sum, comp, hits, matched, G, &s_last)) return;
-    finish_pass<MODE, R>(a, p, G, vs, a.prof ? wall_clock64() : 0ull);
+    finish_pass<MODE, R>(a, p, G, vs, a.prof ? wall_clock64() : 0ull, reinterpret_cast<double *>(lds), st->T,
+                         st->T_pred);
 }
 
 // ---- the persistent registration ------------------------------------------------------------------
@@ -474,6 +535,7 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_register(LoopArgs L) {
     __shared__ float Tf[16];
     __shared__ int32_t s_last;
     __shared__ double vs[64];
+    __shared__ double sT[16], sTp[16];  // this workgroup's copy of the pose and the prediction
     __shared__ __align__(16) unsigned char lds[RED_BYTES];
     const int32_t n = *L.n_dev;
     if (blockIdx.x == 0 && lane == 0) st->n = n;
@@ -482,8 +544,13 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_register(LoopArgs L) {
     if ((int64_t)blockIdx.x * ICP_BLOCK >= (int64_t)n * Split<2>::S) return;
     const IcpArgs &a = L.a;
     if (a.prof && blockIdx.x == 0 && lane == 0) st->t_rel = wall_clock64();  // block 0 takes part in pass 0
+    if (lane < 16) {
+        sT[lane] = st->T[lane];
+        sTp[lane] = st->T_pred[lane];
+    }
+    __syncthreads();
     for (int it = 0; it < L.max_iter; ++it) {
-        if (lane < 16) Tf[lane] = __hip_atomic_load(&st->Tf[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane < 16) Tf[lane] = (float)sT[lane];
         __syncthreads();
         // measurement: block 0's view of the pass (wake-up after the release, search, row and ticket)
         const bool stamp = a.prof && blockIdx.x == 0;
@@ -517,14 +584,14 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_register(LoopArgs L) {
         if (last) {
             const unsigned long long t_ticket = a.prof ? wall_clock64() : 0ull;
             if (reach == 2)
-                finish_pass<MODE, 2>(a, p, G, vs, t_ticket);
+                finish_pass<MODE, 2>(a, p, G, vs, t_ticket, reinterpret_cast<double *>(lds), sT, sTp);
             else
-                finish_pass<MODE, 1>(a, p, G, vs, t_ticket);
-            __threadfence();  // release: pose, done flag and ticket before the generation
-            if (lane == 0) __hip_atomic_store(&st->gen, (uint32_t)(it + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                finish_pass<MODE, 1>(a, p, G, vs, t_ticket, reinterpret_cast<double *>(lds), sT, sTp);
+            stores_done();  // the step, done flag and ticket reset are device-visible before the generation
+            if (lane == 0) put(&st->gen, (uint32_t)(it + 1));
         } else {
-            // poll with relaxed loads (they read through to L2 without invalidating this CU's caches, which
-            // the waves still searching share), then one acquire fence
+            // poll the generation with device-coherent loads; the step and the done flag are read the same way,
+            // so no cache is written back or invalidated
             uint32_t spins = 0;
             while (__hip_atomic_load(&st->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < (uint32_t)(it + 1)) {
                 __builtin_amdgcn_s_sleep(8);
@@ -533,7 +600,12 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_register(LoopArgs L) {
                     return;
                 }
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the pose and done flag written before gen
+            // the pass's step, applied to this workgroup's copy of the pose (the last workgroup's is moved)
+            if (lane == 0 && !__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                double d[6];
+                LW_UNROLL for (int k = 0; k < 6; ++k) d[k] = get(&st->delta[k]);
+                gn::apply_twist(sT, d);
+            }
         }
         __syncthreads();
         if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;

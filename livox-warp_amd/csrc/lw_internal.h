@@ -19,6 +19,7 @@ struct RegState {
     double T[16];       // current pose, float64
     double T_pred[16];  // the prediction the prior pulls toward
     double H[36];       // H of the last accepted accumulation (for the condition number)
+    double delta[6];    // the last accepted step (the persistent loop's workgroups apply it to their pose)
     double acc[64];     // last reduced accumulator (ACC layout)
     double rms;
     double rms0;        // the residual of the first pass (at the prediction)
