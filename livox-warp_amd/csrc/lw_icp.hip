@@ -466,10 +466,9 @@ __device__ inline bool row_and_ticket(const IcpArgs &a, double sum, double comp,
 // The last workgroup's part of a pass: reduce the rows, then (registration) the float64 step in lane 0. With
 // measurement on, lane 0 adds the pass's phases to the counters, in device wall-clock ticks: from the
 // pass's start (st->t_rel) to this workgroup's ticket (t_ticket), the reduction, and the step.
-template <int MODE, int R>
-__device__ inline void finish_pass(const IcpArgs &a, const Pass p, int G, double *vs, unsigned long long t_ticket,
+template <int MODE>
+__device__ inline void finish_pass(const IcpArgs &a, const Pass p, int K, int G, double *vs, unsigned long long t_ticket,
                                    double *sh, double *T, const double *Tp) {
-    constexpr int D = 2 * R + 1, K = D * D * D;
     const int lane = threadIdx.x;
     const double v = reduce_rows<MODE>(a.partials, G);
     vs[lane] = v;
@@ -512,8 +511,9 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_icp(IcpArgs a) {
     if (a.prof && blockIdx.x == 0 && threadIdx.x == 0) a.st->t_rel = wall_clock64();
     wave_pass<MODE, R>(a, p, Tf, G, lds, sum, comp, hits, matched);
     if (!row_and_ticket<MODE>(a, sum, comp, hits, matched, G, &s_last)) return;
-    finish_pass<MODE, R>(a, p, G, vs, a.prof ? wall_clock64() : 0ull, reinterpret_cast<double *>(lds), st->T,
-                         st->T_pred);
+    constexpr int D = 2 * R + 1;
+    finish_pass<MODE>(a, p, D * D * D, G, vs, a.prof ? wall_clock64() : 0ull, reinterpret_cast<double *>(lds), st->T,
+                      st->T_pred);
 }
 
 // ---- the persistent registration ------------------------------------------------------------------
@@ -583,10 +583,7 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_register(LoopArgs L) {
         }
         if (last) {
             const unsigned long long t_ticket = a.prof ? wall_clock64() : 0ull;
-            if (reach == 2)
-                finish_pass<MODE, 2>(a, p, G, vs, t_ticket, reinterpret_cast<double *>(lds), sT, sTp);
-            else
-                finish_pass<MODE, 1>(a, p, G, vs, t_ticket, reinterpret_cast<double *>(lds), sT, sTp);
+            finish_pass<MODE>(a, p, reach == 2 ? 125 : 27, G, vs, t_ticket, reinterpret_cast<double *>(lds), sT, sTp);
             stores_done();  // the step, done flag and ticket reset are device-visible before the generation
             if (lane == 0) put(&st->gen, (uint32_t)(it + 1));
         } else {
