@@ -302,7 +302,9 @@ def main():
     odo.acc_mode = args.acc
     odo.reset(seq.pose(0.05))
     L = odo._lib
-    lw._abi.check(L.lw_prof_enable(odo._ctx, 1), "lw_prof_enable")
+    # the warm-up frames run with the device phase stamps on (where a pass's time goes); the timed region
+    # with the launch events and counters only
+    lw._abi.check(L.lw_prof_enable(odo._ctx, 2), "lw_prof_enable")
 
     def on_frame(*_):
         pass
@@ -314,6 +316,9 @@ def main():
         push(k)
     odo.synchronize()
     torch.cuda.synchronize()
+    phases = np.zeros(15)
+    lw._abi.check(L.lw_prof_phases(odo._ctx, lw._abi.ptr(phases, lw._abi._f64p)), "lw_prof_phases")
+    lw._abi.check(L.lw_prof_enable(odo._ctx, 1), "lw_prof_enable")
     lw._abi.check(L.lw_prof_reset(odo._ctx), "lw_prof_reset")
     f0 = len(odo.traj)
     t0 = time.perf_counter()
@@ -325,8 +330,6 @@ def main():
     timed = len(odo.traj) - f0
     prof = np.zeros(8)
     lw._abi.check(L.lw_prof_read(odo._ctx, lw._abi.ptr(prof, lw._abi._f64p)), "lw_prof_read")
-    phases = np.zeros(10)
-    lw._abi.check(L.lw_prof_phases(odo._ctx, lw._abi.ptr(phases, lw._abi._f64p)), "lw_prof_phases")
     lw._abi.check(L.lw_prof_enable(odo._ctx, 0), "lw_prof_enable")
     fps = timed / el
     launch_s = prof[1] * 1e-3 / max(prof[0], 1.0)
@@ -357,14 +360,17 @@ def main():
                          "avg_launch_us": round(launch_s * 1e6, 3), "point_iterations": int(prof[2]),
                          "lookups": int(prof[3]), "hits": int(prof[4]), "matched": int(prof[5]),
                          "time_share": round(prof[1] * 1e-3 / el, 4),
+                         "phase_frames": args.warmup,
                          "us_per_pass": {"search": round(phases[0] / max(phases[3], 1), 2),
                                          "reduce": round(phases[1] / max(phases[3], 1), 2),
                                          "step": round(phases[2] / max(phases[3], 1), 2),
-                                         "launch": round(launch_s * 1e6 * prof[0] / max(phases[3], 1), 2),
-                                         "block0_wake": round(phases[4] / max(phases[7], 1), 2),
-                                         "block0_search": round(phases[5] / max(phases[7], 1), 2),
-                                         "block0_row_ticket": round(phases[6] / max(phases[7], 1), 2)},
-                         "steps_on_wave": int(phases[8]), "steps_one_lane": int(phases[9])},
+                                         "launch": round(prof[1] * 1e3 / max(prof[6], 1), 2),
+                                         "searcher_wake": round(phases[4] / max(phases[7], 1), 2),
+                                         "searcher_search": round(phases[5] / max(phases[7], 1), 2),
+                                         "searcher_row_flag": round(phases[6] / max(phases[7], 1), 2)},
+                         "steps_on_wave": int(phases[8]), "steps_one_lane": int(phases[9]),
+                         "us_per_wave_step": {k: round(phases[10 + i] / max(phases[14], 1), 3) for i, k in
+                                              enumerate(("setup", "degeneracy", "solve", "update"))}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic("k_register<0>"),
                      "kernel": "k_register<0> (one launch per registration: every accumulation pass and step)",

@@ -195,21 +195,26 @@ int lw_mlp_adam(lw_mlp *m, float lr, float beta1, float beta2, float eps, int32_
  * count * 64 / m. */
 int lw_frame_features(lw_ctx *ctx, int32_t m, float r_max, float e_max, const float *motion, float *out);
 
-/* Live measurement of the accumulation launches (for throughput and roofline reports): lw_prof_enable
- * turns on HIP events around each accumulation launch and device counters; lw_prof_read returns, since
+/* Live measurement of the accumulation launches (for throughput and roofline reports): lw_prof_enable(ctx, 1)
+ * turns on HIP events around each accumulation launch and device counters, lw_prof_enable(ctx, 2) also the
+ * device clock stamps that lw_prof_phases reports (they slow the loop a little), 0 turns both off;
+ * lw_prof_read returns, since
  * lw_prof_reset: out[0] launches timed, out[1] their total duration (ms), out[2] point-iterations, out[3]
  * neighbour lookups, out[4] occupied neighbours hit, out[5] matched points, out[6] launches that did work,
  * out[7] reserved. No counterpart in the reference (its timing is host wall time, odom.py:644, 705). */
 int lw_prof_enable(lw_ctx *ctx, int32_t on);
 int lw_prof_reset(lw_ctx *ctx);
 int lw_prof_read(lw_ctx *ctx, double *out);
-/* Where the registration passes spend their time, since lw_prof_reset, from device wall-clock stamps of each
- * pass's last workgroup (microseconds, summed over passes): out[0] from the pass's start to the last
- * workgroup's ticket (search, terms, partial rows), out[1] the cross-workgroup reduction, out[2] the float64
- * step; out[3] the passes timed. Block 0's view of the persistent loop's passes: out[4] from the release of
- * the previous pass to its wake-up, out[5] its search and terms, out[6] its row write and ticket; out[7] the
- * passes it timed. out[8] steps taken on the whole wave, out[9] steps taken by the general one-lane routine
- * (a degenerate direction, an indefinite system). No counterpart in the reference. */
+/* Where the registration passes spend their time, since lw_prof_reset, from device wall-clock stamps
+ * (microseconds, summed over passes; lw_prof_enable level 2): out[0] from the pass's start to the moment the
+ * workgroup that finishes the pass has every partial row (search, terms, rows), out[1] the rest of the
+ * cross-workgroup reduction, out[2] the float64 step and the pose's publication; out[3] the passes timed.
+ * The first searching workgroup's view of the persistent loop's passes: out[4] from the publication of the
+ * pose to its wake-up, out[5] its search and terms, out[6] its row write and flag; out[7] the passes it
+ * timed. out[8] steps taken on the whole wave, out[9] steps taken by the general one-lane routine
+ * (a degenerate direction, an indefinite system). The wave steps' parts: out[10] set-up (matrices, basis,
+ * prior), out[11] the degeneracy test, out[12] the damped solve, out[13] the pose update and publication;
+ * out[14] the wave steps timed. out holds 15 doubles. No counterpart in the reference. */
 int lw_prof_phases(lw_ctx *ctx, double *out);
 
 /* Host builds of the device's float64 step and per-voxel precision, for tests on machines without a

@@ -48,7 +48,7 @@ void free_all(lw_ctx *c) {
     void *ptrs[] = {c->slots, c->occ, c->vox_key, c->vox_mom, c->vox_slot, c->vox_prec, c->vox_nin, c->vox_quota,
                     c->vox_dirty, c->dirty_list, c->touched, c->ctr, c->pt_slot, c->pt_vid, c->pt_take,
                     c->ds_keys, c->ds_idx, c->ds_xyz, c->blk_cnt, c->blk_off, c->reg, c->partials, c->best_slot,
-                    c->dtmp};
+                    c->dtmp, c->loop_flags};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->prof_dev) (void)hipFree(c->prof_dev);
@@ -151,6 +151,7 @@ int lw_create(lw_ctx **out, int64_t reg_slots, int64_t frame_cap, void *stream) 
     if ((e = dalloc(&c->blk_off, c->max_blocks)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->reg, 1)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->partials, (size_t)lw::icp_max_groups(frame_cap) * 128)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->loop_flags, lw::MAX_LOOP_ROWS)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->best_slot, frame_cap)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->dtmp, frame_cap + 64)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->loc_queue, 16)) != hipSuccess) return fail(e);
@@ -569,11 +570,12 @@ int lw_prof_enable(lw_ctx *c, int32_t on) {
             }
         }
         c->prof_cap = cap;
-        LW_CK(hipMalloc((void **)&c->prof_dev, 16 * sizeof(unsigned long long)));
-        LW_CK(hipMemsetAsync(c->prof_dev, 0, 16 * sizeof(unsigned long long), c->stream));
+        LW_CK(hipMalloc((void **)&c->prof_dev, lw::LW_PROF_COUNTERS * sizeof(unsigned long long)));
+        LW_CK(hipMemsetAsync(c->prof_dev, 0, lw::LW_PROF_COUNTERS * sizeof(unsigned long long), c->stream));
         LW_CK(hipStreamSynchronize(c->stream));
     }
     c->prof_on = on != 0;
+    c->prof_phases = on >= 2;
     return 0;
 }
 
@@ -584,7 +586,7 @@ int lw_prof_reset(lw_ctx *c) {
     c->prof_ms = 0.0;
     c->prof_launches = 0;
     if (c->prof_dev) {
-        LW_CK(hipMemsetAsync(c->prof_dev, 0, 16 * sizeof(unsigned long long), c->stream));
+        LW_CK(hipMemsetAsync(c->prof_dev, 0, lw::LW_PROF_COUNTERS * sizeof(unsigned long long), c->stream));
         LW_CK(hipStreamSynchronize(c->stream));
     }
     return 0;
@@ -594,7 +596,7 @@ int lw_prof_read(lw_ctx *c, double *out) {
     if (!c || !out) return LW_EINVAL;
     LW_CK(hipStreamSynchronize(c->stream));
     LW_CK(prof_harvest(c));
-    unsigned long long v[16] = {0};
+    unsigned long long v[lw::LW_PROF_COUNTERS] = {0};
     if (c->prof_dev) LW_CK(hipMemcpy(v, c->prof_dev, sizeof(v), hipMemcpyDeviceToHost));
     out[0] = (double)c->prof_launches;
     out[1] = c->prof_ms;
@@ -606,7 +608,7 @@ int lw_prof_read(lw_ctx *c, double *out) {
 int lw_prof_phases(lw_ctx *c, double *out) {
     if (!c || !out) return LW_EINVAL;
     LW_CK(hipStreamSynchronize(c->stream));
-    unsigned long long v[16] = {0};
+    unsigned long long v[lw::LW_PROF_COUNTERS] = {0};
     if (c->prof_dev) LW_CK(hipMemcpy(v, c->prof_dev, sizeof(v), hipMemcpyDeviceToHost));
     int khz = 0;
     LW_CK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
@@ -617,6 +619,8 @@ int lw_prof_phases(lw_ctx *c, double *out) {
     out[7] = (double)v[15];
     out[8] = (double)v[5];
     out[9] = (double)v[6];
+    for (int i = 0; i < 4; ++i) out[10 + i] = (double)v[16 + i] * us_per_tick;
+    out[14] = (double)v[20];
     return 0;
 }
 

@@ -19,7 +19,7 @@ namespace gnw {
 
 // LDS layout (doubles) of the step's scratch
 constexpr int W_H = 0, W_HG = 36, W_A = 72, W_J = 108, W_JI = 144, W_M1 = 180, W_M2 = 216, W_L = 252, W_RHS = 288,
-              W_XI = 294, W_Y = 300, W_OUT = 306, W_T = 312, W_DOUBLES = 328;
+              W_Y = 294, W_OUT = 300, W_T = 306, W_DOUBLES = 322;
 
 // Index of (p, q), p <= q, in a row-major upper triangle of a 6x6 matrix (odom.py:387, np.triu_indices(6)).
 __device__ inline int triu(int p, int q) { return p * 6 - (p * (p - 1)) / 2 + (q - p); }
@@ -109,17 +109,18 @@ struct WaveOut {
 
 // The step on all 64 lanes of the calling wave (a one-wave workgroup). sh: W_DOUBLES doubles of LDS. Returns
 // 1 when handled (lane 0's o holds delta, and H_out, the registration's copy of H, has been written), 0 when
-// the caller must run gn::step instead, or -1 for gn::STEP_FEW (corr < min_corr). T, T_pred: the registration
-// state (global memory).
-__device__ inline int wave_step(const double *v, const double *T, const double *T_pred, const gn::StepParams &P,
-                                double *sh, WaveOut &o, double *H_out) {
+// the caller must run gn::step instead, or -1 for gn::STEP_FEW (corr < min_corr). T: the pose. xi: the twist
+// from T to the prediction (gn::twist_between(T, T_pred), read when P.prior_info > 0), computed by the caller
+// (the persistent loop does it while the pass's search runs). stamps (measurement, or null): lane 0 records
+// the device wall clock after the set-up, the degeneracy test and the solve.
+__device__ inline int wave_step(const double *v, const double *T, const double *xi, const gn::StepParams &P,
+                                double *sh, WaveOut &o, double *H_out, unsigned long long *stamps) {
     const int lane = threadIdx.x;
     o.corr = (int32_t)v[27];
     o.n_degen = 0;
     if (o.corr < P.min_corr) return -1;
     double *H = sh + W_H, *Hg = sh + W_HG, *A = sh + W_A, *J = sh + W_J, *Ji = sh + W_JI, *M1 = sh + W_M1,
-           *M2 = sh + W_M2, *L = sh + W_L, *rhs = sh + W_RHS, *xi = sh + W_XI, *y = sh + W_Y, *out = sh + W_OUT,
-           *Ts = sh + W_T;
+           *M2 = sh + W_M2, *L = sh + W_L, *rhs = sh + W_RHS, *y = sh + W_Y, *out = sh + W_OUT, *Ts = sh + W_T;
     if (lane < 16) Ts[lane] = T[lane];
     __syncthreads();
     double Lr = v[31] / (double)o.corr;
@@ -136,11 +137,6 @@ __device__ inline int wave_step(const double *v, const double *T, const double *
         basis_entry(Ts, Lr, lane, bj, bji);
         J[lane] = bj;
         Ji[lane] = bji;
-    }
-    if (lane == 0) {
-        double x[6];
-        gn::twist_between(T, T_pred, x);
-        LW_UNROLL for (int k = 0; k < 6; ++k) xi[k] = x[k];
     }
     __syncthreads();
     double mean = 0.0;
@@ -163,6 +159,7 @@ __device__ inline int wave_step(const double *v, const double *T, const double *
         }
         __syncthreads();
     }
+    if (stamps && lane == 0) stamps[0] = wall_clock64();
     if (P.degen_thr > 0.0) {  // the degeneracy test: J^T Hg J - thr I positive definite
         if (lane < 36) M2[lane] = mul_entry(Hg, J, lane);
         __syncthreads();
@@ -170,6 +167,7 @@ __device__ inline int wave_step(const double *v, const double *T, const double *
         __syncthreads();
         if (!wave_cholesky(M1, P.degen_thr, L)) return 0;  // a direction to hold: the scalar routine does it
     }
+    if (stamps && lane == 0) stamps[1] = wall_clock64();
     if (!wave_cholesky(A, 0.0, L)) return 0;  // indefinite: the scalar routine's LU fallback
     if (lane == 0) {
         for (int i = 0; i < 6; ++i) {
@@ -183,6 +181,7 @@ __device__ inline int wave_step(const double *v, const double *T, const double *
             out[i] = t / L[6 * i + i];
         }
         LW_UNROLL for (int i = 0; i < 6; ++i) o.delta[i] = out[i];
+        if (stamps) stamps[2] = wall_clock64();
     }
     if (lane < 36) H_out[lane] = H[lane];
     return 1;

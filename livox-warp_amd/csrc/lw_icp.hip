@@ -12,15 +12,17 @@
 // (odom.py:246-259, 269-271, 372-375).
 //
 // Reduction: the wave's terms are transposed through LDS and summed per entry in lane order; each
-// workgroup writes one partial row; the last workgroup to finish (atomic ticket) sums the rows in
-// workgroup order and, in a registration, runs the float64 step (lw_gn.h) in one lane.
+// workgroup writes one partial row; the rows are summed in batches of 32 (each batch in row order, then the
+// batch sums in batch order), and in a registration the float64 step (lw_gn.h, on the whole wave when it
+// applies) follows.
 //
 // Two drivers:
-//   k_icp       one pass per launch (lw_icp_accum, and the registration fallback);
-//   k_register  the whole loop in one cooperative launch: after each pass the last workgroup updates the
-//               pose and bumps a generation counter, the others wait for it (bounded spin; co-residency
-//               is what the cooperative launch guarantees), so a registration is one launch with no idle
-//               passes after convergence and reads the thinned count on the device.
+//   k_icp       one pass per launch (lw_icp_accum, and the registration fallback): the last workgroup to
+//               finish (atomic ticket) reduces and steps;
+//   k_register  the whole loop in one cooperative launch: a leader workgroup reduces the rows as they come
+//               in, runs the step and publishes the pose; the searching workgroups wait for it (bounded
+//               spin; co-residency is what the cooperative launch guarantees), so a registration is one
+//               launch with no idle passes after convergence and reads the thinned count on the device.
 //
 // Accumulation forms (acc_mode): 0 float64 (the reference), 1 compensated float32 (Neumaier sums, two-sum
 // merges), 2 naive float32.
@@ -164,7 +166,8 @@ struct IcpArgs {
     RegState *st;
     double *partials;  // [G][128]: sums, compensations
     int32_t *best_slot;
-    unsigned long long *prof;  // measurement counters (see lw_ctx), or null
+    unsigned long long *prof;   // measurement counters (lw_ctx::prof_dev), or null
+    unsigned long long *phase;  // the same counters when the phase clock stamps are on, or null
     int32_t n;
     int32_t mask;
     float inv_v, voxel, max_d2, kappa;
@@ -221,7 +224,7 @@ struct Split<2> {
 template <int MODE>
 __device__ inline void point_terms(const IcpArgs &a, float kappa, float px, float py, float pz, float wx, float wy,
                                    float wz, const Cand &b, Acc<MODE> &acc) {
-    const int32_t vid = a.slots[b.s].vid;
+    const int32_t vid = b.vid;
     const float r0 = wx - b.mx, r1 = wy - b.my, r2 = wz - b.mz;
     const VoxPrec P = a.vox_prec[vid];
     const float M[3][3] = {{P.M[0], P.M[1], P.M[2]}, {P.M[1], P.M[3], P.M[4]}, {P.M[2], P.M[4], P.M[5]}};
@@ -264,17 +267,17 @@ __device__ inline void point_terms(const IcpArgs &a, float kappa, float px, floa
     acc.add(ACC_RANGE, length3(px, py, pz));
 }
 
-// One workgroup's share of a pass: the points of lane groups blockIdx.x, blockIdx.x + G, ... (one group
-// per workgroup whenever the grid is large enough). Lane k < 53 returns the workgroup's entry k.
+// One workgroup's share of a pass: the points of lane groups g, g + G, ... (one group per workgroup whenever
+// the grid is large enough). Lane k < 53 returns the workgroup's entry k.
 template <int MODE, int R>
-__device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf, int G, unsigned char *lds, double &sum,
-                                 double &comp, int32_t &hits, int32_t &matched) {
+__device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf, int g, int G, unsigned char *lds,
+                                 double &sum, double &comp, int32_t &hits, int32_t &matched) {
     constexpr int S = Split<R>::S, C = Split<R>::C;
     constexpr int D = 2 * R + 1, K = D * D * D, KQ = (K + S - 1) / S;
     const int lane = threadIdx.x;
     const int sub = lane % S;
     row_zero<MODE>(lds);
-    for (int64_t base = (int64_t)blockIdx.x * ICP_BLOCK; base < (int64_t)p.n * S; base += (int64_t)G * ICP_BLOCK) {
+    for (int64_t base = (int64_t)g * ICP_BLOCK; base < (int64_t)p.n * S; base += (int64_t)G * ICP_BLOCK) {
         const int32_t i = (int32_t)((base + lane) / S);
         float px = 0.0f, py = 0.0f, pz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
         Cand best{p.max_d2, -1, K, 0.0f, 0.0f, 0.0f};
@@ -295,6 +298,7 @@ __device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf
             o.mx = __shfl_xor(best.mx, off, 64);
             o.my = __shfl_xor(best.my, off, 64);
             o.mz = __shfl_xor(best.mz, off, 64);
+            o.vid = __shfl_xor(best.vid, off, 64);
             if (o.s >= 0 && (best.s < 0 || o.d2 < best.d2 || (o.d2 == best.d2 && o.j < best.j))) best = o;
         }
         if (i < p.n && sub == 0) {
@@ -311,141 +315,224 @@ __device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf
     wave_sum<MODE, S>(lds, sum, comp);
 }
 
-// Sum the G partial rows in workgroup order (the last workgroup, after its ticket; rows read device-coherently), 32 rows in flight
-// per lane. Lane k < RED_COLS returns entry k of the reduced row: the 53 ACC entries, then the measurement
-// counts (hits, matched points) in 53 and 54.
+// ---- the cross-workgroup reduction ----------------------------------------------------------------
+// The G partial rows are summed in batches of RED_BATCH rows, each batch in row order from zero, and the
+// batch sums then in batch order. Both drivers use this order (the one-pass launch's last workgroup reads the
+// batches one after another, the persistent loop's leader each batch as soon as its rows are in), so the two
+// give the same sums bit for bit. Lane k < RED_COLS handles entry k: the 53 ACC entries, then the
+// measurement counts (hits, matched points) in 53 and 54. Rows are read device-coherently.
 constexpr int RED_COLS = ACC_USED + 2;
 constexpr int RED_BATCH = 32;
+constexpr int MAX_BATCHES = MAX_LOOP_ROWS / RED_BATCH;
 
+struct BSum {
+    double e;    // float64 sum: the float64 form's entries, and every form's counts (integers: exact)
+    float s, c;  // float32 sum and compensation of the float32 forms
+};
+
+// Rows [b0, b1) (b1 - b0 <= RED_BATCH) of this lane's column, in row order, all loads in flight together.
 template <int MODE>
-__device__ inline double reduce_rows(const double *partials, int G) {
+__device__ inline BSum batch_sum(const double *partials, int b0, int b1) {
     const int lane = threadIdx.x;
-    if (lane >= RED_COLS) return 0.0;
+    BSum r{0.0, 0.0f, 0.0f};
+    if (lane >= RED_COLS) return r;
     const double *col = partials + lane;
-    if constexpr (MODE == 0) {
-        double t = 0.0;
-        for (int b = 0; b < G; b += RED_BATCH) {
-            double x[RED_BATCH];
-            LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) x[u] = b + u < G ? get(col + (size_t)(b + u) * 128) : 0.0;
-            LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) if (b + u < G) t += x[u];
-        }
-        return t;
-    } else if constexpr (MODE == 2) {
-        float t = 0.0f;
-        double e = 0.0;  // the counts (integers: summed exactly)
-        for (int b = 0; b < G; b += RED_BATCH) {
-            double x[RED_BATCH];
-            LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) x[u] = b + u < G ? get(col + (size_t)(b + u) * 128) : 0.0;
-            LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) {
-                if (b + u >= G) continue;
-                t += (float)x[u];
-                e += x[u];
-            }
-        }
-        return lane < ACC_USED ? (double)t : e;
-    } else {
-        float s = 0.0f, c = 0.0f;
-        double e = 0.0;  // the counts
-        for (int b = 0; b < G; b += RED_BATCH) {
-            double x[RED_BATCH], y[RED_BATCH];
-            LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) {
-                x[u] = b + u < G ? get(col + (size_t)(b + u) * 128) : 0.0;
-                y[u] = b + u < G && lane < ACC_USED ? get(col + (size_t)(b + u) * 128 + 64) : 0.0;
-            }
-            LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) {
-                if (b + u >= G) continue;
-                float t, er;
-                two_sum(s, (float)x[u], t, er);
-                s = t;
-                c = c + (float)y[u] + er;
-                e += x[u];
-            }
-        }
-        return lane < ACC_USED ? (double)s + (double)c : e;
+    double x[RED_BATCH], y[RED_BATCH];
+    LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) {
+        x[u] = b0 + u < b1 ? get(col + (size_t)(b0 + u) * 128) : 0.0;
+        if constexpr (MODE == 1) y[u] = b0 + u < b1 && lane < ACC_USED ? get(col + (size_t)(b0 + u) * 128 + 64) : 0.0;
     }
+    LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) {
+        if (b0 + u >= b1) continue;
+        if constexpr (MODE == 0) {
+            r.e += x[u];
+        } else if constexpr (MODE == 2) {
+            r.s += (float)x[u];
+            r.e += x[u];
+        } else {
+            float t, er;
+            two_sum(r.s, (float)x[u], t, er);
+            r.s = t;
+            r.c = r.c + (float)y[u] + er;
+            r.e += x[u];
+        }
+    }
+    return r;
 }
 
-// After an accepted step (odom.py:794-796): the pose T (the caller's copy: LDS in the persistent loop, the
-// registration state otherwise) moves by delta; the increment is published for the persistent loop's other
-// workgroups, the pose mirrored into the registration state for the host, and the convergence test set.
-__device__ inline void step_tail(RegState *st, double *T, const double *delta, double rms, int32_t n_degen,
-                                 int32_t it) {
+template <int MODE>
+__device__ inline void combine(BSum &acc, const BSum &b) {
+    if constexpr (MODE == 1) {
+        float t, er;
+        two_sum(acc.s, b.s, t, er);
+        acc.s = t;
+        acc.c = acc.c + b.c + er;
+    } else if constexpr (MODE == 2) {
+        acc.s += b.s;
+    }
+    acc.e += b.e;
+}
+
+template <int MODE>
+__device__ inline double entry_of(const BSum &r) {
+    if (threadIdx.x >= RED_COLS) return 0.0;
+    if constexpr (MODE == 0) return r.e;
+    else if constexpr (MODE == 2) return threadIdx.x < ACC_USED ? (double)r.s : r.e;
+    else return threadIdx.x < ACC_USED ? (double)r.s + (double)r.c : r.e;
+}
+
+// Every row, batch after batch (the one-pass launch's last workgroup, after its ticket).
+template <int MODE>
+__device__ inline double reduce_rows(const double *partials, int G) {
+    BSum acc{0.0, 0.0f, 0.0f};
+    for (int b = 0; b < G; b += RED_BATCH) combine<MODE>(acc, batch_sum<MODE>(partials, b, min(b + RED_BATCH, G)));
+    return entry_of<MODE>(acc);
+}
+
+// The persistent loop's leader: polls the G row flags (lane l reads rows l, l + 64, ...), sums each batch whose
+// rows all carry this pass's tag into bs (LDS, MAX_BATCHES x 64), in whatever order the batches complete, and
+// combines the batch sums in batch order once every row is in. t_in: the device clock when the last rows
+// were seen (measurement). False when rows are still missing after SPIN_LIMIT polls.
+constexpr uint32_t SPIN_LIMIT = 1u << 24;  // ~ seconds of polling: a wait that long means the loop is broken
+constexpr int32_t STATUS_SPIN_TIMEOUT = -1;
+
+template <int MODE>
+__device__ inline bool leader_reduce(const double *partials, const uint32_t *flags, int G, uint32_t tag, BSum *bs,
+                                     double &v, unsigned long long &t_in, bool stamp) {
+    const int lane = threadIdx.x;
+    const int nb = (G + RED_BATCH - 1) / RED_BATCH;
+    uint32_t pend = (nb >= 32) ? ~0u : ((1u << nb) - 1u);
+    uint32_t spins = 0;
+    while (pend) {
+        uint32_t f[MAX_LOOP_ROWS / 64];
+        LW_UNROLL for (int k = 0; k < MAX_LOOP_ROWS / 64; ++k) {
+            const int r = 64 * k + lane;
+            f[k] = (64 * k < G && r < G) ? get(&flags[r]) : tag;
+        }
+        uint32_t ready = 0;
+        LW_UNROLL for (int k = 0; k < MAX_LOOP_ROWS / 64; ++k) {
+            const unsigned long long m = __ballot(f[k] >= tag);
+            if ((uint32_t)m == ~0u) ready |= 1u << (2 * k);
+            if ((uint32_t)(m >> 32) == ~0u) ready |= 1u << (2 * k + 1);
+        }
+        ready &= pend;
+        if (ready == 0u) {
+            if (++spins > SPIN_LIMIT) return false;
+            continue;
+        }
+        if (stamp && ready == pend) t_in = wall_clock64();
+        pend &= ~ready;
+        while (ready) {
+            const int b = __builtin_ctz(ready);
+            ready &= ready - 1u;
+            bs[b * ICP_BLOCK + lane] = batch_sum<MODE>(partials, b * RED_BATCH, min(b * RED_BATCH + RED_BATCH, G));
+        }
+    }
+    BSum acc{0.0, 0.0f, 0.0f};
+    for (int b = 0; b < nb; ++b) combine<MODE>(acc, bs[b * ICP_BLOCK + lane]);  // this lane's own entries
+    v = entry_of<MODE>(acc);
+    return true;
+}
+
+// ---- the float64 step ------------------------------------------------------------------------------
+
+// An accepted step (odom.py:794-796), lane 0: record it and move the pose T (LDS in the persistent loop, the
+// registration state otherwise) by delta. True when the convergence test stops the loop (STEP_TOL,
+// odom.py:88, 795-796).
+__device__ inline bool accept_step(RegState *st, double *T, const double *delta, double rms, int32_t n_degen,
+                                   int32_t it) {
     st->rms = rms;
     if (it == 0) st->rms0 = rms;
     st->n_degen = n_degen;
     gn::apply_twist(T, delta);
-    LW_UNROLL for (int k = 0; k < 6; ++k) put(&st->delta[k], delta[k]);
-    LW_UNROLL for (int k = 0; k < 16; ++k) {
-        if (T != st->T) st->T[k] = T[k];
-        st->Tf[k] = (float)T[k];
-    }
     st->iters = it + 1;
-    double nr = sqrt(delta[0] * delta[0] + delta[1] * delta[1] + delta[2] * delta[2]);
-    double nt = sqrt(delta[3] * delta[3] + delta[4] * delta[4] + delta[5] * delta[5]);
-    if (nr < 1e-4 && nt < 1e-4) {  // STEP_TOL (odom.py:88, 795-796)
-        put(&st->done, 1);
+    const double nr = sqrt(delta[0] * delta[0] + delta[1] * delta[1] + delta[2] * delta[2]);
+    const double nt = sqrt(delta[3] * delta[3] + delta[4] * delta[4] + delta[5] * delta[5]);
+    if (nr < 1e-4 && nt < 1e-4) {
         st->status = 3;
+        return true;
     }
+    return false;
 }
 
-// The float64 step after a pass (odom.py:769-796), in one lane: the general routine, for the steps the wave
-// form does not take (a degenerate direction to hold, or an indefinite system). Not inlined: its 6x6
-// temporaries get their own register budget instead of adding to the accumulation's. The parameters come by
-// value: a reference into the kernel's arguments would make the compiler copy the whole argument block to
-// private memory, and every pointer in it would then be read back from there as a generic (flat) pointer.
-__device__ __noinline__ void gauss_newton(RegState *st, const double *v, const gn::StepParams sp, int32_t it,
+// The float64 step after a pass (odom.py:769-796) in one lane: the general routine, for the steps the wave
+// form does not take (a degenerate direction to hold, or an indefinite system). True when the loop stops.
+// Not inlined: its 6x6 temporaries get their own register budget. The parameters come by value: a reference
+// into the kernel's arguments would make the compiler copy the whole argument block to private memory, and
+// every pointer in it would then be read back from there as a generic (flat) pointer.
+__device__ __noinline__ bool gauss_newton(RegState *st, const double *v, const gn::StepParams sp, int32_t it,
                                           double *T, const double *Tp) {
     gn::StepOut o;
     int rc = gn::step(v, T, Tp, sp, o);
     st->corr = o.corr;
     if (rc == gn::STEP_FEW) {  // corr < min_corr: stop (odom.py:770-771)
-        put(&st->done, 1);
         st->status = rc;
-        return;
+        return true;
     }
     for (int k = 0; k < 36; ++k) st->H[k] = o.H[k];
     if (rc == gn::STEP_SINGULAR) {  // LinAlgError: stop (odom.py:791-792)
         st->rms = o.rms;
         if (it == 0) st->rms0 = o.rms;
-        put(&st->done, 1);
         st->status = rc;
-        return;
+        return true;
     }
-    step_tail(st, T, o.delta, o.rms, o.n_degen, it);
+    return accept_step(st, T, o.delta, o.rms, o.n_degen, it);
 }
 
-// The float64 step on the whole wave (lw_gn_wave.h) when it applies, otherwise gauss_newton in lane 0. Same
-// results either way. Inlined: a call would save and restore much of the register file around it; the
-// general routine, the rare path, stays a call.
-__device__ inline void wave_gauss_newton(RegState *st, const double *v, const gn::StepParams &sp, int32_t it,
-                                         double *sh, double *T, const double *Tp, unsigned long long *prof) {
+// The step on the whole wave (lw_gn_wave.h) when it applies, otherwise gauss_newton in lane 0 (same results
+// either way), on the reduced row v. xi: the twist from T to the prediction Tp (LDS, when the prior is on).
+// Every lane returns whether the loop stops; st->done records it. With phase measurement on, a wave step's
+// parts go to ph[16..19] (set-up, degeneracy test, solve, pose update; device wall-clock ticks) and ph[20],
+// and ph[5] / ph[6] count the steps taken on the wave / in one lane.
+__device__ inline bool run_step(RegState *st, const double *v, const gn::StepParams &sp, int32_t it, double *sh,
+                                double *T, const double *Tp, const double *xi, int32_t *s_flag,
+                                unsigned long long *ph) {
     gnw::WaveOut o;
-    const int r = gnw::wave_step(v, T, Tp, sp, sh, o, st->H);
-    if (threadIdx.x != 0) return;
-    if (prof) atomicAdd(&prof[r == 0 ? 6 : 5], 1ull);  // steps taken on the wave / in one lane
-    if (r == 0) {
-        gauss_newton(st, v, sp, it, T, Tp);
-        return;
+    unsigned long long t[4] = {0ull, 0ull, 0ull, 0ull};
+    const unsigned long long t0 = ph ? wall_clock64() : 0ull;
+    const int r = gnw::wave_step(v, T, xi, sp, sh, o, st->H, ph ? t : nullptr);
+    if (threadIdx.x == 0) {
+        bool stop;
+        if (r == 0) {
+            stop = gauss_newton(st, v, sp, it, T, Tp);
+        } else {
+            st->corr = o.corr;
+            if (r < 0) {  // corr < min_corr: stop (odom.py:770-771)
+                st->status = gn::STEP_FEW;
+                stop = true;
+            } else {
+                stop = accept_step(st, T, o.delta, o.rms, 0, it);
+            }
+        }
+        if (stop) st->done = 1;
+        *s_flag = stop ? 1 : 0;
+        if (ph) {
+            atomicAdd(&ph[r == 0 ? 6 : 5], 1ull);
+            if (r > 0) {
+                const unsigned long long t3 = wall_clock64();
+                atomicAdd(&ph[16], t[0] - t0);
+                atomicAdd(&ph[17], t[1] - t[0]);
+                atomicAdd(&ph[18], t[2] - t[1]);
+                atomicAdd(&ph[19], t3 - t[2]);
+                atomicAdd(&ph[20], 1ull);
+            }
+        }
     }
-    st->corr = o.corr;
-    if (r < 0) {  // corr < min_corr: stop (odom.py:770-771)
-        put(&st->done, 1);
-        st->status = gn::STEP_FEW;
-        return;
-    }
-    step_tail(st, T, o.delta, o.rms, 0, it);
+    __syncthreads();
+    return *s_flag != 0;
 }
 
-// Write this workgroup's row (its 53 sums and compensations, and its measurement counts in the spare
-// entries 53 and 54), then take a ticket; true for the last workgroup of the pass (which then sees every row).
+// ---- partial rows ------------------------------------------------------------------------------------
+
+// Write workgroup row g (its 53 sums and compensations, and its measurement counts in the spare entries 53
+// and 54) with device-coherent stores, and wait until they are device-visible.
 template <int MODE>
-__device__ inline bool row_and_ticket(const IcpArgs &a, double sum, double comp, int32_t hits, int32_t matched,
-                                      int G, int32_t *s_last) {
+__device__ inline void write_row(const IcpArgs &a, int g, double sum, double comp, int32_t hits, int32_t matched) {
     const int lane = threadIdx.x;
-    double *row = a.partials + (size_t)blockIdx.x * 128;
+    double *row = a.partials + (size_t)g * 128;
     if (lane < ACC_USED) {
         put(&row[lane], sum);
-        put(&row[64 + lane], comp);
+        if constexpr (MODE == 1) put(&row[64 + lane], comp);
     }
     LW_UNROLL for (int off = 32; off > 0; off >>= 1) {
         hits += __shfl_xor(hits, off, 64);
@@ -455,41 +542,55 @@ __device__ inline bool row_and_ticket(const IcpArgs &a, double sum, double comp,
         put(&row[ACC_USED], (double)hits);
         put(&row[ACC_USED + 1], (double)matched);
     }
-    stores_done();  // the row is device-visible before the ticket says so
-    if (lane == 0) *s_last = (atomicAdd(&a.st->ticket, 1u) == (uint32_t)G - 1);
+    stores_done();
+}
+
+// The one-pass launch: the row, then a ticket; true for the last workgroup of the pass (which then sees
+// every row).
+template <int MODE>
+__device__ inline bool row_and_ticket(const IcpArgs &a, double sum, double comp, int32_t hits, int32_t matched,
+                                      int G, int32_t *s_last) {
+    write_row<MODE>(a, blockIdx.x, sum, comp, hits, matched);
+    if (threadIdx.x == 0) *s_last = (atomicAdd(&a.st->ticket, 1u) == (uint32_t)G - 1);
     __syncthreads();
     const bool last = *s_last != 0;
     __syncthreads();
     return last;
 }
 
-// The last workgroup's part of a pass: reduce the rows, then (registration) the float64 step in lane 0. With
-// measurement on, lane 0 adds the pass's phases to the counters, in device wall-clock ticks: from the
-// pass's start (st->t_rel) to this workgroup's ticket (t_ticket), the reduction, and the step.
+// The one-pass launch's last workgroup: reduce the rows, then (registration) the float64 step. With phase
+// measurement on, lane 0 adds the pass's phases to the counters, in device wall-clock ticks: from the pass's
+// start (st->t_rel) to this workgroup's ticket (t_ticket), the reduction, and the step.
 template <int MODE>
-__device__ inline void finish_pass(const IcpArgs &a, const Pass p, int K, int G, double *vs, unsigned long long t_ticket,
-                                   double *sh, double *T, const double *Tp) {
+__device__ inline void finish_pass(const IcpArgs &a, const Pass p, int K, int G, double *vs, double *xi,
+                                   int32_t *s_flag, unsigned long long t_ticket, double *sh) {
     const int lane = threadIdx.x;
+    RegState *st = a.st;
     const double v = reduce_rows<MODE>(a.partials, G);
     vs[lane] = v;
-    a.st->acc[lane] = lane < ACC_USED ? v : 0.0;
+    st->acc[lane] = lane < ACC_USED ? v : 0.0;
+    if (a.gn && lane == 0 && a.sp.prior_info > 0.0) gn::twist_between(st->T, st->T_pred, xi);
     __syncthreads();
-    const unsigned long long t_red = a.prof ? wall_clock64() : 0ull;
-    if (a.gn) wave_gauss_newton(a.st, vs, a.sp, p.it, sh, T, Tp, a.prof);
+    const unsigned long long t_red = a.phase ? wall_clock64() : 0ull;
+    if (a.gn) {
+        run_step(st, vs, a.sp, p.it, sh, st->T, st->T_pred, xi, s_flag, a.phase);
+        if (lane == 0)  // the lane that moved the pose
+            for (int k = 0; k < 16; ++k) st->Tf[k] = (float)st->T[k];
+    }
     if (lane == 0) {
-        put(&a.st->ticket, 0u);
+        put(&st->ticket, 0u);
         if (a.prof) {
-            const unsigned long long t_step = wall_clock64();
             atomicAdd(&a.prof[0], (unsigned long long)p.n);
             atomicAdd(&a.prof[1], (unsigned long long)p.n * (unsigned long long)K);
             atomicAdd(&a.prof[2], (unsigned long long)vs[ACC_USED]);
             atomicAdd(&a.prof[3], (unsigned long long)vs[ACC_USED + 1]);
             atomicAdd(&a.prof[4], 1ull);
-            atomicAdd(&a.prof[8], t_ticket - a.st->t_rel);
-            atomicAdd(&a.prof[9], t_red - t_ticket);
-            atomicAdd(&a.prof[10], t_step - t_red);
-            atomicAdd(&a.prof[11], 1ull);
-            put(&a.st->t_rel, (unsigned long long)wall_clock64());  // the next pass starts when this one is released
+        }
+        if (a.phase) {
+            atomicAdd(&a.phase[8], t_ticket - st->t_rel);
+            atomicAdd(&a.phase[9], t_red - t_ticket);
+            atomicAdd(&a.phase[10], wall_clock64() - t_red);
+            atomicAdd(&a.phase[11], 1ull);
         }
     }
 }
@@ -500,7 +601,7 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_icp(IcpArgs a) {
     if (st->done) return;
     __shared__ float Tf[16];
     __shared__ int32_t s_last;
-    __shared__ double vs[64];
+    __shared__ double vs[64], xi[6];
     __shared__ __align__(16) unsigned char lds[RED_BYTES];
     if (threadIdx.x < 16) Tf[threadIdx.x] = st->Tf[threadIdx.x];
     __syncthreads();
@@ -508,110 +609,189 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_icp(IcpArgs a) {
     int32_t hits = 0, matched = 0;
     const Pass p{a.n, a.max_d2, a.kappa, a.it};
     const int G = (int)gridDim.x;
-    if (a.prof && blockIdx.x == 0 && threadIdx.x == 0) a.st->t_rel = wall_clock64();
-    wave_pass<MODE, R>(a, p, Tf, G, lds, sum, comp, hits, matched);
+    if (a.phase && blockIdx.x == 0 && threadIdx.x == 0) a.st->t_rel = wall_clock64();
+    wave_pass<MODE, R>(a, p, Tf, blockIdx.x, G, lds, sum, comp, hits, matched);
     if (!row_and_ticket<MODE>(a, sum, comp, hits, matched, G, &s_last)) return;
     constexpr int D = 2 * R + 1;
-    finish_pass<MODE>(a, p, D * D * D, G, vs, a.prof ? wall_clock64() : 0ull, reinterpret_cast<double *>(lds), st->T,
-                      st->T_pred);
+    finish_pass<MODE>(a, p, D * D * D, G, vs, xi, &s_last, a.phase ? wall_clock64() : 0ull,
+                      reinterpret_cast<double *>(lds));
 }
 
 // ---- the persistent registration ------------------------------------------------------------------
+//
+// Workgroup 0 is the leader, the others search. Per pass, searcher g (= blockIdx.x - 1) takes the point
+// groups g, g + G, ... (G = the searchers the pass can use), writes its partial row and then its row flag
+// (the pass's tag); the leader, which searches nothing, sums row batches as their flags come in, runs the
+// step, and publishes the new pose (float32, tagged; pub in RegState), which the searchers poll for. Only
+// the leader keeps the float64 pose; the step's code and the reduction run in one workgroup that always
+// has them in its caches, and the twist to the prediction is computed while the searchers work. Every wait
+// is bounded (SPIN_LIMIT polls, then status STATUS_SPIN_TIMEOUT); co-residency is what the cooperative
+// launch guarantees.
 
 struct LoopArgs {
     IcpArgs a;                // pointers and the fixed scalars; n, max_d2, kappa, it are set per pass
     const int32_t *n_dev;     // thinned count on the device
+    uint32_t *flags;          // MAX_LOOP_ROWS row flags
     int32_t max_iter, wide_iters, min_corr;
     float max_d2_r1, max_d2_r2;
 };
 
-constexpr uint32_t SPIN_LIMIT = 1u << 24;  // ~ seconds of polling: a wait that long means the loop is broken
-constexpr int32_t STATUS_SPIN_TIMEOUT = -1;
+constexpr uint32_t TAG_STOP = 0x80000000u;
+
+__device__ inline int searchers_for(int64_t n, int S) {
+    return (int)min((int64_t)gridDim.x - 1, (n * S + ICP_BLOCK - 1) / ICP_BLOCK);
+}
 
 template <int MODE>
-__global__ __launch_bounds__(ICP_BLOCK) void k_register(LoopArgs L) {
-    RegState *st = L.a.st;
-    const int lane = threadIdx.x;
-    __shared__ float Tf[16];
-    __shared__ int32_t s_last;
-    __shared__ double vs[64];
-    __shared__ double sT[16], sTp[16];  // this workgroup's copy of the pose and the prediction
-    __shared__ __align__(16) unsigned char lds[RED_BYTES];
-    const int32_t n = *L.n_dev;
-    if (blockIdx.x == 0 && lane == 0) st->n = n;
-    if (n < L.min_corr) return;  // odom.py:756-757: the host sees n and keeps the prediction
-    // workgroups that no pass can use (the widest pass needs ceil(n * 4 / 64)) leave now
-    if ((int64_t)blockIdx.x * ICP_BLOCK >= (int64_t)n * Split<2>::S) return;
+__device__ inline void leader(const LoopArgs &L, int32_t n, unsigned char *lds) {
     const IcpArgs &a = L.a;
-    if (a.prof && blockIdx.x == 0 && lane == 0) st->t_rel = wall_clock64();  // block 0 takes part in pass 0
+    RegState *st = a.st;
+    const int lane = threadIdx.x;
+    __shared__ double sT[16], sTp[16], vs[64], xi[6];
+    __shared__ int32_t s_flag;
+    BSum *bs = reinterpret_cast<BSum *>(lds);
+    double *sh = reinterpret_cast<double *>(lds + MAX_BATCHES * ICP_BLOCK * sizeof(BSum));
     if (lane < 16) {
         sT[lane] = st->T[lane];
         sTp[lane] = st->T_pred[lane];
     }
+    unsigned long long t_rel = a.phase ? wall_clock64() : 0ull;
+    if (a.phase && lane == 0) put(&st->t_rel, t_rel);
     __syncthreads();
     for (int it = 0; it < L.max_iter; ++it) {
-        if (lane < 16) Tf[lane] = (float)sT[lane];
-        __syncthreads();
-        // measurement: block 0's view of the pass (wake-up after the release, search, row and ticket)
-        const bool stamp = a.prof && blockIdx.x == 0;
-        const unsigned long long t_wake = stamp ? wall_clock64() : 0ull;
-        const unsigned long long t_pass = stamp ? __hip_atomic_load(&st->t_rel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                                : 0ull;
         const int reach = it < L.wide_iters ? 2 : 1;
-        const Pass p{n, reach == 2 ? L.max_d2_r2 : L.max_d2_r1, st->kappa[it], it};
-        // the workgroups that hold points in this pass (one lane group each while the grid is large
-        // enough): only they write rows and take tickets, so the reduction reads no empty rows and the
-        // sum runs over the same rows, in the same order, as a one-pass launch of that many workgroups
-        const int S = reach == 2 ? Split<2>::S : Split<1>::S;
-        const int G = (int)min((int64_t)gridDim.x, ((int64_t)n * S + ICP_BLOCK - 1) / ICP_BLOCK);
-        bool last = false;
-        if ((int)blockIdx.x < G) {
-            double sum, comp;
-            int32_t hits = 0, matched = 0;
-            if (reach == 2)
-                wave_pass<MODE, 2>(a, p, Tf, G, lds, sum, comp, hits, matched);
-            else
-                wave_pass<MODE, 1>(a, p, Tf, G, lds, sum, comp, hits, matched);
-            const unsigned long long t_found = stamp ? wall_clock64() : 0ull;
-            last = row_and_ticket<MODE>(a, sum, comp, hits, matched, G, &s_last);
-            if (stamp && lane == 0) {
-                atomicAdd(&a.prof[12], t_wake - t_pass);
-                atomicAdd(&a.prof[13], t_found - t_wake);
-                atomicAdd(&a.prof[14], wall_clock64() - t_found);
-                atomicAdd(&a.prof[15], 1ull);
-            }
+        const int G = searchers_for(n, reach == 2 ? Split<2>::S : Split<1>::S);
+        // the twist to the prediction, while the pass's search runs (odom.py:775-781)
+        if (lane == 0 && a.sp.prior_info > 0.0) gn::twist_between(sT, sTp, xi);
+        double v = 0.0;
+        unsigned long long t_in = 0ull;
+        if (!leader_reduce<MODE>(a.partials, L.flags, G, (uint32_t)(it + 1), bs, v, t_in, a.phase != nullptr)) {
+            if (lane == 0) st->status = STATUS_SPIN_TIMEOUT;
+            if (lane < 16) put(&st->pub[lane], (unsigned long long)(TAG_STOP | (uint32_t)(it + 1)) << 32);
+            return;
         }
-        if (last) {
-            const unsigned long long t_ticket = a.prof ? wall_clock64() : 0ull;
-            finish_pass<MODE>(a, p, reach == 2 ? 125 : 27, G, vs, t_ticket, reinterpret_cast<double *>(lds), sT, sTp);
-            stores_done();  // the step, done flag and ticket reset are device-visible before the generation
-            if (lane == 0) put(&st->gen, (uint32_t)(it + 1));
-        } else {
-            // poll the generation with device-coherent loads; the step and the done flag are read the same way,
-            // so no cache is written back or invalidated
-            uint32_t spins = 0;
-            while (__hip_atomic_load(&st->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < (uint32_t)(it + 1)) {
-                __builtin_amdgcn_s_sleep(8);
-                if (++spins > SPIN_LIMIT) {
-                    if (lane == 0) st->status = STATUS_SPIN_TIMEOUT;
-                    return;
-                }
-            }
-            // the pass's step, applied to this workgroup's copy of the pose (the last workgroup's is moved)
-            if (lane == 0 && !__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                double d[6];
-                LW_UNROLL for (int k = 0; k < 6; ++k) d[k] = get(&st->delta[k]);
-                gn::apply_twist(sT, d);
-            }
-        }
+        const unsigned long long t_red = a.phase ? wall_clock64() : 0ull;
+        vs[lane] = v;
+        st->acc[lane] = lane < ACC_USED ? v : 0.0;
         __syncthreads();
-        if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        const bool stop = run_step(st, vs, a.sp, it, sh, sT, sTp, xi, &s_flag, a.phase);
+        unsigned long long t_pub = 0ull;
+        if (a.phase) {  // the publication time, device-visible before the pose (measurement only)
+            t_pub = wall_clock64();
+            if (lane == 0) put(&st->t_rel, t_pub);
+            stores_done();
+        }
+        // publish the pose (moved by an accepted step) to the searchers
+        if (lane < 16) {
+            const uint32_t tag = (uint32_t)(it + 1) | (stop ? TAG_STOP : 0u);
+            put(&st->pub[lane], ((unsigned long long)tag << 32) | (unsigned long long)__float_as_uint((float)sT[lane]));
+        }
+        // off the searchers' path: the host's copies and the measurement
+        if (lane < 16) {
+            st->T[lane] = sT[lane];
+            st->Tf[lane] = (float)sT[lane];
+        }
+        if (lane == 0) {
+            if (a.prof) {
+                const int K = reach == 2 ? 125 : 27;
+                atomicAdd(&a.prof[0], (unsigned long long)n);
+                atomicAdd(&a.prof[1], (unsigned long long)n * (unsigned long long)K);
+                atomicAdd(&a.prof[2], (unsigned long long)vs[ACC_USED]);
+                atomicAdd(&a.prof[3], (unsigned long long)vs[ACC_USED + 1]);
+                atomicAdd(&a.prof[4], 1ull);
+            }
+            if (a.phase) {
+                atomicAdd(&a.phase[8], t_in - t_rel);
+                atomicAdd(&a.phase[9], t_red - t_in);
+                atomicAdd(&a.phase[10], t_pub - t_red);
+                atomicAdd(&a.phase[11], 1ull);
+            }
+        }
+        t_rel = t_pub;
+        if (stop) break;
     }
 }
 
-// Tp: 16 doubles (the prediction) followed by 64 floats (the per-iteration robust-kernel scale)
-__global__ void k_reg_begin(RegState *st, const double *Tp) {
+template <int MODE>
+__device__ inline void searcher(const LoopArgs &L, int32_t n, unsigned char *lds) {
+    const IcpArgs &a = L.a;
+    RegState *st = a.st;
+    const int lane = threadIdx.x;
+    const int g = (int)blockIdx.x - 1;
+    __shared__ float Tf[16];
+    if (lane < 16) Tf[lane] = st->Tf[lane];
+    __syncthreads();
+    const bool stamp = a.phase && g == 0;  // the first searcher's view of the passes
+    unsigned long long t_wake = stamp ? wall_clock64() : 0ull;
+    for (int it = 0; it < L.max_iter; ++it) {
+        const int reach = it < L.wide_iters ? 2 : 1;
+        const Pass p{n, reach == 2 ? L.max_d2_r2 : L.max_d2_r1, st->kappa[it], it};
+        const int G = searchers_for(n, reach == 2 ? Split<2>::S : Split<1>::S);
+        if (g < G) {
+            double sum, comp;
+            int32_t hits = 0, matched = 0;
+            if (reach == 2)
+                wave_pass<MODE, 2>(a, p, Tf, g, G, lds, sum, comp, hits, matched);
+            else
+                wave_pass<MODE, 1>(a, p, Tf, g, G, lds, sum, comp, hits, matched);
+            const unsigned long long t_found = stamp ? wall_clock64() : 0ull;
+            write_row<MODE>(a, g, sum, comp, hits, matched);
+            if (lane == 0) put(&L.flags[g], (uint32_t)(it + 1));
+            if (stamp && lane == 0) {
+                atomicAdd(&a.phase[13], t_found - t_wake);
+                atomicAdd(&a.phase[14], wall_clock64() - t_found);
+                atomicAdd(&a.phase[15], 1ull);
+            }
+        }
+        // wait for the leader's pose of pass it + 1: 16 tagged words, each read with its own tag
+        uint32_t spins = 0;
+        uint32_t tag0;
+        while (true) {
+            const unsigned long long w = lane < 16 ? get(&st->pub[lane]) : 0ull;
+            const uint32_t tag = (uint32_t)(w >> 32);
+            const bool in = lane >= 16 || (tag & ~TAG_STOP) >= (uint32_t)(it + 1);
+            if (__ballot(in) == ~0ull) {
+                tag0 = __shfl(tag, 0, 64);
+                if (lane < 16) Tf[lane] = __uint_as_float((uint32_t)w);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(4);
+            if (++spins > SPIN_LIMIT) {
+                if (lane == 0) st->status = STATUS_SPIN_TIMEOUT;
+                return;
+            }
+        }
+        if (stamp) {
+            t_wake = wall_clock64();
+            if (lane == 0) atomicAdd(&a.phase[12], t_wake - get(&st->t_rel));
+        }
+        __syncthreads();
+        if (tag0 & TAG_STOP) break;
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(ICP_BLOCK) void k_register(LoopArgs L) {
+    RegState *st = L.a.st;
+    __shared__ __align__(16) unsigned char lds[RED_BYTES];
+    const int32_t n = *L.n_dev;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->n = n;
+    if (n < L.min_corr) return;  // odom.py:756-757: the host sees n and keeps the prediction
+    if (blockIdx.x == 0) {
+        leader<MODE>(L, n, lds);
+        return;
+    }
+    // searchers that no pass can use (the widest pass needs ceil(n * 4 / 64)) leave now
+    if ((int64_t)(blockIdx.x - 1) * ICP_BLOCK >= (int64_t)n * Split<2>::S) return;
+    searcher<MODE>(L, n, lds);
+}
+
+// Tp: 16 doubles (the prediction) followed by 64 floats (the per-iteration robust-kernel scale). Also clears
+// the persistent loop's row flags and published pose.
+__global__ void k_reg_begin(RegState *st, const double *Tp, uint32_t *flags) {
     if (threadIdx.x < 64) st->kappa[threadIdx.x] = reinterpret_cast<const float *>(Tp + 16)[threadIdx.x];
+    for (int k = threadIdx.x; k < MAX_LOOP_ROWS; k += blockDim.x) flags[k] = 0u;
+    if (threadIdx.x < 16) st->pub[threadIdx.x] = 0ull;
     if (threadIdx.x != 0) return;
     for (int k = 0; k < 16; ++k) {
         st->T[k] = Tp[k];
@@ -628,7 +808,6 @@ __global__ void k_reg_begin(RegState *st, const double *Tp) {
     st->n_degen = 0;
     st->status = 0;
     st->ticket = 0;
-    st->gen = 0;
     st->n = 0;
 }
 
@@ -656,6 +835,7 @@ IcpArgs base_args(lw_ctx *c) {
     a.partials = c->partials;
     a.best_slot = nullptr;
     a.prof = c->prof_on ? c->prof_dev : nullptr;
+    a.phase = c->prof_on && c->prof_phases ? c->prof_dev : nullptr;
     a.mask = c->mask;
     return a;
 }
@@ -668,7 +848,7 @@ const void *loop_kernel() {
 }  // namespace
 
 hipError_t launch_reg_begin(lw_ctx *c, const double *T_pred_dev) {
-    hipLaunchKernelGGL(k_reg_begin, dim3(1), dim3(64), 0, c->stream, c->reg, T_pred_dev);
+    hipLaunchKernelGGL(k_reg_begin, dim3(1), dim3(64), 0, c->stream, c->reg, T_pred_dev, c->loop_flags);
     return hipGetLastError();
 }
 
@@ -718,9 +898,10 @@ hipError_t launch_register_loop(lw_ctx *c, const int32_t *n_dev, int32_t n_bound
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
             return hipErrorNotSupported;
         // co-resident limit, and at most two workgroups per CU: the loop never needs more than one lane
-        // group per workgroup (8192 thinned points at the 5x5x5 reach), and every extra workgroup
-        // only polls the generation counter
+        // group per searcher (8192 thinned points at the 5x5x5 reach), and every extra workgroup only
+        // polls for the pose
         c->loop_max_groups[acc_mode] = per_cu < 2 ? per_cu * cus : 2 * cus;
+        if (c->loop_max_groups[acc_mode] < 2) return hipErrorNotSupported;
     }
     LoopArgs L;
     L.a = base_args(c);
@@ -729,16 +910,19 @@ hipError_t launch_register_loop(lw_ctx *c, const int32_t *n_dev, int32_t n_bound
     L.a.gn = 1;
     L.a.sp = R.sp;
     L.n_dev = n_dev;
+    L.flags = c->loop_flags;
     L.max_iter = R.max_iter;
     L.wide_iters = R.wide_iters;
     L.min_corr = R.min_corr;
     L.max_d2_r1 = R.max_d2_r1;
     L.max_d2_r2 = R.max_d2_r2;
-    // one lane group per workgroup for the widest pass, up to what can be co-resident
+    // the leader, and one searcher per lane group of the widest pass, up to what can be co-resident
     int64_t g = ((int64_t)(n_bound > 0 ? n_bound : 1) * Split<2>::S + ICP_BLOCK - 1) / ICP_BLOCK;
-    if (g > c->loop_max_groups[acc_mode]) g = c->loop_max_groups[acc_mode];
+    if (g > c->loop_max_groups[acc_mode] - 1) g = c->loop_max_groups[acc_mode] - 1;
+    if (g > MAX_LOOP_ROWS) g = MAX_LOOP_ROWS;
     if (g > icp_max_groups(c->frame_cap)) g = icp_max_groups(c->frame_cap);
     if (g < 1) g = 1;
+    g += 1;
     void *args[] = {&L};
     const bool timed = c->prof_on && c->prof_used < c->prof_cap;
     if (timed) {

@@ -12,6 +12,8 @@ namespace lw {
 constexpr int BLOCK = 256;           // threads per block of the streaming kernels
 constexpr int ICP_BLOCK = 64;        // one wave per accumulation workgroup
 constexpr int ICP_MAX_GROUPS = 2048; // partial rows kept for the cross-workgroup reduction
+constexpr int LW_PROF_COUNTERS = 24; // device measurement counters (lw_ctx::prof_dev)
+constexpr int MAX_LOOP_ROWS = 512;   // searching workgroups (partial rows) of the persistent registration
 constexpr int SCAN_THREADS = 1024;
 
 // Device-resident state of one registration (one Gauss-Newton loop).
@@ -19,8 +21,11 @@ struct RegState {
     double T[16];       // current pose, float64
     double T_pred[16];  // the prediction the prior pulls toward
     double H[36];       // H of the last accepted accumulation (for the condition number)
-    double delta[6];    // the last accepted step (the persistent loop's workgroups apply it to their pose)
     double acc[64];     // last reduced accumulator (ACC layout)
+    // the persistent loop's pose for its searching workgroups: entry k = tag << 32 | float bits of Tf[k],
+    // tag = passes completed, with bit 31 set when the loop stops (one 64-bit store each, so a reader that
+    // sees the tag sees the value written with it)
+    unsigned long long pub[16];
     double rms;
     double rms0;        // the residual of the first pass (at the prediction)
     double cond;
@@ -30,8 +35,7 @@ struct RegState {
     int32_t corr;
     int32_t n_degen;
     int32_t status;     // last step status (gn::STEP_*)
-    uint32_t ticket;    // workgroups finished in the current pass
-    uint32_t gen;       // passes completed (the persistent registration's release counter)
+    uint32_t ticket;    // workgroups finished in the current pass (one launch per pass)
     int32_t n;          // thinned points the registration ran on
     unsigned long long t_rel;  // device wall clock at the current pass's start (measurement only)
     float kappa[64];    // robust-kernel scale per iteration (odom.py:767)
@@ -82,6 +86,7 @@ struct lw_ctx {
     // registration
     lw::RegState *reg = nullptr;
     double *partials = nullptr;
+    uint32_t *loop_flags = nullptr;  // MAX_LOOP_ROWS: the pass each searching workgroup's row belongs to
     int32_t *best_slot = nullptr;
     int64_t *dtmp = nullptr;  // small uploads: poses, keys (frame_cap + 8 entries)
     // pinned host staging
@@ -89,13 +94,17 @@ struct lw_ctx {
     size_t pin_bytes = 0;
     // live measurement of the accumulation launches: HIP event pairs around each launch, and device
     // counters [0] point-iterations [1] neighbour lookups [2] occupied neighbours hit [3] matched points
-    // [4] launches that did work
+    // [4] launches that did work [5] [6] steps on the wave / in one lane [8..11] pass
+    // phases (search, reduce, step, passes) [12..15] block 0's phases (wake, search, row and ticket,
+    // passes) [16..20] wave-step parts (set-up, degeneracy test, solve, pose update, steps); the phase
+    // counters [8..20] only when prof_phases is set (their clock reads and waits slow the loop a little)
     bool prof_on = false;
+    bool prof_phases = false;
     hipEvent_t *prof_ev = nullptr;
     int32_t prof_cap = 0, prof_used = 0;
     double prof_ms = 0.0;
     int64_t prof_launches = 0;
-    unsigned long long *prof_dev = nullptr;
+    unsigned long long *prof_dev = nullptr;  // LW_PROF_COUNTERS measurement counters
     // batched localisation: the work-queue counter, and the last captured launch (replayed while the
     // arguments stay the same)
     uint32_t *loc_queue = nullptr;

@@ -110,7 +110,7 @@ __global__ __launch_bounds__(B) void k_loc(LocArgs a) {
                 if (s < 0) continue;
                 const float mx = best.mx, my = best.my, mz = best.mz;
                 ++matched_total;
-                const VoxPrec P = a.vox_prec[a.slots[s].vid];
+                const VoxPrec P = a.vox_prec[best.vid];
                 const float M[3][3] = {{P.M[0], P.M[1], P.M[2]}, {P.M[1], P.M[3], P.M[4]}, {P.M[2], P.M[4], P.M[5]}};
                 const float r0 = wx - mx, r1 = wy - my, r2 = wz - mz;
                 float Mr[3];

@@ -23,6 +23,7 @@ struct Cand {
     int32_t s;  // slot, -1 for none
     int32_t j;  // position in the reference's neighbour order
     float mx, my, mz;
+    int32_t vid;  // the slot's voxel id (read with the record, so the terms need no second lookup)
 };
 
 struct Table {
@@ -61,6 +62,7 @@ __device__ inline void consider(float wx, float wy, float wz, int32_t s, int32_t
         best.mx = hi.x;
         best.my = hi.y;
         best.mz = hi.z;
+        best.vid = lo.z;
     }
 }
 
