@@ -316,7 +316,7 @@ def main():
         push(k)
     odo.synchronize()
     torch.cuda.synchronize()
-    phases = np.zeros(15)
+    phases = np.zeros(18)
     lw._abi.check(L.lw_prof_phases(odo._ctx, lw._abi.ptr(phases, lw._abi._f64p)), "lw_prof_phases")
     lw._abi.check(L.lw_prof_enable(odo._ctx, 1), "lw_prof_enable")
     lw._abi.check(L.lw_prof_reset(odo._ctx), "lw_prof_reset")
@@ -367,10 +367,13 @@ def main():
                                          "launch": round(prof[1] * 1e3 / max(prof[6], 1), 2),
                                          "searcher_wake": round(phases[4] / max(phases[7], 1), 2),
                                          "searcher_search": round(phases[5] / max(phases[7], 1), 2),
-                                         "searcher_row_flag": round(phases[6] / max(phases[7], 1), 2)},
+                                         "searcher_row_flag": round(phases[6] / max(phases[7], 1), 2),
+                                         "searcher_neighbours": round(phases[15] / max(phases[7], 1), 2),
+                                         "searcher_terms": round(phases[16] / max(phases[7], 1), 2),
+                                         "searcher_row_sum": round(phases[17] / max(phases[7], 1), 2)},
                          "steps_on_wave": int(phases[8]), "steps_one_lane": int(phases[9]),
                          "us_per_wave_step": {k: round(phases[10 + i] / max(phases[14], 1), 3) for i, k in
-                                              enumerate(("setup", "degeneracy", "solve", "update"))}},
+                                              enumerate(("setup", "factor", "substitute", "update"))}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic("k_register<0>"),
                      "kernel": "k_register<0> (one launch per registration: every accumulation pass and step)",

@@ -213,8 +213,10 @@ int lw_prof_read(lw_ctx *ctx, double *out);
  * pose to its wake-up, out[5] its search and terms, out[6] its row write and flag; out[7] the passes it
  * timed. out[8] steps taken on the whole wave, out[9] steps taken by the general one-lane routine
  * (a degenerate direction, an indefinite system). The wave steps' parts: out[10] set-up (matrices, basis,
- * prior), out[11] the degeneracy test, out[12] the damped solve, out[13] the pose update and publication;
- * out[14] the wave steps timed. out holds 15 doubles. No counterpart in the reference. */
+ * prior), out[11] the two Cholesky factorisations (degeneracy test and damped system, side by side), out[12]
+ * the substitution, out[13] the pose update; out[14] the wave steps timed. The first searching workgroup's
+ * search and terms (out[5]) in parts: out[15] point to closest means (neighbour search), out[16] the terms
+ * into its LDS rows, out[17] the sum of its rows; counted with out[7]. out holds 18 doubles. No counterpart in the reference. */
 int lw_prof_phases(lw_ctx *ctx, double *out);
 
 /* Host builds of the device's float64 step and per-voxel precision, for tests on machines without a

@@ -621,6 +621,7 @@ int lw_prof_phases(lw_ctx *c, double *out) {
     out[9] = (double)v[6];
     for (int i = 0; i < 4; ++i) out[10 + i] = (double)v[16 + i] * us_per_tick;
     out[14] = (double)v[20];
+    for (int i = 0; i < 3; ++i) out[15 + i] = (double)v[21 + i] * us_per_tick;
     return 0;
 }
 

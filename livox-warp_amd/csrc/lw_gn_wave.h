@@ -18,8 +18,8 @@ namespace lw {
 namespace gnw {
 
 // LDS layout (doubles) of the step's scratch
-constexpr int W_H = 0, W_HG = 36, W_A = 72, W_J = 108, W_JI = 144, W_M1 = 180, W_M2 = 216, W_L = 252, W_RHS = 288,
-              W_Y = 294, W_OUT = 300, W_T = 306, W_DOUBLES = 322;
+constexpr int W_H = 0, W_HG = 36, W_A = 72, W_J = 108, W_JI = 144, W_M1 = 180, W_M2 = 216, W_M3 = 252, W_L = 288,
+              W_L2 = 324, W_RHS = 360, W_Y = 366, W_OUT = 372, W_T = 378, W_DOUBLES = 394;
 
 // Index of (p, q), p <= q, in a row-major upper triangle of a 6x6 matrix (odom.py:387, np.triu_indices(6)).
 __device__ inline int triu(int p, int q) { return p * 6 - (p * (p - 1)) / 2 + (q - p); }
@@ -40,26 +40,32 @@ __device__ inline double tmul_entry(const double *A, const double *B, int l) {
     return s;
 }
 
-// Cholesky factor of A - shift I into L (lower, LDS), column by column: lane j computes the diagonal, lanes
-// i > j the column below it (gn::pd_above / gn::solve6 order). Every lane returns whether every pivot was
-// positive.
-__device__ inline bool wave_cholesky(const double *A, double shift, double *L) {
+// Two Cholesky factorisations side by side, column by column: lanes 0-31 factor A0 - shift0 I into L0, lanes
+// 32-63 A1 - shift1 I into L1 (lower, LDS). In each half, every lane forms column j's pivot and lane i > j the
+// entry below it (gn::pd_above / gn::solve6 order). pd0 / pd1: whether every pivot of each was positive.
+__device__ inline void wave_cholesky2(const double *A0, double shift0, double *L0, const double *A1, double shift1,
+                                      double *L1, bool &pd0, bool &pd1) {
     const int lane = threadIdx.x;
+    const int li = lane & 31;
+    const double *A = lane < 32 ? A0 : A1;
+    double *L = lane < 32 ? L0 : L1;
+    const double shift = lane < 32 ? shift0 : shift1;
     bool pd = true;
     for (int j = 0; j < 6; ++j) {
         double s = A[6 * j + j] - shift;
         for (int k = 0; k < j; ++k) s -= L[6 * j + k] * L[6 * j + k];
         if (!(s > 0.0)) pd = false;
         const double d = sqrt(s > 0.0 ? s : 1.0);
-        if (lane > j && lane < 6) {
-            double t = A[6 * lane + j];
-            for (int k = 0; k < j; ++k) t -= L[6 * lane + k] * L[6 * j + k];
-            L[6 * lane + j] = t / d;
+        if (li > j && li < 6) {
+            double t = A[6 * li + j];
+            for (int k = 0; k < j; ++k) t -= L[6 * li + k] * L[6 * j + k];
+            L[6 * li + j] = t / d;
         }
-        if (lane == j) L[6 * j + j] = d;
+        if (li == j) L[6 * j + j] = d;
         __syncthreads();
     }
-    return pd;
+    pd0 = __shfl(pd ? 1 : 0, 0, 64) != 0;
+    pd1 = __shfl(pd ? 1 : 0, 32, 64) != 0;
 }
 
 // Entry (r, c) of K = [t]x for the pose T (row-major 4x4 in LDS), as gn::sensor_basis lays it out:
@@ -112,7 +118,8 @@ struct WaveOut {
 // the caller must run gn::step instead, or -1 for gn::STEP_FEW (corr < min_corr). T: the pose. xi: the twist
 // from T to the prediction (gn::twist_between(T, T_pred), read when P.prior_info > 0), computed by the caller
 // (the persistent loop does it while the pass's search runs). stamps (measurement, or null): lane 0 records
-// the device wall clock after the set-up, the degeneracy test and the solve.
+// the device wall clock after the set-up, the two factorisations (the degeneracy test's and the damped
+// system's, side by side) and the substitution.
 __device__ inline int wave_step(const double *v, const double *T, const double *xi, const gn::StepParams &P,
                                 double *sh, WaveOut &o, double *H_out, unsigned long long *stamps) {
     const int lane = threadIdx.x;
@@ -120,7 +127,9 @@ __device__ inline int wave_step(const double *v, const double *T, const double *
     o.n_degen = 0;
     if (o.corr < P.min_corr) return -1;
     double *H = sh + W_H, *Hg = sh + W_HG, *A = sh + W_A, *J = sh + W_J, *Ji = sh + W_JI, *M1 = sh + W_M1,
-           *M2 = sh + W_M2, *L = sh + W_L, *rhs = sh + W_RHS, *y = sh + W_Y, *out = sh + W_OUT, *Ts = sh + W_T;
+           *M2 = sh + W_M2, *M3 = sh + W_M3, *L = sh + W_L, *L2 = sh + W_L2, *rhs = sh + W_RHS, *y = sh + W_Y,
+           *out = sh + W_OUT, *Ts = sh + W_T;
+    const bool prior = P.prior_info > 0.0, degen = P.degen_thr > 0.0;
     if (lane < 16) Ts[lane] = T[lane];
     __syncthreads();
     double Lr = v[31] / (double)o.corr;
@@ -145,14 +154,18 @@ __device__ inline int wave_step(const double *v, const double *T, const double *
     if (lane < 36) {
         double a = H[lane];
         if (lane % 7 == 0) a += 1e-9 * mean + P.damping * H[lane];
-        if (P.prior_info > 0.0) M1[lane] = tmul_entry(Ji, Ji, lane) * P.prior_info;  // P = prior_info J^-T J^-1
+        if (prior) M1[lane] = tmul_entry(Ji, Ji, lane) * P.prior_info;  // P = prior_info J^-T J^-1
+        if (degen) M2[lane] = mul_entry(Hg, J, lane);
         A[lane] = a;
     }
     if (lane < 6) rhs[lane] = -v[21 + lane];
     __syncthreads();
-    if (P.prior_info > 0.0) {
-        if (lane < 36) A[lane] += M1[lane];
-        if (lane < 6) {
+    if (prior || degen) {
+        if (lane < 36) {
+            if (prior) A[lane] += M1[lane];
+            if (degen) M3[lane] = tmul_entry(J, M2, lane);  // J^T Hg J
+        }
+        if (prior && lane < 6) {
             double s = 0.0;
             for (int k = 0; k < 6; ++k) s += M1[6 * lane + k] * xi[k];
             rhs[lane] += s;
@@ -160,15 +173,12 @@ __device__ inline int wave_step(const double *v, const double *T, const double *
         __syncthreads();
     }
     if (stamps && lane == 0) stamps[0] = wall_clock64();
-    if (P.degen_thr > 0.0) {  // the degeneracy test: J^T Hg J - thr I positive definite
-        if (lane < 36) M2[lane] = mul_entry(Hg, J, lane);
-        __syncthreads();
-        if (lane < 36) M1[lane] = tmul_entry(J, M2, lane);
-        __syncthreads();
-        if (!wave_cholesky(M1, P.degen_thr, L)) return 0;  // a direction to hold: the scalar routine does it
-    }
+    // the damped system and the degeneracy test (J^T Hg J - thr I positive definite), factored together
+    bool pd_a, pd_g;
+    wave_cholesky2(A, 0.0, L, degen ? M3 : A, degen ? P.degen_thr : 0.0, L2, pd_a, pd_g);
     if (stamps && lane == 0) stamps[1] = wall_clock64();
-    if (!wave_cholesky(A, 0.0, L)) return 0;  // indefinite: the scalar routine's LU fallback
+    if (degen && !pd_g) return 0;  // a direction to hold: the scalar routine does it
+    if (!pd_a) return 0;           // indefinite: the scalar routine's LU fallback
     if (lane == 0) {
         for (int i = 0; i < 6; ++i) {
             double t = rhs[i];

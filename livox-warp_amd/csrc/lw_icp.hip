@@ -268,10 +268,15 @@ __device__ inline void point_terms(const IcpArgs &a, float kappa, float px, floa
 }
 
 // One workgroup's share of a pass: the points of lane groups g, g + G, ... (one group per workgroup whenever
-// the grid is large enough). Lane k < 53 returns the workgroup's entry k.
+// the grid is large enough). Lane k < 53 returns the workgroup's entry k. pre: when has_pre, this lane's point
+// of group g, already loaded (the persistent loop keeps it in registers from pass to pass). With stamp set,
+// t_found and t_terms receive the device clock after the last lane group's neighbour search and after its
+// terms (measurement).
 template <int MODE, int R>
 __device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf, int g, int G, unsigned char *lds,
-                                 double &sum, double &comp, int32_t &hits, int32_t &matched) {
+                                 double &sum, double &comp, int32_t &hits, int32_t &matched, float3 pre,
+                                 bool has_pre, bool stamp, unsigned long long &t_found,
+                                 unsigned long long &t_terms) {
     constexpr int S = Split<R>::S, C = Split<R>::C;
     constexpr int D = 2 * R + 1, K = D * D * D, KQ = (K + S - 1) / S;
     const int lane = threadIdx.x;
@@ -282,13 +287,20 @@ __device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf
         float px = 0.0f, py = 0.0f, pz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
         Cand best{p.max_d2, -1, K, 0.0f, 0.0f, 0.0f};
         if (i < p.n) {
-            px = a.ds[3 * (int64_t)i];
-            py = a.ds[3 * (int64_t)i + 1];
-            pz = a.ds[3 * (int64_t)i + 2];
+            if (has_pre && base == (int64_t)g * ICP_BLOCK) {
+                px = pre.x;
+                py = pre.y;
+                pz = pre.z;
+            } else {
+                px = a.ds[3 * (int64_t)i];
+                py = a.ds[3 * (int64_t)i + 1];
+                pz = a.ds[3 * (int64_t)i + 2];
+            }
             transform_point(Tf, px, py, pz, wx, wy, wz);
-            search_part<R, C>(Table{a.slots, a.occ, a.mask}, a.inv_v, wx, wy, wz, sub * KQ, min(K, (sub + 1) * KQ),
+            search_part<R, C, false>(Table{a.slots, a.occ, a.mask}, a.inv_v, wx, wy, wz, sub * KQ, min(K, (sub + 1) * KQ),
                                best, hits);
         }
+        if (stamp) t_found = wall_clock64();
         // merge the S parts: smaller distance wins, a tie goes to the earlier neighbour (the serial strict <)
         LW_UNROLL for (int off = 1; off < S; off <<= 1) {
             Cand o;
@@ -311,6 +323,7 @@ __device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf
                 row_add<MODE>(lds, acc);
             }
         }
+        if (stamp) t_terms = wall_clock64();
     }
     wave_sum<MODE, S>(lds, sum, comp);
 }
@@ -482,7 +495,7 @@ __device__ __noinline__ bool gauss_newton(RegState *st, const double *v, const g
 // The step on the whole wave (lw_gn_wave.h) when it applies, otherwise gauss_newton in lane 0 (same results
 // either way), on the reduced row v. xi: the twist from T to the prediction Tp (LDS, when the prior is on).
 // Every lane returns whether the loop stops; st->done records it. With phase measurement on, a wave step's
-// parts go to ph[16..19] (set-up, degeneracy test, solve, pose update; device wall-clock ticks) and ph[20],
+// parts go to ph[16..19] (set-up, factorisations, substitution, pose update; device wall-clock ticks) and ph[20],
 // and ph[5] / ph[6] count the steps taken on the wave / in one lane.
 __device__ inline bool run_step(RegState *st, const double *v, const gn::StepParams &sp, int32_t it, double *sh,
                                 double *T, const double *Tp, const double *xi, int32_t *s_flag,
@@ -610,7 +623,8 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_icp(IcpArgs a) {
     const Pass p{a.n, a.max_d2, a.kappa, a.it};
     const int G = (int)gridDim.x;
     if (a.phase && blockIdx.x == 0 && threadIdx.x == 0) a.st->t_rel = wall_clock64();
-    wave_pass<MODE, R>(a, p, Tf, blockIdx.x, G, lds, sum, comp, hits, matched);
+    unsigned long long t_found, t_terms;
+    wave_pass<MODE, R>(a, p, Tf, blockIdx.x, G, lds, sum, comp, hits, matched, {}, false, false, t_found, t_terms);
     if (!row_and_ticket<MODE>(a, sum, comp, hits, matched, G, &s_last)) return;
     constexpr int D = 2 * R + 1;
     finish_pass<MODE>(a, p, D * D * D, G, vs, xi, &s_last, a.phase ? wall_clock64() : 0ull,
@@ -720,6 +734,12 @@ __device__ inline void searcher(const LoopArgs &L, int32_t n, unsigned char *lds
     const int g = (int)blockIdx.x - 1;
     __shared__ float Tf[16];
     if (lane < 16) Tf[lane] = st->Tf[lane];
+    // this lane's point in its first lane group, for either reach: the same every pass
+    float3 pt[2] = {};
+    LW_UNROLL for (int k = 0; k < 2; ++k) {
+        const int64_t i = ((int64_t)g * ICP_BLOCK + lane) / (k == 0 ? Split<2>::S : Split<1>::S);
+        if (i < n) pt[k] = make_float3(a.ds[3 * i], a.ds[3 * i + 1], a.ds[3 * i + 2]);
+    }
     __syncthreads();
     const bool stamp = a.phase && g == 0;  // the first searcher's view of the passes
     unsigned long long t_wake = stamp ? wall_clock64() : 0ull;
@@ -730,17 +750,23 @@ __device__ inline void searcher(const LoopArgs &L, int32_t n, unsigned char *lds
         if (g < G) {
             double sum, comp;
             int32_t hits = 0, matched = 0;
+            unsigned long long t_found = 0ull, t_terms = 0ull;
             if (reach == 2)
-                wave_pass<MODE, 2>(a, p, Tf, g, G, lds, sum, comp, hits, matched);
+                wave_pass<MODE, 2>(a, p, Tf, g, G, lds, sum, comp, hits, matched, pt[0], true, stamp, t_found,
+                                   t_terms);
             else
-                wave_pass<MODE, 1>(a, p, Tf, g, G, lds, sum, comp, hits, matched);
-            const unsigned long long t_found = stamp ? wall_clock64() : 0ull;
+                wave_pass<MODE, 1>(a, p, Tf, g, G, lds, sum, comp, hits, matched, pt[1], true, stamp, t_found,
+                                   t_terms);
+            const unsigned long long t_summed = stamp ? wall_clock64() : 0ull;
             write_row<MODE>(a, g, sum, comp, hits, matched);
             if (lane == 0) put(&L.flags[g], (uint32_t)(it + 1));
             if (stamp && lane == 0) {
-                atomicAdd(&a.phase[13], t_found - t_wake);
-                atomicAdd(&a.phase[14], wall_clock64() - t_found);
+                atomicAdd(&a.phase[13], t_summed - t_wake);
+                atomicAdd(&a.phase[14], wall_clock64() - t_summed);
                 atomicAdd(&a.phase[15], 1ull);
+                atomicAdd(&a.phase[21], t_found - t_wake);
+                atomicAdd(&a.phase[22], t_terms - t_found);
+                atomicAdd(&a.phase[23], t_summed - t_terms);
             }
         }
         // wait for the leader's pose of pass it + 1: 16 tagged words, each read with its own tag

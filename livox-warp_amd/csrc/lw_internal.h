@@ -94,10 +94,11 @@ struct lw_ctx {
     size_t pin_bytes = 0;
     // live measurement of the accumulation launches: HIP event pairs around each launch, and device
     // counters [0] point-iterations [1] neighbour lookups [2] occupied neighbours hit [3] matched points
-    // [4] launches that did work [5] [6] steps on the wave / in one lane [8..11] pass
-    // phases (search, reduce, step, passes) [12..15] block 0's phases (wake, search, row and ticket,
-    // passes) [16..20] wave-step parts (set-up, degeneracy test, solve, pose update, steps); the phase
-    // counters [8..20] only when prof_phases is set (their clock reads and waits slow the loop a little)
+    // [4] launches that did work; with prof_phases set (their clock reads and waits slow the loop a
+    // little) also [5] [6] steps on the wave / in one lane [8..11] pass phases (search, reduce, step,
+    // passes) [12..15] the first searcher's phases (wake, search and terms, row and flag, passes)
+    // [16..20] wave-step parts (set-up, factorisations, substitution, pose update, steps) [21..23] the
+    // first searcher's neighbour search, terms, row sum
     bool prof_on = false;
     bool prof_phases = false;
     hipEvent_t *prof_ev = nullptr;

@@ -1,13 +1,19 @@
 // The closest-mean neighbour search shared by the odometry accumulation (lw_icp.hip) and the batched
 // localisation (lw_loc.hip): odom.py:308-334 with find_slot (gpu.py:254-265).
 //
-// A lane resolves a chunk of C neighbour keys together. Round 0 loads the bitmap words of every home slot
-// (slot_hash & mask) and then every occupied home slot's 32 B record; a record holding the key is a hit. A
-// key whose home slot holds another key can only lie within the home's displacement bound (Slot::disp, the
-// largest probe distance of the keys homed there, kept by the inserts): a bound of 0 (the usual case) ends the
-// lookup as a miss at once, otherwise the slots up to the bound are read, the records of every key of the
-// chunk still unresolved at a depth loaded together. The slot found is the one find_slot's linear probe
-// reaches (gpu.py:254-265): keys are never removed, so a key homed at h sits within h .. h + disp(h).
+// A lane resolves a chunk of C neighbour keys together. Round 0 loads every key's home slot record (slot_hash
+// & mask; 32 B: key, voxel id, count, mean, displacement bound) at once, with no branch around the loads, so
+// they are all in flight together; a record holding the key is a hit. A key whose home slot holds another key
+// can only lie within the home's displacement bound (Slot::disp, the largest probe distance of the keys homed
+// there, kept by the inserts): a bound of 0 (the usual case, and always for an empty slot) ends the lookup as
+// a miss at once, otherwise the slots up to the bound are read, the records of every key of the chunk still
+// unresolved at a depth loaded together. The slot found is the one find_slot's linear probe reaches
+// (gpu.py:254-265): keys are never removed, so a key homed at h sits within h .. h + disp(h).
+//
+// With BITMAP set, the occupancy bitmap is read first and a key whose home bit is clear loads slot 0's record
+// instead of its own (cached by every lane, so it costs no memory traffic) and is a miss: one more dependent
+// load, for far less traffic on a large, sparse table (the batched localisation's throughput-bound launch);
+// the odometry's latency-bound loop reads the home records directly.
 //
 // The closest mean is chosen by (squared distance, position in the reference's neighbour order), so the
 // order in which candidates are evaluated does not matter: the result is the serial loop's strict-< choice.
@@ -68,7 +74,7 @@ __device__ inline void consider(float wx, float wy, float wz, int32_t s, int32_t
 
 // The closest mean over neighbours [j_lo, j_hi) of the (2R+1)^3 block around the point's voxel (dx outer, dz
 // inner), in chunks of C. hits counts the occupied neighbours found.
-template <int R, int C>
+template <int R, int C, bool BITMAP>
 __device__ inline void search_part(const Table &t, float inv_v, float wx, float wy, float wz, int j_lo, int j_hi,
                                    Cand &best, int32_t &hits) {
     constexpr int D = 2 * R + 1;
@@ -77,28 +83,30 @@ __device__ inline void search_part(const Table &t, float inv_v, float wx, float 
     for (int j0 = j_lo; j0 < j_hi; j0 += C) {
         int64_t key[C];
         int32_t h[C];
-        uint32_t w[C];
         LW_UNROLL for (int c = 0; c < C; ++c) {
-            const int j = j0 + c;
+            const int j = j0 + c;  // past j_hi: a real voxel's key, looked up and then ignored
             const int dx = j / (D * D) - R, dy = (j / D) % D - R, dz = j % D - R;
             key[c] = voxel_key_ijk(bx + dx, by + dy, bz + dz);
             h[c] = slot_hash(key[c]) & t.mask;
         }
-        LW_UNROLL for (int c = 0; c < C; ++c) w[c] = (j0 + c < j_hi) ? t.occ[h[c] >> 5] : 0u;
-        // round 0: the home slots
+        uint32_t live = 0u;  // keys of this chunk inside [j_lo, j_hi) (and, with BITMAP, with their home bit set)
+        LW_UNROLL for (int c = 0; c < C; ++c) live |= (j0 + c < j_hi ? 1u : 0u) << c;
+        if constexpr (BITMAP) {
+            uint32_t w[C];
+            LW_UNROLL for (int c = 0; c < C; ++c) w[c] = t.occ[h[c] >> 5];
+            LW_UNROLL for (int c = 0; c < C; ++c) live &= ~((bit_of(w[c], h[c]) ? 0u : 1u) << c);
+        }
+        // round 0: the home slots, every load issued before any is used
         uint32_t need = 0u;
         int32_t dmax[C];
         {
             int4 lo[C];
             float4 hi[C];
-            LW_UNROLL for (int c = 0; c < C; ++c) {
-                lo[c] = make_int4(-1, -1, -1, 0);
-                hi[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (bit_of(w[c], h[c])) load_slot(t.slots, h[c], lo[c], hi[c]);
-            }
+            LW_UNROLL for (int c = 0; c < C; ++c)
+                load_slot(t.slots, BITMAP && !((live >> c) & 1u) ? 0 : h[c], lo[c], hi[c]);
             LW_UNROLL for (int c = 0; c < C; ++c) {
                 dmax[c] = 0;
-                if (!bit_of(w[c], h[c])) continue;
+                if (!((live >> c) & 1u)) continue;
                 if (key_of(lo[c]) != key[c]) {
                     dmax[c] = __float_as_int(hi[c].w);  // the home's displacement bound
                     if (dmax[c] > 0) need |= 1u << c;
@@ -112,11 +120,7 @@ __device__ inline void search_part(const Table &t, float inv_v, float wx, float 
         for (int probe = 1; need != 0u && probe < PROBES; ++probe) {
             int4 lo[C];
             float4 hi[C];
-            LW_UNROLL for (int c = 0; c < C; ++c) {
-                lo[c] = make_int4(-1, -1, -1, 0);
-                hi[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if ((need >> c) & 1u) load_slot(t.slots, (h[c] + probe) & t.mask, lo[c], hi[c]);
-            }
+            LW_UNROLL for (int c = 0; c < C; ++c) load_slot(t.slots, (h[c] + probe) & t.mask, lo[c], hi[c]);
             LW_UNROLL for (int c = 0; c < C; ++c) {
                 if (!((need >> c) & 1u)) continue;
                 if (key_of(lo[c]) == key[c]) {
