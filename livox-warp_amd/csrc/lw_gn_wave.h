@@ -117,11 +117,12 @@ struct WaveOut {
 // 1 when handled (lane 0's o holds delta, and H_out, the registration's copy of H, has been written), 0 when
 // the caller must run gn::step instead, or -1 for gn::STEP_FEW (corr < min_corr). T: the pose. xi: the twist
 // from T to the prediction (gn::twist_between(T, T_pred), read when P.prior_info > 0), computed by the caller
-// (the persistent loop does it while the pass's search runs). stamps (measurement, or null): lane 0 records
-// the device wall clock after the set-up, the two factorisations (the degeneracy test's and the damped
-// system's, side by side) and the substitution.
+// (the persistent loop does it while the pass's search runs). With stamp set (measurement), lane 0 records
+// the device wall clock in t_setup, t_factor and t_solve after the set-up, the two factorisations (the
+// degeneracy test's and the damped system's, side by side) and the substitution.
 __device__ inline int wave_step(const double *v, const double *T, const double *xi, const gn::StepParams &P,
-                                double *sh, WaveOut &o, double *H_out, unsigned long long *stamps) {
+                                double *sh, WaveOut &o, double *H_out, bool stamp, unsigned long long &t_setup,
+                                unsigned long long &t_factor, unsigned long long &t_solve) {
     const int lane = threadIdx.x;
     o.corr = (int32_t)v[27];
     o.n_degen = 0;
@@ -172,11 +173,11 @@ __device__ inline int wave_step(const double *v, const double *T, const double *
         }
         __syncthreads();
     }
-    if (stamps && lane == 0) stamps[0] = wall_clock64();
+    if (stamp) t_setup = wall_clock64();
     // the damped system and the degeneracy test (J^T Hg J - thr I positive definite), factored together
     bool pd_a, pd_g;
     wave_cholesky2(A, 0.0, L, degen ? M3 : A, degen ? P.degen_thr : 0.0, L2, pd_a, pd_g);
-    if (stamps && lane == 0) stamps[1] = wall_clock64();
+    if (stamp) t_factor = wall_clock64();
     if (degen && !pd_g) return 0;  // a direction to hold: the scalar routine does it
     if (!pd_a) return 0;           // indefinite: the scalar routine's LU fallback
     if (lane == 0) {
@@ -191,7 +192,7 @@ __device__ inline int wave_step(const double *v, const double *T, const double *
             out[i] = t / L[6 * i + i];
         }
         LW_UNROLL for (int i = 0; i < 6; ++i) o.delta[i] = out[i];
-        if (stamps) stamps[2] = wall_clock64();
+        if (stamp) t_solve = wall_clock64();
     }
     if (lane < 36) H_out[lane] = H[lane];
     return 1;

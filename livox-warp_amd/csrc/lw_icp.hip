@@ -501,9 +501,9 @@ __device__ inline bool run_step(RegState *st, const double *v, const gn::StepPar
                                 double *T, const double *Tp, const double *xi, int32_t *s_flag,
                                 unsigned long long *ph) {
     gnw::WaveOut o;
-    unsigned long long t[4] = {0ull, 0ull, 0ull, 0ull};
+    unsigned long long t1 = 0ull, t2 = 0ull, t3 = 0ull;
     const unsigned long long t0 = ph ? wall_clock64() : 0ull;
-    const int r = gnw::wave_step(v, T, xi, sp, sh, o, st->H, ph ? t : nullptr);
+    const int r = gnw::wave_step(v, T, xi, sp, sh, o, st->H, ph != nullptr, t1, t2, t3);
     if (threadIdx.x == 0) {
         bool stop;
         if (r == 0) {
@@ -522,11 +522,11 @@ __device__ inline bool run_step(RegState *st, const double *v, const gn::StepPar
         if (ph) {
             atomicAdd(&ph[r == 0 ? 6 : 5], 1ull);
             if (r > 0) {
-                const unsigned long long t3 = wall_clock64();
-                atomicAdd(&ph[16], t[0] - t0);
-                atomicAdd(&ph[17], t[1] - t[0]);
-                atomicAdd(&ph[18], t[2] - t[1]);
-                atomicAdd(&ph[19], t3 - t[2]);
+                const unsigned long long t4 = wall_clock64();
+                atomicAdd(&ph[16], t1 - t0);
+                atomicAdd(&ph[17], t2 - t1);
+                atomicAdd(&ph[18], t3 - t2);
+                atomicAdd(&ph[19], t4 - t3);
                 atomicAdd(&ph[20], 1ull);
             }
         }
