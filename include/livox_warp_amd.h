@@ -68,7 +68,7 @@ typedef struct lw_reg_result {
 int lw_version(void);
 
 /* Odometry.__init__ buffers (odom.py:461-497): a registration map of reg_slots slots (a power of two,
- * odom.py:468) and frame-sized work buffers of frame_cap points. stream: a hipStream_t, or NULL for the
+ * odom.py:468; 64 to 2^27, a 4 GiB table) and frame-sized work buffers of frame_cap points (up to 2^26). stream: a hipStream_t, or NULL for the
  * default stream. The map starts empty (odom.py:527 -> reset). */
 int lw_create(lw_ctx **out, int64_t reg_slots, int64_t frame_cap, void *stream);
 int lw_destroy(lw_ctx *ctx);

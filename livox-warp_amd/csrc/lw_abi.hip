@@ -106,7 +106,7 @@ extern "C" {
 int lw_version(void) { return 100; }
 
 int lw_create(lw_ctx **out, int64_t reg_slots, int64_t frame_cap, void *stream) {
-    if (!out || !pow2(reg_slots) || reg_slots < 64 || reg_slots > (1ll << 30) || frame_cap <= 0 ||
+    if (!out || !pow2(reg_slots) || reg_slots < 64 || reg_slots > (1ll << 27) || frame_cap <= 0 ||
         frame_cap > (1ll << 26))
         return LW_EINVAL;
     lw_ctx *c = new (std::nothrow) lw_ctx();

@@ -273,8 +273,8 @@ __device__ inline void point_terms(const IcpArgs &a, float kappa, float px, floa
 // t_found and t_terms receive the device clock after the last lane group's neighbour search and after its
 // terms (measurement).
 template <int MODE, int R>
-__device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf, int g, int G, unsigned char *lds,
-                                 double &sum, double &comp, int32_t &hits, int32_t &matched, float3 pre,
+__device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf, const NbrOff *nbr, int g, int G,
+                                 unsigned char *lds, double &sum, double &comp, int32_t &hits, int32_t &matched, float3 pre,
                                  bool has_pre, bool stamp, unsigned long long &t_found,
                                  unsigned long long &t_terms) {
     constexpr int S = Split<R>::S, C = Split<R>::C;
@@ -297,7 +297,7 @@ __device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf
                 pz = a.ds[3 * (int64_t)i + 2];
             }
             transform_point(Tf, px, py, pz, wx, wy, wz);
-            search_part<R, C, false>(Table{a.slots, a.occ, a.mask}, a.inv_v, wx, wy, wz, sub * KQ, min(K, (sub + 1) * KQ),
+            search_part<R, C, false>(Table{a.slots, a.occ, a.mask}, nbr, a.inv_v, wx, wy, wz, sub * KQ, min(K, (sub + 1) * KQ),
                                best, hits);
         }
         if (stamp) t_found = wall_clock64();
@@ -615,8 +615,10 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_icp(IcpArgs a) {
     __shared__ float Tf[16];
     __shared__ int32_t s_last;
     __shared__ double vs[64], xi[6];
+    __shared__ NbrOff nbr[NBR_ROWS];
     __shared__ __align__(16) unsigned char lds[RED_BYTES];
     if (threadIdx.x < 16) Tf[threadIdx.x] = st->Tf[threadIdx.x];
+    nbr_fill(nbr);
     __syncthreads();
     double sum, comp;
     int32_t hits = 0, matched = 0;
@@ -624,7 +626,7 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_icp(IcpArgs a) {
     const int G = (int)gridDim.x;
     if (a.phase && blockIdx.x == 0 && threadIdx.x == 0) a.st->t_rel = wall_clock64();
     unsigned long long t_found, t_terms;
-    wave_pass<MODE, R>(a, p, Tf, blockIdx.x, G, lds, sum, comp, hits, matched, {}, false, false, t_found, t_terms);
+    wave_pass<MODE, R>(a, p, Tf, nbr, blockIdx.x, G, lds, sum, comp, hits, matched, {}, false, false, t_found, t_terms);
     if (!row_and_ticket<MODE>(a, sum, comp, hits, matched, G, &s_last)) return;
     constexpr int D = 2 * R + 1;
     finish_pass<MODE>(a, p, D * D * D, G, vs, xi, &s_last, a.phase ? wall_clock64() : 0ull,
@@ -733,7 +735,9 @@ __device__ inline void searcher(const LoopArgs &L, int32_t n, unsigned char *lds
     const int lane = threadIdx.x;
     const int g = (int)blockIdx.x - 1;
     __shared__ float Tf[16];
+    __shared__ NbrOff nbr[NBR_ROWS];
     if (lane < 16) Tf[lane] = st->Tf[lane];
+    nbr_fill(nbr);
     // this lane's point in its first lane group, for either reach: the same every pass
     float3 pt[2] = {};
     LW_UNROLL for (int k = 0; k < 2; ++k) {
@@ -752,10 +756,10 @@ __device__ inline void searcher(const LoopArgs &L, int32_t n, unsigned char *lds
             int32_t hits = 0, matched = 0;
             unsigned long long t_found = 0ull, t_terms = 0ull;
             if (reach == 2)
-                wave_pass<MODE, 2>(a, p, Tf, g, G, lds, sum, comp, hits, matched, pt[0], true, stamp, t_found,
+                wave_pass<MODE, 2>(a, p, Tf, nbr, g, G, lds, sum, comp, hits, matched, pt[0], true, stamp, t_found,
                                    t_terms);
             else
-                wave_pass<MODE, 1>(a, p, Tf, g, G, lds, sum, comp, hits, matched, pt[1], true, stamp, t_found,
+                wave_pass<MODE, 1>(a, p, Tf, nbr, g, G, lds, sum, comp, hits, matched, pt[1], true, stamp, t_found,
                                    t_terms);
             const unsigned long long t_summed = stamp ? wall_clock64() : 0ull;
             write_row<MODE>(a, g, sum, comp, hits, matched);

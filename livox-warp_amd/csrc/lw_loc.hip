@@ -75,6 +75,8 @@ __global__ __launch_bounds__(B) void k_loc(LocArgs a) {
     __shared__ double Td[16];
     __shared__ float Tf[16];
     __shared__ int32_t s_unit, s_stop;
+    __shared__ NbrOff nbr[NBR_ROWS];
+    nbr_fill(nbr);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int32_t hits = 0, matched_total = 0;
     unsigned long long pt_iters = 0;
@@ -105,7 +107,7 @@ __global__ __launch_bounds__(B) void k_loc(LocArgs a) {
                 float wx, wy, wz;
                 transform_point(Tf, pp[0], pp[1], pp[2], wx, wy, wz);
                 Cand best{a.max_d2, -1, 27, 0.0f, 0.0f, 0.0f};
-                search_part<1, LOC_C, true>(Table{a.slots, a.occ, a.mask}, a.inv_v, wx, wy, wz, 0, 27, best, hits);
+                search_part<1, LOC_C, true>(Table{a.slots, a.occ, a.mask}, nbr, a.inv_v, wx, wy, wz, 0, 27, best, hits);
                 const int32_t s = best.s;
                 if (s < 0) continue;
                 const float mx = best.mx, my = best.my, mz = best.mz;

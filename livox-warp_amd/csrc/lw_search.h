@@ -1,10 +1,15 @@
 // The closest-mean neighbour search shared by the odometry accumulation (lw_icp.hip) and the batched
 // localisation (lw_loc.hip): odom.py:308-334 with find_slot (gpu.py:254-265).
 //
-// A lane resolves a chunk of C neighbour keys together. Round 0 loads every key's home slot record (slot_hash
-// & mask; 32 B: key, voxel id, count, mean, displacement bound) at once, with no branch around the loads, so
-// they are all in flight together; a record holding the key is a hit. A key whose home slot holds another key
-// can only lie within the home's displacement bound (Slot::disp, the largest probe distance of the keys homed
+// A lane resolves a chunk of C neighbour keys together. Keys and hashes come from the point's own voxel key
+// by addition: neighbour j's key is key0 + off(j), off(j) = dx + dy 2^21 + dz 2^42, and its hash product is
+// key0 * HASH_MUL + off(j) * HASH_MUL (mod 2^64), both offsets read from a small LDS table (NbrTable); that
+// holds whenever the point's voxel indices lie at least R inside the 21-bit key range, and a wave with a
+// point outside it computes every key and hash directly (voxel_key_ijk, slot_hash), so the keys are the
+// reference's in every case. Round 0 loads every key's home slot record (slot_hash & mask; 32 B: key, voxel
+// id, count, mean, displacement bound) at once, with no branch around the loads, and the tests that follow
+// are selects, not branches. A record holding the key is a hit. A key whose home slot holds another key can
+// only lie within the home's displacement bound (Slot::disp, the largest probe distance of the keys homed
 // there, kept by the inserts): a bound of 0 (the usual case, and always for an empty slot) ends the lookup as
 // a miss at once, otherwise the slots up to the bound are read, the records of every key of the chunk still
 // unresolved at a depth loaded together. The slot found is the one find_slot's linear probe reaches
@@ -38,9 +43,10 @@ struct Table {
     int32_t mask;
 };
 
-// A slot as two 16-byte loads: lo = {key lo, key hi, voxel id, count bits}, hi = {mean x, y, z, disp bits}.
+// A slot as two 16-byte loads: lo = {key lo, key hi, voxel id, count bits}, hi = {mean x, y, z, disp bits}. s is
+// a slot index (0 <= s < 2^27): its byte offset fits 32 bits, so the load takes the table base as a scalar.
 __device__ inline void load_slot(const Slot *slots, int32_t s, int4 &lo, float4 &hi) {
-    const int4 *p = reinterpret_cast<const int4 *>(slots + s);
+    const int4 *p = reinterpret_cast<const int4 *>(reinterpret_cast<const char *>(slots) + ((uint32_t)s << 5));
     lo = p[0];
     hi = reinterpret_cast<const float4 *>(p)[1];
 }
@@ -51,43 +57,79 @@ __device__ inline int64_t key_of(const int4 &lo) {
 
 __device__ inline bool bit_of(uint32_t word, int32_t s) { return ((word >> (s & 31)) & 1u) != 0u; }
 
-// One candidate: skip count 0 (odom.py:324), then keep it if it is closer, or as close and earlier.
-__device__ inline void consider(float wx, float wy, float wz, int32_t s, int32_t j, const int4 &lo, const float4 &hi,
-                                Cand &best) {
-    if (!(__int_as_float(lo.w) > 0.0f)) return;
-    float d0 = wx - hi.x, d1 = wy - hi.y, d2 = wz - hi.z;
+// One candidate, without branches: taken when ok (the key was found), its count is positive (odom.py:324) and
+// it is closer, or as close and earlier in the order, than the best so far (the serial loop starts from
+// best_d2 = max_d2 and takes strictly smaller distances, odom.py:326-330).
+__device__ inline void consider(bool ok, float wx, float wy, float wz, int32_t s, int32_t j, const int4 &lo,
+                                const float4 &hi, Cand &best) {
+    const float d0 = wx - hi.x, d1 = wy - hi.y, d2 = wz - hi.z;
     float dd = d0 * d0;
     dd = dd + d1 * d1;
     dd = dd + d2 * d2;
-    // the serial loop starts from best_d2 = max_d2 and takes strictly smaller distances (odom.py:326-330)
-    const bool take = best.s < 0 ? dd < best.d2 : (dd < best.d2 || (dd == best.d2 && j < best.j));
-    if (take) {
-        best.d2 = dd;
-        best.s = s;
-        best.j = j;
-        best.mx = hi.x;
-        best.my = hi.y;
-        best.mz = hi.z;
-        best.vid = lo.z;
+    const bool closer = dd < best.d2 || (best.s >= 0 && dd == best.d2 && j < best.j);
+    const bool take = ok && __int_as_float(lo.w) > 0.0f && closer;
+    best.d2 = take ? dd : best.d2;
+    best.s = take ? s : best.s;
+    best.j = take ? j : best.j;
+    best.mx = take ? hi.x : best.mx;
+    best.my = take ? hi.y : best.my;
+    best.mz = take ? hi.z : best.mz;
+    best.vid = take ? lo.z : best.vid;
+}
+
+// Key and hash-product offsets of the neighbours of either reach (dx outer, dz inner), in LDS: rows
+// [0, NBR_ROWS_1) for reach 1, then [NBR_ROWS_1, NBR_ROWS_1 + NBR_ROWS_2) for reach 2; past (2R+1)^3 the rows
+// hold the offsets the same formula gives (the lookups of a padded chunk are made and then ignored).
+struct NbrOff {
+    uint64_t mul;  // off * HASH_MUL (mod 2^64)
+    uint64_t key;  // off = dx + dy 2^21 + dz 2^42 (two's complement)
+};
+constexpr int NBR_ROWS_1 = 48, NBR_ROWS_2 = 144, NBR_ROWS = NBR_ROWS_1 + NBR_ROWS_2;
+
+// Every lane of the workgroup takes part; the caller synchronises before the first search.
+__device__ inline void nbr_fill(NbrOff *tab) {
+    for (int r = threadIdx.x; r < NBR_ROWS; r += blockDim.x) {
+        const int R = r < NBR_ROWS_1 ? 1 : 2, D = 2 * R + 1, j = r < NBR_ROWS_1 ? r : r - NBR_ROWS_1;
+        const int dx = j / (D * D) - R, dy = (j / D) % D - R, dz = j % D - R;
+        const uint64_t off = (uint64_t)(int64_t)dx + ((uint64_t)(int64_t)dy << KEY_BITS) +
+                             ((uint64_t)(int64_t)dz << (2 * KEY_BITS));
+        tab[r] = NbrOff{off * HASH_MUL, off};
     }
 }
 
 // The closest mean over neighbours [j_lo, j_hi) of the (2R+1)^3 block around the point's voxel (dx outer, dz
-// inner), in chunks of C. hits counts the occupied neighbours found.
+// inner), in chunks of C. hits counts the occupied neighbours found. tab: the LDS offset table (nbr_fill).
 template <int R, int C, bool BITMAP>
-__device__ inline void search_part(const Table &t, float inv_v, float wx, float wy, float wz, int j_lo, int j_hi,
-                                   Cand &best, int32_t &hits) {
+__device__ inline void search_part(const Table &t, const NbrOff *tab, float inv_v, float wx, float wy, float wz,
+                                   int j_lo, int j_hi, Cand &best, int32_t &hits) {
     constexpr int D = 2 * R + 1;
+    constexpr int ROW0 = R == 1 ? 0 : NBR_ROWS_1;
     static_assert(C <= 32, "one bit per key of a chunk");
+    static_assert(D * D * D + C - 1 <= (R == 1 ? NBR_ROWS_1 : NBR_ROWS_2), "a chunk starting below (2R+1)^3 stays in the table");
     const int32_t bx = ifloor(wx * inv_v), by = ifloor(wy * inv_v), bz = ifloor(wz * inv_v);
+    // inside the key range by at least R on every axis: keys and hashes by addition
+    constexpr int32_t LO = R - (int32_t)KEY_BIAS, HI = (int32_t)KEY_BIAS - 1 - R;
+    const bool inside = bx >= LO && bx <= HI && by >= LO && by <= HI && bz >= LO && bz <= HI;
+    const bool fast = __ballot(!inside) == 0ull;
+    const uint64_t key0 = (uint64_t)voxel_key_ijk(bx, by, bz);
+    const uint64_t mul0 = key0 * HASH_MUL;
     for (int j0 = j_lo; j0 < j_hi; j0 += C) {
         int64_t key[C];
         int32_t h[C];
-        LW_UNROLL for (int c = 0; c < C; ++c) {
-            const int j = j0 + c;  // past j_hi: a real voxel's key, looked up and then ignored
-            const int dx = j / (D * D) - R, dy = (j / D) % D - R, dz = j % D - R;
-            key[c] = voxel_key_ijk(bx + dx, by + dy, bz + dz);
-            h[c] = slot_hash(key[c]) & t.mask;
+        if (fast) {
+            LW_UNROLL for (int c = 0; c < C; ++c) {
+                const NbrOff o = tab[ROW0 + j0 + c];
+                key[c] = (int64_t)(key0 + o.key);
+                const uint64_t p = mul0 + o.mul;
+                h[c] = (int32_t)(((uint32_t)p ^ (uint32_t)(p >> 29)) & 0x7FFFFFFFu) & t.mask;
+            }
+        } else {
+            LW_UNROLL for (int c = 0; c < C; ++c) {
+                const int j = j0 + c;
+                const int dx = j / (D * D) - R, dy = (j / D) % D - R, dz = j % D - R;
+                key[c] = voxel_key_ijk(bx + dx, by + dy, bz + dz);
+                h[c] = slot_hash(key[c]) & t.mask;
+            }
         }
         uint32_t live = 0u;  // keys of this chunk inside [j_lo, j_hi) (and, with BITMAP, with their home bit set)
         LW_UNROLL for (int c = 0; c < C; ++c) live |= (j0 + c < j_hi ? 1u : 0u) << c;
@@ -105,15 +147,12 @@ __device__ inline void search_part(const Table &t, float inv_v, float wx, float 
             LW_UNROLL for (int c = 0; c < C; ++c)
                 load_slot(t.slots, BITMAP && !((live >> c) & 1u) ? 0 : h[c], lo[c], hi[c]);
             LW_UNROLL for (int c = 0; c < C; ++c) {
-                dmax[c] = 0;
-                if (!((live >> c) & 1u)) continue;
-                if (key_of(lo[c]) != key[c]) {
-                    dmax[c] = __float_as_int(hi[c].w);  // the home's displacement bound
-                    if (dmax[c] > 0) need |= 1u << c;
-                    continue;
-                }
-                ++hits;
-                consider(wx, wy, wz, h[c], j0 + c, lo[c], hi[c], best);
+                const bool in = ((live >> c) & 1u) != 0u;
+                const bool same = key_of(lo[c]) == key[c];
+                dmax[c] = __float_as_int(hi[c].w);  // the home's displacement bound
+                need |= (in && !same && dmax[c] > 0 ? 1u : 0u) << c;
+                hits += in && same ? 1 : 0;
+                consider(in && same, wx, wy, wz, h[c], j0 + c, lo[c], hi[c], best);
             }
         }
         // the displaced keys: every unresolved key of the chunk one slot further per round
@@ -122,14 +161,12 @@ __device__ inline void search_part(const Table &t, float inv_v, float wx, float 
             float4 hi[C];
             LW_UNROLL for (int c = 0; c < C; ++c) load_slot(t.slots, (h[c] + probe) & t.mask, lo[c], hi[c]);
             LW_UNROLL for (int c = 0; c < C; ++c) {
-                if (!((need >> c) & 1u)) continue;
-                if (key_of(lo[c]) == key[c]) {
-                    need &= ~(1u << c);
-                    ++hits;
-                    consider(wx, wy, wz, (h[c] + probe) & t.mask, j0 + c, lo[c], hi[c], best);
-                } else if (probe >= dmax[c]) {
-                    need &= ~(1u << c);  // past the bound: not in the map
-                }
+                const bool in = ((need >> c) & 1u) != 0u;
+                const bool same = key_of(lo[c]) == key[c];
+                hits += in && same ? 1 : 0;
+                consider(in && same, wx, wy, wz, (h[c] + probe) & t.mask, j0 + c, lo[c], hi[c], best);
+                // found, or past the bound: not in the map
+                need &= ~((in && (same || probe >= dmax[c]) ? 1u : 0u) << c);
             }
         }
     }
