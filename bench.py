@@ -316,7 +316,7 @@ def main():
         push(k)
     odo.synchronize()
     torch.cuda.synchronize()
-    phases = np.zeros(18)
+    phases = np.zeros(22)
     lw._abi.check(L.lw_prof_phases(odo._ctx, lw._abi.ptr(phases, lw._abi._f64p)), "lw_prof_phases")
     lw._abi.check(L.lw_prof_enable(odo._ctx, 1), "lw_prof_enable")
     lw._abi.check(L.lw_prof_reset(odo._ctx), "lw_prof_reset")
@@ -370,7 +370,10 @@ def main():
                                          "searcher_row_flag": round(phases[6] / max(phases[7], 1), 2),
                                          "searcher_neighbours": round(phases[15] / max(phases[7], 1), 2),
                                          "searcher_terms": round(phases[16] / max(phases[7], 1), 2),
-                                         "searcher_row_sum": round(phases[17] / max(phases[7], 1), 2)},
+                                         "searcher_row_sum": round(phases[17] / max(phases[7], 1), 2),
+                                         "all_searchers_wake": round(phases[18] / max(phases[20], 1), 2),
+                                         "all_searchers_flag": round(phases[19] / max(phases[20], 1), 2),
+                                         "slowest_searcher_row": round(phases[21], 2)},
                          "steps_on_wave": int(phases[8]), "steps_one_lane": int(phases[9]),
                          "us_per_wave_step": {k: round(phases[10 + i] / max(phases[14], 1), 3) for i, k in
                                               enumerate(("setup", "factor", "substitute", "update"))}},

@@ -216,7 +216,10 @@ int lw_prof_read(lw_ctx *ctx, double *out);
  * prior), out[11] the two Cholesky factorisations (degeneracy test and damped system, side by side), out[12]
  * the substitution, out[13] the pose update; out[14] the wave steps timed. The first searching workgroup's
  * search and terms (out[5]) in parts: out[15] point to closest means (neighbour search), out[16] the terms
- * into its LDS rows, out[17] the sum of its rows; counted with out[7]. out holds 18 doubles. No counterpart in the reference. */
+ * into its LDS rows, out[17] the sum of its rows; counted with out[7]. Every searching workgroup's passes
+ * after the first: out[18] from the pose's publication to its wake-up, out[19] to its row flag; out[20] the
+ * searcher-passes timed; out[21] the slowest searcher's row per pass, on average, from the publication.
+ * out holds 22 doubles. No counterpart in the reference. */
 int lw_prof_phases(lw_ctx *ctx, double *out);
 
 /* Host builds of the device's float64 step and per-voxel precision, for tests on machines without a

@@ -622,6 +622,9 @@ int lw_prof_phases(lw_ctx *c, double *out) {
     for (int i = 0; i < 4; ++i) out[10 + i] = (double)v[16 + i] * us_per_tick;
     out[14] = (double)v[20];
     for (int i = 0; i < 3; ++i) out[15 + i] = (double)v[21 + i] * us_per_tick;
+    for (int i = 0; i < 2; ++i) out[18 + i] = (double)v[24 + i] * us_per_tick;
+    out[20] = (double)v[26];
+    out[21] = v[29] ? (double)v[28] * us_per_tick / (double)v[29] : 0.0;
     return 0;
 }
 

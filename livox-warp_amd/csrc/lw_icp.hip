@@ -335,42 +335,52 @@ __device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf
 // give the same sums bit for bit. Lane k < RED_COLS handles entry k: the 53 ACC entries, then the
 // measurement counts (hits, matched points) in 53 and 54. Rows are read device-coherently.
 constexpr int RED_COLS = ACC_USED + 2;
-constexpr int RED_BATCH = 32;
+constexpr int RED_BATCH = 16;
 constexpr int MAX_BATCHES = MAX_LOOP_ROWS / RED_BATCH;
+static_assert(MAX_BATCHES <= 32, "one bit per batch");
 
 struct BSum {
     double e;    // float64 sum: the float64 form's entries, and every form's counts (integers: exact)
     float s, c;  // float32 sum and compensation of the float32 forms
 };
 
-// Rows [b0, b1) (b1 - b0 <= RED_BATCH) of this lane's column, in row order, all loads in flight together.
+// Row r (r < G), entry k, as the sum adds it; rows at or past G add nothing.
 template <int MODE>
-__device__ inline BSum batch_sum(const double *partials, int b0, int b1) {
+__device__ inline void add_row(BSum &r, bool ok, double x, double y) {
+    if constexpr (MODE == 0) {
+        r.e = ok ? r.e + x : r.e;
+    } else if constexpr (MODE == 2) {
+        r.s = ok ? r.s + (float)x : r.s;
+        r.e = ok ? r.e + x : r.e;
+    } else {
+        float t, er;
+        two_sum(r.s, (float)x, t, er);
+        r.s = ok ? t : r.s;
+        r.c = ok ? r.c + (float)y + er : r.c;
+        r.e = ok ? r.e + x : r.e;
+    }
+}
+
+// The sums of NB batches b[0..NB) (each RED_BATCH rows from row b[k] * RED_BATCH, rows at or past G adding
+// nothing) of this lane's column, in row order from zero. Every load is issued before any is used (addresses
+// clamped to row G - 1, so none is conditional).
+template <int MODE, int NB>
+__device__ inline void sum_batches(const double *partials, int G, const int (&b)[NB], BSum (&out)[NB]) {
     const int lane = threadIdx.x;
-    BSum r{0.0, 0.0f, 0.0f};
-    if (lane >= RED_COLS) return r;
-    const double *col = partials + lane;
-    double x[RED_BATCH], y[RED_BATCH];
-    LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) {
-        x[u] = b0 + u < b1 ? get(col + (size_t)(b0 + u) * 128) : 0.0;
-        if constexpr (MODE == 1) y[u] = b0 + u < b1 && lane < ACC_USED ? get(col + (size_t)(b0 + u) * 128 + 64) : 0.0;
-    }
-    LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) {
-        if (b0 + u >= b1) continue;
-        if constexpr (MODE == 0) {
-            r.e += x[u];
-        } else if constexpr (MODE == 2) {
-            r.s += (float)x[u];
-            r.e += x[u];
-        } else {
-            float t, er;
-            two_sum(r.s, (float)x[u], t, er);
-            r.s = t;
-            r.c = r.c + (float)y[u] + er;
-            r.e += x[u];
+    const int cx = lane < RED_COLS ? lane : 0, cy = 64 + (lane < ACC_USED ? lane : 0);
+    double x[NB][RED_BATCH], y[NB][RED_BATCH];
+    LW_UNROLL for (int k = 0; k < NB; ++k)
+        LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) {
+            const size_t row = (size_t)min(b[k] * RED_BATCH + u, G - 1) * 128;
+            x[k][u] = get(partials + row + cx);
+            if constexpr (MODE == 1) y[k][u] = get(partials + row + cy);
         }
+    LW_UNROLL for (int k = 0; k < NB; ++k) {
+        BSum r{0.0, 0.0f, 0.0f};
+        LW_UNROLL for (int u = 0; u < RED_BATCH; ++u)
+            add_row<MODE>(r, b[k] * RED_BATCH + u < G, x[k][u], MODE == 1 ? y[k][u] : 0.0);
+        out[k] = r;
     }
-    return r;
 }
 
 template <int MODE>
@@ -398,48 +408,71 @@ __device__ inline double entry_of(const BSum &r) {
 template <int MODE>
 __device__ inline double reduce_rows(const double *partials, int G) {
     BSum acc{0.0, 0.0f, 0.0f};
-    for (int b = 0; b < G; b += RED_BATCH) combine<MODE>(acc, batch_sum<MODE>(partials, b, min(b + RED_BATCH, G)));
+    for (int b = 0; b < (G + RED_BATCH - 1) / RED_BATCH; ++b) {
+        const int bb[1] = {b};
+        BSum out[1];
+        sum_batches<MODE, 1>(partials, G, bb, out);
+        combine<MODE>(acc, out[0]);
+    }
     return entry_of<MODE>(acc);
 }
 
-// The persistent loop's leader: polls the G row flags (lane l reads rows l, l + 64, ...), sums each batch whose
-// rows all carry this pass's tag into bs (LDS, MAX_BATCHES x 64), in whatever order the batches complete, and
-// combines the batch sums in batch order once every row is in. t_in: the device clock when the last rows
-// were seen (measurement). False when rows are still missing after SPIN_LIMIT polls.
 constexpr uint32_t SPIN_LIMIT = 1u << 24;  // ~ seconds of polling: a wait that long means the loop is broken
 constexpr int32_t STATUS_SPIN_TIMEOUT = -1;
 
+// One poll of the G row flags (lane l reads rows l, l + 64, ...): bit b set when every row of batch b carries
+// the tag. Loads of rows past G read row G - 1 and count as set.
+__device__ inline uint32_t poll_flags(const uint32_t *flags, int G, uint32_t tag) {
+    const int lane = threadIdx.x;
+    uint32_t f[MAX_LOOP_ROWS / 64];
+    LW_UNROLL for (int k = 0; k < MAX_LOOP_ROWS / 64; ++k) f[k] = get(&flags[min(64 * k + lane, G - 1)]);
+    uint32_t ready = 0u;
+    LW_UNROLL for (int k = 0; k < MAX_LOOP_ROWS / 64; ++k) {
+        const unsigned long long m = __ballot(f[k] >= tag || 64 * k + lane >= G);
+        LW_UNROLL for (int q = 0; q < 64 / RED_BATCH; ++q)
+            if (((m >> (q * RED_BATCH)) & ((1ull << RED_BATCH) - 1ull)) == ((1ull << RED_BATCH) - 1ull))
+                ready |= 1u << (k * (64 / RED_BATCH) + q);
+    }
+    return ready;
+}
+
+// The persistent loop's leader: sums each batch whose rows all carry this pass's tag into bs (LDS,
+// MAX_BATCHES x 64), up to NB batches at a time with all their loads in flight and the next poll of the flags
+// beside them, in whatever order the batches complete, and combines the batch sums in batch order once every
+// row is in. t_in: the device clock when the last rows were seen (measurement). False when rows are still
+// missing after SPIN_LIMIT polls that found nothing new.
 template <int MODE>
 __device__ inline bool leader_reduce(const double *partials, const uint32_t *flags, int G, uint32_t tag, BSum *bs,
                                      double &v, unsigned long long &t_in, bool stamp) {
+    constexpr int NB = MODE == 1 ? 2 : 4;
     const int lane = threadIdx.x;
     const int nb = (G + RED_BATCH - 1) / RED_BATCH;
     uint32_t pend = (nb >= 32) ? ~0u : ((1u << nb) - 1u);
     uint32_t spins = 0;
+    uint32_t ready = poll_flags(flags, G, tag) & pend;
     while (pend) {
-        uint32_t f[MAX_LOOP_ROWS / 64];
-        LW_UNROLL for (int k = 0; k < MAX_LOOP_ROWS / 64; ++k) {
-            const int r = 64 * k + lane;
-            f[k] = (64 * k < G && r < G) ? get(&flags[r]) : tag;
-        }
-        uint32_t ready = 0;
-        LW_UNROLL for (int k = 0; k < MAX_LOOP_ROWS / 64; ++k) {
-            const unsigned long long m = __ballot(f[k] >= tag);
-            if ((uint32_t)m == ~0u) ready |= 1u << (2 * k);
-            if ((uint32_t)(m >> 32) == ~0u) ready |= 1u << (2 * k + 1);
-        }
-        ready &= pend;
         if (ready == 0u) {
             if (++spins > SPIN_LIMIT) return false;
+            ready = poll_flags(flags, G, tag) & pend;
             continue;
         }
         if (stamp && ready == pend) t_in = wall_clock64();
-        pend &= ~ready;
-        while (ready) {
-            const int b = __builtin_ctz(ready);
-            ready &= ready - 1u;
-            bs[b * ICP_BLOCK + lane] = batch_sum<MODE>(partials, b * RED_BATCH, min(b * RED_BATCH + RED_BATCH, G));
+        int b[NB];
+        int n_valid = 0;
+        LW_UNROLL for (int k = 0; k < NB; ++k) {
+            b[k] = ready ? __builtin_ctz(ready) : b[0];
+            if (ready) {
+                ready &= ready - 1u;
+                pend &= ~(1u << b[k]);
+                ++n_valid;
+            }
         }
+        const uint32_t next = pend & ~ready ? poll_flags(flags, G, tag) : 0u;  // in flight with the rows
+        BSum out[NB];
+        sum_batches<MODE, NB>(partials, G, b, out);
+        LW_UNROLL for (int k = 0; k < NB; ++k)
+            if (k < n_valid && lane < RED_COLS) bs[b[k] * ICP_BLOCK + lane] = out[k];
+        ready = (ready | next) & pend;
     }
     BSum acc{0.0, 0.0f, 0.0f};
     for (int b = 0; b < nb; ++b) combine<MODE>(acc, bs[b * ICP_BLOCK + lane]);  // this lane's own entries
@@ -665,6 +698,8 @@ __device__ inline void leader(const LoopArgs &L, int32_t n, unsigned char *lds) 
     const int lane = threadIdx.x;
     __shared__ double sT[16], sTp[16], vs[64], xi[6];
     __shared__ int32_t s_flag;
+    static_assert(MAX_BATCHES * ICP_BLOCK * sizeof(BSum) + gnw::W_DOUBLES * sizeof(double) <= RED_BYTES,
+                  "the batch sums and the step's scratch share the searchers' LDS rows");
     BSum *bs = reinterpret_cast<BSum *>(lds);
     double *sh = reinterpret_cast<double *>(lds + MAX_BATCHES * ICP_BLOCK * sizeof(BSum));
     if (lane < 16) {
@@ -687,6 +722,10 @@ __device__ inline void leader(const LoopArgs &L, int32_t n, unsigned char *lds) 
             return;
         }
         const unsigned long long t_red = a.phase ? wall_clock64() : 0ull;
+        if (a.phase && it > 0 && lane == 0) {  // the pass's slowest searcher, from the pose's publication
+            atomicAdd(&a.phase[28], atomicExch(&a.phase[27], 0ull));
+            atomicAdd(&a.phase[29], 1ull);
+        }
         vs[lane] = v;
         st->acc[lane] = lane < ACC_USED ? v : 0.0;
         __syncthreads();
@@ -747,6 +786,8 @@ __device__ inline void searcher(const LoopArgs &L, int32_t n, unsigned char *lds
     __syncthreads();
     const bool stamp = a.phase && g == 0;  // the first searcher's view of the passes
     unsigned long long t_wake = stamp ? wall_clock64() : 0ull;
+    // every searcher's view (passes after the first): wake-up and row flag after the pose's publication
+    unsigned long long t_pub = 0ull, t_woke = 0ull, sum_wake = 0ull, sum_done = 0ull, n_timed = 0ull;
     for (int it = 0; it < L.max_iter; ++it) {
         const int reach = it < L.wide_iters ? 2 : 1;
         const Pass p{n, reach == 2 ? L.max_d2_r2 : L.max_d2_r1, st->kappa[it], it};
@@ -762,8 +803,15 @@ __device__ inline void searcher(const LoopArgs &L, int32_t n, unsigned char *lds
                 wave_pass<MODE, 1>(a, p, Tf, nbr, g, G, lds, sum, comp, hits, matched, pt[1], true, stamp, t_found,
                                    t_terms);
             const unsigned long long t_summed = stamp ? wall_clock64() : 0ull;
+            if (a.phase && it > 0 && lane == 0)  // the pass's latest row so far (ordered before the flag)
+                atomicMax(&a.phase[27], wall_clock64() - t_pub);
             write_row<MODE>(a, g, sum, comp, hits, matched);
             if (lane == 0) put(&L.flags[g], (uint32_t)(it + 1));
+            if (a.phase && it > 0) {
+                sum_wake += t_woke - t_pub;
+                sum_done += wall_clock64() - t_pub;
+                ++n_timed;
+            }
             if (stamp && lane == 0) {
                 atomicAdd(&a.phase[13], t_summed - t_wake);
                 atomicAdd(&a.phase[14], wall_clock64() - t_summed);
@@ -791,12 +839,21 @@ __device__ inline void searcher(const LoopArgs &L, int32_t n, unsigned char *lds
                 return;
             }
         }
-        if (stamp) {
-            t_wake = wall_clock64();
-            if (lane == 0) atomicAdd(&a.phase[12], t_wake - get(&st->t_rel));
+        if (a.phase) {
+            t_woke = wall_clock64();
+            t_pub = get(&st->t_rel);
+            if (stamp) {
+                t_wake = t_woke;
+                if (lane == 0) atomicAdd(&a.phase[12], t_wake - t_pub);
+            }
         }
         __syncthreads();
         if (tag0 & TAG_STOP) break;
+    }
+    if (a.phase && lane == 0) {
+        atomicAdd(&a.phase[24], sum_wake);
+        atomicAdd(&a.phase[25], sum_done);
+        atomicAdd(&a.phase[26], n_timed);
     }
 }
 

@@ -12,8 +12,8 @@ namespace lw {
 constexpr int BLOCK = 256;           // threads per block of the streaming kernels
 constexpr int ICP_BLOCK = 64;        // one wave per accumulation workgroup
 constexpr int ICP_MAX_GROUPS = 2048; // partial rows kept for the cross-workgroup reduction
-constexpr int LW_PROF_COUNTERS = 24; // device measurement counters (lw_ctx::prof_dev)
-constexpr int MAX_LOOP_ROWS = 512;   // searching workgroups (partial rows) of the persistent registration
+constexpr int LW_PROF_COUNTERS = 32; // device measurement counters (lw_ctx::prof_dev)
+constexpr int MAX_LOOP_ROWS = 256;   // searching workgroups (partial rows) of the persistent registration
 constexpr int SCAN_THREADS = 1024;
 
 // Device-resident state of one registration (one Gauss-Newton loop).
@@ -98,7 +98,9 @@ struct lw_ctx {
     // little) also [5] [6] steps on the wave / in one lane [8..11] pass phases (search, reduce, step,
     // passes) [12..15] the first searcher's phases (wake, search and terms, row and flag, passes)
     // [16..20] wave-step parts (set-up, factorisations, substitution, pose update, steps) [21..23] the
-    // first searcher's neighbour search, terms, row sum
+    // first searcher's neighbour search, terms, row sum [24..26] every searcher's wake-up and row flag after
+    // the pose's publication, and the searcher-passes timed [27] the current pass's latest row (running max)
+    // [28] [29] the latest row per pass, summed, and the passes
     bool prof_on = false;
     bool prof_phases = false;
     hipEvent_t *prof_ev = nullptr;
