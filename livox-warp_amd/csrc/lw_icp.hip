@@ -420,20 +420,34 @@ __device__ inline double reduce_rows(const double *partials, int G) {
 constexpr uint32_t SPIN_LIMIT = 1u << 24;  // ~ seconds of polling: a wait that long means the loop is broken
 constexpr int32_t STATUS_SPIN_TIMEOUT = -1;
 
-// One poll of the G row flags (lane l reads rows l, l + 64, ...): bit b set when every row of batch b carries
-// the tag. Loads of rows past G read row G - 1 and count as set.
-__device__ inline uint32_t poll_flags(const uint32_t *flags, int G, uint32_t tag) {
-    const int lane = threadIdx.x;
+// A poll of the G row flags: lane l reads rows l, l + 64, ... (rows past G read row G - 1). Issuing and
+// evaluating are separate so that several polls can be in flight at once.
+struct Poll {
     uint32_t f[MAX_LOOP_ROWS / 64];
-    LW_UNROLL for (int k = 0; k < MAX_LOOP_ROWS / 64; ++k) f[k] = get(&flags[min(64 * k + lane, G - 1)]);
+};
+
+__device__ inline void poll_issue(const uint32_t *flags, int G, Poll &p) {
+    const int lane = threadIdx.x;
+    LW_UNROLL for (int k = 0; k < MAX_LOOP_ROWS / 64; ++k) p.f[k] = get(&flags[min(64 * k + lane, G - 1)]);
+}
+
+// Bit b set when every row of batch b carries the tag (rows past G count as set).
+__device__ inline uint32_t poll_eval(const Poll &p, int G, uint32_t tag) {
+    const int lane = threadIdx.x;
     uint32_t ready = 0u;
     LW_UNROLL for (int k = 0; k < MAX_LOOP_ROWS / 64; ++k) {
-        const unsigned long long m = __ballot(f[k] >= tag || 64 * k + lane >= G);
+        const unsigned long long m = __ballot(p.f[k] >= tag || 64 * k + lane >= G);
         LW_UNROLL for (int q = 0; q < 64 / RED_BATCH; ++q)
             if (((m >> (q * RED_BATCH)) & ((1ull << RED_BATCH) - 1ull)) == ((1ull << RED_BATCH) - 1ull))
                 ready |= 1u << (k * (64 / RED_BATCH) + q);
     }
     return ready;
+}
+
+__device__ inline uint32_t poll_flags(const uint32_t *flags, int G, uint32_t tag) {
+    Poll p;
+    poll_issue(flags, G, p);
+    return poll_eval(p, G, tag);
 }
 
 // The persistent loop's leader: sums each batch whose rows all carry this pass's tag into bs (LDS,
@@ -452,8 +466,19 @@ __device__ inline bool leader_reduce(const double *partials, const uint32_t *fla
     uint32_t ready = poll_flags(flags, G, tag) & pend;
     while (pend) {
         if (ready == 0u) {
-            if (++spins > SPIN_LIMIT) return false;
-            ready = poll_flags(flags, G, tag) & pend;
+            // nothing new: poll with two polls in flight, so a flag is seen about half a round trip after it lands
+            Poll pa, pb;
+            poll_issue(flags, G, pa);
+            while (true) {
+                poll_issue(flags, G, pb);
+                ready = poll_eval(pa, G, tag) & pend;
+                if (ready != 0u) break;
+                if (++spins > SPIN_LIMIT) return false;
+                poll_issue(flags, G, pa);
+                ready = poll_eval(pb, G, tag) & pend;
+                if (ready != 0u) break;
+                if (++spins > SPIN_LIMIT) return false;
+            }
             continue;
         }
         if (stamp && ready == pend) t_in = wall_clock64();
