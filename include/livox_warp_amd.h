@@ -62,6 +62,9 @@ typedef struct lw_reg_result {
     int32_t weak;
     double rms0; /* the residual of the first pass, at the prediction (no counterpart in the reference: the
                     learned prior's fallback test) */
+    int32_t m;   /* the frame points registered: m as passed, or, for m < 0, the open frame's count read back
+                    with the result (no counterpart in the reference: it saves the host a read of f_count) */
+    int32_t pad;
 } lw_reg_result;
 
 /* Library version (major * 10000 + minor * 100 + patch). */
@@ -94,7 +97,8 @@ int lw_frame_append(lw_ctx *ctx, const float *sx, const uint32_t *sa, const floa
 int lw_frame_count(lw_ctx *ctx, int32_t *out);
 
 /* k_deskew (odom.py:163-178, gpu.py:268-284): f_xyz[i] = Rot(theta s) f_raw[i] + rho s,
- * s = clamp((f_t[i] - t_mid) * inv_dt, -1, 1); xi = (rho, theta). */
+ * s = clamp((f_t[i] - t_mid) * inv_dt, -1, 1); xi = (rho, theta). m < 0: the open frame's point count as it
+ * stands on the device (f_count capped at frame_cap), with no read back to the host. */
 int lw_deskew(lw_ctx *ctx, int32_t m, float t_mid, float inv_dt, const float *xi);
 
 /* Odometry._downsample (odom.py:731-737 with k_ds_insert, odom.py:126-160): keep the first point (in
@@ -108,8 +112,9 @@ int lw_thin_read(lw_ctx *ctx, float *out, int32_t n);
 int lw_thin_load(lw_ctx *ctx, const float *xyz, int32_t n);
 
 /* k_reg_insert (odom.py:186-230), Odometry._insert (odom.py:719-727): add n points posed by T to the
- * map. xyz/attr are device pointers, or NULL for the bound frame buffers f_xyz / f_attr. A voxel stops
- * accumulating at 4096 points; exactly its first points in index order contribute. */
+ * map. xyz/attr are device pointers, or NULL for the bound frame buffers f_xyz / f_attr; with the frame
+ * buffers, n < 0 takes the open frame's count on the device. A voxel stops accumulating at 4096 points;
+ * exactly its first points in index order contribute. */
 int lw_map_insert(lw_ctx *ctx, const float *xyz, const uint32_t *attr, int64_t n, const float *T, float inv_v,
                   float voxel, float max_range, int32_t ret_mask);
 /* reg_stats (odom.py:474, read at odom.py:698): out[0] voxels claimed, out[1] points without a slot. */
@@ -129,7 +134,8 @@ int lw_icp_accum(lw_ctx *ctx, int32_t n, const float *T, float inv_v, float voxe
 
 /* Odometry._register (odom.py:752-803): thin the m deskewed frame points, then run the Gauss-Newton
  * loop on the device -- accumulation, float64 host step (odom.py:387-455) and convergence test -- with
- * one readback at the end. T_pred: the prediction (float64 4x4). */
+ * one readback at the end. T_pred: the prediction (float64 4x4). m < 0: the open frame's count on the
+ * device, returned in res->m. */
 int lw_register(lw_ctx *ctx, int32_t m, const lw_reg_params *p, const double *T_pred, lw_reg_result *res);
 /* The same loop over the already-thinned points (lw_thin / lw_thin_load), n of them. */
 int lw_register_thinned(lw_ctx *ctx, int32_t n, const lw_reg_params *p, const double *T_pred, lw_reg_result *res);

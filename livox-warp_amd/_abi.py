@@ -35,7 +35,7 @@ class RegParams(ctypes.Structure):
 
 class RegResult(ctypes.Structure):
     _fields_ = [("T", _f64 * 16), ("rms", _f64), ("cond", _f64), ("ok", _i32), ("iters", _i32), ("corr", _i32),
-                ("n_ds", _i32), ("degen", _i32), ("weak", _i32), ("rms0", _f64)]
+                ("n_ds", _i32), ("degen", _i32), ("weak", _i32), ("rms0", _f64), ("m", _i32), ("pad", _i32)]
 
 
 class LocParams(ctypes.Structure):

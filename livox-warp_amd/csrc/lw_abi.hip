@@ -229,7 +229,7 @@ int lw_frame_count(lw_ctx *c, int32_t *out) {
 }
 
 int lw_deskew(lw_ctx *c, int32_t m, float t_mid, float inv_dt, const float *xi) {
-    if (!c || !xi || m < 0 || m > c->frame_cap) return LW_EINVAL;
+    if (!c || !xi || m > c->frame_cap) return LW_EINVAL;
     if (!c->f_raw) return LW_ENOTBOUND;
     LW_CK(lw::launch_deskew(c, m, t_mid, inv_dt, xi));
     return 0;
@@ -258,20 +258,25 @@ int lw_thin_load(lw_ctx *c, const float *xyz, int32_t n) {
 
 int lw_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, int64_t n, const float *T, float inv_v,
                   float voxel, float max_range, int32_t ret_mask) {
-    if (!c || !T || n < 0) return LW_EINVAL;
+    if (!c || !T) return LW_EINVAL;
+    const int32_t *n_dev = nullptr;
     if (!xyz && !attr) {
         if (!c->f_xyz) return LW_ENOTBOUND;
         if (n > c->frame_cap) return LW_EINVAL;
+        if (n < 0) {  // the open frame's count, on the device
+            n = c->frame_cap;
+            n_dev = &c->ctr[6];
+        }
         xyz = c->f_xyz;
         attr = c->f_attr;
-    } else if (!xyz || !attr) {
+    } else if (!xyz || !attr || n < 0) {
         return LW_EINVAL;
     }
     lw::Mat4f M;
     std::memcpy(M.m, T, sizeof(M.m));
     for (int64_t a = 0; a < n; a += c->frame_cap) {  // in index order, so the cap keeps the first points
         int32_t m = (int32_t)((n - a) < c->frame_cap ? (n - a) : c->frame_cap);
-        LW_CK(lw::launch_map_insert(c, xyz + 3 * a, attr + a, m, M, inv_v, voxel, max_range, ret_mask));
+        LW_CK(lw::launch_map_insert(c, xyz + 3 * a, attr + a, m, n_dev, M, inv_v, voxel, max_range, ret_mask));
     }
     return 0;
 }
@@ -354,12 +359,14 @@ int lw_icp_accum(lw_ctx *c, int32_t n, const float *T, float inv_v, float voxel,
 namespace {
 
 // Odometry._register from the thinned points on (odom.py:756-803). The thinned count is on the device
-// (n_dev) and n_bound bounds it on the host; n_known >= 0 when the host knows it exactly.
-int run_registration(lw_ctx *c, const int32_t *n_dev, int32_t n_bound, int32_t n_known, const lw_reg_params *p,
-                     const double *T_pred, lw_reg_result *res) {
+// (n_dev) and n_bound bounds it on the host; n_known >= 0 when the host knows it exactly. m_frame: the frame
+// points for res->m, or < 0 to read the open frame's count back with the result.
+int run_registration(lw_ctx *c, const int32_t *n_dev, int32_t n_bound, int32_t n_known, int32_t m_frame,
+                     const lw_reg_params *p, const double *T_pred, lw_reg_result *res) {
     std::memset(res, 0, sizeof(*res));
     std::memcpy(res->T, T_pred, 16 * sizeof(double));
     res->cond = 1.0;
+    res->m = m_frame;
     if (n_known >= 0 && n_known < p->min_corr) {  // odom.py:756-757
         res->n_ds = n_known;
         return 0;
@@ -424,11 +431,19 @@ int run_registration(lw_ctx *c, const int32_t *n_dev, int32_t n_bound, int32_t n
             LW_CK(lw::launch_icp_pass(c, q, p->acc_mode));
         }
     }
+    constexpr size_t M_AT = (sizeof(lw::RegState) + 63) & ~(size_t)63;  // the frame count beside the state
     LW_CK(hipMemcpyAsync(pin_down(c), c->reg, sizeof(lw::RegState), hipMemcpyDeviceToHost, c->stream));
+    if (m_frame < 0)
+        LW_CK(hipMemcpyAsync(pin_down(c) + M_AT, &c->ctr[6], sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     LW_CK(hipStreamSynchronize(c->stream));
     LW_CK(prof_harvest(c));
     lw::RegState st;
     std::memcpy(&st, pin_down(c), sizeof(st));
+    if (m_frame < 0) {
+        int32_t mf;
+        std::memcpy(&mf, pin_down(c) + M_AT, sizeof(mf));
+        res->m = mf < c->frame_cap ? mf : (int32_t)c->frame_cap;
+    }
     if (st.status < 0) return LW_ELOOP;
     if (n < 0) n = st.n;
     res->n_ds = n;
@@ -458,7 +473,7 @@ extern "C" {
 int lw_register_thinned(lw_ctx *c, int32_t n, const lw_reg_params *p, const double *T_pred, lw_reg_result *res) {
     if (!c || bad_params(p) || !T_pred || !res || n < 0 || n > c->frame_cap) return LW_EINVAL;
     LW_CK(lw::launch_fill_i32(c, &c->ctr[8], 1, n));
-    return run_registration(c, &c->ctr[8], n, n, p, T_pred, res);
+    return run_registration(c, &c->ctr[8], n, n, n, p, T_pred, res);
 }
 
 int lw_loc_batch(lw_ctx *c, const float *scans, const int32_t *scan_off, const int32_t *order, int32_t n_queries,
@@ -548,10 +563,13 @@ int lw_set_persistent(lw_ctx *c, int32_t on) {
 }
 
 int lw_register(lw_ctx *c, int32_t m, const lw_reg_params *p, const double *T_pred, lw_reg_result *res) {
-    if (!c || bad_params(p) || !T_pred || !res || m < 0 || m > c->frame_cap) return LW_EINVAL;
+    if (!c || bad_params(p) || !T_pred || !res || m > c->frame_cap) return LW_EINVAL;
     if (!c->f_raw) return LW_ENOTBOUND;
-    LW_CK(lw::launch_thin(c, m, (float)(2.0 / (double)p->reg_voxel), p->min_range, p->max_range, p->ret_mask));
-    return run_registration(c, &c->ctr[5], m, -1, p, T_pred, res);  // the thinned count stays on the device
+    // m < 0: the open frame's count on the device (thinning reads it there; the result carries it back)
+    LW_CK(lw::launch_thin(c, m < 0 ? -1 : m, (float)(2.0 / (double)p->reg_voxel), p->min_range, p->max_range,
+                          p->ret_mask));
+    const int32_t bound = m < 0 ? (int32_t)c->frame_cap : m;
+    return run_registration(c, &c->ctr[5], bound, -1, m < 0 ? -1 : m, p, T_pred, res);  // thinned count on the device
 }
 
 int lw_prof_enable(lw_ctx *c, int32_t on) {

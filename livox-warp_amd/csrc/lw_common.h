@@ -75,6 +75,13 @@ __host__ __device__ inline int32_t slot_hash(int64_t k) {
 
 __device__ inline int32_t ifloor(float x) { return (int32_t)floorf(x); }
 
+// A launch's point count: n, or, when n_dev is given, the count on the device capped at n (the grid covers n).
+__device__ inline int32_t count_of(int32_t n, const int32_t *n_dev) {
+    if (!n_dev) return n;
+    const int32_t d = *n_dev;
+    return d < n ? d : n;
+}
+
 __device__ inline int tag_of(uint32_t a) { return (int)((a >> 8) & 255u); }
 __device__ inline bool ret_ok(uint32_t a, int ret_mask) {
     int r = (int)((a >> 16) & 255u);

@@ -95,11 +95,11 @@ __global__ __launch_bounds__(BLOCK) void k_append_write(const float *sx, const u
 }
 
 // gpu.py:268-284: rotate by the rotation vector theta s (unit quaternion), then add rho s.
-__global__ __launch_bounds__(BLOCK) void k_deskew(const float *raw, const float *t, int32_t m, float t_mid,
-                                                  float inv_dt, float r0, float r1, float r2, float h0, float h1,
-                                                  float h2, float *out) {
+__global__ __launch_bounds__(BLOCK) void k_deskew(const float *raw, const float *t, int32_t m, const int32_t *m_dev,
+                                                  float t_mid, float inv_dt, float r0, float r1, float r2, float h0,
+                                                  float h1, float h2, float *out) {
     int32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= m) return;
+    if (i >= count_of(m, m_dev)) return;
     float s = (t[i] - t_mid) * inv_dt;
     s = fminf(fmaxf(s, -1.0f), 1.0f);
     float px = raw[3 * (int64_t)i], py = raw[3 * (int64_t)i + 1], pz = raw[3 * (int64_t)i + 2];
@@ -125,12 +125,12 @@ __global__ __launch_bounds__(BLOCK) void k_deskew(const float *raw, const float 
 }
 
 // Thinning 1: filters, key at the thinning voxel (sensor frame), claim, lowest index wins.
-__global__ __launch_bounds__(BLOCK) void k_thin_claim(const float *xyz, const uint32_t *attr, int32_t m, float inv_v,
-                                                      float min_range, float max_range, int32_t ret_mask,
-                                                      int32_t ds_mask, int64_t *ds_keys, int32_t *ds_idx,
-                                                      int32_t *pt_slot) {
+__global__ __launch_bounds__(BLOCK) void k_thin_claim(const float *xyz, const uint32_t *attr, int32_t m,
+                                                      const int32_t *m_dev, float inv_v, float min_range,
+                                                      float max_range, int32_t ret_mask, int32_t ds_mask,
+                                                      int64_t *ds_keys, int32_t *ds_idx, int32_t *pt_slot) {
     int32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= m) return;
+    if (i >= count_of(m, m_dev)) return;
     float px = xyz[3 * (int64_t)i], py = xyz[3 * (int64_t)i + 1], pz = xyz[3 * (int64_t)i + 2];
     float r = length3(px, py, pz);
     uint32_t a = attr[i];
@@ -152,20 +152,20 @@ __global__ __launch_bounds__(BLOCK) void k_thin_claim(const float *xyz, const ui
 }
 
 __global__ __launch_bounds__(BLOCK) void k_thin_count(const int32_t *pt_slot, const int32_t *ds_idx, int32_t m,
-                                                      int32_t *blk_cnt) {
+                                                      const int32_t *m_dev, int32_t *blk_cnt) {
     int32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    int32_t s = i < m ? pt_slot[i] : -1;
+    int32_t s = i < count_of(m, m_dev) ? pt_slot[i] : -1;
     bool p = s >= 0 && ds_idx[s] == i;
     int32_t c = __syncthreads_count(p);
     if (threadIdx.x == 0) blk_cnt[blockIdx.x] = c;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_thin_write(const float *xyz, const int32_t *pt_slot, int32_t *ds_idx,
-                                                      int64_t *ds_keys, int32_t m, const int32_t *blk_off,
-                                                      float *ds_xyz) {
+                                                      int64_t *ds_keys, int32_t m, const int32_t *m_dev,
+                                                      const int32_t *blk_off, float *ds_xyz) {
     __shared__ int32_t wsum[BLOCK / 64];
     int32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    int32_t s = i < m ? pt_slot[i] : -1;
+    int32_t s = i < count_of(m, m_dev) ? pt_slot[i] : -1;
     bool p = s >= 0 && ds_idx[s] == i;  // only the winner of a slot resets it, after reading it
     int32_t tot;
     int32_t j = blk_off[blockIdx.x] + block_prefix(p, wsum, &tot);
@@ -193,26 +193,33 @@ hipError_t launch_frame_append(lw_ctx *c, const float *sx, const uint32_t *sa, c
     return hipGetLastError();
 }
 
+// m < 0: the open frame's count on the device (its appended points, at most frame_cap); the grids then cover
+// frame_cap points and the blocks past the count do nothing.
 hipError_t launch_deskew(lw_ctx *c, int32_t m, float t_mid, float inv_dt, const float *xi) {
-    if (m <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_deskew, dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, c->stream, c->f_raw, c->f_t, m,
+    const int32_t *m_dev = m < 0 ? &c->ctr[6] : nullptr;
+    if (m < 0) m = (int32_t)c->frame_cap;
+    if (m == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_deskew, dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, c->stream, c->f_raw, c->f_t, m, m_dev,
                        t_mid, inv_dt, xi[0], xi[1], xi[2], xi[3], xi[4], xi[5], c->f_xyz);
     return hipGetLastError();
 }
 
 hipError_t launch_thin(lw_ctx *c, int32_t m, float inv_v, float min_range, float max_range, int32_t ret_mask) {
-    if (m <= 0) {
+    const int32_t *m_dev = m < 0 ? &c->ctr[6] : nullptr;
+    if (m < 0) m = (int32_t)c->frame_cap;
+    if (m == 0) {
         hipLaunchKernelGGL(k_set_i32_one, dim3(1), dim3(1), 0, c->stream, &c->ctr[5], 0);
         return hipGetLastError();
     }
-    int32_t nb = (m + BLOCK - 1) / BLOCK;
-    hipLaunchKernelGGL(k_thin_claim, dim3(nb), dim3(BLOCK), 0, c->stream, c->f_xyz, c->f_attr, m, inv_v, min_range,
-                       max_range, ret_mask, (int32_t)(c->ds_cap - 1), c->ds_keys, c->ds_idx, c->pt_slot);
-    hipLaunchKernelGGL(k_thin_count, dim3(nb), dim3(BLOCK), 0, c->stream, c->pt_slot, c->ds_idx, m, c->blk_cnt);
+    int32_t nb = (m + BLOCK - 1) / BLOCK;  // blocks past the device count count no points
+    hipLaunchKernelGGL(k_thin_claim, dim3(nb), dim3(BLOCK), 0, c->stream, c->f_xyz, c->f_attr, m, m_dev, inv_v,
+                       min_range, max_range, ret_mask, (int32_t)(c->ds_cap - 1), c->ds_keys, c->ds_idx, c->pt_slot);
+    hipLaunchKernelGGL(k_thin_count, dim3(nb), dim3(BLOCK), 0, c->stream, c->pt_slot, c->ds_idx, m, m_dev,
+                       c->blk_cnt);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_THREADS), 0, c->stream, c->blk_cnt, nb, c->blk_off,
                        &c->ctr[5], 0);
     hipLaunchKernelGGL(k_thin_write, dim3(nb), dim3(BLOCK), 0, c->stream, c->f_xyz, c->pt_slot, c->ds_idx,
-                       c->ds_keys, m, c->blk_off, c->ds_xyz);
+                       c->ds_keys, m, m_dev, c->blk_off, c->ds_xyz);
     return hipGetLastError();
 }
 

@@ -130,13 +130,15 @@ struct Mat4f {
 };
 
 hipError_t launch_map_reset(lw_ctx *c);
-hipError_t launch_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, int32_t n, const Mat4f &T, float inv_v,
-                             float voxel, float max_range, int32_t ret_mask);
+// n_dev: when given, the points to insert are its count capped at n (n sizes the grids).
+hipError_t launch_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, int32_t n, const int32_t *n_dev,
+                             const Mat4f &T, float inv_v, float voxel, float max_range, int32_t ret_mask);
 hipError_t launch_refresh(lw_ctx *c, bool all, float voxel, float noise2, float tight2, float alpha);
 hipError_t launch_map_find(lw_ctx *c, const int64_t *keys_dev, int32_t n, int32_t *out_dev);
 
 hipError_t launch_frame_append(lw_ctx *c, const float *sx, const uint32_t *sa, const float *st, int32_t n,
                                float inv_dt, int32_t k_min, int32_t k_open);
+// m < 0: the open frame's point count on the device (ctr[6], capped at frame_cap)
 hipError_t launch_deskew(lw_ctx *c, int32_t m, float t_mid, float inv_dt, const float *xi);
 hipError_t launch_thin(lw_ctx *c, int32_t m, float inv_v, float min_range, float max_range, int32_t ret_mask);
 hipError_t launch_fill_i32(lw_ctx *c, int32_t *p, int64_t n, int32_t v);

@@ -61,14 +61,14 @@ __device__ inline bool insert_point(const float *xyz, const uint32_t *attr, int3
 }
 
 // 1: claim. A plain read first: a slot's key only ever goes from -1 to its final value.
-__global__ __launch_bounds__(BLOCK) void k_ins_claim(const float *xyz, const uint32_t *attr, int32_t n, Mat4f T,
-                                                     float inv_v, float max_range, int32_t ret_mask, int32_t mask,
-                                                     Slot *slots, uint32_t *occ, int64_t *vox_key, int32_t *vox_slot,
-                                                     VoxMom *vox_mom, int32_t *ctr, int64_t vox_cap,
-                                                     int32_t *pt_slot) {
+__global__ __launch_bounds__(BLOCK) void k_ins_claim(const float *xyz, const uint32_t *attr, int32_t n,
+                                                     const int32_t *n_dev, Mat4f T, float inv_v, float max_range,
+                                                     int32_t ret_mask, int32_t mask, Slot *slots, uint32_t *occ,
+                                                     int64_t *vox_key, int32_t *vox_slot, VoxMom *vox_mom,
+                                                     int32_t *ctr, int64_t vox_cap, int32_t *pt_slot) {
     int32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i == 0) ctr[4] = 0;  // touched count of this insert (read by the next kernel only)
-    if (i >= n) return;
+    if (i >= count_of(n, n_dev)) return;
     int32_t ix, iy, iz;
     float wx, wy, wz;
     if (!insert_point(xyz, attr, i, T, inv_v, max_range, ret_mask, ix, iy, iz, wx, wy, wz)) {
@@ -120,11 +120,11 @@ __global__ __launch_bounds__(BLOCK) void k_ins_claim(const float *xyz, const uin
 }
 
 // 2: count the incoming points per voxel; the first one lists the voxel as touched.
-__global__ __launch_bounds__(BLOCK) void k_ins_count(const int32_t *pt_slot, int32_t n, const Slot *slots,
-                                                     int32_t *pt_vid, int32_t *vox_nin, int32_t *touched,
-                                                     int32_t *ctr) {
+__global__ __launch_bounds__(BLOCK) void k_ins_count(const int32_t *pt_slot, int32_t n, const int32_t *n_dev,
+                                                     const Slot *slots, int32_t *pt_vid, int32_t *vox_nin,
+                                                     int32_t *touched, int32_t *ctr) {
     int32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
+    if (i >= count_of(n, n_dev)) return;
     int32_t s = pt_slot[i];
     int32_t vid = s >= 0 ? slots[s].vid : -1;
     pt_vid[i] = vid;
@@ -136,9 +136,10 @@ __global__ __launch_bounds__(BLOCK) void k_ins_count(const int32_t *pt_slot, int
 __global__ __launch_bounds__(BLOCK) void k_ins_quota(const int32_t *touched, const int32_t *ctr,
                                                      const int32_t *vox_nin, const VoxMom *vox_mom,
                                                      int32_t *vox_quota, const int32_t *pt_vid, int32_t n,
-                                                     uint8_t *pt_take) {
+                                                     const int32_t *n_dev, uint8_t *pt_take) {
     __shared__ int32_t wsum[BLOCK / 64];
     __shared__ int32_t s_vid, s_quota, s_nin;
+    n = count_of(n, n_dev);
     int32_t nt = ctr[4];
     for (int32_t j = blockIdx.x; j < nt; j += gridDim.x) {
         if (threadIdx.x == 0) {
@@ -179,12 +180,13 @@ __global__ __launch_bounds__(BLOCK) void k_ins_quota(const int32_t *touched, con
 }
 
 // 4: accumulate (odom.py:217, 225-228): r relative to the voxel centre, float32 atomics.
-__global__ __launch_bounds__(BLOCK) void k_ins_accum(const float *xyz, const uint32_t *attr, int32_t n, Mat4f T,
-                                                     float inv_v, float voxel, float max_range, int32_t ret_mask,
-                                                     const int32_t *pt_vid, const int32_t *vox_quota,
-                                                     const uint8_t *pt_take, VoxMom *vox_mom) {
+__global__ __launch_bounds__(BLOCK) void k_ins_accum(const float *xyz, const uint32_t *attr, int32_t n,
+                                                     const int32_t *n_dev, Mat4f T, float inv_v, float voxel,
+                                                     float max_range, int32_t ret_mask, const int32_t *pt_vid,
+                                                     const int32_t *vox_quota, const uint8_t *pt_take,
+                                                     VoxMom *vox_mom) {
     int32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
+    if (i >= count_of(n, n_dev)) return;
     int32_t vid = pt_vid[i];
     if (vid < 0) return;
     int32_t q = vox_quota[vid];
@@ -298,18 +300,19 @@ hipError_t launch_map_reset(lw_ctx *c) {
     return hipGetLastError();
 }
 
-hipError_t launch_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, int32_t n, const Mat4f &T, float inv_v,
-                             float voxel, float max_range, int32_t ret_mask) {
+hipError_t launch_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, int32_t n, const int32_t *n_dev,
+                             const Mat4f &T, float inv_v, float voxel, float max_range, int32_t ret_mask) {
     if (n <= 0) return hipSuccess;
     int g = grid_for(n);
-    hipLaunchKernelGGL(k_ins_claim, dim3(g), dim3(BLOCK), 0, c->stream, xyz, attr, n, T, inv_v, max_range, ret_mask,
-                       c->mask, c->slots, c->occ, c->vox_key, c->vox_slot, c->vox_mom, c->ctr, c->vox_cap, c->pt_slot);
-    hipLaunchKernelGGL(k_ins_count, dim3(g), dim3(BLOCK), 0, c->stream, c->pt_slot, n, c->slots, c->pt_vid,
+    hipLaunchKernelGGL(k_ins_claim, dim3(g), dim3(BLOCK), 0, c->stream, xyz, attr, n, n_dev, T, inv_v, max_range,
+                       ret_mask, c->mask, c->slots, c->occ, c->vox_key, c->vox_slot, c->vox_mom, c->ctr, c->vox_cap,
+                       c->pt_slot);
+    hipLaunchKernelGGL(k_ins_count, dim3(g), dim3(BLOCK), 0, c->stream, c->pt_slot, n, n_dev, c->slots, c->pt_vid,
                        c->vox_nin, c->touched, c->ctr);
     hipLaunchKernelGGL(k_ins_quota, dim3(1024), dim3(BLOCK), 0, c->stream, c->touched, c->ctr, c->vox_nin,
-                       c->vox_mom, c->vox_quota, c->pt_vid, n, c->pt_take);
-    hipLaunchKernelGGL(k_ins_accum, dim3(g), dim3(BLOCK), 0, c->stream, xyz, attr, n, T, inv_v, voxel, max_range,
-                       ret_mask, c->pt_vid, c->vox_quota, c->pt_take, c->vox_mom);
+                       c->vox_mom, c->vox_quota, c->pt_vid, n, n_dev, c->pt_take);
+    hipLaunchKernelGGL(k_ins_accum, dim3(g), dim3(BLOCK), 0, c->stream, xyz, attr, n, n_dev, T, inv_v, voxel,
+                       max_range, ret_mask, c->pt_vid, c->vox_quota, c->pt_take, c->vox_mom);
     hipLaunchKernelGGL(k_ins_finish, dim3(256), dim3(BLOCK), 0, c->stream, c->touched, c->ctr, c->vox_nin,
                        c->vox_quota, c->vox_dirty, c->dirty_list);
     return hipGetLastError();

@@ -301,10 +301,13 @@ class Odometry:
 
     def _finalize(self, k: int, on_frame):
         t0 = time.perf_counter()
-        m = min(self._frame_count(), self.frame_cap)
         warm = self.map_frames < self.warmup
         t_mid = (k + 0.5) * self.frame_dt
-        T, xi, ok = self._estimate(m, t_mid, warm)
+        if warm or self.prior is not None:
+            m = min(self._frame_count(), self.frame_cap)
+            T, xi, ok = self._estimate(m, t_mid, warm)
+        else:  # the frame's point count comes back with the registration: no read of it before
+            T, xi, ok, m = self._estimate_counted(t_mid)
         if ok and (warm or self._is_keyframe(T)):
             self._insert(m, T)
         if self.T_prev is not None and not warm:
@@ -351,6 +354,29 @@ class Odometry:
                 self._deskew(m, t_mid, xi)
                 T, ok = self._register(m, T)
         return T, xi, ok
+
+    def _estimate_counted(self, t_mid: float):
+        """_estimate after the warm-up without the learned prior, with the frame's point count left on the
+        device: deskew and registration read it there, and it returns with the registration's result. Same
+        decisions in the same order: when the count is below min_points the registration's result is set aside
+        and the frame keeps the prediction, as if it had not run."""
+        xi = self.xi if self.deskew else np.zeros(6, np.float32)
+        T = self.predict()
+        self._deskew(-1, t_mid, xi)
+        r = self._register_run(-1, T)
+        m = int(r.m)
+        if m < self.min_points:
+            self.stats["skipped"] += 1
+            return T, xi, False, m
+        T, ok = self._register_take(r, T)
+        if ok and self.deskew and self.redeskew and self.T_prev is not None:
+            xi_new = _body_twist(np.linalg.inv(self.T_prev) @ T)
+            if (np.linalg.norm(xi_new[3:] - xi[3:]) > REDESKEW_ROT
+                    or np.linalg.norm(xi_new[:3] - xi[:3]) > REDESKEW_TRANS):
+                xi = xi_new
+                self._deskew(m, t_mid, xi)
+                T, ok = self._register(m, T)
+        return T, xi, ok, m
 
     def _update_velocity(self, d: np.ndarray):
         if np.linalg.norm(d[:3, 3]) < MAX_FRAME_JUMP:
@@ -424,10 +450,18 @@ class Odometry:
 
     def _register(self, m: int, T_pred: np.ndarray):
         """Scan-to-map ICP of the deskewed frame, starting from T_pred. Returns (T, ok)."""
+        return self._register_take(self._register_run(m, T_pred), T_pred)
+
+    def _register_run(self, m: int, T_pred: np.ndarray):
+        """lw_register on the open frame (m < 0: its count on the device); the result, nothing recorded."""
         Tp = np.ascontiguousarray(T_pred, dtype=np.float64)
         r = self._result
         _abi.check(self._lib.lw_register(self._ctx, m, _abi.ctypes.byref(self.reg_params()), _abi.ptr(Tp, _abi._f64p),
                                          _abi.ctypes.byref(r)), "lw_register")
+        return r
+
+    def _register_take(self, r, T_pred: np.ndarray):
+        """Record a registration's result in the stats; returns (T, ok)."""
         self.stats["ds"] = int(r.n_ds)
         self.totals["registrations"] += 1
         if r.n_ds < self.min_corr:

@@ -65,3 +65,54 @@ def test_register_from_frame_buffers(lw, orc, gpu_ok):
     eng.thin_load(ds)
     r2 = eng.register_thinned(len(ds), p, w.T_init)
     assert np.array_equal(np.array(r1.T), np.array(r2.T))
+
+
+def test_device_frame_count(lw, gpu_ok):
+    """m < 0 (the open frame's count left on the device) deskews and registers exactly as the count read back
+    and passed in, the registration returns the count, and an insert of the frame with n < 0 adds the same
+    voxels and counts as with the count passed in."""
+    import torch
+
+    w = lw.synth.single_scan(seed=16)
+    voxel = 0.25
+    dev = torch.device("cuda", 0)
+    sx = torch.from_numpy(np.asarray(w.scan_xyz, np.float32).reshape(-1, 3)).to(dev)
+    sa = torch.from_numpy(np.asarray(w.scan_attr).astype(np.uint32).view(np.int32)).to(dev)
+    st = torch.from_numpy(np.asarray(w.scan_t, np.float32)).to(dev)
+    xi = np.array([0.02, -0.01, 0.0, 0.001, 0.0, 0.004], np.float32)
+
+    def open_frame(eng):
+        eng.frame_reset()
+        eng.frame_append(sx, sa, st, 10.0, -1, 0)  # every point in frame 0 ([0, 0.1) s)
+        m = eng.frame_count()
+        assert m == len(w.scan_t)
+        return m
+
+    # one map (its float moments depend on the atomics' order), both counts
+    eng = lw.Engine(1 << 21, 1 << 15)
+    eng.insert(w.map_xyz, None, np.eye(4), voxel, max_range=1e30)
+    m = open_frame(eng)
+    res = {}
+    for mm in (m, -1):
+        eng.deskew(mm, 0.05, 10.0, xi)
+        r = eng.register(mm, eng.params(reg_voxel=voxel, max_iter=20), w.T_init)
+        assert r.m == m
+        res[mm] = (np.array(r.T).reshape(4, 4), r.iters, r.corr, r.n_ds, r.ok)
+    assert np.array_equal(res[m][0], res[-1][0])
+    assert res[m][1:] == res[-1][1:]
+    T32 = np.ascontiguousarray(res[m][0], dtype=np.float32)
+    eng.close()
+
+    maps = {}
+    for mm in (m, -1):
+        eng = lw.Engine(1 << 21, 1 << 15)
+        open_frame(eng)
+        eng.deskew(mm, 0.05, 10.0, xi)
+        lw._abi.check(eng.lib.lw_map_insert(eng.ctx, None, None, mm, lw._abi.ptr(T32, lw._abi._f32p), 1.0 / voxel,
+                                            voxel, 60.0, 15), "lw_map_insert")
+        maps[mm] = (eng.stats(),) + eng.read_map()
+        eng.close()
+    a, b = maps[m], maps[-1]
+    assert a[0] == b[0]
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[4], b[4])  # keys and counts
+    np.testing.assert_allclose(a[2], b[2], rtol=1e-5, atol=1e-6)  # float atomics: order-dependent rounding
