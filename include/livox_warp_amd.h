@@ -28,6 +28,7 @@ extern "C" {
 #define LW_EINVAL (-1000)
 #define LW_ENOTBOUND (-1001)
 #define LW_ELOOP (-1002) /* the persistent registration loop stopped waiting (a bounded wait ran out) */
+#define LW_ENOLOOP (-1003) /* no cooperative launch on this device: drive the frame step by step instead */
 #define LW_ACC_STRIDE 64
 
 typedef struct lw_ctx lw_ctx;
@@ -131,6 +132,35 @@ int lw_map_find(lw_ctx *ctx, const int64_t *keys_host, int32_t n, int32_t *slots
 int lw_icp_accum(lw_ctx *ctx, int32_t n, const float *T, float inv_v, float voxel, int32_t reach, float max_d2,
                  float noise2, float tight2, float alpha, float kappa, int32_t acc_mode, double *out,
                  int32_t *best_slot);
+
+/* One frame of Odometry._finalize after the warm-up (odom.py:644-692, 712-727), queued as one chain with one
+ * readback: deskew the open frame (its count on the device) with xi, register it from T_pred; when that is
+ * trusted, the re-deskew test against the previous pose (xi_new, the body twist of T_prev^-1 T, in float32,
+ * against redeskew_rot / redeskew_trans) and, if it fires, deskew with xi_new and register again from the first
+ * pose; then the keyframe test against the last keyframe (kf_dist metres, kf_angle degrees; every trusted frame
+ * when has_key is 0) and, if it fires, the insert of the frame at its pose. A frame under min_points points
+ * keeps T_pred untrusted (its registration runs and is discarded). The host supplies the inverses of the
+ * previous and keyframe poses as it computes them. No counterpart in the reference (its host decides between
+ * device steps). */
+typedef struct {
+    double T_pred[16];
+    double T_prev_inv[16]; /* valid when has_prev */
+    double T_key_inv[16];  /* valid when has_key */
+    double kf_dist, kf_angle;
+    double redeskew_rot, redeskew_trans;
+    float xi[6];           /* the constant-velocity twist (zeros when deskewing is off) */
+    float t_mid, inv_dt;
+    int32_t min_points, has_prev, has_key, redeskew; /* redeskew: deskewing and re-deskewing both on */
+} lw_frame_in;
+
+typedef struct {
+    lw_reg_result reg[2];  /* the frame's registrations in order (the first `registrations` are valid) */
+    double T[16];          /* the frame's pose */
+    float xi[6];           /* the twist the frame ended deskewed with */
+    int32_t m, skipped, ok, registrations, keyframe, map_voxels, map_overflow, pad;
+} lw_frame_out;
+
+int lw_odom_frame(lw_ctx *ctx, const lw_reg_params *p, const lw_frame_in *in, lw_frame_out *out);
 
 /* Odometry._register (odom.py:752-803): thin the m deskewed frame points, then run the Gauss-Newton
  * loop on the device -- accumulation, float64 host step (odom.py:387-455) and convergence test -- with

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "liblwamd.so")
 LW_EINVAL = -1000
 LW_ENOTBOUND = -1001
 LW_ELOOP = -1002
+LW_ENOLOOP = -1003
 
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -36,6 +37,19 @@ class RegParams(ctypes.Structure):
 class RegResult(ctypes.Structure):
     _fields_ = [("T", _f64 * 16), ("rms", _f64), ("cond", _f64), ("ok", _i32), ("iters", _i32), ("corr", _i32),
                 ("n_ds", _i32), ("degen", _i32), ("weak", _i32), ("rms0", _f64), ("m", _i32), ("pad", _i32)]
+
+
+class FrameIn(ctypes.Structure):
+    _fields_ = [("T_pred", _f64 * 16), ("T_prev_inv", _f64 * 16), ("T_key_inv", _f64 * 16), ("kf_dist", _f64),
+                ("kf_angle", _f64), ("redeskew_rot", _f64), ("redeskew_trans", _f64), ("xi", _f32 * 6),
+                ("t_mid", _f32), ("inv_dt", _f32), ("min_points", _i32), ("has_prev", _i32), ("has_key", _i32),
+                ("redeskew", _i32)]
+
+
+class FrameOut(ctypes.Structure):
+    _fields_ = [("reg", RegResult * 2), ("T", _f64 * 16), ("xi", _f32 * 6), ("m", _i32), ("skipped", _i32),
+                ("ok", _i32), ("registrations", _i32), ("keyframe", _i32), ("map_voxels", _i32),
+                ("map_overflow", _i32), ("pad", _i32)]
 
 
 class LocParams(ctypes.Structure):
@@ -67,6 +81,7 @@ SIGNATURES = {
     "lw_register": (_i32, [_vp, _i32, ctypes.POINTER(RegParams), _f64p, ctypes.POINTER(RegResult)]),
     "lw_register_thinned": (_i32, [_vp, _i32, ctypes.POINTER(RegParams), _f64p, ctypes.POINTER(RegResult)]),
     "lw_set_persistent": (_i32, [_vp, _i32]),
+    "lw_odom_frame": (_i32, [_vp, ctypes.POINTER(RegParams), ctypes.POINTER(FrameIn), ctypes.POINTER(FrameOut)]),
     "lw_loc_batch": (_i32, [_vp, _vp, _vp, _vp, _i32, _vp, _i32, ctypes.POINTER(LocParams), _vp, _vp, _vp, _vp]),
     "lw_mlp_param_count": (_i32, []),
     "lw_mlp_create": (_i32, [ctypes.POINTER(_vp), _i32, _vp]),
@@ -119,6 +134,10 @@ def check(rc: int, what: str):
             raise LwError(f"{what}: invalid argument")
         if rc == LW_ENOTBOUND:
             raise LwError(f"{what}: frame buffers not bound")
+        if rc == LW_ELOOP:
+            raise LwError(f"{what}: the persistent registration loop stopped waiting")
+        if rc == LW_ENOLOOP:
+            raise LwError(f"{what}: no cooperative launch on this device")
         raise LwError(f"{what}: HIP error {-rc}")
 
 

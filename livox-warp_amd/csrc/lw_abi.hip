@@ -48,7 +48,7 @@ void free_all(lw_ctx *c) {
     void *ptrs[] = {c->slots, c->occ, c->vox_key, c->vox_mom, c->vox_slot, c->vox_prec, c->vox_nin, c->vox_quota,
                     c->vox_dirty, c->dirty_list, c->touched, c->ctr, c->pt_slot, c->pt_vid, c->pt_take,
                     c->ds_keys, c->ds_idx, c->ds_xyz, c->blk_cnt, c->blk_off, c->reg, c->partials, c->best_slot,
-                    c->dtmp, c->loop_flags};
+                    c->dtmp, c->loop_flags, c->reg1, c->fctl};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->prof_dev) (void)hipFree(c->prof_dev);
@@ -150,6 +150,8 @@ int lw_create(lw_ctx **out, int64_t reg_slots, int64_t frame_cap, void *stream) 
     if ((e = dalloc(&c->blk_cnt, c->max_blocks)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->blk_off, c->max_blocks)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->reg, 1)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->reg1, 1)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->fctl, 1)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->partials, (size_t)lw::icp_max_groups(frame_cap) * 128)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->loop_flags, lw::MAX_LOOP_ROWS)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->best_slot, frame_cap)) != hipSuccess) return fail(e);
@@ -358,26 +360,15 @@ int lw_icp_accum(lw_ctx *c, int32_t n, const float *T, float inv_v, float voxel,
 
 namespace {
 
-// Odometry._register from the thinned points on (odom.py:756-803). The thinned count is on the device
-// (n_dev) and n_bound bounds it on the host; n_known >= 0 when the host knows it exactly. m_frame: the frame
-// points for res->m, or < 0 to read the open frame's count back with the result.
-int run_registration(lw_ctx *c, const int32_t *n_dev, int32_t n_bound, int32_t n_known, int32_t m_frame,
-                     const lw_reg_params *p, const double *T_pred, lw_reg_result *res) {
-    std::memset(res, 0, sizeof(*res));
-    std::memcpy(res->T, T_pred, 16 * sizeof(double));
-    res->cond = 1.0;
-    res->m = m_frame;
-    if (n_known >= 0 && n_known < p->min_corr) {  // odom.py:756-757
-        res->n_ds = n_known;
-        return 0;
-    }
+// The Gauss-Newton loop's schedule (odom.py:761-767), computed in float64 and passed as float32 like the
+// reference's launch, and the per-voxel precision it needs refreshed.
+int reg_setup(lw_ctx *c, const lw_reg_params *p, lw::RegLoop &L) {
     const double voxel = p->reg_voxel;
     const float noise2 = (float)((double)p->noise * (double)p->noise);
     const double tf = 0.6 * voxel / std::sqrt(12.0);  // TIGHT_FRACTION (odom.py:89, 761)
     const float tight2 = (float)(tf * tf);
     LW_CK(ensure_precision(c, (float)voxel, noise2, tight2, p->alpha));
-    // the schedule (odom.py:765-767), computed in float64 and passed as float32 like the reference's launch
-    lw::RegLoop L{};
+    L = lw::RegLoop{};
     L.max_iter = p->max_iter;
     L.wide_iters = p->wide_iters;
     L.min_corr = p->min_corr;
@@ -394,6 +385,49 @@ int run_registration(lw_ctx *c, const int32_t *n_dev, int32_t n_bound, int32_t n
     L.sp.degen_thr = p->degen_thr;
     L.sp.min_corr = p->min_corr;
     L.sp.pad = 0;
+    return 0;
+}
+
+// The result of a registration from its final state (odom.py:756-757, 797-803): below min_corr thinned points
+// the prediction, untrusted; else the pose, the residual, the condition number of H (a 6x6 eigen problem on
+// the host: microseconds) and the weak flag.
+void fill_result(const lw::RegState &st, const lw_reg_params *p, lw_reg_result *res) {
+    const int32_t m = res->m;
+    std::memset(res, 0, sizeof(*res));
+    res->m = m;
+    res->n_ds = st.n;
+    res->cond = 1.0;
+    if (st.n < p->min_corr) {
+        std::memcpy(res->T, st.T_pred, 16 * sizeof(double));
+        return;
+    }
+    const double cond = st.corr >= p->min_corr ? lw::gn::cond_of(st.H) : 1.0;
+    std::memcpy(res->T, st.T, 16 * sizeof(double));
+    res->rms = st.rms;
+    res->rms0 = st.rms0;
+    res->cond = cond;
+    res->corr = st.corr;
+    res->iters = st.iters;
+    res->ok = st.corr >= p->min_corr ? 1 : 0;
+    res->degen = st.n_degen;
+    res->weak = (cond < 1e-3 || st.n_degen > 0) ? 1 : 0;  // WEAK_COND (odom.py:90, 801)
+}
+
+// Odometry._register from the thinned points on (odom.py:756-803). The thinned count is on the device
+// (n_dev) and n_bound bounds it on the host; n_known >= 0 when the host knows it exactly. m_frame: the frame
+// points for res->m, or < 0 to read the open frame's count back with the result.
+int run_registration(lw_ctx *c, const int32_t *n_dev, int32_t n_bound, int32_t n_known, int32_t m_frame,
+                     const lw_reg_params *p, const double *T_pred, lw_reg_result *res) {
+    std::memset(res, 0, sizeof(*res));
+    std::memcpy(res->T, T_pred, 16 * sizeof(double));
+    res->cond = 1.0;
+    res->m = m_frame;
+    if (n_known >= 0 && n_known < p->min_corr) {  // odom.py:756-757
+        res->n_ds = n_known;
+        return 0;
+    }
+    lw::RegLoop L;
+    if (int rc = reg_setup(c, p, L)) return rc;
     std::memcpy(pin_up(c), T_pred, 16 * sizeof(double));
     std::memcpy(pin_up(c) + 16 * sizeof(double), L.kappa, 64 * sizeof(float));
     LW_CK(hipMemcpyAsync(c->dtmp, pin_up(c), 16 * sizeof(double) + 64 * sizeof(float), hipMemcpyHostToDevice,
@@ -570,6 +604,97 @@ int lw_register(lw_ctx *c, int32_t m, const lw_reg_params *p, const double *T_pr
                           p->ret_mask));
     const int32_t bound = m < 0 ? (int32_t)c->frame_cap : m;
     return run_registration(c, &c->ctr[5], bound, -1, m < 0 ? -1 : m, p, T_pred, res);  // thinned count on the device
+}
+
+int lw_odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, lw_frame_out *out) {
+    if (!c || bad_params(p) || !in || !out) return LW_EINVAL;
+    if (!c->f_raw) return LW_ENOTBOUND;
+    if (!c->loop_ok) return LW_ENOLOOP;
+    std::memset(out, 0, sizeof(*out));
+    lw::RegLoop L;
+    if (int rc = reg_setup(c, p, L)) return rc;
+    // uploads: the prediction and the kappa schedule (as run_registration), then the frame's control inputs
+    lw::FrameCtl f{};
+    std::memcpy(f.T_pred, in->T_pred, sizeof(f.T_pred));
+    std::memcpy(f.T_prev_inv, in->T_prev_inv, sizeof(f.T_prev_inv));
+    std::memcpy(f.T_key_inv, in->T_key_inv, sizeof(f.T_key_inv));
+    f.kf_dist = in->kf_dist;
+    f.kf_angle = in->kf_angle;
+    f.redeskew_rot = in->redeskew_rot;
+    f.redeskew_trans = in->redeskew_trans;
+    std::memcpy(f.xi0, in->xi, sizeof(f.xi0));
+    f.min_points = in->min_points;
+    f.min_corr = p->min_corr;
+    f.has_prev = in->has_prev;
+    f.has_key = in->has_key;
+    f.redeskew = in->redeskew;
+    f.frame_cap = (int32_t)c->frame_cap;
+    constexpr size_t CTL_AT = 1024;
+    std::memcpy(pin_up(c), in->T_pred, 16 * sizeof(double));
+    std::memcpy(pin_up(c) + 16 * sizeof(double), L.kappa, 64 * sizeof(float));
+    std::memcpy(pin_up(c) + CTL_AT, &f, lw::FRAME_CTL_INPUT);
+    LW_CK(hipMemcpyAsync(c->dtmp, pin_up(c), 16 * sizeof(double) + 64 * sizeof(float), hipMemcpyHostToDevice,
+                         c->stream));
+    LW_CK(hipMemcpyAsync(c->fctl, pin_up(c) + CTL_AT, lw::FRAME_CTL_INPUT, hipMemcpyHostToDevice, c->stream));
+    const float inv_v_thin = (float)(2.0 / (double)p->reg_voxel);
+    const int32_t cap = (int32_t)c->frame_cap;
+    // odom.py:671-681: deskew with the model's twist, register from the prediction
+    LW_CK(lw::launch_deskew(c, -1, in->t_mid, in->inv_dt, in->xi));
+    LW_CK(lw::launch_thin(c, -1, inv_v_thin, p->min_range, p->max_range, p->ret_mask));
+    LW_CK(lw::launch_reg_begin(c, (const double *)c->dtmp));
+    hipError_t e = lw::launch_register_loop(c, &c->ctr[5], cap, L, p->acc_mode);
+    if (e != hipSuccess) {
+        c->loop_ok = false;  // cooperative launches unavailable here: the caller takes the step-by-step path
+        (void)hipGetLastError();
+        LW_CK(hipStreamSynchronize(c->stream));
+        return LW_ENOLOOP;
+    }
+    // odom.py:683-690: the re-deskew test, then (gated on it) deskew and register again from the first pose
+    LW_CK(lw::launch_frame_check(c));
+    const int32_t *redo = &c->fctl->redo;
+    LW_CK(lw::launch_deskew(c, -1, in->t_mid, in->inv_dt, nullptr, c->fctl->xi1, redo));
+    LW_CK(lw::launch_thin(c, -1, inv_v_thin, p->min_range, p->max_range, p->ret_mask, redo));
+    LW_CK(lw::launch_reg_begin(c, (const double *)c->dtmp, redo, c->fctl->T1));
+    LW_CK(lw::launch_register_loop(c, &c->ctr[5], cap, L, p->acc_mode, redo));
+    // odom.py:650-651, 712-727: the keyframe test, then (gated on it) the insert at the frame's pose
+    LW_CK(lw::launch_keyframe_check(c));
+    const lw::Mat4f unused{};
+    LW_CK(lw::launch_map_insert(c, c->f_xyz, c->f_attr, cap, &c->ctr[6], unused, (float)(1.0 / (double)p->reg_voxel),
+                                p->reg_voxel, p->max_range, p->ret_mask, &c->fctl->keyframe, c->fctl->T32));
+    // one readback: both registrations' states, the decisions and the map's counts after the insert
+    constexpr size_t S1 = (sizeof(lw::RegState) + 63) & ~(size_t)63, FC = 2 * S1,
+                     MC = FC + ((sizeof(lw::FrameCtl) + 63) & ~(size_t)63);
+    char *h = pin_down(c);
+    LW_CK(hipMemcpyAsync(h, c->reg, sizeof(lw::RegState), hipMemcpyDeviceToHost, c->stream));
+    LW_CK(hipMemcpyAsync(h + S1, c->reg1, sizeof(lw::RegState), hipMemcpyDeviceToHost, c->stream));
+    LW_CK(hipMemcpyAsync(h + FC, c->fctl, sizeof(lw::FrameCtl), hipMemcpyDeviceToHost, c->stream));
+    LW_CK(hipMemcpyAsync(h + MC, c->ctr, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    LW_CK(hipStreamSynchronize(c->stream));
+    LW_CK(prof_harvest(c));
+    lw::RegState st2, st1;
+    lw::FrameCtl fc;
+    std::memcpy(&st2, h, sizeof(st2));
+    std::memcpy(&st1, h + S1, sizeof(st1));
+    std::memcpy(&fc, h + FC, sizeof(fc));
+    if (st1.status < 0 || (fc.redo && st2.status < 0)) return LW_ELOOP;
+    out->m = fc.m;
+    out->skipped = fc.skipped;
+    out->ok = fc.ok;
+    out->keyframe = fc.keyframe;
+    out->registrations = fc.skipped ? 0 : 1 + fc.redo;
+    std::memcpy(out->T, fc.T, sizeof(out->T));
+    std::memcpy(out->xi, fc.xi1, sizeof(out->xi));
+    std::memcpy(&out->map_voxels, h + MC, sizeof(int32_t));
+    std::memcpy(&out->map_overflow, h + MC + sizeof(int32_t), sizeof(int32_t));
+    if (out->registrations >= 1) {
+        out->reg[0].m = fc.m;
+        fill_result(st1, p, &out->reg[0]);
+    }
+    if (out->registrations >= 2) {
+        out->reg[1].m = fc.m;
+        fill_result(st2, p, &out->reg[1]);
+    }
+    return 0;
 }
 
 int lw_prof_enable(lw_ctx *c, int32_t on) {

@@ -75,6 +75,9 @@ __host__ __device__ inline int32_t slot_hash(int64_t k) {
 
 __device__ inline int32_t ifloor(float x) { return (int32_t)floorf(x); }
 
+// A launch whose work a device flag decides: it runs unless gate is given and holds 0.
+__device__ inline bool gated_off(const int32_t *gate) { return gate && *gate == 0; }
+
 // A launch's point count: n, or, when n_dev is given, the count on the device capped at n (the grid covers n).
 __device__ inline int32_t count_of(int32_t n, const int32_t *n_dev) {
     if (!n_dev) return n;

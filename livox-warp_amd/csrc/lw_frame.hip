@@ -36,8 +36,10 @@ __device__ inline int32_t block_prefix(bool pred, int32_t *wsum, int32_t *total)
 
 // One block: off[b] = base + sum(cnt[0..b)); counter = base + total, base = counter (accumulate) or 0.
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_blocks(const int32_t *cnt, int32_t nb, int32_t *off,
-                                                              int32_t *counter, int32_t accumulate) {
+                                                              int32_t *counter, int32_t accumulate,
+                                                              const int32_t *gate) {
     __shared__ int32_t part[SCAN_THREADS];
+    if (gated_off(gate)) return;
     const int t = threadIdx.x;
     const int32_t chunk = (nb + SCAN_THREADS - 1) / SCAN_THREADS;
     const int32_t a = t * chunk, e = min(nb, a + chunk);
@@ -95,11 +97,22 @@ __global__ __launch_bounds__(BLOCK) void k_append_write(const float *sx, const u
 }
 
 // gpu.py:268-284: rotate by the rotation vector theta s (unit quaternion), then add rho s.
+// xi_dev, when given, holds the twist (rho, theta) in place of the values passed.
 __global__ __launch_bounds__(BLOCK) void k_deskew(const float *raw, const float *t, int32_t m, const int32_t *m_dev,
-                                                  float t_mid, float inv_dt, float r0, float r1, float r2, float h0,
-                                                  float h1, float h2, float *out) {
+                                                  const int32_t *gate, const float *xi_dev, float t_mid, float inv_dt,
+                                                  float r0, float r1, float r2, float h0, float h1, float h2,
+                                                  float *out) {
+    if (gated_off(gate)) return;
     int32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= count_of(m, m_dev)) return;
+    if (xi_dev) {
+        r0 = xi_dev[0];
+        r1 = xi_dev[1];
+        r2 = xi_dev[2];
+        h0 = xi_dev[3];
+        h1 = xi_dev[4];
+        h2 = xi_dev[5];
+    }
     float s = (t[i] - t_mid) * inv_dt;
     s = fminf(fmaxf(s, -1.0f), 1.0f);
     float px = raw[3 * (int64_t)i], py = raw[3 * (int64_t)i + 1], pz = raw[3 * (int64_t)i + 2];
@@ -128,7 +141,9 @@ __global__ __launch_bounds__(BLOCK) void k_deskew(const float *raw, const float 
 __global__ __launch_bounds__(BLOCK) void k_thin_claim(const float *xyz, const uint32_t *attr, int32_t m,
                                                       const int32_t *m_dev, float inv_v, float min_range,
                                                       float max_range, int32_t ret_mask, int32_t ds_mask,
-                                                      int64_t *ds_keys, int32_t *ds_idx, int32_t *pt_slot) {
+                                                      int64_t *ds_keys, int32_t *ds_idx, int32_t *pt_slot,
+                                                      const int32_t *gate) {
+    if (gated_off(gate)) return;
     int32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= count_of(m, m_dev)) return;
     float px = xyz[3 * (int64_t)i], py = xyz[3 * (int64_t)i + 1], pz = xyz[3 * (int64_t)i + 2];
@@ -152,7 +167,8 @@ __global__ __launch_bounds__(BLOCK) void k_thin_claim(const float *xyz, const ui
 }
 
 __global__ __launch_bounds__(BLOCK) void k_thin_count(const int32_t *pt_slot, const int32_t *ds_idx, int32_t m,
-                                                      const int32_t *m_dev, int32_t *blk_cnt) {
+                                                      const int32_t *m_dev, int32_t *blk_cnt, const int32_t *gate) {
+    if (gated_off(gate)) return;
     int32_t i = blockIdx.x * BLOCK + threadIdx.x;
     int32_t s = i < count_of(m, m_dev) ? pt_slot[i] : -1;
     bool p = s >= 0 && ds_idx[s] == i;
@@ -162,8 +178,9 @@ __global__ __launch_bounds__(BLOCK) void k_thin_count(const int32_t *pt_slot, co
 
 __global__ __launch_bounds__(BLOCK) void k_thin_write(const float *xyz, const int32_t *pt_slot, int32_t *ds_idx,
                                                       int64_t *ds_keys, int32_t m, const int32_t *m_dev,
-                                                      const int32_t *blk_off, float *ds_xyz) {
+                                                      const int32_t *blk_off, float *ds_xyz, const int32_t *gate) {
     __shared__ int32_t wsum[BLOCK / 64];
+    if (gated_off(gate)) return;
     int32_t i = blockIdx.x * BLOCK + threadIdx.x;
     int32_t s = i < count_of(m, m_dev) ? pt_slot[i] : -1;
     bool p = s >= 0 && ds_idx[s] == i;  // only the winner of a slot resets it, after reading it
@@ -187,7 +204,7 @@ hipError_t launch_frame_append(lw_ctx *c, const float *sx, const uint32_t *sa, c
     hipLaunchKernelGGL(k_append_count, dim3(nb), dim3(BLOCK), 0, c->stream, st, n, inv_dt, k_min, k_open,
                        c->blk_cnt);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_THREADS), 0, c->stream, c->blk_cnt, nb, c->blk_off,
-                       &c->ctr[6], 1);
+                       &c->ctr[6], 1, nullptr);
     hipLaunchKernelGGL(k_append_write, dim3(nb), dim3(BLOCK), 0, c->stream, sx, sa, st, n, inv_dt, k_min, k_open,
                        c->blk_off, (int32_t)c->frame_cap, c->f_raw, c->f_attr, c->f_t, c->f_frame);
     return hipGetLastError();
@@ -195,16 +212,20 @@ hipError_t launch_frame_append(lw_ctx *c, const float *sx, const uint32_t *sa, c
 
 // m < 0: the open frame's count on the device (its appended points, at most frame_cap); the grids then cover
 // frame_cap points and the blocks past the count do nothing.
-hipError_t launch_deskew(lw_ctx *c, int32_t m, float t_mid, float inv_dt, const float *xi) {
+hipError_t launch_deskew(lw_ctx *c, int32_t m, float t_mid, float inv_dt, const float *xi, const float *xi_dev,
+                         const int32_t *gate) {
     const int32_t *m_dev = m < 0 ? &c->ctr[6] : nullptr;
     if (m < 0) m = (int32_t)c->frame_cap;
     if (m == 0) return hipSuccess;
+    const float z[6] = {0, 0, 0, 0, 0, 0};
+    if (!xi) xi = z;
     hipLaunchKernelGGL(k_deskew, dim3((m + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, c->stream, c->f_raw, c->f_t, m, m_dev,
-                       t_mid, inv_dt, xi[0], xi[1], xi[2], xi[3], xi[4], xi[5], c->f_xyz);
+                       gate, xi_dev, t_mid, inv_dt, xi[0], xi[1], xi[2], xi[3], xi[4], xi[5], c->f_xyz);
     return hipGetLastError();
 }
 
-hipError_t launch_thin(lw_ctx *c, int32_t m, float inv_v, float min_range, float max_range, int32_t ret_mask) {
+hipError_t launch_thin(lw_ctx *c, int32_t m, float inv_v, float min_range, float max_range, int32_t ret_mask,
+                       const int32_t *gate) {
     const int32_t *m_dev = m < 0 ? &c->ctr[6] : nullptr;
     if (m < 0) m = (int32_t)c->frame_cap;
     if (m == 0) {
@@ -213,13 +234,14 @@ hipError_t launch_thin(lw_ctx *c, int32_t m, float inv_v, float min_range, float
     }
     int32_t nb = (m + BLOCK - 1) / BLOCK;  // blocks past the device count count no points
     hipLaunchKernelGGL(k_thin_claim, dim3(nb), dim3(BLOCK), 0, c->stream, c->f_xyz, c->f_attr, m, m_dev, inv_v,
-                       min_range, max_range, ret_mask, (int32_t)(c->ds_cap - 1), c->ds_keys, c->ds_idx, c->pt_slot);
+                       min_range, max_range, ret_mask, (int32_t)(c->ds_cap - 1), c->ds_keys, c->ds_idx, c->pt_slot,
+                       gate);
     hipLaunchKernelGGL(k_thin_count, dim3(nb), dim3(BLOCK), 0, c->stream, c->pt_slot, c->ds_idx, m, m_dev,
-                       c->blk_cnt);
+                       c->blk_cnt, gate);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_THREADS), 0, c->stream, c->blk_cnt, nb, c->blk_off,
-                       &c->ctr[5], 0);
+                       &c->ctr[5], 0, gate);
     hipLaunchKernelGGL(k_thin_write, dim3(nb), dim3(BLOCK), 0, c->stream, c->f_xyz, c->pt_slot, c->ds_idx,
-                       c->ds_keys, m, m_dev, c->blk_off, c->ds_xyz);
+                       c->ds_keys, m, m_dev, c->blk_off, c->ds_xyz, gate);
     return hipGetLastError();
 }
 

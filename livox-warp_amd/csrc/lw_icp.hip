@@ -705,6 +705,7 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_icp(IcpArgs a) {
 struct LoopArgs {
     IcpArgs a;                // pointers and the fixed scalars; n, max_d2, kappa, it are set per pass
     const int32_t *n_dev;     // thinned count on the device
+    const int32_t *gate;      // when given, a device flag that must be nonzero for the loop to run
     uint32_t *flags;          // MAX_LOOP_ROWS row flags
     int32_t max_iter, wide_iters, min_corr;
     float max_d2_r1, max_d2_r2;
@@ -886,6 +887,7 @@ template <int MODE>
 __global__ __launch_bounds__(ICP_BLOCK) void k_register(LoopArgs L) {
     RegState *st = L.a.st;
     __shared__ __align__(16) unsigned char lds[RED_BYTES];
+    if (gated_off(L.gate)) return;  // the state stays as the previous registration left it
     const int32_t n = *L.n_dev;
     if (blockIdx.x == 0 && threadIdx.x == 0) st->n = n;
     if (n < L.min_corr) return;  // odom.py:756-757: the host sees n and keeps the prediction
@@ -898,17 +900,20 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_register(LoopArgs L) {
     searcher<MODE>(L, n, lds);
 }
 
-// Tp: 16 doubles (the prediction) followed by 64 floats (the per-iteration robust-kernel scale). Also clears
-// the persistent loop's row flags and published pose.
-__global__ void k_reg_begin(RegState *st, const double *Tp, uint32_t *flags) {
+// Tp: 16 doubles (the prediction) followed by 64 floats (the per-iteration robust-kernel scale); T_dev, when
+// given, holds the prediction instead. Also clears the persistent loop's row flags and published pose. Does
+// nothing when gate is given and holds 0.
+__global__ void k_reg_begin(RegState *st, const double *Tp, uint32_t *flags, const int32_t *gate, const double *T_dev) {
+    if (gated_off(gate)) return;
     if (threadIdx.x < 64) st->kappa[threadIdx.x] = reinterpret_cast<const float *>(Tp + 16)[threadIdx.x];
     for (int k = threadIdx.x; k < MAX_LOOP_ROWS; k += blockDim.x) flags[k] = 0u;
     if (threadIdx.x < 16) st->pub[threadIdx.x] = 0ull;
     if (threadIdx.x != 0) return;
+    const double *P = T_dev ? T_dev : Tp;
     for (int k = 0; k < 16; ++k) {
-        st->T[k] = Tp[k];
-        st->T_pred[k] = Tp[k];
-        st->Tf[k] = (float)Tp[k];
+        st->T[k] = P[k];
+        st->T_pred[k] = P[k];
+        st->Tf[k] = (float)P[k];
     }
     for (int k = 0; k < 36; ++k) st->H[k] = (k % 7 == 0) ? 1.0 : 0.0;  // np.eye(6) (odom.py:763)
     st->rms = 0.0;
@@ -959,8 +964,8 @@ const void *loop_kernel() {
 
 }  // namespace
 
-hipError_t launch_reg_begin(lw_ctx *c, const double *T_pred_dev) {
-    hipLaunchKernelGGL(k_reg_begin, dim3(1), dim3(64), 0, c->stream, c->reg, T_pred_dev, c->loop_flags);
+hipError_t launch_reg_begin(lw_ctx *c, const double *T_pred_dev, const int32_t *gate, const double *T_dev) {
+    hipLaunchKernelGGL(k_reg_begin, dim3(1), dim3(64), 0, c->stream, c->reg, T_pred_dev, c->loop_flags, gate, T_dev);
     return hipGetLastError();
 }
 
@@ -998,7 +1003,8 @@ hipError_t launch_icp_pass(lw_ctx *c, const IcpPass &p, int acc_mode) {
     return e;
 }
 
-hipError_t launch_register_loop(lw_ctx *c, const int32_t *n_dev, int32_t n_bound, const RegLoop &R, int acc_mode) {
+hipError_t launch_register_loop(lw_ctx *c, const int32_t *n_dev, int32_t n_bound, const RegLoop &R, int acc_mode,
+                                const int32_t *gate) {
     if (acc_mode < 0 || acc_mode > 2 || R.max_iter > 64 || n_bound > c->frame_cap) return hipErrorNotSupported;
     const void *fn = acc_mode == 0 ? loop_kernel<0>() : acc_mode == 1 ? loop_kernel<1>() : loop_kernel<2>();
     if (c->loop_max_groups[acc_mode] == 0) {  // co-resident workgroups, once per context
@@ -1022,6 +1028,7 @@ hipError_t launch_register_loop(lw_ctx *c, const int32_t *n_dev, int32_t n_bound
     L.a.gn = 1;
     L.a.sp = R.sp;
     L.n_dev = n_dev;
+    L.gate = gate;
     L.flags = c->loop_flags;
     L.max_iter = R.max_iter;
     L.wide_iters = R.wide_iters;

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "lw_common.h"
@@ -40,6 +41,25 @@ struct RegState {
     unsigned long long t_rel;  // device wall clock at the current pass's start (measurement only)
     float kappa[64];    // robust-kernel scale per iteration (odom.py:767)
 };
+
+// One frame's chain of device steps and the decisions between them (lw_odom_frame): the inputs the host
+// uploads, then what the decision kernels write.
+struct FrameCtl {
+    double T_pred[16];      // the prediction
+    double T_prev_inv[16];  // the inverse of the previous frame's pose (has_prev)
+    double T_key_inv[16];   // the inverse of the last keyframe's pose (has_key)
+    double kf_dist, kf_angle;           // keyframe thresholds: metres, degrees
+    double redeskew_rot, redeskew_trans;
+    float xi0[6];           // the twist the frame is deskewed with first
+    int32_t min_points, min_corr, has_prev, has_key, redeskew, frame_cap;
+    // written on the device
+    double T1[16];          // the first registration's pose (the second's prediction)
+    double T[16];           // the frame's pose
+    float xi1[6];           // the twist the frame ends deskewed with
+    float T32[16];          // the pose a keyframe insert uses
+    int32_t m, skipped, ok1, redo, ok, keyframe;
+};
+constexpr size_t FRAME_CTL_INPUT = offsetof(FrameCtl, T1);  // bytes the host uploads
 
 }  // namespace lw
 
@@ -85,6 +105,8 @@ struct lw_ctx {
     int64_t max_blocks = 0;
     // registration
     lw::RegState *reg = nullptr;
+    lw::RegState *reg1 = nullptr;   // the first registration of a frame, kept by lw_odom_frame's check
+    lw::FrameCtl *fctl = nullptr;   // lw_odom_frame's inputs and decisions
     double *partials = nullptr;
     uint32_t *loop_flags = nullptr;  // MAX_LOOP_ROWS: the pass each searching workgroup's row belongs to
     int32_t *best_slot = nullptr;
@@ -129,18 +151,29 @@ struct Mat4f {
     float m[16];
 };
 
+__device__ inline Mat4f load_mat4f(const float *p) {
+    Mat4f T;
+    for (int k = 0; k < 16; ++k) T.m[k] = p[k];
+    return T;
+}
+
 hipError_t launch_map_reset(lw_ctx *c);
-// n_dev: when given, the points to insert are its count capped at n (n sizes the grids).
+// n_dev: when given, the points to insert are its count capped at n (n sizes the grids). gate: when given, a
+// device flag that must be nonzero for the insert to happen. T_dev: the pose on the device (replaces T).
 hipError_t launch_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, int32_t n, const int32_t *n_dev,
-                             const Mat4f &T, float inv_v, float voxel, float max_range, int32_t ret_mask);
+                             const Mat4f &T, float inv_v, float voxel, float max_range, int32_t ret_mask,
+                             const int32_t *gate = nullptr, const float *T_dev = nullptr);
 hipError_t launch_refresh(lw_ctx *c, bool all, float voxel, float noise2, float tight2, float alpha);
 hipError_t launch_map_find(lw_ctx *c, const int64_t *keys_dev, int32_t n, int32_t *out_dev);
 
 hipError_t launch_frame_append(lw_ctx *c, const float *sx, const uint32_t *sa, const float *st, int32_t n,
                                float inv_dt, int32_t k_min, int32_t k_open);
-// m < 0: the open frame's point count on the device (ctr[6], capped at frame_cap)
-hipError_t launch_deskew(lw_ctx *c, int32_t m, float t_mid, float inv_dt, const float *xi);
-hipError_t launch_thin(lw_ctx *c, int32_t m, float inv_v, float min_range, float max_range, int32_t ret_mask);
+// m < 0: the open frame's point count on the device (ctr[6], capped at frame_cap). gate: when given, a device
+// flag that must be nonzero for the kernels to do anything. xi_dev: the twist on the device (replaces xi).
+hipError_t launch_deskew(lw_ctx *c, int32_t m, float t_mid, float inv_dt, const float *xi,
+                         const float *xi_dev = nullptr, const int32_t *gate = nullptr);
+hipError_t launch_thin(lw_ctx *c, int32_t m, float inv_v, float min_range, float max_range, int32_t ret_mask,
+                       const int32_t *gate = nullptr);
 hipError_t launch_fill_i32(lw_ctx *c, int32_t *p, int64_t n, int32_t v);
 
 struct IcpPass {
@@ -163,8 +196,13 @@ struct RegLoop {
     float kappa[64];
     gn::StepParams sp;
 };
-hipError_t launch_register_loop(lw_ctx *c, const int32_t *n_dev, int32_t n_bound, const RegLoop &L, int acc_mode);
+hipError_t launch_register_loop(lw_ctx *c, const int32_t *n_dev, int32_t n_bound, const RegLoop &L, int acc_mode,
+                                const int32_t *gate = nullptr);
 int32_t icp_max_groups(int64_t frame_cap);  // partial rows one pass over frame_cap points can write
-hipError_t launch_reg_begin(lw_ctx *c, const double *T_pred_dev);
+hipError_t launch_reg_begin(lw_ctx *c, const double *T_pred_dev, const int32_t *gate = nullptr,
+                            const double *T_dev = nullptr);
+// lw_odom.hip: after a frame's first registration, and after its second
+hipError_t launch_frame_check(lw_ctx *c);
+hipError_t launch_keyframe_check(lw_ctx *c);
 
 }  // namespace lw

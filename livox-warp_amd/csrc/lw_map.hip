@@ -65,7 +65,10 @@ __global__ __launch_bounds__(BLOCK) void k_ins_claim(const float *xyz, const uin
                                                      const int32_t *n_dev, Mat4f T, float inv_v, float max_range,
                                                      int32_t ret_mask, int32_t mask, Slot *slots, uint32_t *occ,
                                                      int64_t *vox_key, int32_t *vox_slot, VoxMom *vox_mom,
-                                                     int32_t *ctr, int64_t vox_cap, int32_t *pt_slot) {
+                                                     int32_t *ctr, int64_t vox_cap, int32_t *pt_slot,
+                                                     const int32_t *gate, const float *T_dev) {
+    if (gated_off(gate)) return;
+    if (T_dev) T = load_mat4f(T_dev);
     int32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i == 0) ctr[4] = 0;  // touched count of this insert (read by the next kernel only)
     if (i >= count_of(n, n_dev)) return;
@@ -122,7 +125,8 @@ __global__ __launch_bounds__(BLOCK) void k_ins_claim(const float *xyz, const uin
 // 2: count the incoming points per voxel; the first one lists the voxel as touched.
 __global__ __launch_bounds__(BLOCK) void k_ins_count(const int32_t *pt_slot, int32_t n, const int32_t *n_dev,
                                                      const Slot *slots, int32_t *pt_vid, int32_t *vox_nin,
-                                                     int32_t *touched, int32_t *ctr) {
+                                                     int32_t *touched, int32_t *ctr, const int32_t *gate) {
+    if (gated_off(gate)) return;
     int32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= count_of(n, n_dev)) return;
     int32_t s = pt_slot[i];
@@ -136,9 +140,10 @@ __global__ __launch_bounds__(BLOCK) void k_ins_count(const int32_t *pt_slot, int
 __global__ __launch_bounds__(BLOCK) void k_ins_quota(const int32_t *touched, const int32_t *ctr,
                                                      const int32_t *vox_nin, const VoxMom *vox_mom,
                                                      int32_t *vox_quota, const int32_t *pt_vid, int32_t n,
-                                                     const int32_t *n_dev, uint8_t *pt_take) {
+                                                     const int32_t *n_dev, uint8_t *pt_take, const int32_t *gate) {
     __shared__ int32_t wsum[BLOCK / 64];
     __shared__ int32_t s_vid, s_quota, s_nin;
+    if (gated_off(gate)) return;
     n = count_of(n, n_dev);
     int32_t nt = ctr[4];
     for (int32_t j = blockIdx.x; j < nt; j += gridDim.x) {
@@ -184,7 +189,9 @@ __global__ __launch_bounds__(BLOCK) void k_ins_accum(const float *xyz, const uin
                                                      const int32_t *n_dev, Mat4f T, float inv_v, float voxel,
                                                      float max_range, int32_t ret_mask, const int32_t *pt_vid,
                                                      const int32_t *vox_quota, const uint8_t *pt_take,
-                                                     VoxMom *vox_mom) {
+                                                     VoxMom *vox_mom, const int32_t *gate, const float *T_dev) {
+    if (gated_off(gate)) return;
+    if (T_dev) T = load_mat4f(T_dev);
     int32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= count_of(n, n_dev)) return;
     int32_t vid = pt_vid[i];
@@ -210,7 +217,9 @@ __global__ __launch_bounds__(BLOCK) void k_ins_accum(const float *xyz, const uin
 
 // 5: finish: clear the per-insert counters, queue the touched voxels for a refresh.
 __global__ __launch_bounds__(BLOCK) void k_ins_finish(const int32_t *touched, int32_t *ctr, int32_t *vox_nin,
-                                                      int32_t *vox_quota, int32_t *vox_dirty, int32_t *dirty_list) {
+                                                      int32_t *vox_quota, int32_t *vox_dirty, int32_t *dirty_list,
+                                                      const int32_t *gate) {
+    if (gated_off(gate)) return;
     int32_t nt = ctr[4];
     for (int32_t j = blockIdx.x * BLOCK + threadIdx.x; j < nt; j += gridDim.x * BLOCK) {
         int32_t vid = touched[j];
@@ -301,20 +310,21 @@ hipError_t launch_map_reset(lw_ctx *c) {
 }
 
 hipError_t launch_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, int32_t n, const int32_t *n_dev,
-                             const Mat4f &T, float inv_v, float voxel, float max_range, int32_t ret_mask) {
+                             const Mat4f &T, float inv_v, float voxel, float max_range, int32_t ret_mask,
+                             const int32_t *gate, const float *T_dev) {
     if (n <= 0) return hipSuccess;
     int g = grid_for(n);
     hipLaunchKernelGGL(k_ins_claim, dim3(g), dim3(BLOCK), 0, c->stream, xyz, attr, n, n_dev, T, inv_v, max_range,
                        ret_mask, c->mask, c->slots, c->occ, c->vox_key, c->vox_slot, c->vox_mom, c->ctr, c->vox_cap,
-                       c->pt_slot);
+                       c->pt_slot, gate, T_dev);
     hipLaunchKernelGGL(k_ins_count, dim3(g), dim3(BLOCK), 0, c->stream, c->pt_slot, n, n_dev, c->slots, c->pt_vid,
-                       c->vox_nin, c->touched, c->ctr);
+                       c->vox_nin, c->touched, c->ctr, gate);
     hipLaunchKernelGGL(k_ins_quota, dim3(1024), dim3(BLOCK), 0, c->stream, c->touched, c->ctr, c->vox_nin,
-                       c->vox_mom, c->vox_quota, c->pt_vid, n, n_dev, c->pt_take);
+                       c->vox_mom, c->vox_quota, c->pt_vid, n, n_dev, c->pt_take, gate);
     hipLaunchKernelGGL(k_ins_accum, dim3(g), dim3(BLOCK), 0, c->stream, xyz, attr, n, n_dev, T, inv_v, voxel,
-                       max_range, ret_mask, c->pt_vid, c->vox_quota, c->pt_take, c->vox_mom);
+                       max_range, ret_mask, c->pt_vid, c->vox_quota, c->pt_take, c->vox_mom, gate, T_dev);
     hipLaunchKernelGGL(k_ins_finish, dim3(256), dim3(BLOCK), 0, c->stream, c->touched, c->ctr, c->vox_nin,
-                       c->vox_quota, c->vox_dirty, c->dirty_list);
+                       c->vox_quota, c->vox_dirty, c->dirty_list, gate);
     return hipGetLastError();
 }
 

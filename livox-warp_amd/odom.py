@@ -152,6 +152,14 @@ class Odometry:
                    "lw_bind_frame")
         self._params = _abi.RegParams()
         self._result = _abi.RegResult()
+        # one frame's whole device chain (lw_odom_frame); numpy views onto its inputs
+        self._fin = _abi.FrameIn()
+        self._fout = _abi.FrameOut()
+        self._fin_T = np.ctypeslib.as_array(self._fin.T_pred)
+        self._fin_prev = np.ctypeslib.as_array(self._fin.T_prev_inv)
+        self._fin_key = np.ctypeslib.as_array(self._fin.T_key_inv)
+        self._fin_xi = np.ctypeslib.as_array(self._fin.xi)
+        self.chain = True  # False: the step-by-step path (also taken when the device has no cooperative launch)
         self._i32 = (_abi.ctypes.c_int32 * 2)()
 
         # tuning (odom.py:499-525)
@@ -303,12 +311,19 @@ class Odometry:
         t0 = time.perf_counter()
         warm = self.map_frames < self.warmup
         t_mid = (k + 0.5) * self.frame_dt
+        chained = False
         if warm or self.prior is not None:
             m = min(self._frame_count(), self.frame_cap)
             T, xi, ok = self._estimate(m, t_mid, warm)
-        else:  # the frame's point count comes back with the registration: no read of it before
+        elif self.chain:  # deskew, registrations, keyframe test and insert queued at once, one readback
+            res = self._frame_chain(t_mid)
+            if res is not None:
+                T, xi, ok, m = res
+                chained = True
+        if not chained and not warm and self.prior is None:
+            # the frame's point count comes back with the registration: no read of it before
             T, xi, ok, m = self._estimate_counted(t_mid)
-        if ok and (warm or self._is_keyframe(T)):
+        if not chained and ok and (warm or self._is_keyframe(T)):
             self._insert(m, T)
         if self.T_prev is not None and not warm:
             self._update_velocity(np.linalg.inv(self.T_prev) @ T)
@@ -354,6 +369,48 @@ class Odometry:
                 self._deskew(m, t_mid, xi)
                 T, ok = self._register(m, T)
         return T, xi, ok
+
+    def _frame_chain(self, t_mid: float):
+        """_estimate and the keyframe insert of a frame after the warm-up (no learned prior) as one device chain
+        (lw_odom_frame): the same decisions -- skip under min_points, the re-deskew test and second registration,
+        the keyframe test -- taken on the device between the steps. Returns (T, xi, ok, m), or None when the
+        device has no cooperative launch (the step-by-step path then runs this and every later frame)."""
+        fi, fo = self._fin, self._fout
+        xi = self.xi if self.deskew else np.zeros(6, np.float32)
+        T_pred = self.predict()
+        self._fin_T[:] = T_pred.ravel()
+        fi.has_prev = 1 if self.T_prev is not None else 0
+        if fi.has_prev:
+            self._fin_prev[:] = np.linalg.inv(self.T_prev).ravel()
+        fi.has_key = 1 if self.T_key is not None else 0
+        if fi.has_key:
+            self._fin_key[:] = np.linalg.inv(self.T_key).ravel()
+        self._fin_xi[:] = xi
+        fi.kf_dist, fi.kf_angle = float(self.kf_dist), float(self.kf_angle)
+        fi.redeskew_rot, fi.redeskew_trans = REDESKEW_ROT, REDESKEW_TRANS
+        fi.t_mid, fi.inv_dt = float(t_mid), 1.0 / self.frame_dt
+        fi.min_points = int(self.min_points)
+        fi.redeskew = 1 if (self.deskew and self.redeskew) else 0
+        rc = self._lib.lw_odom_frame(self._ctx, _abi.ctypes.byref(self.reg_params()), _abi.ctypes.byref(fi),
+                                     _abi.ctypes.byref(fo))
+        if rc == _abi.LW_ENOLOOP:
+            self.chain = False
+            return None
+        _abi.check(rc, "lw_odom_frame")
+        m = int(fo.m)
+        if fo.skipped:
+            self.stats["skipped"] += 1
+            return T_pred, xi, False, m
+        T_from = T_pred
+        for k in range(int(fo.registrations)):  # the stats of each registration, in order
+            T_from, _ = self._register_take(fo.reg[k], T_from)
+        T = np.array(fo.T, dtype=np.float64).reshape(4, 4)
+        if fo.keyframe:  # Odometry._insert's bookkeeping; the insert itself ran in the chain
+            self.map_frames += 1
+            self.T_key = T.copy()
+            self.stats["keyframes"] += 1
+        self._map_counts = (int(fo.map_voxels), int(fo.map_overflow))
+        return T, np.array(fo.xi, dtype=np.float32), bool(fo.ok), m
 
     def _estimate_counted(self, t_mid: float):
         """_estimate after the warm-up without the learned prior, with the frame's point count left on the

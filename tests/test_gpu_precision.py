@@ -65,15 +65,16 @@ def _per_registration(lw, orc, translated, mode):
     odo.reg_voxel = 0.2
     odo.max_iter = 20
     odo.acc_mode = mode
+    odo.chain = False  # step by step: each registration's thinned set and map are read right after it
     L, ctx = odo._lib, odo._ctx
     P = orc.Params(reg_voxel=odo.reg_voxel, max_iter=odo.max_iter, min_corr=odo.min_corr, min_range=odo.min_range,
                    max_range=odo.max_range, noise=odo.noise, ret_mask=odo.ret_mask, alpha=odo.alpha,
                    damping=odo.damping, degen_thr=odo.degen_thr, prior_info=odo.prior_info)
     e32, e64 = [], []
-    inner = odo._register
+    inner = odo._register_take  # every registration's result passes here, right after it ran
 
-    def wrapped(m, T_pred):
-        T, ok = inner(m, T_pred)
+    def wrapped(r, T_pred):
+        T, ok = inner(r, T_pred)
         n = odo.stats["ds"]
         if n < odo.min_corr:
             return T, ok
@@ -89,7 +90,7 @@ def _per_registration(lw, orc, translated, mode):
         e64.append(_pose_err(T, r64.T))
         return T, ok
 
-    odo._register = wrapped
+    odo._register_take = wrapped
     odo.reset(seq.pose(0.05))
     for x, a, t in frames:
         odo.push(torch.from_numpy(x).cuda(), torch.from_numpy(a.view(np.int32)).cuda(), torch.from_numpy(t).cuda(),
