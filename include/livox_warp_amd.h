@@ -169,9 +169,11 @@ int lw_odom_frame(lw_ctx *ctx, const lw_reg_params *p, const lw_frame_in *in, lw
 int lw_register(lw_ctx *ctx, int32_t m, const lw_reg_params *p, const double *T_pred, lw_reg_result *res);
 /* The same loop over the already-thinned points (lw_thin / lw_thin_load), n of them. */
 int lw_register_thinned(lw_ctx *ctx, int32_t n, const lw_reg_params *p, const double *T_pred, lw_reg_result *res);
-/* on = 1 (default): a registration is one cooperative launch that runs every iteration; on = 0: one launch
- * per iteration (the path taken automatically where cooperative launches are unavailable). Same results up
- * to the order of the float64 partial sums. No counterpart in the reference. */
+/* on = 1 (default): a registration is one persistent launch that runs every iteration, a plain launch when
+ * its grid is at most half of what the device holds at once, else a cooperative one; on = 2: always the
+ * cooperative launch; on = 0: one launch per iteration (the path taken automatically where cooperative
+ * launches are unavailable). Same results up to the order of the float64 partial sums. No counterpart in the
+ * reference. */
 int lw_set_persistent(lw_ctx *ctx, int32_t on);
 
 /* Batched global localisation parameters. */

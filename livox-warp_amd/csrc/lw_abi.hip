@@ -591,8 +591,9 @@ int lw_frame_features(lw_ctx *c, int32_t m, float r_max, float e_max, const floa
 }
 
 int lw_set_persistent(lw_ctx *c, int32_t on) {
-    if (!c) return LW_EINVAL;
+    if (!c || on < 0 || on > 2) return LW_EINVAL;
     c->loop_ok = on != 0;
+    c->loop_plain = on == 1;
     return 0;
 }
 

@@ -1019,6 +1019,7 @@ hipError_t launch_register_loop(lw_ctx *c, const int32_t *n_dev, int32_t n_bound
         // group per searcher (8192 thinned points at the 5x5x5 reach), and every extra workgroup only
         // polls for the pose
         c->loop_max_groups[acc_mode] = per_cu < 2 ? per_cu * cus : 2 * cus;
+        c->loop_capacity[acc_mode] = per_cu * cus;
         if (c->loop_max_groups[acc_mode] < 2) return hipErrorNotSupported;
     }
     LoopArgs L;
@@ -1048,7 +1049,15 @@ hipError_t launch_register_loop(lw_ctx *c, const int32_t *n_dev, int32_t n_bound
         hipError_t e = hipEventRecord(c->prof_ev[2 * c->prof_used], c->stream);
         if (e != hipSuccess) return e;
     }
-    hipError_t e = hipLaunchCooperativeKernel(fn, dim3((unsigned)g), dim3(ICP_BLOCK), args, 0, c->stream);
+    // A plain launch when the grid is at most half of what the device holds at once: every workgroup then
+    // becomes resident as soon as the device is free of other work (a kernel on another stream delays some,
+    // it cannot keep them out), which is what the loop's waits need, without the barriers the runtime puts
+    // around a cooperative launch. Otherwise (or on request) the cooperative launch guarantees it.
+    hipError_t e;
+    if (c->loop_plain && 2 * g <= c->loop_capacity[acc_mode])
+        e = hipLaunchKernel(fn, dim3((unsigned)g), dim3(ICP_BLOCK), args, 0, c->stream);
+    else
+        e = hipLaunchCooperativeKernel(fn, dim3((unsigned)g), dim3(ICP_BLOCK), args, 0, c->stream);
     if (e != hipSuccess) return e;
     if (timed) {
         e = hipEventRecord(c->prof_ev[2 * c->prof_used + 1], c->stream);

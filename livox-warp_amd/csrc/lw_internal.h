@@ -139,7 +139,9 @@ struct lw_ctx {
     // the persistent registration: co-resident workgroups per accumulation form (0 = not yet queried), and
     // whether cooperative launches work on this device
     int32_t loop_max_groups[3] = {0, 0, 0};
+    int32_t loop_capacity[3] = {0, 0, 0};  // workgroups of each form the device holds at once
     bool loop_ok = true;
+    bool loop_plain = true;  // plain launches of the loop where its grid is at most half the capacity
     // parameters the cached per-voxel precision was computed with
     bool prec_valid = false;
     float prec_voxel = 0, prec_noise2 = 0, prec_tight2 = 0, prec_alpha = 0;

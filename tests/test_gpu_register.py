@@ -40,15 +40,16 @@ def test_persistent_loop_matches_per_pass(lw, orc, gpu_ok):
     ds = orc.ds_insert(w.scan_xyz, w.scan_attr, 2.0 / voxel, 0.3, 60.0, 15, 1 << 15)
     eng.thin_load(ds)
     out = {}
-    for on in (1, 0):
+    for on in (1, 2, 0):  # persistent (plain launch), persistent (cooperative launch), one launch per pass
         lw._abi.check(eng.lib.lw_set_persistent(eng.ctx, on), "lw_set_persistent")
         for mode in ("fp64", "kahan32"):
             p = eng.params(reg_voxel=voxel, max_iter=20, acc_mode=mode)
             out[(on, mode)] = eng.register_thinned(len(ds), p, w.T_init)
     for mode in ("fp64", "kahan32"):
-        a, b = out[(1, mode)], out[(0, mode)]
-        assert a.iters == b.iters and a.corr == b.corr and a.degen == b.degen and a.ok == b.ok
-        assert np.abs(np.array(a.T) - np.array(b.T)).max() < 1e-9
+        for on in (1, 2):
+            a, b = out[(on, mode)], out[(0, mode)]
+            assert a.iters == b.iters and a.corr == b.corr and a.degen == b.degen and a.ok == b.ok
+            assert np.abs(np.array(a.T) - np.array(b.T)).max() < 1e-9
 
 
 def test_register_from_frame_buffers(lw, orc, gpu_ok):
