@@ -47,8 +47,8 @@ bool pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
 void free_all(lw_ctx *c) {
     void *ptrs[] = {c->slots, c->occ, c->vox_key, c->vox_mom, c->vox_slot, c->vox_prec, c->vox_nin, c->vox_quota,
                     c->vox_dirty, c->dirty_list, c->touched, c->ctr, c->pt_slot, c->pt_vid, c->pt_take,
-                    c->ds_keys, c->ds_idx, c->ds_xyz, c->blk_cnt, c->blk_off, c->reg, c->partials, c->best_slot,
-                    c->dtmp, c->loop_flags, c->reg1, c->fctl};
+                    c->ds_keys, c->ds_idx, c->ds_xyz, c->blk_cnt, c->blk_off, c->chain, c->partials, c->best_slot,
+                    c->dtmp, c->loop_flags};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->prof_dev) (void)hipFree(c->prof_dev);
@@ -75,6 +75,16 @@ hipError_t prof_harvest(lw_ctx *c) {
     return hipSuccess;
 }
 
+// The chain block (lw_ctx::chain): [reg | reg1 | chain_tp | fctl | chain_counts], each part 256-byte aligned.
+constexpr size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+constexpr size_t CH_REG1 = align256(sizeof(lw::RegState));
+constexpr size_t CH_TP = CH_REG1 + align256(sizeof(lw::RegState));
+constexpr size_t CH_TP_BYTES = 16 * sizeof(double) + 64 * sizeof(float);
+constexpr size_t CH_FCTL = CH_TP + align256(CH_TP_BYTES);
+constexpr size_t CH_COUNTS = CH_FCTL + align256(sizeof(lw::FrameCtl));
+constexpr size_t CH_BYTES = CH_COUNTS + 256;
+static_assert(CH_BYTES <= PIN_DOWN, "the chain block fits the pinned readback area");
+
 char *pin_up(lw_ctx *c) { return (char *)c->pin + PIN_UP; }
 char *pin_down(lw_ctx *c) { return (char *)c->pin + PIN_DOWN; }
 
@@ -83,7 +93,8 @@ char *pin_down(lw_ctx *c) { return (char *)c->pin + PIN_DOWN; }
 hipError_t ensure_precision(lw_ctx *c, float voxel, float noise2, float tight2, float alpha) {
     bool same = c->prec_valid && c->prec_voxel == voxel && c->prec_noise2 == noise2 && c->prec_tight2 == tight2 &&
                 c->prec_alpha == alpha;
-    hipError_t e = lw::launch_refresh(c, !same, voxel, noise2, tight2, alpha);
+    if (same && !c->dirty_maybe) return hipSuccess;  // nothing queued since the last refresh
+    hipError_t e = lw::launch_refresh(c, !same, lw::PrecParams{voxel, noise2, tight2, alpha});
     c->prec_valid = true;
     c->prec_voxel = voxel;
     c->prec_noise2 = noise2;
@@ -149,9 +160,12 @@ int lw_create(lw_ctx **out, int64_t reg_slots, int64_t frame_cap, void *stream) 
     if ((e = dalloc(&c->ds_xyz, 3 * frame_cap)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->blk_cnt, c->max_blocks)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->blk_off, c->max_blocks)) != hipSuccess) return fail(e);
-    if ((e = dalloc(&c->reg, 1)) != hipSuccess) return fail(e);
-    if ((e = dalloc(&c->reg1, 1)) != hipSuccess) return fail(e);
-    if ((e = dalloc(&c->fctl, 1)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->chain, CH_BYTES)) != hipSuccess) return fail(e);
+    c->reg = reinterpret_cast<lw::RegState *>(c->chain);
+    c->reg1 = reinterpret_cast<lw::RegState *>(c->chain + CH_REG1);
+    c->chain_tp = reinterpret_cast<double *>(c->chain + CH_TP);
+    c->fctl = reinterpret_cast<lw::FrameCtl *>(c->chain + CH_FCTL);
+    c->chain_counts = reinterpret_cast<int32_t *>(c->chain + CH_COUNTS);
     if ((e = dalloc(&c->partials, (size_t)lw::icp_max_groups(frame_cap) * 128)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->loop_flags, lw::MAX_LOOP_ROWS)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->best_slot, frame_cap)) != hipSuccess) return fail(e);
@@ -210,6 +224,7 @@ int lw_map_reset(lw_ctx *c) {
 int lw_frame_reset(lw_ctx *c) {
     if (!c) return LW_EINVAL;
     LW_CK(lw::launch_fill_i32(c, &c->ctr[6], 1, 0));
+    c->frame_bound = 0;
     return 0;
 }
 
@@ -222,6 +237,7 @@ int lw_frame_append(lw_ctx *c, const float *sx, const uint32_t *sa, const float 
         int32_t m = (int32_t)((n - a) < chunk ? (n - a) : chunk);
         LW_CK(lw::launch_frame_append(c, sx + 3 * a, sa + a, st + a, m, inv_dt, k_min, k_open));
     }
+    c->frame_bound = c->frame_bound + n < c->frame_cap ? c->frame_bound + n : c->frame_cap;
     return 0;
 }
 
@@ -614,7 +630,8 @@ int lw_odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, lw_f
     std::memset(out, 0, sizeof(*out));
     lw::RegLoop L;
     if (int rc = reg_setup(c, p, L)) return rc;
-    // uploads: the prediction and the kappa schedule (as run_registration), then the frame's control inputs
+    // one upload: the prediction and the kappa schedule (as run_registration), then the frame's control inputs,
+    // contiguous in the chain block
     lw::FrameCtl f{};
     std::memcpy(f.T_pred, in->T_pred, sizeof(f.T_pred));
     std::memcpy(f.T_prev_inv, in->T_prev_inv, sizeof(f.T_prev_inv));
@@ -630,19 +647,25 @@ int lw_odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, lw_f
     f.has_key = in->has_key;
     f.redeskew = in->redeskew;
     f.frame_cap = (int32_t)c->frame_cap;
-    constexpr size_t CTL_AT = 1024;
+    constexpr size_t UP_CTL = CH_FCTL - CH_TP;
     std::memcpy(pin_up(c), in->T_pred, 16 * sizeof(double));
     std::memcpy(pin_up(c) + 16 * sizeof(double), L.kappa, 64 * sizeof(float));
-    std::memcpy(pin_up(c) + CTL_AT, &f, lw::FRAME_CTL_INPUT);
-    LW_CK(hipMemcpyAsync(c->dtmp, pin_up(c), 16 * sizeof(double) + 64 * sizeof(float), hipMemcpyHostToDevice,
-                         c->stream));
-    LW_CK(hipMemcpyAsync(c->fctl, pin_up(c) + CTL_AT, lw::FRAME_CTL_INPUT, hipMemcpyHostToDevice, c->stream));
+    std::memcpy(pin_up(c) + UP_CTL, &f, lw::FRAME_CTL_INPUT);
+    LW_CK(hipMemcpyAsync(c->chain_tp, pin_up(c), UP_CTL + lw::FRAME_CTL_INPUT, hipMemcpyHostToDevice, c->stream));
     const float inv_v_thin = (float)(2.0 / (double)p->reg_voxel);
     const int32_t cap = (int32_t)c->frame_cap;
+    // a frame one workgroup can prepare (up to frame_prep_max() points, with a thinning table of at least 4096
+    // slots): deskew, thinning and the registration's start in one launch each time
+    const bool one = c->frame_bound <= lw::frame_prep_max() && c->ds_cap >= 4096;
     // odom.py:671-681: deskew with the model's twist, register from the prediction
-    LW_CK(lw::launch_deskew(c, -1, in->t_mid, in->inv_dt, in->xi));
-    LW_CK(lw::launch_thin(c, -1, inv_v_thin, p->min_range, p->max_range, p->ret_mask));
-    LW_CK(lw::launch_reg_begin(c, (const double *)c->dtmp));
+    if (one) {
+        LW_CK(lw::launch_frame_prep(c, c->chain_tp, in->t_mid, in->inv_dt, in->xi, inv_v_thin, p->min_range,
+                                    p->max_range, p->ret_mask, false));
+    } else {
+        LW_CK(lw::launch_deskew(c, -1, in->t_mid, in->inv_dt, in->xi));
+        LW_CK(lw::launch_thin(c, -1, inv_v_thin, p->min_range, p->max_range, p->ret_mask));
+        LW_CK(lw::launch_reg_begin(c, c->chain_tp));
+    }
     hipError_t e = lw::launch_register_loop(c, &c->ctr[5], cap, L, p->acc_mode);
     if (e != hipSuccess) {
         c->loop_ok = false;  // cooperative launches unavailable here: the caller takes the step-by-step path
@@ -651,32 +674,36 @@ int lw_odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, lw_f
         return LW_ENOLOOP;
     }
     // odom.py:683-690: the re-deskew test, then (gated on it) deskew and register again from the first pose
-    LW_CK(lw::launch_frame_check(c));
     const int32_t *redo = &c->fctl->redo;
-    LW_CK(lw::launch_deskew(c, -1, in->t_mid, in->inv_dt, nullptr, c->fctl->xi1, redo));
-    LW_CK(lw::launch_thin(c, -1, inv_v_thin, p->min_range, p->max_range, p->ret_mask, redo));
-    LW_CK(lw::launch_reg_begin(c, (const double *)c->dtmp, redo, c->fctl->T1));
+    if (one) {
+        LW_CK(lw::launch_frame_prep(c, c->chain_tp, in->t_mid, in->inv_dt, nullptr, inv_v_thin, p->min_range,
+                                    p->max_range, p->ret_mask, true));
+    } else {
+        LW_CK(lw::launch_frame_check(c));
+        LW_CK(lw::launch_deskew(c, -1, in->t_mid, in->inv_dt, nullptr, c->fctl->xi1, redo));
+        LW_CK(lw::launch_thin(c, -1, inv_v_thin, p->min_range, p->max_range, p->ret_mask, redo));
+        LW_CK(lw::launch_reg_begin(c, c->chain_tp, redo, c->fctl->T1));
+    }
     LW_CK(lw::launch_register_loop(c, &c->ctr[5], cap, L, p->acc_mode, redo));
-    // odom.py:650-651, 712-727: the keyframe test, then (gated on it) the insert at the frame's pose
+    // odom.py:650-651, 712-727: the keyframe test, then (gated on it) the insert at the frame's pose, whose
+    // touched voxels get their precision at once (the parameters are this frame's) with the map's counts
     LW_CK(lw::launch_keyframe_check(c));
     const lw::Mat4f unused{};
+    const lw::PrecParams q{c->prec_voxel, c->prec_noise2, c->prec_tight2, c->prec_alpha};
     LW_CK(lw::launch_map_insert(c, c->f_xyz, c->f_attr, cap, &c->ctr[6], unused, (float)(1.0 / (double)p->reg_voxel),
-                                p->reg_voxel, p->max_range, p->ret_mask, &c->fctl->keyframe, c->fctl->T32));
+                                p->reg_voxel, p->max_range, p->ret_mask, &c->fctl->keyframe, c->fctl->T32, &q,
+                                c->chain_counts));
     // one readback: both registrations' states, the decisions and the map's counts after the insert
-    constexpr size_t S1 = (sizeof(lw::RegState) + 63) & ~(size_t)63, FC = 2 * S1,
-                     MC = FC + ((sizeof(lw::FrameCtl) + 63) & ~(size_t)63);
     char *h = pin_down(c);
-    LW_CK(hipMemcpyAsync(h, c->reg, sizeof(lw::RegState), hipMemcpyDeviceToHost, c->stream));
-    LW_CK(hipMemcpyAsync(h + S1, c->reg1, sizeof(lw::RegState), hipMemcpyDeviceToHost, c->stream));
-    LW_CK(hipMemcpyAsync(h + FC, c->fctl, sizeof(lw::FrameCtl), hipMemcpyDeviceToHost, c->stream));
-    LW_CK(hipMemcpyAsync(h + MC, c->ctr, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    LW_CK(hipMemcpyAsync(h, c->chain, CH_BYTES, hipMemcpyDeviceToHost, c->stream));
     LW_CK(hipStreamSynchronize(c->stream));
     LW_CK(prof_harvest(c));
     lw::RegState st2, st1;
     lw::FrameCtl fc;
     std::memcpy(&st2, h, sizeof(st2));
-    std::memcpy(&st1, h + S1, sizeof(st1));
-    std::memcpy(&fc, h + FC, sizeof(fc));
+    std::memcpy(&st1, h + CH_REG1, sizeof(st1));
+    std::memcpy(&fc, h + CH_FCTL, sizeof(fc));
+    if (one && fc.err) return LW_EINVAL;
     if (st1.status < 0 || (fc.redo && st2.status < 0)) return LW_ELOOP;
     out->m = fc.m;
     out->skipped = fc.skipped;
@@ -685,8 +712,8 @@ int lw_odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, lw_f
     out->registrations = fc.skipped ? 0 : 1 + fc.redo;
     std::memcpy(out->T, fc.T, sizeof(out->T));
     std::memcpy(out->xi, fc.xi1, sizeof(out->xi));
-    std::memcpy(&out->map_voxels, h + MC, sizeof(int32_t));
-    std::memcpy(&out->map_overflow, h + MC + sizeof(int32_t), sizeof(int32_t));
+    std::memcpy(&out->map_voxels, h + CH_COUNTS, sizeof(int32_t));
+    std::memcpy(&out->map_overflow, h + CH_COUNTS + sizeof(int32_t), sizeof(int32_t));
     if (out->registrations >= 1) {
         out->reg[0].m = fc.m;
         fill_result(st1, p, &out->reg[0]);

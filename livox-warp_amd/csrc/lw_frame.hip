@@ -9,6 +9,7 @@
 // clear of the whole table is needed.
 #include <climits>
 
+#include "lw_frame.h"
 #include "lw_internal.h"
 
 namespace lw {
@@ -96,8 +97,7 @@ __global__ __launch_bounds__(BLOCK) void k_append_write(const float *sx, const u
     f_frame[j] = k_open;
 }
 
-// gpu.py:268-284: rotate by the rotation vector theta s (unit quaternion), then add rho s.
-// xi_dev, when given, holds the twist (rho, theta) in place of the values passed.
+// gpu.py:268-284 (deskew_point). xi_dev, when given, holds the twist (rho, theta) in place of the values passed.
 __global__ __launch_bounds__(BLOCK) void k_deskew(const float *raw, const float *t, int32_t m, const int32_t *m_dev,
                                                   const int32_t *gate, const float *xi_dev, float t_mid, float inv_dt,
                                                   float r0, float r1, float r2, float h0, float h1, float h2,
@@ -113,28 +113,13 @@ __global__ __launch_bounds__(BLOCK) void k_deskew(const float *raw, const float 
         h1 = xi_dev[4];
         h2 = xi_dev[5];
     }
-    float s = (t[i] - t_mid) * inv_dt;
-    s = fminf(fmaxf(s, -1.0f), 1.0f);
-    float px = raw[3 * (int64_t)i], py = raw[3 * (int64_t)i + 1], pz = raw[3 * (int64_t)i + 2];
-    float vx = h0 * s, vy = h1 * s, vz = h2 * s;
-    float a = length3(vx, vy, vz);
-    float ox = px, oy = py, oz = pz;
-    if (!(a < 1.0e-9f)) {
-        float ax = vx / a, ay = vy / a, az = vz / a;
-        float sh, ch;
-        sincosf(a * 0.5f, &sh, &ch);
-        float qx = ax * sh, qy = ay * sh, qz = az * sh, qw = ch;
-        float c = 2.0f * qw * qw - 1.0f;
-        float d = 2.0f * (qx * px + qy * py + qz * pz);
-        float cx = qy * pz - qz * py, cy = qz * px - qx * pz, cz = qx * py - qy * px;
-        float w2 = 2.0f * qw;
-        ox = px * c + qx * d + cx * w2;
-        oy = py * c + qy * d + cy * w2;
-        oz = pz * c + qz * d + cz * w2;
-    }
-    out[3 * (int64_t)i] = ox + r0 * s;
-    out[3 * (int64_t)i + 1] = oy + r1 * s;
-    out[3 * (int64_t)i + 2] = oz + r2 * s;
+    const float xi[6] = {r0, r1, r2, h0, h1, h2};
+    float ox, oy, oz;
+    deskew_point(raw[3 * (int64_t)i], raw[3 * (int64_t)i + 1], raw[3 * (int64_t)i + 2], t[i], t_mid, inv_dt, xi, ox,
+                 oy, oz);
+    out[3 * (int64_t)i] = ox;
+    out[3 * (int64_t)i + 1] = oy;
+    out[3 * (int64_t)i + 2] = oz;
 }
 
 // Thinning 1: filters, key at the thinning voxel (sensor frame), claim, lowest index wins.
@@ -146,13 +131,11 @@ __global__ __launch_bounds__(BLOCK) void k_thin_claim(const float *xyz, const ui
     if (gated_off(gate)) return;
     int32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= count_of(m, m_dev)) return;
-    float px = xyz[3 * (int64_t)i], py = xyz[3 * (int64_t)i + 1], pz = xyz[3 * (int64_t)i + 2];
-    float r = length3(px, py, pz);
-    uint32_t a = attr[i];
     pt_slot[i] = -1;
-    if (r < min_range || r > max_range) return;
-    if (tag_of(a) != 0 || !ret_ok(a, ret_mask)) return;
-    int64_t key = voxel_key_ijk(ifloor(px * inv_v), ifloor(py * inv_v), ifloor(pz * inv_v));
+    int64_t key;
+    if (!thin_key(xyz[3 * (int64_t)i], xyz[3 * (int64_t)i + 1], xyz[3 * (int64_t)i + 2], attr[i], inv_v, min_range,
+                  max_range, ret_mask, key))
+        return;
     int32_t h = slot_hash(key) & ds_mask;
     for (int probe = 0; probe < PROBES; ++probe) {
         int32_t s = (h + probe) & ds_mask;

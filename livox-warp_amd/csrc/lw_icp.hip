@@ -905,27 +905,7 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_register(LoopArgs L) {
 // nothing when gate is given and holds 0.
 __global__ void k_reg_begin(RegState *st, const double *Tp, uint32_t *flags, const int32_t *gate, const double *T_dev) {
     if (gated_off(gate)) return;
-    if (threadIdx.x < 64) st->kappa[threadIdx.x] = reinterpret_cast<const float *>(Tp + 16)[threadIdx.x];
-    for (int k = threadIdx.x; k < MAX_LOOP_ROWS; k += blockDim.x) flags[k] = 0u;
-    if (threadIdx.x < 16) st->pub[threadIdx.x] = 0ull;
-    if (threadIdx.x != 0) return;
-    const double *P = T_dev ? T_dev : Tp;
-    for (int k = 0; k < 16; ++k) {
-        st->T[k] = P[k];
-        st->T_pred[k] = P[k];
-        st->Tf[k] = (float)P[k];
-    }
-    for (int k = 0; k < 36; ++k) st->H[k] = (k % 7 == 0) ? 1.0 : 0.0;  // np.eye(6) (odom.py:763)
-    st->rms = 0.0;
-    st->rms0 = 0.0;
-    st->cond = 1.0;
-    st->done = 0;
-    st->iters = 0;
-    st->corr = 0;
-    st->n_degen = 0;
-    st->status = 0;
-    st->ticket = 0;
-    st->n = 0;
+    reg_state_begin(st, T_dev ? T_dev : Tp, reinterpret_cast<const float *>(Tp + 16), flags, threadIdx.x, blockDim.x);
 }
 
 template <int MODE>

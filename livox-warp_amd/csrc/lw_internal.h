@@ -58,8 +58,14 @@ struct FrameCtl {
     float xi1[6];           // the twist the frame ends deskewed with
     float T32[16];          // the pose a keyframe insert uses
     int32_t m, skipped, ok1, redo, ok, keyframe;
+    int32_t err;            // 1: the frame held more points than the single-workgroup preparation takes
 };
 constexpr size_t FRAME_CTL_INPUT = offsetof(FrameCtl, T1);  // bytes the host uploads
+
+// The parameters the per-voxel precision is computed with (odom.py:336-368).
+struct PrecParams {
+    float voxel, noise2, tight2, alpha;
+};
 
 }  // namespace lw
 
@@ -107,6 +113,13 @@ struct lw_ctx {
     lw::RegState *reg = nullptr;
     lw::RegState *reg1 = nullptr;   // the first registration of a frame, kept by lw_odom_frame's check
     lw::FrameCtl *fctl = nullptr;   // lw_odom_frame's inputs and decisions
+    // reg, reg1, chain_tp (the prediction and robust-kernel scales lw_odom_frame uploads, right before fctl),
+    // fctl and chain_counts (the map's counts after lw_odom_frame's insert) live in this one block, so that
+    // lw_odom_frame uploads once and reads back once
+    char *chain = nullptr;
+    double *chain_tp = nullptr;
+    int32_t *chain_counts = nullptr;
+    int64_t frame_bound = 0;  // points appended since the last lw_frame_reset (at most frame_cap): bounds ctr[6]
     double *partials = nullptr;
     uint32_t *loop_flags = nullptr;  // MAX_LOOP_ROWS: the pass each searching workgroup's row belongs to
     int32_t *best_slot = nullptr;
@@ -142,7 +155,9 @@ struct lw_ctx {
     int32_t loop_capacity[3] = {0, 0, 0};  // workgroups of each form the device holds at once
     bool loop_ok = true;
     bool loop_plain = true;  // plain launches of the loop where its grid is at most half the capacity
-    // parameters the cached per-voxel precision was computed with
+    // parameters the cached per-voxel precision was computed with, and whether an insert may have queued
+    // voxels for a refresh since the last one
+    bool dirty_maybe = false;
     bool prec_valid = false;
     float prec_voxel = 0, prec_noise2 = 0, prec_tight2 = 0, prec_alpha = 0;
 };
@@ -159,13 +174,43 @@ __device__ inline Mat4f load_mat4f(const float *p) {
     return T;
 }
 
+// A registration's initial state (odom.py:758-763): the pose P (16 doubles) as current pose and prediction,
+// the per-iteration robust-kernel scale, H = I, counters cleared; with the persistent loop's row flags and
+// published pose cleared. Thread tid of nth takes part.
+__device__ inline void reg_state_begin(RegState *st, const double *P, const float *kappa, uint32_t *flags, int tid,
+                                       int nth) {
+    if (tid < 64) st->kappa[tid] = kappa[tid];
+    for (int k = tid; k < MAX_LOOP_ROWS; k += nth) flags[k] = 0u;
+    if (tid < 16) st->pub[tid] = 0ull;
+    if (tid != 0) return;
+    for (int k = 0; k < 16; ++k) {
+        st->T[k] = P[k];
+        st->T_pred[k] = P[k];
+        st->Tf[k] = (float)P[k];
+    }
+    for (int k = 0; k < 36; ++k) st->H[k] = (k % 7 == 0) ? 1.0 : 0.0;  // np.eye(6) (odom.py:763)
+    st->rms = 0.0;
+    st->rms0 = 0.0;
+    st->cond = 1.0;
+    st->done = 0;
+    st->iters = 0;
+    st->corr = 0;
+    st->n_degen = 0;
+    st->status = 0;
+    st->ticket = 0;
+    st->n = 0;
+}
+
 hipError_t launch_map_reset(lw_ctx *c);
 // n_dev: when given, the points to insert are its count capped at n (n sizes the grids). gate: when given, a
 // device flag that must be nonzero for the insert to happen. T_dev: the pose on the device (replaces T).
+// refresh: when given, the touched voxels' precision is computed at once with these parameters instead of
+// being queued for the next refresh. counts: when given, receives the map's counts (ctr[0..1]) after the insert.
 hipError_t launch_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, int32_t n, const int32_t *n_dev,
                              const Mat4f &T, float inv_v, float voxel, float max_range, int32_t ret_mask,
-                             const int32_t *gate = nullptr, const float *T_dev = nullptr);
-hipError_t launch_refresh(lw_ctx *c, bool all, float voxel, float noise2, float tight2, float alpha);
+                             const int32_t *gate = nullptr, const float *T_dev = nullptr,
+                             const PrecParams *refresh = nullptr, int32_t *counts = nullptr);
+hipError_t launch_refresh(lw_ctx *c, bool all, const PrecParams &q);
 hipError_t launch_map_find(lw_ctx *c, const int64_t *keys_dev, int32_t n, int32_t *out_dev);
 
 hipError_t launch_frame_append(lw_ctx *c, const float *sx, const uint32_t *sa, const float *st, int32_t n,
@@ -206,5 +251,12 @@ hipError_t launch_reg_begin(lw_ctx *c, const double *T_pred_dev, const int32_t *
 // lw_odom.hip: after a frame's first registration, and after its second
 hipError_t launch_frame_check(lw_ctx *c);
 hipError_t launch_keyframe_check(lw_ctx *c);
+// lw_odom.hip: deskew (with xi), thinning and the registration's start from Tp (the prediction and the
+// robust-kernel scales) for the open frame in one single-workgroup launch; the frame may hold at most
+// frame_prep_max() points. check: after the first registration -- the frame check first, then the rest
+// (deskew with the check's twist, the registration's start from its pose) only when it asks for a re-deskew.
+int32_t frame_prep_max();
+hipError_t launch_frame_prep(lw_ctx *c, const double *Tp, float t_mid, float inv_dt, const float *xi, float inv_v,
+                             float min_range, float max_range, int32_t ret_mask, bool check);
 
 }  // namespace lw

@@ -15,7 +15,8 @@
 //   3 quota    voxels that would pass the 4096-point cap (odom.py:225) rank their incoming points in
 //              index order, so exactly the first points contribute, as in a serial insert;
 //   4 accum    float32 atomic sums into the moments;
-//   5 finish   reset the per-insert counters, queue the touched voxels for a precision refresh.
+//   5 finish   reset the per-insert counters, queue the touched voxels for a precision refresh (or, when
+//              the caller passes the precision parameters, refresh them at once).
 #include <climits>
 
 #include "lw_internal.h"
@@ -215,17 +216,47 @@ __global__ __launch_bounds__(BLOCK) void k_ins_accum(const float *xyz, const uin
     atomicAdd(&m->cnt, 1);
 }
 
-// 5: finish: clear the per-insert counters, queue the touched voxels for a refresh.
+// The mean and precision of voxel vid from its moments (odom.py:336-368), into its slot and vox_prec.
+__device__ inline void refresh_voxel(int32_t vid, const int64_t *vox_key, const int32_t *vox_slot,
+                                     const VoxMom *vox_mom, Slot *slots, VoxPrec *vox_prec, const PrecParams &q) {
+    VoxMom m = vox_mom[vid];
+    Slot *sl = &slots[vox_slot[vid]];
+    if (m.cnt <= 0) {
+        sl->cnt = 0.0f;
+        return;
+    }
+    int32_t ix, iy, iz;
+    voxel_key_unpack(vox_key[vid], ix, iy, iz);
+    float mu[3];
+    voxel_mean(ix, iy, iz, m.s, m.cnt, q.voxel, mu);
+    sl->mu[0] = mu[0];
+    sl->mu[1] = mu[1];
+    sl->mu[2] = mu[2];
+    sl->cnt = (float)m.cnt;
+    VoxPrec P;
+    P.shape = (float)voxel_precision(m.s, m.o, m.cnt, q.voxel, q.noise2, q.tight2, q.alpha, P.M, P.Mg, P.q);
+    vox_prec[vid] = P;
+}
+
+// 5: finish: clear the per-insert counters; queue the touched voxels for a refresh, or (refresh set) refresh
+// them here with q. counts, when given, receives the map's counts (ctr[0], ctr[1]).
 __global__ __launch_bounds__(BLOCK) void k_ins_finish(const int32_t *touched, int32_t *ctr, int32_t *vox_nin,
                                                       int32_t *vox_quota, int32_t *vox_dirty, int32_t *dirty_list,
-                                                      const int32_t *gate) {
+                                                      const int32_t *gate, int32_t refresh, PrecParams q,
+                                                      const int64_t *vox_key, const int32_t *vox_slot,
+                                                      const VoxMom *vox_mom, Slot *slots, VoxPrec *vox_prec,
+                                                      int32_t *counts) {
     if (gated_off(gate)) return;
+    if (counts && blockIdx.x == 0 && threadIdx.x < 2) counts[threadIdx.x] = ctr[threadIdx.x];
     int32_t nt = ctr[4];
     for (int32_t j = blockIdx.x * BLOCK + threadIdx.x; j < nt; j += gridDim.x * BLOCK) {
         int32_t vid = touched[j];
         vox_nin[vid] = 0;
         vox_quota[vid] = INT_MAX;
-        if (atomicExch(&vox_dirty[vid], 1) == 0) dirty_list[atomicAdd(&ctr[3], 1)] = vid;
+        if (refresh)
+            refresh_voxel(vid, vox_key, vox_slot, vox_mom, slots, vox_prec, q);
+        else if (atomicExch(&vox_dirty[vid], 1) == 0)
+            dirty_list[atomicAdd(&ctr[3], 1)] = vid;
     }
 }
 
@@ -233,30 +264,13 @@ __global__ __launch_bounds__(BLOCK) void k_ins_finish(const int32_t *touched, in
 __global__ __launch_bounds__(BLOCK) void k_refresh(int32_t all, int32_t *ctr, const int32_t *dirty_list,
                                                    int32_t *vox_dirty, const int64_t *vox_key,
                                                    const int32_t *vox_slot, const VoxMom *vox_mom, Slot *slots,
-                                                   VoxPrec *vox_prec, float voxel, float noise2, float tight2,
-                                                   float alpha, int64_t vox_cap) {
+                                                   VoxPrec *vox_prec, PrecParams q, int64_t vox_cap) {
     int32_t cnt = all ? ctr[2] : ctr[3];
     if (cnt > vox_cap) cnt = (int32_t)vox_cap;
     for (int32_t j = blockIdx.x * BLOCK + threadIdx.x; j < cnt; j += gridDim.x * BLOCK) {
         int32_t vid = all ? j : dirty_list[j];
         vox_dirty[vid] = 0;
-        VoxMom m = vox_mom[vid];
-        Slot *sl = &slots[vox_slot[vid]];
-        if (m.cnt <= 0) {
-            sl->cnt = 0.0f;
-            continue;
-        }
-        int32_t ix, iy, iz;
-        voxel_key_unpack(vox_key[vid], ix, iy, iz);
-        float mu[3];
-        voxel_mean(ix, iy, iz, m.s, m.cnt, voxel, mu);
-        sl->mu[0] = mu[0];
-        sl->mu[1] = mu[1];
-        sl->mu[2] = mu[2];
-        sl->cnt = (float)m.cnt;
-        VoxPrec P;
-        P.shape = (float)voxel_precision(m.s, m.o, m.cnt, voxel, noise2, tight2, alpha, P.M, P.Mg, P.q);
-        vox_prec[vid] = P;
+        refresh_voxel(vid, vox_key, vox_slot, vox_mom, slots, vox_prec, q);
     }
 }
 
@@ -311,7 +325,7 @@ hipError_t launch_map_reset(lw_ctx *c) {
 
 hipError_t launch_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, int32_t n, const int32_t *n_dev,
                              const Mat4f &T, float inv_v, float voxel, float max_range, int32_t ret_mask,
-                             const int32_t *gate, const float *T_dev) {
+                             const int32_t *gate, const float *T_dev, const PrecParams *refresh, int32_t *counts) {
     if (n <= 0) return hipSuccess;
     int g = grid_for(n);
     hipLaunchKernelGGL(k_ins_claim, dim3(g), dim3(BLOCK), 0, c->stream, xyz, attr, n, n_dev, T, inv_v, max_range,
@@ -323,16 +337,20 @@ hipError_t launch_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, 
                        c->vox_mom, c->vox_quota, c->pt_vid, n, n_dev, c->pt_take, gate);
     hipLaunchKernelGGL(k_ins_accum, dim3(g), dim3(BLOCK), 0, c->stream, xyz, attr, n, n_dev, T, inv_v, voxel,
                        max_range, ret_mask, c->pt_vid, c->vox_quota, c->pt_take, c->vox_mom, gate, T_dev);
+    const PrecParams q = refresh ? *refresh : PrecParams{};
     hipLaunchKernelGGL(k_ins_finish, dim3(256), dim3(BLOCK), 0, c->stream, c->touched, c->ctr, c->vox_nin,
-                       c->vox_quota, c->vox_dirty, c->dirty_list, gate);
+                       c->vox_quota, c->vox_dirty, c->dirty_list, gate, refresh ? 1 : 0, q, c->vox_key, c->vox_slot,
+                       c->vox_mom, c->slots, c->vox_prec, counts);
+    if (!refresh) c->dirty_maybe = true;
     return hipGetLastError();
 }
 
-hipError_t launch_refresh(lw_ctx *c, bool all, float voxel, float noise2, float tight2, float alpha) {
+hipError_t launch_refresh(lw_ctx *c, bool all, const PrecParams &q) {
     hipLaunchKernelGGL(k_refresh, dim3(all ? 4096 : 512), dim3(BLOCK), 0, c->stream, all ? 1 : 0, c->ctr,
-                       c->dirty_list, c->vox_dirty, c->vox_key, c->vox_slot, c->vox_mom, c->slots, c->vox_prec, voxel, noise2,
-                       tight2, alpha, c->vox_cap);
+                       c->dirty_list, c->vox_dirty, c->vox_key, c->vox_slot, c->vox_mom, c->slots, c->vox_prec, q,
+                       c->vox_cap);
     hipLaunchKernelGGL(k_set_ctr, dim3(1), dim3(1), 0, c->stream, c->ctr, 3, 0);
+    c->dirty_maybe = false;
     return hipGetLastError();
 }
 

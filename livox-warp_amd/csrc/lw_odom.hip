@@ -3,6 +3,9 @@
 // the keyframe test and insert -- is queued at once and read back once (lw_odom_frame in lw_abi.hip). Each
 // decision is one lane of a tiny launch that reads the registration state and writes a FrameCtl flag the
 // next kernels are gated on.
+#include <climits>
+
+#include "lw_frame.h"
 #include "lw_gn.h"
 #include "lw_internal.h"
 
@@ -62,14 +65,9 @@ __device__ inline bool reg_outcome(const RegState *st, const double *T_pred, int
     return has && st->corr >= min_corr;
 }
 
-// After the first registration: keep a copy of its state, then (one lane) the frame's count and skip test
-// (odom.py:676-678) and the re-deskew test (odom.py:683-690) on xi_new = body twist of T_prev^-1 T.
-__global__ void k_frame_check(const RegState *st, RegState *st1, FrameCtl *fc, const int32_t *ctr) {
-    const uint32_t *src = reinterpret_cast<const uint32_t *>(st);
-    uint32_t *dst = reinterpret_cast<uint32_t *>(st1);
-    for (size_t k = threadIdx.x; k < sizeof(RegState) / 4; k += blockDim.x) dst[k] = src[k];
-    if (threadIdx.x != 0) return;
-    const int32_t cnt = ctr[6];
+// After the first registration (one lane): the frame's count and skip test (odom.py:676-678) and the re-deskew
+// test (odom.py:683-690) on xi_new = body twist of T_prev^-1 T. cnt: the frame's appended points.
+__device__ inline void frame_check(const RegState *st, FrameCtl *fc, int32_t cnt) {
     const int32_t m = cnt < fc->frame_cap ? cnt : fc->frame_cap;
     fc->m = m;
     fc->skipped = m < fc->min_points ? 1 : 0;
@@ -97,9 +95,25 @@ __global__ void k_frame_check(const RegState *st, RegState *st1, FrameCtl *fc, c
     for (int k = 0; k < 6; ++k) fc->xi1[k] = redo ? xn[k] : fc->xi0[k];
 }
 
+// Keep the first registration's state in st1 (the second registration reuses st); every thread of the block
+// takes part.
+__device__ inline void keep_state(const RegState *st, RegState *st1) {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(st);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(st1);
+    for (size_t k = threadIdx.x; k < sizeof(RegState) / 4; k += blockDim.x) dst[k] = src[k];
+}
+
+__global__ void k_frame_check(const RegState *st, RegState *st1, FrameCtl *fc, const int32_t *ctr) {
+    keep_state(st, st1);
+    if (threadIdx.x == 0) frame_check(st, fc, ctr[6]);
+}
+
 // After the (possible) second registration: the frame's pose and trust, and the keyframe test
-// (odom.py:650-651, 712-717) against the last keyframe, with the float32 pose an insert uses.
-__global__ void k_keyframe_check(const RegState *st, FrameCtl *fc) {
+// (odom.py:650-651, 712-717) against the last keyframe, with the float32 pose an insert uses. counts gets the
+// map's counts (ctr[0..1]) as they stand before the insert (which, when it runs, writes them again after).
+__global__ void k_keyframe_check(const RegState *st, FrameCtl *fc, const int32_t *ctr, int32_t *counts) {
+    counts[0] = ctr[0];
+    counts[1] = ctr[1];
     double T[16];
     bool ok;
     if (fc->redo) {
@@ -129,7 +143,288 @@ __global__ void k_keyframe_check(const RegState *st, FrameCtl *fc) {
     fc->keyframe = key ? 1 : 0;
 }
 
+// ---- one workgroup: deskew, thinning and the registration's start ------------------------------------
+//
+// A frame of at most PREP_MAX points is deskewed and thinned by one workgroup of PREP_THREADS threads in one
+// launch, instead of the grid kernels' five (deskew, claim, count, scan, write), and the registration state is
+// set up in the same launch. Point i is handled by thread i mod PREP_THREADS in round i / PREP_THREADS, so
+// the rounds cover the points in index order. The thinning claims keys in a table in LDS (PREP_SLOTS slots,
+// linear probing over PROBES slots like the global table), where each key keeps its lowest point index; the
+// winners are then compacted in index order from per-(round, wave) counts. Where no key is dropped (every key
+// finds a slot within PROBES of its home), the result does not depend on the table -- one point per thinning
+// voxel, the one of lowest index, in index order -- so the LDS table uses its own hash over every key bit (the
+// global hash's low bits leave out the z index: a column's voxels would share a home). The workgroup also
+// shows that the global table drops nothing: it counts the keys homed in each aligned block of PREP_BLOCK
+// global slots (block indices folded onto PREP_COUNTERS counters, which can only raise the counts), and at
+// most PREP_BLOCK_KEYS homes per block leave every cluster of the global table shorter than 64 slots (a run of
+// L >= 64 full slots holds L keys homed inside it, but meets at most (L - 1) / PREP_BLOCK + 2 blocks, which
+// hold at most (L - 1) / 2 + 2 PREP_BLOCK_KEYS < L keys), so no key probes past its 64th slot, in any
+// insertion order. When the check fails, or a key finds no free slot
+// in the LDS table, the workgroup runs the claims again in the global table, exactly as the grid kernels do.
+constexpr int PREP_THREADS = 1024;
+constexpr int PREP_ROUNDS = 32;
+constexpr int PREP_SLOTS = 8192;
+constexpr int PREP_WAVES = PREP_THREADS / 64;
+constexpr int PREP_BATCH = 4;  // rounds whose loads are issued together
+constexpr int PREP_BLOCK = 32, PREP_BLOCK_KEYS = 16, PREP_COUNTERS = 8192;
+static_assert(2 * PREP_BLOCK_KEYS <= PREP_BLOCK && 4 * PREP_BLOCK_KEYS <= PROBES, "no cluster of PROBES slots");
+static_assert(PREP_ROUNDS % PREP_BATCH == 0 && PREP_ROUNDS <= 32, "one bit per round");
+
+struct PrepArgs {
+    const float *raw, *t;  // the frame: raw points and times
+    const uint32_t *attr;
+    float *xyz;            // the deskewed points
+    const int32_t *m_dev;  // the frame's appended points (capped at frame_cap)
+    int32_t frame_cap;
+    float t_mid, inv_dt;
+    float xi[6];           // the deskew twist (the first preparation)
+    float inv_v, min_range, max_range;
+    int32_t ret_mask;
+    int32_t ds_mask;       // the global thinning table (the fallback)
+    int64_t *ds_keys;
+    int32_t *ds_idx;
+    float *ds_xyz;         // the thinned points, in index order
+    int32_t *n_out;        // their count
+    RegState *st;          // the registration state to set up
+    const double *Tp;      // the prediction (16 doubles) and the robust-kernel scales (64 floats)
+    uint32_t *flags;       // the persistent loop's row flags
+    RegState *st1;         // given: the frame check runs first, and the rest only when it asks for a re-deskew
+    FrameCtl *fc;
+};
+
+// The claim of one key in the LDS table: the slot, or -1 when none is free within PROBES slots; fresh when
+// this claim put the key in.
+__device__ inline int32_t lds_claim(int64_t *tkey, int32_t *tidx, int64_t key, int32_t i, bool &fresh) {
+    const int32_t h = (int32_t)(((uint64_t)key * HASH_MUL) >> (64 - 13));
+    static_assert(PREP_SLOTS == 1 << 13, "the LDS hash takes 13 bits");
+    fresh = false;
+    for (int probe = 0; probe < PROBES; ++probe) {
+        const int32_t s = (h + probe) & (PREP_SLOTS - 1);
+        const unsigned long long prev =
+            atomicCAS(reinterpret_cast<unsigned long long *>(&tkey[s]), ~0ull, (unsigned long long)key);
+        if ((int64_t)prev == -1 || (int64_t)prev == key) {
+            atomicMin(&tidx[s], i);
+            fresh = (int64_t)prev == -1;
+            return s;
+        }
+    }
+    return -1;
+}
+
+// The claim of one key in the global table (k_thin_claim's probe).
+__device__ inline int32_t global_claim(int64_t *ds_keys, int32_t *ds_idx, int32_t ds_mask, int64_t key, int32_t i) {
+    const int32_t h = slot_hash(key) & ds_mask;
+    for (int probe = 0; probe < PROBES; ++probe) {
+        const int32_t s = (h + probe) & ds_mask;
+        const unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long *>(&ds_keys[s]),
+                                                  0xFFFFFFFFFFFFFFFFull, (unsigned long long)key);
+        if ((int64_t)prev == -1 || (int64_t)prev == key) {
+            atomicMin(&ds_idx[s], i);
+            return s;
+        }
+    }
+    return -1;
+}
+
+__global__ __launch_bounds__(PREP_THREADS) void k_frame_prep(PrepArgs A) {
+    __shared__ int64_t tkey[PREP_SLOTS];
+    __shared__ int32_t tidx[PREP_SLOTS];
+    __shared__ int32_t cnt[PREP_ROUNDS * PREP_WAVES];
+    __shared__ int32_t blk[PREP_COUNTERS];
+    __shared__ double s_T[16];
+    __shared__ float s_xi[6];
+    __shared__ int32_t s_go, s_over;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (A.st1) {  // after the first registration: keep its state; the skip and re-deskew tests
+        keep_state(A.st, A.st1);
+        if (tid == 0) {
+            frame_check(A.st, A.fc, *A.m_dev);
+            s_go = A.fc->redo;
+            for (int k = 0; k < 16; ++k) s_T[k] = A.fc->T1[k];
+            for (int k = 0; k < 6; ++k) s_xi[k] = A.fc->xi1[k];
+        }
+    } else if (tid == 0) {
+        s_go = 1;
+        A.fc->err = 0;
+        for (int k = 0; k < 16; ++k) s_T[k] = A.Tp[k];
+        for (int k = 0; k < 6; ++k) s_xi[k] = A.xi[k];
+    }
+    for (int s = tid; s < PREP_SLOTS; s += PREP_THREADS) {
+        tkey[s] = -1;
+        tidx[s] = INT_MAX;
+    }
+    for (int s = tid; s < PREP_COUNTERS; s += PREP_THREADS) blk[s] = 0;
+    if (tid == 0) s_over = 0;
+    __syncthreads();
+    if (!s_go) return;
+    reg_state_begin(A.st, s_T, reinterpret_cast<const float *>(A.Tp + 16), A.flags, tid, PREP_THREADS);
+    const int32_t m = count_of(A.frame_cap, A.m_dev);
+    if (m > PREP_THREADS * PREP_ROUNDS) {  // more than the caller's bound: thin nothing, and say so
+        if (tid == 0) {
+            A.fc->err = 1;
+            *A.n_out = 0;
+        }
+        return;
+    }
+    const int rounds = (m + PREP_THREADS - 1) / PREP_THREADS;
+    float xi[6];
+    for (int k = 0; k < 6; ++k) xi[k] = s_xi[k];
+    // deskew, filters, claims in LDS; each point's slot stays in a register (slot[r] for round r) and the
+    // loads of PREP_BATCH rounds are issued together
+    int32_t slot[PREP_ROUNDS];
+    LW_UNROLL for (int r = 0; r < PREP_ROUNDS; ++r) slot[r] = -1;
+    LW_UNROLL for (int b = 0; b < PREP_ROUNDS; b += PREP_BATCH) {
+        if (b >= rounds) break;
+        float px[PREP_BATCH], py[PREP_BATCH], pz[PREP_BATCH], pt[PREP_BATCH];
+        uint32_t pa[PREP_BATCH];
+        LW_UNROLL for (int u = 0; u < PREP_BATCH; ++u) {
+            const int32_t i = (b + u) * PREP_THREADS + tid;
+            const int32_t ic = i < m ? i : m - 1;  // m >= 1 here; a clamped load is not used
+            px[u] = A.raw[3 * ic];
+            py[u] = A.raw[3 * ic + 1];
+            pz[u] = A.raw[3 * ic + 2];
+            pt[u] = A.t[ic];
+            pa[u] = A.attr[ic];
+        }
+        LW_UNROLL for (int u = 0; u < PREP_BATCH; ++u) {
+            const int32_t i = (b + u) * PREP_THREADS + tid;
+            bool v = false;
+            int64_t key = 0;
+            if (i < m) {
+                float qx, qy, qz;
+                deskew_point(px[u], py[u], pz[u], pt[u], A.t_mid, A.inv_dt, xi, qx, qy, qz);
+                A.xyz[3 * i] = qx;
+                A.xyz[3 * i + 1] = qy;
+                A.xyz[3 * i + 2] = qz;
+                v = thin_key(qx, qy, qz, pa[u], A.inv_v, A.min_range, A.max_range, A.ret_mask, key);
+            }
+            if (v) {
+                bool fresh;
+                slot[b + u] = lds_claim(tkey, tidx, key, i, fresh);
+                // no free slot, or one block of the global table with more homes than the bound allows
+                if (slot[b + u] < 0 ||
+                    (fresh && atomicAdd(&blk[((slot_hash(key) & A.ds_mask) / PREP_BLOCK) & (PREP_COUNTERS - 1)],
+                                        1) == PREP_BLOCK_KEYS))
+                    s_over = 1;
+            }
+        }
+    }
+    __syncthreads();
+    const bool glob = s_over != 0;  // not shown drop-free: the claims again, in the global table
+    if (glob) {
+        LW_UNROLL for (int r = 0; r < PREP_ROUNDS; ++r) {
+            if (r >= rounds) break;
+            const int32_t i = r * PREP_THREADS + tid;
+            slot[r] = -1;
+            int64_t key;
+            if (i < m && thin_key(A.xyz[3 * i], A.xyz[3 * i + 1], A.xyz[3 * i + 2], A.attr[i], A.inv_v, A.min_range,
+                                  A.max_range, A.ret_mask, key))
+                slot[r] = global_claim(A.ds_keys, A.ds_idx, A.ds_mask, key, i);
+        }
+        __threadfence();
+        __syncthreads();
+    }
+    // winners (the lowest index of each claimed slot), counted per round and wave
+    uint32_t win = 0u;
+    LW_UNROLL for (int r = 0; r < PREP_ROUNDS; ++r) {
+        if (r >= rounds) break;
+        const int32_t i = r * PREP_THREADS + tid;
+        const int32_t s = slot[r];
+        bool w = false;
+        if (s >= 0)
+            w = (glob ? __hip_atomic_load(&A.ds_idx[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : tidx[s]) == i;
+        win |= (w ? 1u : 0u) << r;
+        const unsigned long long bw = __ballot(w);
+        if (lane == 0) cnt[r * PREP_WAVES + wave] = __popcll(bw);
+    }
+    __syncthreads();
+    // exclusive prefix over (round, wave) in that order: wave 0, eight entries a lane
+    if (wave == 0) {
+        constexpr int E = PREP_ROUNDS * PREP_WAVES / 64;
+        const int n = rounds * PREP_WAVES;
+        int32_t v[E];
+        int32_t sum = 0;
+        for (int e = 0; e < E; ++e) {
+            const int idx = lane * E + e;
+            v[e] = idx < n ? cnt[idx] : 0;
+            sum += v[e];
+        }
+        int32_t incl = sum;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int32_t y = __shfl_up(incl, d);
+            if (lane >= d) incl += y;
+        }
+        int32_t run = incl - sum;
+        for (int e = 0; e < E; ++e) {
+            const int idx = lane * E + e;
+            if (idx < n) cnt[idx] = run;
+            run += v[e];
+        }
+        if (lane == 63) *A.n_out = incl;
+    }
+    __syncthreads();
+    // the winners in index order, PREP_BATCH rounds' loads at a time
+    LW_UNROLL for (int b = 0; b < PREP_ROUNDS; b += PREP_BATCH) {
+        if (b >= rounds) break;
+        float x[PREP_BATCH], y[PREP_BATCH], z[PREP_BATCH];
+        LW_UNROLL for (int u = 0; u < PREP_BATCH; ++u) {
+            const int32_t i = (b + u) * PREP_THREADS + tid;
+            if ((win >> (b + u)) & 1u) {
+                x[u] = A.xyz[3 * i];
+                y[u] = A.xyz[3 * i + 1];
+                z[u] = A.xyz[3 * i + 2];
+            }
+        }
+        LW_UNROLL for (int u = 0; u < PREP_BATCH; ++u) {
+            const bool w = ((win >> (b + u)) & 1u) != 0u;
+            const unsigned long long bw = __ballot(w);
+            if (w) {
+                const int32_t j = cnt[(b + u) * PREP_WAVES + wave] + __popcll(bw & ((1ull << lane) - 1ull));
+                A.ds_xyz[3 * j] = x[u];
+                A.ds_xyz[3 * j + 1] = y[u];
+                A.ds_xyz[3 * j + 2] = z[u];
+                if (glob) {  // the global table is left clean, as the grid kernels leave it
+                    A.ds_keys[slot[b + u]] = -1;
+                    A.ds_idx[slot[b + u]] = INT_MAX;
+                }
+            }
+        }
+    }
+}
+
 }  // namespace
+
+int32_t frame_prep_max() { return PREP_THREADS * PREP_ROUNDS; }
+
+hipError_t launch_frame_prep(lw_ctx *c, const double *Tp, float t_mid, float inv_dt, const float *xi, float inv_v,
+                             float min_range, float max_range, int32_t ret_mask, bool check) {
+    PrepArgs A{};
+    A.raw = c->f_raw;
+    A.t = c->f_t;
+    A.attr = c->f_attr;
+    A.xyz = c->f_xyz;
+    A.m_dev = &c->ctr[6];
+    A.frame_cap = (int32_t)c->frame_cap;
+    A.t_mid = t_mid;
+    A.inv_dt = inv_dt;
+    for (int k = 0; k < 6; ++k) A.xi[k] = xi ? xi[k] : 0.0f;
+    A.inv_v = inv_v;
+    A.min_range = min_range;
+    A.max_range = max_range;
+    A.ret_mask = ret_mask;
+    A.ds_mask = (int32_t)(c->ds_cap - 1);
+    A.ds_keys = c->ds_keys;
+    A.ds_idx = c->ds_idx;
+    A.ds_xyz = c->ds_xyz;
+    A.n_out = &c->ctr[5];
+    A.st = c->reg;
+    A.Tp = Tp;
+    A.flags = c->loop_flags;
+    A.st1 = check ? c->reg1 : nullptr;
+    A.fc = c->fctl;
+    hipLaunchKernelGGL(k_frame_prep, dim3(1), dim3(PREP_THREADS), 0, c->stream, A);
+    return hipGetLastError();
+}
 
 hipError_t launch_frame_check(lw_ctx *c) {
     hipLaunchKernelGGL(k_frame_check, dim3(1), dim3(64), 0, c->stream, c->reg, c->reg1, c->fctl, c->ctr);
@@ -137,7 +432,7 @@ hipError_t launch_frame_check(lw_ctx *c) {
 }
 
 hipError_t launch_keyframe_check(lw_ctx *c) {
-    hipLaunchKernelGGL(k_keyframe_check, dim3(1), dim3(1), 0, c->stream, c->reg, c->fctl);
+    hipLaunchKernelGGL(k_keyframe_check, dim3(1), dim3(1), 0, c->stream, c->reg, c->fctl, c->ctr, c->chain_counts);
     return hipGetLastError();
 }
 

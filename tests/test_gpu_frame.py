@@ -79,3 +79,65 @@ def test_thinning_matches_serial(lw, orc, xi, gpu_ok):
     # the table is left clean: the same call again gives the same result
     n2 = eng.thin(m, 2.0 / 0.2, 0.3, 60.0, 0b1111)
     assert np.array_equal(eng.thin_read(n2), orc.ds_insert(desk, attr, 10.0, 0.3, 60.0, 0b1111, 1 << 14))
+
+
+@pytest.mark.parametrize("case", ["scan", "scattered", "large"])
+def test_frame_chain_preparation(lw, orc, case, gpu_ok):
+    """The deskew and thinning lw_odom_frame runs before its first registration: for a frame of up to 32768
+    points one workgroup does both (its thinning table in LDS; "scattered" holds more thinning voxels than that
+    table, so the workgroup takes the global table instead), for a larger frame ("large") the grid kernels.
+    The deskewed frame is the grid deskew kernel's bit for bit, the thinned points are the oracle's serial
+    thinning (winners and order) bit for bit, and the global thinning table is left clean."""
+    import ctypes
+
+    rng = np.random.default_rng({"scan": 1, "scattered": 2, "large": 3}[case])
+    if case == "scan":
+        seq = lw.synth.Sequence(seed=23, seconds=1.0)
+        xyz, attr, t = next(seq.frames(1))
+        t = (t - t.min()).astype(np.float32) * np.float32(0.9)
+        cap = 1 << 15
+    else:
+        n = 20000 if case == "scattered" else 40000
+        xyz = rng.uniform(-45.0, 45.0, size=(n, 3)).astype(np.float32)
+        attr = (rng.integers(0, 4, n).astype(np.uint32) << np.uint32(16))
+        t = np.sort(rng.uniform(0.0, 0.0999, n)).astype(np.float32)
+        cap = 1 << 15 if case == "scattered" else 1 << 16
+    attr = attr.copy()
+    attr[::37] |= np.uint32(1 << 8)  # tagged points take no part in the thinning
+    m = len(t)
+    assert (m <= 32768) == (case != "large")
+    eng = lw.Engine(1 << 18, cap)
+    eng.insert(xyz, attr, np.eye(4), 0.2, max_range=60.0)  # something to register against
+    eng.frame_reset()
+    eng.frame_append(_dev(xyz, torch.float32), _dev(attr.view(np.int32), torch.int32), _dev(t, torch.float32),
+                     10.0, -1, 0)
+    xi0 = np.array([0.05, -0.02, 0.01, 0.01, -0.02, 0.06], np.float32)
+    p = eng.params(reg_voxel=0.2, max_iter=3, min_corr=10)
+    fi, fo = lw._abi.FrameIn(), lw._abi.FrameOut()
+    np.ctypeslib.as_array(fi.T_pred)[:] = np.eye(4).ravel()
+    np.ctypeslib.as_array(fi.T_prev_inv)[:] = np.eye(4).ravel()
+    np.ctypeslib.as_array(fi.T_key_inv)[:] = np.eye(4).ravel()
+    np.ctypeslib.as_array(fi.xi)[:] = xi0
+    fi.kf_dist, fi.kf_angle, fi.redeskew_rot, fi.redeskew_trans = 1e9, 1e9, 1e9, 1e9
+    fi.t_mid, fi.inv_dt = 0.05, 10.0
+    fi.min_points, fi.has_prev, fi.has_key, fi.redeskew = 1, 0, 1, 0
+    lw._abi.check(eng.lib.lw_odom_frame(eng.ctx, ctypes.byref(p), ctypes.byref(fi), ctypes.byref(fo)),
+                  "lw_odom_frame")
+    assert fo.m == m and fo.registrations == 1 and not fo.keyframe
+    torch.cuda.synchronize()
+    desk = eng.f_xyz[:m].cpu().numpy().copy()
+    n_ds = int(fo.reg[0].n_ds)
+    got = eng.thin_read(n_ds)
+    ds_cap = 1 << (cap - 1).bit_length()
+    ref = orc.ds_insert(desk, attr, 2.0 / 0.2, p.min_range, p.max_range, p.ret_mask, ds_cap)
+    print(f"{case}: {m} points, {n_ds} thinned")
+    assert n_ds == len(ref) and np.array_equal(got, ref)
+    # the grid deskew from the same raw points, and the grid thinning on a clean table
+    eng.deskew(-1, 0.05, 10.0, xi0)
+    torch.cuda.synchronize()
+    assert np.array_equal(eng.f_xyz[:m].cpu().numpy(), desk)
+    n2 = eng.thin(m, 2.0 / 0.2, p.min_range, p.max_range, p.ret_mask)
+    assert n2 == n_ds and np.array_equal(eng.thin_read(n2), ref)
+    if case == "scattered":
+        assert n_ds > 8192  # more thinning voxels than the workgroup's table
+    eng.close()
