@@ -103,6 +103,13 @@ hipError_t ensure_precision(lw_ctx *c, float voxel, float noise2, float tight2, 
     return e;
 }
 
+// Apply a deferred lw_frame_reset before anything reads the frame's count (ctr[6]).
+hipError_t frame_count_ready(lw_ctx *c) {
+    if (!c->count_reset) return hipSuccess;
+    c->count_reset = false;
+    return lw::launch_fill_i32(c, &c->ctr[6], 1, 0);
+}
+
 int read_ctr(lw_ctx *c, int idx, int32_t *out) {
     LW_CK(hipMemcpyAsync(pin_down(c), &c->ctr[idx], sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     LW_CK(hipStreamSynchronize(c->stream));
@@ -223,7 +230,7 @@ int lw_map_reset(lw_ctx *c) {
 
 int lw_frame_reset(lw_ctx *c) {
     if (!c) return LW_EINVAL;
-    LW_CK(lw::launch_fill_i32(c, &c->ctr[6], 1, 0));
+    c->count_reset = true;  // applied by the next append (which then starts its count from 0) or reader
     c->frame_bound = 0;
     return 0;
 }
@@ -235,7 +242,8 @@ int lw_frame_append(lw_ctx *c, const float *sx, const uint32_t *sa, const float 
     const int64_t chunk = c->max_blocks * (int64_t)lw::BLOCK;
     for (int64_t a = 0; a < n; a += chunk) {  // chunks in order keep the points in input order
         int32_t m = (int32_t)((n - a) < chunk ? (n - a) : chunk);
-        LW_CK(lw::launch_frame_append(c, sx + 3 * a, sa + a, st + a, m, inv_dt, k_min, k_open));
+        LW_CK(lw::launch_frame_append(c, sx + 3 * a, sa + a, st + a, m, inv_dt, k_min, k_open, c->count_reset));
+        c->count_reset = false;
     }
     c->frame_bound = c->frame_bound + n < c->frame_cap ? c->frame_bound + n : c->frame_cap;
     return 0;
@@ -243,12 +251,14 @@ int lw_frame_append(lw_ctx *c, const float *sx, const uint32_t *sa, const float 
 
 int lw_frame_count(lw_ctx *c, int32_t *out) {
     if (!c || !out) return LW_EINVAL;
+    LW_CK(frame_count_ready(c));
     return read_ctr(c, 6, out);
 }
 
 int lw_deskew(lw_ctx *c, int32_t m, float t_mid, float inv_dt, const float *xi) {
     if (!c || !xi || m > c->frame_cap) return LW_EINVAL;
     if (!c->f_raw) return LW_ENOTBOUND;
+    LW_CK(frame_count_ready(c));
     LW_CK(lw::launch_deskew(c, m, t_mid, inv_dt, xi));
     return 0;
 }
@@ -277,6 +287,7 @@ int lw_thin_load(lw_ctx *c, const float *xyz, int32_t n) {
 int lw_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, int64_t n, const float *T, float inv_v,
                   float voxel, float max_range, int32_t ret_mask) {
     if (!c || !T) return LW_EINVAL;
+    LW_CK(frame_count_ready(c));
     const int32_t *n_dev = nullptr;
     if (!xyz && !attr) {
         if (!c->f_xyz) return LW_ENOTBOUND;
@@ -616,6 +627,7 @@ int lw_set_persistent(lw_ctx *c, int32_t on) {
 int lw_register(lw_ctx *c, int32_t m, const lw_reg_params *p, const double *T_pred, lw_reg_result *res) {
     if (!c || bad_params(p) || !T_pred || !res || m > c->frame_cap) return LW_EINVAL;
     if (!c->f_raw) return LW_ENOTBOUND;
+    LW_CK(frame_count_ready(c));
     // m < 0: the open frame's count on the device (thinning reads it there; the result carries it back)
     LW_CK(lw::launch_thin(c, m < 0 ? -1 : m, (float)(2.0 / (double)p->reg_voxel), p->min_range, p->max_range,
                           p->ret_mask));
@@ -627,6 +639,7 @@ int lw_odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, lw_f
     if (!c || bad_params(p) || !in || !out) return LW_EINVAL;
     if (!c->f_raw) return LW_ENOTBOUND;
     if (!c->loop_ok) return LW_ENOLOOP;
+    LW_CK(frame_count_ready(c));
     std::memset(out, 0, sizeof(*out));
     lw::RegLoop L;
     if (int rc = reg_setup(c, p, L)) return rc;

@@ -44,4 +44,36 @@ __device__ inline bool thin_key(float px, float py, float pz, uint32_t a, float 
     return true;
 }
 
+// The single-workgroup kernels (frame preparation, batch append) count their kept points per (round, wave):
+// cnt[r * 16 + w] for round r < rounds and wave w of a 1024-thread workgroup. Wave 0 (every lane) turns the
+// counts into exclusive offsets in (round, wave) order, in place, and returns the total in every lane.
+constexpr int ONE_THREADS = 1024;
+constexpr int ONE_WAVES = ONE_THREADS / 64;
+constexpr int ONE_ROUNDS = 32;  // points a thread takes: at most ONE_THREADS * ONE_ROUNDS points a launch
+
+__device__ inline int32_t scan_round_counts(int32_t *cnt, int rounds) {
+    constexpr int E = ONE_ROUNDS * ONE_WAVES / 64;
+    const int lane = threadIdx.x & 63;
+    const int n = rounds * ONE_WAVES;
+    int32_t v[E];
+    int32_t sum = 0;
+    for (int e = 0; e < E; ++e) {
+        const int idx = lane * E + e;
+        v[e] = idx < n ? cnt[idx] : 0;
+        sum += v[e];
+    }
+    int32_t incl = sum;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int32_t y = __shfl_up(incl, d);
+        if (lane >= d) incl += y;
+    }
+    int32_t run = incl - sum;
+    for (int e = 0; e < E; ++e) {
+        const int idx = lane * E + e;
+        if (idx < n) cnt[idx] = run;
+        run += v[e];
+    }
+    return __shfl(incl, 63);
+}
+
 }  // namespace lw

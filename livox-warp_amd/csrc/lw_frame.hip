@@ -64,11 +64,90 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_blocks(const int32_t *cnt
     if (t == 0) *counter = base + part[SCAN_THREADS - 1];
 }
 
-__device__ inline bool append_pred(const float *st, int32_t i, float inv_dt, int32_t k_min, int32_t k_open) {
-    float t = st[i];
+// A point of time t belongs to the open frame: its frame index floor(max(t, 0) / dt) lies in [k_min, k_open].
+__device__ inline bool frame_pred(float t, float inv_dt, int32_t k_min, int32_t k_open) {
     float tt = t > 0.0f ? t : 0.0f;  // wp.max(t, 0.0)
     int32_t k = ifloor(tt * inv_dt);
     return !(k > k_open || k < k_min);
+}
+
+__device__ inline bool append_pred(const float *st, int32_t i, float inv_dt, int32_t k_min, int32_t k_open) {
+    return frame_pred(st[i], inv_dt, k_min, k_open);
+}
+
+// One workgroup appends a batch of at most ONE_THREADS * ONE_ROUNDS points in one launch (the grid path's
+// count, scan and write): point i is taken by thread i mod ONE_THREADS in round i / ONE_THREADS, the kept
+// points are counted per (round, wave), and written in input order after the frame's count, which the launch
+// then advances (reset: the count starts from 0); points past cap are counted and not written, as in the grid
+// path.
+constexpr int APPEND_BATCH = 8;  // rounds whose loads are issued together
+
+__global__ __launch_bounds__(ONE_THREADS) void k_append_one(const float *sx, const uint32_t *sa, const float *st,
+                                                            int32_t n, float inv_dt, int32_t k_min, int32_t k_open,
+                                                            int32_t *count, int32_t reset, int32_t cap,
+                                                            float *f_raw, uint32_t *f_attr, float *f_t,
+                                                            int32_t *f_frame) {
+    __shared__ int32_t cnt[ONE_ROUNDS * ONE_WAVES];
+    __shared__ int32_t s_base;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int rounds = (n + ONE_THREADS - 1) / ONE_THREADS;  // the caller keeps n within ONE_ROUNDS rounds
+    uint32_t keep = 0u;
+    LW_UNROLL for (int b = 0; b < ONE_ROUNDS; b += APPEND_BATCH) {
+        if (b >= rounds) break;
+        float tv[APPEND_BATCH];
+        LW_UNROLL for (int u = 0; u < APPEND_BATCH; ++u) {
+            const int32_t i = (b + u) * ONE_THREADS + tid;
+            tv[u] = st[i < n ? i : n - 1];
+        }
+        LW_UNROLL for (int u = 0; u < APPEND_BATCH; ++u) {
+            const int32_t i = (b + u) * ONE_THREADS + tid;
+            const bool p = b + u < rounds && i < n && frame_pred(tv[u], inv_dt, k_min, k_open);
+            keep |= (p ? 1u : 0u) << (b + u);
+            const unsigned long long bw = __ballot(p);
+            if (lane == 0 && b + u < rounds) cnt[(b + u) * ONE_WAVES + wave] = __popcll(bw);
+        }
+    }
+    __syncthreads();
+    if (wave == 0) {
+        const int32_t total = scan_round_counts(cnt, rounds);
+        if (lane == 0) {
+            const int32_t base = reset ? 0 : *count;
+            s_base = base;
+            *count = base + total;
+        }
+    }
+    __syncthreads();
+    const int32_t base = s_base;
+    LW_UNROLL for (int b = 0; b < ONE_ROUNDS; b += APPEND_BATCH) {
+        if (b >= rounds) break;
+        float x[APPEND_BATCH], y[APPEND_BATCH], z[APPEND_BATCH], tv[APPEND_BATCH];
+        uint32_t a[APPEND_BATCH];
+        LW_UNROLL for (int u = 0; u < APPEND_BATCH; ++u) {
+            const int32_t i = (b + u) * ONE_THREADS + tid;
+            if ((keep >> (b + u)) & 1u) {
+                x[u] = sx[3 * i];
+                y[u] = sx[3 * i + 1];
+                z[u] = sx[3 * i + 2];
+                a[u] = sa[i];
+                tv[u] = st[i];
+            }
+        }
+        LW_UNROLL for (int u = 0; u < APPEND_BATCH; ++u) {
+            const bool w = ((keep >> (b + u)) & 1u) != 0u;
+            const unsigned long long bw = __ballot(w);
+            if (w) {
+                const int32_t j = base + cnt[(b + u) * ONE_WAVES + wave] + __popcll(bw & ((1ull << lane) - 1ull));
+                if (j < cap) {
+                    f_raw[3 * j] = x[u];
+                    f_raw[3 * j + 1] = y[u];
+                    f_raw[3 * j + 2] = z[u];
+                    f_attr[j] = a[u];
+                    f_t[j] = tv[u];
+                    f_frame[j] = k_open;
+                }
+            }
+        }
+    }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_append_count(const float *st, int32_t n, float inv_dt, int32_t k_min,
@@ -180,14 +259,20 @@ __global__ __launch_bounds__(BLOCK) void k_thin_write(const float *xyz, const in
 }  // namespace
 
 hipError_t launch_frame_append(lw_ctx *c, const float *sx, const uint32_t *sa, const float *st, int32_t n,
-                               float inv_dt, int32_t k_min, int32_t k_open) {
+                               float inv_dt, int32_t k_min, int32_t k_open, bool reset) {
     if (n <= 0) return hipSuccess;
+    if (n <= ONE_THREADS * ONE_ROUNDS) {  // one launch
+        hipLaunchKernelGGL(k_append_one, dim3(1), dim3(ONE_THREADS), 0, c->stream, sx, sa, st, n, inv_dt, k_min,
+                           k_open, &c->ctr[6], reset ? 1 : 0, (int32_t)c->frame_cap, c->f_raw, c->f_attr, c->f_t,
+                           c->f_frame);
+        return hipGetLastError();
+    }
     int32_t nb = (n + BLOCK - 1) / BLOCK;
     if (nb > c->max_blocks) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_append_count, dim3(nb), dim3(BLOCK), 0, c->stream, st, n, inv_dt, k_min, k_open,
                        c->blk_cnt);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_THREADS), 0, c->stream, c->blk_cnt, nb, c->blk_off,
-                       &c->ctr[6], 1, nullptr);
+                       &c->ctr[6], reset ? 0 : 1, nullptr);
     hipLaunchKernelGGL(k_append_write, dim3(nb), dim3(BLOCK), 0, c->stream, sx, sa, st, n, inv_dt, k_min, k_open,
                        c->blk_off, (int32_t)c->frame_cap, c->f_raw, c->f_attr, c->f_t, c->f_frame);
     return hipGetLastError();

@@ -120,6 +120,7 @@ struct lw_ctx {
     double *chain_tp = nullptr;
     int32_t *chain_counts = nullptr;
     int64_t frame_bound = 0;  // points appended since the last lw_frame_reset (at most frame_cap): bounds ctr[6]
+    bool count_reset = false;  // lw_frame_reset's zeroing of ctr[6], left to the next append or reader of it
     double *partials = nullptr;
     uint32_t *loop_flags = nullptr;  // MAX_LOOP_ROWS: the pass each searching workgroup's row belongs to
     int32_t *best_slot = nullptr;
@@ -213,8 +214,9 @@ hipError_t launch_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, 
 hipError_t launch_refresh(lw_ctx *c, bool all, const PrecParams &q);
 hipError_t launch_map_find(lw_ctx *c, const int64_t *keys_dev, int32_t n, int32_t *out_dev);
 
+// reset: the frame's count starts from 0 (a reset the caller deferred), else it grows from its value.
 hipError_t launch_frame_append(lw_ctx *c, const float *sx, const uint32_t *sa, const float *st, int32_t n,
-                               float inv_dt, int32_t k_min, int32_t k_open);
+                               float inv_dt, int32_t k_min, int32_t k_open, bool reset);
 // m < 0: the open frame's point count on the device (ctr[6], capped at frame_cap). gate: when given, a device
 // flag that must be nonzero for the kernels to do anything. xi_dev: the twist on the device (replaces xi).
 hipError_t launch_deskew(lw_ctx *c, int32_t m, float t_mid, float inv_dt, const float *xi,

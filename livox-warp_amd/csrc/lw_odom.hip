@@ -161,10 +161,10 @@ __global__ void k_keyframe_check(const RegState *st, FrameCtl *fc, const int32_t
 // hold at most (L - 1) / 2 + 2 PREP_BLOCK_KEYS < L keys), so no key probes past its 64th slot, in any
 // insertion order. When the check fails, or a key finds no free slot
 // in the LDS table, the workgroup runs the claims again in the global table, exactly as the grid kernels do.
-constexpr int PREP_THREADS = 1024;
-constexpr int PREP_ROUNDS = 32;
+constexpr int PREP_THREADS = ONE_THREADS;
+constexpr int PREP_ROUNDS = ONE_ROUNDS;
 constexpr int PREP_SLOTS = 8192;
-constexpr int PREP_WAVES = PREP_THREADS / 64;
+constexpr int PREP_WAVES = ONE_WAVES;
 constexpr int PREP_BATCH = 4;  // rounds whose loads are issued together
 constexpr int PREP_BLOCK = 32, PREP_BLOCK_KEYS = 16, PREP_COUNTERS = 8192;
 static_assert(2 * PREP_BLOCK_KEYS <= PREP_BLOCK && 4 * PREP_BLOCK_KEYS <= PROBES, "no cluster of PROBES slots");
@@ -338,29 +338,10 @@ __global__ __launch_bounds__(PREP_THREADS) void k_frame_prep(PrepArgs A) {
         if (lane == 0) cnt[r * PREP_WAVES + wave] = __popcll(bw);
     }
     __syncthreads();
-    // exclusive prefix over (round, wave) in that order: wave 0, eight entries a lane
+    // exclusive offsets over (round, wave) in that order
     if (wave == 0) {
-        constexpr int E = PREP_ROUNDS * PREP_WAVES / 64;
-        const int n = rounds * PREP_WAVES;
-        int32_t v[E];
-        int32_t sum = 0;
-        for (int e = 0; e < E; ++e) {
-            const int idx = lane * E + e;
-            v[e] = idx < n ? cnt[idx] : 0;
-            sum += v[e];
-        }
-        int32_t incl = sum;
-        for (int d = 1; d < 64; d <<= 1) {
-            const int32_t y = __shfl_up(incl, d);
-            if (lane >= d) incl += y;
-        }
-        int32_t run = incl - sum;
-        for (int e = 0; e < E; ++e) {
-            const int idx = lane * E + e;
-            if (idx < n) cnt[idx] = run;
-            run += v[e];
-        }
-        if (lane == 63) *A.n_out = incl;
+        const int32_t total = scan_round_counts(cnt, rounds);
+        if (lane == 0) *A.n_out = total;
     }
     __syncthreads();
     // the winners in index order, PREP_BATCH rounds' loads at a time

@@ -189,6 +189,7 @@ class Odometry:
         # learned pose-increment prior (prior.LearnedPrior) in place of the constant-velocity guess; None: off
         self.prior = None
         self._map_counts = None  # (voxels, overflow) as last read back; None after a map change
+        self._inv_memo = []  # (pose, its inverse), the latest first (_inv)
         # running totals beside the reference's per-frame stats: registrations, Gauss-Newton steps, prior use
         self.totals = {"registrations": 0, "gn_iters": 0, "prior_used": 0, "prior_fallbacks": 0}
 
@@ -248,6 +249,16 @@ class Odometry:
         self.xi = np.zeros(6, np.float32)
         self.xi_s = np.zeros(6)
         self._zero_count()
+
+    def _inv(self, T: np.ndarray) -> np.ndarray:
+        """np.linalg.inv(T), remembered for the last two poses asked for: the previous frame's pose and the last
+        keyframe's are inverted once each (and once in all when they are the same pose), not at every use."""
+        for M, Mi in self._inv_memo:
+            if M is T or np.array_equal(M, T):
+                return Mi
+        Mi = np.linalg.inv(T)
+        self._inv_memo = [(T.copy(), Mi), self._inv_memo[0]] if self._inv_memo else [(T.copy(), Mi)]
+        return Mi
 
     def predict(self) -> np.ndarray:
         return self.T @ self.delta
@@ -326,7 +337,7 @@ class Odometry:
         if not chained and ok and (warm or self._is_keyframe(T)):
             self._insert(m, T)
         if self.T_prev is not None and not warm:
-            self._update_velocity(np.linalg.inv(self.T_prev) @ T)
+            self._update_velocity(self._inv(self.T_prev) @ T)
             if self.prior is not None:
                 self.prior.observe(self.T_prev, T)
         self.T_prev = self.T = T
@@ -362,7 +373,7 @@ class Odometry:
                 self._deskew(m, t_mid, xi)
                 T, ok = self._register(m, self.predict())
         if ok and self.deskew and self.redeskew and self.T_prev is not None:
-            xi_new = _body_twist(np.linalg.inv(self.T_prev) @ T)
+            xi_new = _body_twist(self._inv(self.T_prev) @ T)
             if (np.linalg.norm(xi_new[3:] - xi[3:]) > REDESKEW_ROT
                     or np.linalg.norm(xi_new[:3] - xi[:3]) > REDESKEW_TRANS):
                 xi = xi_new
@@ -381,10 +392,10 @@ class Odometry:
         self._fin_T[:] = T_pred.ravel()
         fi.has_prev = 1 if self.T_prev is not None else 0
         if fi.has_prev:
-            self._fin_prev[:] = np.linalg.inv(self.T_prev).ravel()
+            self._fin_prev[:] = self._inv(self.T_prev).ravel()
         fi.has_key = 1 if self.T_key is not None else 0
         if fi.has_key:
-            self._fin_key[:] = np.linalg.inv(self.T_key).ravel()
+            self._fin_key[:] = self._inv(self.T_key).ravel()
         self._fin_xi[:] = xi
         fi.kf_dist, fi.kf_angle = float(self.kf_dist), float(self.kf_angle)
         fi.redeskew_rot, fi.redeskew_trans = REDESKEW_ROT, REDESKEW_TRANS
@@ -427,7 +438,7 @@ class Odometry:
             return T, xi, False, m
         T, ok = self._register_take(r, T)
         if ok and self.deskew and self.redeskew and self.T_prev is not None:
-            xi_new = _body_twist(np.linalg.inv(self.T_prev) @ T)
+            xi_new = _body_twist(self._inv(self.T_prev) @ T)
             if (np.linalg.norm(xi_new[3:] - xi[3:]) > REDESKEW_ROT
                     or np.linalg.norm(xi_new[:3] - xi[:3]) > REDESKEW_TRANS):
                 xi = xi_new
@@ -473,7 +484,7 @@ class Odometry:
     def _is_keyframe(self, T: np.ndarray) -> bool:
         if self.T_key is None:
             return True
-        d = np.linalg.inv(self.T_key) @ T
+        d = self._inv(self.T_key) @ T
         ang = math.degrees(np.linalg.norm(_rotvec(d)))
         return np.linalg.norm(d[:3, 3]) >= self.kf_dist or ang >= self.kf_angle
 
