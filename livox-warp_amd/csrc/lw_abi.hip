@@ -60,6 +60,7 @@ void free_all(lw_ctx *c) {
         delete[] c->prof_ev;
     }
     if (c->pin) (void)hipHostFree(c->pin);
+    if (c->chain_ev) (void)hipEventDestroy(c->chain_ev);
 }
 
 // Add the elapsed times of the recorded launch pairs (call after the stream has been synchronised).
@@ -180,6 +181,7 @@ int lw_create(lw_ctx **out, int64_t reg_slots, int64_t frame_cap, void *stream) 
     if ((e = dalloc(&c->loc_queue, 16)) != hipSuccess) return fail(e);
     if ((e = hipHostMalloc(&c->pin, PIN_BYTES, hipHostMallocDefault)) != hipSuccess) return fail(e);
     c->pin_bytes = PIN_BYTES;
+    if ((e = hipEventCreateWithFlags(&c->chain_ev, hipEventDisableTiming)) != hipSuccess) return fail(e);
     hipLaunchKernelGGL(k_fill_i64, dim3(1024), dim3(256), 0, c->stream, c->ds_keys, c->ds_cap, (int64_t)-1);
     if ((e = lw::launch_fill_i32(c, c->ds_idx, c->ds_cap, INT_MAX)) != hipSuccess) return fail(e);
     if ((e = lw::launch_fill_i32(c, c->ctr, 64, 0)) != hipSuccess) return fail(e);
@@ -699,17 +701,21 @@ int lw_odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, lw_f
     }
     LW_CK(lw::launch_register_loop(c, &c->ctr[5], cap, L, p->acc_mode, redo));
     // odom.py:650-651, 712-727: the keyframe test, then (gated on it) the insert at the frame's pose, whose
-    // touched voxels get their precision at once (the parameters are this frame's) with the map's counts
+    // touched voxels get their precision at once (the parameters are this frame's)
     LW_CK(lw::launch_keyframe_check(c));
-    const lw::Mat4f unused{};
     const lw::PrecParams q{c->prec_voxel, c->prec_noise2, c->prec_tight2, c->prec_alpha};
-    LW_CK(lw::launch_map_insert(c, c->f_xyz, c->f_attr, cap, &c->ctr[6], unused, (float)(1.0 / (double)p->reg_voxel),
-                                p->reg_voxel, p->max_range, p->ret_mask, &c->fctl->keyframe, c->fctl->T32, &q,
-                                c->chain_counts));
-    // one readback: both registrations' states, the decisions and the map's counts after the insert
+    const lw::InsertArgs ins{c->f_xyz, c->f_attr, cap, &c->ctr[6], lw::Mat4f{}, (float)(1.0 / (double)p->reg_voxel),
+                             p->reg_voxel, p->max_range, p->ret_mask, &c->fctl->keyframe, c->fctl->T32, &q,
+                             c->chain_counts};
+    LW_CK(lw::launch_insert_head(c, ins));
+    // one readback, once the insert's claims have fixed the map's counts: both registrations' states, the
+    // decisions and the counts; the host waits for it alone, so the rest of the insert runs on while the host
+    // goes on (anything it queues next runs after it)
     char *h = pin_down(c);
     LW_CK(hipMemcpyAsync(h, c->chain, CH_BYTES, hipMemcpyDeviceToHost, c->stream));
-    LW_CK(hipStreamSynchronize(c->stream));
+    LW_CK(hipEventRecord(c->chain_ev, c->stream));
+    LW_CK(lw::launch_insert_tail(c, ins));
+    LW_CK(hipEventSynchronize(c->chain_ev));
     LW_CK(prof_harvest(c));
     lw::RegState st2, st1;
     lw::FrameCtl fc;

@@ -121,6 +121,7 @@ struct lw_ctx {
     int32_t *chain_counts = nullptr;
     int64_t frame_bound = 0;  // points appended since the last lw_frame_reset (at most frame_cap): bounds ctr[6]
     bool count_reset = false;  // lw_frame_reset's zeroing of ctr[6], left to the next append or reader of it
+    hipEvent_t chain_ev = nullptr;  // lw_odom_frame's readback (the host waits for it, not for the insert's tail)
     double *partials = nullptr;
     uint32_t *loop_flags = nullptr;  // MAX_LOOP_ROWS: the pass each searching workgroup's row belongs to
     int32_t *best_slot = nullptr;
@@ -206,11 +207,28 @@ hipError_t launch_map_reset(lw_ctx *c);
 // n_dev: when given, the points to insert are its count capped at n (n sizes the grids). gate: when given, a
 // device flag that must be nonzero for the insert to happen. T_dev: the pose on the device (replaces T).
 // refresh: when given, the touched voxels' precision is computed at once with these parameters instead of
-// being queued for the next refresh. counts: when given, receives the map's counts (ctr[0..1]) after the insert.
+// being queued for the next refresh.
 hipError_t launch_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, int32_t n, const int32_t *n_dev,
                              const Mat4f &T, float inv_v, float voxel, float max_range, int32_t ret_mask,
                              const int32_t *gate = nullptr, const float *T_dev = nullptr,
-                             const PrecParams *refresh = nullptr, int32_t *counts = nullptr);
+                             const PrecParams *refresh = nullptr);
+// The same insert in two parts: the head (claims, per-voxel counts) leaves the map's counts final, and writes
+// them to counts when given; the tail (quotas, accumulation, finish) completes the moments and precision.
+struct InsertArgs {
+    const float *xyz;
+    const uint32_t *attr;
+    int32_t n;
+    const int32_t *n_dev;
+    Mat4f T;
+    float inv_v, voxel, max_range;
+    int32_t ret_mask;
+    const int32_t *gate;
+    const float *T_dev;
+    const PrecParams *refresh;
+    int32_t *counts;
+};
+hipError_t launch_insert_head(lw_ctx *c, const InsertArgs &a);
+hipError_t launch_insert_tail(lw_ctx *c, const InsertArgs &a);
 hipError_t launch_refresh(lw_ctx *c, bool all, const PrecParams &q);
 hipError_t launch_map_find(lw_ctx *c, const int64_t *keys_dev, int32_t n, int32_t *out_dev);
 

@@ -124,10 +124,13 @@ __global__ __launch_bounds__(BLOCK) void k_ins_claim(const float *xyz, const uin
 }
 
 // 2: count the incoming points per voxel; the first one lists the voxel as touched.
+// counts, when given, receives the map's counts (ctr[0..1]), final once the claims are done.
 __global__ __launch_bounds__(BLOCK) void k_ins_count(const int32_t *pt_slot, int32_t n, const int32_t *n_dev,
                                                      const Slot *slots, int32_t *pt_vid, int32_t *vox_nin,
-                                                     int32_t *touched, int32_t *ctr, const int32_t *gate) {
+                                                     int32_t *touched, int32_t *ctr, const int32_t *gate,
+                                                     int32_t *counts) {
     if (gated_off(gate)) return;
+    if (counts && blockIdx.x == 0 && threadIdx.x < 2) counts[threadIdx.x] = ctr[threadIdx.x];
     int32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= count_of(n, n_dev)) return;
     int32_t s = pt_slot[i];
@@ -239,15 +242,13 @@ __device__ inline void refresh_voxel(int32_t vid, const int64_t *vox_key, const 
 }
 
 // 5: finish: clear the per-insert counters; queue the touched voxels for a refresh, or (refresh set) refresh
-// them here with q. counts, when given, receives the map's counts (ctr[0], ctr[1]).
+// them here with q.
 __global__ __launch_bounds__(BLOCK) void k_ins_finish(const int32_t *touched, int32_t *ctr, int32_t *vox_nin,
                                                       int32_t *vox_quota, int32_t *vox_dirty, int32_t *dirty_list,
                                                       const int32_t *gate, int32_t refresh, PrecParams q,
                                                       const int64_t *vox_key, const int32_t *vox_slot,
-                                                      const VoxMom *vox_mom, Slot *slots, VoxPrec *vox_prec,
-                                                      int32_t *counts) {
+                                                      const VoxMom *vox_mom, Slot *slots, VoxPrec *vox_prec) {
     if (gated_off(gate)) return;
-    if (counts && blockIdx.x == 0 && threadIdx.x < 2) counts[threadIdx.x] = ctr[threadIdx.x];
     int32_t nt = ctr[4];
     for (int32_t j = blockIdx.x * BLOCK + threadIdx.x; j < nt; j += gridDim.x * BLOCK) {
         int32_t vid = touched[j];
@@ -323,26 +324,39 @@ hipError_t launch_map_reset(lw_ctx *c) {
     return hipGetLastError();
 }
 
+hipError_t launch_insert_head(lw_ctx *c, const InsertArgs &a) {
+    if (a.n <= 0) return hipSuccess;
+    const int g = grid_for(a.n);
+    hipLaunchKernelGGL(k_ins_claim, dim3(g), dim3(BLOCK), 0, c->stream, a.xyz, a.attr, a.n, a.n_dev, a.T, a.inv_v,
+                       a.max_range, a.ret_mask, c->mask, c->slots, c->occ, c->vox_key, c->vox_slot, c->vox_mom, c->ctr,
+                       c->vox_cap, c->pt_slot, a.gate, a.T_dev);
+    hipLaunchKernelGGL(k_ins_count, dim3(g), dim3(BLOCK), 0, c->stream, c->pt_slot, a.n, a.n_dev, c->slots, c->pt_vid,
+                       c->vox_nin, c->touched, c->ctr, a.gate, a.counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_insert_tail(lw_ctx *c, const InsertArgs &a) {
+    if (a.n <= 0) return hipSuccess;
+    const int g = grid_for(a.n);
+    hipLaunchKernelGGL(k_ins_quota, dim3(1024), dim3(BLOCK), 0, c->stream, c->touched, c->ctr, c->vox_nin,
+                       c->vox_mom, c->vox_quota, c->pt_vid, a.n, a.n_dev, c->pt_take, a.gate);
+    hipLaunchKernelGGL(k_ins_accum, dim3(g), dim3(BLOCK), 0, c->stream, a.xyz, a.attr, a.n, a.n_dev, a.T, a.inv_v,
+                       a.voxel, a.max_range, a.ret_mask, c->pt_vid, c->vox_quota, c->pt_take, c->vox_mom, a.gate,
+                       a.T_dev);
+    const PrecParams q = a.refresh ? *a.refresh : PrecParams{};
+    hipLaunchKernelGGL(k_ins_finish, dim3(256), dim3(BLOCK), 0, c->stream, c->touched, c->ctr, c->vox_nin,
+                       c->vox_quota, c->vox_dirty, c->dirty_list, a.gate, a.refresh ? 1 : 0, q, c->vox_key,
+                       c->vox_slot, c->vox_mom, c->slots, c->vox_prec);
+    if (!a.refresh) c->dirty_maybe = true;
+    return hipGetLastError();
+}
+
 hipError_t launch_map_insert(lw_ctx *c, const float *xyz, const uint32_t *attr, int32_t n, const int32_t *n_dev,
                              const Mat4f &T, float inv_v, float voxel, float max_range, int32_t ret_mask,
-                             const int32_t *gate, const float *T_dev, const PrecParams *refresh, int32_t *counts) {
-    if (n <= 0) return hipSuccess;
-    int g = grid_for(n);
-    hipLaunchKernelGGL(k_ins_claim, dim3(g), dim3(BLOCK), 0, c->stream, xyz, attr, n, n_dev, T, inv_v, max_range,
-                       ret_mask, c->mask, c->slots, c->occ, c->vox_key, c->vox_slot, c->vox_mom, c->ctr, c->vox_cap,
-                       c->pt_slot, gate, T_dev);
-    hipLaunchKernelGGL(k_ins_count, dim3(g), dim3(BLOCK), 0, c->stream, c->pt_slot, n, n_dev, c->slots, c->pt_vid,
-                       c->vox_nin, c->touched, c->ctr, gate);
-    hipLaunchKernelGGL(k_ins_quota, dim3(1024), dim3(BLOCK), 0, c->stream, c->touched, c->ctr, c->vox_nin,
-                       c->vox_mom, c->vox_quota, c->pt_vid, n, n_dev, c->pt_take, gate);
-    hipLaunchKernelGGL(k_ins_accum, dim3(g), dim3(BLOCK), 0, c->stream, xyz, attr, n, n_dev, T, inv_v, voxel,
-                       max_range, ret_mask, c->pt_vid, c->vox_quota, c->pt_take, c->vox_mom, gate, T_dev);
-    const PrecParams q = refresh ? *refresh : PrecParams{};
-    hipLaunchKernelGGL(k_ins_finish, dim3(256), dim3(BLOCK), 0, c->stream, c->touched, c->ctr, c->vox_nin,
-                       c->vox_quota, c->vox_dirty, c->dirty_list, gate, refresh ? 1 : 0, q, c->vox_key, c->vox_slot,
-                       c->vox_mom, c->slots, c->vox_prec, counts);
-    if (!refresh) c->dirty_maybe = true;
-    return hipGetLastError();
+                             const int32_t *gate, const float *T_dev, const PrecParams *refresh) {
+    const InsertArgs a{xyz, attr, n, n_dev, T, inv_v, voxel, max_range, ret_mask, gate, T_dev, refresh, nullptr};
+    hipError_t e = launch_insert_head(c, a);
+    return e != hipSuccess ? e : launch_insert_tail(c, a);
 }
 
 hipError_t launch_refresh(lw_ctx *c, bool all, const PrecParams &q) {
