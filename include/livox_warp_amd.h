@@ -260,6 +260,47 @@ int lw_prof_read(lw_ctx *ctx, double *out);
  * out holds 22 doubles. No counterpart in the reference. */
 int lw_prof_phases(lw_ctx *ctx, double *out);
 
+/* ---- the integration map (the reference's Pipeline map, python/livox_warp/gpu.py) ----------------------
+ * The consumer of the odometry's frames: voxel records keyed and probed like the registration map (key,
+ * hits, carving misses, float32 sums of x, y, z and reflectivity over the first 4096 hits, first and last
+ * time), the device pose table the inserts and rays read, and free-space carving. One context per map, on
+ * its own stream (void *: a hipStream_t). Point arrays are device pointers. */
+typedef struct lw_imap lw_imap;
+
+/* The raw-return filters of an insert (the keep_raw fields of gpu.py:77-99 View, applied as gpu.py:156-167). */
+typedef struct lw_map_view {
+    float min_range, max_range;
+    int32_t min_refl;
+    int32_t noise_mask; /* bit k keeps points whose noise confidence (tag bits 0-1) is k */
+    int32_t ret_mask;   /* bit k keeps return k */
+} lw_map_view;
+
+/* slots: a power of two in [64, 2^27] (Pipeline(map_capacity), gpu.py:1172-1219); the map starts cleared. */
+int lw_imap_create(lw_imap **out, int64_t slots, void *stream);
+int lw_imap_destroy(lw_imap *m);
+/* Pipeline.clear_map (gpu.py:1304-1312). */
+int lw_imap_clear(lw_imap *m);
+/* The poses of frame ids idx[0..n) (host arrays; mats: n row-major 4x4 float32 sensor-to-world matrices),
+ * written to slot id & 65535 of the device pose table: Pipeline.flush_poses (gpu.py:1346-1365). */
+int lw_imap_set_poses(lw_imap *m, const int32_t *idx, const float *mats, int32_t n);
+/* Pipeline.map_insert / k_map_insert (gpu.py:1421-1434, 373-413): n staged points (xyz, attribute bits,
+ * time, frame id) through their frames' poses into voxels of size voxel. */
+int lw_imap_insert(lw_imap *m, const float *sx, const uint32_t *sa, const float *st, const int32_t *sf, int32_t n,
+                   const lw_map_view *v, float voxel);
+/* Pipeline.carve / k_carve (gpu.py:1436-1452, 470-595): every `every`-th return's ray, up to max_steps voxels
+ * and max_range metres, adds `every` misses to the occupied voxels whose surface patch it pierces. */
+int lw_imap_carve(lw_imap *m, const float *sx, const uint32_t *sa, const int32_t *sf, int32_t n, float voxel,
+                  int32_t every, float max_range, int32_t max_steps);
+/* Pipeline.rebase_clock (gpu.py:1367-1372): first and last times of the occupied slots become old - shift. */
+int lw_imap_rebase_clock(lw_imap *m, float shift);
+/* out[0] voxels claimed, out[1] points that found no slot within 64 probes (Pipeline.map_stats). Syncs. */
+int lw_imap_stats(lw_imap *m, int32_t *out);
+/* The records as they lie in device memory (bytes = 64 x slots), to host memory / from it. Sync. 64-byte
+ * record: int64 key (-1 empty), int32 hits, int32 misses, float sums[4] (x, y, z, reflectivity), float
+ * first, float last, 24 bytes unused. */
+int lw_imap_read(lw_imap *m, void *out, int64_t bytes);
+int lw_imap_load(lw_imap *m, const void *in, int64_t bytes);
+
 /* Host builds of the device's float64 step and per-voxel precision, for tests on machines without a
  * GPU (same source as the device code). */
 int lw_host_gn_step(const double *v, const double *T, const double *T_pred, double damping, double prior_info,

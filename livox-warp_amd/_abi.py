@@ -57,6 +57,11 @@ class LocParams(ctypes.Structure):
                 ("min_count", _i32), ("block", _i32), ("grid", _i32), ("use_graph", _i32)]
 
 
+class MapView(ctypes.Structure):
+    _fields_ = [("min_range", _f32), ("max_range", _f32), ("min_refl", _i32), ("noise_mask", _i32),
+                ("ret_mask", _i32)]
+
+
 # name -> (restype, argtypes); every symbol include/livox_warp_amd.h declares
 SIGNATURES = {
     "lw_version": (_i32, []),
@@ -104,6 +109,16 @@ SIGNATURES = {
     "lw_host_apply_twist": (_i32, [_f64p, _f64p]),
     "lw_host_twist_between": (_i32, [_f64p, _f64p, _f64p]),
     "lw_host_cond": (_f64, [_f64p]),
+    "lw_imap_create": (_i32, [ctypes.POINTER(_vp), _i64, _vp]),
+    "lw_imap_destroy": (_i32, [_vp]),
+    "lw_imap_clear": (_i32, [_vp]),
+    "lw_imap_set_poses": (_i32, [_vp, _i32p, _f32p, _i32]),
+    "lw_imap_insert": (_i32, [_vp, _vp, _vp, _vp, _vp, _i32, ctypes.POINTER(MapView), _f32]),
+    "lw_imap_carve": (_i32, [_vp, _vp, _vp, _vp, _i32, _f32, _i32, _f32, _i32]),
+    "lw_imap_rebase_clock": (_i32, [_vp, _f32]),
+    "lw_imap_stats": (_i32, [_vp, _i32p]),
+    "lw_imap_read": (_i32, [_vp, _vp, _i64]),
+    "lw_imap_load": (_i32, [_vp, _vp, _i64]),
 }
 
 _lib = None

@@ -47,6 +47,16 @@ class PoseTable:
             self.owner[js, 1] = epoch
             self.dirty.update(js.tolist())
 
+    def drain(self):
+        """The slots edited since the last drain, with their poses, and the edit set cleared: the host half of
+        Pipeline.flush_poses (gpu.py:1346-1365), for a device consumer to upload."""
+        with self._lock:
+            if not self.dirty:
+                return np.zeros(0, np.int32), np.zeros((0, 4, 4), np.float32)
+            idx = np.fromiter(self.dirty, dtype=np.int32, count=len(self.dirty))
+            self.dirty.clear()
+            return idx, self.poses[idx].copy()
+
     def pose(self, frame: int):
         """(4x4 float32 pose, frame id that wrote it, epoch) of a frame id's slot."""
         j = frame & (POSE_SLOTS - 1)

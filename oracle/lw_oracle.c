@@ -15,6 +15,8 @@
  *   python/livox_warp/odom.py     :72-94 constants, :97-123 k_frame_append, :126-160 k_ds_insert,
  *                                  :163-178 k_deskew, :181-230 k_reg_insert, :233-272 accumulate,
  *                                  :275-375 k_icp, :378-384 k_reduce
+ *   python/livox_warp/gpu.py      :55-73 map constants, :156-167 keep_raw, :373-413 k_map_insert,
+ *                                  :415-467 map_centroid and local_plane, :470-595 k_carve
  *
  * Where the reference's bits come from Warp internals that are not in the reference repository,
  * this file makes a stated choice and the result is "parity unpinned" at the bit level (pinned by
@@ -25,7 +27,10 @@
  *     float32 (M and Mg do not depend on the eigenbasis chosen, only on the eigenvalues);
  *   - wp.quat_from_axis_angle / wp.quat_rotate (gpu.py:275): the textbook unit-quaternion rotation
  *     v' = v (2w^2 - 1) + 2 (q.v) q + 2 w (q x v) with sinf/cosf;
- *   - 1 << ret for a return index >= 31 (undefined for a 32-bit int): such a point is rejected.
+ *   - 1 << ret for a return index >= 31 (undefined for a 32-bit int): such a point is rejected;
+ *   - wp.eig3 in local_plane (gpu.py:453): a float64 cyclic Jacobi on the float32 covariance
+ *     (lwo_sym_eig3, the same sweeps and stopping test as the device's), so the surface test and the
+ *     normal are float64 where the reference's are float32.
  */
 #include <math.h>
 #include <stdint.h>
@@ -438,5 +443,232 @@ void lwo_reduce(const double *acc, double *out)
         double s = 0.0;
         for (int k = 0; k < LWO_ACC_SLOTS; ++k) s += acc[(size_t)k * LWO_ACC_STRIDE + j];
         out[j] = s;
+    }
+}
+
+/* ---- the integration map (gpu.py:373-595): insert and free-space carving ---------------------- */
+/*
+ * Record arrays as the reference keeps them (gpu.py:1213-1219): keys (-1 empty), acc (4 floats a slot: x, y,
+ * z, reflectivity sums), cnt, last, first, miss; stats[0] voxels claimed, stats[1] points without a slot.
+ * poses: 65536 row-major 4x4 float32 matrices indexed by frame id & 65535.
+ */
+#define LWO_POSE_MASK 65535
+#define LWO_MAP_COUNT_CAP 4096 /* gpu.py:55 */
+
+static int keep_raw(const float *p, uint32_t a, float min_range, float max_range, int min_refl, int noise_mask,
+                    int ret_mask)
+{
+    float r = sqrtf(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    if (r < min_range || r > max_range) return 0;
+    if ((int)(a & 255u) < min_refl) return 0;
+    if (((noise_mask >> (tag_of(a) & 3)) & 1) == 0) return 0;
+    if (((ret_mask >> (ret_of(a) & 3)) & 1) == 0) return 0;
+    return 1;
+}
+
+static void xform_f(const float *T, const float *p, float *w)
+{
+    for (int r = 0; r < 3; ++r) {
+        float a = T[r * 4 + 0] * p[0];
+        a = a + T[r * 4 + 1] * p[1];
+        a = a + T[r * 4 + 2] * p[2];
+        w[r] = a + T[r * 4 + 3];
+    }
+}
+
+void lwo_map_insert(const float *xyz, const uint32_t *attr, const float *t, const int32_t *frame, int n,
+                    const float *poses, float min_range, float max_range, int min_refl, int noise_mask, int ret_mask,
+                    float inv_voxel, int32_t mask, int64_t *keys, float *acc, int32_t *cnt, float *last,
+                    float *first, int32_t *stats)
+{
+    for (int i = 0; i < n; ++i) {
+        const float *p = xyz + 3 * (int64_t)i;
+        if (!keep_raw(p, attr[i], min_range, max_range, min_refl, noise_mask, ret_mask)) continue;
+        float w[3];
+        xform_f(poses + (int64_t)(frame[i] & LWO_POSE_MASK) * 16, p, w);
+        int64_t key = lwo_voxel_key_ijk((int32_t)floorf(w[0] * inv_voxel), (int32_t)floorf(w[1] * inv_voxel),
+                                        (int32_t)floorf(w[2] * inv_voxel));
+        int32_t h = lwo_slot_hash(key) & mask;
+        int done = 0;
+        for (int probe = 0; probe < LWO_PROBES && !done; ++probe) {
+            int32_t s = (h + probe) & mask;
+            int64_t prev = keys[s];
+            if (prev == -1) keys[s] = key;
+            if (prev == -1 || prev == key) {
+                if (prev == -1) stats[0]++;
+                int32_t old = cnt[s]++;
+                if (old < LWO_MAP_COUNT_CAP) {
+                    acc[4 * s] += w[0];
+                    acc[4 * s + 1] += w[1];
+                    acc[4 * s + 2] += w[2];
+                    acc[4 * s + 3] += (float)(attr[i] & 255u);
+                }
+                if (t[i] > last[s]) last[s] = t[i];
+                if (t[i] < first[s]) first[s] = t[i];
+                done = 1;
+            }
+        }
+        if (!done) stats[1]++;
+    }
+}
+
+/* Cyclic Jacobi for a symmetric 3x3 (float64): the device's sweep order, rotation and stopping test. */
+static void jrot3(double *A, double *V, int P, int Q)
+{
+    double apq = A[3 * P + Q];
+    if (apq == 0.0) return;
+    double theta = (A[3 * Q + Q] - A[3 * P + P]) / (2.0 * apq);
+    double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+    double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+    for (int k = 0; k < 3; ++k) {
+        double akp = A[3 * k + P], akq = A[3 * k + Q];
+        A[3 * k + P] = c * akp - sn * akq;
+        A[3 * k + Q] = sn * akp + c * akq;
+    }
+    for (int k = 0; k < 3; ++k) {
+        double apk = A[3 * P + k], aqk = A[3 * Q + k];
+        A[3 * P + k] = c * apk - sn * aqk;
+        A[3 * Q + k] = sn * apk + c * aqk;
+    }
+    A[3 * P + Q] = 0.0;
+    A[3 * Q + P] = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        double vkp = V[3 * k + P], vkq = V[3 * k + Q];
+        V[3 * k + P] = c * vkp - sn * vkq;
+        V[3 * k + Q] = sn * vkp + c * vkq;
+    }
+}
+
+void lwo_sym_eig3(const double *A_in, double *ev, double *V)
+{
+    double A[9];
+    for (int i = 0; i < 9; ++i) A[i] = A_in[i];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) V[3 * i + j] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        double off = 0.0, dia = 0.0;
+        for (int i = 0; i < 3; ++i) {
+            dia += A[3 * i + i] * A[3 * i + i];
+            for (int j = i + 1; j < 3; ++j) off += A[3 * i + j] * A[3 * i + j];
+        }
+        if (off == 0.0 || off <= 1e-36 * dia) break;
+        jrot3(A, V, 0, 1);
+        jrot3(A, V, 0, 2);
+        jrot3(A, V, 1, 2);
+    }
+    for (int i = 0; i < 3; ++i) ev[i] = A[3 * i + i];
+}
+
+static void centroid(int32_t s, const float *acc, const int32_t *cnt, float *c)
+{
+    float m = 1.0f / (float)(cnt[s] < LWO_MAP_COUNT_CAP ? cnt[s] : LWO_MAP_COUNT_CAP);
+    c[0] = acc[4 * s] * m;
+    c[1] = acc[4 * s + 1] * m;
+    c[2] = acc[4 * s + 2] * m;
+}
+
+/* gpu.py:420-467: (normal x, y, z, w) of the surface through voxel (ix, iy, iz), centroids relative to o. */
+void lwo_local_plane(int32_t ix, int32_t iy, int32_t iz, const float *o, const int64_t *keys, const float *acc,
+                     const int32_t *cnt, int32_t mask, float *out)
+{
+    float cs[3] = {0, 0, 0}, cc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, cn = 0.0f;
+    for (int gx = -1; gx < 2; ++gx)
+        for (int gy = -1; gy < 2; ++gy)
+            for (int gz = -1; gz < 2; ++gz) {
+                int32_t sn = lwo_find_slot(lwo_voxel_key_ijk(ix + gx, iy + gy, iz + gz), keys, mask);
+                if (sn < 0 || cnt[sn] <= 0) continue;
+                float c[3], q[3];
+                centroid(sn, acc, cnt, c);
+                for (int a = 0; a < 3; ++a) q[a] = c[a] - o[a];
+                for (int a = 0; a < 3; ++a) cs[a] = cs[a] + q[a];
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) cc[3 * a + b] = cc[3 * a + b] + q[a] * q[b];
+                cn = cn + 1.0f;
+            }
+    if (cn < 4.0f) {
+        out[0] = 0.0f; out[1] = 0.0f; out[2] = 1.0f; out[3] = 0.0f;
+        return;
+    }
+    float mu[3] = {cs[0] / cn, cs[1] / cn, cs[2] / cn};
+    double cov[9], ev[3], Q[9];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) cov[3 * a + b] = (double)(cc[3 * a + b] / cn - mu[a] * mu[b]);
+    lwo_sym_eig3(cov, ev, Q);
+    int k = 0;
+    if (ev[1] < ev[k]) k = 1;
+    if (ev[2] < ev[k]) k = 2;
+    int j = (k + 1) % 3, j2 = (k + 2) % 3;
+    double mid = ev[j] < ev[j2] ? ev[j] : ev[j2];
+    out[0] = (float)Q[k];
+    out[1] = (float)Q[3 + k];
+    out[2] = (float)Q[6 + k];
+    out[3] = ev[k] < 0.25 * mid ? 2.0f : 1.0f;
+}
+
+void lwo_carve(const float *xyz, const uint32_t *attr, const int32_t *frames, const float *poses, int n, int every,
+               float inv_voxel, int32_t mask, float max_range, int max_steps, const int64_t *keys, const float *acc,
+               const int32_t *cnt, int32_t *miss)
+{
+    for (int64_t i = 0; i < n; i += every) {
+        if (tag_of(attr[i]) != 0) continue;
+        const float *T = poses + (int64_t)(frames[i] & LWO_POSE_MASK) * 16;
+        float o[3] = {T[3], T[7], T[11]}, w[3], d[3];
+        xform_f(T, xyz + 3 * i, w);
+        for (int a = 0; a < 3; ++a) d[a] = w[a] - o[a];
+        float L = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        if (L < 0.05f || L > max_range) continue;
+        for (int a = 0; a < 3; ++a) d[a] = d[a] / L;
+        float voxel = 1.0f / inv_voxel;
+        float margin = fmaxf(3.0f * voxel, 0.03f * L);
+        float pe[4];
+        lwo_local_plane((int32_t)floorf(w[0] * inv_voxel), (int32_t)floorf(w[1] * inv_voxel),
+                        (int32_t)floorf(w[2] * inv_voxel), w, keys, acc, cnt, mask, pe);
+        if (pe[3] > 0.5f) {
+            float cosine = fabsf(pe[0] * d[0] + pe[1] * d[1] + pe[2] * d[2]);
+            margin = fmaxf(margin, 1.5f * voxel / fmaxf(cosine, 0.05f));
+        }
+        float stop = (L - margin) * inv_voxel;
+        if (stop <= 1.0f) continue;
+        float pos[3] = {o[0] * inv_voxel, o[1] * inv_voxel, o[2] * inv_voxel};
+        int32_t ix[3], st[3] = {1, 1, 1};
+        float tm[3] = {1.0e30f, 1.0e30f, 1.0e30f}, td[3] = {1.0e30f, 1.0e30f, 1.0e30f};
+        for (int a = 0; a < 3; ++a) {
+            ix[a] = (int32_t)floorf(pos[a]);
+            if (d[a] > 1.0e-9f) {
+                tm[a] = ((float)(ix[a] + 1) - pos[a]) / d[a];
+                td[a] = 1.0f / d[a];
+            } else if (d[a] < -1.0e-9f) {
+                st[a] = -1;
+                tm[a] = (pos[a] - (float)ix[a]) / (-d[a]);
+                td[a] = -1.0f / d[a];
+            }
+        }
+        for (int step = 0; step < max_steps; ++step) {
+            int a = (tm[0] < tm[1] && tm[0] < tm[2]) ? 0 : (tm[1] < tm[2] ? 1 : 2);
+            float tt = tm[a];
+            ix[a] += st[a];
+            tm[a] = tm[a] + td[a];
+            if (tt >= stop) break;
+            int32_t s = lwo_find_slot(lwo_voxel_key_ijk(ix[0], ix[1], ix[2]), keys, mask);
+            if (s < 0 || cnt[s] <= 0) continue;
+            float cen[3], rel[3], pl[4];
+            centroid(s, acc, cnt, cen);
+            for (int b = 0; b < 3; ++b) rel[b] = cen[b] - o[b];
+            int pierced = 0;
+            lwo_local_plane(ix[0], ix[1], ix[2], cen, keys, acc, cnt, mask, pl);
+            if (pl[3] > 1.5f) {
+                float dn = d[0] * pl[0] + d[1] * pl[1] + d[2] * pl[2];
+                if (fabsf(dn) > 1.0e-4f) {
+                    float f = (rel[0] * pl[0] + rel[1] * pl[1] + rel[2] * pl[2]) / dn, x[3];
+                    for (int b = 0; b < 3; ++b) x[b] = d[b] * f - rel[b];
+                    if (sqrtf(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]) < 0.6f * voxel) pierced = 1;
+                }
+            } else {
+                float rd = rel[0] * d[0] + rel[1] * d[1] + rel[2] * d[2], q[3];
+                for (int b = 0; b < 3; ++b) q[b] = rel[b] - rd * d[b];
+                if (sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]) < 0.5f * voxel) pierced = 1;
+            }
+            if (pierced) miss[s] += every;
+        }
     }
 }

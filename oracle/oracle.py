@@ -69,6 +69,12 @@ _SIGS = {
     "lwo_icp": (None, ["RP", _I, "RP", "R", "R", _I, _i64p, "RP", "RP", _i32p, _I, "R", "R", "R", "R", "R", _f64p,
                        _i32p]),
     "lwo_reduce": (None, [_f64p, _f64p]),
+    "lwo_map_insert": (None, [_f32p, _u32p, _f32p, _i32p, _I, _f32p, ctypes.c_float, ctypes.c_float, _I, _I, _I,
+                              ctypes.c_float, ctypes.c_int32, _i64p, _f32p, _i32p, _f32p, _f32p, _i32p]),
+    "lwo_sym_eig3": (None, [_f64p, _f64p, _f64p]),
+    "lwo_local_plane": (None, [ctypes.c_int32] * 3 + [_f32p, _i64p, _f32p, _i32p, ctypes.c_int32, _f32p]),
+    "lwo_carve": (None, [_f32p, _u32p, _i32p, _f32p, _I, _I, ctypes.c_float, ctypes.c_int32, ctypes.c_float, _I,
+                         _i64p, _f32p, _i32p, _i32p]),
 }
 
 
@@ -649,3 +655,72 @@ def loc_register(omap: OracleMap, scan, T0, voxel: float, iters: int = 10, min_c
         if max(float(np.linalg.norm(delta[:3])), float(np.linalg.norm(delta[3:])) * 5.0) < 2e-4:
             break
     return T, score, count, steps
+
+
+# ---- the integration map (gpu.py:373-595) ------------------------------------------------------------------
+
+MAP_COUNT_CAP = 4096  # gpu.py:55
+POSE_SLOTS = 1 << 16  # gpu.py:56
+
+
+@dataclass
+class IntegrationMap:
+    """The reference's integration-map arrays (gpu.py:1213-1219) for `cap` slots, cleared as clear_map
+    (gpu.py:1304-1312) leaves them."""
+    cap: int
+    keys: np.ndarray = None
+    acc: np.ndarray = None
+    cnt: np.ndarray = None
+    last: np.ndarray = None
+    first: np.ndarray = None
+    miss: np.ndarray = None
+    stats: np.ndarray = None
+
+    def __post_init__(self):
+        self.keys = np.full(self.cap, -1, np.int64)
+        self.acc = np.zeros((self.cap, 4), np.float32)
+        self.cnt = np.zeros(self.cap, np.int32)
+        self.last = np.zeros(self.cap, np.float32)
+        self.first = np.full(self.cap, 1.0e30, np.float32)
+        self.miss = np.zeros(self.cap, np.int32)
+        self.stats = np.zeros(2, np.int32)
+
+
+def map_insert(m: IntegrationMap, xyz, attr, t, frame, poses, voxel: float, min_range=0.0, max_range=1e30,
+               min_refl=0, noise_mask=0b1111, ret_mask=0b1111):
+    """k_map_insert (gpu.py:373-413) as Warp's CPU backend runs it: points in index order. poses: (65536, 4, 4)
+    float32 by frame id & 65535."""
+    xyz, t = _f32(xyz).reshape(-1, 3), _f32(t)
+    attr = np.ascontiguousarray(attr, np.uint32)
+    frame = np.ascontiguousarray(frame, np.int32)
+    P = _f32(poses).reshape(POSE_SLOTS, 16)
+    inv = np.float32(1.0) / np.float32(max(voxel, 1e-4))
+    lib().lwo_map_insert(_p(xyz, _f32p), _p(attr, _u32p), _p(t, _f32p), _p(frame, _i32p), len(t), _p(P, _f32p),
+                         min_range, max_range, int(min_refl), int(noise_mask), int(ret_mask), float(inv), m.cap - 1,
+                         _p(m.keys, _i64p), _p(m.acc, _f32p), _p(m.cnt, _i32p), _p(m.last, _f32p),
+                         _p(m.first, _f32p), _p(m.stats, _i32p))
+
+
+def local_plane(m: IntegrationMap, ix: int, iy: int, iz: int, origin) -> np.ndarray:
+    out = np.zeros(4, np.float32)
+    o = _f32(origin)
+    lib().lwo_local_plane(ix, iy, iz, _p(o, _f32p), _p(m.keys, _i64p), _p(m.acc, _f32p), _p(m.cnt, _i32p),
+                          m.cap - 1, _p(out, _f32p))
+    return out
+
+
+def carve(m: IntegrationMap, xyz, attr, frame, poses, voxel: float, every: int = 2, max_range: float = 40.0,
+          max_steps: int = 4096):
+    """k_carve (gpu.py:470-595) in index order: misses added to m.miss."""
+    xyz = _f32(xyz).reshape(-1, 3)
+    attr = np.ascontiguousarray(attr, np.uint32)
+    frame = np.ascontiguousarray(frame, np.int32)
+    P = _f32(poses).reshape(POSE_SLOTS, 16)
+    inv = np.float32(1.0) / np.float32(max(voxel, 1e-4))
+    lib().lwo_carve(_p(xyz, _f32p), _p(attr, _u32p), _p(frame, _i32p), _p(P, _f32p), len(attr), max(1, int(every)),
+                    float(inv), m.cap - 1, float(max_range), int(max_steps), _p(m.keys, _i64p), _p(m.acc, _f32p),
+                    _p(m.cnt, _i32p), _p(m.miss, _i32p))
+
+
+def identity_poses() -> np.ndarray:
+    return np.tile(np.eye(4, dtype=np.float32), (POSE_SLOTS, 1, 1))
