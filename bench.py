@@ -209,6 +209,78 @@ def loc_cpu_baseline(W, hyps, n: int):
 MLP_FLOPS_PER_ROW = 2.0 * (152 * 256 + 256 * 256 + 256 * 256 + 256 * 6)  # forward multiply-adds x 2
 
 
+IMAP_VOXEL = 0.05  # the integration map's voxel in the reference's odometry checks (check_odometry.py:27)
+IMAP_IN_BYTES = 24  # per point: xyz 12, attribute 4, time 4, frame id 4
+IMAP_REC_BYTES = 64  # one record read and written per kept point
+
+
+def integration_map(args, torch, lw, dev, X, A, Tt, seq, n_frames: int, per: int):
+    """The consumer of the frames (lw.IntegrationMap: the reference Pipeline's map_insert and carve): every
+    generated frame inserted through its ground-truth pose into 0.05 m voxels, then carved (every 2nd return),
+    each pass timed with events on the map's stream; the oracle's serial insert and carve on a sample beside
+    it."""
+    fdt = 0.1
+    F = torch.floor(Tt / fdt).to(torch.int32)  # frame ids, as Pipeline.stage computes them
+    ids = np.unique(F.cpu().numpy())
+    pt = lw.PoseTable()
+    poses = {}
+    for k in ids:
+        poses[int(k)] = seq.pose((int(k) + 0.5) * fdt).astype(np.float32)
+        pt.set_pose(int(k), poses[int(k)])
+    mp = lw.IntegrationMap(pt, dev, map_capacity=1 << 23)
+    view = lw.imap.View(min_range=0.3, max_range=60.0)
+    sx = [X[k].contiguous() for k in range(n_frames)]
+    sa, st, sf = [A[k] for k in range(n_frames)], [Tt[k] for k in range(n_frames)], [F[k].contiguous() for k in range(n_frames)]
+    mp.map_insert(sx[0], sa[0], st[0], sf[0], per, view, IMAP_VOXEL)  # warm-up (and the pose upload)
+    mp.carve(sx[0], sa[0], sf[0], per, IMAP_VOXEL, every=2)
+    mp.clear_map()
+    torch.cuda.synchronize()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    e[0].record(mp.stream)
+    for k in range(n_frames):
+        mp.map_insert(sx[k], sa[k], st[k], sf[k], per, view, IMAP_VOXEL)
+    e[1].record(mp.stream)
+    for k in range(n_frames):
+        mp.carve(sx[k], sa[k], sf[k], per, IMAP_VOXEL, every=2)
+    e[2].record(mp.stream)
+    torch.cuda.synchronize()
+    ins_ms, carve_ms = e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2])
+    occ, dropped = mp.map_stats()
+    r = mp.records()
+    o = r["key"] >= 0
+    points = n_frames * per
+    ins_gbs = points * (IMAP_IN_BYTES + IMAP_REC_BYTES) / (ins_ms * 1e-3) / 1e9
+    # the oracle on a sample: the first frames, insert then carve
+    oracle = _oracle()
+    om = oracle.IntegrationMap(1 << 23)
+    P = oracle.identity_poses()
+    for k, T in poses.items():
+        P[k & 0xFFFF] = T
+    xs, as_, ts, fs = X.cpu().numpy(), A.cpu().numpy().view(np.uint32), Tt.cpu().numpy(), F.cpu().numpy()
+    t0 = time.perf_counter()
+    done = 0
+    while done < n_frames and time.perf_counter() - t0 < 10.0:
+        oracle.map_insert(om, xs[done], as_[done], ts[done], fs[done], P, IMAP_VOXEL, min_range=0.3, max_range=60.0)
+        oracle.carve(om, xs[done], as_[done], fs[done], P, IMAP_VOXEL, every=2)
+        done += 1
+    cpu_s = time.perf_counter() - t0
+    return {
+        "workload": f"{n_frames} frames x {per} points of the odometry sequence through their ground-truth poses, "
+                    f"{IMAP_VOXEL} m voxels in 2^23 slots, then carving every 2nd return",
+        "insert_points_per_s": round(points / (ins_ms * 1e-3), 1),
+        "carve_rays_per_s": round(points / 2 / (carve_ms * 1e-3), 1),
+        "insert_ms_per_frame": round(ins_ms / n_frames, 4), "carve_ms_per_frame": round(carve_ms / n_frames, 4),
+        "voxels": occ, "dropped": dropped, "carved_voxels": int((r["miss"][o] > 2 * r["cnt"][o]).sum()),
+        "roofline": {"bound": "hbm", "achieved": round(ins_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ins_gbs / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "k_imap_insert",
+                     "bytes_per_point": IMAP_IN_BYTES + IMAP_REC_BYTES},
+        "cpu_baseline": {"value": round(done / cpu_s, 3), "unit": "frames/s (insert + carve)", "cores": 1,
+                         "kind": "port", "sample": f"the first {done} frames through the oracle's serial map insert "
+                                                   f"and carve", "seconds": round(cpu_s, 3)},
+        "frames_per_s": round(n_frames / ((ins_ms + carve_ms) * 1e-3), 1),
+    }
+
+
 def learned_prior(args, torch, lw, dev):
     """The learned pose-increment prior (shipped weights, trained on 200 x 600 frames of the plain GPU
     odometry) against the constant-velocity guess on held-out office sequences, and the training step's speed
@@ -263,6 +335,7 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=150, help="frames of the CPU baseline sample (0: skip)")
     ap.add_argument("--acc", default="fp64", choices=["fp64", "kahan32", "fp32"])
     ap.add_argument("--no-loc", action="store_true", help="skip the batched localisation leg")
+    ap.add_argument("--no-imap", action="store_true", help="skip the integration-map leg")
     ap.add_argument("--loc-queries", type=int, default=4)
     ap.add_argument("--loc-hyps", type=int, default=4096)
     ap.add_argument("--loc-reps", type=int, default=3)
@@ -383,6 +456,8 @@ def main():
                      "bytes_per_launch": round(bytes_per_launch, 1)},
         "gen_seconds": round(t_gen, 2),
     }
+    if not args.no_imap:
+        out["integration_map"] = integration_map(args, torch, lw, dev, X, A, Tt, seq, n_frames, per)
     if not args.no_loc:
         out["localisation"] = localisation(args, torch, lw, dev)
     if not args.no_prior:
