@@ -36,14 +36,17 @@ struct MapView {
     int32_t min_refl, noise_mask, ret_mask;
 };
 
-hipError_t imap_clear(hipStream_t s, MapRec *recs, int64_t cap, int32_t *stats);
+// The brick bits a map of cap slots keeps (a power of two, at least 1024).
+int32_t imap_brick_bits(int64_t cap);
+hipError_t imap_clear(hipStream_t s, MapRec *recs, int64_t cap, int32_t *stats, uint32_t *bricks);
+hipError_t imap_brick_rebuild(hipStream_t s, const MapRec *recs, int64_t cap, uint32_t *bricks);
 hipError_t imap_pose_scatter(hipStream_t s, const int32_t *idx, const float *mats, int32_t n, float *poses);
 hipError_t imap_insert(hipStream_t s, const float *sx, const uint32_t *sa, const float *st, const int32_t *sf,
                        int32_t n, const float *poses, const MapView &v, float inv_voxel, int32_t mask, MapRec *recs,
-                       int32_t *stats);
+                       int32_t *stats, uint32_t *bricks);
 hipError_t imap_carve(hipStream_t s, const float *sx, const uint32_t *sa, const int32_t *sf, int32_t n, int32_t every,
                       const float *poses, float inv_voxel, int32_t mask, float max_range, int32_t max_steps,
-                      MapRec *recs);
+                      MapRec *recs, const uint32_t *bricks);
 hipError_t imap_rebase(hipStream_t s, MapRec *recs, int64_t cap, float shift);
 
 }  // namespace lw

@@ -5,7 +5,10 @@
 // Layout in HBM: one 64-byte record per slot (MapRec: key, hit count, carving misses, the float32 sums of x, y,
 // z and reflectivity, first and last time), so an insert's claim and its five atomics, and a ray's probe and
 // the centroid it reads, each touch one cache line; the reference keeps six parallel arrays. Poses: 65536
-// row-major float32 4x4 matrices indexed by frame id & 65535, scattered from the host's edits.
+// row-major float32 4x4 matrices indexed by frame id & 65535, scattered from the host's edits. Bricks: one bit
+// per hashed 8x8x8-voxel brick (cap / 8 bits, L2-resident), set when a voxel in the brick is claimed, so a ray
+// steps through empty bricks without probing the table (a clear bit means no voxel of the brick is in the map;
+// a set bit, shared by colliding bricks, only means the voxels are probed).
 #include <climits>
 #include <cmath>
 
@@ -18,6 +21,30 @@ namespace lw {
 namespace {
 
 constexpr int IMAP_BLOCK = 256;
+
+// The brick bit of the voxel with key k: its 8x8x8 brick (each 21-bit key field divided by 8, so voxels whose
+// keys coincide share a brick), hashed onto nbits (a power of two).
+__device__ inline uint32_t brick_bit(int64_t key, int32_t nbits) {
+    const uint64_t k = (uint64_t)key;
+    const uint64_t b = ((k & KEY_MASK) >> 3) | ((((k >> KEY_BITS) & KEY_MASK) >> 3) << KEY_BITS) |
+                       ((((k >> (2 * KEY_BITS)) & KEY_MASK) >> 3) << (2 * KEY_BITS));
+    return (uint32_t)(slot_hash((int64_t)b) & (nbits - 1));
+}
+
+__device__ inline void brick_set(uint32_t *bricks, uint32_t b) { atomicOr(&bricks[b >> 5], 1u << (b & 31)); }
+
+__global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// The brick bits of every occupied slot (after records were loaded from the host).
+__global__ void k_brick_rebuild(const MapRec *recs, int64_t cap, uint32_t *bricks, int32_t nbits) {
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < cap; s += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t key = recs[s].key;
+        if (key != -1) brick_set(bricks, brick_bit(key, nbits));
+    }
+}
 
 __global__ void k_imap_clear(MapRec *recs, int64_t cap, int32_t *stats) {
     for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < cap; s += (int64_t)gridDim.x * blockDim.x) {
@@ -58,7 +85,7 @@ __device__ inline bool keep_raw(float px, float py, float pz, uint32_t a, const 
 __global__ __launch_bounds__(IMAP_BLOCK) void k_imap_insert(const float *sx, const uint32_t *sa, const float *st,
                                                             const int32_t *sf, int32_t n, const float *poses,
                                                             MapView v, float inv_voxel, int32_t mask, MapRec *recs,
-                                                            int32_t *stats) {
+                                                            int32_t *stats, uint32_t *bricks, int32_t nbits) {
     const int32_t i = blockIdx.x * IMAP_BLOCK + threadIdx.x;
     if (i >= n) return;
     const float px = sx[3 * (int64_t)i], py = sx[3 * (int64_t)i + 1], pz = sx[3 * (int64_t)i + 2];
@@ -75,7 +102,10 @@ __global__ __launch_bounds__(IMAP_BLOCK) void k_imap_insert(const float *sx, con
         const unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long *>(&r->key),
                                                   0xFFFFFFFFFFFFFFFFull, (unsigned long long)key);
         if ((int64_t)prev == -1 || (int64_t)prev == key) {
-            if ((int64_t)prev == -1) atomicAdd(&stats[0], 1);
+            if ((int64_t)prev == -1) {
+                atomicAdd(&stats[0], 1);
+                brick_set(bricks, brick_bit(key, nbits));
+            }
             if (atomicAdd(&r->cnt, 1) < IMAP_COUNT_CAP) {
                 unsafeAtomicAdd(&r->acc.x, wx);
                 unsafeAtomicAdd(&r->acc.y, wy);
@@ -110,32 +140,67 @@ __device__ inline void imap_centroid(const MapRec &r, float &cx, float &cy, floa
     cz = r.acc.z * m;
 }
 
+// A slot's first 32 bytes: {key lo, key hi, hits, misses} and the sums.
+__device__ inline void load_head(const MapRec *recs, int32_t s, int4 &lo, float4 &acc) {
+    const int4 *p = reinterpret_cast<const int4 *>(recs + s);
+    lo = p[0];
+    acc = reinterpret_cast<const float4 *>(p)[1];
+}
+
 // gpu.py:420-467 (local_plane): the normal of the surface through voxel (ix, iy, iz) from the centroids of
 // the occupied voxels around it (relative to origin o), w = 2 for a surface, 1 for at least 4 occupied voxels
-// that are not one, 0 for fewer. The float32 covariance is formed as the reference forms it; its
-// eigen-decomposition is float64 Jacobi (the reference's wp.eig3 is float32), as for the voxel precision.
+// that are not one, 0 for fewer. The 27 neighbours go in three slabs of nine (gx = -1, 0, 1): a slab's home
+// records are loaded together, its keys not at home probe on together, and the centroids then enter the
+// sums in the reference's (gx, gy, gz) order. The float32 covariance is formed as the reference forms it;
+// its eigen-decomposition is float64 Jacobi (the reference's wp.eig3 is float32), as for the voxel precision.
 __device__ inline float4 local_plane(int32_t ix, int32_t iy, int32_t iz, float ox, float oy, float oz,
                                      const MapRec *recs, int32_t mask) {
     float cs[3] = {0.0f, 0.0f, 0.0f};
     float cc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     float cn = 0.0f;
-    for (int gx = -1; gx < 2; ++gx)
-        for (int gy = -1; gy < 2; ++gy)
-            for (int gz = -1; gz < 2; ++gz) {
-                const int32_t sn = imap_find(recs, mask, voxel_key_ijk(ix + gx, iy + gy, iz + gz));
-                if (sn >= 0) {
-                    const MapRec r = recs[sn];
-                    if (r.cnt > 0) {
-                        float c0, c1, c2;
-                        imap_centroid(r, c0, c1, c2);
-                        const float q[3] = {c0 - ox, c1 - oy, c2 - oz};
-                        for (int a = 0; a < 3; ++a) cs[a] = cs[a] + q[a];
-                        for (int a = 0; a < 3; ++a)
-                            for (int b = 0; b < 3; ++b) cc[3 * a + b] = cc[3 * a + b] + q[a] * q[b];
-                        cn = cn + 1.0f;
-                    }
+    LW_UNROLL for (int gx = -1; gx < 2; ++gx) {
+        int64_t key[9];
+        int32_t h[9];
+        int4 lo[9];
+        float4 acc[9];
+        LW_UNROLL for (int c = 0; c < 9; ++c) {
+            key[c] = voxel_key_ijk(ix + gx, iy + c / 3 - 1, iz + c % 3 - 1);
+            h[c] = slot_hash(key[c]) & mask;
+        }
+        LW_UNROLL for (int c = 0; c < 9; ++c) load_head(recs, h[c], lo[c], acc[c]);
+        // keys whose home holds another key probe on, one slot further per round, all of the slab together
+        uint32_t need = 0u;
+        LW_UNROLL for (int c = 0; c < 9; ++c) {
+            const int64_t k = (int64_t)(((uint64_t)(uint32_t)lo[c].y << 32) | (uint64_t)(uint32_t)lo[c].x);
+            need |= (k != key[c] && k != -1 ? 1u : 0u) << c;
+        }
+        for (int probe = 1; need != 0u && probe < PROBES; ++probe) {
+            LW_UNROLL for (int c = 0; c < 9; ++c) {
+                if (!((need >> c) & 1u)) continue;
+                int4 l;
+                float4 q;
+                load_head(recs, (h[c] + probe) & mask, l, q);
+                const int64_t k = (int64_t)(((uint64_t)(uint32_t)l.y << 32) | (uint64_t)(uint32_t)l.x);
+                if (k == key[c] || k == -1) {
+                    lo[c] = l;
+                    acc[c] = q;
+                    need &= ~(1u << c);
                 }
             }
+        }
+        LW_UNROLL for (int c = 0; c < 9; ++c) {
+            const int64_t k = (int64_t)(((uint64_t)(uint32_t)lo[c].y << 32) | (uint64_t)(uint32_t)lo[c].x);
+            const bool found = ((need >> c) & 1u) == 0u && k == key[c];  // past PROBES: not found
+            if (found && lo[c].z > 0) {
+                const float m = 1.0f / (float)(lo[c].z < IMAP_COUNT_CAP ? lo[c].z : IMAP_COUNT_CAP);
+                const float q[3] = {acc[c].x * m - ox, acc[c].y * m - oy, acc[c].z * m - oz};
+                for (int a = 0; a < 3; ++a) cs[a] = cs[a] + q[a];
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) cc[3 * a + b] = cc[3 * a + b] + q[a] * q[b];
+                cn = cn + 1.0f;
+            }
+        }
+    }
     if (cn < 4.0f) return make_float4(0.0f, 0.0f, 1.0f, 0.0f);
     const float mu[3] = {cs[0] / cn, cs[1] / cn, cs[2] / cn};
     double cov[9];
@@ -154,12 +219,18 @@ __device__ inline float4 local_plane(int32_t ix, int32_t iy, int32_t iz, float o
 
 // gpu.py:470-595 (k_carve): walk every `every`-th return's ray through the map (a 3D DDA from the sensor
 // origin) up to a margin short of the return, and add `every` misses to each occupied voxel whose surface
-// patch the ray pierces.
+// patch the ray pierces. A ray takes CARVE_LANES lanes: each runs the same float32 DDA (so every lane sees the
+// reference's voxel sequence) in rounds of CARVE_LANES steps, and lane j looks up the round's j-th voxel, so
+// the lanes of a ray wait on memory together, once a round, instead of once a voxel.
+constexpr int CARVE_LANES = 8;
 __global__ __launch_bounds__(IMAP_BLOCK) void k_imap_carve(const float *sx, const uint32_t *sa, const int32_t *sf,
                                                            int32_t n, int32_t every, const float *poses,
                                                            float inv_voxel, int32_t mask, float max_range,
-                                                           int32_t max_steps, MapRec *recs) {
-    const int64_t i = (int64_t)(blockIdx.x * IMAP_BLOCK + threadIdx.x) * every;
+                                                           int32_t max_steps, MapRec *recs, const uint32_t *bricks,
+                                                           int32_t nbits) {
+    const int64_t gt = (int64_t)blockIdx.x * IMAP_BLOCK + threadIdx.x;
+    const int32_t j = (int32_t)(gt % CARVE_LANES);
+    const int64_t i = gt / CARVE_LANES * every;
     if (i >= n) return;
     if (tag_of(sa[i]) != 0) return;
     const float *T = poses + (int64_t)(sf[i] & (IMAP_POSE_SLOTS - 1)) * 16;
@@ -213,23 +284,55 @@ __global__ __launch_bounds__(IMAP_BLOCK) void k_imap_carve(const float *sx, cons
         tmz = (pz - (float)iz) / (-dz);
         tdz = -1.0f / dz;
     }
-    for (int32_t step = 0; step < max_steps; ++step) {
-        float t;
-        if (tmx < tmy && tmx < tmz) {
-            t = tmx;
-            ix += sxs;
-            tmx = tmx + tdx;
-        } else if (tmy < tmz) {
-            t = tmy;
-            iy += sys;
-            tmy = tmy + tdy;
-        } else {
-            t = tmz;
-            iz += szs;
-            tmz = tmz + tdz;
+    int32_t bx = INT_MIN, by = 0, bz = 0;  // the brick of this lane's last voxel, and whether it is empty
+    bool empty = true;
+    int32_t step = 0;
+    bool done = false;
+    while (!done) {
+        // a round: CARVE_LANES steps of the DDA (the same on every lane of the ray); this lane keeps the j-th
+        int32_t vx = 0, vy = 0, vz = 0;
+        bool have = false;
+        LW_UNROLL for (int u = 0; u < CARVE_LANES; ++u) {
+            if (!done) {
+                if (step >= max_steps) {
+                    done = true;
+                } else {
+                    ++step;
+                    float t;
+                    if (tmx < tmy && tmx < tmz) {
+                        t = tmx;
+                        ix += sxs;
+                        tmx = tmx + tdx;
+                    } else if (tmy < tmz) {
+                        t = tmy;
+                        iy += sys;
+                        tmy = tmy + tdy;
+                    } else {
+                        t = tmz;
+                        iz += szs;
+                        tmz = tmz + tdz;
+                    }
+                    if (t >= stop) {
+                        done = true;
+                    } else if (u == j) {
+                        vx = ix;
+                        vy = iy;
+                        vz = iz;
+                        have = true;
+                    }
+                }
+            }
         }
-        if (t >= stop) return;
-        const int32_t s = imap_find(recs, mask, voxel_key_ijk(ix, iy, iz));
+        if (!have) continue;
+        if ((vx >> 3) != bx || (vy >> 3) != by || (vz >> 3) != bz) {
+            bx = vx >> 3;
+            by = vy >> 3;
+            bz = vz >> 3;
+            const uint32_t b = brick_bit(voxel_key_ijk(vx, vy, vz), nbits);
+            empty = ((bricks[b >> 5] >> (b & 31)) & 1u) == 0u;
+        }
+        if (empty) continue;  // no voxel of this brick is in the map
+        const int32_t s = imap_find(recs, mask, voxel_key_ijk(vx, vy, vz));
         if (s < 0) continue;
         const MapRec r = recs[s];
         if (r.cnt <= 0) continue;
@@ -237,7 +340,7 @@ __global__ __launch_bounds__(IMAP_BLOCK) void k_imap_carve(const float *sx, cons
         imap_centroid(r, cx, cy, cz);
         const float rx = cx - ox, ry = cy - oy, rz = cz - oz;
         bool pierced = false;
-        const float4 pl = local_plane(ix, iy, iz, cx, cy, cz, recs, mask);
+        const float4 pl = local_plane(vx, vy, vz, cx, cy, cz, recs, mask);
         if (pl.w > 1.5f) {
             float dn = dx * pl.x;
             dn = dn + dy * pl.y;
@@ -278,8 +381,20 @@ int grid_of(int64_t n, int64_t max_grid) {
 
 }  // namespace
 
-hipError_t imap_clear(hipStream_t s, MapRec *recs, int64_t cap, int32_t *stats) {
+int32_t imap_brick_bits(int64_t cap) { return cap / 8 > 1024 ? (int32_t)(cap / 8) : 1024; }
+
+hipError_t imap_clear(hipStream_t s, MapRec *recs, int64_t cap, int32_t *stats, uint32_t *bricks) {
     hipLaunchKernelGGL(k_imap_clear, dim3(grid_of(cap, 8192)), dim3(IMAP_BLOCK), 0, s, recs, cap, stats);
+    const int64_t words = imap_brick_bits(cap) / 32;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_of(words, 4096)), dim3(IMAP_BLOCK), 0, s, bricks, words, 0u);
+    return hipGetLastError();
+}
+
+hipError_t imap_brick_rebuild(hipStream_t s, const MapRec *recs, int64_t cap, uint32_t *bricks) {
+    const int64_t words = imap_brick_bits(cap) / 32;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_of(words, 4096)), dim3(IMAP_BLOCK), 0, s, bricks, words, 0u);
+    hipLaunchKernelGGL(k_brick_rebuild, dim3(grid_of(cap, 8192)), dim3(IMAP_BLOCK), 0, s, recs, cap, bricks,
+                       imap_brick_bits(cap));
     return hipGetLastError();
 }
 
@@ -292,20 +407,20 @@ hipError_t imap_pose_scatter(hipStream_t s, const int32_t *idx, const float *mat
 
 hipError_t imap_insert(hipStream_t s, const float *sx, const uint32_t *sa, const float *st, const int32_t *sf,
                        int32_t n, const float *poses, const MapView &v, float inv_voxel, int32_t mask, MapRec *recs,
-                       int32_t *stats) {
+                       int32_t *stats, uint32_t *bricks) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_imap_insert, dim3(grid_of(n, INT_MAX)), dim3(IMAP_BLOCK), 0, s, sx, sa, st, sf, n, poses, v,
-                       inv_voxel, mask, recs, stats);
+                       inv_voxel, mask, recs, stats, bricks, imap_brick_bits((int64_t)mask + 1));
     return hipGetLastError();
 }
 
 hipError_t imap_carve(hipStream_t s, const float *sx, const uint32_t *sa, const int32_t *sf, int32_t n, int32_t every,
                       const float *poses, float inv_voxel, int32_t mask, float max_range, int32_t max_steps,
-                      MapRec *recs) {
+                      MapRec *recs, const uint32_t *bricks) {
     if (n <= 0) return hipSuccess;
     const int64_t rays = ((int64_t)n + every - 1) / every;
-    hipLaunchKernelGGL(k_imap_carve, dim3(grid_of(rays, INT_MAX)), dim3(IMAP_BLOCK), 0, s, sx, sa, sf, n, every, poses,
-                       inv_voxel, mask, max_range, max_steps, recs);
+    hipLaunchKernelGGL(k_imap_carve, dim3(grid_of(rays * CARVE_LANES, INT_MAX)), dim3(IMAP_BLOCK), 0, s, sx, sa, sf, n, every, poses,
+                       inv_voxel, mask, max_range, max_steps, recs, bricks, imap_brick_bits((int64_t)mask + 1));
     return hipGetLastError();
 }
 
@@ -330,6 +445,7 @@ struct lw_imap {
     lw::MapRec *recs = nullptr;
     int32_t *stats = nullptr;
     float *poses = nullptr;    // IMAP_POSE_SLOTS x 16
+    uint32_t *bricks = nullptr;  // imap_brick_bits(cap) bits
     int32_t *up_idx = nullptr;  // pose upload staging on the device
     float *up_mat = nullptr;
     void *pin = nullptr;       // pinned host staging: the pose uploads, then the stats readback
@@ -349,6 +465,7 @@ constexpr size_t IMAP_PIN = (size_t)lw::IMAP_POSE_SLOTS * (16 * sizeof(float) + 
 void imap_free(lw_imap *m) {
     if (m->recs) (void)hipFree(m->recs);
     if (m->stats) (void)hipFree(m->stats);
+    if (m->bricks) (void)hipFree(m->bricks);
     if (m->poses) (void)hipFree(m->poses);
     if (m->up_idx) (void)hipFree(m->up_idx);
     if (m->up_mat) (void)hipFree(m->up_mat);
@@ -380,6 +497,7 @@ int lw_imap_create(lw_imap **out, int64_t slots, void *stream) {
     m->cap = slots;
     if ((e = hipMalloc((void **)&m->recs, (size_t)slots * sizeof(lw::MapRec))) != hipSuccess) return fail(e);
     if ((e = hipMalloc((void **)&m->stats, 64)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc((void **)&m->bricks, (size_t)lw::imap_brick_bits(slots) / 8)) != hipSuccess) return fail(e);
     if ((e = hipMalloc((void **)&m->poses, (size_t)lw::IMAP_POSE_SLOTS * 16 * sizeof(float))) != hipSuccess)
         return fail(e);
     if ((e = hipMalloc((void **)&m->up_idx, (size_t)lw::IMAP_POSE_SLOTS * sizeof(int32_t))) != hipSuccess)
@@ -391,7 +509,7 @@ int lw_imap_create(lw_imap **out, int64_t slots, void *stream) {
     hipLaunchKernelGGL(k_pose_identity, dim3((lw::IMAP_POSE_SLOTS * 16 + 255) / 256), dim3(256), 0, m->stream,
                        m->poses);
     if ((e = hipGetLastError()) != hipSuccess) return fail(e);
-    if ((e = lw::imap_clear(m->stream, m->recs, m->cap, m->stats)) != hipSuccess) return fail(e);
+    if ((e = lw::imap_clear(m->stream, m->recs, m->cap, m->stats, m->bricks)) != hipSuccess) return fail(e);
     if ((e = hipEventRecord(m->up_done, m->stream)) != hipSuccess) return fail(e);
     if ((e = hipStreamSynchronize(m->stream)) != hipSuccess) return fail(e);
     *out = m;
@@ -408,7 +526,7 @@ int lw_imap_destroy(lw_imap *m) {
 
 int lw_imap_clear(lw_imap *m) {
     if (!m) return LW_EINVAL;
-    IMAP_CK(lw::imap_clear(m->stream, m->recs, m->cap, m->stats));
+    IMAP_CK(lw::imap_clear(m->stream, m->recs, m->cap, m->stats, m->bricks));
     return 0;
 }
 
@@ -432,7 +550,8 @@ int lw_imap_insert(lw_imap *m, const float *sx, const uint32_t *sa, const float 
     if (!m || !v || n < 0 || (n > 0 && (!sx || !sa || !st || !sf))) return LW_EINVAL;
     const lw::MapView mv{v->min_range, v->max_range, v->min_refl, v->noise_mask, v->ret_mask};
     const float inv = 1.0f / (voxel > 1e-4f ? voxel : 1e-4f);  // 1.0 / max(voxel, 1e-4) (gpu.py:1430)
-    IMAP_CK(lw::imap_insert(m->stream, sx, sa, st, sf, n, m->poses, mv, inv, (int32_t)(m->cap - 1), m->recs, m->stats));
+    IMAP_CK(lw::imap_insert(m->stream, sx, sa, st, sf, n, m->poses, mv, inv, (int32_t)(m->cap - 1), m->recs, m->stats,
+                            m->bricks));
     return 0;
 }
 
@@ -442,7 +561,7 @@ int lw_imap_carve(lw_imap *m, const float *sx, const uint32_t *sa, const int32_t
     every = every > 1 ? every : 1;  // max(1, int(every)) (gpu.py:1443)
     const float inv = 1.0f / (voxel > 1e-4f ? voxel : 1e-4f);
     IMAP_CK(lw::imap_carve(m->stream, sx, sa, sf, n, every, m->poses, inv, (int32_t)(m->cap - 1), max_range, max_steps,
-                           m->recs));
+                           m->recs, m->bricks));
     return 0;
 }
 
@@ -472,6 +591,7 @@ int lw_imap_load(lw_imap *m, const void *in, int64_t bytes) {
     if (!m || !in || bytes != m->cap * (int64_t)sizeof(lw::MapRec)) return LW_EINVAL;
     IMAP_CK(hipStreamSynchronize(m->stream));
     IMAP_CK(hipMemcpy(m->recs, in, (size_t)bytes, hipMemcpyHostToDevice));
+    IMAP_CK(lw::imap_brick_rebuild(m->stream, m->recs, m->cap, m->bricks));
     return 0;
 }
 
