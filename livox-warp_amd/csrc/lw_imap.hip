@@ -222,7 +222,7 @@ __device__ inline float4 local_plane(int32_t ix, int32_t iy, int32_t iz, float o
 // patch the ray pierces. A ray takes CARVE_LANES lanes: each runs the same float32 DDA (so every lane sees the
 // reference's voxel sequence) in rounds of CARVE_LANES steps, and lane j looks up the round's j-th voxel, so
 // the lanes of a ray wait on memory together, once a round, instead of once a voxel.
-constexpr int CARVE_LANES = 8;
+constexpr int CARVE_LANES = 16;
 __global__ __launch_bounds__(IMAP_BLOCK) void k_imap_carve(const float *sx, const uint32_t *sa, const int32_t *sf,
                                                            int32_t n, int32_t every, const float *poses,
                                                            float inv_voxel, int32_t mask, float max_range,
