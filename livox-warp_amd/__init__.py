@@ -6,15 +6,17 @@ Import with importlib (the directory name carries a hyphen):
     odo = lw.Odometry(lw.PoseTable())
 
 Submodules: odom (the drop-in Odometry), engine (one context driven directly), pose_table (the pose sink),
-imap (the integration map the frames feed, with free-space carving), loc (batched global localisation), prior (the learned pose-increment prior), synth (seeded synthetic scenes,
+imap (the integration map the frames feed, with free-space carving), worker (the odometry on its own thread
+and stream), loc (batched global localisation), prior (the learned pose-increment prior), synth (seeded synthetic scenes,
 scans and walks), _abi (ctypes bindings of liblwamd.so).
 """
 
-from . import _abi, engine, imap, loc, odom, pose_table, prior, synth  # noqa: F401
+from . import _abi, engine, imap, loc, odom, pose_table, prior, synth, worker  # noqa: F401
 from .engine import Engine  # noqa: F401
 from .imap import IntegrationMap  # noqa: F401
 from .odom import Odometry  # noqa: F401
 from .pose_table import PoseTable  # noqa: F401
+from .worker import OdomWorker  # noqa: F401
 
-__all__ = ["Odometry", "Engine", "PoseTable", "IntegrationMap", "odom", "engine", "imap", "loc", "pose_table", "prior",
-           "synth"]
+__all__ = ["Odometry", "Engine", "PoseTable", "IntegrationMap", "OdomWorker", "odom", "engine", "imap", "loc",
+           "pose_table", "prior", "synth", "worker"]
