@@ -301,6 +301,20 @@ int lw_imap_stats(lw_imap *m, int32_t *out);
 int lw_imap_read(lw_imap *m, void *out, int64_t bytes);
 int lw_imap_load(lw_imap *m, const void *in, int64_t bytes);
 
+/* ---- LVXR recordings (the reference's firing-level reader, python/livox_warp/lvxr.py:76-260) ----------------
+ * Decode n_packets point packets of a recording already in device memory (bytes: the whole file) into rows of
+ * the firing arrays: packet p's datagram starts at byte pk_off[p] (its 18-byte data header first), holds
+ * pk_nf[p] firings of data type pk_type[p] (0-5, 7, 8; dual = 1: the Mid-40 dual profile, two plain records
+ * a firing), starts pk_t_ns[p] ns after the recording's time base, and fills rows pk_first[p] ..
+ * pk_first[p] + pk_nf[p] - 1. Each row: t (float64 s, firing j at pk_t_ns + j interval_ns), `returns`
+ * (1-3) return slots of xyz (float32 m), reflectivity, tag and valid (uint8), and the firing's unit direction
+ * (float32, NaN where none is known). All arrays are device pointers; asynchronous on stream (a hipStream_t).
+ * Replaces read_firings' decoding (lvxr.py:189-260); the host keeps the framing and the time base. */
+int lw_lvxr_decode(const uint8_t *bytes, const int64_t *pk_off, const int32_t *pk_type, const int32_t *pk_nf,
+                   const int64_t *pk_t_ns, const int64_t *pk_first, int32_t n_packets, int32_t dual, int64_t interval_ns,
+                   int32_t returns, double *t, float *xyz, uint8_t *refl, uint8_t *tag, uint8_t *valid, float *dir,
+                   void *stream);
+
 /* Host builds of the device's float64 step and per-voxel precision, for tests on machines without a
  * GPU (same source as the device code). */
 int lw_host_gn_step(const double *v, const double *T, const double *T_pred, double damping, double prior_info,

@@ -119,6 +119,7 @@ SIGNATURES = {
     "lw_imap_stats": (_i32, [_vp, _i32p]),
     "lw_imap_read": (_i32, [_vp, _vp, _i64]),
     "lw_imap_load": (_i32, [_vp, _vp, _i64]),
+    "lw_lvxr_decode": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -1,0 +1,181 @@
+"""LVXR recordings replayed through the device: the reference's firing-level reader (python/livox_warp/lvxr.py)
+with the records decoded on the GPU (lw_lvxr_decode).
+
+File layout (crates/livox/src/record.rs:1-5): `LVXR2\\n`, dev_type:u8, profile:u8 (LVXR1: no profile byte), then
+`host_ns:u64 len:u16 datagram[len]` records. A point datagram is an 18-byte data header (version, slot, id,
+rsvd, status u32, timestamp type, data type, timestamp u64) and records of one data type (cartesian or
+spherical, one to three returns; the Mid-40 dual profile sends plain records in pairs, two returns of one
+firing). The host walks the framing, computes each packet's time on the live path's time base (net.rs Sink:
+reset when the clock goes back, shifted over forward jumps of a second or more) and chooses the packets the
+reference keeps; the whole file goes to the device once and one thread per firing decodes it. Firings come
+back as device tensors, in time order, as the reference's Firings holds them.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, replace
+
+import numpy as np
+
+from . import _abi
+
+HEADER_LEN = 18
+CLOCK_JUMP_NS = 1_000_000_000
+PROFILE_STANDARD, PROFILE_MID40_DUAL = 0, 1
+MID40_DUAL_RECORDS = 100
+# data type -> (bytes per record, returns per record, spherical)
+LAYOUTS = {0: (13, 1, False), 1: (9, 1, True), 2: (14, 1, False), 3: (10, 1, True), 4: (28, 2, False),
+           5: (16, 2, True), 7: (42, 3, False), 8: (22, 3, True)}
+FAST_DEVICES = {2, 3, 7}  # 240k samples/s; the rest 100k
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+@dataclass
+class Firings:
+    """Every firing of a recording, in time order (device tensors). R is the most returns any firing carries."""
+
+    t: object  # (n,) float64 s since the time base
+    xyz: object  # (n, R, 3) float32 m, sensor frame; zero where a return slot is empty
+    refl: object  # (n, R) uint8
+    tag: object  # (n, R) uint8
+    valid: object  # (n, R) bool
+    direction: object  # (n, 3) float32 unit firing direction, NaN where unknown
+    dev_type: int
+    profile: int
+    interval: float  # s between firings
+
+    @property
+    def returns(self) -> int:
+        return self.xyz.shape[1]
+
+    @property
+    def misses(self):
+        return ~self.valid.any(dim=1)
+
+    def head(self, seconds: float) -> Firings:
+        k = self.t < self.t[0] + seconds
+        return replace(self, t=self.t[k], xyz=self.xyz[k], refl=self.refl[k], tag=self.tag[k], valid=self.valid[k],
+                       direction=self.direction[k])
+
+    def points(self, clean: bool = True, min_range: float = 0.3):
+        """The returns as flat device tensors (xyz, attr, t float64, firing index), in time order; attr packs
+        reflectivity, tag << 8 and return << 16. clean: only untagged returns beyond min_range."""
+        torch = _torch()
+        f, r = torch.nonzero(self.valid, as_tuple=True)  # firing-major, as the firings are time-ordered
+        xyz = self.xyz[f, r]
+        tag = self.tag[f, r].to(torch.int32)
+        attr = self.refl[f, r].to(torch.int32) | (tag << 8) | (r.to(torch.int32) << 16)
+        if clean:
+            keep = (tag == 0) & (torch.linalg.vector_norm(xyz, dim=1) > min_range)
+            f, xyz, attr = f[keep], xyz[keep], attr[keep]
+        return xyz.contiguous(), attr.contiguous(), self.t[f].contiguous(), f
+
+
+def _framing(data: bytes):
+    """(dev_type, profile, datagram offsets, lengths): the records of the file, a truncated last one dropped."""
+    if data[:6] == b"LVXR1\n":
+        dev, profile, off = data[6], PROFILE_STANDARD, 7
+    elif data[:6] == b"LVXR2\n":
+        dev, profile, off = data[6], data[7], 8
+    else:
+        raise ValueError("not an LVXR recording")
+    offs, lens = [], []
+    n = len(data)
+    mv = memoryview(data)
+    while off + 10 <= n:
+        ln = mv[off + 8] | (mv[off + 9] << 8)
+        if off + 10 + ln > n:
+            break
+        offs.append(off + 10)
+        lens.append(ln)
+        off += 10 + ln
+    return int(dev), int(profile), np.array(offs, np.int64), np.array(lens, np.int64)
+
+
+def _stamps(buf: np.ndarray, offs: np.ndarray) -> np.ndarray:
+    """Each packet's header timestamp in ns (type 3, GPS: hour and microseconds of the hour)."""
+    le = np.array([1 << (8 * j) for j in range(8)], np.uint64)
+    raw = buf[offs[:, None] + 10 + np.arange(8)].astype(np.uint64)
+    ts = (raw * le).sum(axis=1, dtype=np.uint64)
+    gps = buf[offs + 8] == 3
+    if gps.any():
+        hour = buf[offs[gps] + 13].astype(np.uint64)
+        us = (buf[offs[gps][:, None] + 14 + np.arange(4)].astype(np.uint64) * le[:4]).sum(axis=1, dtype=np.uint64)
+        ts[gps] = (hour * np.uint64(3_600_000_000) + us) * np.uint64(1000)
+    return ts
+
+
+def _clock(ts: np.ndarray, spans: np.ndarray) -> np.ndarray:
+    """ns since the time base per packet: a packet earlier than the base starts a new base; one more than half
+    a packet late by a second or more moves the base by the gap, so time stays continuous."""
+    out = np.empty(len(ts), np.int64)
+    base, expected = None, None
+    for i, (t, span) in enumerate(zip(ts.tolist(), spans.tolist())):
+        if base is None or t < base:
+            base, expected = t, None
+        if expected is not None and span > 0 and t > expected + span // 2 and t - expected >= CLOCK_JUMP_NS:
+            base += t - expected
+        expected = t + span
+        out[i] = t - base
+    return out
+
+
+def read_firings(path: str | os.PathLike, device=None, stream=None) -> Firings:
+    """Every firing of an LVXR recording, decoded on `device` (the reference's read_firings)."""
+    torch = _torch()
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    with open(path, "rb") as f:
+        data = f.read()
+    dev_type, profile, offs, lens = _framing(data)
+    buf = np.frombuffer(data, np.uint8)
+    interval = 4_167 if dev_type in FAST_DEVICES else 10_000
+    dual = profile == PROFILE_MID40_DUAL
+    ok = lens >= HEADER_LEN
+    offs, lens = offs[ok], lens[ok]
+    dtypes = buf[offs + 9].astype(np.int64)
+    point = np.isin(dtypes, list(LAYOUTS))
+    offs, lens, dtypes = offs[point], lens[point], dtypes[point]
+    sizes = np.array([LAYOUTS[int(d)][0] for d in dtypes], np.int64)
+    records = (lens - HEADER_LEN) // np.maximum(sizes, 1)
+    # every point packet moves the clock, the malformed ones too, before any is rejected
+    fpp = records // 2 if dual else records
+    t_pk = _clock(_stamps(buf, offs), fpp * interval) if len(offs) else np.zeros(0, np.int64)
+    whole = (lens - HEADER_LEN) == records * sizes
+    if dual:
+        whole &= np.isin(dtypes, [0, 1]) & (records == MID40_DUAL_RECORDS)
+    offs, lens, dtypes, t_pk, fpp = offs[whole], lens[whole], dtypes[whole], t_pk[whole], fpp[whole]
+    if not len(offs):
+        raise ValueError(f"{path}: no point packets")
+    R = max([LAYOUTS[int(d)][1] for d in np.unique(dtypes)] + [2 if dual else 1])
+    # rows in the reference's concatenation order (by data type, then datagram length, then file order), so a
+    # stable sort by time breaks ties as it does
+    order = np.lexsort((np.arange(len(offs)), lens, dtypes))
+    first = np.zeros(len(offs), np.int64)
+    first[order] = np.concatenate([[0], np.cumsum(fpp[order])[:-1]])
+    n = int(fpp.sum())
+    bytes_dev = torch.from_numpy(buf.copy()).to(dev)
+    tab = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in
+           (offs, dtypes.astype(np.int32), fpp.astype(np.int32), t_pk, first)]
+    t = torch.empty(n, dtype=torch.float64, device=dev)
+    xyz = torch.empty((n, R, 3), dtype=torch.float32, device=dev)
+    refl = torch.empty((n, R), dtype=torch.uint8, device=dev)
+    tag = torch.empty((n, R), dtype=torch.uint8, device=dev)
+    valid = torch.empty((n, R), dtype=torch.uint8, device=dev)
+    direction = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    _abi.check(_abi.lib().lw_lvxr_decode(bytes_dev.data_ptr(), *[a.data_ptr() for a in tab], len(offs),
+                                         1 if dual else 0, interval, R, t.data_ptr(), xyz.data_ptr(), refl.data_ptr(),
+                                         tag.data_ptr(), valid.data_ptr(), direction.data_ptr(),
+                                         ctypes.c_void_p(s.cuda_stream)), "lw_lvxr_decode")
+    with torch.cuda.stream(s):
+        idx = torch.sort(t, stable=True).indices
+        out = Firings(t=t[idx], xyz=xyz[idx], refl=refl[idx], tag=tag[idx], valid=valid[idx].bool(),
+                      direction=direction[idx], dev_type=dev_type, profile=profile, interval=interval * 1e-9)
+    return out
