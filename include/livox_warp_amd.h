@@ -315,6 +315,35 @@ int lw_lvxr_decode(const uint8_t *bytes, const int64_t *pk_off, const int32_t *p
                    int32_t returns, double *t, float *xyz, uint8_t *refl, uint8_t *tag, uint8_t *valid, float *dir,
                    void *stream);
 
+/* ---- The prior map's dense grid and coarse scoring (python/livox_warp/mapgrid.py, localize.py:55-123) -----
+ * Geometry, host pointers read at the call: geo = {origin x, y, z, cell, 1 / cell} (float32), dims = {nx, ny,
+ * nz} with nx ny nz < 2^31; cell c = (iz ny + iy) nx + ix. Map points (xyz, n < 2^23) are sorted by cell and
+ * start (ncell + 1 int32) is the prefix of the per-cell counts (PriorGrid.prepare, mapgrid.py:215-241). Every
+ * other array is a device pointer; each call is asynchronous on stream (a hipStream_t).
+ *
+ * lw_grid_build: the seeds nn (int32 per cell: the map point nearest the cell centre, -1 beyond the band) and
+ * the distance field df (uint8 cm per cell, 255 beyond the band): seed, unpack, the n_steps jump-flood steps
+ * (host array, 8 4 2 1 1 in the reference), each reading every seed before writing any (nn_tmp: a second
+ * ncell int32 array, scratch), then df. Replaces PriorGrid.build's launches (mapgrid.py:275-290). */
+int lw_grid_build(const float *xyz, int32_t n, const float *geo, const int32_t *dims, const int32_t *steps,
+                  int32_t n_steps, int32_t *nn, int32_t *nn_tmp, uint8_t *df, void *stream);
+/* Distance of each of n points, placed in the map by T (host, row-major 4x4 float32), to the nearest map point
+ * (exact within one cell, else through the seeds), 1e3 beyond the band. Replaces PriorGrid.distances /
+ * k_changes_T (mapgrid.py:170-193, 292-300). */
+int lw_grid_distances(const float *pts, int32_t n, const float *T, const float *geo, const int32_t *dims,
+                      const int32_t *start, const int32_t *nn, const float *xyz, float *out, void *stream);
+/* The soft count (sum over the n_pts points within tau_cm of the map of 1 - d^2 / tau^2, d from df) of every
+ * (position, heading): out[p n_yaw + k] for position p (x, y, z; gravity-aligned, list them x-fastest for
+ * speed) and heading yaw0 + k dyaw; -1 where the position is outside the grid or within free_cm of a mapped
+ * surface. Replaces k_score_grid (localize.py:55-92). */
+int lw_score_grid(const float *pts, int32_t n_pts, const float *pos, int32_t n_pos, int32_t n_yaw, float yaw0,
+                  float dyaw, const float *geo, const int32_t *dims, const uint8_t *df, float tau_cm, float free_cm,
+                  float *out, void *stream);
+/* The same soft count for a list of (x, y, z, yaw) hypotheses (hyp 16-byte aligned). Replaces k_score_list
+ * (localize.py:95-123). */
+int lw_score_list(const float *pts, int32_t n_pts, const float *hyp, int32_t n_hyp, const float *geo,
+                  const int32_t *dims, const uint8_t *df, float tau_cm, float *out, void *stream);
+
 /* Host builds of the device's float64 step and per-voxel precision, for tests on machines without a
  * GPU (same source as the device code). */
 int lw_host_gn_step(const double *v, const double *T, const double *T_pred, double damping, double prior_info,

@@ -7,17 +7,20 @@ Import with importlib (the directory name carries a hyphen):
 
 Submodules: odom (the drop-in Odometry), engine (one context driven directly), pose_table (the pose sink),
 imap (the integration map the frames feed, with free-space carving), worker (the odometry on its own thread
-and stream), lvxr (LVXR recordings decoded on the device), loc (batched global localisation), prior (the
+and stream), lvxr (LVXR recordings decoded on the device), mapgrid (the prior map's distance-field grid),
+localize (the coarse stages of global localisation over it), loc (batched global localisation), prior (the
 learned pose-increment prior), synth (seeded synthetic scenes, scans and walks), _abi (ctypes bindings of
 liblwamd.so).
 """
 
-from . import _abi, engine, imap, loc, lvxr, odom, pose_table, prior, synth, worker  # noqa: F401
+from . import _abi, engine, imap, loc, localize, lvxr, mapgrid, odom, pose_table, prior, synth, worker  # noqa: F401
 from .engine import Engine  # noqa: F401
 from .imap import IntegrationMap  # noqa: F401
+from .localize import CoarseSearch  # noqa: F401
+from .mapgrid import PriorGrid  # noqa: F401
 from .odom import Odometry  # noqa: F401
 from .pose_table import PoseTable  # noqa: F401
 from .worker import OdomWorker  # noqa: F401
 
 __all__ = ["Odometry", "Engine", "PoseTable", "IntegrationMap", "OdomWorker", "odom", "engine", "imap", "loc",
-           "pose_table", "prior", "synth", "worker", "lvxr"]
+           "pose_table", "prior", "synth", "worker", "lvxr", "mapgrid", "localize", "PriorGrid", "CoarseSearch"]

@@ -1,0 +1,385 @@
+// The prior map's dense grid and the coarse scoring of global localisation (the reference's
+// python/livox_warp/mapgrid.py:19-193 and localize.py:55-123).
+//
+// The grid holds, per 5 cm cell, the map point nearest the cell centre (seeded by the points in the cell,
+// then jump-flooded over a band of about 0.8 m) and that distance in cm as a byte: the distance field the
+// global search scores (position, heading) hypotheses with. Cells are x-fastest: c = (iz ny + iy) nx + ix.
+//
+// MI355X layout choices:
+//  - the jump flood reads every seed of a step before it writes any (two seed arrays, swapped each step),
+//    so a build is deterministic; the reference updates in place, and this order is one its launch admits;
+//  - k_score_grid gives the 64 lanes of a wave 64 consecutive positions and one heading at a time: the
+//    caller lists positions x-fastest, so for every scan point the wave's 64 distance-field reads fall in a
+//    few consecutive cache lines (0.25 m apart = 5 bytes). Each thread carries four headings, so four
+//    independent reads are in flight per point; the scan's points are staged in LDS and read by broadcast;
+//  - each hypothesis's sum runs over the points in order in one thread, float32 as the reference's, so the
+//    scores are exact, not merely close.
+#include <algorithm>
+#include <cmath>
+#include <utility>
+
+#include "../../include/livox_warp_amd.h"
+#include "lw_common.h"
+
+namespace lw {
+
+namespace {
+
+constexpr int GRID_IDX_BITS = 23;            // mapgrid.py:20
+constexpr int32_t GRID_EMPTY = 0x7FFFFFFF;   // mapgrid.py:24
+constexpr int SCORE_WAVES = 4;               // waves per workgroup, one heading group each
+constexpr int SCORE_YPT = 4;                 // headings per thread
+constexpr int SCORE_CHUNK = 1024;            // scan points staged in LDS per round
+
+struct GridGeo {
+    float ox, oy, oz, cell, inv;
+    int32_t nx, ny, nz;
+};
+
+struct Mat34 {
+    float m[12];
+};
+
+__device__ inline int32_t grid_flat(const GridGeo &g, int32_t ix, int32_t iy, int32_t iz) {
+    if (ix < 0 || iy < 0 || iz < 0 || ix >= g.nx || iy >= g.ny || iz >= g.nz) return -1;
+    return (iz * g.ny + iy) * g.nx + ix;
+}
+
+__device__ inline int32_t grid_at(const GridGeo &g, float x, float y, float z) {
+    return grid_flat(g, ifloor((x - g.ox) * g.inv), ifloor((y - g.oy) * g.inv), ifloor((z - g.oz) * g.inv));
+}
+
+__device__ inline void cell_ijk(const GridGeo &g, int32_t c, int32_t &ix, int32_t &iy, int32_t &iz) {
+    ix = c % g.nx;
+    const int32_t r = c / g.nx;
+    iy = r % g.ny;
+    iz = r / g.ny;
+}
+
+__device__ inline void cell_centre(const GridGeo &g, int32_t ix, int32_t iy, int32_t iz, float &x, float &y,
+                                   float &z) {
+    x = g.ox + ((float)ix + 0.5f) * g.cell;
+    y = g.oy + ((float)iy + 0.5f) * g.cell;
+    z = g.oz + ((float)iz + 0.5f) * g.cell;
+}
+
+__device__ inline float dist2_to(float x, float y, float z, const float *q) {
+    const float d0 = x - q[0], d1 = y - q[1], d2 = z - q[2];
+    return d0 * d0 + d1 * d1 + d2 * d2;
+}
+
+__global__ __launch_bounds__(256) void k_grid_fill(int32_t *a, int64_t n, int32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = v;
+}
+
+// mapgrid.py:103-112: each point claims its cell; the smallest (mm to the centre, index) key wins.
+__global__ __launch_bounds__(256) void k_grid_seed(const float *xyz, int32_t n, GridGeo g, int32_t *nn) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *p = xyz + 3 * (int64_t)i;
+    const int32_t c = grid_at(g, p[0], p[1], p[2]);
+    if (c < 0) return;
+    int32_t ix, iy, iz;
+    float cx, cy, cz;
+    cell_ijk(g, c, ix, iy, iz);
+    cell_centre(g, ix, iy, iz, cx, cy, cz);
+    const float cc[3] = {cx, cy, cz};
+    const int32_t dmm = min((int32_t)(sqrtf(dist2_to(p[0], p[1], p[2], cc)) * 1000.0f), 255);
+    atomicMin(nn + c, (dmm << GRID_IDX_BITS) | i);
+}
+
+// mapgrid.py:115-123: the winning key's point index, -1 for an unclaimed cell.
+__global__ __launch_bounds__(256) void k_grid_unpack(int32_t *nn, int64_t n) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = nn[c];
+        nn[c] = v == GRID_EMPTY ? -1 : (v & ((1 << GRID_IDX_BITS) - 1));
+    }
+}
+
+// mapgrid.py:126-155: the nearest of the cell's own seed and the 26 seeds `step` cells away (src -> dst).
+__global__ __launch_bounds__(256) void k_grid_jfa(const int32_t *__restrict__ src, int32_t *__restrict__ dst,
+                                                  const float *__restrict__ xyz, GridGeo g, int32_t step) {
+    const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t cl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cl >= ncell) return;
+    const int32_t c = (int32_t)cl;
+    int32_t ix, iy, iz;
+    float px, py, pz;
+    cell_ijk(g, c, ix, iy, iz);
+    cell_centre(g, ix, iy, iz, px, py, pz);
+    int32_t best = src[c];
+    float bd = 1.0e30f;
+    if (best >= 0) bd = dist2_to(px, py, pz, xyz + 3 * (int64_t)best);
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int32_t j = grid_flat(g, ix + dx * step, iy + dy * step, iz + dz * step);
+                if (j < 0 || j == c) continue;
+                const int32_t s = src[j];
+                if (s >= 0 && s != best) {
+                    const float d2 = dist2_to(px, py, pz, xyz + 3 * (int64_t)s);
+                    if (d2 < bd) {
+                        bd = d2;
+                        best = s;
+                    }
+                }
+            }
+    dst[c] = best;
+}
+
+// mapgrid.py:158-167: the cell centre's distance to its seed in cm (rounded), 255 beyond the band.
+__global__ __launch_bounds__(256) void k_grid_df(const int32_t *nn, const float *xyz, GridGeo g, uint8_t *df) {
+    const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t cl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cl >= ncell) return;
+    const int32_t c = (int32_t)cl;
+    const int32_t s = nn[c];
+    if (s < 0) {
+        df[c] = 255;
+        return;
+    }
+    int32_t ix, iy, iz;
+    float cx, cy, cz;
+    cell_ijk(g, c, ix, iy, iz);
+    cell_centre(g, ix, iy, iz, cx, cy, cz);
+    const float d = sqrtf(dist2_to(cx, cy, cz, xyz + 3 * (int64_t)s)) * 100.0f;
+    df[c] = (uint8_t)min((int32_t)(d + 0.5f), 255);
+}
+
+// mapgrid.py:65-100 nearest_point: exact within one cell, else the nearest of the 27 cells' seeds, else -1.
+__device__ int32_t nearest_point(const GridGeo &g, const int32_t *start, const int32_t *nn, const float *xyz, float wx,
+                                 float wy, float wz) {
+    const int32_t ix = ifloor((wx - g.ox) * g.inv), iy = ifloor((wy - g.oy) * g.inv), iz = ifloor((wz - g.oz) * g.inv);
+    int32_t best = -1, seed = -1;
+    float bd = g.cell * g.cell, sd = 1.0e30f;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int32_t c = grid_flat(g, ix + dx, iy + dy, iz + dz);
+                if (c < 0) continue;
+                const int32_t k1 = start[c + 1];
+                for (int32_t k = start[c]; k < k1; ++k) {
+                    const float d2 = dist2_to(wx, wy, wz, xyz + 3 * (int64_t)k);
+                    if (d2 < bd) {
+                        bd = d2;
+                        best = k;
+                    }
+                }
+                if (best < 0) {
+                    const int32_t s = nn[c];
+                    if (s >= 0) {
+                        const float d2 = dist2_to(wx, wy, wz, xyz + 3 * (int64_t)s);
+                        if (d2 < sd) {
+                            sd = d2;
+                            seed = s;
+                        }
+                    }
+                }
+            }
+    return best >= 0 ? best : seed;
+}
+
+// mapgrid.py:170-193 k_changes_T: each point placed by T, its distance to the map, 1e3 beyond the band.
+__global__ __launch_bounds__(256) void k_grid_distances(const float *pts, int32_t n, Mat34 T, GridGeo g,
+                                                        const int32_t *start, const int32_t *nn, const float *xyz,
+                                                        float *out) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *p = pts + 3 * (int64_t)i;
+    float w[3];
+    for (int r = 0; r < 3; ++r) w[r] = T.m[4 * r] * p[0] + T.m[4 * r + 1] * p[1] + T.m[4 * r + 2] * p[2] + T.m[4 * r + 3];
+    const int32_t s = nearest_point(g, start, nn, xyz, w[0], w[1], w[2]);
+    out[i] = s >= 0 ? sqrtf(dist2_to(w[0], w[1], w[2], xyz + 3 * (int64_t)s)) : 1.0e3f;
+}
+
+// Stage points [base, base + cnt) into LDS (x, y, z interleaved).
+__device__ inline void stage_points(float *sp, const float *pts, int32_t base, int32_t cnt) {
+    for (int k = threadIdx.x; k < 3 * cnt; k += blockDim.x) sp[k] = pts[3 * (int64_t)base + k];
+}
+
+// localize.py:55-92 k_score_grid. Workgroup (x, y): positions 64 x .. 64 x + 63 (one per lane), headings of
+// groups SCORE_WAVES y .. (one group of SCORE_YPT per wave). Positions outside the grid or within free_cm of
+// a mapped surface score -1.
+__global__ __launch_bounds__(64 * SCORE_WAVES) void k_score_grid(const float *__restrict__ pts, int32_t n_pts,
+                                                                 const float *__restrict__ pos, int32_t n_pos,
+                                                                 int32_t n_yaw, float yaw0, float dyaw, GridGeo g,
+                                                                 const uint8_t *__restrict__ df, float tau_cm,
+                                                                 float free_cm, float *__restrict__ out) {
+    __shared__ float sp[3 * SCORE_CHUNK];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int32_t ip = blockIdx.x * 64 + lane;
+    const int32_t iy0 = (blockIdx.y * SCORE_WAVES + wave) * SCORE_YPT;
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    bool active = false;
+    if (ip < n_pos && iy0 < n_yaw) {
+        cx = pos[3 * (int64_t)ip];
+        cy = pos[3 * (int64_t)ip + 1];
+        cz = pos[3 * (int64_t)ip + 2];
+        const int32_t cc = grid_at(g, cx, cy, cz);
+        active = cc >= 0 && !((float)df[cc] < free_cm);
+        if (!active)
+            for (int y = 0; y < SCORE_YPT && iy0 + y < n_yaw; ++y) out[(int64_t)ip * n_yaw + iy0 + y] = -1.0f;
+    }
+    float co[SCORE_YPT], si[SCORE_YPT], s[SCORE_YPT];
+#pragma unroll
+    for (int y = 0; y < SCORE_YPT; ++y) {
+        double sd, cd;
+        sincos((double)(yaw0 + (float)(iy0 + y) * dyaw), &sd, &cd);
+        co[y] = (float)cd;
+        si[y] = (float)sd;
+        s[y] = 0.0f;
+    }
+    const float inv_t2 = 1.0f / (tau_cm * tau_cm);
+    for (int32_t base = 0; base < n_pts; base += SCORE_CHUNK) {
+        const int32_t cnt = min(SCORE_CHUNK, n_pts - base);
+        __syncthreads();
+        stage_points(sp, pts, base, cnt);
+        __syncthreads();
+        if (!active) continue;
+#pragma unroll 2
+        for (int32_t k = 0; k < cnt; ++k) {
+            const float px = sp[3 * k], py = sp[3 * k + 1], pz = sp[3 * k + 2];
+            const int32_t iz = ifloor((cz + pz - g.oz) * g.inv);
+            int32_t j[SCORE_YPT];
+#pragma unroll
+            for (int y = 0; y < SCORE_YPT; ++y) {
+                const float wx = cx + co[y] * px - si[y] * py, wy = cy + si[y] * px + co[y] * py;
+                j[y] = grid_flat(g, ifloor((wx - g.ox) * g.inv), ifloor((wy - g.oy) * g.inv), iz);
+            }
+            uint32_t v[SCORE_YPT];
+#pragma unroll
+            for (int y = 0; y < SCORE_YPT; ++y) v[y] = j[y] >= 0 ? (uint32_t)df[j[y]] : 0u;
+#pragma unroll
+            for (int y = 0; y < SCORE_YPT; ++y) {
+                const float d = (float)v[y];
+                if (j[y] >= 0 && d < tau_cm) s[y] += 1.0f - d * d * inv_t2;
+            }
+        }
+    }
+    if (active)
+        for (int y = 0; y < SCORE_YPT && iy0 + y < n_yaw; ++y) out[(int64_t)ip * n_yaw + iy0 + y] = s[y];
+}
+
+// localize.py:95-123 k_score_list: one thread per (x, y, z, yaw) hypothesis.
+__global__ __launch_bounds__(256) void k_score_list(const float *__restrict__ pts, int32_t n_pts,
+                                                    const float *__restrict__ hyp, int32_t n_hyp, GridGeo g,
+                                                    const uint8_t *__restrict__ df, float tau_cm,
+                                                    float *__restrict__ out) {
+    __shared__ float sp[3 * SCORE_CHUNK];
+    const int32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = h < n_hyp;
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f, co = 1.0f, si = 0.0f, s = 0.0f;
+    if (active) {
+        const float4 q = reinterpret_cast<const float4 *>(hyp)[h];
+        cx = q.x;
+        cy = q.y;
+        cz = q.z;
+        double sd, cd;
+        sincos((double)q.w, &sd, &cd);
+        co = (float)cd;
+        si = (float)sd;
+    }
+    const float inv_t2 = 1.0f / (tau_cm * tau_cm);
+    for (int32_t base = 0; base < n_pts; base += SCORE_CHUNK) {
+        const int32_t cnt = min(SCORE_CHUNK, n_pts - base);
+        __syncthreads();
+        stage_points(sp, pts, base, cnt);
+        __syncthreads();
+        if (!active) continue;
+#pragma unroll 4
+        for (int32_t k = 0; k < cnt; ++k) {
+            const float px = sp[3 * k], py = sp[3 * k + 1], pz = sp[3 * k + 2];
+            const float wx = cx + co * px - si * py, wy = cy + si * px + co * py, wz = cz + pz;
+            const int32_t j = grid_at(g, wx, wy, wz);
+            if (j >= 0) {
+                const float d = (float)df[j];
+                if (d < tau_cm) s += 1.0f - d * d * inv_t2;
+            }
+        }
+    }
+    if (active) out[h] = s;
+}
+
+bool geo_of(const float *geo, const int32_t *dims, GridGeo &g) {
+    if (!geo || !dims || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return false;
+    if ((int64_t)dims[0] * dims[1] * dims[2] > (int64_t)INT32_MAX - 1) return false;
+    if (!(geo[3] > 0.0f) || !(geo[4] > 0.0f)) return false;
+    g = GridGeo{geo[0], geo[1], geo[2], geo[3], geo[4], dims[0], dims[1], dims[2]};
+    return true;
+}
+
+int launched() {
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : -(int)e;
+}
+
+unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
+
+}  // namespace
+
+}  // namespace lw
+
+using lw::GridGeo;
+
+extern "C" int lw_grid_build(const float *xyz, int32_t n, const float *geo, const int32_t *dims, const int32_t *steps,
+                             int32_t n_steps, int32_t *nn, int32_t *nn_tmp, uint8_t *df, void *stream) {
+    GridGeo g;
+    if (!lw::geo_of(geo, dims, g) || n < 0 || n >= (1 << lw::GRID_IDX_BITS) || n_steps < 0 || n_steps > 32 || !nn ||
+        !nn_tmp || !df || nn == nn_tmp || (n > 0 && !xyz) || (n_steps > 0 && !steps))
+        return LW_EINVAL;
+    for (int k = 0; k < n_steps; ++k)
+        if (steps[k] <= 0) return LW_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
+    // the steps alternate the two arrays and end in nn
+    int32_t *a = (n_steps % 2) ? nn_tmp : nn, *b = (n_steps % 2) ? nn : nn_tmp;
+    const unsigned fill_blocks = (unsigned)std::min<int64_t>(lw::blocks_for(ncell), 65536);
+    hipLaunchKernelGGL(lw::k_grid_fill, dim3(fill_blocks), dim3(256), 0, s, a, ncell, lw::GRID_EMPTY);
+    if (n > 0) hipLaunchKernelGGL(lw::k_grid_seed, dim3(lw::blocks_for(n)), dim3(256), 0, s, xyz, n, g, a);
+    hipLaunchKernelGGL(lw::k_grid_unpack, dim3(fill_blocks), dim3(256), 0, s, a, ncell);
+    for (int k = 0; k < n_steps; ++k) {
+        hipLaunchKernelGGL(lw::k_grid_jfa, dim3(lw::blocks_for(ncell)), dim3(256), 0, s, a, b, xyz, g, steps[k]);
+        std::swap(a, b);
+    }
+    hipLaunchKernelGGL(lw::k_grid_df, dim3(lw::blocks_for(ncell)), dim3(256), 0, s, nn, xyz, g, df);
+    return lw::launched();
+}
+
+extern "C" int lw_grid_distances(const float *pts, int32_t n, const float *T, const float *geo, const int32_t *dims,
+                                 const int32_t *start, const int32_t *nn, const float *xyz, float *out, void *stream) {
+    GridGeo g;
+    if (!lw::geo_of(geo, dims, g) || n < 0 || !T) return LW_EINVAL;
+    if (n == 0) return 0;
+    if (!pts || !start || !nn || !xyz || !out) return LW_EINVAL;
+    lw::Mat34 m;
+    for (int k = 0; k < 12; ++k) m.m[k] = T[k];
+    hipLaunchKernelGGL(lw::k_grid_distances, dim3(lw::blocks_for(n)), dim3(256), 0, (hipStream_t)stream, pts, n, m, g,
+                       start, nn, xyz, out);
+    return lw::launched();
+}
+
+extern "C" int lw_score_grid(const float *pts, int32_t n_pts, const float *pos, int32_t n_pos, int32_t n_yaw,
+                             float yaw0, float dyaw, const float *geo, const int32_t *dims, const uint8_t *df,
+                             float tau_cm, float free_cm, float *out, void *stream) {
+    GridGeo g;
+    if (!lw::geo_of(geo, dims, g) || n_pts < 0 || n_pos < 0 || n_yaw <= 0 || !(tau_cm > 0.0f)) return LW_EINVAL;
+    if (n_pos == 0) return 0;
+    if ((n_pts > 0 && !pts) || !pos || !df || !out) return LW_EINVAL;
+    const int32_t groups = (n_yaw + lw::SCORE_YPT - 1) / lw::SCORE_YPT;
+    const dim3 grid((unsigned)((n_pos + 63) / 64), (unsigned)((groups + lw::SCORE_WAVES - 1) / lw::SCORE_WAVES));
+    hipLaunchKernelGGL(lw::k_score_grid, grid, dim3(64 * lw::SCORE_WAVES), 0, (hipStream_t)stream, pts, n_pts, pos,
+                       n_pos, n_yaw, yaw0, dyaw, g, df, tau_cm, free_cm, out);
+    return lw::launched();
+}
+
+extern "C" int lw_score_list(const float *pts, int32_t n_pts, const float *hyp, int32_t n_hyp, const float *geo,
+                             const int32_t *dims, const uint8_t *df, float tau_cm, float *out, void *stream) {
+    GridGeo g;
+    if (!lw::geo_of(geo, dims, g) || n_pts < 0 || n_hyp < 0 || !(tau_cm > 0.0f)) return LW_EINVAL;
+    if (n_hyp == 0) return 0;
+    if ((n_pts > 0 && !pts) || !hyp || !df || !out || ((uintptr_t)hyp & 15)) return LW_EINVAL;
+    hipLaunchKernelGGL(lw::k_score_list, dim3(lw::blocks_for(n_hyp)), dim3(256), 0, (hipStream_t)stream, pts, n_pts,
+                       hyp, n_hyp, g, df, tau_cm, out);
+    return lw::launched();
+}

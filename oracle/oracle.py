@@ -51,6 +51,7 @@ _f64p = ctypes.POINTER(ctypes.c_double)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _i64p = ctypes.POINTER(ctypes.c_int64)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
 _libs = {}
 LIB64_PATH = os.path.join(HERE, "liblw_oracle64.so")
 
@@ -75,6 +76,13 @@ _SIGS = {
     "lwo_local_plane": (None, [ctypes.c_int32] * 3 + [_f32p, _i64p, _f32p, _i32p, ctypes.c_int32, _f32p]),
     "lwo_carve": (None, [_f32p, _u32p, _i32p, _f32p, _I, _I, ctypes.c_float, ctypes.c_int32, ctypes.c_float, _I,
                          _i64p, _f32p, _i32p, _i32p]),
+    "lwo_grid_build": (None, [_f32p, ctypes.c_int32, _f32p, _i32p, _i32p, ctypes.c_int32, _i32p, _i32p, _u8p]),
+    "lwo_grid_nearest": (ctypes.c_int32, [_f32p, _f32p, _i32p, _i32p, _i32p, _f32p]),
+    "lwo_grid_distances": (None, [_f32p, ctypes.c_int32, _f32p, _f32p, _i32p, _i32p, _i32p, _f32p, _f32p]),
+    "lwo_score_grid": (None, [_f32p, ctypes.c_int32, _f32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                              ctypes.c_float, _f32p, _i32p, _u8p, ctypes.c_float, ctypes.c_float, _f32p]),
+    "lwo_score_list": (None, [_f32p, ctypes.c_int32, _f32p, ctypes.c_int32, _f32p, _i32p, _u8p, ctypes.c_float,
+                              _f32p]),
 }
 
 
@@ -724,3 +732,68 @@ def carve(m: IntegrationMap, xyz, attr, frame, poses, voxel: float, every: int =
 
 def identity_poses() -> np.ndarray:
     return np.tile(np.eye(4, dtype=np.float32), (POSE_SLOTS, 1, 1))
+
+
+# ---- the prior-map grid and its coarse scoring (mapgrid.py, localize.py:55-123) ----------------------
+
+GRID_JFA_STEPS = (8, 4, 2, 1, 1)  # mapgrid.py:25
+
+
+def _geo(origin, cell: float):
+    """The grid geometry as the restatement takes it: origin x, y, z, cell and 1 / cell in float32."""
+    return np.array([*np.asarray(origin, np.float32), cell, 1.0 / cell], np.float32)
+
+
+def grid_build(xyz, origin, cell: float, dims, steps=GRID_JFA_STEPS):
+    """mapgrid.py:275-290 on the host: (nn int32 per cell, df uint8 per cell) of points already sorted by cell."""
+    xyz = _f32(xyz).reshape(-1, 3)
+    dims = np.ascontiguousarray(dims, np.int32)
+    ncell = int(np.prod(dims.astype(np.int64)))
+    nn, tmp, df = np.empty(ncell, np.int32), np.empty(ncell, np.int32), np.empty(ncell, np.uint8)
+    st = np.ascontiguousarray(steps, np.int32)
+    lib().lwo_grid_build(_p(xyz, _f32p), len(xyz), _p(_geo(origin, cell), _f32p), _p(dims, _i32p), _p(st, _i32p),
+                         len(st), _p(nn, _i32p), _p(tmp, _i32p), _p(df, _u8p))
+    return nn, df
+
+
+def grid_distances(pts, T, origin, cell: float, dims, start, nn, xyz):
+    """mapgrid.py:170-193 (k_changes_T): each point's distance to the nearest map point, 1e3 beyond the band."""
+    pts = _f32(pts).reshape(-1, 3)
+    out = np.empty(len(pts), np.float32)
+    lib().lwo_grid_distances(_p(pts, _f32p), len(pts), _p(_f32(T).reshape(16), _f32p),
+                             _p(_geo(origin, cell), _f32p), _p(np.ascontiguousarray(dims, np.int32), _i32p),
+                             _p(np.ascontiguousarray(start, np.int32), _i32p),
+                             _p(np.ascontiguousarray(nn, np.int32), _i32p), _p(_f32(xyz).reshape(-1, 3), _f32p),
+                             _p(out, _f32p))
+    return out
+
+
+def grid_nearest(w, origin, cell: float, dims, start, nn, xyz) -> int:
+    """mapgrid.py:65-100 nearest_point for one query."""
+    return int(lib().lwo_grid_nearest(_p(_f32(w).reshape(3), _f32p), _p(_geo(origin, cell), _f32p),
+                                      _p(np.ascontiguousarray(dims, np.int32), _i32p),
+                                      _p(np.ascontiguousarray(start, np.int32), _i32p),
+                                      _p(np.ascontiguousarray(nn, np.int32), _i32p),
+                                      _p(_f32(xyz).reshape(-1, 3), _f32p)))
+
+
+def score_grid(pts, pos, n_yaw: int, yaw0: float, dyaw: float, origin, cell: float, dims, df, tau_cm: float,
+               free_cm: float):
+    """localize.py:55-92 (k_score_grid): (len(pos) * n_yaw,) soft counts, position-major."""
+    pts, pos = _f32(pts).reshape(-1, 3), _f32(pos).reshape(-1, 3)
+    out = np.empty(len(pos) * n_yaw, np.float32)
+    lib().lwo_score_grid(_p(pts, _f32p), len(pts), _p(pos, _f32p), len(pos), n_yaw, float(yaw0), float(dyaw),
+                         _p(_geo(origin, cell), _f32p), _p(np.ascontiguousarray(dims, np.int32), _i32p),
+                         _p(np.ascontiguousarray(df, np.uint8), _u8p), float(tau_cm), float(free_cm),
+                         _p(out, _f32p))
+    return out
+
+
+def score_list(pts, hyp, origin, cell: float, dims, df, tau_cm: float):
+    """localize.py:95-123 (k_score_list): the soft count of each (x, y, z, yaw) hypothesis."""
+    pts, hyp = _f32(pts).reshape(-1, 3), _f32(hyp).reshape(-1, 4)
+    out = np.empty(len(hyp), np.float32)
+    lib().lwo_score_list(_p(pts, _f32p), len(pts), _p(hyp, _f32p), len(hyp), _p(_geo(origin, cell), _f32p),
+                         _p(np.ascontiguousarray(dims, np.int32), _i32p), _p(np.ascontiguousarray(df, np.uint8), _u8p),
+                         float(tau_cm), _p(out, _f32p))
+    return out
