@@ -11,8 +11,9 @@ registration).
 Printed: one JSON line with frames/s (value), points/s, drift against the ground truth, the roofline of the
 dominant kernel (the persistent registration kernel: algorithmic bytes from its counters, duration from HIP
 events on its stream), the CPU baseline (the oracle's serial restatement on a bounded sample of the same
-sequence, one host thread) and, unless --no-loc, the batched global localisation (hypotheses/s, map build,
-top-1 recovery, with its own roofline and CPU baseline).
+sequence, one host thread) and, unless switched off, legs for the integration map (--no-imap), the prior-map
+grid and coarse search (--no-grid), the batched global localisation (--no-loc) and the learned prior
+(--no-prior), each with its own roofline and CPU baseline.
 """
 
 from __future__ import annotations
@@ -281,6 +282,93 @@ def integration_map(args, torch, lw, dev, X, A, Tt, seq, n_frames: int, per: int
     }
 
 
+GRID_SPACING = 0.04  # the prior map: the multi-room floor's surfaces every 4 cm (5.1M points)
+GRID_HEIGHTS = tuple(round(1.0 + 0.1 * k, 2) for k in range(8))  # sensor heights tried (n_heights = 8)
+GRID_QUERY = (30.2, 20.4, 2.9)  # the scan's true x, y (m) and heading (rad); sensor 1.4 m up, level
+# VALU lane-instructions per (hypothesis, point) lookup in k_score_grid's inner loop: 411 for the 16 lookups
+# of one unrolled iteration (4 positions x 4 headings) in the gfx950 code this repository builds
+GRID_VALU_PER_LOOKUP = 411 / 16
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz, one VALU lane-op a cycle each
+
+
+def prior_grid(args, torch, lw, dev):
+    """The prior map's grid (lw.PriorGrid, the reference's mapgrid.py) built from the multi-room floor, and
+    the coarse search over it (lw.CoarseSearch, localize.py's k_score_grid, local maxima, the medium grid and
+    suppression) for one level scan at 8 sensor heights; the oracle's serial k_score_grid on a sample of the
+    same hypotheses beside it (and compared with the device's scores for them)."""
+    import math
+
+    S = lw.synth.multi_room_floor(31)
+    mp = lw.synth.sample_surfaces(S, GRID_SPACING, 32, prune=False)
+    t0 = time.perf_counter()
+    prep = lw.PriorGrid.prepare({"xyz": mp})
+    prep_s = time.perf_counter() - t0
+    grid = lw.PriorGrid(prep, dev)
+    grid.build()  # first build: module load and allocator warm-up
+    grid.build()
+    x, y, yaw = GRID_QUERY
+    T = lw.synth.pose_matrix(np.array([x, y, 1.4]), yaw, 0.0, 0.0)
+    xyz, _, _ = lw.synth.scan(S, lw.synth.Rosette(41), lambda t: T, 0.0, 20000, 51)
+    pts = xyz.astype(np.float64)
+    _, first = np.unique(np.floor(pts / 0.05).astype(np.int64), axis=0, return_index=True)
+    pts = pts[np.sort(first)]
+    _, first = np.unique(np.floor(pts / 0.15).astype(np.int64), axis=0, return_index=True)
+    spread = pts[np.sort(first)]
+    rng = np.random.default_rng(0)
+    cs = lw.CoarseSearch(grid, dev)
+    sub_c = spread[rng.choice(len(spread), min(cs.n_coarse_pts, len(spread)), replace=False)]
+    sub_m = pts[rng.choice(len(pts), min(cs.n_mid_pts, len(pts)), replace=False)]
+    cs.coarse(sub_c, GRID_HEIGHTS[:1])  # warm-up
+    t0 = time.perf_counter()
+    pos, shape, dyaw, sc = cs.coarse(sub_c, GRID_HEIGHTS)
+    coarse_s = time.perf_counter() - t0
+    kernel_ms = cs.last_kernel_ms
+    hyps = int(np.prod(shape))
+    lookups = hyps * len(sub_c)
+    t0 = time.perf_counter()
+    Ts, mid, _ = cs.search(sub_c, sub_m, GRID_HEIGHTS)
+    search_s = time.perf_counter() - t0
+    err = [(float(np.linalg.norm(Tc[:3, 3] - T[:3, 3])),
+            abs(math.degrees((math.atan2(Tc[1, 0], Tc[0, 0]) - yaw + math.pi) % (2 * math.pi) - math.pi))) for Tc in Ts]
+    rank = next((i for i, (dt, da) in enumerate(err) if dt < 0.15 and da < 3.0), None)
+    # the oracle on a sample: whole positions (every heading) from the middle of the list, about 10 s
+    oracle = _oracle()
+    n_yaw = shape[3]
+    mid_pos = len(pos) // 2
+    df_host = grid.df.cpu().numpy()
+    done, exact = 0, True
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < 10.0 and mid_pos + done < len(pos):
+        a, b = mid_pos + done, min(len(pos), mid_pos + done + 64)
+        want = oracle.score_grid(sub_c, pos[a:b], n_yaw, 0.0, dyaw, prep["origin"], prep["cell"], prep["dims"],
+                                 df_host, cs.tau_coarse, cs.free_cm)
+        exact &= bool(np.array_equal(want, sc[a * n_yaw:b * n_yaw]))
+        done = b - mid_pos
+    cpu_s = time.perf_counter() - t0
+    cpu_lookups = done * n_yaw * len(sub_c)
+    rate = lookups / (kernel_ms * 1e-3)
+    valu = rate * GRID_VALU_PER_LOOKUP / 1e12
+    return {
+        "workload": f"multi-room floor 60 x 40 m at {GRID_SPACING} m ({grid.n} map points, {grid.ncell} cells of 5 cm), "
+                    f"one level scan ({len(pts)} points after 5 cm thinning, {len(sub_c)} coarse, {len(sub_m)} medium): "
+                    f"{shape[1]} x {shape[2]} positions at 0.25 m x {len(GRID_HEIGHTS)} heights x {n_yaw} headings",
+        "prepare_s": round(prep_s, 3), "build_s": round(grid.build_s, 3),
+        "build_kernels_ms": round(grid.build_kernel_ms, 3),
+        "hypotheses": hyps, "coarse_kernel_ms": round(kernel_ms, 3), "coarse_s": round(coarse_s, 3),
+        "lookups_per_s": round(rate, 1), "hypotheses_per_s": round(hyps / (kernel_ms * 1e-3), 1),
+        "search_s": round(search_s, 3), "true_pose_rank": rank, "candidates": len(Ts),
+        "best_medium_score": round(float(mid[0]), 4) if len(mid) else None,
+        "roofline": {"bound": "valu", "achieved": round(valu, 3), "peak": round(VALU_PEAK_TOPS, 2),
+                     "unit": "T lane-instr/s", "frac": round(valu / VALU_PEAK_TOPS, 4), "traffic": None,
+                     "kernel": "k_score_grid", "valu_per_lookup": round(GRID_VALU_PER_LOOKUP, 3),
+                     "df_bytes_per_lookup": 1},
+        "cpu_baseline": {"value": round(cpu_lookups / max(cpu_s, 1e-9), 1), "unit": "lookups/s", "cores": 1,
+                         "kind": "port", "sample": f"{done} positions x {n_yaw} headings x {len(sub_c)} "
+                                                   f"points through the oracle's serial k_score_grid",
+                         "seconds": round(cpu_s, 3), "scores_identical": exact},
+    }
+
+
 def learned_prior(args, torch, lw, dev):
     """The learned pose-increment prior (shipped weights, trained on 200 x 600 frames of the plain GPU
     odometry) against the constant-velocity guess on held-out office sequences, and the training step's speed
@@ -336,6 +424,7 @@ def main():
     ap.add_argument("--acc", default="fp64", choices=["fp64", "kahan32", "fp32"])
     ap.add_argument("--no-loc", action="store_true", help="skip the batched localisation leg")
     ap.add_argument("--no-imap", action="store_true", help="skip the integration-map leg")
+    ap.add_argument("--no-grid", action="store_true", help="skip the prior-grid / coarse-search leg")
     ap.add_argument("--loc-queries", type=int, default=4)
     ap.add_argument("--loc-hyps", type=int, default=4096)
     ap.add_argument("--loc-reps", type=int, default=3)
@@ -458,6 +547,8 @@ def main():
     }
     if not args.no_imap:
         out["integration_map"] = integration_map(args, torch, lw, dev, X, A, Tt, seq, n_frames, per)
+    if not args.no_grid:
+        out["prior_grid"] = prior_grid(args, torch, lw, dev)
     if not args.no_loc:
         out["localisation"] = localisation(args, torch, lw, dev)
     if not args.no_prior:

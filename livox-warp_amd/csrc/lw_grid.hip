@@ -10,8 +10,9 @@
 //    so a build is deterministic; the reference updates in place, and this order is one its launch admits;
 //  - k_score_grid gives the 64 lanes of a wave 64 consecutive positions and one heading at a time: the
 //    caller lists positions x-fastest, so for every scan point the wave's 64 distance-field reads fall in a
-//    few consecutive cache lines (0.25 m apart = 5 bytes). Each thread carries four headings, so four
-//    independent reads are in flight per point; the scan's points are staged in LDS and read by broadcast;
+//    few consecutive cache lines (0.25 m apart = 5 bytes). Each thread carries four positions and four
+//    headings, so sixteen independent reads are in flight per point and a heading's products serve all
+//    four positions; the scan's points are staged in LDS and read by broadcast; the loop has no branch;
 //  - each hypothesis's sum runs over the points in order in one thread, float32 as the reference's, so the
 //    scores are exact, not merely close.
 #include <algorithm>
@@ -29,6 +30,7 @@ constexpr int GRID_IDX_BITS = 23;            // mapgrid.py:20
 constexpr int32_t GRID_EMPTY = 0x7FFFFFFF;   // mapgrid.py:24
 constexpr int SCORE_WAVES = 4;               // waves per workgroup, one heading group each
 constexpr int SCORE_YPT = 4;                 // headings per thread
+constexpr int SCORE_PPT = 4;                 // positions per thread
 constexpr int SCORE_CHUNK = 1024;            // scan points staged in LDS per round
 
 struct GridGeo {
@@ -198,9 +200,38 @@ __device__ inline void stage_points(float *sp, const float *pts, int32_t base, i
     for (int k = threadIdx.x; k < 3 * cnt; k += blockDim.x) sp[k] = pts[3 * (int64_t)base + k];
 }
 
-// localize.py:55-92 k_score_grid. Workgroup (x, y): positions 64 x .. 64 x + 63 (one per lane), headings of
-// groups SCORE_WAVES y .. (one group of SCORE_YPT per wave). Positions outside the grid or within free_cm of
-// a mapped surface score -1.
+// The soft count's lookups in three phases, so a thread's reads are all in flight before any is used: the cell
+// of (wx, wy) at row iz (a cell outside the grid reads cell 0 and is masked off), the distance-field bytes,
+// then the term 1 - d^2 / tau^2 where d < tau, else +0 (adding +0 to a sum that starts at +0 leaves it as it
+// was, so each sum is the reference's, without a branch).
+// floor(x) as int32 in one instruction (v_cvt_flr_i32_f32: the same value as (int32_t)floorf(x) for finite x in
+// the int32 range, which every grid coordinate here is).
+__device__ inline int32_t ifloor1(float x) {
+    int32_t r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// The cell index in 24-bit multiplies: exact while nx < 2^24 and ny nz < 2^24 (checked by the launcher), and
+// ncell < 2^31; a coordinate outside the grid gives a garbage index that the mask turns into 0.
+__device__ inline uint32_t soft_cell(const GridGeo &g, float wx, float wy, int32_t iz, bool &ok) {
+    const int32_t ix = ifloor1((wx - g.ox) * g.inv), iy = ifloor1((wy - g.oy) * g.inv);
+    ok = ((uint32_t)ix < (uint32_t)g.nx) & ((uint32_t)iy < (uint32_t)g.ny) & ((uint32_t)iz < (uint32_t)g.nz);
+    const uint32_t row = __umul24((uint32_t)iz, (uint32_t)g.ny) + (uint32_t)iy;
+    const uint32_t c = __umul24(row, (uint32_t)g.nx) + (uint32_t)ix;
+    return c & (0u - (uint32_t)ok);  // a mask, not a branch
+}
+
+__device__ inline float soft_value(uint32_t byte, bool ok, float tau_cm, float inv_t2) {
+    const float d = (float)byte;
+    const float t = 1.0f - d * d * inv_t2;
+    return (ok & (d < tau_cm)) ? t : 0.0f;
+}
+
+// localize.py:55-92 k_score_grid. Workgroup (x, y): positions SCORE_PPT 64 x .. (lane l holds positions
+// base + l, base + 64 + l, ...: a wave reads 64 consecutive positions at once), headings of groups
+// SCORE_WAVES y .. (one group of SCORE_YPT per wave). The heading products of a point are shared by the
+// thread's positions. Positions outside the grid or within free_cm of a mapped surface score -1.
 __global__ __launch_bounds__(64 * SCORE_WAVES) void k_score_grid(const float *__restrict__ pts, int32_t n_pts,
                                                                  const float *__restrict__ pos, int32_t n_pos,
                                                                  int32_t n_yaw, float yaw0, float dyaw, GridGeo g,
@@ -208,27 +239,35 @@ __global__ __launch_bounds__(64 * SCORE_WAVES) void k_score_grid(const float *__
                                                                  float free_cm, float *__restrict__ out) {
     __shared__ float sp[3 * SCORE_CHUNK];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int32_t ip = blockIdx.x * 64 + lane;
     const int32_t iy0 = (blockIdx.y * SCORE_WAVES + wave) * SCORE_YPT;
-    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-    bool active = false;
-    if (ip < n_pos && iy0 < n_yaw) {
-        cx = pos[3 * (int64_t)ip];
-        cy = pos[3 * (int64_t)ip + 1];
-        cz = pos[3 * (int64_t)ip + 2];
-        const int32_t cc = grid_at(g, cx, cy, cz);
-        active = cc >= 0 && !((float)df[cc] < free_cm);
-        if (!active)
-            for (int y = 0; y < SCORE_YPT && iy0 + y < n_yaw; ++y) out[(int64_t)ip * n_yaw + iy0 + y] = -1.0f;
+    float cx[SCORE_PPT], cy[SCORE_PPT], cz[SCORE_PPT];
+    bool act[SCORE_PPT];
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < SCORE_PPT; ++q) {
+        const int32_t ip = (blockIdx.x * SCORE_PPT + q) * 64 + lane;
+        cx[q] = cy[q] = cz[q] = 0.0f;
+        act[q] = false;
+        if (ip < n_pos && iy0 < n_yaw) {
+            cx[q] = pos[3 * (int64_t)ip];
+            cy[q] = pos[3 * (int64_t)ip + 1];
+            cz[q] = pos[3 * (int64_t)ip + 2];
+            const int32_t cc = grid_at(g, cx[q], cy[q], cz[q]);
+            act[q] = cc >= 0 && !((float)df[cc] < free_cm);
+            if (!act[q])
+                for (int y = 0; y < SCORE_YPT && iy0 + y < n_yaw; ++y) out[(int64_t)ip * n_yaw + iy0 + y] = -1.0f;
+        }
+        any = any || act[q];
     }
-    float co[SCORE_YPT], si[SCORE_YPT], s[SCORE_YPT];
+    float co[SCORE_YPT], si[SCORE_YPT], s[SCORE_PPT][SCORE_YPT];
 #pragma unroll
     for (int y = 0; y < SCORE_YPT; ++y) {
         double sd, cd;
         sincos((double)(yaw0 + (float)(iy0 + y) * dyaw), &sd, &cd);
         co[y] = (float)cd;
         si[y] = (float)sd;
-        s[y] = 0.0f;
+#pragma unroll
+        for (int q = 0; q < SCORE_PPT; ++q) s[q][y] = 0.0f;
     }
     const float inv_t2 = 1.0f / (tau_cm * tau_cm);
     for (int32_t base = 0; base < n_pts; base += SCORE_CHUNK) {
@@ -236,29 +275,37 @@ __global__ __launch_bounds__(64 * SCORE_WAVES) void k_score_grid(const float *__
         __syncthreads();
         stage_points(sp, pts, base, cnt);
         __syncthreads();
-        if (!active) continue;
-#pragma unroll 2
+        if (!any) continue;
         for (int32_t k = 0; k < cnt; ++k) {
             const float px = sp[3 * k], py = sp[3 * k + 1], pz = sp[3 * k + 2];
-            const int32_t iz = ifloor((cz + pz - g.oz) * g.inv);
-            int32_t j[SCORE_YPT];
+            int32_t iz[SCORE_PPT];
+#pragma unroll
+            for (int q = 0; q < SCORE_PPT; ++q) iz[q] = ifloor1((cz[q] + pz - g.oz) * g.inv);
+            uint32_t c[SCORE_PPT][SCORE_YPT];
+            bool ok[SCORE_PPT][SCORE_YPT];
 #pragma unroll
             for (int y = 0; y < SCORE_YPT; ++y) {
-                const float wx = cx + co[y] * px - si[y] * py, wy = cy + si[y] * px + co[y] * py;
-                j[y] = grid_flat(g, ifloor((wx - g.ox) * g.inv), ifloor((wy - g.oy) * g.inv), iz);
-            }
-            uint32_t v[SCORE_YPT];
+                const float a = co[y] * px, b = si[y] * py, e = si[y] * px, f = co[y] * py;
 #pragma unroll
-            for (int y = 0; y < SCORE_YPT; ++y) v[y] = j[y] >= 0 ? (uint32_t)df[j[y]] : 0u;
-#pragma unroll
-            for (int y = 0; y < SCORE_YPT; ++y) {
-                const float d = (float)v[y];
-                if (j[y] >= 0 && d < tau_cm) s[y] += 1.0f - d * d * inv_t2;
+                for (int q = 0; q < SCORE_PPT; ++q) c[q][y] = soft_cell(g, cx[q] + a - b, cy[q] + e + f, iz[q], ok[q][y]);
             }
+            uint32_t v[SCORE_PPT][SCORE_YPT];
+#pragma unroll
+            for (int q = 0; q < SCORE_PPT; ++q)
+#pragma unroll
+                for (int y = 0; y < SCORE_YPT; ++y) v[q][y] = df[c[q][y]];
+#pragma unroll
+            for (int q = 0; q < SCORE_PPT; ++q)
+#pragma unroll
+                for (int y = 0; y < SCORE_YPT; ++y) s[q][y] += soft_value(v[q][y], ok[q][y], tau_cm, inv_t2);
         }
     }
-    if (active)
-        for (int y = 0; y < SCORE_YPT && iy0 + y < n_yaw; ++y) out[(int64_t)ip * n_yaw + iy0 + y] = s[y];
+#pragma unroll
+    for (int q = 0; q < SCORE_PPT; ++q) {
+        const int32_t ip = (blockIdx.x * SCORE_PPT + q) * 64 + lane;
+        if (act[q])
+            for (int y = 0; y < SCORE_YPT && iy0 + y < n_yaw; ++y) out[(int64_t)ip * n_yaw + iy0 + y] = s[q][y];
+    }
 }
 
 // localize.py:95-123 k_score_list: one thread per (x, y, z, yaw) hypothesis.
@@ -290,12 +337,10 @@ __global__ __launch_bounds__(256) void k_score_list(const float *__restrict__ pt
 #pragma unroll 4
         for (int32_t k = 0; k < cnt; ++k) {
             const float px = sp[3 * k], py = sp[3 * k + 1], pz = sp[3 * k + 2];
-            const float wx = cx + co * px - si * py, wy = cy + si * px + co * py, wz = cz + pz;
-            const int32_t j = grid_at(g, wx, wy, wz);
-            if (j >= 0) {
-                const float d = (float)df[j];
-                if (d < tau_cm) s += 1.0f - d * d * inv_t2;
-            }
+            bool ok;
+            const uint32_t c = soft_cell(g, cx + co * px - si * py, cy + si * px + co * py,
+                                         ifloor1((cz + pz - g.oz) * g.inv), ok);
+            s += soft_value(df[c], ok, tau_cm, inv_t2);
         }
     }
     if (active) out[h] = s;
@@ -304,6 +349,7 @@ __global__ __launch_bounds__(256) void k_score_list(const float *__restrict__ pt
 bool geo_of(const float *geo, const int32_t *dims, GridGeo &g) {
     if (!geo || !dims || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return false;
     if ((int64_t)dims[0] * dims[1] * dims[2] > (int64_t)INT32_MAX - 1) return false;
+    if (dims[0] >= (1 << 24) || (int64_t)dims[1] * dims[2] >= (1 << 24)) return false;  // the 24-bit index
     if (!(geo[3] > 0.0f) || !(geo[4] > 0.0f)) return false;
     g = GridGeo{geo[0], geo[1], geo[2], geo[3], geo[4], dims[0], dims[1], dims[2]};
     return true;
@@ -367,7 +413,8 @@ extern "C" int lw_score_grid(const float *pts, int32_t n_pts, const float *pos, 
     if (n_pos == 0) return 0;
     if ((n_pts > 0 && !pts) || !pos || !df || !out) return LW_EINVAL;
     const int32_t groups = (n_yaw + lw::SCORE_YPT - 1) / lw::SCORE_YPT;
-    const dim3 grid((unsigned)((n_pos + 63) / 64), (unsigned)((groups + lw::SCORE_WAVES - 1) / lw::SCORE_WAVES));
+    const dim3 grid((unsigned)((n_pos + 64 * lw::SCORE_PPT - 1) / (64 * lw::SCORE_PPT)),
+                    (unsigned)((groups + lw::SCORE_WAVES - 1) / lw::SCORE_WAVES));
     hipLaunchKernelGGL(lw::k_score_grid, grid, dim3(64 * lw::SCORE_WAVES), 0, (hipStream_t)stream, pts, n_pts, pos,
                        n_pos, n_yaw, yaw0, dyaw, g, df, tau_cm, free_cm, out);
     return lw::launched();

@@ -102,6 +102,7 @@ class PriorGrid:
         self.df = torch.empty(self.ncell, dtype=torch.uint8, device=d)
         self.ready = False
         self.build_s = 0.0
+        self.build_kernel_ms = 0.0
 
     def gpu_bytes(self) -> int:
         return self.n * (12 + 12 + 4) + self.ncell * (4 + 4 + 1)
@@ -122,12 +123,16 @@ class PriorGrid:
                 dst.copy_(torch.from_numpy(np.ascontiguousarray(src)).reshape(dst.shape))
             tmp = torch.empty(self.ncell, dtype=torch.int32, device=self.device)
             st = np.ascontiguousarray(steps, np.int32)
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ev[0].record(s)
             _abi.check(_abi.lib().lw_grid_build(self.xyz.data_ptr(), self.n, *self._geo_args(),
                                                 _abi.ptr(st, _abi._i32p), len(st), self.nn.data_ptr(), tmp.data_ptr(),
                                                 self.df.data_ptr(), ctypes.c_void_p(s.cuda_stream)), "lw_grid_build")
+            ev[1].record(s)
         s.synchronize()
         del tmp
         self.build_s = time.perf_counter() - t0
+        self.build_kernel_ms = ev[0].elapsed_time(ev[1])  # the build's kernels alone (no upload)
         self.ready = True
 
     def distances(self, pts, n: int, T_map_from_pts: np.ndarray, out, stream=None):
