@@ -12,7 +12,7 @@ Printed: one JSON line with frames/s (value), points/s, drift against the ground
 dominant kernel (the persistent registration kernel: algorithmic bytes from its counters, duration from HIP
 events on its stream), the CPU baseline (the oracle's serial restatement on a bounded sample of the same
 sequence, one host thread) and, unless switched off, legs for the integration map (--no-imap), the prior-map
-grid and coarse search (--no-grid), the batched global localisation (--no-loc) and the learned prior
+grid and coarse search (--no-grid), LVXR replay (--no-lvxr), the odometry worker (--no-worker), the batched global localisation (--no-loc) and the learned prior
 (--no-prior), each with its own roofline and CPU baseline.
 """
 
@@ -369,6 +369,122 @@ def prior_grid(args, torch, lw, dev):
     }
 
 
+LVXR_PER_PACKET = 96  # extended-cartesian records a packet (data type 2, 14 bytes each)
+LVXR_OUT_BYTES = 8 + 12 + 1 + 1 + 1 + 12  # per firing written: time, xyz, reflectivity, tag, valid, direction
+
+
+def _write_lvxr(path, xyz, attr, t):
+    """The points as an LVXR2 recording (dev type 1, standard profile) of extended-cartesian packets: millimetre
+    coordinates, reflectivity and tag, 96 records a packet stamped (ns) with its first record's time."""
+    n = len(t) // LVXR_PER_PACKET * LVXR_PER_PACKET
+    rec = np.dtype([("x", "<i4"), ("y", "<i4"), ("z", "<i4"), ("refl", "u1"), ("tag", "u1")])
+    pk = np.dtype([("host", "<u8"), ("len", "<u2"), ("head", "u1", (10,)), ("ts", "<u8"),
+                   ("rec", rec, (LVXR_PER_PACKET,))])
+    a = np.zeros(n // LVXR_PER_PACKET, pk)
+    ts = np.round(t[:n:LVXR_PER_PACKET].astype(np.float64) * 1e9).astype(np.uint64)
+    a["host"], a["ts"], a["len"] = ts, ts, 18 + 14 * LVXR_PER_PACKET
+    a["head"][:, 0], a["head"][:, 2], a["head"][:, 9] = 5, 1, 2  # version, device id, data type
+    mm = np.round(xyz[:n].astype(np.float64) * 1000).astype("<i4").reshape(-1, LVXR_PER_PACKET, 3)
+    a["rec"]["x"], a["rec"]["y"], a["rec"]["z"] = mm[..., 0], mm[..., 1], mm[..., 2]
+    a["rec"]["refl"] = (attr[:n] & 255).reshape(-1, LVXR_PER_PACKET)
+    a["rec"]["tag"] = ((attr[:n] >> 8) & 255).reshape(-1, LVXR_PER_PACKET)
+    with open(path, "wb") as f:
+        f.write(b"LVXR2\n" + bytes([1, 0]) + a.tobytes())
+    return n
+
+
+def lvxr_replay(args, torch, lw, dev, frames):
+    """LVXR replay (lw.lvxr.read_firings, the reference's lvxr.read_firings): the odometry sequence's points
+    written as a recording, read back with the decode on the device (kernel timed with HIP events), and the
+    oracle's serial reader on the first packets beside it, compared firing for firing."""
+    import shutil
+    import tempfile
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import lvxr_oracle
+
+    xyz = np.concatenate([f[0] for f in frames])
+    attr = np.concatenate([f[1] for f in frames]).astype(np.uint32)
+    t = np.concatenate([f[2] for f in frames])
+    tmp = tempfile.mkdtemp()
+    try:
+        path = os.path.join(tmp, "walk.lvxr")
+        n = _write_lvxr(path, xyz, attr, t)
+        lw.lvxr.read_firings(path, dev)  # warm-up
+        tm = {}
+        t0 = time.perf_counter()
+        f = lw.lvxr.read_firings(path, dev, timing=tm)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        # the oracle on a sample: the first packets, about 10 s of serial Python
+        sample = os.path.join(tmp, "sample.lvxr")
+        k = 64
+        while True:
+            m = _write_lvxr(sample, xyz[:k * LVXR_PER_PACKET], attr[:k * LVXR_PER_PACKET], t[:k * LVXR_PER_PACKET])
+            data = open(sample, "rb").read()
+            t1 = time.perf_counter()
+            ref = lvxr_oracle.read(data)
+            spent = time.perf_counter() - t1
+            if spent > 3.0 or k * 2 * LVXR_PER_PACKET > n:
+                break
+            k *= 2
+        fs = lw.lvxr.read_firings(sample, dev)
+        same = bool(np.array_equal(fs.t.cpu().numpy(), ref[2]) and np.array_equal(fs.xyz.cpu().numpy(), ref[3])
+                    and np.array_equal(fs.valid.cpu().numpy(), ref[6]))
+    finally:
+        shutil.rmtree(tmp)
+    dec_s = tm["decode_ms"] * 1e-3
+    gbs = tm["firings"] * (14 + LVXR_OUT_BYTES) / dec_s / 1e9
+    return {
+        "workload": f"{len(frames)} office frames ({n} points) as an LVXR2 recording of {tm['packets']} "
+                    f"extended-cartesian packets ({tm['file_bytes'] / 1e6:.1f} MB)",
+        "firings_per_s": round(tm["firings"] / dec_s, 1), "decode_ms": round(tm["decode_ms"], 4),
+        "host_framing_s": round(tm["host_s"], 4), "read_firings_s": round(wall, 4),
+        "firings_per_s_end_to_end": round(tm["firings"] / wall, 1),
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "k_lvxr_decode",
+                     "bytes_per_firing": 14 + LVXR_OUT_BYTES},
+        "cpu_baseline": {"value": round(m / spent, 1), "unit": "firings/s", "cores": 1, "kind": "port",
+                         "sample": f"the first {k} packets ({m} firings) through the oracle's serial reader",
+                         "seconds": round(spent, 3), "identical": same},
+    }
+
+
+def odom_worker(args, torch, lw, dev, X, A, Tt, spans, seq, n_frames: int, per: int):
+    """The threaded path (lw.OdomWorker, the reference's slam_worker.OdomWorker): the same frames submitted
+    one 10k-point batch at a time from the caller's stream (owned copies in, owned copies of each deskewed
+    frame out), polled and released as they come; frames/s from the first submit until the worker is idle and
+    the last frame is polled, against the direct Odometry's rate on the same frames."""
+    w = lw.worker.OdomWorker(lw.PoseTable(), dev, max_batches=n_frames + 1, out_slots=64, frame_cap=1 << 15,
+                             reg_slots=1 << 22)
+    try:
+        w.configure(reg_voxel=0.2, max_iter=20)
+        w.reset(seq.pose(0.05))
+        torch.cuda.synchronize()
+        got = 0
+        t0 = time.perf_counter()
+        for k in range(n_frames):
+            w.submit(X[k], A[k], Tt[k], per, spans[k][0], spans[k][1])
+            got += len(w.poll())
+            w.release_consumed()
+        deadline = time.perf_counter() + 120.0
+        while (not w.idle() or w.status()["processed"] < n_frames) and time.perf_counter() < deadline:
+            got += len(w.poll())
+            w.release_consumed()
+            time.sleep(0.001)  # a caller polling at 1 kHz
+        time.sleep(0.01)
+        got += len(w.poll())
+        torch.cuda.current_stream().synchronize()
+        wall = time.perf_counter() - t0
+        w.release_consumed()
+        st = w.status()
+    finally:
+        w.close()
+    return {"workload": f"{n_frames} office frames x {per} points, one batch a frame, through the worker's three "
+                        f"streams", "frames": got, "frames_per_s": round(got / wall, 1), "drops": st["drops"],
+            "busy_ms": st["busy_ms"]}
+
+
 def learned_prior(args, torch, lw, dev):
     """The learned pose-increment prior (shipped weights, trained on 200 x 600 frames of the plain GPU
     odometry) against the constant-velocity guess on held-out office sequences, and the training step's speed
@@ -425,6 +541,8 @@ def main():
     ap.add_argument("--no-loc", action="store_true", help="skip the batched localisation leg")
     ap.add_argument("--no-imap", action="store_true", help="skip the integration-map leg")
     ap.add_argument("--no-grid", action="store_true", help="skip the prior-grid / coarse-search leg")
+    ap.add_argument("--no-lvxr", action="store_true", help="skip the LVXR replay leg")
+    ap.add_argument("--no-worker", action="store_true", help="skip the odometry-worker leg")
     ap.add_argument("--loc-queries", type=int, default=4)
     ap.add_argument("--loc-hyps", type=int, default=4096)
     ap.add_argument("--loc-reps", type=int, default=3)
@@ -549,6 +667,10 @@ def main():
         out["integration_map"] = integration_map(args, torch, lw, dev, X, A, Tt, seq, n_frames, per)
     if not args.no_grid:
         out["prior_grid"] = prior_grid(args, torch, lw, dev)
+    if not args.no_lvxr:
+        out["lvxr_replay"] = lvxr_replay(args, torch, lw, dev, frames)
+    if not args.no_worker:
+        out["odom_worker"] = odom_worker(args, torch, lw, dev, X, A, Tt, spans, seq, n_frames, per)
     if not args.no_loc:
         out["localisation"] = localisation(args, torch, lw, dev)
     if not args.no_prior:

@@ -323,10 +323,12 @@ int lw_lvxr_decode(const uint8_t *bytes, const int64_t *pk_off, const int32_t *p
  *
  * lw_grid_build: the seeds nn (int32 per cell: the map point nearest the cell centre, -1 beyond the band) and
  * the distance field df (uint8 cm per cell, 255 beyond the band): seed, unpack, the n_steps jump-flood steps
- * (host array, 8 4 2 1 1 in the reference), each reading every seed before writing any (nn_tmp: a second
- * ncell int32 array, scratch), then df. Replaces PriorGrid.build's launches (mapgrid.py:275-290). */
+ * (host array, 8 4 2 1 1 in the reference), each reading every seed before writing any, then df. work:
+ * device scratch of at least lw_grid_work_bytes(dims) bytes (two 16-byte seed records per cell), 16-byte
+ * aligned. Replaces PriorGrid.build's launches (mapgrid.py:275-290). */
+int64_t lw_grid_work_bytes(const int32_t *dims);
 int lw_grid_build(const float *xyz, int32_t n, const float *geo, const int32_t *dims, const int32_t *steps,
-                  int32_t n_steps, int32_t *nn, int32_t *nn_tmp, uint8_t *df, void *stream);
+                  int32_t n_steps, int32_t *nn, void *work, int64_t work_bytes, uint8_t *df, void *stream);
 /* Distance of each of n points, placed in the map by T (host, row-major 4x4 float32), to the nearest map point
  * (exact within one cell, else through the seeds), 1e3 beyond the band. Replaces PriorGrid.distances /
  * k_changes_T (mapgrid.py:170-193, 292-300). */

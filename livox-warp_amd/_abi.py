@@ -120,7 +120,8 @@ SIGNATURES = {
     "lw_imap_read": (_i32, [_vp, _vp, _i64]),
     "lw_imap_load": (_i32, [_vp, _vp, _i64]),
     "lw_lvxr_decode": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "lw_grid_build": (_i32, [_vp, _i32, _f32p, _i32p, _i32p, _i32, _vp, _vp, _vp, _vp]),
+    "lw_grid_work_bytes": (_i64, [_i32p]),
+    "lw_grid_build": (_i32, [_vp, _i32, _f32p, _i32p, _i32p, _i32, _vp, _vp, _i64, _vp, _vp]),
     "lw_grid_distances": (_i32, [_vp, _i32, _f32p, _f32p, _i32p, _vp, _vp, _vp, _vp, _vp]),
     "lw_score_grid": (_i32, [_vp, _i32, _vp, _i32, _i32, _f32, _f32, _f32p, _i32p, _vp, _f32, _f32, _vp, _vp]),
     "lw_score_list": (_i32, [_vp, _i32, _vp, _i32, _f32p, _i32p, _vp, _f32, _vp, _vp]),

@@ -7,7 +7,9 @@
 //
 // MI355X layout choices:
 //  - the jump flood reads every seed of a step before it writes any (two seed arrays, swapped each step),
-//    so a build is deterministic; the reference updates in place, and this order is one its launch admits;
+//    so a build is deterministic; the reference updates in place, and this order is one its launch admits.
+//    A seed travels as a 16-byte record (index and coordinates), so a step's 27 neighbour reads are
+//    coherent loads of neighbouring records, not gathers of random map points;
 //  - k_score_grid gives the 64 lanes of a wave 64 consecutive positions and one heading at a time: the
 //    caller lists positions x-fastest, so for every scan point the wave's 64 distance-field reads fall in a
 //    few consecutive cache lines (0.25 m apart = 5 bytes). Each thread carries four positions and four
@@ -40,6 +42,13 @@ struct GridGeo {
 
 struct Mat34 {
     float m[12];
+};
+
+// One cell's seed during the flood: the map point's index (-1: none) with its coordinates beside it, so a
+// step reads its 27 neighbours' seeds as coherent 16-byte records instead of gathering their points.
+struct __align__(16) Seed {
+    int32_t idx;
+    float x, y, z;
 };
 
 __device__ inline int32_t grid_flat(const GridGeo &g, int32_t ix, int32_t iy, int32_t iz) {
@@ -91,17 +100,31 @@ __global__ __launch_bounds__(256) void k_grid_seed(const float *xyz, int32_t n, 
     atomicMin(nn + c, (dmm << GRID_IDX_BITS) | i);
 }
 
-// mapgrid.py:115-123: the winning key's point index, -1 for an unclaimed cell.
-__global__ __launch_bounds__(256) void k_grid_unpack(int32_t *nn, int64_t n) {
+// mapgrid.py:115-123: the winning key's point index (-1 for an unclaimed cell), as a seed record.
+__global__ __launch_bounds__(256) void k_grid_unpack(const int32_t *__restrict__ keys, const float *__restrict__ xyz,
+                                                     int64_t n, Seed *__restrict__ rec) {
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t v = nn[c];
-        nn[c] = v == GRID_EMPTY ? -1 : (v & ((1 << GRID_IDX_BITS) - 1));
+        const int32_t v = keys[c];
+        Seed r{-1, 0.0f, 0.0f, 0.0f};
+        if (v != GRID_EMPTY) {
+            r.idx = v & ((1 << GRID_IDX_BITS) - 1);
+            const float *q = xyz + 3 * (int64_t)r.idx;
+            r.x = q[0];
+            r.y = q[1];
+            r.z = q[2];
+        }
+        rec[c] = r;
     }
 }
 
+__device__ inline float dist2_xyz(float x, float y, float z, const Seed &q) {
+    const float d0 = x - q.x, d1 = y - q.y, d2 = z - q.z;
+    return d0 * d0 + d1 * d1 + d2 * d2;
+}
+
 // mapgrid.py:126-155: the nearest of the cell's own seed and the 26 seeds `step` cells away (src -> dst).
-__global__ __launch_bounds__(256) void k_grid_jfa(const int32_t *__restrict__ src, int32_t *__restrict__ dst,
-                                                  const float *__restrict__ xyz, GridGeo g, int32_t step) {
+__global__ __launch_bounds__(256) void k_grid_jfa(const Seed *__restrict__ src, Seed *__restrict__ dst, GridGeo g,
+                                                  int32_t step) {
     const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
     const int64_t cl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (cl >= ncell) return;
@@ -110,17 +133,17 @@ __global__ __launch_bounds__(256) void k_grid_jfa(const int32_t *__restrict__ sr
     float px, py, pz;
     cell_ijk(g, c, ix, iy, iz);
     cell_centre(g, ix, iy, iz, px, py, pz);
-    int32_t best = src[c];
+    Seed best = src[c];
     float bd = 1.0e30f;
-    if (best >= 0) bd = dist2_to(px, py, pz, xyz + 3 * (int64_t)best);
+    if (best.idx >= 0) bd = dist2_xyz(px, py, pz, best);
     for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dx = -1; dx <= 1; ++dx) {
                 const int32_t j = grid_flat(g, ix + dx * step, iy + dy * step, iz + dz * step);
                 if (j < 0 || j == c) continue;
-                const int32_t s = src[j];
-                if (s >= 0 && s != best) {
-                    const float d2 = dist2_to(px, py, pz, xyz + 3 * (int64_t)s);
+                const Seed s = src[j];
+                if (s.idx >= 0 && s.idx != best.idx) {
+                    const float d2 = dist2_xyz(px, py, pz, s);
                     if (d2 < bd) {
                         bd = d2;
                         best = s;
@@ -130,14 +153,17 @@ __global__ __launch_bounds__(256) void k_grid_jfa(const int32_t *__restrict__ sr
     dst[c] = best;
 }
 
-// mapgrid.py:158-167: the cell centre's distance to its seed in cm (rounded), 255 beyond the band.
-__global__ __launch_bounds__(256) void k_grid_df(const int32_t *nn, const float *xyz, GridGeo g, uint8_t *df) {
+// mapgrid.py:158-167 with the seed indices written out: nn per cell, and the cell centre's distance to its
+// seed in cm (rounded), 255 beyond the band.
+__global__ __launch_bounds__(256) void k_grid_df(const Seed *__restrict__ rec, GridGeo g, int32_t *__restrict__ nn,
+                                                 uint8_t *__restrict__ df) {
     const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
     const int64_t cl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (cl >= ncell) return;
     const int32_t c = (int32_t)cl;
-    const int32_t s = nn[c];
-    if (s < 0) {
+    const Seed s = rec[c];
+    nn[c] = s.idx;
+    if (s.idx < 0) {
         df[c] = 255;
         return;
     }
@@ -145,7 +171,7 @@ __global__ __launch_bounds__(256) void k_grid_df(const int32_t *nn, const float 
     float cx, cy, cz;
     cell_ijk(g, c, ix, iy, iz);
     cell_centre(g, ix, iy, iz, cx, cy, cz);
-    const float d = sqrtf(dist2_to(cx, cy, cz, xyz + 3 * (int64_t)s)) * 100.0f;
+    const float d = sqrtf(dist2_xyz(cx, cy, cz, s)) * 100.0f;
     df[c] = (uint8_t)min((int32_t)(d + 0.5f), 255);
 }
 
@@ -368,27 +394,33 @@ unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) 
 
 using lw::GridGeo;
 
+extern "C" int64_t lw_grid_work_bytes(const int32_t *dims) {
+    if (!dims || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return -1;
+    return 2 * (int64_t)sizeof(lw::Seed) * dims[0] * dims[1] * dims[2];
+}
+
 extern "C" int lw_grid_build(const float *xyz, int32_t n, const float *geo, const int32_t *dims, const int32_t *steps,
-                             int32_t n_steps, int32_t *nn, int32_t *nn_tmp, uint8_t *df, void *stream) {
+                             int32_t n_steps, int32_t *nn, void *work, int64_t work_bytes, uint8_t *df, void *stream) {
     GridGeo g;
     if (!lw::geo_of(geo, dims, g) || n < 0 || n >= (1 << lw::GRID_IDX_BITS) || n_steps < 0 || n_steps > 32 || !nn ||
-        !nn_tmp || !df || nn == nn_tmp || (n > 0 && !xyz) || (n_steps > 0 && !steps))
+        !work || !df || (n > 0 && !xyz) || (n_steps > 0 && !steps) || ((uintptr_t)work & 15))
         return LW_EINVAL;
+    const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
+    if (work_bytes < 2 * (int64_t)sizeof(lw::Seed) * ncell) return LW_EINVAL;
     for (int k = 0; k < n_steps; ++k)
         if (steps[k] <= 0) return LW_EINVAL;
     hipStream_t s = (hipStream_t)stream;
-    const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
-    // the steps alternate the two arrays and end in nn
-    int32_t *a = (n_steps % 2) ? nn_tmp : nn, *b = (n_steps % 2) ? nn : nn_tmp;
+    lw::Seed *a = static_cast<lw::Seed *>(work), *b = a + ncell;
     const unsigned fill_blocks = (unsigned)std::min<int64_t>(lw::blocks_for(ncell), 65536);
-    hipLaunchKernelGGL(lw::k_grid_fill, dim3(fill_blocks), dim3(256), 0, s, a, ncell, lw::GRID_EMPTY);
-    if (n > 0) hipLaunchKernelGGL(lw::k_grid_seed, dim3(lw::blocks_for(n)), dim3(256), 0, s, xyz, n, g, a);
-    hipLaunchKernelGGL(lw::k_grid_unpack, dim3(fill_blocks), dim3(256), 0, s, a, ncell);
+    // the seeds' keys go through nn, which the last kernel overwrites with the indices
+    hipLaunchKernelGGL(lw::k_grid_fill, dim3(fill_blocks), dim3(256), 0, s, nn, ncell, lw::GRID_EMPTY);
+    if (n > 0) hipLaunchKernelGGL(lw::k_grid_seed, dim3(lw::blocks_for(n)), dim3(256), 0, s, xyz, n, g, nn);
+    hipLaunchKernelGGL(lw::k_grid_unpack, dim3(fill_blocks), dim3(256), 0, s, nn, xyz, ncell, a);
     for (int k = 0; k < n_steps; ++k) {
-        hipLaunchKernelGGL(lw::k_grid_jfa, dim3(lw::blocks_for(ncell)), dim3(256), 0, s, a, b, xyz, g, steps[k]);
+        hipLaunchKernelGGL(lw::k_grid_jfa, dim3(lw::blocks_for(ncell)), dim3(256), 0, s, a, b, g, steps[k]);
         std::swap(a, b);
     }
-    hipLaunchKernelGGL(lw::k_grid_df, dim3(lw::blocks_for(ncell)), dim3(256), 0, s, nn, xyz, g, df);
+    hipLaunchKernelGGL(lw::k_grid_df, dim3(lw::blocks_for(ncell)), dim3(256), 0, s, a, g, nn, df);
     return lw::launched();
 }
 

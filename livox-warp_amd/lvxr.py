@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import time
 from dataclasses import dataclass, replace
 
 import numpy as np
@@ -127,10 +128,12 @@ def _clock(ts: np.ndarray, spans: np.ndarray) -> np.ndarray:
     return out
 
 
-def read_firings(path: str | os.PathLike, device=None, stream=None) -> Firings:
-    """Every firing of an LVXR recording, decoded on `device` (the reference's read_firings)."""
+def read_firings(path: str | os.PathLike, device=None, stream=None, timing: dict | None = None) -> Firings:
+    """Every firing of an LVXR recording, decoded on `device` (the reference's read_firings). timing: a dict
+    that receives the host framing time (s), the decode kernel's time (ms, HIP events) and the byte counts."""
     torch = _torch()
     dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    t0 = time.perf_counter()
     with open(path, "rb") as f:
         data = f.read()
     dev_type, profile, offs, lens = _framing(data)
@@ -170,10 +173,19 @@ def read_firings(path: str | os.PathLike, device=None, stream=None) -> Firings:
     valid = torch.empty((n, R), dtype=torch.uint8, device=dev)
     direction = torch.empty((n, 3), dtype=torch.float32, device=dev)
     s = stream if stream is not None else torch.cuda.current_stream(dev)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)] if timing is not None else None
+    if ev:
+        timing["host_s"] = time.perf_counter() - t0
+        ev[0].record(s)
     _abi.check(_abi.lib().lw_lvxr_decode(bytes_dev.data_ptr(), *[a.data_ptr() for a in tab], len(offs),
                                          1 if dual else 0, interval, R, t.data_ptr(), xyz.data_ptr(), refl.data_ptr(),
                                          tag.data_ptr(), valid.data_ptr(), direction.data_ptr(),
                                          ctypes.c_void_p(s.cuda_stream)), "lw_lvxr_decode")
+    if ev:
+        ev[1].record(s)
+        ev[1].synchronize()
+        timing.update(decode_ms=ev[0].elapsed_time(ev[1]), firings=n, returns=R, packets=len(offs),
+                      record_bytes=int((lens - HEADER_LEN).sum()), file_bytes=len(data))
     with torch.cuda.stream(s):
         idx = torch.sort(t, stable=True).indices
         out = Firings(t=t[idx], xyz=xyz[idx], refl=refl[idx], tag=tag[idx], valid=valid[idx].bool(),

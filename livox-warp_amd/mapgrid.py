@@ -112,7 +112,7 @@ class PriorGrid:
 
     def build(self, stream=None, steps=JFA_STEPS):
         """Upload the points and build the grid on `stream` (the current stream if None); returns when done. The
-        jump flood's second seed array lives only for the build."""
+        jump flood's two arrays of seed records (32 bytes a cell) live only for the build."""
         torch = _torch()
         t0 = time.perf_counter()
         h = self._host
@@ -121,13 +121,15 @@ class PriorGrid:
             for dst, src in ((self.xyz, h["xyz"]), (self.nrm, h["normal"]), (self.planar, h["planarity"]),
                              (self.start, h["start"])):
                 dst.copy_(torch.from_numpy(np.ascontiguousarray(src)).reshape(dst.shape))
-            tmp = torch.empty(self.ncell, dtype=torch.int32, device=self.device)
+            L = _abi.lib()
+            wb = int(L.lw_grid_work_bytes(_abi.ptr(self.dims32, _abi._i32p)))
+            tmp = torch.empty(wb, dtype=torch.uint8, device=self.device)
             st = np.ascontiguousarray(steps, np.int32)
             ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
             ev[0].record(s)
-            _abi.check(_abi.lib().lw_grid_build(self.xyz.data_ptr(), self.n, *self._geo_args(),
-                                                _abi.ptr(st, _abi._i32p), len(st), self.nn.data_ptr(), tmp.data_ptr(),
-                                                self.df.data_ptr(), ctypes.c_void_p(s.cuda_stream)), "lw_grid_build")
+            _abi.check(L.lw_grid_build(self.xyz.data_ptr(), self.n, *self._geo_args(), _abi.ptr(st, _abi._i32p),
+                                       len(st), self.nn.data_ptr(), tmp.data_ptr(), wb, self.df.data_ptr(),
+                                       ctypes.c_void_p(s.cuda_stream)), "lw_grid_build")
             ev[1].record(s)
         s.synchronize()
         del tmp

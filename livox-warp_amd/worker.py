@@ -376,8 +376,10 @@ class OdomWorker:
             sx, sa, st = (sn.array for sn in item.snaps)
             self.odom.epoch = item.epoch
             self.odom.push(sx, sa, st, item.n, item.t_min, item.t_max, self._on_frame)
-            s.synchronize()  # the odometry has read the copies before they are freed
             for sn in item.snaps:
+                # the allocator keeps each copy until the odometry's work queued so far has read it, so the
+                # thread moves on without waiting for the stream
+                sn.array.record_stream(s)
                 sn.release()
             dt = (time.perf_counter() - t0) * 1e3
             with self._lock:
