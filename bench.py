@@ -185,9 +185,53 @@ def localisation(args, torch, lw, dev):
                       "points_per_s": round(n_map / t_map, 1), "algorithmic_GBps": round(map_bytes / t_map / 1e9, 2)},
         "gen_seconds": round(t_gen, 2),
     }
+    res["global_search"] = global_search(torch, lw, dev, W, loc)
     if args.cpu_frames > 0:
         res["cpu_baseline"] = loc_cpu_baseline(W, hyps, args.loc_cpu_hyps)
     return res
+
+
+def _thin(pts: np.ndarray, voxel: float) -> np.ndarray:
+    """The first point of every voxel, in input order."""
+    _, first = np.unique(np.floor(pts / voxel).astype(np.int64), axis=0, return_index=True)
+    return pts[np.sort(first)]
+
+
+def global_search(torch, lw, dev, W, loc):
+    """The reference's global search order (localize.py:603-700) on the same queries: the distance-field
+    scoring first (a PriorGrid of the same floor at 4 cm, the coarse and medium stages at the known sensor
+    height of a level scan), then the batched ICP on each query's 16 candidates; top-1 = the ICP's best score."""
+    t0 = time.perf_counter()
+    grid = lw.PriorGrid({"xyz": lw.synth.sample_surfaces(W.scene, GRID_SPACING, LOC_SEED + 1, prune=False)}, dev)
+    grid.build()
+    cs = lw.CoarseSearch(grid, dev)
+    t_grid = time.perf_counter() - t0
+    rng = np.random.default_rng(LOC_SEED + 11)
+    cands, t_search, first = [], 0.0, []
+    for q, scan in enumerate(W.scans):
+        pts = _thin(np.asarray(scan, np.float64), 0.05)
+        spread = _thin(pts, 0.15)
+        sub_c = spread[rng.choice(len(spread), min(cs.n_coarse_pts, len(spread)), replace=False)]
+        sub_m = pts[rng.choice(len(pts), min(cs.n_mid_pts, len(pts)), replace=False)]
+        t1 = time.perf_counter()
+        Ts, _, _ = cs.search(sub_c, sub_m, [1.4])
+        t_search += time.perf_counter() - t1
+        Ts = np.concatenate([Ts, np.repeat(Ts[-1:], max(0, cs.n_icp - len(Ts)), 0)])[:cs.n_icp]
+        first.append(_pose_err(W.T_true[q], Ts[0]))
+        cands.append(Ts)
+    hyps = np.stack(cands)
+    loc.stream.synchronize()
+    t1 = time.perf_counter()
+    out = loc.run(W.scans, hyps)
+    loc.stream.synchronize()
+    t_icp = time.perf_counter() - t1
+    T, S = out[0].cpu().numpy(), out[1].cpu().numpy()
+    best = [_pose_err(W.T_true[q], T[q, int(np.argmax(S[q]))]) for q in range(len(W.scans))]
+    return {"grid_build_s": round(t_grid, 3), "search_s_per_query": round(t_search / len(W.scans), 4),
+            "icp_s": round(t_icp, 4), "candidates": int(hyps.shape[1]),
+            "coarse_first_error": [[round(e[0], 3), round(e[1], 2)] for e in first],
+            "top1_recovered": sum(1 for e in best if e[0] < 0.3 and e[1] < 3.0), "queries": len(W.scans),
+            "top1_error": [[round(e[0], 3), round(e[1], 2)] for e in best]}
 
 
 def loc_cpu_baseline(W, hyps, n: int):

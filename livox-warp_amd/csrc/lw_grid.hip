@@ -123,15 +123,13 @@ __device__ inline float dist2_xyz(float x, float y, float z, const Seed &q) {
 }
 
 // mapgrid.py:126-155: the nearest of the cell's own seed and the 26 seeds `step` cells away (src -> dst).
+// Launched over (x tiles of 256, rows, slabs): the cell's coordinates come from the launch, not divisions.
 __global__ __launch_bounds__(256) void k_grid_jfa(const Seed *__restrict__ src, Seed *__restrict__ dst, GridGeo g,
                                                   int32_t step) {
-    const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
-    const int64_t cl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (cl >= ncell) return;
-    const int32_t c = (int32_t)cl;
-    int32_t ix, iy, iz;
+    const int32_t ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y, iz = blockIdx.z;
+    if (ix >= g.nx) return;
+    const int32_t c = (iz * g.ny + iy) * g.nx + ix;
     float px, py, pz;
-    cell_ijk(g, c, ix, iy, iz);
     cell_centre(g, ix, iy, iz, px, py, pz);
     Seed best = src[c];
     float bd = 1.0e30f;
@@ -157,19 +155,16 @@ __global__ __launch_bounds__(256) void k_grid_jfa(const Seed *__restrict__ src, 
 // seed in cm (rounded), 255 beyond the band.
 __global__ __launch_bounds__(256) void k_grid_df(const Seed *__restrict__ rec, GridGeo g, int32_t *__restrict__ nn,
                                                  uint8_t *__restrict__ df) {
-    const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
-    const int64_t cl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (cl >= ncell) return;
-    const int32_t c = (int32_t)cl;
+    const int32_t ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y, iz = blockIdx.z;
+    if (ix >= g.nx) return;
+    const int32_t c = (iz * g.ny + iy) * g.nx + ix;
     const Seed s = rec[c];
     nn[c] = s.idx;
     if (s.idx < 0) {
         df[c] = 255;
         return;
     }
-    int32_t ix, iy, iz;
     float cx, cy, cz;
-    cell_ijk(g, c, ix, iy, iz);
     cell_centre(g, ix, iy, iz, cx, cy, cz);
     const float d = sqrtf(dist2_xyz(cx, cy, cz, s)) * 100.0f;
     df[c] = (uint8_t)min((int32_t)(d + 0.5f), 255);
@@ -376,6 +371,7 @@ bool geo_of(const float *geo, const int32_t *dims, GridGeo &g) {
     if (!geo || !dims || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return false;
     if ((int64_t)dims[0] * dims[1] * dims[2] > (int64_t)INT32_MAX - 1) return false;
     if (dims[0] >= (1 << 24) || (int64_t)dims[1] * dims[2] >= (1 << 24)) return false;  // the 24-bit index
+    if (dims[1] > 65535 || dims[2] > 65535) return false;  // rows and slabs index the launch grid
     if (!(geo[3] > 0.0f) || !(geo[4] > 0.0f)) return false;
     g = GridGeo{geo[0], geo[1], geo[2], geo[3], geo[4], dims[0], dims[1], dims[2]};
     return true;
@@ -416,11 +412,12 @@ extern "C" int lw_grid_build(const float *xyz, int32_t n, const float *geo, cons
     hipLaunchKernelGGL(lw::k_grid_fill, dim3(fill_blocks), dim3(256), 0, s, nn, ncell, lw::GRID_EMPTY);
     if (n > 0) hipLaunchKernelGGL(lw::k_grid_seed, dim3(lw::blocks_for(n)), dim3(256), 0, s, xyz, n, g, nn);
     hipLaunchKernelGGL(lw::k_grid_unpack, dim3(fill_blocks), dim3(256), 0, s, nn, xyz, ncell, a);
+    const dim3 cells(lw::blocks_for(g.nx), (unsigned)g.ny, (unsigned)g.nz);
     for (int k = 0; k < n_steps; ++k) {
-        hipLaunchKernelGGL(lw::k_grid_jfa, dim3(lw::blocks_for(ncell)), dim3(256), 0, s, a, b, g, steps[k]);
+        hipLaunchKernelGGL(lw::k_grid_jfa, cells, dim3(256), 0, s, a, b, g, steps[k]);
         std::swap(a, b);
     }
-    hipLaunchKernelGGL(lw::k_grid_df, dim3(lw::blocks_for(ncell)), dim3(256), 0, s, a, g, nn, df);
+    hipLaunchKernelGGL(lw::k_grid_df, cells, dim3(256), 0, s, a, g, nn, df);
     return lw::launched();
 }
 

@@ -148,3 +148,39 @@ def test_cond_matches_numpy(lw):
         ev = np.linalg.eigvalsh(H / np.outer(dg, dg))
         ref = max(ev[0], 0.0) / max(ev[-1], 1e-12)
         assert abs(L.lw_host_cond(A.ptr(H, A._f64p)) - ref) <= 1e-9 * max(ref, 1e-6)
+
+
+def test_grid_and_replay_entry_points_reject_bad_arguments(lw):
+    """The prior-grid, scoring and LVXR entry points refuse malformed calls before touching the device."""
+    L, A = lw._abi.lib(), lw._abi
+    E = A.LW_EINVAL
+    geo = np.array([0, 0, 0, 0.05, 20.0], np.float32)
+    dims = np.array([10, 10, 10], np.int32)
+    g, d = A.ptr(geo, A._f32p), A.ptr(dims, A._i32p)
+    steps = np.array([8, 4, 2, 1, 1], np.int32)
+    st = A.ptr(steps, A._i32p)
+    fake = 4096  # never dereferenced: each call below fails its checks first
+    wb = L.lw_grid_work_bytes(d)
+    assert wb == 2 * 16 * 1000
+    assert L.lw_grid_work_bytes(A.ptr(np.array([0, 5, 5], np.int32), A._i32p)) == -1
+    assert L.lw_grid_build(fake, 1 << 23, g, d, st, 5, fake, fake, wb, fake, None) == E  # too many points
+    assert L.lw_grid_build(fake, 10, g, d, st, 5, fake, fake, wb - 1, fake, None) == E  # work too small
+    assert L.lw_grid_build(fake, 10, g, d, st, 5, fake, fake + 8, wb, fake, None) == E  # work misaligned
+    assert L.lw_grid_build(fake, 10, g, d, A.ptr(np.array([8, 0], np.int32), A._i32p), 2, fake, fake, wb, fake,
+                           None) == E  # a zero step
+    for bad in ([0, 10, 10], [1 << 24, 1, 1], [10, 70000, 1], [46341, 46341, 2]):
+        assert L.lw_grid_build(fake, 10, g, A.ptr(np.array(bad, np.int32), A._i32p), st, 5, fake, fake, 1 << 40,
+                               fake, None) == E, bad
+    assert L.lw_grid_build(fake, 10, A.ptr(np.array([0, 0, 0, 0.0, 0.0], np.float32), A._f32p), d, st, 5, fake,
+                           fake, wb, fake, None) == E  # zero cell
+    assert L.lw_score_grid(fake, 10, fake, 5, 0, 0.0, 0.1, g, d, fake, 25.0, 10.0, fake, None) == E  # no headings
+    assert L.lw_score_grid(fake, 10, fake, 5, 180, 0.0, 0.1, g, d, fake, 0.0, 10.0, fake, None) == E  # tau 0
+    assert L.lw_score_grid(fake, 10, fake, 0, 180, 0.0, 0.1, g, d, fake, 25.0, 10.0, fake, None) == 0  # nothing
+    assert L.lw_score_list(fake, 10, fake + 4, 5, g, d, fake, 12.0, fake, None) == E  # hypotheses misaligned
+    assert L.lw_score_list(fake, 10, fake, 0, g, d, fake, 12.0, fake, None) == 0
+    assert L.lw_grid_distances(fake, 0, A.ptr(np.eye(4, dtype=np.float32), A._f32p), g, d, fake, fake, fake, fake,
+                               None) == 0
+    args = [fake] * 6
+    assert L.lw_lvxr_decode(*args, 3, 0, 10_000, 4, *[fake] * 6, None) == E  # four returns
+    assert L.lw_lvxr_decode(*args, 3, 0, 0, 1, *[fake] * 6, None) == E  # no interval
+    assert L.lw_lvxr_decode(*args, 0, 0, 10_000, 1, *[fake] * 6, None) == 0  # no packets

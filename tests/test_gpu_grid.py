@@ -2,7 +2,8 @@
 lw_score_list) against the oracle: the seeds and the distance field identical cell for cell, distances and
 scores identical bit for bit (every hypothesis's sum runs in point order in float32, as the oracle's), the
 hand-made field's closed-form sums, and on the 60 x 40 m multi-room floor (5.1M map points, 105M cells) the
-coarse and medium stages keep the true pose among the candidates they hand to the ICP."""
+coarse and medium stages keep the true pose among the candidates they hand to the ICP, and the batched ICP on
+those candidates returns it as its best."""
 
 import math
 
@@ -100,7 +101,7 @@ def test_search_keeps_the_true_pose(lw, gpu_ok):
     grid.build()
     cs = lw.CoarseSearch(grid, "cuda:0")
     rng = np.random.default_rng(33)
-    ranks = []
+    ranks, scans, cands, truth = [], [], [], []
     for q, (x, y, yaw) in enumerate(((6.3, 5.1, 0.7), (30.2, 20.4, 2.9), (53.5, 33.0, -1.8))):
         T = lw.synth.pose_matrix(np.array([x, y, 1.4]), yaw, 0.0, 0.0)
         xyz, _, _ = lw.synth.scan(S, lw.synth.Rosette(40 + q), lambda t, T=T: T, 0.0, 20000, 50 + q)
@@ -115,4 +116,27 @@ def test_search_keeps_the_true_pose(lw, gpu_ok):
         print(f"query {q}: {len(pts)} points, coarse kernel {cs.last_kernel_ms:.2f} ms, true pose at rank "
               f"{hit[0] if hit else None} of {len(Ts)}, best medium score {mid[0]:.3f}")
         ranks.append(hit[0] if hit else None)
+        scans.append(xyz)
+        cands.append(Ts[:16] if len(Ts) >= 16 else np.concatenate([Ts, np.repeat(Ts[-1:], 16 - len(Ts), 0)]))
+        truth.append(T)
     assert all(r is not None for r in ranks)
+    # then the batched ICP on the candidates, as the reference's search does: its best score is the truth
+    loc = lw.loc.BatchLocalizer(mp, voxel=0.1, reg_slots=1 << 22, device="cuda:0")
+    Tf, S, _, _ = (a.cpu().numpy() for a in loc.run(scans, np.stack(cands)))
+    for q in range(3):
+        Tb = Tf[q, int(np.argmax(S[q]))]
+        assert np.linalg.norm(Tb[:3, 3] - truth[q][:3, 3]) < 0.1, q
+
+
+def test_edge_cases(lw, small):
+    """No scan points: free positions score 0, gated ones -1; no hypotheses or distances: nothing happens."""
+    prep, grid = small
+    cs = lw.CoarseSearch(grid, "cuda:0")
+    pos, shape, dyaw, sc = cs.coarse(np.zeros((0, 3)), [0.6])
+    want = O.score_grid(np.zeros((0, 3), np.float32), pos, shape[3], 0.0, dyaw, prep["origin"], prep["cell"],
+                        prep["dims"], grid.df.cpu().numpy(), cs.tau_coarse, cs.free_cm)
+    assert np.array_equal(sc, want) and set(np.unique(sc).tolist()) <= {-1.0, 0.0}
+    assert len(cs.score_list(np.zeros((5, 3)), np.zeros((0, 4)), cs.tau_mid)) == 0
+    out = torch.full((4,), 7.0, device="cuda")
+    grid.distances(torch.zeros((4, 3), device="cuda"), 0, np.eye(4), out)
+    assert torch.all(out == 7.0)

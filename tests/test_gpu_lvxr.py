@@ -124,3 +124,18 @@ def test_dual_known_answer(lw, gpu_ok, tmp_path):
     assert np.array_equal(t[::2], t[1::2]) and np.isclose(t[100], 0.0005, rtol=1e-9, atol=0)
     assert all(int(((ret == 0) & (mask & 1 > 0) | (ret == 1) & (mask & 2 > 0)).sum()) == want
                for mask, want in ((3, 200), (1, 100), (2, 100)))
+
+
+def test_recordings_without_points_raise(lw, gpu_ok, tmp_path):
+    """A recording of IMU packets only, or a header alone, holds no firings: read_firings raises."""
+    rng = np.random.default_rng(3)
+    p = tmp_path / "imu.lvxr"
+    W.write(p, 1, 0, [W.header(6, 1000 * k) + W.records(rng, 6, 1) for k in range(5)])
+    with pytest.raises(ValueError):
+        lw.lvxr.read_firings(p, "cuda:0")
+    q = tmp_path / "empty.lvxr"
+    W.write(q, 1, 0, [])
+    with pytest.raises(ValueError):
+        lw.lvxr.read_firings(q, "cuda:0")
+    with pytest.raises(ValueError):
+        lw.lvxr._framing(b"LVXR9\n\x01\x00")  # not a recording
