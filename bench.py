@@ -403,7 +403,8 @@ def prior_grid(args, torch, lw, dev):
         "search_s": round(search_s, 3), "true_pose_rank": rank, "candidates": len(Ts),
         "best_medium_score": round(float(mid[0]), 4) if len(mid) else None,
         "roofline": {"bound": "valu", "achieved": round(valu, 3), "peak": round(VALU_PEAK_TOPS, 2),
-                     "unit": "T lane-instr/s", "frac": round(valu / VALU_PEAK_TOPS, 4), "traffic": None,
+                     "unit": "T lane-instr/s", "frac": round(valu / VALU_PEAK_TOPS, 4),
+                     "traffic": measured_traffic("k_score_grid"),
                      "kernel": "k_score_grid", "valu_per_lookup": round(GRID_VALU_PER_LOOKUP, 3),
                      "df_bytes_per_lookup": 1},
         "cpu_baseline": {"value": round(cpu_lookups / max(cpu_s, 1e-9), 1), "unit": "lookups/s", "cores": 1,

@@ -2,7 +2,7 @@
 increment from the last four increments and the frame's range / elevation histograms, in place of the
 constant-velocity guess of Odometry.predict (python/livox_warp/odom.py:591-592, used at odom.py:666).
 
-Plain ICP stays the judge: the registration starts from the learned guess, and when the first pass's residual
+Plain ICP keeps the last word: the registration starts from the learned guess, and when the first pass's residual
 at that start exceeds `fallback` times the running median of those residuals, the frame is registered again
 from the constant-velocity guess (Odometry._estimate).
 
