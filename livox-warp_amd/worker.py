@@ -96,31 +96,48 @@ class _Lane:
         self.stream = torch.cuda.Stream(device=device)  # default priority
         self.copies = 0
 
-    def own(self, a, produced_on, pool: _Pool | None = None):
-        """A copy of device tensor `a`, made on this lane's stream after everything queued on `produced_on`;
-        into a pool slot when a pool is given (None when the pool is exhausted), else a new buffer."""
+    def own_many(self, arrays, produced_on, pools=None):
+        """Owned copies of several device tensors at once (one wait on the producer, one completion event they
+        share): a list of snaps, or None when a pool is exhausted (every slot taken for them is given back)."""
         torch = _torch()
-        slot = None
-        n = a.shape[0]
-        if pool is not None:
+        pools = list(pools) if pools is not None else [None] * len(arrays)
+        dsts, slots = [], []
+        for a, pool in zip(arrays, pools):
+            if pool is None:
+                with torch.cuda.stream(self.stream):
+                    dsts.append(torch.empty_like(a))
+                slots.append(None)
+                continue
             slot = pool.take()
             if slot is None:
+                for p, sl in zip(pools, slots):
+                    if sl is not None:
+                        p.give(sl)
                 return None
-            dst = pool.bufs[slot][:n]
-        else:
-            with torch.cuda.stream(self.stream):
-                dst = torch.empty_like(a)
+            dsts.append(pool.bufs[slot][: a.shape[0]])
+            slots.append(slot)
         self.stream.wait_stream(produced_on)
         with torch.cuda.stream(self.stream):
-            dst.copy_(a, non_blocking=True)
+            for d, a in zip(dsts, arrays):
+                d.copy_(a, non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(self.stream)
-        a.record_stream(self.stream)  # the source stays allocated until the copy has read it
-        self.copies += 1
-        return _Snap(dst, ev, pool, slot)
+        for a in arrays:
+            a.record_stream(self.stream)  # each source stays allocated until the copy has read it
+        self.copies += len(arrays)
+        return [_Snap(d, ev, p, sl) for d, p, sl in zip(dsts, pools, slots)]
 
     def status(self) -> dict:
         return {"name": self.name, "copies": self.copies}
+
+
+def _events(snaps):
+    """The distinct completion events of a group of snaps (copies made together share one)."""
+    out = []
+    for sn in snaps or ():
+        if all(sn.event is not e.event for e in out):
+            out.append(sn)
+    return out
 
 
 @dataclass
@@ -224,17 +241,12 @@ class OdomWorker:
             if self._pending >= self.max_batches:
                 self.drops["input_full"] += 1
                 return False
-        snaps = []
-        for a in (sx, sa, st):
-            s = self.w_in.own(a[:n], produced_on=self.render_stream)
-            if s is None:
-                break
-            s.release_source_on(self.render_stream)
-            snaps.append(s)
-        if len(snaps) < 3:
+        snaps = self.w_in.own_many([sx[:n], sa[:n], st[:n]], produced_on=self.render_stream)
+        if snaps is None:
             with self._lock:
                 self.drops["input_own"] += 1
             return False
+        snaps[0].release_source_on(self.render_stream)  # the caller's next writes wait for the copies
         with self._lock:
             self._pending += 1
         self.q_in.put(_Batch(self.gen, epoch, snaps, n, t_min, t_max))
@@ -293,7 +305,7 @@ class OdomWorker:
                 with self._lock:
                     self.drops["stale"] += 1
                 continue
-            for s in r.snaps or ():
+            for s in _events(r.snaps):
                 s.wait(self.render_stream)
             self.frames += 1
             self._consumed.append(r)
@@ -371,7 +383,7 @@ class OdomWorker:
                 return
             t0 = time.perf_counter()
             s = self.w.stream
-            for sn in item.snaps:
+            for sn in _events(item.snaps):
                 sn.wait(s)  # the odometry's kernels wait for the input copies
             sx, sa, st = (sn.array for sn in item.snaps)
             self.odom.epoch = item.epoch
@@ -391,18 +403,12 @@ class OdomWorker:
 
     def _on_frame(self, k, T, fx, fa, ft, ff, m):
         """On the odometry thread when a frame's pose is final: copy its points out for the caller."""
-        snaps = []
-        for arr, pool in zip((fx, fa, ft, ff), self.pools):
-            sn = self.w_out.own(arr[:m], produced_on=self.w.stream, pool=pool)
-            if sn is None:
-                for x in snaps:
-                    x.release()
-                snaps = None
-                with self._lock:
-                    self.drops["output_pool"] += 1
-                break
-            sn.release_source_on(self.w.stream)  # the next frame's writes to fx..ff wait for this copy
-            snaps.append(sn)
+        snaps = self.w_out.own_many([fx[:m], fa[:m], ft[:m], ff[:m]], produced_on=self.w.stream, pools=self.pools)
+        if snaps is None:
+            with self._lock:
+                self.drops["output_pool"] += 1
+        else:
+            snaps[0].release_source_on(self.w.stream)  # the next frame's writes to fx..ff wait for the copies
         o = self.odom
         self.q_out.put(FrameResult(self._worker_gen, k, np.array(T, dtype=np.float64),
                                    np.array(o.xi, dtype=np.float32), m, snaps, dict(o.stats)))
