@@ -310,6 +310,17 @@ int lw_imap_load(lw_imap *m, const void *in, int64_t bytes);
  * (1-3) return slots of xyz (float32 m), reflectivity, tag and valid (uint8), and the firing's unit direction
  * (float32, NaN where none is known). All arrays are device pointers; asynchronous on stream (a hipStream_t).
  * Replaces read_firings' decoding (lvxr.py:189-260); the host keeps the framing and the time base. */
+/* Host functions (no device): the recording's framing, `host_ns:u64 len:u16 datagram[len]` records from byte
+ * `start` of data[0, n) (record.rs:1-5; a truncated last record dropped): offs[k] is datagram k's first byte
+ * and lens[k] its length, for the first cap; *count = the number of records (cap 0: count only; more than a
+ * nonzero cap: LW_EINVAL). Replaces the framing loop of lvxr.py:77-95 (_packets). */
+int lw_lvxr_index(const uint8_t *data, int64_t n, int64_t start, int64_t *offs, int32_t *lens, int64_t cap,
+                  int64_t *count);
+/* The live path's time base (net.rs Sink, lvxr.py:116-134 _time_base): out[i] = ns of packet i since the base,
+ * from its header stamp ts[i] (ns) and its span spans[i] (firings x interval, ns); a stamp below the base
+ * starts a new base, a packet later than expected by more than half its span and by at least jump_ns moves
+ * the base by the gap. */
+int lw_lvxr_clock(const uint64_t *ts, const int64_t *spans, int64_t n, int64_t jump_ns, int64_t *out);
 int lw_lvxr_decode(const uint8_t *bytes, const int64_t *pk_off, const int32_t *pk_type, const int32_t *pk_nf,
                    const int64_t *pk_t_ns, const int64_t *pk_first, int32_t n_packets, int32_t dual, int64_t interval_ns,
                    int32_t returns, double *t, float *xyz, uint8_t *refl, uint8_t *tag, uint8_t *valid, float *dir,

@@ -119,6 +119,8 @@ SIGNATURES = {
     "lw_imap_stats": (_i32, [_vp, _i32p]),
     "lw_imap_read": (_i32, [_vp, _vp, _i64]),
     "lw_imap_load": (_i32, [_vp, _vp, _i64]),
+    "lw_lvxr_index": (_i32, [_vp, _i64, _i64, _vp, _vp, _i64, _i64p]),
+    "lw_lvxr_clock": (_i32, [_vp, _vp, _i64, _i64, _vp]),
     "lw_lvxr_decode": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lw_grid_work_bytes": (_i64, [_i32p]),
     "lw_grid_build": (_i32, [_vp, _i32, _f32p, _i32p, _i32p, _i32, _vp, _vp, _i64, _vp, _vp]),

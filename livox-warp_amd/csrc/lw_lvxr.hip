@@ -157,6 +157,48 @@ __global__ __launch_bounds__(128) void k_lvxr_decode(const uint8_t *bytes, const
 
 }  // namespace lw
 
+// Host side: the two sequential walks of a recording (record.rs:1-5 framing; net.rs Sink's time base).
+
+extern "C" int lw_lvxr_index(const uint8_t *data, int64_t n, int64_t start, int64_t *offs, int32_t *lens, int64_t cap,
+                             int64_t *count) {
+    if (!data || n < 0 || start < 0 || cap < 0 || !count || (cap > 0 && (!offs || !lens))) return LW_EINVAL;
+    int64_t off = start, k = 0;
+    while (off + 10 <= n) {
+        const int64_t len = (int64_t)data[off + 8] | ((int64_t)data[off + 9] << 8);
+        if (off + 10 + len > n) break;  // a truncated last record is dropped
+        if (k < cap) {
+            offs[k] = off + 10;
+            lens[k] = (int32_t)len;
+        }
+        ++k;
+        off += 10 + len;
+    }
+    *count = k;
+    return (cap > 0 && k > cap) ? LW_EINVAL : 0;  // cap 0: count only
+}
+
+extern "C" int lw_lvxr_clock(const uint64_t *ts, const int64_t *spans, int64_t n, int64_t jump_ns, int64_t *out) {
+    if (n < 0 || (n > 0 && (!ts || !spans || !out))) return LW_EINVAL;
+    uint64_t base = 0, expected = 0;
+    bool have_base = false, have_expected = false;
+    for (int64_t i = 0; i < n; ++i) {
+        const uint64_t t = ts[i];
+        const int64_t span = spans[i];
+        if (!have_base || t < base) {  // the clock went back: a new base
+            base = t;
+            have_base = true;
+            have_expected = false;
+        }
+        // late by a second or more (and by more than half a packet): close the gap
+        if (have_expected && span > 0 && t > expected + (uint64_t)(span / 2) && t - expected >= (uint64_t)jump_ns)
+            base += t - expected;
+        expected = t + (uint64_t)span;
+        have_expected = true;
+        out[i] = (int64_t)(t - base);
+    }
+    return 0;
+}
+
 extern "C" int lw_lvxr_decode(const uint8_t *bytes, const int64_t *pk_off, const int32_t *pk_type, const int32_t *pk_nf,
                               const int64_t *pk_t_ns, const int64_t *pk_first, int32_t n_packets, int32_t dual,
                               int64_t interval_ns, int32_t returns, double *t, float *xyz, uint8_t *refl,

@@ -6,8 +6,8 @@ File layout (crates/livox/src/record.rs:1-5): `LVXR2\\n`, dev_type:u8, profile:u
 rsvd, status u32, timestamp type, data type, timestamp u64) and records of one data type (cartesian or
 spherical, one to three returns; the Mid-40 dual profile sends plain records in pairs, two returns of one
 firing). The host walks the framing, computes each packet's time on the live path's time base (net.rs Sink:
-reset when the clock goes back, shifted over forward jumps of a second or more) and chooses the packets the
-reference keeps; the whole file goes to the device once and one thread per firing decodes it. Firings come
+reset when the clock goes back, shifted over forward jumps of a second or more; both walks native, in
+liblwamd.so's host functions) and chooses the packets the reference keeps; the whole file goes to the device once and one thread per firing decodes it. Firings come
 back as device tensors, in time order, as the reference's Firings holds them.
 """
 
@@ -80,24 +80,24 @@ class Firings:
 
 
 def _framing(data: bytes):
-    """(dev_type, profile, datagram offsets, lengths): the records of the file, a truncated last one dropped."""
+    """(dev_type, profile, datagram offsets, lengths): the records of the file, a truncated last one dropped
+    (the walk is native: lw_lvxr_index)."""
     if data[:6] == b"LVXR1\n":
         dev, profile, off = data[6], PROFILE_STANDARD, 7
     elif data[:6] == b"LVXR2\n":
         dev, profile, off = data[6], data[7], 8
     else:
         raise ValueError("not an LVXR recording")
-    offs, lens = [], []
-    n = len(data)
-    mv = memoryview(data)
-    while off + 10 <= n:
-        ln = mv[off + 8] | (mv[off + 9] << 8)
-        if off + 10 + ln > n:
-            break
-        offs.append(off + 10)
-        lens.append(ln)
-        off += 10 + ln
-    return int(dev), int(profile), np.array(offs, np.int64), np.array(lens, np.int64)
+    L = _abi.lib()
+    buf = np.frombuffer(data, np.uint8)
+    count = ctypes.c_int64(0)
+    _abi.check(L.lw_lvxr_index(buf.ctypes.data, len(buf), off, None, None, 0, ctypes.byref(count)), "lw_lvxr_index")
+    offs = np.empty(count.value, np.int64)
+    lens = np.empty(count.value, np.int32)
+    if count.value:
+        _abi.check(L.lw_lvxr_index(buf.ctypes.data, len(buf), off, offs.ctypes.data, lens.ctypes.data, count.value,
+                                   ctypes.byref(count)), "lw_lvxr_index")
+    return int(dev), int(profile), offs, lens.astype(np.int64)
 
 
 def _stamps(buf: np.ndarray, offs: np.ndarray) -> np.ndarray:
@@ -115,16 +115,12 @@ def _stamps(buf: np.ndarray, offs: np.ndarray) -> np.ndarray:
 
 def _clock(ts: np.ndarray, spans: np.ndarray) -> np.ndarray:
     """ns since the time base per packet: a packet earlier than the base starts a new base; one more than half
-    a packet late by a second or more moves the base by the gap, so time stays continuous."""
+    a packet late by a second or more moves the base by the gap, so time stays continuous (lw_lvxr_clock)."""
+    ts = np.ascontiguousarray(ts, np.uint64)
+    spans = np.ascontiguousarray(spans, np.int64)
     out = np.empty(len(ts), np.int64)
-    base, expected = None, None
-    for i, (t, span) in enumerate(zip(ts.tolist(), spans.tolist())):
-        if base is None or t < base:
-            base, expected = t, None
-        if expected is not None and span > 0 and t > expected + span // 2 and t - expected >= CLOCK_JUMP_NS:
-            base += t - expected
-        expected = t + span
-        out[i] = t - base
+    _abi.check(_abi.lib().lw_lvxr_clock(ts.ctypes.data, spans.ctypes.data, len(ts), CLOCK_JUMP_NS, out.ctypes.data),
+               "lw_lvxr_clock")
     return out
 
 

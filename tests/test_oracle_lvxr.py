@@ -127,3 +127,20 @@ def test_dual_known_answer(tmp_path):
     assert np.array_equal(pt[::2], pt[1::2]) and np.isclose(np.float32(pt[100]), np.float32(0.0005))
     assert np.array_equal(px[:, 0], (2000 + np.arange(200) % 100 * 100).astype(np.float32) * np.float32(1e-3))
     assert np.all(pa & 255 == 200)
+
+
+def test_native_time_base_matches_oracle(lw):
+    """lw_lvxr_clock (the product's native walk) against the oracle's time base on random stamp sequences with
+    steady packets, jitter, resets backwards, jumps forward of more and less than a second, and empty spans."""
+    rng = np.random.default_rng(12)
+    for trial in range(40):
+        n = int(rng.integers(1, 400))
+        spans = rng.choice([0, 480_000, 960_000, 1_000_000], n).astype(np.int64)
+        steps = spans + rng.integers(-3000, 3000, n)
+        ev = rng.random(n)
+        steps[ev < 0.03] = -rng.integers(1, 5_000_000_000, int((ev < 0.03).sum()))  # the clock goes back
+        steps[(ev > 0.97)] += rng.integers(900_000_000, 3_000_000_000, int((ev > 0.97).sum()))  # pauses
+        ts = np.maximum(np.cumsum(steps) + 10_000_000_000, 0).astype(np.uint64)
+        got = lw.lvxr._clock(ts, spans)
+        want = X.time_base([int(t) for t in ts], [int(s) for s in spans])
+        assert np.array_equal(got, np.array(want, np.int64)), trial
