@@ -357,6 +357,22 @@ int lw_score_grid(const float *pts, int32_t n_pts, const float *pos, int32_t n_p
 int lw_score_list(const float *pts, int32_t n_pts, const float *hyp, int32_t n_hyp, const float *geo,
                   const int32_t *dims, const uint8_t *df, float tau_cm, float *out, void *stream);
 
+/* Point-to-plane ICP of k poses (T: device, k row-major float32 4x4, k <= 65535) at once against the grid's
+ * map points, normals (nrm) and planarity: one Gauss-Newton accumulation each, into out (device, k x 64
+ * doubles in the odometry's ACC layout: H upper triangle 0-20, b 21-26, matched count 27, weighted error 28,
+ * weight sum 30). Correspondences within sqrt(max_d2) of the nearest map point; precision n n^T (1 - alpha) +
+ * alpha I on planar points, 4 alpha I elsewhere; Geman-McClure weight with scale c2. work: device scratch of
+ * lw_plane_work_bytes(k) bytes. Replaces k_icp_batch + k_reduce_batch (localize.py:126-180). */
+int64_t lw_plane_work_bytes(int32_t k);
+int lw_plane_icp(const float *pts, int32_t n_pts, const float *T, int32_t k, const float *geo, const int32_t *dims,
+                 const int32_t *start, const int32_t *nn, const float *xyz, const float *nrm, const float *planar,
+                 float max_d2, float c2, float alpha, void *work, int64_t work_bytes, double *out, void *stream);
+/* Per pose, the scan points whose nearest map point lies within radii[0..2] (host array): counts (device, k x 3
+ * int32, zeroed by the call). Replaces k_eval_batch (localize.py:183-206). */
+int lw_plane_eval(const float *pts, int32_t n_pts, const float *T, int32_t k, const float *geo, const int32_t *dims,
+                  const int32_t *start, const int32_t *nn, const float *xyz, const float *radii, int32_t *counts,
+                  void *stream);
+
 /* Host builds of the device's float64 step and per-voxel precision, for tests on machines without a
  * GPU (same source as the device code). */
 int lw_host_gn_step(const double *v, const double *T, const double *T_pred, double damping, double prior_info,

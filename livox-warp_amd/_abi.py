@@ -124,6 +124,10 @@ SIGNATURES = {
     "lw_lvxr_decode": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lw_grid_work_bytes": (_i64, [_i32p]),
     "lw_grid_build": (_i32, [_vp, _i32, _f32p, _i32p, _i32p, _i32, _vp, _vp, _i64, _vp, _vp]),
+    "lw_plane_work_bytes": (_i64, [_i32]),
+    "lw_plane_icp": (_i32, [_vp, _i32, _vp, _i32, _f32p, _i32p, _vp, _vp, _vp, _vp, _vp, _f32, _f32, _f32, _vp, _i64, _vp,
+                             _vp]),
+    "lw_plane_eval": (_i32, [_vp, _i32, _vp, _i32, _f32p, _i32p, _vp, _vp, _vp, _f32p, _vp, _vp]),
     "lw_grid_distances": (_i32, [_vp, _i32, _f32p, _f32p, _i32p, _vp, _vp, _vp, _vp, _vp]),
     "lw_score_grid": (_i32, [_vp, _i32, _vp, _i32, _i32, _f32, _f32, _f32p, _i32p, _vp, _f32, _f32, _vp, _vp]),
     "lw_score_list": (_i32, [_vp, _i32, _vp, _i32, _f32p, _i32p, _vp, _f32, _vp, _vp]),

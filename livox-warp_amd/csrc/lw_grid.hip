@@ -367,6 +367,133 @@ __global__ __launch_bounds__(256) void k_score_list(const float *__restrict__ pt
     if (active) out[h] = s;
 }
 
+// ---- point-to-plane ICP over the grid (localize.py:126-210) ----------------------------------------------
+
+constexpr int PLANE_CHUNKS = 64;  // point chunks per hypothesis, one workgroup each
+constexpr int PLANE_SUMS = 30;    // H (21), b (6), count, weighted error, weight sum
+constexpr float PLANAR = 0.5f;    // mapgrid.py:26
+
+__device__ inline double wave_sum_f64(double v) {
+    _Pragma("unroll") for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// localize.py:126-168 k_icp_batch for the points of chunk blockIdx.x of hypothesis blockIdx.y: each term in
+// float32 as the reference forms it, widened to float64 and summed per thread, then per wave and per
+// workgroup in a fixed order, into the chunk's partial row (PLANE_SUMS doubles, 32 apart).
+__global__ __launch_bounds__(256) void k_plane_icp(const float *__restrict__ pts, int32_t n_pts,
+                                                   const float *__restrict__ Tk, GridGeo g,
+                                                   const int32_t *__restrict__ start, const int32_t *__restrict__ nn,
+                                                   const float *__restrict__ xyz, const float *__restrict__ nrm,
+                                                   const float *__restrict__ planar, float max_d2, float c2,
+                                                   float alpha, double *__restrict__ partials) {
+    __shared__ double red[4][PLANE_SUMS];
+    const int h = blockIdx.y, chunk = blockIdx.x;
+    const float *T = Tk + 16 * h;
+    const int32_t lo = (int32_t)((int64_t)n_pts * chunk / PLANE_CHUNKS),
+                  hi = (int32_t)((int64_t)n_pts * (chunk + 1) / PLANE_CHUNKS);
+    double acc[PLANE_SUMS];
+    _Pragma("unroll") for (int k = 0; k < PLANE_SUMS; ++k) acc[k] = 0.0;
+    for (int32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const float *p = pts + 3 * (int64_t)i;
+        float w[3];
+        for (int r = 0; r < 3; ++r) w[r] = T[4 * r] * p[0] + T[4 * r + 1] * p[1] + T[4 * r + 2] * p[2] + T[4 * r + 3];
+        const int32_t s = nearest_point(g, start, nn, xyz, w[0], w[1], w[2]);
+        if (s < 0) continue;
+        const float *q = xyz + 3 * (int64_t)s;
+        const float r0 = w[0] - q[0], r1 = w[1] - q[1], r2 = w[2] - q[2];
+        if (r0 * r0 + r1 * r1 + r2 * r2 > max_d2) continue;
+        const float *n = nrm + 3 * (int64_t)s;
+        float M[3][3] = {{alpha, 0.0f, 0.0f}, {0.0f, alpha, 0.0f}, {0.0f, 0.0f, alpha}};
+        if (planar[s] > PLANAR && sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) > 0.5f) {
+            const float f = 1.0f - alpha;  // the normal at full weight, along the surface only a little
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) M[a][b] = n[a] * n[b] * f + M[a][b];
+        } else {
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) M[a][b] = M[a][b] * 4.0f;  // a weak point-to-point pull
+        }
+        const float Mr[3] = {M[0][0] * r0 + M[0][1] * r1 + M[0][2] * r2, M[1][0] * r0 + M[1][1] * r1 + M[1][2] * r2,
+                             M[2][0] * r0 + M[2][1] * r1 + M[2][2] * r2};
+        const float d2 = r0 * Mr[0] + r1 * Mr[1] + r2 * Mr[2];
+        float wgt = 1.0f / (1.0f + d2 / c2);
+        wgt = wgt * wgt;  // Geman-McClure
+        const float J[3][6] = {{1.0f, 0.0f, 0.0f, 0.0f, w[2], -w[1]},
+                               {0.0f, 1.0f, 0.0f, -w[2], 0.0f, w[0]},
+                               {0.0f, 0.0f, 1.0f, w[1], -w[0], 0.0f}};
+        int idx = 0;
+        _Pragma("unroll") for (int a = 0; a < 6; ++a) {
+            _Pragma("unroll") for (int b = a; b < 6; ++b) {
+                float v = M[0][0] * J[0][a] * J[0][b] + M[0][1] * J[0][a] * J[1][b] + M[0][2] * J[0][a] * J[2][b];
+                v += M[1][0] * J[1][a] * J[0][b] + M[1][1] * J[1][a] * J[1][b] + M[1][2] * J[1][a] * J[2][b];
+                v += M[2][0] * J[2][a] * J[0][b] + M[2][1] * J[2][a] * J[1][b] + M[2][2] * J[2][a] * J[2][b];
+                acc[idx++] += (double)(wgt * v);
+            }
+        }
+        _Pragma("unroll") for (int a = 0; a < 6; ++a)
+            acc[21 + a] += (double)(wgt * (Mr[0] * J[0][a] + Mr[1] * J[1][a] + Mr[2] * J[2][a]));
+        acc[27] += 1.0;
+        acc[28] += (double)(wgt * (r0 * Mr[0] + r1 * Mr[1] + r2 * Mr[2]));
+        acc[29] += (double)wgt;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    _Pragma("unroll") for (int k = 0; k < PLANE_SUMS; ++k) {
+        const double v = wave_sum_f64(acc[k]);
+        if (lane == 0) red[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < PLANE_SUMS) {
+        const int k = threadIdx.x;
+        partials[((int64_t)h * PLANE_CHUNKS + chunk) * 32 + k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+    }
+}
+
+// localize.py:171-180 k_reduce_batch: hypothesis blockIdx.x's chunks summed in chunk order into its
+// accumulator row (64 doubles: H 0-20, b 21-26, count 27, weighted error 28, weight sum 30, the rest 0).
+__global__ __launch_bounds__(64) void k_plane_reduce(const double *__restrict__ partials, double *__restrict__ out) {
+    const int h = blockIdx.x, j = threadIdx.x;
+    const int src = j < 29 ? j : (j == 30 ? 29 : -1);  // the weight sum sits at ACC_WEIGHT = 30, ACC_RMS stays 0
+    double s = 0.0;
+    if (src >= 0)
+        for (int c = 0; c < PLANE_CHUNKS; ++c) s += partials[((int64_t)h * PLANE_CHUNKS + c) * 32 + src];
+    out[(int64_t)h * 64 + j] = s;
+}
+
+// localize.py:183-206 k_eval_batch: per hypothesis, the scan points whose nearest map point is within
+// radii[0], radii[1], radii[2] (counts zeroed by the launcher; integer sums, so the counts are exact).
+__global__ __launch_bounds__(256) void k_plane_eval(const float *__restrict__ pts, int32_t n_pts,
+                                                    const float *__restrict__ Tk, GridGeo g,
+                                                    const int32_t *__restrict__ start, const int32_t *__restrict__ nn,
+                                                    const float *__restrict__ xyz, float rad0, float rad1, float rad2,
+                                                    int32_t *__restrict__ counts) {
+    const int h = blockIdx.y, chunk = blockIdx.x;
+    const float *T = Tk + 16 * h;
+    const int32_t lo = (int32_t)((int64_t)n_pts * chunk / PLANE_CHUNKS),
+                  hi = (int32_t)((int64_t)n_pts * (chunk + 1) / PLANE_CHUNKS);
+    int32_t c0 = 0, c1 = 0, c2 = 0;
+    for (int32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const float *p = pts + 3 * (int64_t)i;
+        float w[3];
+        for (int r = 0; r < 3; ++r) w[r] = T[4 * r] * p[0] + T[4 * r + 1] * p[1] + T[4 * r + 2] * p[2] + T[4 * r + 3];
+        const int32_t s = nearest_point(g, start, nn, xyz, w[0], w[1], w[2]);
+        if (s < 0) continue;
+        const float d = sqrtf(dist2_to(w[0], w[1], w[2], xyz + 3 * (int64_t)s));
+        c0 += d < rad0;
+        c1 += d < rad1;
+        c2 += d < rad2;
+    }
+    _Pragma("unroll") for (int off = 32; off > 0; off >>= 1) {
+        c0 += __shfl_xor(c0, off, 64);
+        c1 += __shfl_xor(c1, off, 64);
+        c2 += __shfl_xor(c2, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (c0) atomicAdd(counts + 3 * h, c0);
+        if (c1) atomicAdd(counts + 3 * h + 1, c1);
+        if (c2) atomicAdd(counts + 3 * h + 2, c2);
+    }
+}
+
 bool geo_of(const float *geo, const int32_t *dims, GridGeo &g) {
     if (!geo || !dims || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return false;
     if ((int64_t)dims[0] * dims[1] * dims[2] > (int64_t)INT32_MAX - 1) return false;
@@ -418,6 +545,42 @@ extern "C" int lw_grid_build(const float *xyz, int32_t n, const float *geo, cons
         std::swap(a, b);
     }
     hipLaunchKernelGGL(lw::k_grid_df, cells, dim3(256), 0, s, a, g, nn, df);
+    return lw::launched();
+}
+
+extern "C" int64_t lw_plane_work_bytes(int32_t k) {
+    return k < 0 ? -1 : (int64_t)k * lw::PLANE_CHUNKS * 32 * (int64_t)sizeof(double);
+}
+
+extern "C" int lw_plane_icp(const float *pts, int32_t n_pts, const float *T, int32_t k, const float *geo,
+                            const int32_t *dims, const int32_t *start, const int32_t *nn, const float *xyz,
+                            const float *nrm, const float *planar, float max_d2, float c2, float alpha, void *work,
+                            int64_t work_bytes, double *out, void *stream) {
+    GridGeo g;
+    if (!lw::geo_of(geo, dims, g) || n_pts < 0 || k < 0 || k > 65535 || !(c2 > 0.0f)) return LW_EINVAL;
+    if (k == 0) return 0;
+    if ((n_pts > 0 && !pts) || !T || !start || !nn || !xyz || !nrm || !planar || !work || !out ||
+        work_bytes < lw_plane_work_bytes(k))
+        return LW_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    double *partials = static_cast<double *>(work);
+    hipLaunchKernelGGL(lw::k_plane_icp, dim3(lw::PLANE_CHUNKS, (unsigned)k), dim3(256), 0, s, pts, n_pts, T, g, start,
+                       nn, xyz, nrm, planar, max_d2, c2, alpha, partials);
+    hipLaunchKernelGGL(lw::k_plane_reduce, dim3((unsigned)k), dim3(64), 0, s, partials, out);
+    return lw::launched();
+}
+
+extern "C" int lw_plane_eval(const float *pts, int32_t n_pts, const float *T, int32_t k, const float *geo,
+                             const int32_t *dims, const int32_t *start, const int32_t *nn, const float *xyz,
+                             const float *radii, int32_t *counts, void *stream) {
+    GridGeo g;
+    if (!lw::geo_of(geo, dims, g) || n_pts < 0 || k < 0 || k > 65535 || !radii) return LW_EINVAL;
+    if (k == 0) return 0;
+    if ((n_pts > 0 && !pts) || !T || !start || !nn || !xyz || !counts) return LW_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(lw::k_grid_fill, dim3(1), dim3(256), 0, s, counts, (int64_t)3 * k, 0);
+    hipLaunchKernelGGL(lw::k_plane_eval, dim3(lw::PLANE_CHUNKS, (unsigned)k), dim3(256), 0, s, pts, n_pts, T, g, start,
+                       nn, xyz, radii[0], radii[1], radii[2], counts);
     return lw::launched();
 }
 

@@ -29,6 +29,22 @@ def _torch():
     return torch
 
 
+EVAL_RADII = (0.03, 0.05, 0.10)  # a pose's fit: fractions of the scan within these distances (m) of the map
+_TRIU = np.triu_indices(6)  # the order the accumulator stores the upper triangle of the 6x6 normal matrix in
+ACC_H, ACC_B, ACC_COUNT = 0, 21, 27
+
+
+def twist_apply(T: np.ndarray, delta: np.ndarray) -> np.ndarray:
+    """A Gauss-Newton step (translation, rotation vector) left-multiplied in the map frame (localize.py:227-233)."""
+    from scipy.spatial.transform import Rotation
+
+    Rz = Rotation.from_rotvec(delta[3:]).as_matrix()
+    out = T.copy()
+    out[:3, :3] = Rz @ T[:3, :3]
+    out[:3, 3] = Rz @ T[:3, 3] + delta[:3]
+    return out
+
+
 def rotz(a: float) -> np.ndarray:
     c, s = math.cos(a), math.sin(a)
     return np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
@@ -40,6 +56,7 @@ class CoarseSearch:
     PTS_CAP = 1 << 17  # scan points after 5 cm thinning (localize.py:343)
     SCORE_CAP = 1 << 23  # (position, heading) hypotheses per coarse launch
     HYP_CAP = 1 << 17  # listed hypotheses per launch
+    ICP_CAP = 16  # poses refined or evaluated together
 
     def __init__(self, grid: PriorGrid, device=None):
         torch = _torch()
@@ -50,6 +67,12 @@ class CoarseSearch:
         self.d_pos = torch.empty((self.SCORE_CAP // 64, 3), dtype=torch.float32, device=d)
         self.d_score = torch.empty(self.SCORE_CAP, dtype=torch.float32, device=d)
         self.d_hyp = torch.empty((self.HYP_CAP, 4), dtype=torch.float32, device=d)
+        self.d_T = torch.empty((self.ICP_CAP, 16), dtype=torch.float32, device=d)
+        self.d_acc = torch.empty((self.ICP_CAP, 64), dtype=torch.float64, device=d)
+        self.d_counts = torch.empty((self.ICP_CAP, 3), dtype=torch.int32, device=d)
+        wb = int(_abi.lib().lw_plane_work_bytes(self.ICP_CAP))
+        self.d_work = torch.empty(wb, dtype=torch.uint8, device=d)
+        self.seed = 0  # refine() draws its subsets from a fresh generator with this seed
         # tuning (localize.py:375-388)
         self.xy_step = 0.25
         self.yaw_step_deg = 2.0
@@ -224,3 +247,118 @@ class CoarseSearch:
             T[:3, 3] = (x, y, z)
             Ts.append(T)
         return np.array(Ts).reshape(-1, 4, 4), ms[win] / max(len(pm), 1), sc[best] / max(len(pc), 1)
+
+    # ---- point-to-plane ICP against the grid (localize.py:517-600) ----------------------------------------
+
+    def _poses_up(self, Ts, s):
+        torch = _torch()
+        k = len(Ts)
+        if k > self.ICP_CAP:
+            raise ValueError(f"{k} poses; at most {self.ICP_CAP} together")
+        with torch.cuda.stream(s):
+            self.d_T[:k].copy_(torch.from_numpy(np.ascontiguousarray(np.array(Ts, np.float32).reshape(k, 16))))
+        return k
+
+    def icp(self, Ts, n_pts: int, schedule=((0.40, 8), (0.20, 8), (0.10, 8), (0.05, 10)), stream=None):
+        """Point-to-plane ICP of every pose in Ts at once against the map, the scan already uploaded (its first
+        n_pts points, see upload_scan): per stage (correspondence distance m, iterations) one accumulation on the
+        device per iteration, then the damped 6x6 solve per pose on the host, until no pose moves more than
+        2e-4 (rotation weighted x5) (localize.py:517-553)."""
+        Ts = [np.array(T, dtype=np.float64) for T in Ts]
+        for dmax, iters in schedule:
+            for _ in range(iters):
+                v = self.accumulate(Ts, n_pts, dmax, stream)
+                k = len(Ts)
+                moved = 0.0
+                for h in range(k):
+                    if v[h, ACC_COUNT] < 30:
+                        continue
+                    H = np.zeros((6, 6))
+                    H[_TRIU] = v[h, ACC_H:ACC_H + 21]
+                    H += np.triu(H, 1).T
+                    b = v[h, ACC_B:ACC_B + 6]
+                    A = H + np.diag(1e-3 * np.diag(H) + 1e-9)
+                    try:
+                        delta = np.linalg.solve(A, -b)
+                    except np.linalg.LinAlgError:
+                        continue
+                    Ts[h] = twist_apply(Ts[h], delta)
+                    moved = max(moved, float(np.linalg.norm(delta[:3])), float(np.linalg.norm(delta[3:])) * 5.0)
+                if moved < 2e-4:
+                    break
+        return Ts
+
+    def accumulate(self, Ts, n_pts: int, dmax: float, stream=None, alpha: float = 0.05) -> np.ndarray:
+        """One point-to-plane accumulation for every pose in Ts against the map (correspondences within dmax,
+        Geman-McClure scale dmax / 2): (k, 64) float64 rows in the accumulator layout, read back."""
+        torch = _torch()
+        g = self.grid
+        s = self._stream(stream)
+        k = self._poses_up(Ts, s)
+        c2 = (dmax * 0.5) ** 2
+        with torch.cuda.stream(s):
+            _abi.check(_abi.lib().lw_plane_icp(self.d_pts.data_ptr(), n_pts, self.d_T.data_ptr(), k, *g._geo_args(),
+                                               g.start.data_ptr(), g.nn.data_ptr(), g.xyz.data_ptr(), g.nrm.data_ptr(),
+                                               g.planar.data_ptr(), float(dmax * dmax), float(c2), float(alpha),
+                                               self.d_work.data_ptr(), self.d_work.numel(), self.d_acc.data_ptr(),
+                                               ctypes.c_void_p(s.cuda_stream)), "lw_plane_icp")
+            return self.d_acc[:k].cpu().numpy()
+
+    def evaluate(self, Ts, n_pts: int, stream=None) -> np.ndarray:
+        """Fraction of the uploaded scan within each of EVAL_RADII of the map, per pose (localize.py:555-565)."""
+        torch = _torch()
+        g = self.grid
+        s = self._stream(stream)
+        k = self._poses_up(Ts, s)
+        radii = np.array(EVAL_RADII, np.float32)
+        with torch.cuda.stream(s):
+            _abi.check(_abi.lib().lw_plane_eval(self.d_pts.data_ptr(), n_pts, self.d_T.data_ptr(), k, *g._geo_args(),
+                                                g.start.data_ptr(), g.nn.data_ptr(), g.xyz.data_ptr(),
+                                                _abi.ptr(radii, _abi._f32p), self.d_counts.data_ptr(),
+                                                ctypes.c_void_p(s.cuda_stream)), "lw_plane_eval")
+            c = self.d_counts[:k].cpu().numpy()
+        return c / max(n_pts, 1)
+
+    def upload_scan(self, pts: np.ndarray, stream=None) -> int:
+        """The scan (sensor frame) that icp() and evaluate() read: returns its point count."""
+        return self._upload_pts(np.asarray(pts, np.float32), self._stream(stream))
+
+    def refine(self, xyz: np.ndarray, T0: np.ndarray, stream=None, schedule=((0.30, 6), (0.12, 8), (0.05, 8)),
+               starts=(), switch_margin: float = 0.05):
+        """Point-to-plane ICP of a point set against the map from T0 (the pose-tracking step, localize.py:
+        567-612): one point per 5 cm voxel, T0 and the offset starts (x, y, z m, optional heading deg about the
+        vertical through the points' centre) refined together; an offset start wins only if its 3 cm fit beats
+        the plain result by switch_margin. Returns (T, fit, info)."""
+        rng = np.random.default_rng(self.seed)
+        pts = np.asarray(xyz, dtype=np.float32)
+        key = np.floor(pts / 0.05).astype(np.int64)
+        _, first = np.unique(key, axis=0, return_index=True)
+        pts = pts[np.sort(first)]
+        if len(pts) > self.PTS_CAP:
+            pts = pts[rng.choice(len(pts), self.PTS_CAP, replace=False)]
+        n = len(pts)
+        if n < 200:
+            raise RuntimeError(f"too few points to refine ({n} after 5 cm thinning)")
+        self.upload_scan(pts, stream)
+        T0 = np.asarray(T0, dtype=np.float64)
+        inits = [T0]
+        c_map = T0[:3, :3] @ pts.mean(axis=0).astype(np.float64) + T0[:3, 3]
+        for off in list(starts)[: self.ICP_CAP - 2]:
+            Ti = T0.copy()
+            off = np.asarray(off, dtype=np.float64)
+            if len(off) > 3 and off[3] != 0.0:
+                Rz = rotz(math.radians(off[3]))
+                Ti[:3, :3] = Rz @ Ti[:3, :3]
+                Ti[:3, 3] = Rz @ (Ti[:3, 3] - c_map) + c_map
+            Ti[:3, 3] += off[:3]
+            inits.append(Ti)
+        Ts = self.icp(inits, n, schedule, stream)
+        fits = self.evaluate(Ts + [T0], n, stream)
+        fit_start = fits[-1]
+        fits = fits[:-1]
+        best = 0
+        for i in range(1, len(Ts)):
+            if fits[i, 0] > fits[best, 0] + (switch_margin if best == 0 else 0.0):
+                best = i
+        return Ts[best], fits[best], {"switched": best != 0, "plain_fit": float(fits[0, 0]), "n": n,
+                                      "fit_start": float(fit_start[0])}

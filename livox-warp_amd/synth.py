@@ -353,12 +353,13 @@ def scan(S: Scene, rosette: Rosette, pose_fn, t0: float, n: int, seed: int, nois
     return xyz, attr, t.astype(np.float32)
 
 
-def sample_surfaces(S: Scene, spacing: float, seed: int = 7, prune: bool = True) -> np.ndarray:
+def sample_surfaces(S: Scene, spacing: float, seed: int = 7, prune: bool = True, normals: bool = False):
     """Points on every visible surface of the scene (room inside faces, box faces except the bottom) on a
     uniform grid of the given spacing with a seeded sub-spacing offset; world float32 (n, 3). prune=False keeps
-    the few points hidden inside other boxes (faster for very large scenes)."""
+    the few points hidden inside other boxes (faster for very large scenes). normals=True also returns each
+    point's face normal (float32 (n, 3), unit, sign arbitrary): (points, normals)."""
     rng = np.random.default_rng(seed)
-    out = []
+    out, nrm = [], []
 
     def face(origin, u, v, lu, lv):
         nu, nv = max(int(lu / spacing), 1), max(int(lv / spacing), 1)
@@ -366,6 +367,7 @@ def sample_surfaces(S: Scene, spacing: float, seed: int = 7, prune: bool = True)
         b = (np.arange(nv) + rng.uniform(0, 1)) * (lv / nv)
         A, B = np.meshgrid(a, b, indexing="ij")
         out.append(origin[None] + A.reshape(-1, 1) * u[None] + B.reshape(-1, 1) * v[None])
+        nrm.append(np.broadcast_to(np.cross(u, v), (A.size, 3)))
 
     ex, ey, ez = np.eye(3)
 
@@ -382,13 +384,15 @@ def sample_surfaces(S: Scene, spacing: float, seed: int = 7, prune: bool = True)
     for lo, hi in zip(S.box_lo, S.box_hi):
         boxfaces(lo, hi, lo[2] > S.room_lo[2] + 1e-6)
     pts = np.concatenate(out)
-    if not prune:
-        return pts.astype(np.float32)
-    # drop points hidden inside a box (faces that touch another box or the floor)
-    inside = np.zeros(len(pts), bool)
-    for lo, hi in zip(S.box_lo, S.box_hi):
-        inside |= np.all((pts > lo + 1e-6) & (pts < hi - 1e-6), axis=1)
-    return pts[~inside].astype(np.float32)
+    keep = np.ones(len(pts), bool)
+    if prune:  # drop points hidden inside a box (faces that touch another box or the floor)
+        for lo, hi in zip(S.box_lo, S.box_hi):
+            keep &= ~np.all((pts > lo + 1e-6) & (pts < hi - 1e-6), axis=1)
+    pts = (pts[keep] if prune else pts).astype(np.float32)
+    if not normals:
+        return pts
+    n = np.concatenate(nrm)
+    return pts, (n[keep] if prune else n).astype(np.float32)
 
 
 @dataclass

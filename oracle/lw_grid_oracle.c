@@ -9,7 +9,9 @@
  *   python/livox_warp/mapgrid.py   :19-26 constants, :38-62 grid_flat, grid_at, cell_centre,
  *                                   :65-100 nearest_point, :103-112 k_seed, :115-123 k_unpack,
  *                                   :126-155 k_jfa, :158-167 k_df, :170-193 k_changes_T, :275-290 build
- *   python/livox_warp/localize.py  :55-92 k_score_grid, :95-123 k_score_list
+ *   python/livox_warp/localize.py  :55-92 k_score_grid, :95-123 k_score_list, :126-180 k_icp_batch and
+ *                                   k_reduce_batch, :183-206 k_eval_batch
+ *   python/livox_warp/odom.py      :234-256 accumulate (the ACC layout, :72-82)
  *
  * Choices where the reference leaves the bits open ("parity unpinned" at the bit level, pinned by the
  * properties the tests check instead):
@@ -237,5 +239,80 @@ void lwo_score_list(const float *pts, int32_t n_pts, const float *hyp, int32_t n
     for (int32_t h = 0; h < n_hyp; ++h) {
         const float *q = hyp + 4 * (int64_t)h;
         out[h] = soft_count(pts, n_pts, q, (float)cos((double)q[3]), (float)sin((double)q[3]), geo, dims, df, tau_cm);
+    }
+}
+
+static void xform_row(const float *T, const float *p, float *w)
+{
+    for (int r = 0; r < 3; ++r) w[r] = T[4 * r] * p[0] + T[4 * r + 1] * p[1] + T[4 * r + 2] * p[2] + T[4 * r + 3];
+}
+
+/* localize.py:126-180 k_icp_batch then k_reduce_batch, in point order: hypothesis h's accumulator row out[64 h ..]
+ * (H upper triangle 0-20, b 21-26, count 27, weighted error 28, weight sum 30; odom.py:73-82), each term in
+ * float32 as the reference forms it, summed in float64. T: k row-major float32 4x4 poses. */
+void lwo_plane_icp(const float *pts, int32_t n_pts, const float *T, int32_t k, const float *geo, const int32_t *dims,
+                   const int32_t *start, const int32_t *nn, const float *xyz, const float *nrm, const float *planar,
+                   float max_d2, float c2, float alpha, double *out)
+{
+    for (int32_t h = 0; h < k; ++h) {
+        double *acc = out + 64 * (int64_t)h;
+        for (int j = 0; j < 64; ++j) acc[j] = 0.0;
+        for (int32_t i = 0; i < n_pts; ++i) {
+            float w[3];
+            xform_row(T + 16 * (int64_t)h, pts + 3 * (int64_t)i, w);
+            int32_t s = lwo_grid_nearest(w, geo, dims, start, nn, xyz);
+            if (s < 0) continue;
+            const float *q = xyz + 3 * (int64_t)s;
+            float r[3] = {w[0] - q[0], w[1] - q[1], w[2] - q[2]};
+            if (r[0] * r[0] + r[1] * r[1] + r[2] * r[2] > max_d2) continue;
+            const float *n = nrm + 3 * (int64_t)s;
+            float M[3][3] = {{alpha, 0.0f, 0.0f}, {0.0f, alpha, 0.0f}, {0.0f, 0.0f, alpha}};
+            if (planar[s] > 0.5f && sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]) > 0.5f) {
+                float f = 1.0f - alpha;
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) M[a][b] = n[a] * n[b] * f + M[a][b];
+            } else {
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) M[a][b] = M[a][b] * 4.0f;
+            }
+            float Mr[3];
+            for (int a = 0; a < 3; ++a) Mr[a] = M[a][0] * r[0] + M[a][1] * r[1] + M[a][2] * r[2];
+            float d2 = r[0] * Mr[0] + r[1] * Mr[1] + r[2] * Mr[2];
+            float wgt = 1.0f / (1.0f + d2 / c2);
+            wgt = wgt * wgt;
+            float J[3][6] = {{1.0f, 0.0f, 0.0f, 0.0f, w[2], -w[1]},
+                             {0.0f, 1.0f, 0.0f, -w[2], 0.0f, w[0]},
+                             {0.0f, 0.0f, 1.0f, w[1], -w[0], 0.0f}};
+            int idx = 0;
+            for (int a = 0; a < 6; ++a)
+                for (int b = a; b < 6; ++b) {
+                    float v = M[0][0] * J[0][a] * J[0][b] + M[0][1] * J[0][a] * J[1][b] + M[0][2] * J[0][a] * J[2][b];
+                    v += M[1][0] * J[1][a] * J[0][b] + M[1][1] * J[1][a] * J[1][b] + M[1][2] * J[1][a] * J[2][b];
+                    v += M[2][0] * J[2][a] * J[0][b] + M[2][1] * J[2][a] * J[1][b] + M[2][2] * J[2][a] * J[2][b];
+                    acc[idx++] += (double)(wgt * v);
+                }
+            for (int a = 0; a < 6; ++a) acc[21 + a] += (double)(wgt * (Mr[0] * J[0][a] + Mr[1] * J[1][a] + Mr[2] * J[2][a]));
+            acc[27] += 1.0;
+            acc[28] += (double)(wgt * (r[0] * Mr[0] + r[1] * Mr[1] + r[2] * Mr[2]));
+            acc[30] += (double)wgt;
+        }
+    }
+}
+
+/* localize.py:183-206 k_eval_batch: per hypothesis, the points whose nearest map point is within each radius. */
+void lwo_plane_eval(const float *pts, int32_t n_pts, const float *T, int32_t k, const float *geo, const int32_t *dims,
+                    const int32_t *start, const int32_t *nn, const float *xyz, const float *radii, int32_t *counts)
+{
+    for (int32_t h = 0; h < k; ++h) {
+        counts[3 * h] = counts[3 * h + 1] = counts[3 * h + 2] = 0;
+        for (int32_t i = 0; i < n_pts; ++i) {
+            float w[3];
+            xform_row(T + 16 * (int64_t)h, pts + 3 * (int64_t)i, w);
+            int32_t s = lwo_grid_nearest(w, geo, dims, start, nn, xyz);
+            if (s < 0) continue;
+            float d = sqrtf(dist2(w, xyz + 3 * (int64_t)s));
+            for (int r = 0; r < 3; ++r)
+                if (d < radii[r]) counts[3 * h + r] += 1;
+        }
     }
 }

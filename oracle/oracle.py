@@ -83,6 +83,10 @@ _SIGS = {
                               ctypes.c_float, _f32p, _i32p, _u8p, ctypes.c_float, ctypes.c_float, _f32p]),
     "lwo_score_list": (None, [_f32p, ctypes.c_int32, _f32p, ctypes.c_int32, _f32p, _i32p, _u8p, ctypes.c_float,
                               _f32p]),
+    "lwo_plane_icp": (None, [_f32p, ctypes.c_int32, _f32p, ctypes.c_int32, _f32p, _i32p, _i32p, _i32p, _f32p, _f32p,
+                             _f32p, ctypes.c_float, ctypes.c_float, ctypes.c_float, _f64p]),
+    "lwo_plane_eval": (None, [_f32p, ctypes.c_int32, _f32p, ctypes.c_int32, _f32p, _i32p, _i32p, _i32p, _f32p, _f32p,
+                              _i32p]),
 }
 
 
@@ -797,3 +801,34 @@ def score_list(pts, hyp, origin, cell: float, dims, df, tau_cm: float):
                          _p(np.ascontiguousarray(dims, np.int32), _i32p), _p(np.ascontiguousarray(df, np.uint8), _u8p),
                          float(tau_cm), _p(out, _f32p))
     return out
+
+
+EVAL_RADII = (0.03, 0.05, 0.10)  # localize.py:46
+
+
+def plane_icp(pts, Ts, grid: dict, nn, max_d2: float, c2: float, alpha: float = 0.05):
+    """localize.py:126-180 (k_icp_batch, k_reduce_batch): (k, 64) accumulator rows for the poses Ts (k, 4, 4).
+    grid: a prepared prior map (origin, cell, dims, start, xyz, normal, planarity)."""
+    pts = _f32(pts).reshape(-1, 3)
+    T = _f32(Ts).reshape(-1, 16)
+    out = np.zeros((len(T), 64))
+    lib().lwo_plane_icp(_p(pts, _f32p), len(pts), _p(T, _f32p), len(T), _p(_geo(grid["origin"], grid["cell"]), _f32p),
+                        _p(np.ascontiguousarray(grid["dims"], np.int32), _i32p),
+                        _p(np.ascontiguousarray(grid["start"], np.int32), _i32p),
+                        _p(np.ascontiguousarray(nn, np.int32), _i32p), _p(_f32(grid["xyz"]), _f32p),
+                        _p(_f32(grid["normal"]), _f32p), _p(_f32(grid["planarity"]), _f32p), float(max_d2), float(c2),
+                        float(alpha), _p(out, _f64p))
+    return out
+
+
+def plane_eval(pts, Ts, grid: dict, nn, radii=EVAL_RADII):
+    """localize.py:183-206 (k_eval_batch): (k, 3) counts of points within each radius of the map."""
+    pts = _f32(pts).reshape(-1, 3)
+    T = _f32(Ts).reshape(-1, 16)
+    counts = np.zeros((len(T), 3), np.int32)
+    lib().lwo_plane_eval(_p(pts, _f32p), len(pts), _p(T, _f32p), len(T), _p(_geo(grid["origin"], grid["cell"]), _f32p),
+                         _p(np.ascontiguousarray(grid["dims"], np.int32), _i32p),
+                         _p(np.ascontiguousarray(grid["start"], np.int32), _i32p),
+                         _p(np.ascontiguousarray(nn, np.int32), _i32p), _p(_f32(grid["xyz"]), _f32p),
+                         _p(np.array(radii, np.float32), _f32p), _p(counts, _i32p))
+    return counts

@@ -140,3 +140,49 @@ def test_edge_cases(lw, small):
     out = torch.full((4,), 7.0, device="cuda")
     grid.distances(torch.zeros((4, 3), device="cuda"), 0, np.eye(4), out)
     assert torch.all(out == 7.0)
+
+
+@pytest.fixture(scope="module")
+def room_grid(lw, gpu_ok):
+    S = lw.synth.box_room(1)
+    xyz, nrm = lw.synth.sample_surfaces(S, 0.04, 3, normals=True)
+    prep = lw.mapgrid.PriorGrid.prepare({"xyz": xyz, "normal": nrm})
+    grid = lw.PriorGrid(prep, "cuda:0")
+    grid.build()
+    return S, prep, grid
+
+
+def test_plane_icp_and_eval_match_oracle(lw, room_grid):
+    """One accumulation for 16 poses (lw_plane_icp) against the oracle's serial sums: counts exact, every
+    float64 sum within 1e-9 of the row's scale (the float32 terms are the same; only the order of the float64
+    additions differs); the fit counts (lw_plane_eval) exact."""
+    S, prep, grid = room_grid
+    cs = lw.CoarseSearch(grid, "cuda:0")
+    T_true, pts = G._room_scan(lw, S, noise=0.02)
+    rng = np.random.default_rng(17)
+    Ts = [G._perturbed(lw, T_true, rng.normal(0, 0.05, 3), rng.normal(0, 2.0, 3)) for _ in range(15)] + [T_true]
+    n = cs.upload_scan(pts)
+    nn = grid.nn.cpu().numpy()
+    for dmax in (0.40, 0.05):
+        got = cs.accumulate(Ts, n, dmax)
+        want = O.plane_icp(pts, Ts, prep, nn, dmax * dmax, (dmax * 0.5) ** 2)
+        assert np.array_equal(got[:, 27], want[:, 27]) and np.all(want[:, 27] > 0)
+        scale = np.abs(want).max(axis=1, keepdims=True)
+        assert np.all(np.abs(got - want) <= 1e-9 * scale)
+    fits = cs.evaluate(Ts, n)
+    want = O.plane_eval(pts, Ts, prep, nn) / n
+    assert np.array_equal(fits, want)
+
+
+def test_refine_recovers_the_pose(lw, room_grid):
+    """refine() (localize.py:567-612) from 6 cm and 3 degrees off, with an offset start beside it: the true pose
+    within 1 cm and 0.2 degrees, most of the scan within 3 cm of the map."""
+    S, prep, grid = room_grid
+    cs = lw.CoarseSearch(grid, "cuda:0")
+    T_true, pts = G._room_scan(lw, S, n=8000, noise=0.02)
+    T0 = G._perturbed(lw, T_true)
+    T, fit, info = cs.refine(pts, T0, starts=[(0.5, 0.0, 0.0, 10.0)])
+    d = np.linalg.inv(T_true) @ T
+    ang = np.degrees(np.arccos(np.clip((np.trace(d[:3, :3]) - 1) / 2, -1, 1)))
+    print(f"refined: {np.linalg.norm(d[:3, 3]) * 100:.2f} cm, {ang:.3f} deg, fit {fit.round(3)}, {info}")
+    assert np.linalg.norm(d[:3, 3]) < 0.01 and ang < 0.2 and fit[0] > 0.6 and not info["switched"]

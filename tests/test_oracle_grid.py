@@ -170,3 +170,62 @@ def test_local_maxima_and_nms(lw):
     scores = np.array([3.0, 2.0, 1.0, 1.5, 2.5])
     assert CS.nms(hyps, scores, 16, 0.1, 1.5).tolist() == [0, 3, 2]
     assert CS.nms(hyps, scores, 2, 0.1, 1.5).tolist() == [0, 3]
+
+
+@pytest.fixture(scope="module")
+def room(lw):
+    """The 8 x 6 x 3 m box room sampled every 4 cm with its face normals, prepared and built by the oracle."""
+    S = lw.synth.box_room(1)
+    xyz, nrm = lw.synth.sample_surfaces(S, 0.04, 3, normals=True)
+    prep = lw.mapgrid.PriorGrid.prepare({"xyz": xyz, "normal": nrm})
+    nn, _ = O.grid_build(prep["xyz"], prep["origin"], prep["cell"], prep["dims"])
+    return S, prep, nn
+
+
+def _room_scan(lw, S, n=4000, noise=0.0):
+    T = lw.synth.pose_matrix(np.array([2.5, 3.0, 1.4]), 0.3, 0.02, -0.01)
+    xyz, _, _ = lw.synth.scan(S, lw.synth.Rosette(5), lambda t: T, 0.0, n, 9, noise=noise)
+    return T, xyz.astype(np.float32)
+
+
+def _perturbed(lw, T, dp=(0.06, -0.05, 0.03), deg=(2.0, -1.0, 1.5)):
+    d = np.r_[dp, np.radians(deg)]
+    return lw.localize.twist_apply(T, d)
+
+
+def test_plane_icp_drives_to_the_true_pose(lw, room):
+    """The oracle's accumulations, solved as the reference's host loop does (localize.py:531-553), move a pose
+    8 cm and 3 degrees off back onto a noise-free scan's true pose: a wrong sign or index in the Jacobian, the
+    precision or the right-hand side would not converge there."""
+    S, prep, nn = room
+    T_true, pts = _room_scan(lw, S)
+    T = _perturbed(lw, T_true)
+    tri = np.triu_indices(6)
+    for dmax, iters in ((0.40, 8), (0.20, 8), (0.10, 8), (0.05, 10)):
+        for _ in range(iters):
+            v = O.plane_icp(pts, [T], prep, nn, dmax * dmax, (dmax * 0.5) ** 2)[0]
+            assert v[27] >= 30
+            H = np.zeros((6, 6))
+            H[tri] = v[:21]
+            H += np.triu(H, 1).T
+            assert np.all(np.linalg.eigvalsh(H) > -1e-9)  # a sum of weighted J^T M J: positive semidefinite
+            delta = np.linalg.solve(H + np.diag(1e-3 * np.diag(H) + 1e-9), -v[21:27])
+            T = lw.localize.twist_apply(T, delta)
+            if max(np.linalg.norm(delta[:3]), 5 * np.linalg.norm(delta[3:])) < 2e-4:
+                break
+    d = np.linalg.inv(T_true) @ T
+    assert np.linalg.norm(d[:3, 3]) < 3e-3
+    assert np.degrees(np.arccos(np.clip((np.trace(d[:3, :3]) - 1) / 2, -1, 1))) < 0.05
+
+
+def test_plane_eval_counts_match_kdtree(lw, room):
+    """k_eval_batch's counts: within 3 and 5 cm the nearest point is exact (one cell), so the counts equal a k-d
+    tree's up to float32 rounding at the radius; within 10 cm the seeds may be a little farther, never nearer."""
+    S, prep, nn = room
+    T_true, pts = _room_scan(lw, S, noise=0.03)
+    c = O.plane_eval(pts, [T_true], prep, nn)[0]
+    w = pts.astype(np.float64) @ T_true[:3, :3].T + T_true[:3, 3]
+    d, _ = cKDTree(prep["xyz"].astype(np.float64)).query(w)
+    want = [int((d < r).sum()) for r in O.EVAL_RADII]
+    assert abs(int(c[0]) - want[0]) <= 2 and abs(int(c[1]) - want[1]) <= 2
+    assert want[2] * 0.97 <= c[2] <= want[2] + 2
