@@ -229,6 +229,13 @@ def global_search(torch, lw, dev, W, loc):
         fits = cs.evaluate(Tr, n)
         t_ref += time.perf_counter() - t1
         ref_best.append(_pose_err(W.T_true[q], Tr[int(np.argmax(fits[:, 0]))]))
+    # and the whole flow without the known height: heights from the scan's horizontal surfaces (localize())
+    t1 = time.perf_counter()
+    full = []
+    for q, scan in enumerate(W.scans):
+        Tl, info = cs.localize(scan)
+        full.append((_pose_err(W.T_true[q], Tl), info["unique"]))
+    t_full = (time.perf_counter() - t1) / len(W.scans)
     hyps = np.stack(cands)
     loc.stream.synchronize()
     t1 = time.perf_counter()
@@ -242,6 +249,10 @@ def global_search(torch, lw, dev, W, loc):
             "coarse_first_error": [[round(e[0], 3), round(e[1], 2)] for e in first],
             "top1_recovered": sum(1 for e in best if e[0] < 0.3 and e[1] < 3.0), "queries": len(W.scans),
             "top1_error": [[round(e[0], 3), round(e[1], 2)] for e in best],
+            "localize": {"s_per_query": round(t_full, 3),
+                         "top1_recovered": sum(1 for e, _ in full if e[0] < 0.3 and e[1] < 3.0),
+                         "unique": sum(1 for _, u in full if u),
+                         "top1_error": [[round(e[0], 3), round(e[1], 2)] for e, _ in full]},
             "plane_icp": {"s_per_query": round(t_ref / len(W.scans), 4),
                           "top1_recovered": sum(1 for e in ref_best if e[0] < 0.3 and e[1] < 3.0),
                           "top1_error": [[round(e[0], 3), round(e[1], 2)] for e in ref_best]}}

@@ -8,8 +8,9 @@ tolerance and more points (lw_score_list), and greedy non-maximum suppression ke
 refines (loc.BatchLocalizer, or the reference's own). The host parts (grid layout, local maxima, the pool of
 best scores, suppression) follow the reference step for step, so the same scores pick the same candidates.
 
-Not here: the scan preparation that finds "up" and the sensor heights (ScanPrep, localize.py:243-337, built
-on open3d's normals and RANSAC planes): the caller passes gravity-aligned points and the heights to try.
+ScanPrep here is the part of the reference's (localize.py:243-337) that the heights need: the 5 cm voxel
+means, per-point normals from the 30 nearest neighbours within 0.2 m, and the height profile of the
+horizontal points. The reference also finds "up" from RANSAC planes (open3d); here the caller gives it.
 """
 
 from __future__ import annotations
@@ -43,6 +44,61 @@ def twist_apply(T: np.ndarray, delta: np.ndarray) -> np.ndarray:
     out[:3, :3] = Rz @ T[:3, :3]
     out[:3, 3] = Rz @ T[:3, 3] + delta[:3]
     return out
+
+
+def rot_to(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """The rotation taking unit vector a onto unit vector b (localize.py:211-219)."""
+    a = np.asarray(a, float) / np.linalg.norm(a)
+    b = np.asarray(b, float) / np.linalg.norm(b)
+    v = np.cross(a, b)
+    c = float(a @ b)
+    if np.linalg.norm(v) < 1e-9:
+        return np.eye(3) if c > 0 else np.diag([1.0, -1.0, -1.0])  # a half turn about x
+    K = np.array([[0, -v[2], v[1]], [v[2], 0, -v[0]], [-v[1], v[0], 0]])
+    return np.eye(3) + K + K @ K / (1.0 + c)
+
+
+def pose_difference(A: np.ndarray, B: np.ndarray):
+    """(translation m, rotation deg) between two 4x4 poses (localize.py:236-240)."""
+    d = np.linalg.inv(A) @ B
+    ang = math.degrees(math.acos(np.clip((np.trace(d[:3, :3]) - 1) / 2, -1, 1)))
+    return float(np.linalg.norm(A[:3, 3] - B[:3, 3])), ang
+
+
+class ScanPrep:
+    """A scan in the sensor frame prepared for the height search (localize.py:243-337, less the RANSAC planes):
+    one point per 5 cm voxel (the mean of its points, voxels counted from half a voxel below the scan's
+    minimum as open3d's voxel_down_sample does) and each point's normal, the least-spread direction of its 30
+    nearest neighbours within 0.2 m (sign arbitrary)."""
+
+    def __init__(self, xyz: np.ndarray, voxel: float = 0.05, radius: float = 0.2, max_nn: int = 30):
+        from scipy.spatial import cKDTree
+
+        p = np.asarray(xyz, np.float64)
+        key = np.floor((p - (p.min(axis=0) - 0.5 * voxel)) / voxel).astype(np.int64)
+        _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
+        inv = inv.reshape(-1)
+        sums = np.zeros((len(cnt), 3))
+        np.add.at(sums, inv, p)
+        self.pts = sums / cnt[:, None]
+        d, nb = cKDTree(self.pts).query(self.pts, k=min(max_nn, len(self.pts)), distance_upper_bound=radius)
+        d, nb = np.atleast_2d(d), np.atleast_2d(nb)
+        ok = np.isfinite(d)
+        q = self.pts[np.where(ok, nb, 0)] * ok[..., None]
+        cnt = np.maximum(ok.sum(axis=1), 1)[:, None]
+        mean = q.sum(axis=1) / cnt
+        c = (q - mean[:, None, :]) * ok[..., None]
+        cov = np.einsum("nki,nkj->nij", c, c)
+        _, V = np.linalg.eigh(cov)
+        self.normals = V[:, :, 0] * (ok.sum(axis=1) >= 3)[:, None]  # fewer than 3 neighbours: no normal
+
+    def height_profile(self, up: np.ndarray, lo: float, n_bins: int, bin_m: float):
+        """Histogram of heights (along up) of the scan's horizontal points, relative to the sensor
+        (localize.py:331-336)."""
+        horiz = np.abs(self.normals @ up) > 0.97
+        z = self.pts[horiz] @ up
+        h, _ = np.histogram(z, bins=n_bins, range=(lo, lo + n_bins * bin_m))
+        return h.astype(np.float32)
 
 
 def rotz(a: float) -> np.ndarray:
@@ -83,6 +139,12 @@ class CoarseSearch:
         self.n_mid_pts = 2048
         self.keep_coarse = 128  # coarse local maxima re-scored on the finer grid
         self.n_icp = 16  # medium winners handed to the ICP
+        self.n_heights = 8  # sensor heights tried
+        # unique: inliers (3 cm) at least accept_inliers, and the best answer elsewhere at least accept_gap lower
+        # and below accept_ratio of it (localize.py:390-394)
+        self.accept_inliers = 0.45
+        self.accept_gap = 0.15
+        self.accept_ratio = 0.75
         self.last_kernel_ms = 0.0
 
     def _stream(self, stream):
@@ -362,3 +424,82 @@ class CoarseSearch:
                 best = i
         return Ts[best], fits[best], {"switched": best != 0, "plain_fit": float(fits[0, 0]), "n": n,
                                       "fit_start": float(fit_start[0])}
+
+    # ---- the heights and the whole search for one gravity candidate (localize.py:426-446, 603-720) ---------
+
+    def heights(self, prep: ScanPrep, up: np.ndarray):
+        """Sensor heights where the scan's horizontal surfaces line up with the map's, best first
+        (localize.py:426-446): the square roots of both height histograms, smoothed over +-2 bins, correlated;
+        the best n_heights shifts at least 8 cm apart that put the sensor within the map's height +-0.5 m."""
+        h_map, lo, bin_m = self.grid.levels
+        span = 6.0
+        nb = int(round(2 * span / bin_m))
+        h_scan = prep.height_profile(up, -span, nb, bin_m)
+        if h_scan.sum() < 50:
+            return None
+        k = np.array([0.25, 0.5, 1.0, 0.5, 0.25], np.float32)
+        hm = np.convolve(np.sqrt(h_map), k, mode="same")
+        hs = np.convolve(np.sqrt(h_scan), k, mode="same")
+        corr = np.correlate(hm, hs, mode="full")
+        shifts = np.arange(len(corr)) - (nb - 1)
+        z_sensor = lo + shifts * bin_m + span
+        ok = (z_sensor >= self.grid.lo[2] - 0.5) & (z_sensor <= self.grid.hi[2] + 0.5)
+        corr = np.where(ok, corr, -1.0)
+        out = []
+        for i in np.argsort(-corr):
+            if corr[i] <= 0 or len(out) >= self.n_heights:
+                break
+            z = float(z_sensor[i])
+            if all(abs(z - o) > 0.08 for o in out):
+                out.append(z)
+        return out
+
+    def _unique(self, best: float, alt: float) -> bool:
+        return best >= self.accept_inliers and best - alt >= self.accept_gap and alt <= self.accept_ratio * best
+
+    @staticmethod
+    def _runner_up(results) -> float:
+        best = results[0][3]
+        for r in results[1:]:
+            dt, ang = pose_difference(best, r[3])
+            if dt > 1.0 or ang > 15.0:
+                return r[0]
+        return 0.0
+
+    def localize(self, xyz: np.ndarray, up=(0.0, 0.0, 1.0), stream=None):
+        """Sensor-to-map pose of a sensor-frame scan for gravity `up` (sensor frame), and diagnostics: the
+        reference's localize() (localize.py:614-720) for one gravity candidate: 5 cm points (at most PTS_CAP),
+        the coarse subset (1024, one per 15 cm voxel first) and the medium subset (2048), the heights (or every
+        0.25 m of the map's height when the scan shows no horizontal surface), the coarse and medium stages,
+        the point-to-plane ICP of the candidates and the 3 cm fit; unique as the reference decides it."""
+        rng = np.random.default_rng(self.seed)
+        prep = ScanPrep(xyz)
+        pts = prep.pts
+        if len(pts) > self.PTS_CAP:
+            pts = pts[rng.choice(len(pts), self.PTS_CAP, replace=False)]
+        n_pts = len(pts)
+        if n_pts < 200:
+            raise RuntimeError(f"scan too sparse to localise ({n_pts} points after 5 cm thinning)")
+        key = np.floor(pts / 0.15).astype(np.int64)
+        _, first = np.unique(key, axis=0, return_index=True)
+        spread = pts[first]
+        sub_c = spread[rng.choice(len(spread), min(self.n_coarse_pts, len(spread)), replace=False)]
+        sub_m = pts[rng.choice(n_pts, min(self.n_mid_pts, n_pts), replace=False)]
+        up = np.asarray(up, float) / np.linalg.norm(up)
+        Rg = rot_to(up, np.array([0.0, 0.0, 1.0]))
+        heights = self.heights(prep, up)
+        if not heights:
+            heights = list(np.arange(self.grid.lo[2], self.grid.hi[2] + 1e-6, self.xy_step))
+        Ts, mid, coarse = self.search(sub_c, sub_m, heights, Rg, stream)
+        if not len(Ts):
+            raise RuntimeError("no hypothesis survived (is the scan inside the map?)")
+        self.upload_scan(pts, stream)
+        Tr = self.icp(list(Ts), n_pts, stream=stream)
+        fr = self.evaluate(Tr, n_pts, stream)
+        results = sorted([(float(fr[j, 0]), float(fr[j, 1]), float(fr[j, 2]), Tr[j], float(mid[j]))
+                          for j in range(len(Tr))], key=lambda r: -r[0])
+        top = results[0]
+        alt = self._runner_up(results)
+        return top[3], {"inliers": top[0], "inliers5": top[1], "inliers10": top[2], "runner_up": alt,
+                        "unique": self._unique(top[0], alt), "heights": heights, "scan_points": n_pts,
+                        "hypotheses": int(len(heights) * 360 / self.yaw_step_deg) if heights else 0}

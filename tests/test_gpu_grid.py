@@ -186,3 +186,24 @@ def test_refine_recovers_the_pose(lw, room_grid):
     ang = np.degrees(np.arccos(np.clip((np.trace(d[:3, :3]) - 1) / 2, -1, 1)))
     print(f"refined: {np.linalg.norm(d[:3, 3]) * 100:.2f} cm, {ang:.3f} deg, fit {fit.round(3)}, {info}")
     assert np.linalg.norm(d[:3, 3]) < 0.01 and ang < 0.2 and fit[0] > 0.6 and not info["switched"]
+
+
+def test_localize_without_heights(lw, gpu_ok):
+    """The reference's localize() flow for a level sensor (gravity given): the heights from the scan's floor
+    and ceiling against the map's levels, then coarse, medium, the grid ICP and the 3 cm fit, on scans from
+    three rooms of the multi-room floor (map with face normals)."""
+    S = lw.synth.multi_room_floor(31)
+    mp, mn = lw.synth.sample_surfaces(S, 0.04, 32, prune=False, normals=True)
+    grid = lw.PriorGrid({"xyz": mp, "normal": mn}, "cuda:0")
+    grid.build()
+    cs = lw.CoarseSearch(grid, "cuda:0")
+    ok = 0
+    for q, (x, y, yaw) in enumerate(((6.3, 5.1, 0.7), (30.2, 20.4, 2.9), (53.5, 33.0, -1.8))):
+        T = lw.synth.pose_matrix(np.array([x, y, 1.4]), yaw, 0.0, 0.0)
+        xyz, _, _ = lw.synth.scan(S, lw.synth.Rosette(40 + q), lambda t, T=T: T, 0.0, 20000, 50 + q)
+        Tb, info = cs.localize(xyz)
+        dt, ang = lw.localize.pose_difference(T, Tb)
+        print(f"query {q}: {dt * 100:.1f} cm, {ang:.2f} deg, inliers {info['inliers']:.3f}, runner-up "
+              f"{info['runner_up']:.3f}, unique {info['unique']}, heights {[round(h, 2) for h in info['heights'][:3]]}")
+        ok += dt < 0.05 and ang < 1.0
+    assert ok == 3

@@ -229,3 +229,28 @@ def test_plane_eval_counts_match_kdtree(lw, room):
     want = [int((d < r).sum()) for r in O.EVAL_RADII]
     assert abs(int(c[0]) - want[0]) <= 2 and abs(int(c[1]) - want[1]) <= 2
     assert want[2] * 0.97 <= c[2] <= want[2] + 2
+
+
+def test_scan_prep_and_heights(lw, room):
+    """The scan's floor and ceiling points get vertical normals, and the height search (localize.py:426-446)
+    puts the sensor at its true 1.4 m above the floor among its first candidates, from the map's levels."""
+    S, prep, nn = room
+    lvl = lw.mapgrid.horizontal_levels(prep["xyz"], prep["normal"], prep["planarity"])
+    assert lvl[0].sum() > 0
+    T, pts = _room_scan(lw, S, n=20000, noise=0.01)
+    sp = lw.localize.ScanPrep(pts)
+    assert 0 < len(sp.pts) < len(pts) and np.allclose(np.linalg.norm(sp.normals[np.any(sp.normals, 1)], axis=1), 1)
+    up = T[:3, :3].T @ np.array([0.0, 0.0, 1.0])  # gravity in the sensor frame
+    horiz = np.abs(sp.normals @ up) > 0.97
+    z = sp.pts[horiz] @ up
+    # the floor 1.4 m below the sensor, the ceiling 1.6 m above
+    assert np.sum(np.abs(z + 1.4) < 0.05) > 100 and np.sum(np.abs(z - 1.6) < 0.05) > 100
+
+    class _Search:  # the parts of CoarseSearch the heights read
+        grid = type("G", (), {"levels": lvl, "lo": prep["lo"], "hi": prep["hi"]})
+        n_heights = 8
+
+    hs = lw.localize.CoarseSearch.heights(_Search(), sp, up)
+    print("heights", hs, "true", T[2, 3])
+    assert hs and any(abs(h - T[2, 3]) < 0.04 for h in hs[:3])
+    assert np.allclose(lw.localize.rot_to(up, np.array([0, 0, 1.0])) @ up, [0, 0, 1])
