@@ -229,7 +229,8 @@ def global_search(torch, lw, dev, W, loc):
         fits = cs.evaluate(Tr, n)
         t_ref += time.perf_counter() - t1
         ref_best.append(_pose_err(W.T_true[q], Tr[int(np.argmax(fits[:, 0]))]))
-    # and the whole flow without the known height: heights from the scan's horizontal surfaces (localize())
+    # and the whole flow with nothing known: gravity from the scan's planes, heights from its horizontal
+    # surfaces (localize())
     t1 = time.perf_counter()
     full = []
     for q, scan in enumerate(W.scans):

@@ -71,7 +71,8 @@ class ScanPrep:
     minimum as open3d's voxel_down_sample does) and each point's normal, the least-spread direction of its 30
     nearest neighbours within 0.2 m (sign arbitrary)."""
 
-    def __init__(self, xyz: np.ndarray, voxel: float = 0.05, radius: float = 0.2, max_nn: int = 30):
+    def __init__(self, xyz: np.ndarray, voxel: float = 0.05, radius: float = 0.2, max_nn: int = 30,
+                 max_tilt_deg: float = 45.0, seed: int = 0):
         from scipy.spatial import cKDTree
 
         p = np.asarray(xyz, np.float64)
@@ -91,6 +92,65 @@ class ScanPrep:
         cov = np.einsum("nki,nkj->nij", c, c)
         _, V = np.linalg.eigh(cov)
         self.normals = V[:, :, 0] * (ok.sum(axis=1) >= 3)[:, None]  # fewer than 3 neighbours: no normal
+        # up to 8 planes, each the best of 1000 three-point samples (inliers within 3 cm) of the points no
+        # earlier plane took, while one holds at least 150 points (localize.py:259-273)
+        rng = np.random.default_rng(seed)
+        self.planes = []  # (inliers, unit normal, offset, inlier indices)
+        idx = np.arange(len(self.pts))
+        for _ in range(8):
+            if len(idx) < 300:
+                break
+            P = self.pts[idx]
+            tri = P[rng.integers(0, len(P), (1000, 3))]
+            nrm = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+            ln = np.linalg.norm(nrm, axis=1)
+            good = ln > 1e-12
+            nrm = nrm[good] / ln[good, None]
+            off = -np.einsum("ij,ij->i", nrm, tri[good, 0])
+            cnt = np.zeros(len(nrm), np.int64)
+            for a in range(0, len(nrm), 64):
+                cnt[a:a + 64] = (np.abs(P @ nrm[a:a + 64].T + off[a:a + 64]) < 0.03).sum(axis=0)
+            if not len(cnt):
+                break
+            b = int(np.argmax(cnt))
+            inl = np.flatnonzero(np.abs(P @ nrm[b] + off[b]) < 0.03)
+            if len(inl) < 150:
+                break
+            self.planes.append((len(inl), nrm[b], float(off[b]), idx[inl]))
+            idx = np.delete(idx, inl)
+        self.max_tilt = max_tilt_deg
+
+    def up_candidates(self):
+        """[(up unit vector, evidence)] ranked by plane support, both signs of each horizontal family: one normal
+        fitted to all planes of a family together, the least-spread direction of the sum of their scatter
+        matrices (localize.py:276-310); a level sensor either way up when no plane is horizontal enough."""
+        cos_tilt = math.cos(math.radians(self.max_tilt))
+        fams = []  # [support, [plane indices], seed normal]
+        for i, (cnt, n, _, _) in enumerate(self.planes):
+            if abs(n[2]) < cos_tilt:
+                continue  # a wall, for a sensor that is roughly level (or upside down)
+            for f in fams:
+                if abs(float(n @ f[2])) > math.cos(math.radians(10.0)):
+                    f[0] += cnt
+                    f[1].append(i)
+                    break
+            else:
+                fams.append([cnt, [i], n])
+        out = []
+        for support, members, n0 in sorted(fams, key=lambda f: -f[0])[:2]:
+            S = np.zeros((3, 3))
+            for i in members:
+                P = self.pts[self.planes[i][3]]
+                P = P - P.mean(0)
+                S += P.T @ P
+            _, V = np.linalg.eigh(S)
+            n = V[:, 0]
+            if n @ n0 < 0:
+                n = -n
+            out += [(n, support), (-n, support)]
+        if not out:
+            out = [(np.array([0.0, 0.0, 1.0]), 0), (np.array([0.0, 0.0, -1.0]), 0)]
+        return out
 
     def height_profile(self, up: np.ndarray, lo: float, n_bins: int, bin_m: float):
         """Histogram of heights (along up) of the scan's horizontal points, relative to the sensor
@@ -286,6 +346,18 @@ class CoarseSearch:
         ms = self.score_list(pts_m, hyps, self.tau_mid, stream)
         return hyps, ms, self.nms(hyps, ms, self.n_icp, 0.1, 1.5)
 
+    def coarse_candidate(self, pts_c: np.ndarray, heights, Rg: np.ndarray, stream=None) -> dict:
+        """The coarse stage for one gravity candidate (localize.py:633-647): its kept local maxima as (x, y, z,
+        yaw) hypotheses, their scores, and the best score per point."""
+        pc = np.asarray(pts_c, float) @ Rg.T
+        pos, shape, dyaw, sc = self.coarse(pc, heights, stream)
+        n_yaw = shape[3]
+        best = self.local_maxima(sc, shape, self.keep_coarse)
+        hyps = np.concatenate([pos[best // n_yaw], ((best % n_yaw) * dyaw)[:, None]], 1) if len(best) else \
+            np.zeros((0, 4))
+        return {"Rg": Rg, "heights": heights, "coarse": hyps, "coarse_score": sc[best],
+                "best_coarse": float(sc[best[0]]) / max(len(pc), 1) if len(best) else 0.0, "hypotheses": len(sc)}
+
     def search(self, pts_c: np.ndarray, pts_m: np.ndarray, heights, Rg: np.ndarray | None = None, stream=None):
         """Coarse, local maxima and medium stages for one gravity candidate: pts_c / pts_m are the coarse and
         medium subsets of the scan in the sensor frame, Rg rotates the sensor frame to the gravity-aligned one
@@ -466,14 +538,15 @@ class CoarseSearch:
                 return r[0]
         return 0.0
 
-    def localize(self, xyz: np.ndarray, up=(0.0, 0.0, 1.0), stream=None):
-        """Sensor-to-map pose of a sensor-frame scan for gravity `up` (sensor frame), and diagnostics: the
-        reference's localize() (localize.py:614-720) for one gravity candidate: 5 cm points (at most PTS_CAP),
-        the coarse subset (1024, one per 15 cm voxel first) and the medium subset (2048), the heights (or every
-        0.25 m of the map's height when the scan shows no horizontal surface), the coarse and medium stages,
-        the point-to-plane ICP of the candidates and the 3 cm fit; unique as the reference decides it."""
+    def localize(self, xyz: np.ndarray, up=None, up_prior=None, stream=None):
+        """Sensor-to-map pose of a sensor-frame scan, and diagnostics: the reference's localize()
+        (localize.py:614-720). 5 cm points (at most PTS_CAP), the coarse subset (1024, one per 15 cm voxel first)
+        and the medium subset (2048); gravity candidates from the scan's planes (or `up`, sensor frame, alone),
+        ordered by up_prior if given; for each, the heights (or every 0.25 m of the map's height when the scan
+        shows no horizontal surface) and the coarse stage; then, best coarse score first, the medium stage, the
+        grid ICP of its candidates and the 3 cm fit, until one candidate's best answer is unique."""
         rng = np.random.default_rng(self.seed)
-        prep = ScanPrep(xyz)
+        prep = ScanPrep(xyz, seed=self.seed)
         pts = prep.pts
         if len(pts) > self.PTS_CAP:
             pts = pts[rng.choice(len(pts), self.PTS_CAP, replace=False)]
@@ -485,21 +558,55 @@ class CoarseSearch:
         spread = pts[first]
         sub_c = spread[rng.choice(len(spread), min(self.n_coarse_pts, len(spread)), replace=False)]
         sub_m = pts[rng.choice(n_pts, min(self.n_mid_pts, n_pts), replace=False)]
-        up = np.asarray(up, float) / np.linalg.norm(up)
-        Rg = rot_to(up, np.array([0.0, 0.0, 1.0]))
-        heights = self.heights(prep, up)
-        if not heights:
-            heights = list(np.arange(self.grid.lo[2], self.grid.hi[2] + 1e-6, self.xy_step))
-        Ts, mid, coarse = self.search(sub_c, sub_m, heights, Rg, stream)
-        if not len(Ts):
+        if up is not None:
+            ups = [(np.asarray(up, float) / np.linalg.norm(up), 0)]
+        else:
+            ups = prep.up_candidates()
+            if up_prior is not None:
+                upp = np.asarray(up_prior, float) / np.linalg.norm(up_prior)
+                ups.sort(key=lambda u: -float(u[0] @ upp))
+        cands = []
+        for u, support in ups:
+            Rg = rot_to(u, np.array([0.0, 0.0, 1.0]))  # sensor -> gravity-aligned sensor frame
+            heights = self.heights(prep, u)
+            if not heights:
+                heights = list(np.arange(self.grid.lo[2], self.grid.hi[2] + 1e-6, self.xy_step))
+            c = self.coarse_candidate(sub_c, heights, Rg, stream)
+            c.update(up=u, support=support)
+            cands.append(c)
+        cands.sort(key=lambda c: -c["best_coarse"])
+        results, refined = [], 0
+        for ci, c in enumerate(cands):
+            if not len(c["coarse"]):
+                continue
+            Rg = c["Rg"]
+            hyps, ms, win = self.medium(c["coarse"], np.asarray(sub_m, float) @ Rg.T, stream)
+            Ts = []
+            for i in win:
+                x, y, z, yaw = hyps[i]
+                T = np.eye(4)
+                T[:3, :3] = rotz(yaw) @ Rg
+                T[:3, 3] = (x, y, z)
+                Ts.append(T)
+            if not Ts:
+                continue
+            self.upload_scan(pts, stream)
+            Tr = self.icp(Ts, n_pts, stream=stream)
+            fr = self.evaluate(Tr, n_pts, stream)
+            for j, i in enumerate(win):
+                results.append((float(fr[j, 0]), float(fr[j, 1]), float(fr[j, 2]), Tr[j], ci,
+                                float(ms[i]) / max(len(sub_m), 1)))
+            refined += 1
+            results.sort(key=lambda r: -r[0])
+            if self._unique(results[0][0], self._runner_up(results)):
+                break  # this candidate's best answer is good and unique
+        if not results:
             raise RuntimeError("no hypothesis survived (is the scan inside the map?)")
-        self.upload_scan(pts, stream)
-        Tr = self.icp(list(Ts), n_pts, stream=stream)
-        fr = self.evaluate(Tr, n_pts, stream)
-        results = sorted([(float(fr[j, 0]), float(fr[j, 1]), float(fr[j, 2]), Tr[j], float(mid[j]))
-                          for j in range(len(Tr))], key=lambda r: -r[0])
+        results.sort(key=lambda r: -r[0])
         top = results[0]
         alt = self._runner_up(results)
+        c = cands[top[4]]
         return top[3], {"inliers": top[0], "inliers5": top[1], "inliers10": top[2], "runner_up": alt,
-                        "unique": self._unique(top[0], alt), "heights": heights, "scan_points": n_pts,
-                        "hypotheses": int(len(heights) * 360 / self.yaw_step_deg) if heights else 0}
+                        "unique": self._unique(top[0], alt), "up": c["up"], "heights": c["heights"],
+                        "gravity_candidates": len(cands), "refined_candidates": refined,
+                        "hypotheses": sum(c["hypotheses"] for c in cands), "scan_points": n_pts}

@@ -254,3 +254,16 @@ def test_scan_prep_and_heights(lw, room):
     print("heights", hs, "true", T[2, 3])
     assert hs and any(abs(h - T[2, 3]) < 0.04 for h in hs[:3])
     assert np.allclose(lw.localize.rot_to(up, np.array([0, 0, 1.0])) @ up, [0, 0, 1])
+
+
+def test_up_candidates_from_planes(lw, room):
+    """A scan from a sensor pitched 8 and rolled -5 degrees: the planes' horizontal family gives "up" in the
+    sensor frame within half a degree, first and then negated (the scanner may be upside down)."""
+    S, prep, nn = room
+    T = lw.synth.pose_matrix(np.array([2.5, 3.0, 1.4]), 0.3, np.radians(8.0), np.radians(-5.0))
+    xyz, _, _ = lw.synth.scan(S, lw.synth.Rosette(5), lambda t: T, 0.0, 20000, 9, noise=0.01)
+    ups = lw.localize.ScanPrep(xyz).up_candidates()
+    true_up = T[:3, :3].T @ np.array([0.0, 0.0, 1.0])
+    ang = np.degrees(np.arccos(np.clip(abs(float(ups[0][0] @ true_up)), -1, 1)))
+    print("up candidates", [(np.round(u, 3), s) for u, s in ups], "error", round(ang, 3))
+    assert ang < 0.5 and ups[0][1] > 0 and np.allclose(ups[1][0], -ups[0][0])
