@@ -79,17 +79,18 @@ class Firings:
         return xyz.contiguous(), attr.contiguous(), self.t[f].contiguous(), f
 
 
-def _framing(data: bytes):
-    """(dev_type, profile, datagram offsets, lengths): the records of the file, a truncated last one dropped
-    (the walk is native: lw_lvxr_index)."""
-    if data[:6] == b"LVXR1\n":
-        dev, profile, off = data[6], PROFILE_STANDARD, 7
-    elif data[:6] == b"LVXR2\n":
-        dev, profile, off = data[6], data[7], 8
+def _framing(data):
+    """(dev_type, profile, datagram offsets, lengths): the records of the file (bytes or a uint8 array), a
+    truncated last one dropped (the walk is native: lw_lvxr_index)."""
+    buf = np.ascontiguousarray(np.frombuffer(data, np.uint8) if isinstance(data, (bytes, bytearray)) else data)
+    head = buf[:8].tobytes()
+    if head[:6] == b"LVXR1\n" and len(head) >= 7:
+        dev, profile, off = head[6], PROFILE_STANDARD, 7
+    elif head[:6] == b"LVXR2\n" and len(head) >= 8:
+        dev, profile, off = head[6], head[7], 8
     else:
         raise ValueError("not an LVXR recording")
     L = _abi.lib()
-    buf = np.frombuffer(data, np.uint8)
     count = ctypes.c_int64(0)
     _abi.check(L.lw_lvxr_index(buf.ctypes.data, len(buf), off, None, None, 0, ctypes.byref(count)), "lw_lvxr_index")
     offs = np.empty(count.value, np.int64)
@@ -131,9 +132,8 @@ def read_firings(path: str | os.PathLike, device=None, stream=None, timing: dict
     dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
     t0 = time.perf_counter()
     with open(path, "rb") as f:
-        data = f.read()
-    dev_type, profile, offs, lens = _framing(data)
-    buf = np.frombuffer(data, np.uint8)
+        buf = np.fromfile(f, np.uint8)  # writable: it goes to the device without another copy
+    dev_type, profile, offs, lens = _framing(buf)
     interval = 4_167 if dev_type in FAST_DEVICES else 10_000
     dual = profile == PROFILE_MID40_DUAL
     ok = lens >= HEADER_LEN
@@ -159,7 +159,7 @@ def read_firings(path: str | os.PathLike, device=None, stream=None, timing: dict
     first = np.zeros(len(offs), np.int64)
     first[order] = np.concatenate([[0], np.cumsum(fpp[order])[:-1]])
     n = int(fpp.sum())
-    bytes_dev = torch.from_numpy(buf.copy()).to(dev)
+    bytes_dev = torch.from_numpy(buf).to(dev)
     tab = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in
            (offs, dtypes.astype(np.int32), fpp.astype(np.int32), t_pk, first)]
     t = torch.empty(n, dtype=torch.float64, device=dev)
@@ -181,7 +181,7 @@ def read_firings(path: str | os.PathLike, device=None, stream=None, timing: dict
         ev[1].record(s)
         ev[1].synchronize()
         timing.update(decode_ms=ev[0].elapsed_time(ev[1]), firings=n, returns=R, packets=len(offs),
-                      record_bytes=int((lens - HEADER_LEN).sum()), file_bytes=len(data))
+                      record_bytes=int((lens - HEADER_LEN).sum()), file_bytes=len(buf))
     with torch.cuda.stream(s):
         idx = torch.sort(t, stable=True).indices
         out = Firings(t=t[idx], xyz=xyz[idx], refl=refl[idx], tag=tag[idx], valid=valid[idx].bool(),
