@@ -346,12 +346,13 @@ int lw_grid_build(const float *xyz, int32_t n, const float *geo, const int32_t *
 int lw_grid_distances(const float *pts, int32_t n, const float *T, const float *geo, const int32_t *dims,
                       const int32_t *start, const int32_t *nn, const float *xyz, float *out, void *stream);
 /* The soft count (sum over the n_pts points within tau_cm of the map of 1 - d^2 / tau^2, d from df) of every
- * (position, heading): out[p n_yaw + k] for position p (x, y, z; gravity-aligned, list them x-fastest for
- * speed) and heading yaw0 + k dyaw; -1 where the position is outside the grid or within free_cm of a mapped
- * surface. Replaces k_score_grid (localize.py:55-92). */
+ * (position, heading): out[r n_yaw + k] for position p (x, y, z; gravity-aligned, list them x-fastest for
+ * speed), r = out_row[p] (device int32 per position, or NULL for r = p), heading yaw0 + k dyaw; -1 where the
+ * position is outside the grid or within free_cm of a mapped surface. Replaces k_score_grid
+ * (localize.py:55-92). */
 int lw_score_grid(const float *pts, int32_t n_pts, const float *pos, int32_t n_pos, int32_t n_yaw, float yaw0,
                   float dyaw, const float *geo, const int32_t *dims, const uint8_t *df, float tau_cm, float free_cm,
-                  float *out, void *stream);
+                  const int32_t *out_row, float *out, void *stream);
 /* The same soft count for a list of (x, y, z, yaw) hypotheses (hyp 16-byte aligned). Replaces k_score_list
  * (localize.py:95-123). */
 int lw_score_list(const float *pts, int32_t n_pts, const float *hyp, int32_t n_hyp, const float *geo,

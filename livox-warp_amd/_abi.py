@@ -129,7 +129,7 @@ SIGNATURES = {
                              _vp]),
     "lw_plane_eval": (_i32, [_vp, _i32, _vp, _i32, _f32p, _i32p, _vp, _vp, _vp, _f32p, _vp, _vp]),
     "lw_grid_distances": (_i32, [_vp, _i32, _f32p, _f32p, _i32p, _vp, _vp, _vp, _vp, _vp]),
-    "lw_score_grid": (_i32, [_vp, _i32, _vp, _i32, _i32, _f32, _f32, _f32p, _i32p, _vp, _f32, _f32, _vp, _vp]),
+    "lw_score_grid": (_i32, [_vp, _i32, _vp, _i32, _i32, _f32, _f32, _f32p, _i32p, _vp, _f32, _f32, _vp, _vp, _vp]),
     "lw_score_list": (_i32, [_vp, _i32, _vp, _i32, _f32p, _i32p, _vp, _f32, _vp, _vp]),
 }
 

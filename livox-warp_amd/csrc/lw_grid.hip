@@ -257,10 +257,12 @@ __global__ __launch_bounds__(64 * SCORE_WAVES) void k_score_grid(const float *__
                                                                  const float *__restrict__ pos, int32_t n_pos,
                                                                  int32_t n_yaw, float yaw0, float dyaw, GridGeo g,
                                                                  const uint8_t *__restrict__ df, float tau_cm,
-                                                                 float free_cm, float *__restrict__ out) {
+                                                                 float free_cm, const int32_t *__restrict__ out_row,
+                                                                 float *__restrict__ out) {
     __shared__ float sp[3 * SCORE_CHUNK];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int32_t iy0 = (blockIdx.y * SCORE_WAVES + wave) * SCORE_YPT;
+    int64_t row[SCORE_PPT];
     float cx[SCORE_PPT], cy[SCORE_PPT], cz[SCORE_PPT];
     bool act[SCORE_PPT];
     bool any = false;
@@ -269,6 +271,7 @@ __global__ __launch_bounds__(64 * SCORE_WAVES) void k_score_grid(const float *__
         const int32_t ip = (blockIdx.x * SCORE_PPT + q) * 64 + lane;
         cx[q] = cy[q] = cz[q] = 0.0f;
         act[q] = false;
+        row[q] = ip < n_pos ? (out_row ? (int64_t)out_row[ip] : (int64_t)ip) : 0;
         if (ip < n_pos && iy0 < n_yaw) {
             cx[q] = pos[3 * (int64_t)ip];
             cy[q] = pos[3 * (int64_t)ip + 1];
@@ -276,7 +279,7 @@ __global__ __launch_bounds__(64 * SCORE_WAVES) void k_score_grid(const float *__
             const int32_t cc = grid_at(g, cx[q], cy[q], cz[q]);
             act[q] = cc >= 0 && !((float)df[cc] < free_cm);
             if (!act[q])
-                for (int y = 0; y < SCORE_YPT && iy0 + y < n_yaw; ++y) out[(int64_t)ip * n_yaw + iy0 + y] = -1.0f;
+                for (int y = 0; y < SCORE_YPT && iy0 + y < n_yaw; ++y) out[row[q] * n_yaw + iy0 + y] = -1.0f;
         }
         any = any || act[q];
     }
@@ -322,11 +325,9 @@ __global__ __launch_bounds__(64 * SCORE_WAVES) void k_score_grid(const float *__
         }
     }
 #pragma unroll
-    for (int q = 0; q < SCORE_PPT; ++q) {
-        const int32_t ip = (blockIdx.x * SCORE_PPT + q) * 64 + lane;
+    for (int q = 0; q < SCORE_PPT; ++q)
         if (act[q])
-            for (int y = 0; y < SCORE_YPT && iy0 + y < n_yaw; ++y) out[(int64_t)ip * n_yaw + iy0 + y] = s[q][y];
-    }
+            for (int y = 0; y < SCORE_YPT && iy0 + y < n_yaw; ++y) out[row[q] * n_yaw + iy0 + y] = s[q][y];
 }
 
 // localize.py:95-123 k_score_list: one thread per (x, y, z, yaw) hypothesis.
@@ -599,7 +600,7 @@ extern "C" int lw_grid_distances(const float *pts, int32_t n, const float *T, co
 
 extern "C" int lw_score_grid(const float *pts, int32_t n_pts, const float *pos, int32_t n_pos, int32_t n_yaw,
                              float yaw0, float dyaw, const float *geo, const int32_t *dims, const uint8_t *df,
-                             float tau_cm, float free_cm, float *out, void *stream) {
+                             float tau_cm, float free_cm, const int32_t *out_row, float *out, void *stream) {
     GridGeo g;
     if (!lw::geo_of(geo, dims, g) || n_pts < 0 || n_pos < 0 || n_yaw <= 0 || !(tau_cm > 0.0f)) return LW_EINVAL;
     if (n_pos == 0) return 0;
@@ -608,7 +609,7 @@ extern "C" int lw_score_grid(const float *pts, int32_t n_pts, const float *pos, 
     const dim3 grid((unsigned)((n_pos + 64 * lw::SCORE_PPT - 1) / (64 * lw::SCORE_PPT)),
                     (unsigned)((groups + lw::SCORE_WAVES - 1) / lw::SCORE_WAVES));
     hipLaunchKernelGGL(lw::k_score_grid, grid, dim3(64 * lw::SCORE_WAVES), 0, (hipStream_t)stream, pts, n_pts, pos,
-                       n_pos, n_yaw, yaw0, dyaw, g, df, tau_cm, free_cm, out);
+                       n_pos, n_yaw, yaw0, dyaw, g, df, tau_cm, free_cm, out_row, out);
     return lw::launched();
 }
 

@@ -238,10 +238,14 @@ class CoarseSearch:
         dyaw = math.radians(self.yaw_step_deg)
         m = self._upload_pts(pts_g, s)
         pos_x = np.ascontiguousarray(pos.reshape(nz, nx, ny, 3).transpose(0, 2, 1, 3).reshape(-1, 3))
+        # each x-fastest position's row in the reference's (height, x, y) order, where the kernel writes it
+        rows = np.ascontiguousarray(np.arange(len(pos), dtype=np.int32).reshape(nz, nx, ny).transpose(0, 2, 1)).reshape(-1)
         per = max(1, min(self.d_pos.shape[0], self.SCORE_CAP // n_yaw))
-        out = np.empty(len(pos) * n_yaw, np.float32)
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
         ms = 0.0
+        with torch.cuda.stream(s):
+            out = torch.empty(len(pos) * n_yaw, dtype=torch.float32, device=self.device)
+            d_rows = torch.from_numpy(rows).to(self.device)
         for a in range(0, len(pos_x), per):
             b = min(len(pos_x), a + per)
             with torch.cuda.stream(s):
@@ -249,13 +253,14 @@ class CoarseSearch:
                 ev[0].record(s)
                 _abi.check(_abi.lib().lw_score_grid(self.d_pts.data_ptr(), m, self.d_pos.data_ptr(), b - a, n_yaw, 0.0,
                                                     dyaw, *g._geo_args(), g.df.data_ptr(), float(self.tau_coarse),
-                                                    float(self.free_cm), self.d_score.data_ptr(),
+                                                    float(self.free_cm), d_rows[a:b].data_ptr(), out.data_ptr(),
                                                     ctypes.c_void_p(s.cuda_stream)), "lw_score_grid")
                 ev[1].record(s)
-                out[a * n_yaw:b * n_yaw] = self.d_score[: (b - a) * n_yaw].cpu().numpy()
+            ev[1].synchronize()  # the next chunk's positions overwrite this one's
             ms += ev[0].elapsed_time(ev[1])
         self.last_kernel_ms = ms
-        sc = np.ascontiguousarray(out.reshape(nz, ny, nx, n_yaw).transpose(0, 2, 1, 3)).reshape(-1)
+        with torch.cuda.stream(s):
+            sc = out.cpu().numpy()
         return pos, (nz, nx, ny, n_yaw), dyaw, sc
 
     @staticmethod

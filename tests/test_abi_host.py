@@ -173,9 +173,9 @@ def test_grid_and_replay_entry_points_reject_bad_arguments(lw):
                                fake, None) == E, bad
     assert L.lw_grid_build(fake, 10, A.ptr(np.array([0, 0, 0, 0.0, 0.0], np.float32), A._f32p), d, st, 5, fake,
                            fake, wb, fake, None) == E  # zero cell
-    assert L.lw_score_grid(fake, 10, fake, 5, 0, 0.0, 0.1, g, d, fake, 25.0, 10.0, fake, None) == E  # no headings
-    assert L.lw_score_grid(fake, 10, fake, 5, 180, 0.0, 0.1, g, d, fake, 0.0, 10.0, fake, None) == E  # tau 0
-    assert L.lw_score_grid(fake, 10, fake, 0, 180, 0.0, 0.1, g, d, fake, 25.0, 10.0, fake, None) == 0  # nothing
+    assert L.lw_score_grid(fake, 10, fake, 5, 0, 0.0, 0.1, g, d, fake, 25.0, 10.0, None, fake, None) == E  # no headings
+    assert L.lw_score_grid(fake, 10, fake, 5, 180, 0.0, 0.1, g, d, fake, 0.0, 10.0, None, fake, None) == E  # tau 0
+    assert L.lw_score_grid(fake, 10, fake, 0, 180, 0.0, 0.1, g, d, fake, 25.0, 10.0, None, fake, None) == 0  # nothing
     assert L.lw_score_list(fake, 10, fake + 4, 5, g, d, fake, 12.0, fake, None) == E  # hypotheses misaligned
     assert L.lw_score_list(fake, 10, fake, 0, g, d, fake, 12.0, fake, None) == 0
     assert L.lw_grid_distances(fake, 0, A.ptr(np.eye(4, dtype=np.float32), A._f32p), g, d, fake, fake, fake, fake,

@@ -80,7 +80,7 @@ def test_closed_form_on_device(lw, gpu_ok):
     out = torch.empty(12, dtype=torch.float32, device="cuda")
     L, A = lw._abi.lib(), lw._abi
     A.check(L.lw_score_grid(pts.data_ptr(), 3, pos.data_ptr(), 3, 4, 0.0, math.pi / 2, A.ptr(geo, A._f32p),
-                            A.ptr(dims, A._i32p), d_df.data_ptr(), 25.0, 10.0, out.data_ptr(),
+                            A.ptr(dims, A._i32p), d_df.data_ptr(), 25.0, 10.0, None, out.data_ptr(),
                             ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "lw_score_grid")
     sc = out.cpu().numpy().reshape(3, 4)
     assert np.isclose(sc[0, 0], 1 + (1 - 25 / 625), rtol=1e-6) and np.isclose(sc[0, 1], (1 - 100 / 625) + 1, rtol=1e-6)
