@@ -615,3 +615,30 @@ class CoarseSearch:
                         "unique": self._unique(top[0], alt), "up": c["up"], "heights": c["heights"],
                         "gravity_candidates": len(cands), "refined_candidates": refined,
                         "hypotheses": sum(c["hypotheses"] for c in cands), "scan_points": n_pts}
+
+
+GlobalLocalizer = CoarseSearch  # the reference's name for the search (localize.py:339)
+
+
+class Localizer:
+    """Single-stream convenience wrapper (localize.py:765-790): builds the grid from a map's points and normals
+    and searches or refines on one stream."""
+
+    def __init__(self, map_xyz: np.ndarray, map_normal: np.ndarray, map_planarity: np.ndarray | None = None,
+                 device=None, stream=None):
+        torch = _torch()
+        host = {"xyz": map_xyz, "normal": map_normal}
+        if map_planarity is not None:
+            host["planarity"] = map_planarity
+        self.grid = PriorGrid(host, device)
+        self.loc = GlobalLocalizer(self.grid, self.grid.device)
+        self.stream = stream if stream is not None else torch.cuda.current_stream(self.grid.device)
+        self.grid.build(self.stream)
+
+    def search(self, xyz: np.ndarray, up_prior=None):
+        """GlobalLocalizer.localize on this stream: the pose of a sensor-frame scan, and diagnostics."""
+        return self.loc.localize(xyz, up_prior=up_prior, stream=self.stream)
+
+    def refine(self, xyz: np.ndarray, T0: np.ndarray, **kw):
+        """GlobalLocalizer.refine on this stream: ICP of a point set from the known pose T0."""
+        return self.loc.refine(xyz, T0, self.stream, **kw)

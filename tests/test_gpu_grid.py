@@ -195,15 +195,14 @@ def test_localize_without_heights(lw, gpu_ok):
     them from a sensor pitched 6 and rolled 4 degrees."""
     S = lw.synth.multi_room_floor(31)
     mp, mn = lw.synth.sample_surfaces(S, 0.04, 32, prune=False, normals=True)
-    grid = lw.PriorGrid({"xyz": mp, "normal": mn}, "cuda:0")
-    grid.build()
-    cs = lw.CoarseSearch(grid, "cuda:0")
+    L = lw.Localizer(mp, mn, device="cuda:0")  # the reference's wrapper: grid built on its stream
+    cs = L.loc
     ok = 0
     for q, (x, y, yaw, pitch, roll) in enumerate(((6.3, 5.1, 0.7, 0, 0), (30.2, 20.4, 2.9, 6, 4),
                                                    (53.5, 33.0, -1.8, 0, 0))):
         T = lw.synth.pose_matrix(np.array([x, y, 1.4]), yaw, np.radians(pitch), np.radians(roll))
         xyz, _, _ = lw.synth.scan(S, lw.synth.Rosette(40 + q), lambda t, T=T: T, 0.0, 20000, 50 + q)
-        Tb, info = cs.localize(xyz)
+        Tb, info = L.search(xyz)
         dt, ang = lw.localize.pose_difference(T, Tb)
         print(f"query {q}: {dt * 100:.1f} cm, {ang:.2f} deg, inliers {info['inliers']:.3f}, runner-up "
               f"{info['runner_up']:.3f}, unique {info['unique']}, heights {[round(h, 2) for h in info['heights'][:3]]}, "
