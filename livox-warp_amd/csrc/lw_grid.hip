@@ -151,6 +151,53 @@ __global__ __launch_bounds__(256) void k_grid_jfa(const Seed *__restrict__ src, 
     dst[c] = best;
 }
 
+// The same step for steps of 1 and 2 cells through LDS: a workgroup holds a 64 x 4 x 4 block of cells and its
+// halo of S cells (outside the grid: idx -1, which the step skips as it skips an empty seed), so each seed
+// record is read from memory once per block instead of once per neighbour. Same neighbour order and ties as
+// k_grid_jfa, so the same result.
+constexpr int JFA_TX = 64, JFA_TY = 4, JFA_TZ = 4;
+
+template <int S>
+__global__ __launch_bounds__(256) void k_grid_jfa_lds(const Seed *__restrict__ src, Seed *__restrict__ dst,
+                                                      GridGeo g) {
+    constexpr int LX = JFA_TX + 2 * S, LY = JFA_TY + 2 * S, LZ = JFA_TZ + 2 * S;
+    __shared__ Seed tile[LZ * LY * LX];
+    const int x0 = blockIdx.x * JFA_TX, y0 = blockIdx.y * JFA_TY, z0 = blockIdx.z * JFA_TZ;
+    for (int t = threadIdx.x; t < LX * LY * LZ; t += blockDim.x) {
+        const int lx = t % LX, ly = (t / LX) % LY, lz = t / (LX * LY);
+        const int j = grid_flat(g, x0 - S + lx, y0 - S + ly, z0 - S + lz);
+        tile[t] = j >= 0 ? src[j] : Seed{-1, 0.0f, 0.0f, 0.0f};
+    }
+    __syncthreads();
+    const int lx = threadIdx.x % JFA_TX, ly = threadIdx.x / JFA_TX;
+    const int ix = x0 + lx, iy = y0 + ly;
+    if (ix >= g.nx || iy >= g.ny) return;
+    for (int lz = 0; lz < JFA_TZ; ++lz) {
+        const int iz = z0 + lz;
+        if (iz >= g.nz) break;
+        const int32_t c = (iz * g.ny + iy) * g.nx + ix;
+        float px, py, pz;
+        cell_centre(g, ix, iy, iz, px, py, pz);
+        Seed best = tile[((lz + S) * LY + (ly + S)) * LX + (lx + S)];
+        float bd = 1.0e30f;
+        if (best.idx >= 0) bd = dist2_xyz(px, py, pz, best);
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    if (dx == 0 && dy == 0 && dz == 0) continue;
+                    const Seed sd = tile[((lz + S + dz * S) * LY + (ly + S + dy * S)) * LX + (lx + S + dx * S)];
+                    if (sd.idx >= 0 && sd.idx != best.idx) {
+                        const float d2 = dist2_xyz(px, py, pz, sd);
+                        if (d2 < bd) {
+                            bd = d2;
+                            best = sd;
+                        }
+                    }
+                }
+        dst[c] = best;
+    }
+}
+
 // mapgrid.py:158-167 with the seed indices written out: nn per cell, and the cell centre's distance to its
 // seed in cm (rounded), 255 beyond the band.
 __global__ __launch_bounds__(256) void k_grid_df(const Seed *__restrict__ rec, GridGeo g, int32_t *__restrict__ nn,
@@ -541,8 +588,15 @@ extern "C" int lw_grid_build(const float *xyz, int32_t n, const float *geo, cons
     if (n > 0) hipLaunchKernelGGL(lw::k_grid_seed, dim3(lw::blocks_for(n)), dim3(256), 0, s, xyz, n, g, nn);
     hipLaunchKernelGGL(lw::k_grid_unpack, dim3(fill_blocks), dim3(256), 0, s, nn, xyz, ncell, a);
     const dim3 cells(lw::blocks_for(g.nx), (unsigned)g.ny, (unsigned)g.nz);
+    const dim3 tiles((unsigned)((g.nx + lw::JFA_TX - 1) / lw::JFA_TX), (unsigned)((g.ny + lw::JFA_TY - 1) / lw::JFA_TY),
+                     (unsigned)((g.nz + lw::JFA_TZ - 1) / lw::JFA_TZ));
     for (int k = 0; k < n_steps; ++k) {
-        hipLaunchKernelGGL(lw::k_grid_jfa, cells, dim3(256), 0, s, a, b, g, steps[k]);
+        if (steps[k] == 1)
+            hipLaunchKernelGGL(lw::k_grid_jfa_lds<1>, tiles, dim3(256), 0, s, a, b, g);
+        else if (steps[k] == 2)
+            hipLaunchKernelGGL(lw::k_grid_jfa_lds<2>, tiles, dim3(256), 0, s, a, b, g);
+        else
+            hipLaunchKernelGGL(lw::k_grid_jfa, cells, dim3(256), 0, s, a, b, g, steps[k]);
         std::swap(a, b);
     }
     hipLaunchKernelGGL(lw::k_grid_df, cells, dim3(256), 0, s, a, g, nn, df);
