@@ -403,7 +403,6 @@ __global__ __launch_bounds__(256) void k_score_list(const float *__restrict__ pt
         stage_points(sp, pts, base, cnt);
         __syncthreads();
         if (!active) continue;
-#pragma unroll 4
         for (int32_t k = 0; k < cnt; ++k) {
             const float px = sp[3 * k], py = sp[3 * k + 1], pz = sp[3 * k + 2];
             bool ok;
