@@ -63,6 +63,9 @@ class PriorGrid:
         dims = np.ceil((hi + pad - origin) / cell).astype(np.int64)
         if int(np.prod(dims)) >= (1 << 31) - 1:
             raise ValueError(f"grid of {dims.tolist()} cells exceeds 2^31 cells")
+        if dims[0] >= (1 << 24) or dims[1] * dims[2] >= (1 << 24) or dims[1] > 65535 or dims[2] > 65535:
+            raise ValueError(f"grid of {dims.tolist()} cells: the kernels take at most 2^24 - 1 cells in x, "
+                             f"2^24 - 1 rows (y times z) and 65535 in y and in z")
         ijk = np.floor((xyz - origin) / cell).astype(np.int64)
         key = (ijk[:, 2] * dims[1] + ijk[:, 1]) * dims[0] + ijk[:, 0]
         order = np.argsort(key, kind="stable")

@@ -267,3 +267,12 @@ def test_up_candidates_from_planes(lw, room):
     ang = np.degrees(np.arccos(np.clip(abs(float(ups[0][0] @ true_up)), -1, 1)))
     print("up candidates", [(np.round(u, 3), s) for u, s in ups], "error", round(ang, 3))
     assert ang < 0.5 and ups[0][1] > 0 and np.allclose(ups[1][0], -ups[0][0])
+
+
+def test_prepare_refuses_grids_the_kernels_cannot_take(lw):
+    """A map 4 km long in y at 5 cm has more rows than the kernels index: a clear error."""
+    xyz = np.array([[0.0, 0.0, 0.0], [1.0, 4000.0, 1.0]], np.float32)
+    with pytest.raises(ValueError, match="rows"):
+        lw.mapgrid.PriorGrid.prepare({"xyz": xyz})
+    with pytest.raises(ValueError, match="points"):
+        lw.mapgrid.PriorGrid.prepare({"xyz": np.zeros((1 << 23, 3), np.float32)})
