@@ -342,7 +342,8 @@ def integration_map(args, torch, lw, dev, X, A, Tt, seq, n_frames: int, per: int
         "insert_ms_per_frame": round(ins_ms / n_frames, 4), "carve_ms_per_frame": round(carve_ms / n_frames, 4),
         "voxels": occ, "dropped": dropped, "carved_voxels": int((r["miss"][o] > 2 * r["cnt"][o]).sum()),
         "roofline": {"bound": "hbm", "achieved": round(ins_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ins_gbs / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "k_imap_insert",
+                     "frac": round(ins_gbs / HBM_PEAK_GBS, 6), "traffic": measured_traffic("k_imap_insert"),
+                     "kernel": "k_imap_insert",
                      "bytes_per_point": IMAP_IN_BYTES + IMAP_REC_BYTES},
         "cpu_baseline": {"value": round(done / cpu_s, 3), "unit": "frames/s (insert + carve)", "cores": 1,
                          "kind": "port", "sample": f"the first {done} frames through the oracle's serial map insert "
