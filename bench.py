@@ -468,6 +468,19 @@ def _write_lvxr(path, xyz, attr, t):
     return n
 
 
+def _scaled_traffic(kernel: str, units: int):
+    """HBM bytes of a launch of `units` work items, from a profiled launch of another size whose traffic is
+    linear in them (each item read and written once): the profile's bytes per item times units."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            e = json.load(f).get(kernel)
+        if e is None or not e.get("units"):
+            return None
+        return round(float(e["fetch_bytes"] + e["write_bytes"]) / e["units"] * units, 1)
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def lvxr_replay(args, torch, lw, dev, frames):
     """LVXR replay (lw.lvxr.read_firings, the reference's lvxr.read_firings): the odometry sequence's points
     written as a recording, read back with the decode on the device (kernel timed with HIP events), and the
@@ -517,7 +530,8 @@ def lvxr_replay(args, torch, lw, dev, frames):
         "host_framing_s": round(tm["host_s"], 4), "read_firings_s": round(wall, 4),
         "firings_per_s_end_to_end": round(tm["firings"] / wall, 1),
         "roofline": {"bound": "hbm", "achieved": round(gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "k_lvxr_decode",
+                     "frac": round(gbs / HBM_PEAK_GBS, 6), "traffic": _scaled_traffic("k_lvxr_decode", tm["firings"]),
+                     "kernel": "k_lvxr_decode",
                      "bytes_per_firing": 14 + LVXR_OUT_BYTES},
         "cpu_baseline": {"value": round(m / spent, 1), "unit": "firings/s", "cores": 1, "kind": "port",
                          "sample": f"the first {k} packets ({m} firings) through the oracle's serial reader",
