@@ -465,9 +465,11 @@ int run_registration(lw_ctx *c, const int32_t *n_dev, int32_t n_bound, int32_t n
     hipError_t e = hipErrorNotSupported;
     if (c->loop_ok) {
         e = lw::launch_register_loop(c, n_dev, n_bound, L, p->acc_mode);
-        if (e != hipSuccess) {
-            c->loop_ok = false;     // cooperative launches unavailable here: one launch per pass from now on
+        if (e == hipErrorNotSupported) {
+            c->loop_ok = false;     // no persistent launch on this device: one launch per pass from now on
             (void)hipGetLastError();
+        } else if (e != hipSuccess) {
+            return -(int)e;
         }
     }
     int32_t n = n_known;
@@ -525,8 +527,10 @@ int run_registration(lw_ctx *c, const int32_t *n_dev, int32_t n_bound, int32_t n
     return 0;
 }
 
+// The registration state keeps 64 robust-kernel scales, so max_iter is at most 64.
 bool bad_params(const lw_reg_params *p) {
-    return !p || p->max_iter < 0 || p->acc_mode < 0 || p->acc_mode > 2 || !(p->reg_voxel > 0.0f);
+    return !p || p->max_iter < 0 || p->max_iter > 64 || p->acc_mode < 0 || p->acc_mode > 2 ||
+           !(p->reg_voxel > 0.0f);
 }
 
 }  // namespace
@@ -682,12 +686,13 @@ int lw_odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, lw_f
         LW_CK(lw::launch_reg_begin(c, c->chain_tp));
     }
     hipError_t e = lw::launch_register_loop(c, &c->ctr[5], cap, L, p->acc_mode);
-    if (e != hipSuccess) {
-        c->loop_ok = false;  // cooperative launches unavailable here: the caller takes the step-by-step path
+    if (e == hipErrorNotSupported) {
+        c->loop_ok = false;  // no persistent launch on this device: the caller takes the step-by-step path
         (void)hipGetLastError();
         LW_CK(hipStreamSynchronize(c->stream));
         return LW_ENOLOOP;
     }
+    if (e != hipSuccess) return -(int)e;
     // odom.py:683-690: the re-deskew test, then (gated on it) deskew and register again from the first pose
     const int32_t *redo = &c->fctl->redo;
     if (one) {

@@ -63,10 +63,11 @@ __global__ void k_imap_clear(MapRec *recs, int64_t cap, int32_t *stats) {
 
 // gpu.py:1346-1365 (k_pose_scatter, pose part): pose idx[k] = mats[k].
 __global__ void k_pose_scatter(const int32_t *idx, const float *mats, int32_t n, float *poses) {
-    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n * 16) return;
-    const int32_t e = k / 16, c = k % 16;
-    poses[(int64_t)(idx[e] & (IMAP_POSE_SLOTS - 1)) * 16 + c] = mats[k];
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < (int64_t)n * 16;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = k / 16, c = k % 16;
+        poses[(int64_t)(idx[e] & (IMAP_POSE_SLOTS - 1)) * 16 + c] = mats[k];
+    }
 }
 
 // gpu.py:156-167 (keep_raw): the raw-return filters.

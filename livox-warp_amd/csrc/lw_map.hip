@@ -61,18 +61,11 @@ __device__ inline bool insert_point(const float *xyz, const uint32_t *attr, int3
     return true;
 }
 
-// 1: claim. A plain read first: a slot's key only ever goes from -1 to its final value.
-__global__ __launch_bounds__(BLOCK) void k_ins_claim(const float *xyz, const uint32_t *attr, int32_t n,
-                                                     const int32_t *n_dev, Mat4f T, float inv_v, float max_range,
-                                                     int32_t ret_mask, int32_t mask, Slot *slots, uint32_t *occ,
-                                                     int64_t *vox_key, int32_t *vox_slot, VoxMom *vox_mom,
-                                                     int32_t *ctr, int64_t vox_cap, int32_t *pt_slot,
-                                                     const int32_t *gate, const float *T_dev) {
-    if (gated_off(gate)) return;
-    if (T_dev) T = load_mat4f(T_dev);
-    int32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i == 0) ctr[4] = 0;  // touched count of this insert (read by the next kernel only)
-    if (i >= count_of(n, n_dev)) return;
+// The claim of point i (k_ins_claim's body).
+__device__ inline void claim_point(const float *xyz, const uint32_t *attr, int32_t i, const Mat4f &T, float inv_v,
+                                   float max_range, int32_t ret_mask, int32_t mask, Slot *slots, uint32_t *occ,
+                                   int64_t *vox_key, int32_t *vox_slot, VoxMom *vox_mom, int32_t *ctr,
+                                   int64_t vox_cap, int32_t *pt_slot) {
     int32_t ix, iy, iz;
     float wx, wy, wz;
     if (!insert_point(xyz, attr, i, T, inv_v, max_range, ret_mask, ix, iy, iz, wx, wy, wz)) {
@@ -123,6 +116,22 @@ __global__ __launch_bounds__(BLOCK) void k_ins_claim(const float *xyz, const uin
     pt_slot[i] = -1;
 }
 
+// 1: claim. A plain read first: a slot's key only ever goes from -1 to its final value.
+__global__ __launch_bounds__(BLOCK) void k_ins_claim(const float *xyz, const uint32_t *attr, int32_t n,
+                                                     const int32_t *n_dev, Mat4f T, float inv_v, float max_range,
+                                                     int32_t ret_mask, int32_t mask, Slot *slots, uint32_t *occ,
+                                                     int64_t *vox_key, int32_t *vox_slot, VoxMom *vox_mom,
+                                                     int32_t *ctr, int64_t vox_cap, int32_t *pt_slot,
+                                                     const int32_t *gate, const float *T_dev) {
+    if (gated_off(gate)) return;
+    if (T_dev) T = load_mat4f(T_dev);
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[4] = 0;  // touched count of this insert (read by the next kernel)
+    const int32_t cnt = count_of(n, n_dev);
+    for (int32_t i = blockIdx.x * BLOCK + threadIdx.x; i < cnt; i += gridDim.x * BLOCK)
+        claim_point(xyz, attr, i, T, inv_v, max_range, ret_mask, mask, slots, occ, vox_key, vox_slot, vox_mom, ctr,
+                    vox_cap, pt_slot);
+}
+
 // 2: count the incoming points per voxel; the first one lists the voxel as touched.
 // counts, when given, receives the map's counts (ctr[0..1]), final once the claims are done.
 __global__ __launch_bounds__(BLOCK) void k_ins_count(const int32_t *pt_slot, int32_t n, const int32_t *n_dev,
@@ -131,13 +140,13 @@ __global__ __launch_bounds__(BLOCK) void k_ins_count(const int32_t *pt_slot, int
                                                      int32_t *counts) {
     if (gated_off(gate)) return;
     if (counts && blockIdx.x == 0 && threadIdx.x < 2) counts[threadIdx.x] = ctr[threadIdx.x];
-    int32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= count_of(n, n_dev)) return;
-    int32_t s = pt_slot[i];
-    int32_t vid = s >= 0 ? slots[s].vid : -1;
-    pt_vid[i] = vid;
-    if (vid < 0) return;
-    if (atomicAdd(&vox_nin[vid], 1) == 0) touched[atomicAdd(&ctr[4], 1)] = vid;
+    const int32_t cnt = count_of(n, n_dev);
+    for (int32_t i = blockIdx.x * BLOCK + threadIdx.x; i < cnt; i += gridDim.x * BLOCK) {
+        int32_t s = pt_slot[i];
+        int32_t vid = s >= 0 ? slots[s].vid : -1;
+        pt_vid[i] = vid;
+        if (vid >= 0 && atomicAdd(&vox_nin[vid], 1) == 0) touched[atomicAdd(&ctr[4], 1)] = vid;
+    }
 }
 
 // 3: per touched voxel, its quota under the cap; voxels that overflow rank their points in index order.
@@ -196,27 +205,28 @@ __global__ __launch_bounds__(BLOCK) void k_ins_accum(const float *xyz, const uin
                                                      VoxMom *vox_mom, const int32_t *gate, const float *T_dev) {
     if (gated_off(gate)) return;
     if (T_dev) T = load_mat4f(T_dev);
-    int32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= count_of(n, n_dev)) return;
-    int32_t vid = pt_vid[i];
-    if (vid < 0) return;
-    int32_t q = vox_quota[vid];
-    if (q != INT_MAX && !(q > 0 && pt_take[i])) return;
-    int32_t ix, iy, iz;
-    float wx, wy, wz;
-    insert_point(xyz, attr, i, T, inv_v, max_range, ret_mask, ix, iy, iz, wx, wy, wz);
-    float r0 = wx - center_of(ix, voxel), r1 = wy - center_of(iy, voxel), r2 = wz - center_of(iz, voxel);
-    VoxMom *m = &vox_mom[vid];
-    unsafeAtomicAdd(&m->s[0], r0);
-    unsafeAtomicAdd(&m->s[1], r1);
-    unsafeAtomicAdd(&m->s[2], r2);
-    unsafeAtomicAdd(&m->o[0], r0 * r0);
-    unsafeAtomicAdd(&m->o[1], r0 * r1);
-    unsafeAtomicAdd(&m->o[2], r0 * r2);
-    unsafeAtomicAdd(&m->o[3], r1 * r1);
-    unsafeAtomicAdd(&m->o[4], r1 * r2);
-    unsafeAtomicAdd(&m->o[5], r2 * r2);
-    atomicAdd(&m->cnt, 1);
+    const int32_t cnt = count_of(n, n_dev);
+    for (int32_t i = blockIdx.x * BLOCK + threadIdx.x; i < cnt; i += gridDim.x * BLOCK) {
+        int32_t vid = pt_vid[i];
+        if (vid < 0) continue;
+        int32_t q = vox_quota[vid];
+        if (q != INT_MAX && !(q > 0 && pt_take[i])) continue;
+        int32_t ix, iy, iz;
+        float wx, wy, wz;
+        insert_point(xyz, attr, i, T, inv_v, max_range, ret_mask, ix, iy, iz, wx, wy, wz);
+        float r0 = wx - center_of(ix, voxel), r1 = wy - center_of(iy, voxel), r2 = wz - center_of(iz, voxel);
+        VoxMom *m = &vox_mom[vid];
+        unsafeAtomicAdd(&m->s[0], r0);
+        unsafeAtomicAdd(&m->s[1], r1);
+        unsafeAtomicAdd(&m->s[2], r2);
+        unsafeAtomicAdd(&m->o[0], r0 * r0);
+        unsafeAtomicAdd(&m->o[1], r0 * r1);
+        unsafeAtomicAdd(&m->o[2], r0 * r2);
+        unsafeAtomicAdd(&m->o[3], r1 * r1);
+        unsafeAtomicAdd(&m->o[4], r1 * r2);
+        unsafeAtomicAdd(&m->o[5], r2 * r2);
+        atomicAdd(&m->cnt, 1);
+    }
 }
 
 // The mean and precision of voxel vid from its moments (odom.py:336-368), into its slot and vox_prec.
@@ -279,21 +289,22 @@ __global__ void k_set_ctr(int32_t *ctr, int32_t idx, int32_t v) { ctr[idx] = v; 
 
 __global__ __launch_bounds__(BLOCK) void k_map_find(const Slot *slots, const uint32_t *occ, int32_t mask,
                                                     const int64_t *keys, int32_t n, int32_t *out) {
-    int32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    int64_t key = keys[i];
-    int32_t h = slot_hash(key) & mask, res = -1;
-    for (int probe = 0; probe < PROBES; ++probe) {
-        int32_t s = (h + probe) & mask;
-        if (!((occ[s >> 5] >> (s & 31)) & 1u)) break;
-        if (slots[s].key == key) {
-            res = s;
-            break;
+    for (int32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        int64_t key = keys[i];
+        int32_t h = slot_hash(key) & mask, res = -1;
+        for (int probe = 0; probe < PROBES; ++probe) {
+            int32_t s = (h + probe) & mask;
+            if (!((occ[s >> 5] >> (s & 31)) & 1u)) break;
+            if (slots[s].key == key) {
+                res = s;
+                break;
+            }
         }
+        out[i] = res;
     }
-    out[i] = res;
 }
 
+// Workgroups for n items, capped at max_grid: every kernel launched with it strides over its items by the grid.
 inline int grid_for(int64_t n, int block = BLOCK, int64_t max_grid = 65535) {
     int64_t g = (n + block - 1) / block;
     if (g < 1) g = 1;

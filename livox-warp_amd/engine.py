@@ -46,9 +46,17 @@ class Engine:
         self._i32 = (ctypes.c_int32 * 2)()
 
     def close(self):
-        if self.ctx is not None and self.ctx.value:
-            self.lib.lw_destroy(self.ctx)
-            self.ctx = None
+        """Wait for the context's queued work and free it (idempotent)."""
+        ctx = getattr(self, "ctx", None)
+        self.ctx = None
+        if ctx is not None and ctx.value:
+            _abi.check(self.lib.lw_destroy(ctx), "lw_destroy")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
         try:

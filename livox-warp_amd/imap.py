@@ -70,14 +70,24 @@ class IntegrationMap:
         self._i32 = (ctypes.c_int32 * 2)()
         self.map_occupied = self.map_dropped = 0
 
-    def __del__(self):
+    def close(self):
+        """Wait for the map's queued work and free it (idempotent)."""
         m = getattr(self, "_m", None)
+        self._m = None
         if m is not None and m.value:
-            try:
-                self._lib.lw_imap_destroy(m)
-            except Exception:
-                pass
-            self._m = None
+            _abi.check(self._lib.lw_imap_destroy(m), "lw_imap_destroy")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     @staticmethod
     def _dev(a, dtypes, n: int, width: int = 1) -> int:

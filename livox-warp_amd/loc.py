@@ -55,6 +55,19 @@ class BatchLocalizer:
         self.tiles = {}
         self._bufs = None
 
+    def close(self):
+        """Free the map's context, with the captured graph it holds (idempotent)."""
+        eng = getattr(self, "eng", None)
+        self._bufs = None
+        if eng is not None:
+            eng.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
     def map_stats(self):
         return self.eng.stats()
 

@@ -159,16 +159,20 @@ def read_firings(path: str | os.PathLike, device=None, stream=None, timing: dict
     first = np.zeros(len(offs), np.int64)
     first[order] = np.concatenate([[0], np.cumsum(fpp[order])[:-1]])
     n = int(fpp.sum())
-    bytes_dev = torch.from_numpy(buf).to(dev)
-    tab = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in
-           (offs, dtypes.astype(np.int32), fpp.astype(np.int32), t_pk, first)]
-    t = torch.empty(n, dtype=torch.float64, device=dev)
-    xyz = torch.empty((n, R, 3), dtype=torch.float32, device=dev)
-    refl = torch.empty((n, R), dtype=torch.uint8, device=dev)
-    tag = torch.empty((n, R), dtype=torch.uint8, device=dev)
-    valid = torch.empty((n, R), dtype=torch.uint8, device=dev)
-    direction = torch.empty((n, 3), dtype=torch.float32, device=dev)
     s = stream if stream is not None else torch.cuda.current_stream(dev)
+    # every device buffer is allocated and uploaded on the decode's stream (after what the caller's current
+    # stream has queued), so the caching allocator ties each to the stream that reads it
+    s.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(s):
+        bytes_dev = torch.from_numpy(buf).to(dev)
+        tab = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in
+               (offs, dtypes.astype(np.int32), fpp.astype(np.int32), t_pk, first)]
+        t = torch.empty(n, dtype=torch.float64, device=dev)
+        xyz = torch.empty((n, R, 3), dtype=torch.float32, device=dev)
+        refl = torch.empty((n, R), dtype=torch.uint8, device=dev)
+        tag = torch.empty((n, R), dtype=torch.uint8, device=dev)
+        valid = torch.empty((n, R), dtype=torch.uint8, device=dev)
+        direction = torch.empty((n, 3), dtype=torch.float32, device=dev)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)] if timing is not None else None
     if ev:
         timing["host_s"] = time.perf_counter() - t0
@@ -186,4 +190,9 @@ def read_firings(path: str | os.PathLike, device=None, stream=None, timing: dict
         idx = torch.sort(t, stable=True).indices
         out = Firings(t=t[idx], xyz=xyz[idx], refl=refl[idx], tag=tag[idx], valid=valid[idx].bool(),
                       direction=direction[idx], dev_type=dev_type, profile=profile, interval=interval * 1e-9)
+    if s != torch.cuda.current_stream(dev):
+        # the caller reads the result on its current stream: that stream waits for the decode and the sort
+        torch.cuda.current_stream(dev).wait_stream(s)
+        for a in (out.t, out.xyz, out.refl, out.tag, out.valid, out.direction):
+            a.record_stream(torch.cuda.current_stream(dev))
     return out

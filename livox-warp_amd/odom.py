@@ -195,14 +195,26 @@ class Odometry:
 
         self.reset(np.eye(4, dtype=np.float64))
 
-    def __del__(self):
+    def close(self):
+        """Wait for the instance's queued work and free its context (map, frame work buffers, pinned readback
+        memory). Idempotent; the instance is unusable afterwards. Call it before the interpreter exits rather than
+        leaving the context to the garbage collector during shutdown."""
         ctx = getattr(self, "_ctx", None)
+        self._ctx = None
         if ctx is not None and ctx.value:
-            try:
-                self._lib.lw_destroy(ctx)
-            except Exception:
-                pass
-            self._ctx = None
+            _abi.check(self._lib.lw_destroy(ctx), "lw_destroy")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     # ---- stream helpers --------------------------------------------------------------------
 

@@ -101,14 +101,24 @@ class MLP:
         self._loss = torch.zeros(1, dtype=torch.float32, device=self.device)
         self.steps = 0
 
-    def __del__(self):
+    def close(self):
+        """Wait for the network's queued work and free its device state (idempotent)."""
         h = getattr(self, "h", None)
+        self.h = None
         if h is not None and h.value:
-            try:
-                self.lib.lw_mlp_destroy(h)
-            except Exception:
-                pass
-            self.h = None
+            _abi.check(self.lib.lw_mlp_destroy(h), "lw_mlp_destroy")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def set_stream(self, stream):
         self.stream = stream
