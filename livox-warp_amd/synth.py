@@ -311,6 +311,37 @@ class Walk:
         return self.state(np.arange(0.0, seconds, step))[0]
 
 
+class InPlace:
+    """The reference simulator's other two motions about one spot of a walk (sources.py:197-200, 264-285):
+    "static" holds the walk's start pose; "turn" holds it for 1.5 s, then turns in place with yaw
+    0.7 sin(0.5 u) and pitch 0.03 sin(1.1 u) added to the start attitude, u = (t - 1.5) 2 pi / 40 (one slow
+    swing of about 35 degrees over 15 s). Same interface as Walk."""
+
+    HOLD = 1.5
+    OMEGA = 2.0 * math.pi / 40.0
+
+    def __init__(self, walk: Walk, motion: str):
+        if motion not in ("static", "turn"):
+            raise ValueError(f"unknown in-place motion {motion!r}")
+        self.walk = walk
+        self.turn = motion == "turn"
+        p, yaw, pitch, roll = walk.state(0.0)
+        self.p0, self.yaw0, self.pitch0, self.roll0 = np.asarray(p, float), float(yaw), float(pitch), float(roll)
+
+    def state(self, t):
+        t = np.asarray(t, float)
+        u = np.maximum(t - self.HOLD, 0.0) * self.OMEGA
+        dyaw = 0.7 * np.sin(0.5 * u) if self.turn else np.zeros_like(t)
+        dpitch = 0.03 * np.sin(1.1 * u) if self.turn else np.zeros_like(t)
+        p = np.broadcast_to(self.p0, np.shape(t) + (3,)).copy()
+        return p, self.yaw0 + dyaw, self.pitch0 + dpitch, self.roll0 + np.zeros_like(t)
+
+    pose = Walk.pose
+
+    def path(self, seconds: float = 60.0, step: float = 0.05) -> np.ndarray:
+        return self.walk.path(seconds, step)  # the scene is furnished around the walk, as for the walk itself
+
+
 class Rosette:
     """Rose-curve scan directions: off-axis angle HALF_FOV * |sin(k theta)|, theta advancing with time, the whole
     pattern precessing slowly so successive frames fill the circle (k rational, from the seed)."""
@@ -475,8 +506,8 @@ class Sequence:
     """A seeded odometry sequence: a walk through the office, scanned at 100 kHz, cut into batches on demand."""
 
     def __init__(self, seed: int = 21, seconds: float = 60.0, offset=(0.0, 0.0, 0.0), points_per_frame: int = 10_000,
-                 frame_dt: float = 0.1):
-        self.walk = Walk(seed)
+                 frame_dt: float = 0.1, motion: str = "walk"):
+        self.walk = Walk(seed) if motion == "walk" else InPlace(Walk(seed), motion)
         self.offset = np.asarray(offset, float)
         self.scene = office(seed + 1, path=self.walk.path(seconds + 1.0)).translated(self.offset)
         self.rosette = Rosette(seed + 2)
