@@ -16,7 +16,7 @@ liblwamd.so).
 from . import _abi, engine, imap, loc, localize, lvxr, mapgrid, odom, pose_table, prior, synth, worker  # noqa: F401
 from .engine import Engine  # noqa: F401
 from .imap import IntegrationMap  # noqa: F401
-from .localize import CoarseSearch, GlobalLocalizer, Localizer  # noqa: F401
+from .localize import CoarseSearch, GlobalLocalizer  # noqa: F401
 from .mapgrid import PriorGrid  # noqa: F401
 from .odom import Odometry  # noqa: F401
 from .pose_table import PoseTable  # noqa: F401
@@ -24,4 +24,4 @@ from .worker import OdomWorker  # noqa: F401
 
 __all__ = ["Odometry", "Engine", "PoseTable", "IntegrationMap", "OdomWorker", "odom", "engine", "imap", "loc",
            "pose_table", "prior", "synth", "worker", "lvxr", "mapgrid", "localize", "PriorGrid", "CoarseSearch",
-           "GlobalLocalizer", "Localizer"]
+           "GlobalLocalizer"]

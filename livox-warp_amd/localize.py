@@ -8,9 +8,9 @@ tolerance and more points (lw_score_list), and greedy non-maximum suppression ke
 refines (loc.BatchLocalizer, or the reference's own). The host parts (grid layout, local maxima, the pool of
 best scores, suppression) follow the reference step for step, so the same scores pick the same candidates.
 
-ScanPrep here is the part of the reference's (localize.py:243-337) that the heights need: the 5 cm voxel
-means, per-point normals from the 30 nearest neighbours within 0.2 m, and the height profile of the
-horizontal points. The reference also finds "up" from RANSAC planes (open3d); here the caller gives it.
+The reference's scan preparation (RANSAC gravity candidates, normals, the height search over the map's levels)
+and its whole localize() / refine() flow are not part of this path: a caller gives the sensor height and the
+gravity-aligned scan.
 """
 
 from __future__ import annotations
@@ -46,121 +46,6 @@ def twist_apply(T: np.ndarray, delta: np.ndarray) -> np.ndarray:
     return out
 
 
-def rot_to(a: np.ndarray, b: np.ndarray) -> np.ndarray:
-    """The rotation taking unit vector a onto unit vector b (localize.py:211-219)."""
-    a = np.asarray(a, float) / np.linalg.norm(a)
-    b = np.asarray(b, float) / np.linalg.norm(b)
-    v = np.cross(a, b)
-    c = float(a @ b)
-    if np.linalg.norm(v) < 1e-9:
-        return np.eye(3) if c > 0 else np.diag([1.0, -1.0, -1.0])  # a half turn about x
-    K = np.array([[0, -v[2], v[1]], [v[2], 0, -v[0]], [-v[1], v[0], 0]])
-    return np.eye(3) + K + K @ K / (1.0 + c)
-
-
-def pose_difference(A: np.ndarray, B: np.ndarray):
-    """(translation m, rotation deg) between two 4x4 poses (localize.py:236-240)."""
-    d = np.linalg.inv(A) @ B
-    ang = math.degrees(math.acos(np.clip((np.trace(d[:3, :3]) - 1) / 2, -1, 1)))
-    return float(np.linalg.norm(A[:3, 3] - B[:3, 3])), ang
-
-
-class ScanPrep:
-    """A scan in the sensor frame prepared for the height search (localize.py:243-337, less the RANSAC planes):
-    one point per 5 cm voxel (the mean of its points, voxels counted from half a voxel below the scan's
-    minimum as open3d's voxel_down_sample does) and each point's normal, the least-spread direction of its 30
-    nearest neighbours within 0.2 m (sign arbitrary)."""
-
-    def __init__(self, xyz: np.ndarray, voxel: float = 0.05, radius: float = 0.2, max_nn: int = 30,
-                 max_tilt_deg: float = 45.0, seed: int = 0):
-        from scipy.spatial import cKDTree
-
-        p = np.asarray(xyz, np.float64)
-        key = np.floor((p - (p.min(axis=0) - 0.5 * voxel)) / voxel).astype(np.int64)
-        _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
-        inv = inv.reshape(-1)
-        sums = np.zeros((len(cnt), 3))
-        np.add.at(sums, inv, p)
-        self.pts = sums / cnt[:, None]
-        d, nb = cKDTree(self.pts).query(self.pts, k=min(max_nn, len(self.pts)), distance_upper_bound=radius)
-        d, nb = np.atleast_2d(d), np.atleast_2d(nb)
-        ok = np.isfinite(d)
-        q = self.pts[np.where(ok, nb, 0)] * ok[..., None]
-        cnt = np.maximum(ok.sum(axis=1), 1)[:, None]
-        mean = q.sum(axis=1) / cnt
-        c = (q - mean[:, None, :]) * ok[..., None]
-        cov = np.einsum("nki,nkj->nij", c, c)
-        _, V = np.linalg.eigh(cov)
-        self.normals = V[:, :, 0] * (ok.sum(axis=1) >= 3)[:, None]  # fewer than 3 neighbours: no normal
-        # up to 8 planes, each the best of 1000 three-point samples (inliers within 3 cm) of the points no
-        # earlier plane took, while one holds at least 150 points (localize.py:259-273)
-        rng = np.random.default_rng(seed)
-        self.planes = []  # (inliers, unit normal, offset, inlier indices)
-        idx = np.arange(len(self.pts))
-        for _ in range(8):
-            if len(idx) < 300:
-                break
-            P = self.pts[idx]
-            tri = P[rng.integers(0, len(P), (1000, 3))]
-            nrm = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
-            ln = np.linalg.norm(nrm, axis=1)
-            good = ln > 1e-12
-            nrm = nrm[good] / ln[good, None]
-            off = -np.einsum("ij,ij->i", nrm, tri[good, 0])
-            cnt = np.zeros(len(nrm), np.int64)
-            for a in range(0, len(nrm), 64):
-                cnt[a:a + 64] = (np.abs(P @ nrm[a:a + 64].T + off[a:a + 64]) < 0.03).sum(axis=0)
-            if not len(cnt):
-                break
-            b = int(np.argmax(cnt))
-            inl = np.flatnonzero(np.abs(P @ nrm[b] + off[b]) < 0.03)
-            if len(inl) < 150:
-                break
-            self.planes.append((len(inl), nrm[b], float(off[b]), idx[inl]))
-            idx = np.delete(idx, inl)
-        self.max_tilt = max_tilt_deg
-
-    def up_candidates(self):
-        """[(up unit vector, evidence)] ranked by plane support, both signs of each horizontal family: one normal
-        fitted to all planes of a family together, the least-spread direction of the sum of their scatter
-        matrices (localize.py:276-310); a level sensor either way up when no plane is horizontal enough."""
-        cos_tilt = math.cos(math.radians(self.max_tilt))
-        fams = []  # [support, [plane indices], seed normal]
-        for i, (cnt, n, _, _) in enumerate(self.planes):
-            if abs(n[2]) < cos_tilt:
-                continue  # a wall, for a sensor that is roughly level (or upside down)
-            for f in fams:
-                if abs(float(n @ f[2])) > math.cos(math.radians(10.0)):
-                    f[0] += cnt
-                    f[1].append(i)
-                    break
-            else:
-                fams.append([cnt, [i], n])
-        out = []
-        for support, members, n0 in sorted(fams, key=lambda f: -f[0])[:2]:
-            S = np.zeros((3, 3))
-            for i in members:
-                P = self.pts[self.planes[i][3]]
-                P = P - P.mean(0)
-                S += P.T @ P
-            _, V = np.linalg.eigh(S)
-            n = V[:, 0]
-            if n @ n0 < 0:
-                n = -n
-            out += [(n, support), (-n, support)]
-        if not out:
-            out = [(np.array([0.0, 0.0, 1.0]), 0), (np.array([0.0, 0.0, -1.0]), 0)]
-        return out
-
-    def height_profile(self, up: np.ndarray, lo: float, n_bins: int, bin_m: float):
-        """Histogram of heights (along up) of the scan's horizontal points, relative to the sensor
-        (localize.py:331-336)."""
-        horiz = np.abs(self.normals @ up) > 0.97
-        z = self.pts[horiz] @ up
-        h, _ = np.histogram(z, bins=n_bins, range=(lo, lo + n_bins * bin_m))
-        return h.astype(np.float32)
-
-
 def rotz(a: float) -> np.ndarray:
     c, s = math.cos(a), math.sin(a)
     return np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
@@ -188,7 +73,6 @@ class CoarseSearch:
         self.d_counts = torch.empty((self.ICP_CAP, 3), dtype=torch.int32, device=d)
         wb = int(_abi.lib().lw_plane_work_bytes(self.ICP_CAP))
         self.d_work = torch.empty(wb, dtype=torch.uint8, device=d)
-        self.seed = 0  # refine() draws its subsets from a fresh generator with this seed
         # tuning (localize.py:375-388)
         self.xy_step = 0.25
         self.yaw_step_deg = 2.0
@@ -199,12 +83,6 @@ class CoarseSearch:
         self.n_mid_pts = 2048
         self.keep_coarse = 128  # coarse local maxima re-scored on the finer grid
         self.n_icp = 16  # medium winners handed to the ICP
-        self.n_heights = 8  # sensor heights tried
-        # unique: inliers (3 cm) at least accept_inliers, and the best answer elsewhere at least accept_gap lower
-        # and below accept_ratio of it (localize.py:390-394)
-        self.accept_inliers = 0.45
-        self.accept_gap = 0.15
-        self.accept_ratio = 0.75
         self.last_kernel_ms = 0.0
 
     def _stream(self, stream):
@@ -462,183 +340,4 @@ class CoarseSearch:
         """The scan (sensor frame) that icp() and evaluate() read: returns its point count."""
         return self._upload_pts(np.asarray(pts, np.float32), self._stream(stream))
 
-    def refine(self, xyz: np.ndarray, T0: np.ndarray, stream=None, schedule=((0.30, 6), (0.12, 8), (0.05, 8)),
-               starts=(), switch_margin: float = 0.05):
-        """Point-to-plane ICP of a point set against the map from T0 (the pose-tracking step, localize.py:
-        567-612): one point per 5 cm voxel, T0 and the offset starts (x, y, z m, optional heading deg about the
-        vertical through the points' centre) refined together; an offset start wins only if its 3 cm fit beats
-        the plain result by switch_margin. Returns (T, fit, info)."""
-        rng = np.random.default_rng(self.seed)
-        pts = np.asarray(xyz, dtype=np.float32)
-        key = np.floor(pts / 0.05).astype(np.int64)
-        _, first = np.unique(key, axis=0, return_index=True)
-        pts = pts[np.sort(first)]
-        if len(pts) > self.PTS_CAP:
-            pts = pts[rng.choice(len(pts), self.PTS_CAP, replace=False)]
-        n = len(pts)
-        if n < 200:
-            raise RuntimeError(f"too few points to refine ({n} after 5 cm thinning)")
-        self.upload_scan(pts, stream)
-        T0 = np.asarray(T0, dtype=np.float64)
-        inits = [T0]
-        c_map = T0[:3, :3] @ pts.mean(axis=0).astype(np.float64) + T0[:3, 3]
-        for off in list(starts)[: self.ICP_CAP - 2]:
-            Ti = T0.copy()
-            off = np.asarray(off, dtype=np.float64)
-            if len(off) > 3 and off[3] != 0.0:
-                Rz = rotz(math.radians(off[3]))
-                Ti[:3, :3] = Rz @ Ti[:3, :3]
-                Ti[:3, 3] = Rz @ (Ti[:3, 3] - c_map) + c_map
-            Ti[:3, 3] += off[:3]
-            inits.append(Ti)
-        Ts = self.icp(inits, n, schedule, stream)
-        fits = self.evaluate(Ts + [T0], n, stream)
-        fit_start = fits[-1]
-        fits = fits[:-1]
-        best = 0
-        for i in range(1, len(Ts)):
-            if fits[i, 0] > fits[best, 0] + (switch_margin if best == 0 else 0.0):
-                best = i
-        return Ts[best], fits[best], {"switched": best != 0, "plain_fit": float(fits[0, 0]), "n": n,
-                                      "fit_start": float(fit_start[0])}
-
-    # ---- the heights and the whole search for one gravity candidate (localize.py:426-446, 603-720) ---------
-
-    def heights(self, prep: ScanPrep, up: np.ndarray):
-        """Sensor heights where the scan's horizontal surfaces line up with the map's, best first
-        (localize.py:426-446): the square roots of both height histograms, smoothed over +-2 bins, correlated;
-        the best n_heights shifts at least 8 cm apart that put the sensor within the map's height +-0.5 m."""
-        h_map, lo, bin_m = self.grid.levels
-        span = 6.0
-        nb = int(round(2 * span / bin_m))
-        h_scan = prep.height_profile(up, -span, nb, bin_m)
-        if h_scan.sum() < 50:
-            return None
-        k = np.array([0.25, 0.5, 1.0, 0.5, 0.25], np.float32)
-        hm = np.convolve(np.sqrt(h_map), k, mode="same")
-        hs = np.convolve(np.sqrt(h_scan), k, mode="same")
-        corr = np.correlate(hm, hs, mode="full")
-        shifts = np.arange(len(corr)) - (nb - 1)
-        z_sensor = lo + shifts * bin_m + span
-        ok = (z_sensor >= self.grid.lo[2] - 0.5) & (z_sensor <= self.grid.hi[2] + 0.5)
-        corr = np.where(ok, corr, -1.0)
-        out = []
-        for i in np.argsort(-corr):
-            if corr[i] <= 0 or len(out) >= self.n_heights:
-                break
-            z = float(z_sensor[i])
-            if all(abs(z - o) > 0.08 for o in out):
-                out.append(z)
-        return out
-
-    def _unique(self, best: float, alt: float) -> bool:
-        return best >= self.accept_inliers and best - alt >= self.accept_gap and alt <= self.accept_ratio * best
-
-    @staticmethod
-    def _runner_up(results) -> float:
-        best = results[0][3]
-        for r in results[1:]:
-            dt, ang = pose_difference(best, r[3])
-            if dt > 1.0 or ang > 15.0:
-                return r[0]
-        return 0.0
-
-    def localize(self, xyz: np.ndarray, up=None, up_prior=None, stream=None):
-        """Sensor-to-map pose of a sensor-frame scan, and diagnostics: the reference's localize()
-        (localize.py:614-720). 5 cm points (at most PTS_CAP), the coarse subset (1024, one per 15 cm voxel first)
-        and the medium subset (2048); gravity candidates from the scan's planes (or `up`, sensor frame, alone),
-        ordered by up_prior if given; for each, the heights (or every 0.25 m of the map's height when the scan
-        shows no horizontal surface) and the coarse stage; then, best coarse score first, the medium stage, the
-        grid ICP of its candidates and the 3 cm fit, until one candidate's best answer is unique."""
-        rng = np.random.default_rng(self.seed)
-        prep = ScanPrep(xyz, seed=self.seed)
-        pts = prep.pts
-        if len(pts) > self.PTS_CAP:
-            pts = pts[rng.choice(len(pts), self.PTS_CAP, replace=False)]
-        n_pts = len(pts)
-        if n_pts < 200:
-            raise RuntimeError(f"scan too sparse to localise ({n_pts} points after 5 cm thinning)")
-        key = np.floor(pts / 0.15).astype(np.int64)
-        _, first = np.unique(key, axis=0, return_index=True)
-        spread = pts[first]
-        sub_c = spread[rng.choice(len(spread), min(self.n_coarse_pts, len(spread)), replace=False)]
-        sub_m = pts[rng.choice(n_pts, min(self.n_mid_pts, n_pts), replace=False)]
-        if up is not None:
-            ups = [(np.asarray(up, float) / np.linalg.norm(up), 0)]
-        else:
-            ups = prep.up_candidates()
-            if up_prior is not None:
-                upp = np.asarray(up_prior, float) / np.linalg.norm(up_prior)
-                ups.sort(key=lambda u: -float(u[0] @ upp))
-        cands = []
-        for u, support in ups:
-            Rg = rot_to(u, np.array([0.0, 0.0, 1.0]))  # sensor -> gravity-aligned sensor frame
-            heights = self.heights(prep, u)
-            if not heights:
-                heights = list(np.arange(self.grid.lo[2], self.grid.hi[2] + 1e-6, self.xy_step))
-            c = self.coarse_candidate(sub_c, heights, Rg, stream)
-            c.update(up=u, support=support)
-            cands.append(c)
-        cands.sort(key=lambda c: -c["best_coarse"])
-        results, refined = [], 0
-        for ci, c in enumerate(cands):
-            if not len(c["coarse"]):
-                continue
-            Rg = c["Rg"]
-            hyps, ms, win = self.medium(c["coarse"], np.asarray(sub_m, float) @ Rg.T, stream)
-            Ts = []
-            for i in win:
-                x, y, z, yaw = hyps[i]
-                T = np.eye(4)
-                T[:3, :3] = rotz(yaw) @ Rg
-                T[:3, 3] = (x, y, z)
-                Ts.append(T)
-            if not Ts:
-                continue
-            self.upload_scan(pts, stream)
-            Tr = self.icp(Ts, n_pts, stream=stream)
-            fr = self.evaluate(Tr, n_pts, stream)
-            for j, i in enumerate(win):
-                results.append((float(fr[j, 0]), float(fr[j, 1]), float(fr[j, 2]), Tr[j], ci,
-                                float(ms[i]) / max(len(sub_m), 1)))
-            refined += 1
-            results.sort(key=lambda r: -r[0])
-            if self._unique(results[0][0], self._runner_up(results)):
-                break  # this candidate's best answer is good and unique
-        if not results:
-            raise RuntimeError("no hypothesis survived (is the scan inside the map?)")
-        results.sort(key=lambda r: -r[0])
-        top = results[0]
-        alt = self._runner_up(results)
-        c = cands[top[4]]
-        return top[3], {"inliers": top[0], "inliers5": top[1], "inliers10": top[2], "runner_up": alt,
-                        "unique": self._unique(top[0], alt), "up": c["up"], "heights": c["heights"],
-                        "gravity_candidates": len(cands), "refined_candidates": refined,
-                        "hypotheses": sum(c["hypotheses"] for c in cands), "scan_points": n_pts}
-
-
 GlobalLocalizer = CoarseSearch  # the reference's name for the search (localize.py:339)
-
-
-class Localizer:
-    """Single-stream convenience wrapper (localize.py:765-790): builds the grid from a map's points and normals
-    and searches or refines on one stream."""
-
-    def __init__(self, map_xyz: np.ndarray, map_normal: np.ndarray, map_planarity: np.ndarray | None = None,
-                 device=None, stream=None):
-        torch = _torch()
-        host = {"xyz": map_xyz, "normal": map_normal}
-        if map_planarity is not None:
-            host["planarity"] = map_planarity
-        self.grid = PriorGrid(host, device)
-        self.loc = GlobalLocalizer(self.grid, self.grid.device)
-        self.stream = stream if stream is not None else torch.cuda.current_stream(self.grid.device)
-        self.grid.build(self.stream)
-
-    def search(self, xyz: np.ndarray, up_prior=None):
-        """GlobalLocalizer.localize on this stream: the pose of a sensor-frame scan, and diagnostics."""
-        return self.loc.localize(xyz, up_prior=up_prior, stream=self.stream)
-
-    def refine(self, xyz: np.ndarray, T0: np.ndarray, **kw):
-        """GlobalLocalizer.refine on this stream: ICP of a point set from the known pose T0."""
-        return self.loc.refine(xyz, T0, self.stream, **kw)

@@ -30,17 +30,6 @@ def _torch():
     return torch
 
 
-def horizontal_levels(xyz: np.ndarray, normal: np.ndarray, planarity: np.ndarray, bin_m: float = 0.02):
-    """Histogram of the heights of a map's horizontal surfaces (points with a near-vertical, trusted normal)
-    (mapgrid.py:196-204): (counts float32, first edge, bin)."""
-    nz = np.abs(normal[:, 2].astype(np.float32))
-    sel = (nz > 0.97) & (planarity.astype(np.float32) > PLANAR)
-    lo, hi = float(xyz[:, 2].min()), float(xyz[:, 2].max())
-    edges = np.arange(lo - 0.5, hi + 0.5 + bin_m, bin_m)
-    h, _ = np.histogram(xyz[sel, 2], bins=edges)
-    return h.astype(np.float32), float(edges[0]), bin_m
-
-
 class PriorGrid:
     """A prior map on the device: points sorted by cell, unit normals, planarity, and the grid (the reference's
     PriorGrid, mapgrid.py:207-300). Construction allocates; build() fills on a stream and waits for it; after
@@ -49,7 +38,7 @@ class PriorGrid:
     @staticmethod
     def prepare(host: dict, cell: float = CELL, pad: float = 0.6) -> dict:
         """Host preparation (mapgrid.py:215-241): float32 arrays sorted by grid cell, unit normals, the grid
-        geometry and cell prefix, horizontal levels. A map without normals gets zero normals (no levels)."""
+        geometry and cell prefix. A map without normals gets zero normals."""
         if host.get("_prepared"):
             return host
         xyz = np.asarray(host["xyz"], dtype=np.float32)
@@ -78,8 +67,7 @@ class PriorGrid:
         nrm = np.ascontiguousarray(nrm / np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-6), dtype=np.float32)
         planar = np.ascontiguousarray(planar[order])
         return {"_prepared": True, "xyz": xyz, "normal": nrm, "planarity": planar, "lo": lo, "hi": hi,
-                "origin": origin, "cell": float(cell), "dims": dims, "start": start,
-                "levels": horizontal_levels(xyz, nrm, planar)}
+                "origin": origin, "cell": float(cell), "dims": dims, "start": start}
 
     def __init__(self, host: dict, device=None):
         torch = _torch()
@@ -95,7 +83,6 @@ class PriorGrid:
         # geometry as the kernels take it: origin, cell and 1 / cell in float32; nx, ny, nz
         self.geo = np.array([*self.origin, self.cell, 1.0 / self.cell], np.float32)
         self.dims32 = np.ascontiguousarray(self.dims, np.int32)
-        self.levels = host["levels"]
         d = self.device
         self.xyz = torch.empty((self.n, 3), dtype=torch.float32, device=d)
         self.nrm = torch.empty((self.n, 3), dtype=torch.float32, device=d)

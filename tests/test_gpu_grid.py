@@ -172,40 +172,3 @@ def test_plane_icp_and_eval_match_oracle(lw, room_grid):
     fits = cs.evaluate(Ts, n)
     want = O.plane_eval(pts, Ts, prep, nn) / n
     assert np.array_equal(fits, want)
-
-
-def test_refine_recovers_the_pose(lw, room_grid):
-    """refine() (localize.py:567-612) from 6 cm and 3 degrees off, with an offset start beside it: the true pose
-    within 1 cm and 0.2 degrees, most of the scan within 3 cm of the map."""
-    S, prep, grid = room_grid
-    cs = lw.CoarseSearch(grid, "cuda:0")
-    T_true, pts = G._room_scan(lw, S, n=8000, noise=0.02)
-    T0 = G._perturbed(lw, T_true)
-    T, fit, info = cs.refine(pts, T0, starts=[(0.5, 0.0, 0.0, 10.0)])
-    d = np.linalg.inv(T_true) @ T
-    ang = np.degrees(np.arccos(np.clip((np.trace(d[:3, :3]) - 1) / 2, -1, 1)))
-    print(f"refined: {np.linalg.norm(d[:3, 3]) * 100:.2f} cm, {ang:.3f} deg, fit {fit.round(3)}, {info}")
-    assert np.linalg.norm(d[:3, 3]) < 0.01 and ang < 0.2 and fit[0] > 0.6 and not info["switched"]
-
-
-def test_localize_without_heights(lw, gpu_ok):
-    """The reference's localize() flow with nothing given but the scan: "up" from the scan's planes (both
-    signs), the heights from its floor and ceiling against the map's levels, then coarse, medium, the grid
-    ICP and the 3 cm fit, on scans from three rooms of the multi-room floor (map with face normals), one of
-    them from a sensor pitched 6 and rolled 4 degrees."""
-    S = lw.synth.multi_room_floor(31)
-    mp, mn = lw.synth.sample_surfaces(S, 0.04, 32, prune=False, normals=True)
-    L = lw.Localizer(mp, mn, device="cuda:0")  # the reference's wrapper: grid built on its stream
-    cs = L.loc
-    ok = 0
-    for q, (x, y, yaw, pitch, roll) in enumerate(((6.3, 5.1, 0.7, 0, 0), (30.2, 20.4, 2.9, 6, 4),
-                                                   (53.5, 33.0, -1.8, 0, 0))):
-        T = lw.synth.pose_matrix(np.array([x, y, 1.4]), yaw, np.radians(pitch), np.radians(roll))
-        xyz, _, _ = lw.synth.scan(S, lw.synth.Rosette(40 + q), lambda t, T=T: T, 0.0, 20000, 50 + q)
-        Tb, info = L.search(xyz)
-        dt, ang = lw.localize.pose_difference(T, Tb)
-        print(f"query {q}: {dt * 100:.1f} cm, {ang:.2f} deg, inliers {info['inliers']:.3f}, runner-up "
-              f"{info['runner_up']:.3f}, unique {info['unique']}, heights {[round(h, 2) for h in info['heights'][:3]]}, "
-              f"{info['refined_candidates']} of {info['gravity_candidates']} gravity candidates refined")
-        ok += dt < 0.05 and ang < 1.0
-    assert ok == 3

@@ -13,157 +13,20 @@ oracle and the reference's ground-truth gates:
     64-slot map that overflows (odom.py:225-230, 698-702).
 """
 
-import ctypes
-import math
-from concurrent.futures import ThreadPoolExecutor
-
 import numpy as np
 import pytest
 import torch
+from seqcheck import Recorder as _Recorder
+from seqcheck import err as _err
+from seqcheck import gt_errors as _gt_errors
+from seqcheck import odometry as _odometry
+from seqcheck import params as _params
+from seqcheck import read_map as _read_map
+from seqcheck import replay as _replay
 
 pytestmark = pytest.mark.gpu
 
 C2_FRAMES = 600
-REPLICAS = 8
-
-
-def _err(A, B):
-    d = np.linalg.inv(A) @ B
-    return float(np.linalg.norm(A[:3, 3] - B[:3, 3])), float(
-        np.degrees(np.arccos(np.clip((np.trace(d[:3, :3]) - 1) / 2, -1, 1))))
-
-
-def _gt_errors(seq, traj):
-    T0 = seq.pose(traj[0][1])
-    et, er, dist = [], [], 0.0
-    for (k, tm, T), nxt in zip(traj, traj[1:] + [None]):
-        gt = np.linalg.inv(T0) @ seq.pose(tm)
-        est = np.linalg.inv(traj[0][2]) @ T
-        et.append(float(np.linalg.norm(est[:3, 3] - gt[:3, 3])))
-        d = est[:3, :3].T @ gt[:3, :3]
-        er.append(float(np.degrees(np.arccos(np.clip((np.trace(d) - 1) / 2, -1, 1)))))
-        if nxt is not None:
-            dist += float(np.linalg.norm(seq.pose(nxt[1])[:3, 3] - seq.pose(tm)[:3, 3]))
-    return np.array(et), np.array(er), dist
-
-
-def _odometry(lw, reg_slots=1 << 22):
-    odo = lw.Odometry(lw.PoseTable(), "cuda:0", reg_slots=reg_slots, frame_cap=1 << 15)
-    odo.reg_voxel = 0.2
-    odo.max_iter = 20
-    return odo
-
-
-def _read_map(odo):
-    """The device map in the reference's per-slot layout (lw_map_read)."""
-    cap = odo.reg_cap
-    keys, rsum = np.empty(cap, np.int64), np.empty((cap, 3), np.float32)
-    router, rcnt = np.empty((cap, 3, 3), np.float32), np.empty(cap, np.int32)
-    P = ctypes.POINTER
-    assert odo._lib.lw_map_read(odo._ctx, keys.ctypes.data_as(P(ctypes.c_int64)), rsum.ctypes.data_as(P(ctypes.c_float)),
-                                router.ctypes.data_as(P(ctypes.c_float)), rcnt.ctypes.data_as(P(ctypes.c_int32))) == 0
-    return keys, rsum, router, rcnt
-
-
-def _params(orc, odo):
-    return orc.Params(reg_voxel=odo.reg_voxel, max_iter=odo.max_iter, min_corr=odo.min_corr,
-                      min_range=odo.min_range, max_range=odo.max_range, noise=odo.noise, ret_mask=odo.ret_mask,
-                      alpha=odo.alpha, damping=odo.damping, degen_thr=odo.degen_thr, prior_info=odo.prior_info)
-
-
-class _Recorder:
-    """Watches an Odometry on the chain path: the inputs of each chained frame (prediction, first twist) and,
-    after the frame, the device data the oracle needs -- raw and deskewed points, attributes, times, the thinned
-    set of each registration -- plus every map insert (points and pose)."""
-
-    def __init__(self, lw, odo):
-        self.lw, self.odo = lw, odo
-        self.L, self.ctx = odo._lib, odo._ctx
-        self.events = []  # ("insert", xyz, attr, T) | ("reg", ds, T_start, T_gpu, frame)
-        self.thin_checks = []  # (deskewed xyz, attr, thinned set) of the frame's last registration
-        self.deskew_checks = []  # (raw, t, t_mid, xi, deskewed)
-        self.frames = 0
-        self.chained = None
-        inner = odo._frame_chain
-
-        def chain(t_mid):
-            out = inner(t_mid)
-            fo, fi = odo._fout, odo._fin
-            self.chained = dict(T_pred=np.array(fi.T_pred).reshape(4, 4), xi0=np.array(fi.xi, np.float32),
-                                t_mid=float(fi.t_mid), regs=[(np.array(r.T).reshape(4, 4), int(r.n_ds), bool(r.ok))
-                                                             for r in fo.reg[: int(fo.registrations)]],
-                                skipped=bool(fo.skipped), keyframe=bool(fo.keyframe), xi=np.array(fo.xi, np.float32))
-            return out
-
-        odo._frame_chain = chain
-        self.keyframes = 0
-
-    def _thin(self, m, xi, t_mid):
-        o = self.odo
-        x = np.ascontiguousarray(xi, np.float32)
-        assert self.L.lw_deskew(self.ctx, m, t_mid, 1.0 / o.frame_dt, x.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) == 0
-        n = (ctypes.c_int32 * 2)()
-        assert self.L.lw_thin(self.ctx, m, 2.0 / o.reg_voxel, o.min_range, o.max_range, int(o.ret_mask), n) == 0
-        return self._read_thin(int(n[0]))
-
-    def _read_thin(self, n):
-        ds = np.empty((n, 3), np.float32)
-        assert self.L.lw_thin_read(self.ctx, ds.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n) == 0
-        return ds
-
-    def on_frame(self, k, T, fx, fa, ft, ff, m):
-        o = self.odo
-        self.frames += 1
-        inserted = o.stats["keyframes"] > self.keyframes
-        self.keyframes = o.stats["keyframes"]
-        c, self.chained = self.chained, None
-        torch.cuda.synchronize()
-        xyz = fx[:m].cpu().numpy()
-        attr = fa[:m].cpu().numpy().view(np.uint32)
-        self._registrations(c, m, k, xyz, attr)
-        if inserted:  # the frame's insert follows its registrations
-            self.events.append(("insert", xyz, attr, T.copy()))
-
-    def _registrations(self, c, m, k, xyz, attr):
-        o = self.odo
-        if c is None or c["skipped"] or not c["regs"]:
-            return
-        raw, t = o.f_raw[:m].cpu().numpy(), o.f_t[:m].cpu().numpy()
-        last = self._read_thin(c["regs"][-1][1])
-        self.thin_checks.append((xyz, attr, last))
-        self.deskew_checks.append((raw, t, c["t_mid"], c["xi"], xyz))
-        sets = [last]
-        if len(c["regs"]) == 2:
-            sets = [self._thin(m, c["xi0"], c["t_mid"]), last]
-            # the same deskew and thinning with the final twist give the chain's last set back exactly
-            assert np.array_equal(self._thin(m, c["xi"], c["t_mid"]), last)
-        start = c["T_pred"]
-        for (Tg, n_ds, ok), ds in zip(c["regs"], sets):
-            assert len(ds) == n_ds
-            self.events.append(("reg", ds, start.copy(), Tg, k))
-            start = Tg
-
-
-def _replay(orc, events, P, slots):
-    """Every registration of `events` through the oracle, on replicas of the lock-step map run in threads (the
-    C accumulation releases the interpreter lock); replica 0 keeps the full map for the final comparison."""
-    regs = [i for i, e in enumerate(events) if e[0] == "reg"]
-    out = {}
-
-    def worker(r):
-        om = orc.OracleMap(slots if r == 0 else 1 << 20)
-        mine = set(regs[r::REPLICAS])
-        for i, e in enumerate(events):
-            if e[0] == "insert":
-                om.insert(e[1], e[2], e[3], P.reg_voxel, P.max_range, P.ret_mask)
-            elif i in mine:
-                res = orc.register(om, e[1], e[2], P)
-                out[i] = (res, _err(e[3], res.T))
-        return om
-
-    with ThreadPoolExecutor(REPLICAS) as ex:
-        maps = list(ex.map(worker, range(REPLICAS)))
-    return out, maps[0]
 
 
 @pytest.fixture(scope="module")

@@ -231,44 +231,6 @@ def test_plane_eval_counts_match_kdtree(lw, room):
     assert want[2] * 0.97 <= c[2] <= want[2] + 2
 
 
-def test_scan_prep_and_heights(lw, room):
-    """The scan's floor and ceiling points get vertical normals, and the height search (localize.py:426-446)
-    puts the sensor at its true 1.4 m above the floor among its first candidates, from the map's levels."""
-    S, prep, nn = room
-    lvl = lw.mapgrid.horizontal_levels(prep["xyz"], prep["normal"], prep["planarity"])
-    assert lvl[0].sum() > 0
-    T, pts = _room_scan(lw, S, n=20000, noise=0.01)
-    sp = lw.localize.ScanPrep(pts)
-    assert 0 < len(sp.pts) < len(pts) and np.allclose(np.linalg.norm(sp.normals[np.any(sp.normals, 1)], axis=1), 1)
-    up = T[:3, :3].T @ np.array([0.0, 0.0, 1.0])  # gravity in the sensor frame
-    horiz = np.abs(sp.normals @ up) > 0.97
-    z = sp.pts[horiz] @ up
-    # the floor 1.4 m below the sensor, the ceiling 1.6 m above
-    assert np.sum(np.abs(z + 1.4) < 0.05) > 100 and np.sum(np.abs(z - 1.6) < 0.05) > 100
-
-    class _Search:  # the parts of CoarseSearch the heights read
-        grid = type("G", (), {"levels": lvl, "lo": prep["lo"], "hi": prep["hi"]})
-        n_heights = 8
-
-    hs = lw.localize.CoarseSearch.heights(_Search(), sp, up)
-    print("heights", hs, "true", T[2, 3])
-    assert hs and any(abs(h - T[2, 3]) < 0.04 for h in hs[:3])
-    assert np.allclose(lw.localize.rot_to(up, np.array([0, 0, 1.0])) @ up, [0, 0, 1])
-
-
-def test_up_candidates_from_planes(lw, room):
-    """A scan from a sensor pitched 8 and rolled -5 degrees: the planes' horizontal family gives "up" in the
-    sensor frame within half a degree, first and then negated (the scanner may be upside down)."""
-    S, prep, nn = room
-    T = lw.synth.pose_matrix(np.array([2.5, 3.0, 1.4]), 0.3, np.radians(8.0), np.radians(-5.0))
-    xyz, _, _ = lw.synth.scan(S, lw.synth.Rosette(5), lambda t: T, 0.0, 20000, 9, noise=0.01)
-    ups = lw.localize.ScanPrep(xyz).up_candidates()
-    true_up = T[:3, :3].T @ np.array([0.0, 0.0, 1.0])
-    ang = np.degrees(np.arccos(np.clip(abs(float(ups[0][0] @ true_up)), -1, 1)))
-    print("up candidates", [(np.round(u, 3), s) for u, s in ups], "error", round(ang, 3))
-    assert ang < 0.5 and ups[0][1] > 0 and np.allclose(ups[1][0], -ups[0][0])
-
-
 def test_prepare_refuses_grids_the_kernels_cannot_take(lw):
     """A map 4 km long in y at 5 cm has more rows than the kernels index: a clear error."""
     xyz = np.array([[0.0, 0.0, 0.0], [1.0, 4000.0, 1.0]], np.float32)
