@@ -1,0 +1,150 @@
+"""Batched global localisation at full size: the 60 x 40 x 4 m multi-room floor sampled every 20 mm (about 20.5M
+map points) voxelised at 0.1 m into 2^24 slots, 4 seeded query scans of 5k-30k points, 4096 hypotheses each
+(x, y uniform over the floor, yaw uniform, z / roll / pitch at the truth), 10 Gauss-Newton iterations each, all in
+one persistent launch replayed from its captured graph (lw_loc_batch):
+
+  * the map build (prior_map.py:57-91 semantics with k_reg_insert's record) against the oracle's serial insert at
+    the same table size: the occupied slots bit for bit, every voxel's count exact, its moments within float32
+    summation order;
+  * 64 sampled (query, hypothesis) units -- the 8 best-scoring and 8 random ones of every query -- against the
+    oracle's loc_register on the same map: pose within 0.1 mm / 0.001 deg, steps exact, matched count within 2,
+    score within 1e-3;
+  * graph replays equal the eager launch bit for bit;
+  * top-1 recovery from the path that recovers: the distance-field coarse and medium search (PriorGrid at 4 cm, a
+    level sensor at its 1.4 m) hands 16 candidates a query to the same batched ICP; uniform hypotheses alone land
+    in no query's reach-1 basin (0.15 m), so their top-1 is reported, not gated.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+VOXEL = 0.1
+SLOTS = 1 << 24
+HYPS = 4096
+SEED = 31
+
+
+def _err(A, B):
+    d = np.linalg.inv(A) @ B
+    return float(np.linalg.norm(A[:3, 3] - B[:3, 3])), float(
+        math.degrees(math.acos(max(-1.0, min(1.0, (np.trace(d[:3, :3]) - 1) / 2)))))
+
+
+@pytest.fixture(scope="module")
+def c3(lw, orc):
+    W = lw.synth.loc_workload(seed=SEED, n_queries=4)
+    rng = np.random.default_rng(SEED + 7)
+    hyps = np.stack([lw.loc.hypotheses(rng, HYPS, W.T_true[q], (0.5, 0.5), (59.5, 39.5))
+                     for q in range(len(W.scans))])
+    print(f"C3: {len(W.map_xyz)} map points, scans {[len(s) for s in W.scans]}, {HYPS} hypotheses each", flush=True)
+    loc = lw.loc.BatchLocalizer(torch.from_numpy(W.map_xyz).cuda(), voxel=VOXEL, reg_slots=SLOTS, chunk=1 << 22)
+    tile, _ = loc.autotune([W.scans[0]], hyps[:1, :256])
+    staged = loc._stage(W.scans, hyps)
+    eager = [x.clone() for x in loc.run(W.scans, hyps, tile=tile, staged=staged)]
+    loc.stream.synchronize()
+    replays = []
+    for _ in range(2):
+        got = loc.run(W.scans, hyps, tile=tile, use_graph=True, staged=staged)
+        loc.stream.synchronize()
+        replays.append([x.clone() for x in got])
+        torch.cuda.synchronize()
+    gmap = loc.eng.read_map()
+    gstats = loc.map_stats()
+    om = orc.OracleMap(SLOTS)
+    om.insert(W.map_xyz, np.zeros(len(W.map_xyz), np.uint32), np.eye(4), VOXEL, max_range=1e30)
+    yield W, hyps, loc, tile, eager, replays, gmap, gstats, om
+    loc.close()
+
+
+def test_c3_map_build_matches_the_oracle(c3, gpu_ok):
+    W, hyps, loc, tile, eager, replays, gmap, gstats, om = c3
+    keys, rsum, router, rcnt = gmap
+    assert gstats == tuple(int(x) for x in om.stats) and gstats[1] == 0
+    assert np.array_equal(keys != -1, om.keys != -1)  # the same occupied slots in the same 2^24-slot table
+    occ = np.nonzero(keys != -1)[0]
+    og, oo = np.argsort(keys[occ]), np.argsort(om.keys[occ])
+    sg, so = occ[og], occ[oo]
+    assert np.array_equal(keys[sg], om.keys[so])
+    assert np.array_equal(rcnt[sg], om.rcnt[so])
+    c = om.rcnt[so].astype(np.float64)[:, None]
+    mean_err = float(np.abs(rsum[sg] / c - om.rsum[so] / c).max())
+    cov_g, cov_o = router[sg].reshape(-1, 9) / c, om.router[so].reshape(-1, 9) / c
+    cov_err = float((np.abs(cov_g - cov_o) / (np.abs(cov_o).max(axis=1, keepdims=True) + 1e-12)).max())
+    print(f"C3 map: {gstats[0]} voxels, occupancy and counts exact, means within {mean_err * 1e6:.3f} um, "
+          f"second moments within {cov_err:.2e} relative", flush=True)
+    assert gstats[0] > 700_000 and mean_err < 1e-5 * VOXEL and cov_err < 1e-4
+
+
+def test_c3_graph_replays_equal_eager(c3, gpu_ok):
+    W, hyps, loc, tile, eager, replays, gmap, gstats, om = c3
+    for rep in replays:
+        for a, b in zip(rep, eager):
+            assert torch.equal(a, b)
+
+
+def test_c3_sampled_units_match_the_oracle(c3, orc, gpu_ok):
+    from concurrent.futures import ThreadPoolExecutor
+
+    W, hyps, loc, tile, eager, replays, gmap, gstats, om = c3
+    T, S, C, I = (x.cpu().numpy() for x in eager)
+    rng = np.random.default_rng(5)
+    units = []
+    for q in range(len(W.scans)):
+        best = np.argsort(-S[q], kind="stable")[:8]
+        rest = rng.choice(np.setdiff1d(np.arange(HYPS), best), 8, replace=False)
+        units += [(q, int(h)) for h in np.concatenate([best, rest])]
+
+    def one(u):
+        q, h = u
+        return orc.loc_register(om, W.scans[q], hyps[q, h], VOXEL, iters=10)
+
+    with ThreadPoolExecutor(8) as ex:
+        want = list(ex.map(one, units))
+    worst, steps_same, counts = (0.0, 0.0), 0, 0
+    for (q, h), (Tw, sw, cw, iw) in zip(units, want):
+        e = _err(Tw, T[q, h])
+        worst = (max(worst[0], e[0]), max(worst[1], e[1]))
+        assert e[0] < 1e-4 and e[1] < 1e-3, (q, h, e)
+        assert I[q, h] == iw, (q, h, I[q, h], iw)
+        assert abs(int(C[q, h]) - cw) <= 2 and abs(S[q, h] - sw) <= 1e-3 * max(1.0, abs(sw)), (q, h)
+        counts += int(C[q, h]) >= 30
+    print(f"C3 units: {len(units)} (q, h) vs the oracle, worst {worst[0] * 1e3:.5f} mm / {worst[1]:.6f} deg, "
+          f"{counts} with 30+ matches; tile {tile}", flush=True)
+    assert counts >= 32
+
+
+def test_c3_top1_from_coarse_candidates(c3, lw, gpu_ok):
+    W, hyps, loc, tile, eager, replays, gmap, gstats, om = c3
+    T, S, _, _ = (x.cpu().numpy() for x in eager)
+    uniform = [_err(W.T_true[q], T[q, int(np.argmax(S[q]))]) for q in range(len(W.scans))]
+    mxyz, mnrm = lw.synth.sample_surfaces(W.scene, 0.04, SEED + 1, prune=False, normals=True)
+    grid = lw.PriorGrid({"xyz": mxyz, "normal": mnrm}, "cuda:0")
+    grid.build()
+    cs = lw.CoarseSearch(grid, "cuda:0")
+    rng = np.random.default_rng(SEED + 11)
+    cands = []
+    for scan in W.scans:
+        pts = np.asarray(scan, np.float64)
+        _, first = np.unique(np.floor(pts / 0.05).astype(np.int64), axis=0, return_index=True)
+        pts = pts[np.sort(first)]
+        _, f2 = np.unique(np.floor(pts / 0.15).astype(np.int64), axis=0, return_index=True)
+        spread = pts[np.sort(f2)]
+        sub_c = spread[rng.choice(len(spread), min(cs.n_coarse_pts, len(spread)), replace=False)]
+        sub_m = pts[rng.choice(len(pts), min(cs.n_mid_pts, len(pts)), replace=False)]
+        Ts, _, _ = cs.search(sub_c, sub_m, [1.4])
+        Ts = np.concatenate([Ts, np.repeat(Ts[-1:], max(0, cs.n_icp - len(Ts)), 0)])[:cs.n_icp]
+        cands.append(Ts)
+    out = loc.run(W.scans, np.stack(cands))
+    loc.stream.synchronize()
+    Tc, Sc = out[0].cpu().numpy(), out[1].cpu().numpy()
+    found = [_err(W.T_true[q], Tc[q, int(np.argmax(Sc[q]))]) for q in range(len(W.scans))]
+    ok = sum(1 for e in found if e[0] < 0.3 and e[1] < 3.0)
+    print(f"C3 top-1: uniform hypotheses {sum(1 for e in uniform if e[0] < 0.3 and e[1] < 3.0)} of 4 "
+          f"{[(round(a, 2), round(b, 1)) for a, b in uniform]}; coarse candidates into the batched ICP {ok} of 4 "
+          f"{[(round(a, 3), round(b, 2)) for a, b in found]}", flush=True)
+    assert ok >= 3
