@@ -123,6 +123,12 @@ int lw_map_stats(lw_ctx *ctx, int32_t *out);
 /* The map in the reference's per-slot layout (odom.py:470-473), copied to host memory: keys[cap],
  * rsum[cap*3], router[cap*9], rcnt[cap]. Any pointer may be NULL. */
 int lw_map_read(lw_ctx *ctx, int64_t *keys, float *rsum, float *router, int32_t *rcnt);
+/* The map's voxels in the order they were claimed (a voxel keeps its place until the map is reset), in the
+ * reference's record layout: keys[v], rsum[v*3], router[v*9], rcnt[v] for the first min(voxels, cap); *count
+ * receives the number of voxels. Any array may be NULL. Waits for the context's stream. No counterpart in the
+ * reference (a compact form of lw_map_read: V records instead of one per slot). */
+int lw_map_read_voxels(lw_ctx *ctx, int32_t cap, int64_t *keys, float *rsum, float *router, int32_t *rcnt,
+                       int32_t *count);
 /* Slot of a key, or -1 (gpu.find_slot, gpu.py:254-265), evaluated on the device for n keys. */
 int lw_map_find(lw_ctx *ctx, const int64_t *keys_host, int32_t n, int32_t *slots_host);
 

@@ -82,6 +82,7 @@ SIGNATURES = {
     "lw_map_stats": (_i32, [_vp, _i32p]),
     "lw_map_read": (_i32, [_vp, _i64p, _f32p, _f32p, _i32p]),
     "lw_map_find": (_i32, [_vp, _i64p, _i32, _i32p]),
+    "lw_map_read_voxels": (_i32, [_vp, _i32, _i64p, _f32p, _f32p, _i32p, _i32p]),
     "lw_icp_accum": (_i32, [_vp, _i32, _f32p, _f32, _f32, _i32, _f32, _f32, _f32, _f32, _f32, _i32, _f64p, _i32p]),
     "lw_register": (_i32, [_vp, _i32, ctypes.POINTER(RegParams), _f64p, ctypes.POINTER(RegResult)]),
     "lw_register_thinned": (_i32, [_vp, _i32, ctypes.POINTER(RegParams), _f64p, ctypes.POINTER(RegResult)]),

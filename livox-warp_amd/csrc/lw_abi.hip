@@ -345,6 +345,32 @@ int lw_map_read(lw_ctx *c, int64_t *keys, float *rsum, float *router, int32_t *r
     return 0;
 }
 
+int lw_map_read_voxels(lw_ctx *c, int32_t cap, int64_t *keys, float *rsum, float *router, int32_t *rcnt,
+                       int32_t *count) {
+    if (!c || cap < 0 || !count) return LW_EINVAL;
+    int32_t nvox = 0;
+    if (int rc = read_ctr(c, 2, &nvox)) return rc;  // after everything queued on the context's stream
+    if (nvox > c->vox_cap) nvox = (int32_t)c->vox_cap;
+    *count = nvox;
+    const int32_t n = nvox < cap ? nvox : cap;
+    if (n == 0) return 0;
+    std::vector<int64_t> k(n);
+    std::vector<lw::VoxMom> mom(n);
+    LW_CK(hipMemcpyAsync(k.data(), c->vox_key, n * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    LW_CK(hipMemcpyAsync(mom.data(), c->vox_mom, n * sizeof(lw::VoxMom), hipMemcpyDeviceToHost, c->stream));
+    LW_CK(hipStreamSynchronize(c->stream));
+    const int map3[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+    for (int32_t v = 0; v < n; ++v) {
+        if (keys) keys[v] = k[v];
+        if (rsum)
+            for (int a = 0; a < 3; ++a) rsum[3 * (int64_t)v + a] = mom[v].s[a];
+        if (router)
+            for (int a = 0; a < 9; ++a) router[9 * (int64_t)v + a] = mom[v].o[map3[a]];
+        if (rcnt) rcnt[v] = mom[v].cnt;
+    }
+    return 0;
+}
+
 int lw_map_find(lw_ctx *c, const int64_t *keys_host, int32_t n, int32_t *slots_host) {
     if (!c || n < 0 || n > c->frame_cap || (n > 0 && (!keys_host || !slots_host))) return LW_EINVAL;
     if (n == 0) return 0;

@@ -1,6 +1,7 @@
 """Helpers for the GPU tests that check whole odometry runs against the oracle registration by registration:
-a recorder that watches an Odometry on its default (chain) path and keeps what each registration ran on, and a
-threaded replay of those registrations through the oracle on lock-step replicas of the map."""
+a recorder that watches an Odometry on its default (chain) path and keeps what each registration ran on -- its
+thinned points, its start pose and the device's map as it stood (read back after every insert) -- and a threaded
+replay of those registrations through the oracle, plus the oracle's own serial build of the same inserts."""
 
 import ctypes
 from concurrent.futures import ThreadPoolExecutor
@@ -107,6 +108,20 @@ class Recorder:
         self._registrations(c, m, k, xyz, attr)
         if inserted:  # the frame's insert follows its registrations
             self.events.append(("insert", xyz, attr, T.copy()))
+            self.events.append(("map",) + self.read_voxels())
+
+    def read_voxels(self):
+        """The device map's voxels in claim order (lw_map_read_voxels): keys, rsum, router, rcnt."""
+        cnt = (ctypes.c_int32 * 1)()
+        assert self.L.lw_map_read_voxels(self.ctx, 0, None, None, None, None, cnt) == 0
+        V = int(cnt[0])
+        keys, rsum = np.empty(V, np.int64), np.empty((V, 3), np.float32)
+        router, rcnt = np.empty((V, 3, 3), np.float32), np.empty(V, np.int32)
+        P = ctypes.POINTER
+        assert self.L.lw_map_read_voxels(self.ctx, V, keys.ctypes.data_as(P(ctypes.c_int64)),
+                                         rsum.ctypes.data_as(P(ctypes.c_float)), router.ctypes.data_as(P(ctypes.c_float)),
+                                         rcnt.ctypes.data_as(P(ctypes.c_int32)), cnt) == 0
+        return keys, rsum, router, rcnt
 
     def _registrations(self, c, m, k, xyz, attr):
         o = self.odo
@@ -128,23 +143,58 @@ class Recorder:
             start = Tg
 
 
+class SnapshotMap:
+    """An oracle map that takes the device's map as it stands: new voxels are placed by the reference's linear probe
+    from their home slots (voxels keep their claim order, so only the newly claimed ones move in), and every
+    voxel's moments are copied in."""
+
+    def __init__(self, orc, slots: int):
+        self.orc = orc
+        self.om = orc.OracleMap(slots)
+        self.slot_of = np.zeros(0, np.int64)
+
+    def load(self, keys, rsum, router, rcnt):
+        om, old = self.om, len(self.slot_of)
+        mask = om.cap - 1
+        homes = self.orc.slot_hash_np(keys[old:]).astype(np.int64) & mask
+        new = np.empty(len(keys) - old, np.int64)
+        for j, (key, h) in enumerate(zip(keys[old:], homes)):
+            s = int(h)
+            while om.keys[s] != -1:
+                s = (s + 1) & mask
+            om.keys[s] = key
+            new[j] = s
+        self.slot_of = np.concatenate([self.slot_of, new])
+        om.rsum[self.slot_of] = rsum
+        om.router[self.slot_of] = router
+        om.rcnt[self.slot_of] = rcnt
+
+
 def replay(orc, events, P, slots):
-    """Every registration of `events` through the oracle, on replicas of the lock-step map run in threads (the
-    C accumulation releases the interpreter lock); replica 0 keeps the full map for the final comparison."""
+    """Every registration of `events` through the oracle on the device's own map at that point (replicas in
+    threads: the C accumulation releases the interpreter lock), and the oracle's serial build of the same
+    inserts (the map-build comparison). Returns ({event index: (oracle result, (m, deg) from the device's)},
+    the serial build)."""
     regs = [i for i, e in enumerate(events) if e[0] == "reg"]
     out = {}
 
     def worker(r):
-        om = orc.OracleMap(slots if r == 0 else 1 << 20)
+        if r == REPLICAS:  # the serial build of the device's inserts
+            om = orc.OracleMap(slots)
+            for e in events:
+                if e[0] == "insert":
+                    om.insert(e[1], e[2], e[3], P.reg_voxel, P.max_range, P.ret_mask)
+            return om
+        sm = SnapshotMap(orc, 1 << 20)
         mine = set(regs[r::REPLICAS])
         for i, e in enumerate(events):
-            if e[0] == "insert":
-                om.insert(e[1], e[2], e[3], P.reg_voxel, P.max_range, P.ret_mask)
+            if e[0] == "map":
+                sm.load(*e[1:])
             elif i in mine:
-                res = orc.register(om, e[1], e[2], P)
+                res = orc.register(sm.om, e[1], e[2], P)
                 out[i] = (res, err(e[3], res.T))
-        return om
+        return None
 
-    with ThreadPoolExecutor(REPLICAS) as ex:
-        maps = list(ex.map(worker, range(REPLICAS)))
-    return out, maps[0]
+    with ThreadPoolExecutor(REPLICAS + 1) as ex:
+        maps = list(ex.map(worker, range(REPLICAS + 1)))
+    return out, maps[REPLICAS]

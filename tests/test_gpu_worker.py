@@ -228,40 +228,54 @@ def test_snapshot_detector_negative_control(lw, gpu_ok):
 def test_worker_generations_and_drops(lw, gpu_ok):
     seq, batches = _batches(lw, seconds=2.0)
     w = _worker(lw, max_batches=2, out_slots=1)
+
+    def up(xyz, attr, t):
+        return torch.from_numpy(xyz).cuda(), torch.from_numpy(attr).cuda(), torch.from_numpy(t).cuda()
+
     try:
+        # a full queue: the worker falls behind a caller that does not wait, and the queue of 2 fills
         w.reset(seq.pose(0.05))
+        fast = w.odom.push
+        w.odom.push = lambda *a, **k: (time.sleep(0.02), fast(*a, **k))[1]
         sent = 0
-        for xyz, attr, t in batches:  # faster than the worker: the queue of 2 fills
-            sx, sa, st = torch.from_numpy(xyz).cuda(), torch.from_numpy(attr).cuda(), torch.from_numpy(t).cuda()
-            sent += bool(w.submit(sx, sa, st, len(t), float(t[0]), float(t[-1])))
+        for xyz, attr, t in batches:
+            sent += bool(w.submit(*up(xyz, attr, t), len(t), float(t[0]), float(t[-1])))
         _wait_idle(w)
+        del w.odom.push
         st = w.status()
-        print(st)
+        print(st, flush=True)
         assert st["drops"]["input_full"] > 0 and st["drops"]["input_full"] + sent == len(batches)
         assert st["processed"] == sent
-        # one output slot and nothing released: the second frame finds the pool empty
+        # an exhausted output pool: one output slot, nothing released, every batch let through
+        w.reset(seq.pose(0.05))
+        for xyz, attr, t in batches:
+            assert w.submit(*up(xyz, attr, t), len(t), float(t[0]), float(t[-1]))
+            _wait_idle(w)
+        st = w.status()
+        print(st, flush=True)
         assert st["drops"]["output_pool"] > 0
         res = w.poll()
-        assert res and all(r.gen == 1 for r in res)
+        assert res and all(r.gen == 2 for r in res) and sum(r.snaps is not None for r in res) == 1
         w.release_consumed()
-        # a reset: results still queued for generation 1 are dropped, the next frames start from the new pose
+        # a reset: results still queued for generation 2 are dropped, the next frames start from the new pose
         T0 = seq.pose(0.05).copy()
         T0[:3, 3] += np.array([10.0, 0.0, 0.0])
+        for xyz, attr, t in batches[:12]:  # leave results of generation 2 queued
+            w.submit(*up(xyz, attr, t), len(t), float(t[0]), float(t[-1]))
         w.reset(T0)
         for xyz, attr, t in batches[:6]:
-            sx, sa, st_ = torch.from_numpy(xyz).cuda(), torch.from_numpy(attr).cuda(), torch.from_numpy(t).cuda()
-            w.submit(sx, sa, st_, len(t), float(t[0]), float(t[-1]))
+            w.submit(*up(xyz, attr, t), len(t), float(t[0]), float(t[-1]))
             _wait_idle(w)
         res = w.poll()
-        assert res and all(r.gen == 2 for r in res)
+        assert res and all(r.gen == 3 for r in res)
         assert np.abs(res[0].T[:3, 3] - T0[:3, 3]).max() < 0.5  # the frame after the reset is posed near T0
         w.release_consumed()
         with pytest.raises(AttributeError):
             w.configure(no_such_tuning=1)
         # a batch larger than a staging slot is dropped and counted, not truncated
-        big = torch.zeros((w.batch_cap + 1, 3), device="cuda:0")
-        assert not w.submit(big, torch.zeros(w.batch_cap + 1, dtype=torch.int32, device="cuda:0"),
-                            torch.zeros(w.batch_cap + 1, device="cuda:0"), w.batch_cap + 1, 0.0, 0.0)
+        n = w.batch_cap + 1
+        assert not w.submit(torch.zeros((n, 3), device="cuda:0"), torch.zeros(n, dtype=torch.int32, device="cuda:0"),
+                            torch.zeros(n, device="cuda:0"), n, 0.0, 0.0)
         assert w.status()["drops"]["input_own"] == 1
     finally:
         w.close()
