@@ -29,6 +29,7 @@ extern "C" {
 #define LW_ENOTBOUND (-1001)
 #define LW_ELOOP (-1002) /* the persistent registration loop stopped waiting (a bounded wait ran out) */
 #define LW_ENOLOOP (-1003) /* no cooperative launch on this device: drive the frame step by step instead */
+#define LW_ERANGE (-1004)  /* a size beyond what an allocation-free entry point's fixed scratch holds */
 #define LW_ACC_STRIDE 64
 
 typedef struct lw_ctx lw_ctx;
@@ -374,6 +375,17 @@ int64_t lw_plane_work_bytes(int32_t k);
 int lw_plane_icp(const float *pts, int32_t n_pts, const float *T, int32_t k, const float *geo, const int32_t *dims,
                  const int32_t *start, const int32_t *nn, const float *xyz, const float *nrm, const float *planar,
                  float max_d2, float c2, float alpha, void *work, int64_t work_bytes, double *out, void *stream);
+/* lw_plane_icp with no workspace argument and no allocation of any kind: the partial rows live in a fixed device
+ * array of the library (one per GPU, 16 KiB a pose, LW_PLANE_NOALLOC_MAX poses: 4 MiB, held until the process
+ * ends). No counterpart in the reference: it is for callers that cannot allocate -- code captured into a HIP
+ * graph, tight serving loops, harnesses that forbid hipMalloc. k above LW_PLANE_NOALLOC_MAX returns LW_ERANGE.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU (two streams, two threads) race,
+ * so a caller serialises them. The same results as lw_plane_icp, bit for bit. */
+#define LW_PLANE_NOALLOC_MAX 256
+int lw_plane_icp_noalloc(const float *pts, int32_t n_pts, const float *T, int32_t k, const float *geo,
+                         const int32_t *dims, const int32_t *start, const int32_t *nn, const float *xyz,
+                         const float *nrm, const float *planar, float max_d2, float c2, float alpha, double *out,
+                         void *stream);
 /* Per pose, the scan points whose nearest map point lies within radii[0..2] (host array): counts (device, k x 3
  * int32, zeroed by the call). Replaces k_eval_batch (localize.py:183-206). */
 int lw_plane_eval(const float *pts, int32_t n_pts, const float *T, int32_t k, const float *geo, const int32_t *dims,

@@ -16,6 +16,8 @@ LW_EINVAL = -1000
 LW_ENOTBOUND = -1001
 LW_ELOOP = -1002
 LW_ENOLOOP = -1003
+LW_ERANGE = -1004
+LW_PLANE_NOALLOC_MAX = 256  # poses lw_plane_icp_noalloc takes at once
 
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -126,6 +128,8 @@ SIGNATURES = {
     "lw_grid_work_bytes": (_i64, [_i32p]),
     "lw_grid_build": (_i32, [_vp, _i32, _f32p, _i32p, _i32p, _i32, _vp, _vp, _i64, _vp, _vp]),
     "lw_plane_work_bytes": (_i64, [_i32]),
+    "lw_plane_icp_noalloc": (_i32, [_vp, _i32, _vp, _i32, _f32p, _i32p, _vp, _vp, _vp, _vp, _vp, _f32, _f32, _f32, _vp,
+                                    _vp]),
     "lw_plane_icp": (_i32, [_vp, _i32, _vp, _i32, _f32p, _i32p, _vp, _vp, _vp, _vp, _vp, _f32, _f32, _f32, _vp, _i64, _vp,
                              _vp]),
     "lw_plane_eval": (_i32, [_vp, _i32, _vp, _i32, _f32p, _i32p, _vp, _vp, _vp, _f32p, _vp, _vp]),
@@ -166,6 +170,8 @@ def check(rc: int, what: str):
             raise LwError(f"{what}: the persistent registration loop stopped waiting")
         if rc == LW_ENOLOOP:
             raise LwError(f"{what}: no cooperative launch on this device")
+        if rc == LW_ERANGE:
+            raise LwError(f"{what}: size beyond the fixed scratch of an allocation-free entry point")
         raise LwError(f"{what}: HIP error {-rc}")
 
 

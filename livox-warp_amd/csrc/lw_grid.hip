@@ -19,6 +19,7 @@
 //    scores are exact, not merely close.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <utility>
 
 #include "../../include/livox_warp_amd.h"
@@ -618,6 +619,54 @@ extern "C" int lw_plane_icp(const float *pts, int32_t n_pts, const float *T, int
         return LW_EINVAL;
     hipStream_t s = (hipStream_t)stream;
     double *partials = static_cast<double *>(work);
+    hipLaunchKernelGGL(lw::k_plane_icp, dim3(lw::PLANE_CHUNKS, (unsigned)k), dim3(256), 0, s, pts, n_pts, T, g, start,
+                       nn, xyz, nrm, planar, max_d2, c2, alpha, partials);
+    hipLaunchKernelGGL(lw::k_plane_reduce, dim3((unsigned)k), dim3(64), 0, s, partials, out);
+    return lw::launched();
+}
+
+// ---- lw_plane_icp without a workspace argument ---------------------------------------------------------------
+// The partial rows live in a file-scope device array, one copy per GPU for the life of the process. Host code
+// never passes the array's name to a kernel (on the host it is a stand-in, not a device address): the entry point
+// looks up each device's address once, with that device current, and passes that.
+static __device__ __align__(256) double scratch[(size_t)LW_PLANE_NOALLOC_MAX * lw::PLANE_CHUNKS * 32];
+
+namespace {
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[SCRATCH_DEVICES] = {};  // per device ordinal, filled on the first call there
+
+int scratch_of_current_device(double **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return -(int)e;
+    if (dev < 0 || dev >= SCRATCH_DEVICES) return LW_EINVAL;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return e != hipSuccess ? -(int)e : LW_EINVAL;
+        g_scratch[dev] = p;
+    }
+    *out = static_cast<double *>(g_scratch[dev]);
+    return 0;
+}
+}  // namespace
+
+extern "C" int lw_plane_icp_noalloc(const float *pts, int32_t n_pts, const float *T, int32_t k, const float *geo,
+                                    const int32_t *dims, const int32_t *start, const int32_t *nn, const float *xyz,
+                                    const float *nrm, const float *planar, float max_d2, float c2, float alpha,
+                                    double *out, void *stream) {
+    GridGeo g;
+    if (!lw::geo_of(geo, dims, g) || n_pts < 0 || k < 0 || !(c2 > 0.0f)) return LW_EINVAL;
+    if (k > LW_PLANE_NOALLOC_MAX) return LW_ERANGE;
+    if (k == 0) return 0;
+    if ((n_pts > 0 && !pts) || !T || !start || !nn || !xyz || !nrm || !planar || !out) return LW_EINVAL;
+    double *partials = nullptr;
+    if (int rc = scratch_of_current_device(&partials)) return rc;
+    // every partial row's 30 sums are written by its chunk's workgroup before the reduction reads them, so the
+    // scratch needs no clearing between calls
+    hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(lw::k_plane_icp, dim3(lw::PLANE_CHUNKS, (unsigned)k), dim3(256), 0, s, pts, n_pts, T, g, start,
                        nn, xyz, nrm, planar, max_d2, c2, alpha, partials);
     hipLaunchKernelGGL(lw::k_plane_reduce, dim3((unsigned)k), dim3(64), 0, s, partials, out);
