@@ -169,6 +169,33 @@ typedef struct {
 
 int lw_odom_frame(lw_ctx *ctx, const lw_reg_params *p, const lw_frame_in *in, lw_frame_out *out);
 
+/* ---- The batched odometry: one frame of each of S independent sequences per call. No counterpart in the
+ * reference (it runs one Odometry per stream); each sequence's frame goes through lw_odom_frame's chain --
+ * deskew, thinning, registration, the re-deskew test and second registration, the keyframe test, the insert into
+ * that sequence's own map -- but every step is one launch for the whole batch (a sequence's registration runs in
+ * one workgroup), with one upload and one readback. The S contexts keep their maps and buffers (any per-context
+ * entry point can read them between calls); they must be bound and share the batch's stream. */
+typedef struct lw_batch lw_batch;
+/* A sequence's frame: whole (its m points already cut from the stream, m <= 32768 and <= frame_cap), device
+ * pointers to its points (float32 x 3), attribute bits and times; warm: a warm-up frame, inserted at T_pred
+ * unregistered (odom.py:669-670). */
+typedef struct {
+    const float *xyz;
+    const uint32_t *attr;
+    const float *t;
+    int32_t m;
+    int32_t warm;
+} lw_batch_src;
+int lw_batch_create(lw_batch **out, lw_ctx *const *ctxs, int32_t S, void *stream);
+int lw_batch_destroy(lw_batch *b);
+/* in[s], src[s] -> out[s] for every sequence, as lw_odom_frame (the warm-up frames' registrations count 0). */
+int lw_batch_frame(lw_batch *b, const lw_reg_params *p, const lw_frame_in *in, const lw_batch_src *src,
+                   lw_frame_out *out);
+/* The batch's registration launches measured as lw_prof_* measures a context's (out: 8 doubles). */
+int lw_batch_prof_enable(lw_batch *b, int32_t on);
+int lw_batch_prof_reset(lw_batch *b);
+int lw_batch_prof_read(lw_batch *b, double *out);
+
 /* Odometry._register (odom.py:752-803): thin the m deskewed frame points, then run the Gauss-Newton
  * loop on the device -- accumulation, float64 host step (odom.py:387-455) and convergence test -- with
  * one readback at the end. T_pred: the prediction (float64 4x4). m < 0: the open frame's count on the

@@ -54,6 +54,12 @@ class FrameOut(ctypes.Structure):
                 ("map_overflow", _i32), ("pad", _i32)]
 
 
+class BatchSrc(ctypes.Structure):
+    # device addresses as 64-bit integers (the same layout as the C pointers), so numpy can view arrays of these
+    _fields_ = [("xyz", ctypes.c_uint64), ("attr", ctypes.c_uint64), ("t", ctypes.c_uint64), ("m", _i32),
+                ("warm", _i32)]
+
+
 class LocParams(ctypes.Structure):
     _fields_ = [("damping", _f64), ("reg_voxel", _f32), ("noise", _f32), ("alpha", _f32), ("iters", _i32),
                 ("min_count", _i32), ("block", _i32), ("grid", _i32), ("use_graph", _i32)]
@@ -85,6 +91,12 @@ SIGNATURES = {
     "lw_map_read": (_i32, [_vp, _i64p, _f32p, _f32p, _i32p]),
     "lw_map_find": (_i32, [_vp, _i64p, _i32, _i32p]),
     "lw_map_read_voxels": (_i32, [_vp, _i32, _i64p, _f32p, _f32p, _i32p, _i32p]),
+    "lw_batch_create": (_i32, [_vp, _vp, _i32, _vp]),
+    "lw_batch_destroy": (_i32, [_vp]),
+    "lw_batch_frame": (_i32, [_vp, _vp, _vp, _vp, _vp]),
+    "lw_batch_prof_enable": (_i32, [_vp, _i32]),
+    "lw_batch_prof_reset": (_i32, [_vp]),
+    "lw_batch_prof_read": (_i32, [_vp, _f64p]),
     "lw_icp_accum": (_i32, [_vp, _i32, _f32p, _f32, _f32, _i32, _f32, _f32, _f32, _f32, _f32, _i32, _f64p, _i32p]),
     "lw_register": (_i32, [_vp, _i32, ctypes.POINTER(RegParams), _f64p, ctypes.POINTER(RegResult)]),
     "lw_register_thinned": (_i32, [_vp, _i32, ctypes.POINTER(RegParams), _f64p, ctypes.POINTER(RegResult)]),

@@ -884,3 +884,314 @@ int lw_host_twist_between(const double *T_from, const double *T_to, double *xi) 
 double lw_host_cond(const double *H) { return H ? lw::gn::cond_of(H) : -1.0; }
 
 }  // extern "C"
+
+// ---- the batched odometry: one frame of each of S sequences per call ----------------------------------------
+
+namespace {
+constexpr size_t al256(size_t n) { return (n + 255) & ~(size_t)255; }
+constexpr size_t TP_BYTES = 16 * sizeof(double) + 64 * sizeof(float);  // a prediction and its kappa schedule
+constexpr int BATCH_EVENTS = 1024;                                      // timed registration launches in flight
+}  // namespace
+
+struct lw_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int32_t S = 0;
+    std::vector<lw_ctx *> ctx;
+    int64_t frame_cap = 0;  // the smallest of the contexts'
+    // one device block: [prep1 | prep2 | ins | tp | fc] uploaded with every frame, [fc | st1 | st | counts] read
+    // back, then the descriptors that never change [reg1 | reg2 | key]
+    char *dev = nullptr;
+    size_t o_prep2 = 0, o_ins = 0, o_tp = 0, o_fc = 0, o_st1 = 0, o_st = 0, o_counts = 0, o_reg1 = 0, o_reg2 = 0,
+           o_key = 0, o_end = 0;
+    char *pin = nullptr;  // pinned mirror of [prep1 .. counts]
+    unsigned long long *prof_dev = nullptr;
+    bool prof_on = false;
+    hipEvent_t *ev = nullptr;
+    int ev_used = 0;
+    double prof_ms = 0.0;
+    int64_t prof_launches = 0;
+};
+
+namespace {
+
+template <class T>
+T *at(lw_batch *b, size_t off) {
+    return reinterpret_cast<T *>(b->dev + off);
+}
+template <class T>
+T *pin_at(lw_batch *b, size_t off) {
+    return reinterpret_cast<T *>(b->pin + off);
+}
+
+void batch_free(lw_batch *b) {
+    if (b->dev) (void)hipFree(b->dev);
+    if (b->pin) (void)hipHostFree(b->pin);
+    if (b->prof_dev) (void)hipFree(b->prof_dev);
+    if (b->ev) {
+        for (int i = 0; i < 2 * BATCH_EVENTS; ++i) (void)hipEventDestroy(b->ev[i]);
+        delete[] b->ev;
+    }
+}
+
+hipError_t batch_harvest(lw_batch *b) {
+    for (int i = 0; i < b->ev_used; ++i) {
+        float ms = 0.0f;
+        hipError_t e = hipEventElapsedTime(&ms, b->ev[2 * i], b->ev[2 * i + 1]);
+        if (e != hipSuccess) return e;
+        b->prof_ms += ms;
+        b->prof_launches++;
+    }
+    b->ev_used = 0;
+    return hipSuccess;
+}
+
+hipError_t batch_register(lw_batch *b, size_t o_reg, const lw::RegLoop &L, int acc_mode) {
+    const bool timed = b->prof_on && b->ev_used < BATCH_EVENTS;
+    hipError_t e;
+    if (timed && (e = hipEventRecord(b->ev[2 * b->ev_used], b->stream)) != hipSuccess) return e;
+    if ((e = lw::launch_register_seq(b->stream, at<lw::SeqReg>(b, o_reg), b->S, L, acc_mode,
+                                     b->prof_on ? b->prof_dev : nullptr)) != hipSuccess)
+        return e;
+    if (timed) {
+        if ((e = hipEventRecord(b->ev[2 * b->ev_used + 1], b->stream)) != hipSuccess) return e;
+        b->ev_used++;
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lw_batch_create(lw_batch **out, lw_ctx *const *ctxs, int32_t S, void *stream) {
+    if (!out || !ctxs || S <= 0) return LW_EINVAL;
+    *out = nullptr;
+    auto *b = new (std::nothrow) lw_batch();
+    if (!b) return LW_EINVAL;
+    b->stream = (hipStream_t)stream;
+    b->S = S;
+    b->ctx.assign(ctxs, ctxs + S);
+    b->frame_cap = INT64_MAX;
+    hipError_t e = hipGetDevice(&b->device);
+    for (int32_t s = 0; s < S && e == hipSuccess; ++s) {
+        lw_ctx *c = ctxs[s];
+        if (!c || c->stream != b->stream || c->device != b->device || !c->f_xyz || c->ds_cap < 4096) {
+            delete b;
+            return c && !c->f_xyz ? LW_ENOTBOUND : LW_EINVAL;
+        }
+        b->frame_cap = c->frame_cap < b->frame_cap ? c->frame_cap : b->frame_cap;
+    }
+    if (e != hipSuccess) {
+        delete b;
+        return -(int)e;
+    }
+    const size_t n = (size_t)S;
+    b->o_prep2 = al256(n * sizeof(lw::PrepArgs));
+    b->o_ins = b->o_prep2 + al256(n * sizeof(lw::PrepArgs));
+    b->o_tp = b->o_ins + al256(n * sizeof(lw::InsSeq));
+    b->o_fc = b->o_tp + al256(n * TP_BYTES);
+    b->o_st1 = b->o_fc + al256(n * sizeof(lw::FrameCtl));
+    b->o_st = b->o_st1 + al256(n * sizeof(lw::RegState));
+    b->o_counts = b->o_st + al256(n * sizeof(lw::RegState));
+    b->o_reg1 = b->o_counts + al256(n * 2 * sizeof(int32_t));
+    b->o_reg2 = b->o_reg1 + al256(n * sizeof(lw::SeqReg));
+    b->o_key = b->o_reg2 + al256(n * sizeof(lw::SeqReg));
+    b->o_end = b->o_key + al256(n * sizeof(lw::KeySeq));
+    auto fail = [&](hipError_t err) {
+        batch_free(b);
+        delete b;
+        return -(int)err;
+    };
+    if ((e = hipMalloc((void **)&b->dev, b->o_end)) != hipSuccess) return fail(e);
+    if ((e = hipHostMalloc((void **)&b->pin, b->o_reg1, hipHostMallocDefault)) != hipSuccess) return fail(e);
+    if ((e = hipMemsetAsync(b->dev, 0, b->o_end, b->stream)) != hipSuccess) return fail(e);
+    // the descriptors that never change: each sequence's two registrations and its keyframe decision
+    std::vector<char> fixed(b->o_end - b->o_reg1);
+    auto *reg1 = reinterpret_cast<lw::SeqReg *>(fixed.data());
+    auto *reg2 = reinterpret_cast<lw::SeqReg *>(fixed.data() + (b->o_reg2 - b->o_reg1));
+    auto *key = reinterpret_cast<lw::KeySeq *>(fixed.data() + (b->o_key - b->o_reg1));
+    for (int32_t s = 0; s < S; ++s) {
+        lw_ctx *c = ctxs[s];
+        lw::FrameCtl *fc = at<lw::FrameCtl>(b, b->o_fc) + s;
+        lw::RegState *st = at<lw::RegState>(b, b->o_st) + s;
+        reg1[s] = lw::SeqReg{c->ds_xyz, &c->ctr[5], c->slots, c->occ, c->vox_prec, st, &fc->run1, c->mask, 0};
+        reg2[s] = lw::SeqReg{c->ds_xyz, &c->ctr[5], c->slots, c->occ, c->vox_prec, st, &fc->redo, c->mask, 0};
+        key[s] = lw::KeySeq{st, fc, c->ctr, at<int32_t>(b, b->o_counts) + 2 * s};
+    }
+    if ((e = hipMemcpyAsync(b->dev + b->o_reg1, fixed.data(), fixed.size(), hipMemcpyHostToDevice, b->stream)) !=
+        hipSuccess)
+        return fail(e);
+    if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return fail(e);
+    *out = b;
+    return 0;
+}
+
+int lw_batch_destroy(lw_batch *b) {
+    if (!b) return LW_EINVAL;
+    (void)hipStreamSynchronize(b->stream);
+    batch_free(b);
+    delete b;
+    return 0;
+}
+
+int lw_batch_frame(lw_batch *b, const lw_reg_params *p, const lw_frame_in *in, const lw_batch_src *src,
+                   lw_frame_out *out) {
+    if (!b || bad_params(p) || !in || !src || !out) return LW_EINVAL;
+    const int32_t S = b->S;
+    for (int32_t s = 0; s < S; ++s)
+        if (src[s].m < 0 || src[s].m > b->frame_cap || src[s].m > lw::frame_prep_max() ||
+            (src[s].m > 0 && (!src[s].xyz || !src[s].attr || !src[s].t)))
+            return LW_EINVAL;
+    lw::RegLoop L;
+    for (int32_t s = 0; s < S; ++s)  // the same parameters for every map; refreshes only when they change
+        if (int rc = reg_setup(b->ctx[s], p, L)) return rc;
+    const float inv_v_thin = (float)(2.0 / (double)p->reg_voxel);
+    const lw::PrecParams q{b->ctx[0]->prec_voxel, b->ctx[0]->prec_noise2, b->ctx[0]->prec_tight2, b->ctx[0]->prec_alpha};
+    auto *prep1 = pin_at<lw::PrepArgs>(b, 0);
+    auto *prep2 = pin_at<lw::PrepArgs>(b, b->o_prep2);
+    auto *ins = pin_at<lw::InsSeq>(b, b->o_ins);
+    auto *fcs = pin_at<lw::FrameCtl>(b, b->o_fc);
+    for (int32_t s = 0; s < S; ++s) {
+        lw_ctx *c = b->ctx[s];
+        const lw_frame_in &fi = in[s];
+        const bool warm = src[s].warm != 0;
+        lw::FrameCtl *fc_dev = at<lw::FrameCtl>(b, b->o_fc) + s;
+        const double *tp_dev = reinterpret_cast<const double *>(b->dev + b->o_tp + (size_t)s * TP_BYTES);
+        char *tp = b->pin + b->o_tp + (size_t)s * TP_BYTES;
+        std::memcpy(tp, fi.T_pred, 16 * sizeof(double));
+        std::memcpy(tp + 16 * sizeof(double), L.kappa, 64 * sizeof(float));
+        lw::FrameCtl f{};
+        std::memcpy(f.T_pred, fi.T_pred, sizeof(f.T_pred));
+        std::memcpy(f.T_prev_inv, fi.T_prev_inv, sizeof(f.T_prev_inv));
+        std::memcpy(f.T_key_inv, fi.T_key_inv, sizeof(f.T_key_inv));
+        f.kf_dist = fi.kf_dist;
+        f.kf_angle = fi.kf_angle;
+        f.redeskew_rot = fi.redeskew_rot;
+        f.redeskew_trans = fi.redeskew_trans;
+        std::memcpy(f.xi0, fi.xi, sizeof(f.xi0));
+        f.min_points = fi.min_points;
+        f.min_corr = p->min_corr;
+        f.has_prev = warm ? 0 : fi.has_prev;
+        f.has_key = fi.has_key;
+        f.redeskew = warm ? 0 : fi.redeskew;
+        f.frame_cap = (int32_t)c->frame_cap;
+        f.warm = warm ? 1 : 0;
+        f.run1 = warm ? 0 : 1;
+        f.m_in = src[s].m;
+        fcs[s] = f;
+        lw::PrepArgs A{};
+        A.raw = src[s].xyz;
+        A.t = src[s].t;
+        A.attr = src[s].attr;
+        A.xyz = c->f_xyz;
+        A.m_dev = &fc_dev->m_in;
+        A.frame_cap = (int32_t)c->frame_cap;
+        A.t_mid = fi.t_mid;
+        A.inv_dt = fi.inv_dt;
+        for (int k = 0; k < 6; ++k) A.xi[k] = fi.xi[k];
+        A.inv_v = inv_v_thin;
+        A.min_range = p->min_range;
+        A.max_range = p->max_range;
+        A.ret_mask = p->ret_mask;
+        A.ds_mask = (int32_t)(c->ds_cap - 1);
+        A.ds_keys = c->ds_keys;
+        A.ds_idx = c->ds_idx;
+        A.ds_xyz = c->ds_xyz;
+        A.n_out = &c->ctr[5];
+        A.st = at<lw::RegState>(b, b->o_st) + s;
+        A.Tp = tp_dev;
+        A.flags = c->loop_flags;
+        A.st1 = nullptr;
+        A.fc = fc_dev;
+        prep1[s] = A;
+        A.st1 = at<lw::RegState>(b, b->o_st1) + s;  // the check: keep the first registration, then maybe redo
+        prep2[s] = A;
+        ins[s] = lw::InsSeq{lw::map_refs(c), c->f_xyz, src[s].attr, (int32_t)c->frame_cap, &fc_dev->m_in, fc_dev->T32,
+                            &fc_dev->keyframe, at<int32_t>(b, b->o_counts) + 2 * s};
+    }
+    LW_CK(hipMemcpyAsync(b->dev, b->pin, b->o_st1, hipMemcpyHostToDevice, b->stream));
+    // deskew and thin every frame, register, the re-deskew test and (gated) the second preparation and
+    // registration, the keyframe decisions, then every keyframe's insert with its voxels refreshed
+    LW_CK(lw::launch_frame_prep_seq(b->stream, at<lw::PrepArgs>(b, 0), S));
+    LW_CK(batch_register(b, b->o_reg1, L, p->acc_mode));
+    LW_CK(lw::launch_frame_prep_seq(b->stream, at<lw::PrepArgs>(b, b->o_prep2), S));
+    LW_CK(batch_register(b, b->o_reg2, L, p->acc_mode));
+    LW_CK(lw::launch_keyframe_check_seq(b->stream, at<lw::KeySeq>(b, b->o_key), S));
+    LW_CK(lw::launch_insert_seq(b->stream, at<lw::InsSeq>(b, b->o_ins), S, (int32_t)b->frame_cap,
+                                (float)(1.0 / (double)p->reg_voxel), p->reg_voxel, p->max_range, p->ret_mask, q));
+    LW_CK(hipMemcpyAsync(b->pin + b->o_fc, b->dev + b->o_fc, b->o_reg1 - b->o_fc, hipMemcpyDeviceToHost, b->stream));
+    LW_CK(hipStreamSynchronize(b->stream));
+    LW_CK(batch_harvest(b));
+    const auto *st1s = pin_at<lw::RegState>(b, b->o_st1);
+    const auto *sts = pin_at<lw::RegState>(b, b->o_st);
+    const auto *counts = pin_at<int32_t>(b, b->o_counts);
+    int rc = 0;
+    for (int32_t s = 0; s < S; ++s) {
+        const lw::FrameCtl &fc = fcs[s];
+        lw_frame_out &o = out[s];
+        std::memset(&o, 0, sizeof(o));
+        if (fc.err) rc = LW_EINVAL;
+        o.m = fc.m;
+        o.skipped = fc.skipped;
+        o.ok = fc.ok;
+        o.keyframe = fc.keyframe;
+        o.registrations = (fc.skipped || fc.warm) ? 0 : 1 + fc.redo;
+        std::memcpy(o.T, fc.T, sizeof(o.T));
+        std::memcpy(o.xi, fc.xi1, sizeof(o.xi));
+        o.map_voxels = counts[2 * s];
+        o.map_overflow = counts[2 * s + 1];
+        if (o.registrations >= 1) {
+            o.reg[0].m = fc.m;
+            fill_result(st1s[s], p, &o.reg[0]);
+        }
+        if (o.registrations >= 2) {
+            o.reg[1].m = fc.m;
+            fill_result(sts[s], p, &o.reg[1]);
+        }
+    }
+    return rc;
+}
+
+int lw_batch_prof_enable(lw_batch *b, int32_t on) {
+    if (!b) return LW_EINVAL;
+    if (on && !b->ev) {
+        b->ev = new (std::nothrow) hipEvent_t[2 * BATCH_EVENTS];
+        if (!b->ev) return LW_EINVAL;
+        for (int i = 0; i < 2 * BATCH_EVENTS; ++i) LW_CK(hipEventCreate(&b->ev[i]));
+        LW_CK(hipMalloc((void **)&b->prof_dev, lw::LW_PROF_COUNTERS * sizeof(unsigned long long)));
+        LW_CK(hipMemsetAsync(b->prof_dev, 0, lw::LW_PROF_COUNTERS * sizeof(unsigned long long), b->stream));
+        LW_CK(hipStreamSynchronize(b->stream));
+    }
+    b->prof_on = on != 0;
+    return 0;
+}
+
+int lw_batch_prof_reset(lw_batch *b) {
+    if (!b) return LW_EINVAL;
+    LW_CK(hipStreamSynchronize(b->stream));
+    b->ev_used = 0;
+    b->prof_ms = 0.0;
+    b->prof_launches = 0;
+    if (b->prof_dev) {
+        LW_CK(hipMemsetAsync(b->prof_dev, 0, lw::LW_PROF_COUNTERS * sizeof(unsigned long long), b->stream));
+        LW_CK(hipStreamSynchronize(b->stream));
+    }
+    return 0;
+}
+
+int lw_batch_prof_read(lw_batch *b, double *out) {
+    if (!b || !out) return LW_EINVAL;
+    LW_CK(hipStreamSynchronize(b->stream));
+    LW_CK(batch_harvest(b));
+    unsigned long long v[lw::LW_PROF_COUNTERS] = {0};
+    if (b->prof_dev) LW_CK(hipMemcpy(v, b->prof_dev, sizeof(v), hipMemcpyDeviceToHost));
+    out[0] = (double)b->prof_launches;
+    out[1] = b->prof_ms;
+    for (int i = 0; i < 5; ++i) out[2 + i] = (double)v[i];
+    out[7] = 0.0;
+    return 0;
+}
+
+}  // extern "C"
+

@@ -22,6 +22,17 @@ constexpr int ACC_H = 0, ACC_B = 21, ACC_COUNT = 27, ACC_ERR = 28, ACC_RMS = 29,
               ACC_RANGE = 31, ACC_HG = 32;            // odom.py:73-81
 constexpr int ACC_USED = 53;                          // 21 + 6 + 5 + 21
 
+// The calling thread's lane in its wave (64 lanes).
+__device__ inline int lane_id() { return (int)(threadIdx.x & 63u); }
+
+// A barrier among the lanes of one wave, with LDS written before it visible to every lane after it (the
+// workgroup barrier would also wait for the workgroup's other waves, which may be at another barrier).
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One hash-table slot (32 B): the reference's int64 key, the dense voxel id of its records, and a copy of
 // the voxel's mean and count (float), so the closest-mean search reads one slot per neighbour and nothing
 // else. Mean and count are written by the precision refresh (count 0 until then: the voxel is skipped).

@@ -12,6 +12,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "lw_common.h"
 #include "lw_gn.h"
 
 namespace lw {
@@ -45,7 +46,7 @@ __device__ inline double tmul_entry(const double *A, const double *B, int l) {
 // entry below it (gn::pd_above / gn::solve6 order). pd0 / pd1: whether every pivot of each was positive.
 __device__ inline void wave_cholesky2(const double *A0, double shift0, double *L0, const double *A1, double shift1,
                                       double *L1, bool &pd0, bool &pd1) {
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     const int li = lane & 31;
     const double *A = lane < 32 ? A0 : A1;
     double *L = lane < 32 ? L0 : L1;
@@ -62,7 +63,7 @@ __device__ inline void wave_cholesky2(const double *A0, double shift0, double *L
             L[6 * li + j] = t / d;
         }
         if (li == j) L[6 * j + j] = d;
-        __syncthreads();
+        wave_sync();
     }
     pd0 = __shfl(pd ? 1 : 0, 0, 64) != 0;
     pd1 = __shfl(pd ? 1 : 0, 32, 64) != 0;
@@ -113,7 +114,7 @@ struct WaveOut {
     int32_t n_degen;
 };
 
-// The step on all 64 lanes of the calling wave (a one-wave workgroup). sh: W_DOUBLES doubles of LDS. Returns
+// The step on all 64 lanes of the calling wave (the other waves of its workgroup, if any, take no part). sh: W_DOUBLES doubles of LDS. Returns
 // 1 when handled (lane 0's o holds delta, and H_out, the registration's copy of H, has been written), 0 when
 // the caller must run gn::step instead, or -1 for gn::STEP_FEW (corr < min_corr). T: the pose. xi: the twist
 // from T to the prediction (gn::twist_between(T, T_pred), read when P.prior_info > 0), computed by the caller
@@ -123,7 +124,7 @@ struct WaveOut {
 __device__ inline int wave_step(const double *v, const double *T, const double *xi, const gn::StepParams &P,
                                 double *sh, WaveOut &o, double *H_out, bool stamp, unsigned long long &t_setup,
                                 unsigned long long &t_factor, unsigned long long &t_solve) {
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     o.corr = (int32_t)v[27];
     o.n_degen = 0;
     if (o.corr < P.min_corr) return -1;
@@ -132,7 +133,7 @@ __device__ inline int wave_step(const double *v, const double *T, const double *
            *out = sh + W_OUT, *Ts = sh + W_T;
     const bool prior = P.prior_info > 0.0, degen = P.degen_thr > 0.0;
     if (lane < 16) Ts[lane] = T[lane];
-    __syncthreads();
+    wave_sync();
     double Lr = v[31] / (double)o.corr;
     if (!(Lr > 1.0)) Lr = 1.0;
     const double rr = v[29] > 0.0 ? v[29] : 0.0;
@@ -148,7 +149,7 @@ __device__ inline int wave_step(const double *v, const double *T, const double *
         J[lane] = bj;
         Ji[lane] = bji;
     }
-    __syncthreads();
+    wave_sync();
     double mean = 0.0;
     for (int i = 0; i < 6; ++i) mean += H[7 * i];
     mean /= 6.0;
@@ -160,7 +161,7 @@ __device__ inline int wave_step(const double *v, const double *T, const double *
         A[lane] = a;
     }
     if (lane < 6) rhs[lane] = -v[21 + lane];
-    __syncthreads();
+    wave_sync();
     if (prior || degen) {
         if (lane < 36) {
             if (prior) A[lane] += M1[lane];
@@ -171,7 +172,7 @@ __device__ inline int wave_step(const double *v, const double *T, const double *
             for (int k = 0; k < 6; ++k) s += M1[6 * lane + k] * xi[k];
             rhs[lane] += s;
         }
-        __syncthreads();
+        wave_sync();
     }
     if (stamp) t_setup = wall_clock64();
     // the damped system and the degeneracy test (J^T Hg J - thr I positive definite), factored together

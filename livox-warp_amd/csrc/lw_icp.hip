@@ -83,7 +83,7 @@ constexpr int RED_BYTES = ICP_BLOCK * RED_STRIDE * 8;
 // across the neighbour search.
 template <int MODE>
 __device__ inline void row_zero(unsigned char *lds) {
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     if constexpr (MODE == 0) {
         double *red = reinterpret_cast<double *>(lds);
         LW_UNROLL for (int k = 0; k < ACC_USED; ++k) red[lane * RED_STRIDE + k] = 0.0;
@@ -99,7 +99,7 @@ __device__ inline void row_zero(unsigned char *lds) {
 
 template <int MODE>
 __device__ inline void row_add(unsigned char *lds, const Acc<MODE> &acc) {
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     if constexpr (MODE == 0) {
         double *red = reinterpret_cast<double *>(lds);
         LW_UNROLL for (int k = 0; k < ACC_USED; ++k) red[lane * RED_STRIDE + k] += acc.s[k];
@@ -122,7 +122,7 @@ __device__ inline void row_add(unsigned char *lds, const Acc<MODE> &acc) {
 // compensated form).
 template <int MODE, int S>
 __device__ inline void wave_sum(unsigned char *lds, double &sum, double &comp) {
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     sum = 0.0;
     comp = 0.0;
     __syncthreads();
@@ -279,7 +279,7 @@ __device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf
                                  unsigned long long &t_terms) {
     constexpr int S = Split<R>::S, C = Split<R>::C;
     constexpr int D = 2 * R + 1, K = D * D * D, KQ = (K + S - 1) / S;
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     const int sub = lane % S;
     row_zero<MODE>(lds);
     for (int64_t base = (int64_t)g * ICP_BLOCK; base < (int64_t)p.n * S; base += (int64_t)G * ICP_BLOCK) {
@@ -550,8 +550,8 @@ __device__ __noinline__ bool gauss_newton(RegState *st, const double *v, const g
     return accept_step(st, T, o.delta, o.rms, o.n_degen, it);
 }
 
-// The step on the whole wave (lw_gn_wave.h) when it applies, otherwise gauss_newton in lane 0 (same results
-// either way), on the reduced row v. xi: the twist from T to the prediction Tp (LDS, when the prior is on).
+// The step on the whole calling wave (lw_gn_wave.h) when it applies, otherwise gauss_newton in its lane 0 (same
+// results either way), on the reduced row v. xi: the twist from T to the prediction Tp (LDS, when the prior is on).
 // Every lane returns whether the loop stops; st->done records it. With phase measurement on, a wave step's
 // parts go to ph[16..19] (set-up, factorisations, substitution, pose update; device wall-clock ticks) and ph[20],
 // and ph[5] / ph[6] count the steps taken on the wave / in one lane.
@@ -562,7 +562,7 @@ __device__ inline bool run_step(RegState *st, const double *v, const gn::StepPar
     unsigned long long t1 = 0ull, t2 = 0ull, t3 = 0ull;
     const unsigned long long t0 = ph ? wall_clock64() : 0ull;
     const int r = gnw::wave_step(v, T, xi, sp, sh, o, st->H, ph != nullptr, t1, t2, t3);
-    if (threadIdx.x == 0) {
+    if (lane_id() == 0) {
         bool stop;
         if (r == 0) {
             stop = gauss_newton(st, v, sp, it, T, Tp);
@@ -589,7 +589,7 @@ __device__ inline bool run_step(RegState *st, const double *v, const gn::StepPar
             }
         }
     }
-    __syncthreads();
+    wave_sync();
     return *s_flag != 0;
 }
 
@@ -900,6 +900,99 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_register(LoopArgs L) {
     searcher<MODE>(L, n, lds);
 }
 
+// ---- a whole registration in one workgroup (the batched odometry) ---------------------------------------
+//
+// Independent sequences need no cross-workgroup exchange at all: one workgroup of SEQ_WAVES waves runs a
+// sequence's whole Gauss-Newton loop. Per pass the waves take the point groups wave, wave + SEQ_WAVES, ...
+// (wave_pass), their rows meet in LDS and are added in wave order (the first RED_BATCH rows of the
+// persistent loop's reduction, in the same arithmetic), and wave 0 takes the float64 step while the others
+// wait at the workgroup barrier. The rows of the four waves use most of the CU's LDS, so a CU runs one
+// sequence at a time and the grid is the batch.
+constexpr int SEQ_WAVES = 4;
+static_assert(SEQ_WAVES <= RED_BATCH, "the wave rows are one batch of the reduction");
+static_assert(gnw::W_DOUBLES * sizeof(double) <= RED_BYTES, "the step's scratch fits one wave's rows");
+
+template <int MODE>
+__global__ __launch_bounds__(64 * SEQ_WAVES) void k_register_seq(const SeqReg *D, RegLoop L,
+                                                                unsigned long long *prof) {
+    __shared__ __align__(16) unsigned char lds[SEQ_WAVES][RED_BYTES];
+    __shared__ NbrOff nbr[NBR_ROWS];
+    __shared__ float Tf[16];
+    __shared__ double sT[16], sTp[16], vs[64], xi[6], rows[SEQ_WAVES][2][64];
+    __shared__ int32_t s_flag;
+    const SeqReg d = D[blockIdx.x];
+    if (gated_off(d.gate)) return;  // the state stays as the frame preparation left it
+    const int lane = lane_id(), wave = (int)(threadIdx.x >> 6);
+    RegState *st = d.st;
+    const int32_t n = *d.n_dev;
+    if (threadIdx.x == 0) st->n = n;
+    if (n < L.min_corr) return;  // odom.py:756-757: the host sees n and keeps the prediction
+    IcpArgs a{};
+    a.ds = d.ds;
+    a.slots = d.slots;
+    a.occ = d.occ;
+    a.vox_prec = d.vox_prec;
+    a.st = st;
+    a.mask = d.mask;
+    a.inv_v = L.inv_v;
+    a.voxel = L.voxel;
+    a.gn = 1;
+    a.sp = L.sp;
+    if (threadIdx.x < 16) {
+        sT[threadIdx.x] = st->T[threadIdx.x];
+        sTp[threadIdx.x] = st->T_pred[threadIdx.x];
+        Tf[threadIdx.x] = st->Tf[threadIdx.x];
+    }
+    nbr_fill(nbr);
+    __syncthreads();
+    unsigned long long t_found, t_terms;
+    for (int it = 0; it < L.max_iter; ++it) {
+        const int reach = it < L.wide_iters ? 2 : 1;
+        const Pass p{n, reach == 2 ? L.max_d2_r2 : L.max_d2_r1, st->kappa[it], it};
+        double sum, comp;
+        int32_t hits = 0, matched = 0;
+        if (reach == 2)
+            wave_pass<MODE, 2>(a, p, Tf, nbr, wave, SEQ_WAVES, lds[wave], sum, comp, hits, matched, {}, false, false,
+                               t_found, t_terms);
+        else
+            wave_pass<MODE, 1>(a, p, Tf, nbr, wave, SEQ_WAVES, lds[wave], sum, comp, hits, matched, {}, false, false,
+                               t_found, t_terms);
+        LW_UNROLL for (int off = 32; off > 0; off >>= 1) {
+            hits += __shfl_xor(hits, off, 64);
+            matched += __shfl_xor(matched, off, 64);
+        }
+        rows[wave][0][lane] = lane < ACC_USED ? sum : (lane == ACC_USED ? (double)hits : (double)matched);
+        rows[wave][1][lane] = comp;
+        __syncthreads();
+        if (wave == 0) {
+            BSum r{0.0, 0.0f, 0.0f};
+            LW_UNROLL for (int w = 0; w < SEQ_WAVES; ++w) add_row<MODE>(r, true, rows[w][0][lane], rows[w][1][lane]);
+            const double v = entry_of<MODE>(r);
+            vs[lane] = v;
+            st->acc[lane] = lane < ACC_USED ? v : 0.0;
+            if (lane == 0 && L.sp.prior_info > 0.0) gn::twist_between(sT, sTp, xi);
+            wave_sync();
+            // the step's scratch: wave 1's rows, free until the next pass
+            run_step(st, vs, L.sp, it, reinterpret_cast<double *>(lds[1]), sT, sTp, xi, &s_flag, nullptr);
+            if (lane < 16) {
+                st->T[lane] = sT[lane];
+                Tf[lane] = (float)sT[lane];
+                st->Tf[lane] = Tf[lane];
+            }
+            if (lane == 0 && prof) {
+                const int K = reach == 2 ? 125 : 27;
+                atomicAdd(&prof[0], (unsigned long long)n);
+                atomicAdd(&prof[1], (unsigned long long)n * (unsigned long long)K);
+                atomicAdd(&prof[2], (unsigned long long)vs[ACC_USED]);
+                atomicAdd(&prof[3], (unsigned long long)vs[ACC_USED + 1]);
+                atomicAdd(&prof[4], 1ull);
+            }
+        }
+        __syncthreads();
+        if (s_flag) break;
+    }
+}
+
 // Tp: 16 doubles (the prediction) followed by 64 floats (the per-iteration robust-kernel scale); T_dev, when
 // given, holds the prediction instead. Also clears the persistent loop's row flags and published pose. Does
 // nothing when gate is given and holds 0.
@@ -981,6 +1074,19 @@ hipError_t launch_icp_pass(lw_ctx *c, const IcpPass &p, int acc_mode) {
         c->prof_used++;
     }
     return e;
+}
+
+hipError_t launch_register_seq(hipStream_t s, const SeqReg *D, int32_t S, const RegLoop &L, int acc_mode,
+                               unsigned long long *prof) {
+    if (S <= 0) return hipSuccess;
+    if (L.max_iter > 64) return hipErrorInvalidValue;
+    switch (acc_mode) {
+    case 0: hipLaunchKernelGGL(k_register_seq<0>, dim3(S), dim3(64 * SEQ_WAVES), 0, s, D, L, prof); break;
+    case 1: hipLaunchKernelGGL(k_register_seq<1>, dim3(S), dim3(64 * SEQ_WAVES), 0, s, D, L, prof); break;
+    case 2: hipLaunchKernelGGL(k_register_seq<2>, dim3(S), dim3(64 * SEQ_WAVES), 0, s, D, L, prof); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_register_loop(lw_ctx *c, const int32_t *n_dev, int32_t n_bound, const RegLoop &R, int acc_mode,

@@ -52,6 +52,9 @@ struct FrameCtl {
     double redeskew_rot, redeskew_trans;
     float xi0[6];           // the twist the frame is deskewed with first
     int32_t min_points, min_corr, has_prev, has_key, redeskew, frame_cap;
+    // the batched chain's (lw_batch_frame): a warm-up frame (inserted at T_pred unregistered), the first
+    // registration's gate (1 unless warm), the frame's point count
+    int32_t warm, run1, m_in, pad_in;
     // written on the device
     double T1[16];          // the first registration's pose (the second's prediction)
     double T[16];           // the frame's pose
@@ -61,6 +64,37 @@ struct FrameCtl {
     int32_t err;            // 1: the frame held more points than the single-workgroup preparation takes
 };
 constexpr size_t FRAME_CTL_INPUT = offsetof(FrameCtl, T1);  // bytes the host uploads
+
+// One frame's deskew, thinning and registration start in one workgroup (k_frame_prep, lw_odom.hip).
+struct PrepArgs {
+    const float *raw, *t;  // the frame: raw points and times
+    const uint32_t *attr;
+    float *xyz;            // the deskewed points
+    const int32_t *m_dev;  // the frame's appended points (capped at frame_cap)
+    int32_t frame_cap;
+    float t_mid, inv_dt;
+    float xi[6];           // the deskew twist (the first preparation)
+    float inv_v, min_range, max_range;
+    int32_t ret_mask;
+    int32_t ds_mask;       // the global thinning table (the fallback)
+    int64_t *ds_keys;
+    int32_t *ds_idx;
+    float *ds_xyz;         // the thinned points, in index order
+    int32_t *n_out;        // their count
+    struct RegState *st;   // the registration state to set up
+    const double *Tp;      // the prediction (16 doubles) and the robust-kernel scales (64 floats)
+    uint32_t *flags;       // the persistent loop's row flags
+    struct RegState *st1;  // given: the frame check runs first, and the rest only when it asks for a re-deskew
+    struct FrameCtl *fc;
+};
+
+// One sequence's keyframe decision in a batch (k_keyframe_check's inputs).
+struct KeySeq {
+    const RegState *st;
+    FrameCtl *fc;
+    const int32_t *ctr;
+    int32_t *counts;
+};
 
 // The parameters the per-voxel precision is computed with (odom.py:336-368).
 struct PrecParams {
@@ -227,7 +261,42 @@ struct InsertArgs {
     const PrecParams *refresh;
     int32_t *counts;
 };
+// The arrays of one registration map and its insert scratch, as the insert kernels take them.
+struct MapRefs {
+    Slot *slots;
+    uint32_t *occ;
+    int64_t *vox_key;
+    int32_t *vox_slot;
+    VoxMom *vox_mom;
+    VoxPrec *vox_prec;
+    int32_t *ctr;
+    int32_t *pt_slot, *pt_vid;
+    uint8_t *pt_take;
+    int32_t *vox_nin, *touched, *vox_quota, *vox_dirty, *dirty_list;
+    int64_t vox_cap;
+    int32_t mask;
+};
+MapRefs map_refs(lw_ctx *c);
+// One sequence's insert in a batch: its map, its points (n, or the count at n_dev capped at n), its float32
+// pose on the device, a gate that must be nonzero, and where the map's counts go once the claims are done.
+struct InsSeq {
+    MapRefs M;
+    const float *xyz;
+    const uint32_t *attr;
+    int32_t n;
+    const int32_t *n_dev;
+    const float *T_dev;
+    const int32_t *gate;
+    int32_t *counts;
+};
+// Every sequence's insert (D: S descriptors on the device), touched voxels refreshed with q at once; n_bound
+// bounds every sequence's points.
+hipError_t launch_insert_seq(hipStream_t s, const InsSeq *D, int32_t S, int32_t n_bound, float inv_v, float voxel,
+                             float max_range, int32_t ret_mask, const PrecParams &q);
 hipError_t launch_insert_head(lw_ctx *c, const InsertArgs &a);
+// S frame preparations (one workgroup each, D: S PrepArgs on the device) and S keyframe decisions.
+hipError_t launch_frame_prep_seq(hipStream_t s, const PrepArgs *D, int32_t S);
+hipError_t launch_keyframe_check_seq(hipStream_t s, const KeySeq *D, int32_t S);
 hipError_t launch_insert_tail(lw_ctx *c, const InsertArgs &a);
 hipError_t launch_refresh(lw_ctx *c, bool all, const PrecParams &q);
 hipError_t launch_map_find(lw_ctx *c, const int64_t *keys_dev, int32_t n, int32_t *out_dev);
@@ -265,6 +334,23 @@ struct RegLoop {
 };
 hipError_t launch_register_loop(lw_ctx *c, const int32_t *n_dev, int32_t n_bound, const RegLoop &L, int acc_mode,
                                 const int32_t *gate = nullptr);
+// One sequence's registration in a batch: its thinned points and their count, its map, its registration state
+// (set up by the frame preparation), and a gate that must be nonzero for it to run.
+struct SeqReg {
+    const float *ds;
+    const int32_t *n_dev;
+    const Slot *slots;
+    const uint32_t *occ;
+    const VoxPrec *vox_prec;
+    RegState *st;
+    const int32_t *gate;
+    int32_t mask;
+    int32_t pad;
+};
+// S registrations (D: S descriptors on the device), one workgroup each, every Gauss-Newton pass and step inside
+// the workgroup. prof: the measurement counters, or null.
+hipError_t launch_register_seq(hipStream_t s, const SeqReg *D, int32_t S, const RegLoop &L, int acc_mode,
+                               unsigned long long *prof);
 int32_t icp_max_groups(int64_t frame_cap);  // partial rows one pass over frame_cap points can write
 hipError_t launch_reg_begin(lw_ctx *c, const double *T_pred_dev, const int32_t *gate = nullptr,
                             const double *T_dev = nullptr);

@@ -61,135 +61,118 @@ __device__ inline bool insert_point(const float *xyz, const uint32_t *attr, int3
     return true;
 }
 
-// The claim of point i (k_ins_claim's body).
-__device__ inline void claim_point(const float *xyz, const uint32_t *attr, int32_t i, const Mat4f &T, float inv_v,
-                                   float max_range, int32_t ret_mask, int32_t mask, Slot *slots, uint32_t *occ,
-                                   int64_t *vox_key, int32_t *vox_slot, VoxMom *vox_mom, int32_t *ctr,
-                                   int64_t vox_cap, int32_t *pt_slot) {
+// The claim of point i: probe and CAS-claim its voxel's slot (a claimer hands out the voxel id).
+__device__ inline void claim_point(const MapRefs &M, const float *xyz, const uint32_t *attr, int32_t i, const Mat4f &T,
+                                   float inv_v, float max_range, int32_t ret_mask) {
     int32_t ix, iy, iz;
     float wx, wy, wz;
     if (!insert_point(xyz, attr, i, T, inv_v, max_range, ret_mask, ix, iy, iz, wx, wy, wz)) {
-        pt_slot[i] = -1;
+        M.pt_slot[i] = -1;
         return;
     }
     int64_t key = voxel_key_ijk(ix, iy, iz);
-    int32_t h = slot_hash(key) & mask;
+    int32_t h = slot_hash(key) & M.mask;
     for (int probe = 0; probe < PROBES; ++probe) {
-        int32_t s = (h + probe) & mask;
-        int64_t k = __hip_atomic_load(&slots[s].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int32_t s = (h + probe) & M.mask;
+        int64_t k = __hip_atomic_load(&M.slots[s].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (k == key) {
-            pt_slot[i] = s;
+            M.pt_slot[i] = s;
             return;
         }
         if (k == -1) {
-            unsigned long long prev = atomicCAS((unsigned long long *)&slots[s].key, 0xFFFFFFFFFFFFFFFFull,
+            unsigned long long prev = atomicCAS((unsigned long long *)&M.slots[s].key, 0xFFFFFFFFFFFFFFFFull,
                                                 (unsigned long long)key);
             if ((int64_t)prev == -1) {
-                int32_t vid = atomicAdd(&ctr[2], 1);
-                if (vid >= vox_cap) {  // cannot happen while vox_cap == cap; keep the slot unusable
-                    slots[s].vid = -1;
-                    pt_slot[i] = -1;
-                    atomicAdd(&ctr[1], 1);
+                int32_t vid = atomicAdd(&M.ctr[2], 1);
+                if (vid >= M.vox_cap) {  // cannot happen while vox_cap == cap; keep the slot unusable
+                    M.slots[s].vid = -1;
+                    M.pt_slot[i] = -1;
+                    atomicAdd(&M.ctr[1], 1);
                     return;
                 }
-                slots[s].vid = vid;
-                vox_key[vid] = key;
-                vox_slot[vid] = s;
+                M.slots[s].vid = vid;
+                M.vox_key[vid] = key;
+                M.vox_slot[vid] = s;
                 VoxMom z;
                 for (int a = 0; a < 3; ++a) z.s[a] = 0.0f;
                 for (int a = 0; a < 6; ++a) z.o[a] = 0.0f;
                 z.cnt = 0;
-                vox_mom[vid] = z;
-                atomicOr(&occ[s >> 5], 1u << (s & 31));
-                if (probe > 0) atomicMax(&slots[h].disp, probe);  // the home slot bounds this key's probe
-                atomicAdd(&ctr[0], 1);
-                pt_slot[i] = s;
+                M.vox_mom[vid] = z;
+                atomicOr(&M.occ[s >> 5], 1u << (s & 31));
+                if (probe > 0) atomicMax(&M.slots[h].disp, probe);  // the home slot bounds this key's probe
+                atomicAdd(&M.ctr[0], 1);
+                M.pt_slot[i] = s;
                 return;
             }
             if ((int64_t)prev == key) {
-                pt_slot[i] = s;
+                M.pt_slot[i] = s;
                 return;
             }
         }
     }
-    atomicAdd(&ctr[1], 1);  // no slot within the probe window (odom.py:230)
-    pt_slot[i] = -1;
+    atomicAdd(&M.ctr[1], 1);  // no slot within the probe window (odom.py:230)
+    M.pt_slot[i] = -1;
 }
+
+// Each insert kernel's work for block b of nb (the single-map kernels: blockIdx.x of gridDim.x; the batched
+// ones: the same over x, the map from blockIdx.y).
 
 // 1: claim. A plain read first: a slot's key only ever goes from -1 to its final value.
-__global__ __launch_bounds__(BLOCK) void k_ins_claim(const float *xyz, const uint32_t *attr, int32_t n,
-                                                     const int32_t *n_dev, Mat4f T, float inv_v, float max_range,
-                                                     int32_t ret_mask, int32_t mask, Slot *slots, uint32_t *occ,
-                                                     int64_t *vox_key, int32_t *vox_slot, VoxMom *vox_mom,
-                                                     int32_t *ctr, int64_t vox_cap, int32_t *pt_slot,
-                                                     const int32_t *gate, const float *T_dev) {
-    if (gated_off(gate)) return;
-    if (T_dev) T = load_mat4f(T_dev);
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[4] = 0;  // touched count of this insert (read by the next kernel)
-    const int32_t cnt = count_of(n, n_dev);
-    for (int32_t i = blockIdx.x * BLOCK + threadIdx.x; i < cnt; i += gridDim.x * BLOCK)
-        claim_point(xyz, attr, i, T, inv_v, max_range, ret_mask, mask, slots, occ, vox_key, vox_slot, vox_mom, ctr,
-                    vox_cap, pt_slot);
+__device__ inline void claim_blocks(const MapRefs &M, const float *xyz, const uint32_t *attr, int32_t cnt,
+                                    const Mat4f &T, float inv_v, float max_range, int32_t ret_mask, int b, int nb) {
+    if (b == 0 && threadIdx.x == 0) M.ctr[4] = 0;  // touched count of this insert (read by the next kernel)
+    for (int32_t i = b * BLOCK + threadIdx.x; i < cnt; i += nb * BLOCK)
+        claim_point(M, xyz, attr, i, T, inv_v, max_range, ret_mask);
 }
 
-// 2: count the incoming points per voxel; the first one lists the voxel as touched.
-// counts, when given, receives the map's counts (ctr[0..1]), final once the claims are done.
-__global__ __launch_bounds__(BLOCK) void k_ins_count(const int32_t *pt_slot, int32_t n, const int32_t *n_dev,
-                                                     const Slot *slots, int32_t *pt_vid, int32_t *vox_nin,
-                                                     int32_t *touched, int32_t *ctr, const int32_t *gate,
-                                                     int32_t *counts) {
-    if (gated_off(gate)) return;
-    if (counts && blockIdx.x == 0 && threadIdx.x < 2) counts[threadIdx.x] = ctr[threadIdx.x];
-    const int32_t cnt = count_of(n, n_dev);
-    for (int32_t i = blockIdx.x * BLOCK + threadIdx.x; i < cnt; i += gridDim.x * BLOCK) {
-        int32_t s = pt_slot[i];
-        int32_t vid = s >= 0 ? slots[s].vid : -1;
-        pt_vid[i] = vid;
-        if (vid >= 0 && atomicAdd(&vox_nin[vid], 1) == 0) touched[atomicAdd(&ctr[4], 1)] = vid;
+// 2: count the incoming points per voxel; the first one lists the voxel as touched. counts, when given,
+// receives the map's counts (ctr[0..1]), final once the claims are done.
+__device__ inline void count_blocks(const MapRefs &M, int32_t cnt, int32_t *counts, int b, int nb) {
+    if (counts && b == 0 && threadIdx.x < 2) counts[threadIdx.x] = M.ctr[threadIdx.x];
+    for (int32_t i = b * BLOCK + threadIdx.x; i < cnt; i += nb * BLOCK) {
+        int32_t s = M.pt_slot[i];
+        int32_t vid = s >= 0 ? M.slots[s].vid : -1;
+        M.pt_vid[i] = vid;
+        if (vid >= 0 && atomicAdd(&M.vox_nin[vid], 1) == 0) M.touched[atomicAdd(&M.ctr[4], 1)] = vid;
     }
 }
 
 // 3: per touched voxel, its quota under the cap; voxels that overflow rank their points in index order.
-__global__ __launch_bounds__(BLOCK) void k_ins_quota(const int32_t *touched, const int32_t *ctr,
-                                                     const int32_t *vox_nin, const VoxMom *vox_mom,
-                                                     int32_t *vox_quota, const int32_t *pt_vid, int32_t n,
-                                                     const int32_t *n_dev, uint8_t *pt_take, const int32_t *gate) {
+__device__ inline void quota_blocks(const MapRefs &M, int32_t n, int b, int nb) {
     __shared__ int32_t wsum[BLOCK / 64];
     __shared__ int32_t s_vid, s_quota, s_nin;
-    if (gated_off(gate)) return;
-    n = count_of(n, n_dev);
-    int32_t nt = ctr[4];
-    for (int32_t j = blockIdx.x; j < nt; j += gridDim.x) {
+    int32_t nt = M.ctr[4];
+    for (int32_t j = b; j < nt; j += nb) {
         if (threadIdx.x == 0) {
-            s_vid = touched[j];
-            s_nin = vox_nin[s_vid];
-            s_quota = REG_CAP - vox_mom[s_vid].cnt;
+            s_vid = M.touched[j];
+            s_nin = M.vox_nin[s_vid];
+            s_quota = REG_CAP - M.vox_mom[s_vid].cnt;
         }
         __syncthreads();
         int32_t vid = s_vid, quota = s_quota, nin = s_nin;
         __syncthreads();
         if (nin <= quota) {
-            if (threadIdx.x == 0) vox_quota[vid] = INT_MAX;
+            if (threadIdx.x == 0) M.vox_quota[vid] = INT_MAX;
             continue;
         }
-        if (threadIdx.x == 0) vox_quota[vid] = quota;
+        if (threadIdx.x == 0) M.vox_quota[vid] = quota;
         if (quota <= 0) continue;
         // rank this voxel's points in index order
         int32_t base = 0;
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         for (int32_t t0 = 0; t0 < n; t0 += BLOCK) {
             int32_t i = t0 + threadIdx.x;
-            bool match = i < n && pt_vid[i] == vid;
-            unsigned long long b = __ballot(match);
-            int32_t lp = __popcll(b & ((1ull << lane) - 1ull));
-            if (lane == 0) wsum[wave] = __popcll(b);
+            bool match = i < n && M.pt_vid[i] == vid;
+            unsigned long long bal = __ballot(match);
+            int32_t lp = __popcll(bal & ((1ull << lane) - 1ull));
+            if (lane == 0) wsum[wave] = __popcll(bal);
             __syncthreads();
             int32_t off = 0, tot = 0;
             for (int w = 0; w < BLOCK / 64; ++w) {
                 off += (w < wave) ? wsum[w] : 0;
                 tot += wsum[w];
             }
-            if (match) pt_take[i] = (base + off + lp) < quota ? 1 : 0;
+            if (match) M.pt_take[i] = (base + off + lp) < quota ? 1 : 0;
             base += tot;
             __syncthreads();
             if (base >= nin) break;  // every point of the voxel is ranked (uniform: base and nin are)
@@ -198,24 +181,19 @@ __global__ __launch_bounds__(BLOCK) void k_ins_quota(const int32_t *touched, con
 }
 
 // 4: accumulate (odom.py:217, 225-228): r relative to the voxel centre, float32 atomics.
-__global__ __launch_bounds__(BLOCK) void k_ins_accum(const float *xyz, const uint32_t *attr, int32_t n,
-                                                     const int32_t *n_dev, Mat4f T, float inv_v, float voxel,
-                                                     float max_range, int32_t ret_mask, const int32_t *pt_vid,
-                                                     const int32_t *vox_quota, const uint8_t *pt_take,
-                                                     VoxMom *vox_mom, const int32_t *gate, const float *T_dev) {
-    if (gated_off(gate)) return;
-    if (T_dev) T = load_mat4f(T_dev);
-    const int32_t cnt = count_of(n, n_dev);
-    for (int32_t i = blockIdx.x * BLOCK + threadIdx.x; i < cnt; i += gridDim.x * BLOCK) {
-        int32_t vid = pt_vid[i];
+__device__ inline void accum_blocks(const MapRefs &M, const float *xyz, const uint32_t *attr, int32_t cnt,
+                                    const Mat4f &T, float inv_v, float voxel, float max_range, int32_t ret_mask,
+                                    int b, int nb) {
+    for (int32_t i = b * BLOCK + threadIdx.x; i < cnt; i += nb * BLOCK) {
+        int32_t vid = M.pt_vid[i];
         if (vid < 0) continue;
-        int32_t q = vox_quota[vid];
-        if (q != INT_MAX && !(q > 0 && pt_take[i])) continue;
+        int32_t q = M.vox_quota[vid];
+        if (q != INT_MAX && !(q > 0 && M.pt_take[i])) continue;
         int32_t ix, iy, iz;
         float wx, wy, wz;
         insert_point(xyz, attr, i, T, inv_v, max_range, ret_mask, ix, iy, iz, wx, wy, wz);
         float r0 = wx - center_of(ix, voxel), r1 = wy - center_of(iy, voxel), r2 = wz - center_of(iz, voxel);
-        VoxMom *m = &vox_mom[vid];
+        VoxMom *m = &M.vox_mom[vid];
         unsafeAtomicAdd(&m->s[0], r0);
         unsafeAtomicAdd(&m->s[1], r1);
         unsafeAtomicAdd(&m->s[2], r2);
@@ -253,22 +231,85 @@ __device__ inline void refresh_voxel(int32_t vid, const int64_t *vox_key, const 
 
 // 5: finish: clear the per-insert counters; queue the touched voxels for a refresh, or (refresh set) refresh
 // them here with q.
-__global__ __launch_bounds__(BLOCK) void k_ins_finish(const int32_t *touched, int32_t *ctr, int32_t *vox_nin,
-                                                      int32_t *vox_quota, int32_t *vox_dirty, int32_t *dirty_list,
-                                                      const int32_t *gate, int32_t refresh, PrecParams q,
-                                                      const int64_t *vox_key, const int32_t *vox_slot,
-                                                      const VoxMom *vox_mom, Slot *slots, VoxPrec *vox_prec) {
-    if (gated_off(gate)) return;
-    int32_t nt = ctr[4];
-    for (int32_t j = blockIdx.x * BLOCK + threadIdx.x; j < nt; j += gridDim.x * BLOCK) {
-        int32_t vid = touched[j];
-        vox_nin[vid] = 0;
-        vox_quota[vid] = INT_MAX;
+__device__ inline void finish_blocks(const MapRefs &M, int32_t refresh, const PrecParams &q, int b, int nb) {
+    int32_t nt = M.ctr[4];
+    for (int32_t j = b * BLOCK + threadIdx.x; j < nt; j += nb * BLOCK) {
+        int32_t vid = M.touched[j];
+        M.vox_nin[vid] = 0;
+        M.vox_quota[vid] = INT_MAX;
         if (refresh)
-            refresh_voxel(vid, vox_key, vox_slot, vox_mom, slots, vox_prec, q);
-        else if (atomicExch(&vox_dirty[vid], 1) == 0)
-            dirty_list[atomicAdd(&ctr[3], 1)] = vid;
+            refresh_voxel(vid, M.vox_key, M.vox_slot, M.vox_mom, M.slots, M.vox_prec, q);
+        else if (atomicExch(&M.vox_dirty[vid], 1) == 0)
+            M.dirty_list[atomicAdd(&M.ctr[3], 1)] = vid;
     }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_ins_claim(MapRefs M, const float *xyz, const uint32_t *attr, int32_t n,
+                                                     const int32_t *n_dev, Mat4f T, float inv_v, float max_range,
+                                                     int32_t ret_mask, const int32_t *gate, const float *T_dev) {
+    if (gated_off(gate)) return;
+    if (T_dev) T = load_mat4f(T_dev);
+    claim_blocks(M, xyz, attr, count_of(n, n_dev), T, inv_v, max_range, ret_mask, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_ins_count(MapRefs M, int32_t n, const int32_t *n_dev, const int32_t *gate,
+                                                     int32_t *counts) {
+    if (gated_off(gate)) return;
+    count_blocks(M, count_of(n, n_dev), counts, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_ins_quota(MapRefs M, int32_t n, const int32_t *n_dev, const int32_t *gate) {
+    if (gated_off(gate)) return;
+    quota_blocks(M, count_of(n, n_dev), blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_ins_accum(MapRefs M, const float *xyz, const uint32_t *attr, int32_t n,
+                                                     const int32_t *n_dev, Mat4f T, float inv_v, float voxel,
+                                                     float max_range, int32_t ret_mask, const int32_t *gate,
+                                                     const float *T_dev) {
+    if (gated_off(gate)) return;
+    if (T_dev) T = load_mat4f(T_dev);
+    accum_blocks(M, xyz, attr, count_of(n, n_dev), T, inv_v, voxel, max_range, ret_mask, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_ins_finish(MapRefs M, const int32_t *gate, int32_t refresh, PrecParams q) {
+    if (gated_off(gate)) return;
+    finish_blocks(M, refresh, q, blockIdx.x, gridDim.x);
+}
+
+// The batched forms: one insert per sequence (blockIdx.y), each with its own map, points, pose and gate.
+__global__ __launch_bounds__(BLOCK) void k_ins_claim_seq(const InsSeq *D, float inv_v, float max_range,
+                                                         int32_t ret_mask) {
+    const InsSeq d = D[blockIdx.y];
+    if (gated_off(d.gate)) return;
+    claim_blocks(d.M, d.xyz, d.attr, count_of(d.n, d.n_dev), load_mat4f(d.T_dev), inv_v, max_range, ret_mask,
+                 blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_ins_count_seq(const InsSeq *D) {
+    const InsSeq d = D[blockIdx.y];
+    if (gated_off(d.gate)) return;
+    count_blocks(d.M, count_of(d.n, d.n_dev), d.counts, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_ins_quota_seq(const InsSeq *D) {
+    const InsSeq d = D[blockIdx.y];
+    if (gated_off(d.gate)) return;
+    quota_blocks(d.M, count_of(d.n, d.n_dev), blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_ins_accum_seq(const InsSeq *D, float inv_v, float voxel, float max_range,
+                                                         int32_t ret_mask) {
+    const InsSeq d = D[blockIdx.y];
+    if (gated_off(d.gate)) return;
+    accum_blocks(d.M, d.xyz, d.attr, count_of(d.n, d.n_dev), load_mat4f(d.T_dev), inv_v, voxel, max_range, ret_mask,
+                 blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_ins_finish_seq(const InsSeq *D, PrecParams q) {
+    const InsSeq d = D[blockIdx.y];
+    if (gated_off(d.gate)) return;
+    finish_blocks(d.M, 1, q, blockIdx.x, gridDim.x);
 }
 
 // Per-voxel mean and precision for the queued voxels (all == 1: every voxel handed out).
@@ -304,6 +345,32 @@ __global__ __launch_bounds__(BLOCK) void k_map_find(const Slot *slots, const uin
     }
 }
 
+}  // namespace
+
+MapRefs map_refs(lw_ctx *c) {
+    MapRefs M;
+    M.slots = c->slots;
+    M.occ = c->occ;
+    M.vox_key = c->vox_key;
+    M.vox_slot = c->vox_slot;
+    M.vox_mom = c->vox_mom;
+    M.vox_prec = c->vox_prec;
+    M.ctr = c->ctr;
+    M.pt_slot = c->pt_slot;
+    M.pt_vid = c->pt_vid;
+    M.pt_take = c->pt_take;
+    M.vox_nin = c->vox_nin;
+    M.touched = c->touched;
+    M.vox_quota = c->vox_quota;
+    M.vox_dirty = c->vox_dirty;
+    M.dirty_list = c->dirty_list;
+    M.vox_cap = c->vox_cap;
+    M.mask = c->mask;
+    return M;
+}
+
+namespace {
+
 // Workgroups for n items, capped at max_grid: every kernel launched with it strides over its items by the grid.
 inline int grid_for(int64_t n, int block = BLOCK, int64_t max_grid = 65535) {
     int64_t g = (n + block - 1) / block;
@@ -338,27 +405,35 @@ hipError_t launch_map_reset(lw_ctx *c) {
 hipError_t launch_insert_head(lw_ctx *c, const InsertArgs &a) {
     if (a.n <= 0) return hipSuccess;
     const int g = grid_for(a.n);
-    hipLaunchKernelGGL(k_ins_claim, dim3(g), dim3(BLOCK), 0, c->stream, a.xyz, a.attr, a.n, a.n_dev, a.T, a.inv_v,
-                       a.max_range, a.ret_mask, c->mask, c->slots, c->occ, c->vox_key, c->vox_slot, c->vox_mom, c->ctr,
-                       c->vox_cap, c->pt_slot, a.gate, a.T_dev);
-    hipLaunchKernelGGL(k_ins_count, dim3(g), dim3(BLOCK), 0, c->stream, c->pt_slot, a.n, a.n_dev, c->slots, c->pt_vid,
-                       c->vox_nin, c->touched, c->ctr, a.gate, a.counts);
+    const MapRefs M = map_refs(c);
+    hipLaunchKernelGGL(k_ins_claim, dim3(g), dim3(BLOCK), 0, c->stream, M, a.xyz, a.attr, a.n, a.n_dev, a.T, a.inv_v,
+                       a.max_range, a.ret_mask, a.gate, a.T_dev);
+    hipLaunchKernelGGL(k_ins_count, dim3(g), dim3(BLOCK), 0, c->stream, M, a.n, a.n_dev, a.gate, a.counts);
     return hipGetLastError();
 }
 
 hipError_t launch_insert_tail(lw_ctx *c, const InsertArgs &a) {
     if (a.n <= 0) return hipSuccess;
     const int g = grid_for(a.n);
-    hipLaunchKernelGGL(k_ins_quota, dim3(1024), dim3(BLOCK), 0, c->stream, c->touched, c->ctr, c->vox_nin,
-                       c->vox_mom, c->vox_quota, c->pt_vid, a.n, a.n_dev, c->pt_take, a.gate);
-    hipLaunchKernelGGL(k_ins_accum, dim3(g), dim3(BLOCK), 0, c->stream, a.xyz, a.attr, a.n, a.n_dev, a.T, a.inv_v,
-                       a.voxel, a.max_range, a.ret_mask, c->pt_vid, c->vox_quota, c->pt_take, c->vox_mom, a.gate,
-                       a.T_dev);
+    const MapRefs M = map_refs(c);
+    hipLaunchKernelGGL(k_ins_quota, dim3(1024), dim3(BLOCK), 0, c->stream, M, a.n, a.n_dev, a.gate);
+    hipLaunchKernelGGL(k_ins_accum, dim3(g), dim3(BLOCK), 0, c->stream, M, a.xyz, a.attr, a.n, a.n_dev, a.T, a.inv_v,
+                       a.voxel, a.max_range, a.ret_mask, a.gate, a.T_dev);
     const PrecParams q = a.refresh ? *a.refresh : PrecParams{};
-    hipLaunchKernelGGL(k_ins_finish, dim3(256), dim3(BLOCK), 0, c->stream, c->touched, c->ctr, c->vox_nin,
-                       c->vox_quota, c->vox_dirty, c->dirty_list, a.gate, a.refresh ? 1 : 0, q, c->vox_key,
-                       c->vox_slot, c->vox_mom, c->slots, c->vox_prec);
+    hipLaunchKernelGGL(k_ins_finish, dim3(256), dim3(BLOCK), 0, c->stream, M, a.gate, a.refresh ? 1 : 0, q);
     if (!a.refresh) c->dirty_maybe = true;
+    return hipGetLastError();
+}
+
+hipError_t launch_insert_seq(hipStream_t s, const InsSeq *D, int32_t S, int32_t n_bound, float inv_v, float voxel,
+                             float max_range, int32_t ret_mask, const PrecParams &q) {
+    if (S <= 0 || n_bound <= 0) return hipSuccess;
+    const int g = grid_for(n_bound);
+    hipLaunchKernelGGL(k_ins_claim_seq, dim3(g, S), dim3(BLOCK), 0, s, D, inv_v, max_range, ret_mask);
+    hipLaunchKernelGGL(k_ins_count_seq, dim3(g, S), dim3(BLOCK), 0, s, D);
+    hipLaunchKernelGGL(k_ins_quota_seq, dim3(64, S), dim3(BLOCK), 0, s, D);
+    hipLaunchKernelGGL(k_ins_accum_seq, dim3(g, S), dim3(BLOCK), 0, s, D, inv_v, voxel, max_range, ret_mask);
+    hipLaunchKernelGGL(k_ins_finish_seq, dim3(8, S), dim3(BLOCK), 0, s, D, q);
     return hipGetLastError();
 }
 

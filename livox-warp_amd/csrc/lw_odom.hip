@@ -111,11 +111,20 @@ __global__ void k_frame_check(const RegState *st, RegState *st1, FrameCtl *fc, c
 // After the (possible) second registration: the frame's pose and trust, and the keyframe test
 // (odom.py:650-651, 712-717) against the last keyframe, with the float32 pose an insert uses. counts gets the
 // map's counts (ctr[0..1]) as they stand before the insert (which, when it runs, writes them again after).
-__global__ void k_keyframe_check(const RegState *st, FrameCtl *fc, const int32_t *ctr, int32_t *counts) {
+__device__ inline void keyframe_decide(const RegState *st, FrameCtl *fc, const int32_t *ctr, int32_t *counts) {
     counts[0] = ctr[0];
     counts[1] = ctr[1];
     double T[16];
     bool ok;
+    if (fc->warm) {  // odom.py:650-651, 669-670: a warm-up frame keeps its pose and is inserted when it has points
+        for (int k = 0; k < 16; ++k) {
+            fc->T[k] = fc->T_pred[k];
+            fc->T32[k] = (float)fc->T_pred[k];
+        }
+        fc->ok = fc->skipped ? 0 : 1;
+        fc->keyframe = fc->ok;
+        return;
+    }
     if (fc->redo) {
         ok = reg_outcome(st, fc->T1, fc->min_corr, T);
     } else {
@@ -141,6 +150,17 @@ __global__ void k_keyframe_check(const RegState *st, FrameCtl *fc, const int32_t
         }
     }
     fc->keyframe = key ? 1 : 0;
+}
+
+__global__ void k_keyframe_check(const RegState *st, FrameCtl *fc, const int32_t *ctr, int32_t *counts) {
+    keyframe_decide(st, fc, ctr, counts);
+}
+
+__global__ void k_keyframe_check_seq(const KeySeq *D, int32_t S) {
+    const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const KeySeq d = D[s];
+    keyframe_decide(d.st, d.fc, d.ctr, d.counts);
 }
 
 // ---- one workgroup: deskew, thinning and the registration's start ------------------------------------
@@ -170,27 +190,6 @@ constexpr int PREP_BLOCK = 32, PREP_BLOCK_KEYS = 16, PREP_COUNTERS = 8192;
 static_assert(2 * PREP_BLOCK_KEYS <= PREP_BLOCK && 4 * PREP_BLOCK_KEYS <= PROBES, "no cluster of PROBES slots");
 static_assert(PREP_ROUNDS % PREP_BATCH == 0 && PREP_ROUNDS <= 32, "one bit per round");
 
-struct PrepArgs {
-    const float *raw, *t;  // the frame: raw points and times
-    const uint32_t *attr;
-    float *xyz;            // the deskewed points
-    const int32_t *m_dev;  // the frame's appended points (capped at frame_cap)
-    int32_t frame_cap;
-    float t_mid, inv_dt;
-    float xi[6];           // the deskew twist (the first preparation)
-    float inv_v, min_range, max_range;
-    int32_t ret_mask;
-    int32_t ds_mask;       // the global thinning table (the fallback)
-    int64_t *ds_keys;
-    int32_t *ds_idx;
-    float *ds_xyz;         // the thinned points, in index order
-    int32_t *n_out;        // their count
-    RegState *st;          // the registration state to set up
-    const double *Tp;      // the prediction (16 doubles) and the robust-kernel scales (64 floats)
-    uint32_t *flags;       // the persistent loop's row flags
-    RegState *st1;         // given: the frame check runs first, and the rest only when it asks for a re-deskew
-    FrameCtl *fc;
-};
 
 // The claim of one key in the LDS table: the slot, or -1 when none is free within PROBES slots; fresh when
 // this claim put the key in.
@@ -226,7 +225,7 @@ __device__ inline int32_t global_claim(int64_t *ds_keys, int32_t *ds_idx, int32_
     return -1;
 }
 
-__global__ __launch_bounds__(PREP_THREADS) void k_frame_prep(PrepArgs A) {
+__device__ __forceinline__ void frame_prep(const PrepArgs &A) {
     __shared__ int64_t tkey[PREP_SLOTS];
     __shared__ int32_t tidx[PREP_SLOTS];
     __shared__ int32_t cnt[PREP_ROUNDS * PREP_WAVES];
@@ -373,9 +372,29 @@ __global__ __launch_bounds__(PREP_THREADS) void k_frame_prep(PrepArgs A) {
     }
 }
 
+__global__ __launch_bounds__(PREP_THREADS) void k_frame_prep(PrepArgs A) { frame_prep(A); }
+
+// One frame per workgroup: frame blockIdx.x of a batch.
+__global__ __launch_bounds__(PREP_THREADS) void k_frame_prep_seq(const PrepArgs *D) {
+    const PrepArgs A = D[blockIdx.x];
+    frame_prep(A);
+}
+
 }  // namespace
 
 int32_t frame_prep_max() { return PREP_THREADS * PREP_ROUNDS; }
+
+hipError_t launch_frame_prep_seq(hipStream_t s, const PrepArgs *D, int32_t S) {
+    if (S <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_frame_prep_seq, dim3(S), dim3(PREP_THREADS), 0, s, D);
+    return hipGetLastError();
+}
+
+hipError_t launch_keyframe_check_seq(hipStream_t s, const KeySeq *D, int32_t S) {
+    if (S <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_keyframe_check_seq, dim3((S + 63) / 64), dim3(64), 0, s, D, S);
+    return hipGetLastError();
+}
 
 hipError_t launch_frame_prep(lw_ctx *c, const double *Tp, float t_mid, float inv_dt, const float *xi, float inv_v,
                              float min_range, float max_range, int32_t ret_mask, bool check) {

@@ -198,3 +198,78 @@ def replay(orc, events, P, slots):
     with ThreadPoolExecutor(REPLICAS + 1) as ex:
         maps = list(ex.map(worker, range(REPLICAS + 1)))
     return out, maps[REPLICAS]
+
+
+def read_voxels(L, ctx):
+    """A context's map voxels in claim order (lw_map_read_voxels): keys, rsum, router, rcnt."""
+    cnt = (ctypes.c_int32 * 1)()
+    assert L.lw_map_read_voxels(ctx, 0, None, None, None, None, cnt) == 0
+    V = int(cnt[0])
+    keys, rsum = np.empty(V, np.int64), np.empty((V, 3), np.float32)
+    router, rcnt = np.empty((V, 3, 3), np.float32), np.empty(V, np.int32)
+    P = ctypes.POINTER
+    assert L.lw_map_read_voxels(ctx, V, keys.ctypes.data_as(P(ctypes.c_int64)), rsum.ctypes.data_as(P(ctypes.c_float)),
+                                router.ctypes.data_as(P(ctypes.c_float)), rcnt.ctypes.data_as(P(ctypes.c_int32)),
+                                cnt) == 0
+    return keys, rsum, router, rcnt
+
+
+class BatchRecorder:
+    """Watches chosen sequences of a BatchOdometry: after each step, what each of their registrations ran on
+    (the thinned points read from the sequence's context; the first registration's set rebuilt by the context's
+    own deskew and thinning from the frame's raw points), the start poses, the device's map after every insert,
+    and the frame's thinning and deskew for the oracle checks. events[s] has the form replay() takes."""
+
+    def __init__(self, bo, seqs):
+        self.bo, self.seqs = bo, list(seqs)
+        self.events = {s: [] for s in self.seqs}
+        self.thin_checks = []  # (deskewed xyz, attr, thinned set of the last registration)
+        self.deskew_checks = []  # (raw, t, t_mid, xi, deskewed)
+
+    def _thin(self, s, m, xi, t_mid):
+        bo, L, ctx = self.bo, self.bo._lib, self.bo.context(s)
+        x = np.ascontiguousarray(xi, np.float32)
+        assert L.lw_deskew(ctx, m, t_mid, 1.0 / bo.frame_dt, x.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) == 0
+        n = (ctypes.c_int32 * 2)()
+        assert L.lw_thin(ctx, m, 2.0 / bo.reg_voxel, bo.min_range, bo.max_range, int(bo.ret_mask), n) == 0
+        return self._read_thin(s, int(n[0]))
+
+    def _read_thin(self, s, n):
+        ds = np.empty((n, 3), np.float32)
+        assert self.bo._lib.lw_thin_read(self.bo.context(s), ds.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n) == 0
+        return ds
+
+    def after_step(self, X, A, Tt, k):
+        bo = self.bo
+        out, vin = bo.last, bo._vin
+        torch.cuda.synchronize()
+        for s in self.seqs:
+            L, ctx = bo._lib, bo.context(s)
+            m = int(out["m"][s])
+            nreg = int(out["registrations"][s])
+            if nreg > 0:
+                raw, attr, t = X[s, :m], A[s, :m], Tt[s, :m]
+                f_raw, f_xyz, f_attr, f_t, _ = bo._bufs[s]
+                xyz = f_xyz[:m].cpu().numpy()
+                a = attr.cpu().numpy().view(np.uint32)
+                last = self._read_thin(s, int(out["reg"]["n_ds"][s, nreg - 1]))
+                t_mid, xi0, xi = float(vin["t_mid"][s]), vin["xi"][s].copy(), out["xi"][s].copy()
+                self.thin_checks.append((xyz, a, last))
+                self.deskew_checks.append((raw.cpu().numpy(), t.cpu().numpy(), t_mid, xi, xyz))
+                sets = [last]
+                if nreg == 2:
+                    # the frame into the context's own buffers, then its first deskew and thinning again
+                    f_raw[:m].copy_(raw)
+                    f_attr[:m].copy_(attr)
+                    f_t[:m].copy_(t)
+                    torch.cuda.synchronize()
+                    sets = [self._thin(s, m, xi0, t_mid), last]
+                    assert np.array_equal(self._thin(s, m, xi, t_mid), last)
+                start = vin["T_pred"][s].reshape(4, 4).copy()
+                for r, ds in enumerate(sets):
+                    Tg = out["reg"]["T"][s, r].reshape(4, 4).copy()
+                    assert len(ds) == int(out["reg"]["n_ds"][s, r])
+                    self.events[s].append(("reg", ds, start, Tg, k))
+                    start = Tg
+            if out["keyframe"][s]:
+                self.events[s].append(("map",) + read_voxels(L, ctx))
