@@ -598,6 +598,55 @@ def learned_prior(args, torch, lw, dev):
     return res
 
 
+def multi_sequence(args, torch, lw, dev, seq, single_fps: float):
+    """Independent sequences stepped together (BatchOdometry): args.batch_seqs maps, each fed one of
+    args.batch_variants noise realisations of the same walk (frames_device seeds 100..), 30 untimed frames, then
+    args.batch_frames timed frames; summed frames/s, the multiple of the single-stream headline, and the batched
+    registration kernel's algorithmic bytes over its HIP-event launch time."""
+    S, V, F = args.batch_seqs, args.batch_variants, SEQ_PREFIX + args.batch_frames
+    var = [seq.frames_device(F, dev, seed=100 + v) for v in range(V)]
+    X = torch.stack([var[s % V][0] for s in range(S)]).transpose(0, 1).contiguous()
+    A = torch.stack([var[s % V][1] for s in range(S)]).transpose(0, 1).contiguous()
+    T = torch.stack([var[s % V][2] for s in range(S)]).transpose(0, 1).contiguous()
+    del var
+    bo = lw.BatchOdometry(S, dev, reg_slots=1 << 18, frame_cap=1 << 15)
+    try:
+        bo.reg_voxel, bo.max_iter, bo.acc_mode = 0.2, 20, args.acc
+        bo.reset(np.stack([seq.pose(0.05)] * S))
+        for k in range(SEQ_PREFIX):
+            bo.step(X[k], A[k], T[k], k)
+        torch.cuda.synchronize()
+        bo.prof_enable(True)
+        bo.prof_reset()
+        regs0 = bo.totals["registrations"]
+        t0 = time.perf_counter()
+        for k in range(SEQ_PREFIX, F):
+            bo.step(X[k], A[k], T[k], k)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        pr = bo.prof_read()
+        regs = bo.totals["registrations"] - regs0
+        drifts = [drift(seq, bo.traj[s])["final_m"] for s in range(0, S, S // 4)]
+    finally:
+        bo.close()
+        del X, A, T
+        torch.cuda.empty_cache()
+    fps = S * args.batch_frames / el
+    launch_s = pr[1] * 1e-3 / max(pr[0], 1.0)
+    bpl = accumulation_bytes(pr) / max(pr[0], 1.0)
+    achieved = bpl / launch_s / 1e9 if launch_s > 0 else 0.0
+    return {"sequences": S, "noise_variants": V, "frames_timed_per_sequence": args.batch_frames,
+            "value": round(fps, 1), "unit": "frames/s (summed over sequences)",
+            "x_single_stream": round(fps / single_fps, 2), "ms_per_frame_step": round(el * 1e3 / args.batch_frames, 3),
+            "registrations": int(regs), "final_drift_m_sampled": [round(d, 4) for d in drifts],
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic("k_register_seq<0>"),
+                         "kernel": "k_register_seq<0> (one launch per chain step: every sequence's whole "
+                                   "registration, one workgroup each)",
+                         "launches": int(pr[0]), "avg_launch_us": round(launch_s * 1e6, 2),
+                         "bytes_per_launch": round(bpl, 1), "time_share": round(pr[1] * 1e-3 / el, 4)}}
+
+
 def copy_probe(torch, dev, mib: int):
     """A plain device-to-device copy of known size, for calibrating hardware byte counters against."""
     n = mib * (1 << 20) // 4
@@ -737,6 +786,9 @@ def main():
     ap.add_argument("--no-grid", action="store_true", help="skip the prior-grid / coarse-search leg")
     ap.add_argument("--no-lvxr", action="store_true", help="skip the LVXR replay leg")
     ap.add_argument("--no-worker", action="store_true", help="skip the odometry-worker leg")
+    ap.add_argument("--batch-seqs", type=int, default=256, help="sequences of the multi-sequence leg (0: skip)")
+    ap.add_argument("--batch-variants", type=int, default=16, help="distinct noise realisations among them")
+    ap.add_argument("--batch-frames", type=int, default=100, help="timed frames per sequence")
     ap.add_argument("--loc-queries", type=int, default=4)
     ap.add_argument("--loc-hyps", type=int, default=4096)
     ap.add_argument("--loc-reps", type=int, default=3)
@@ -778,6 +830,8 @@ def main():
         out = odometry_headline(args, torch, lw, dev, odo, X, A, Tt, spans, seq, n_frames, per, t_gen)
     finally:
         odo.close()
+    if args.batch_seqs > 0:
+        out["multi_sequence"] = multi_sequence(args, torch, lw, dev, seq, out["value"])
     if not args.no_c1:
         out["single_registration"] = single_registration(args, torch, lw, dev)
     if not args.no_imap:

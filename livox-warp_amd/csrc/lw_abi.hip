@@ -446,7 +446,7 @@ int reg_setup(lw_ctx *c, const lw_reg_params *p, lw::RegLoop &L) {
 // The result of a registration from its final state (odom.py:756-757, 797-803): below min_corr thinned points
 // the prediction, untrusted; else the pose, the residual, the condition number of H (a 6x6 eigen problem on
 // the host: microseconds) and the weak flag.
-void fill_result(const lw::RegState &st, const lw_reg_params *p, lw_reg_result *res) {
+void fill_result(const lw::RegState &st, const lw_reg_params *p, lw_reg_result *res, bool cond_done = false) {
     const int32_t m = res->m;
     std::memset(res, 0, sizeof(*res));
     res->m = m;
@@ -456,7 +456,7 @@ void fill_result(const lw::RegState &st, const lw_reg_params *p, lw_reg_result *
         std::memcpy(res->T, st.T_pred, 16 * sizeof(double));
         return;
     }
-    const double cond = st.corr >= p->min_corr ? lw::gn::cond_of(st.H) : 1.0;
+    const double cond = cond_done ? st.cond : (st.corr >= p->min_corr ? lw::gn::cond_of(st.H) : 1.0);
     std::memcpy(res->T, st.T, 16 * sizeof(double));
     res->rms = st.rms;
     res->rms0 = st.rms0;
@@ -1141,13 +1141,13 @@ int lw_batch_frame(lw_batch *b, const lw_reg_params *p, const lw_frame_in *in, c
         std::memcpy(o.xi, fc.xi1, sizeof(o.xi));
         o.map_voxels = counts[2 * s];
         o.map_overflow = counts[2 * s + 1];
-        if (o.registrations >= 1) {
+        if (o.registrations >= 1) {  // the condition numbers come from the registration kernel
             o.reg[0].m = fc.m;
-            fill_result(st1s[s], p, &o.reg[0]);
+            fill_result(st1s[s], p, &o.reg[0], true);
         }
         if (o.registrations >= 2) {
             o.reg[1].m = fc.m;
-            fill_result(sts[s], p, &o.reg[1]);
+            fill_result(sts[s], p, &o.reg[1], true);
         }
     }
     return rc;

@@ -991,6 +991,9 @@ __global__ __launch_bounds__(64 * SEQ_WAVES) void k_register_seq(const SeqReg *D
         __syncthreads();
         if (s_flag) break;
     }
+    // odom.py:797-800: the condition number of the last accepted H, here (a 6x6 eigen problem in one lane: a
+    // batch's hundreds of them would otherwise queue up on the host)
+    if (threadIdx.x == 0) st->cond = st->corr >= L.min_corr ? gn::cond_of(st->H) : 1.0;
 }
 
 // Tp: 16 doubles (the prediction) followed by 64 floats (the per-iteration robust-kernel scale); T_dev, when

@@ -269,7 +269,7 @@ class BatchRecorder:
                 for r, ds in enumerate(sets):
                     Tg = out["reg"]["T"][s, r].reshape(4, 4).copy()
                     assert len(ds) == int(out["reg"]["n_ds"][s, r])
-                    self.events[s].append(("reg", ds, start, Tg, k))
+                    self.events[s].append(("reg", ds, start, Tg, k, float(out["reg"]["cond"][s, r])))
                     start = Tg
             if out["keyframe"][s]:
                 self.events[s].append(("map",) + read_voxels(L, ctx))

@@ -48,15 +48,17 @@ def run(lw, orc):
 
 def test_batch_registrations_match_the_oracle(run, orc, gpu_ok):
     seq, X, A, Tt, M, rec, outs, P, traj, totals = run
-    worst, n = (0.0, 0.0), 0
+    worst, n, cond_rel = (0.0, 0.0), 0, 0.0
     for s, ev in rec.events.items():
         out, _ = seqcheck.replay(orc, ev, P, 1 << 20)
         errs = np.array([e for _, e in out.values()])
         worst = (max(worst[0], errs[:, 0].max()), max(worst[1], errs[:, 1].max()))
         n += len(out)
+        # the condition number the registration kernel computes from its last H (odom.py:797-800)
+        cond_rel = max(cond_rel, max(abs(ev[i][5] - res.cond) / res.cond for i, (res, _) in out.items()))
     print(f"batch: {n} registrations of 2 watched sequences vs the oracle: max {worst[0] * 1e3:.4f} mm / "
-          f"{worst[1]:.5f} deg; totals {totals}", flush=True)
-    assert n >= 2 * (FRAMES - 12) and worst[0] < 1e-3 and worst[1] < 0.01
+          f"{worst[1]:.5f} deg, condition numbers within {cond_rel:.2e} relative; totals {totals}", flush=True)
+    assert n >= 2 * (FRAMES - 12) and worst[0] < 1e-3 and worst[1] < 0.01 and cond_rel < 1e-3
 
 
 def test_batch_thinning_and_deskew_match_the_oracle(run, orc, gpu_ok):
