@@ -144,7 +144,7 @@ def localisation(args, torch, lw, dev):
     V = loc.map_stats()[0]
     map_bytes = 16.0 * n_map + 60.0 * V  # xyz + attr per point, one record written per new voxel
     # tile autotuner on a slice of the workload, then the timed launches (graph-replayed)
-    tile, table = loc.autotune([W.scans[0]], hyps[:1, :256])
+    tile, table, spilled = loc.autotune([W.scans[0]], hyps[:1, :256])
     staged = loc._stage(W.scans, hyps)
     L = loc.eng.lib
     ctx = loc.eng.ctx
@@ -172,7 +172,8 @@ def localisation(args, torch, lw, dev):
         "value": round(n_units * args.loc_reps / el, 1), "unit": "hypotheses/s",
         "workload": f"{len(W.scans)} queries x {H} hypotheses, scans {[len(s) for s in W.scans]} points, map "
                     f"{n_map} points at 20 mm -> {V} voxels of 0.1 m in 2^{args.loc_slots} slots, 10 GN iterations at reach 1",
-        "seed": LOC_SEED, "tile": {"block": tile[0], "workgroups_per_cu": tile[1]}, "autotune_ms": table,
+        "seed": LOC_SEED, "tile": {"block": tile[0], "workgroups_per_cu": tile[1], "chunk": tile[2]}, "autotune_ms": table,
+        "autotune_skipped_spilling": [f"{b}x{g}x{c}" for b, g, c in spilled],
         "launch_ms": round(launch_s * 1e3, 3), "graph": True,
         "top1_recovered": sum(1 for e in best if e[0] < 0.3 and e[1] < 3.0), "queries": len(W.scans),
         "top1_error": [[round(e[0], 3), round(e[1], 2)] for e in best],
@@ -573,12 +574,13 @@ def learned_prior(args, torch, lw, dev):
                        f"10k points/frame, both guesses on the same frames")
     # training step at batch B: forward, loss, backward (2x the forward's products), Adam
     B = 4096
-    net = lw.prior.MLP(batch_cap=B, device=dev)
+    net = lw.prior.MLP(batch_cap=B, device=dev, stream=torch.cuda.Stream(device=dev))
     net.set_params(lw.prior.init_params(np.random.default_rng(0)))
     g = torch.Generator(device=dev)
     g.manual_seed(0)
     X = torch.randn((B, 152), generator=g, device=dev)
     Y = torch.randn((B, 6), generator=g, device=dev)
+    net.stream.wait_stream(torch.cuda.current_stream(dev))
     for _ in range(3):
         net.train_step(X, Y, 1e-4)
     torch.cuda.synchronize()
@@ -590,11 +592,20 @@ def learned_prior(args, torch, lw, dev):
     e1.record(net.stream)
     torch.cuda.synchronize()
     step_s = e0.elapsed_time(e1) * 1e-3 / steps
+    e0.record(net.stream)
+    for _ in range(steps):  # the same step launch by launch, for comparison
+        net.train_step(X, Y, 1e-4, graph=False)
+    e1.record(net.stream)
+    torch.cuda.synchronize()
+    eager_s = e0.elapsed_time(e1) * 1e-3 / steps
+    net.close()
     tflops = 3.0 * MLP_FLOPS_PER_ROW * B / step_s / 1e12
     res["training_step"] = {"batch": B, "ms": round(step_s * 1e3, 4), "samples_per_s": round(B / step_s, 1),
+                            "eager_ms": round(eager_s * 1e3, 4),
                             "roofline": {"bound": "mfma", "achieved": round(tflops, 3), "peak": 2500.0,
                                          "unit": "TFLOP/s", "frac": round(tflops / 2500.0, 6),
-                                         "kernel": "k_gemm (bf16 16x16x32 MFMA, fp32 accumulate)"}}
+                                         "kernel": "k_gemm (bf16 16x16x32 MFMA, fp32 accumulate), the step "
+                                                   "replayed from its captured hipGraph"}}
     return res
 
 

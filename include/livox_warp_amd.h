@@ -222,6 +222,7 @@ typedef struct lw_loc_params {
     int32_t grid;      /* persistent workgroups (0: four per compute unit) */
     int32_t use_graph; /* capture the launch in a hipGraph and replay it while the arguments stay the same
                           (needs a context stream other than the null stream) */
+    int32_t chunk;     /* tile: neighbour keys a lane resolves together, 7 or 14 (0: 14) */
 } lw_loc_params;
 
 /* GlobalLocalizer._icp (localize.py:517-553, kernels :126-179), batched: n_hyp hypotheses for each of n_queries
@@ -234,6 +235,10 @@ typedef struct lw_loc_params {
 int lw_loc_batch(lw_ctx *ctx, const float *scans, const int32_t *scan_off, const int32_t *order, int32_t n_queries,
                  const double *hyp_T, int32_t n_hyp, const lw_loc_params *p, double *out_T, double *out_score,
                  int32_t *out_count, int32_t *out_iters);
+/* The compiled kernel of a tile (block 64 / 128 / 256 / 512, chunk 7 / 14): info[0] scratch bytes per lane
+ * (register spills), info[1] registers per lane, info[2] static LDS bytes, info[3] largest block. The tile
+ * autotuner skips tiles that spill. Needs a device. No counterpart in the reference. */
+int lw_loc_tile_info(int32_t block, int32_t chunk, int32_t *info);
 
 /* ---- Learned pose-increment prior. No counterpart in the reference: it replaces the constant-velocity guess
  * of Odometry.predict (odom.py:591-592, used by _estimate at odom.py:666) when enabled, and plain ICP from that
@@ -261,6 +266,12 @@ int lw_mlp_loss(lw_mlp *m, const float *y, const float *target, int32_t n, float
 int lw_mlp_backward(lw_mlp *m, int32_t n);
 /* One Adam step (bias-corrected; step counts from 1) on the float32 masters, refreshing the BF16 copy. */
 int lw_mlp_adam(lw_mlp *m, float lr, float beta1, float beta2, float eps, int32_t step);
+/* One training step -- forward, loss into *loss (device float), backward, Adam at `step` -- on x (n x 152
+ * float32, device, row stride ldx) and target (n x 6 float32, device). use_graph = 1 (with a stream other than
+ * the null stream): the step's launches are captured once into a hipGraph and replayed while x, ldx, target,
+ * loss and n stay the same, Adam's scalars set before each replay; same results as the eager sequence. */
+int lw_mlp_train_step(lw_mlp *m, const float *x, int64_t ldx, const float *target, int32_t n, float lr, float beta1,
+                      float beta2, float eps, int32_t step, float *loss, int32_t use_graph);
 
 /* The 152 inputs of the open frame's m points (the bound f_raw buffer, sensor frame) into out (device):
  * motion[24] as given, then 64 range bins over [0, r_max) and 64 elevation bins over [-e_max, e_max), each

@@ -62,7 +62,7 @@ class BatchSrc(ctypes.Structure):
 
 class LocParams(ctypes.Structure):
     _fields_ = [("damping", _f64), ("reg_voxel", _f32), ("noise", _f32), ("alpha", _f32), ("iters", _i32),
-                ("min_count", _i32), ("block", _i32), ("grid", _i32), ("use_graph", _i32)]
+                ("min_count", _i32), ("block", _i32), ("grid", _i32), ("use_graph", _i32), ("chunk", _i32)]
 
 
 class MapView(ctypes.Structure):
@@ -103,6 +103,7 @@ SIGNATURES = {
     "lw_set_persistent": (_i32, [_vp, _i32]),
     "lw_odom_frame": (_i32, [_vp, ctypes.POINTER(RegParams), ctypes.POINTER(FrameIn), ctypes.POINTER(FrameOut)]),
     "lw_loc_batch": (_i32, [_vp, _vp, _vp, _vp, _i32, _vp, _i32, ctypes.POINTER(LocParams), _vp, _vp, _vp, _vp]),
+    "lw_loc_tile_info": (_i32, [_i32, _i32, _i32p]),
     "lw_mlp_param_count": (_i32, []),
     "lw_mlp_create": (_i32, [ctypes.POINTER(_vp), _i32, _vp]),
     "lw_mlp_destroy": (_i32, [_vp]),
@@ -114,6 +115,7 @@ SIGNATURES = {
     "lw_mlp_loss": (_i32, [_vp, _vp, _vp, _i32, _vp]),
     "lw_mlp_backward": (_i32, [_vp, _i32]),
     "lw_mlp_adam": (_i32, [_vp, _f32, _f32, _f32, _f32, _i32]),
+    "lw_mlp_train_step": (_i32, [_vp, _vp, _i64, _vp, _i32, _f32, _f32, _f32, _f32, _i32, _vp, _i32]),
     "lw_frame_features": (_i32, [_vp, _i32, _f32, _f32, _f32p, _vp]),
     "lw_prof_enable": (_i32, [_vp, _i32]),
     "lw_prof_reset": (_i32, [_vp]),

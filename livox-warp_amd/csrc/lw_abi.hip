@@ -574,7 +574,8 @@ int lw_loc_batch(lw_ctx *c, const float *scans, const int32_t *scan_off, const i
                  int32_t *out_count, int32_t *out_iters) {
     if (!c || !p || !scans || !scan_off || !order || !hyp_T || !out_T || !out_score || !out_count || !out_iters ||
         n_queries <= 0 || n_hyp <= 0 || (int64_t)n_queries * n_hyp > (1ll << 30) || p->iters < 0 || p->iters > 64 ||
-        !(p->reg_voxel > 0.0f) || (p->block != 64 && p->block != 128 && p->block != 256 && p->block != 512))
+        !(p->reg_voxel > 0.0f) || (p->block != 64 && p->block != 128 && p->block != 256 && p->block != 512) ||
+        (p->chunk != 0 && p->chunk != 7 && p->chunk != 14))
         return LW_EINVAL;
     const double voxel = p->reg_voxel;
     const float noise2 = (float)((double)p->noise * (double)p->noise);
@@ -595,6 +596,7 @@ int lw_loc_batch(lw_ctx *c, const float *scans, const int32_t *scan_off, const i
     L.iters = p->iters;
     L.min_count = p->min_count;
     L.block = p->block;
+    L.chunk = p->chunk ? p->chunk : 14;
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
     L.grid = p->grid > 0 ? p->grid : 4 * cus;
@@ -635,6 +637,18 @@ int lw_loc_batch(lw_ctx *c, const float *scans, const int32_t *scan_off, const i
         LW_CK(hipEventRecord(c->prof_ev[2 * c->prof_used + 1], c->stream));
         c->prof_used++;
     }
+    return 0;
+}
+
+int lw_loc_tile_info(int32_t block, int32_t chunk, int32_t *info) {
+    if (!info || (block != 64 && block != 128 && block != 256 && block != 512) || (chunk != 7 && chunk != 14))
+        return LW_EINVAL;
+    hipFuncAttributes fa{};
+    LW_CK(lw::loc_tile_attributes(block, chunk, &fa));
+    info[0] = (int32_t)fa.localSizeBytes;
+    info[1] = (int32_t)fa.numRegs;
+    info[2] = (int32_t)fa.sharedSizeBytes;
+    info[3] = (int32_t)fa.maxThreadsPerBlock;
     return 0;
 }
 

@@ -152,50 +152,94 @@ __global__ __launch_bounds__(256) void k_grid_jfa(const Seed *__restrict__ src, 
     dst[c] = best;
 }
 
-// The same step for steps of 1 and 2 cells through LDS: a workgroup holds a 64 x 4 x 4 block of cells and its
-// halo of S cells (outside the grid: idx -1, which the step skips as it skips an empty seed), so each seed
-// record is read from memory once per block instead of once per neighbour. Same neighbour order and ties as
-// k_grid_jfa, so the same result.
-constexpr int JFA_TX = 64, JFA_TY = 4, JFA_TZ = 4;
+// The same step through LDS, for steps up to JFA_MAX_STEP cells. A step of s only ever combines cells whose
+// coordinates agree mod s, so a workgroup takes JFA_TX consecutive cells of JFA_TY rows y0, y0 + s, ...,
+// y0 + (JFA_TY - 1) s, and walks the planes z = rz, rz + s, ... of its residue rz. It keeps three planes of
+// that lattice in LDS -- z - s, z and z + s, each with a halo of s cells in x and one lattice row in y
+// (outside the grid: idx -1, which the step skips as it skips an empty seed) -- and loads each plane once, the
+// next one while it computes the current: a seed record is read (JFA_TX + 2 s)(JFA_TY + 2) / (JFA_TX JFA_TY)
+// times, 1.3-1.9x for steps 1-16, instead of 27 times. Same neighbour order and ties as k_grid_jfa, so the
+// same result.
+constexpr int JFA_TX = 64, JFA_TY = 8, JFA_MAX_STEP = 16, JFA_LX = JFA_TX + 2 * JFA_MAX_STEP, JFA_LY = JFA_TY + 2;
+constexpr int JFA_PREFETCH = (JFA_LX * JFA_LY + 255) / 256;
 
-template <int S>
-__global__ __launch_bounds__(256) void k_grid_jfa_lds(const Seed *__restrict__ src, Seed *__restrict__ dst,
-                                                      GridGeo g) {
-    constexpr int LX = JFA_TX + 2 * S, LY = JFA_TY + 2 * S, LZ = JFA_TZ + 2 * S;
-    __shared__ Seed tile[LZ * LY * LX];
-    const int x0 = blockIdx.x * JFA_TX, y0 = blockIdx.y * JFA_TY, z0 = blockIdx.z * JFA_TZ;
-    for (int t = threadIdx.x; t < LX * LY * LZ; t += blockDim.x) {
-        const int lx = t % LX, ly = (t / LX) % LY, lz = t / (LX * LY);
-        const int j = grid_flat(g, x0 - S + lx, y0 - S + ly, z0 - S + lz);
-        tile[t] = j >= 0 ? src[j] : Seed{-1, 0.0f, 0.0f, 0.0f};
+// Rows of the lattice of residue ry (mod s) and the workgroups per residue that cover them.
+inline int32_t jfa_y_tiles(int32_t ny, int32_t s) { return ((ny + s - 1) / s + JFA_TY - 1) / JFA_TY; }
+
+// Plane zz of a slab tile's lattice (with its halo) into registers, then into an LDS slot.
+__device__ inline void jfa_fetch(const Seed *__restrict__ src, const GridGeo &g, int32_t x0, int32_t y0, int32_t s,
+                                 int32_t zz, Seed (&pre)[JFA_PREFETCH]) {
+    const int32_t lxs = JFA_TX + 2 * s, n_load = lxs * JFA_LY;
+#pragma unroll
+    for (int k = 0; k < JFA_PREFETCH; ++k) {
+        const int t = threadIdx.x + 256 * k;
+        int32_t j = -1;
+        if (t < n_load) j = grid_flat(g, x0 - s + t % lxs, y0 + (t / lxs - 1) * s, zz);
+        pre[k] = j >= 0 ? src[j] : Seed{-1, 0.0f, 0.0f, 0.0f};
     }
+}
+
+__device__ inline void jfa_store(Seed (*plane)[JFA_LX], int32_t s, const Seed (&pre)[JFA_PREFETCH]) {
+    const int32_t lxs = JFA_TX + 2 * s, n_load = lxs * JFA_LY;
+#pragma unroll
+    for (int k = 0; k < JFA_PREFETCH; ++k) {
+        const int t = threadIdx.x + 256 * k;
+        if (t < n_load) plane[t / lxs][t % lxs] = pre[k];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_grid_jfa_slab(const Seed *__restrict__ src, Seed *__restrict__ dst,
+                                                       GridGeo g, int32_t s, int32_t y_tiles) {
+    __shared__ Seed win[3][JFA_LY][JFA_LX];
+    const int32_t ry = blockIdx.y / y_tiles, ty = blockIdx.y % y_tiles, rz = blockIdx.z;
+    const int32_t x0 = blockIdx.x * JFA_TX, y0 = ry + ty * JFA_TY * s;
+    if (y0 >= g.ny || rz >= g.nz) return;  // the whole workgroup: no row of this tile in the grid
+    Seed pre[JFA_PREFETCH];
+    int lo = 0, mid = 1, hi = 2;  // slots of planes z - s, z, z + s
+    jfa_fetch(src, g, x0, y0, s, rz - s, pre);
+    jfa_store(win[lo], s, pre);
+    jfa_fetch(src, g, x0, y0, s, rz, pre);
+    jfa_store(win[mid], s, pre);
+    jfa_fetch(src, g, x0, y0, s, rz + s, pre);
+    jfa_store(win[hi], s, pre);
     __syncthreads();
-    const int lx = threadIdx.x % JFA_TX, ly = threadIdx.x / JFA_TX;
-    const int ix = x0 + lx, iy = y0 + ly;
-    if (ix >= g.nx || iy >= g.ny) return;
-    for (int lz = 0; lz < JFA_TZ; ++lz) {
-        const int iz = z0 + lz;
-        if (iz >= g.nz) break;
-        const int32_t c = (iz * g.ny + iy) * g.nx + ix;
-        float px, py, pz;
-        cell_centre(g, ix, iy, iz, px, py, pz);
-        Seed best = tile[((lz + S) * LY + (ly + S)) * LX + (lx + S)];
-        float bd = 1.0e30f;
-        if (best.idx >= 0) bd = dist2_xyz(px, py, pz, best);
-        for (int dz = -1; dz <= 1; ++dz)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    if (dx == 0 && dy == 0 && dz == 0) continue;
-                    const Seed sd = tile[((lz + S + dz * S) * LY + (ly + S + dy * S)) * LX + (lx + S + dx * S)];
-                    if (sd.idx >= 0 && sd.idx != best.idx) {
-                        const float d2 = dist2_xyz(px, py, pz, sd);
-                        if (d2 < bd) {
-                            bd = d2;
-                            best = sd;
+    for (int32_t z = rz; z < g.nz; z += s) {
+        jfa_fetch(src, g, x0, y0, s, z + 2 * s, pre);  // in flight while this plane is computed
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const int lx = threadIdx.x % JFA_TX, ly = threadIdx.x / JFA_TX + 4 * half;
+            const int32_t ix = x0 + lx, iy = y0 + ly * s;
+            if (ix < g.nx && iy < g.ny) {
+                float px, py, pz;
+                cell_centre(g, ix, iy, z, px, py, pz);
+                Seed best = win[mid][ly + 1][lx + s];
+                float bd = 1.0e30f;
+                if (best.idx >= 0) bd = dist2_xyz(px, py, pz, best);
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const int slot = dz < 0 ? lo : (dz == 0 ? mid : hi);
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            if (dx == 0 && dy == 0 && dz == 0) continue;
+                            const Seed sd = win[slot][ly + 1 + dy][lx + s + dx * s];
+                            if (sd.idx >= 0 && sd.idx != best.idx) {
+                                const float d2 = dist2_xyz(px, py, pz, sd);
+                                if (d2 < bd) {
+                                    bd = d2;
+                                    best = sd;
+                                }
+                            }
                         }
-                    }
                 }
-        dst[c] = best;
+                dst[(z * g.ny + iy) * g.nx + ix] = best;
+            }
+        }
+        __syncthreads();  // every lane is done with plane z - s
+        jfa_store(win[lo], s, pre);  // it becomes plane z + 2 s
+        const int t = lo;
+        lo = mid;
+        mid = hi;
+        hi = t;
+        __syncthreads();
     }
 }
 
@@ -588,15 +632,15 @@ extern "C" int lw_grid_build(const float *xyz, int32_t n, const float *geo, cons
     if (n > 0) hipLaunchKernelGGL(lw::k_grid_seed, dim3(lw::blocks_for(n)), dim3(256), 0, s, xyz, n, g, nn);
     hipLaunchKernelGGL(lw::k_grid_unpack, dim3(fill_blocks), dim3(256), 0, s, nn, xyz, ncell, a);
     const dim3 cells(lw::blocks_for(g.nx), (unsigned)g.ny, (unsigned)g.nz);
-    const dim3 tiles((unsigned)((g.nx + lw::JFA_TX - 1) / lw::JFA_TX), (unsigned)((g.ny + lw::JFA_TY - 1) / lw::JFA_TY),
-                     (unsigned)((g.nz + lw::JFA_TZ - 1) / lw::JFA_TZ));
     for (int k = 0; k < n_steps; ++k) {
-        if (steps[k] == 1)
-            hipLaunchKernelGGL(lw::k_grid_jfa_lds<1>, tiles, dim3(256), 0, s, a, b, g);
-        else if (steps[k] == 2)
-            hipLaunchKernelGGL(lw::k_grid_jfa_lds<2>, tiles, dim3(256), 0, s, a, b, g);
-        else
-            hipLaunchKernelGGL(lw::k_grid_jfa, cells, dim3(256), 0, s, a, b, g, steps[k]);
+        const int32_t st = steps[k];
+        if (st <= lw::JFA_MAX_STEP) {
+            const int32_t yt = lw::jfa_y_tiles(g.ny, st);
+            const dim3 slabs((unsigned)((g.nx + lw::JFA_TX - 1) / lw::JFA_TX), (unsigned)(st * yt), (unsigned)st);
+            hipLaunchKernelGGL(lw::k_grid_jfa_slab, slabs, dim3(256), 0, s, a, b, g, st, yt);
+        } else {
+            hipLaunchKernelGGL(lw::k_grid_jfa, cells, dim3(256), 0, s, a, b, g, st);
+        }
         std::swap(a, b);
     }
     hipLaunchKernelGGL(lw::k_grid_df, cells, dim3(256), 0, s, a, g, nn, df);

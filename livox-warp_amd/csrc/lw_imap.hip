@@ -81,44 +81,102 @@ __device__ inline bool keep_raw(float px, float py, float pz, uint32_t a, const 
 }
 
 // gpu.py:373-413 (k_map_insert): the point through its frame's pose, claimed at its voxel's key; hits keep
-// counting, the float32 sums stop after IMAP_COUNT_CAP of them (atomicAdd hands out unique old counts, so
-// exactly that many add), first and last time by atomic min and max.
+// counting, the float32 sums stop after IMAP_COUNT_CAP of them, first and last time by atomic min and max.
+//
+// Points arrive in scan order, so neighbouring lanes often fall in one voxel: a wave first combines each run of
+// consecutive lanes with the same key (a segmented shuffle reduction of the sums, the count and the time
+// range), and the run's first lane claims the slot and makes one set of atomics for the whole run. The count
+// still grows by one per point; atomicAdd on it hands out the run's range of old counts, and exactly the
+// points below IMAP_COUNT_CAP add to the sums: all of the run at once, or, for the run that crosses the cap,
+// its first (cap - old) points one by one. Which points make up the capped sums depends on the arrival order,
+// as it does in the reference's per-point atomics.
 __global__ __launch_bounds__(IMAP_BLOCK) void k_imap_insert(const float *sx, const uint32_t *sa, const float *st,
                                                             const int32_t *sf, int32_t n, const float *poses,
                                                             MapView v, float inv_voxel, int32_t mask, MapRec *recs,
                                                             int32_t *stats, uint32_t *bricks, int32_t nbits) {
     const int32_t i = blockIdx.x * IMAP_BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const float px = sx[3 * (int64_t)i], py = sx[3 * (int64_t)i + 1], pz = sx[3 * (int64_t)i + 2];
-    const uint32_t a = sa[i];
-    if (!keep_raw(px, py, pz, a, v)) return;
-    const float *T = poses + (int64_t)(sf[i] & (IMAP_POSE_SLOTS - 1)) * 16;
-    float wx, wy, wz;
-    transform_point(T, px, py, pz, wx, wy, wz);
-    const int64_t key = voxel_key_ijk(ifloor(wx * inv_voxel), ifloor(wy * inv_voxel), ifloor(wz * inv_voxel));
-    const int32_t h = slot_hash(key) & mask;
-    const float t = st[i];
-    for (int probe = 0; probe < PROBES; ++probe) {
-        MapRec *r = &recs[(h + probe) & mask];
-        const unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long *>(&r->key),
-                                                  0xFFFFFFFFFFFFFFFFull, (unsigned long long)key);
-        if ((int64_t)prev == -1 || (int64_t)prev == key) {
-            if ((int64_t)prev == -1) {
-                atomicAdd(&stats[0], 1);
-                brick_set(bricks, brick_bit(key, nbits));
-            }
-            if (atomicAdd(&r->cnt, 1) < IMAP_COUNT_CAP) {
-                unsafeAtomicAdd(&r->acc.x, wx);
-                unsafeAtomicAdd(&r->acc.y, wy);
-                unsafeAtomicAdd(&r->acc.z, wz);
-                unsafeAtomicAdd(&r->acc.w, (float)(a & 255u));
-            }
-            atomicMax(&r->last, t);
-            atomicMin(&r->first, t);
-            return;
+    const int lane = lane_id();
+    bool live = i < n;
+    float wx = 0.0f, wy = 0.0f, wz = 0.0f, refl = 0.0f, t = 0.0f;
+    int64_t key = -2 - lane;  // a dead lane's key matches no other lane's (voxel keys are non-negative)
+    if (live) {
+        const float px = sx[3 * (int64_t)i], py = sx[3 * (int64_t)i + 1], pz = sx[3 * (int64_t)i + 2];
+        const uint32_t a = sa[i];
+        live = keep_raw(px, py, pz, a, v);
+        if (live) {
+            const float *T = poses + (int64_t)(sf[i] & (IMAP_POSE_SLOTS - 1)) * 16;
+            transform_point(T, px, py, pz, wx, wy, wz);
+            key = voxel_key_ijk(ifloor(wx * inv_voxel), ifloor(wy * inv_voxel), ifloor(wz * inv_voxel));
+            refl = (float)(a & 255u);
+            t = st[i];
         }
     }
-    atomicAdd(&stats[1], 1);  // no slot along this probe chain
+    // runs of equal keys: a run starts where the key differs from the lane below's
+    const int64_t below = __shfl_up(key, 1, 64);
+    const uint64_t heads = __ballot(lane == 0 || below != key);
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
+    const int head = 63 - __clzll(heads & upto);
+    const uint64_t after = heads & ~upto;
+    const int last = after ? __ffsll((long long)after) - 2 : 63;
+    // segmented sums over lanes [lane, last]: the run's totals end up at its head
+    float rx = wx, ry = wy, rz = wz, rr = refl, t_min = live ? t : 1.0e30f, t_max = live ? t : -1.0e30f;
+    int32_t cnt = live ? 1 : 0;
+    LW_UNROLL for (int off = 1; off < 64; off <<= 1) {
+        const float ox = __shfl_down(rx, off, 64), oy = __shfl_down(ry, off, 64), oz = __shfl_down(rz, off, 64);
+        const float orr = __shfl_down(rr, off, 64), omin = __shfl_down(t_min, off, 64);
+        const float omax = __shfl_down(t_max, off, 64);
+        const int32_t oc = __shfl_down(cnt, off, 64);
+        if (lane + off <= last) {
+            rx = rx + ox;
+            ry = ry + oy;
+            rz = rz + oz;
+            rr = rr + orr;
+            t_min = fminf(t_min, omin);
+            t_max = fmaxf(t_max, omax);
+            cnt += oc;
+        }
+    }
+    // the head claims the slot (probing as find_slot does) and counts the whole run
+    int32_t slot = -1, old = 0;
+    if (lane == head && live) {
+        const int32_t h = slot_hash(key) & mask;
+        for (int probe = 0; probe < PROBES; ++probe) {
+            MapRec *r = &recs[(h + probe) & mask];
+            const unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long *>(&r->key),
+                                                      0xFFFFFFFFFFFFFFFFull, (unsigned long long)key);
+            if ((int64_t)prev == -1 || (int64_t)prev == key) {
+                if ((int64_t)prev == -1) {
+                    atomicAdd(&stats[0], 1);
+                    brick_set(bricks, brick_bit(key, nbits));
+                }
+                slot = (h + probe) & mask;
+                old = atomicAdd(&r->cnt, cnt);
+                atomicMax(&r->last, t_max);
+                atomicMin(&r->first, t_min);
+                break;
+            }
+        }
+        if (slot < 0) atomicAdd(&stats[1], cnt);  // no slot along this probe chain
+    }
+    slot = __shfl(slot, head, 64);
+    old = __shfl(old, head, 64);
+    if (slot < 0 || !live) return;
+    MapRec *r = &recs[slot];
+    const int32_t room = IMAP_COUNT_CAP - old;  // points of this run that still add to the sums
+    const int32_t run = __shfl(cnt, head, 64);
+    if (room >= run) {
+        if (lane == head) {
+            unsafeAtomicAdd(&r->acc.x, rx);
+            unsafeAtomicAdd(&r->acc.y, ry);
+            unsafeAtomicAdd(&r->acc.z, rz);
+            unsafeAtomicAdd(&r->acc.w, rr);
+        }
+    } else if (lane - head < room) {  // the run that crosses the cap: its first points, one by one
+        unsafeAtomicAdd(&r->acc.x, wx);
+        unsafeAtomicAdd(&r->acc.y, wy);
+        unsafeAtomicAdd(&r->acc.z, wz);
+        unsafeAtomicAdd(&r->acc.w, refl);
+    }
 }
 
 // gpu.py:247-265 (find_slot) on the records: the slot holding key, or -1.

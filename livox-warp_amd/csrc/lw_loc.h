@@ -18,12 +18,14 @@ struct LocLaunch {
     int32_t *out_count;       // Q * H: matched points of the last pass (device)
     int32_t *out_iters;       // Q * H: steps taken (device)
     unsigned long long *prof; // measurement counters, or null (part of a captured launch's identity)
-    int32_t Q, H, iters, min_count, block, grid;
+    int32_t Q, H, iters, min_count, block, grid, chunk;
     float inv_v, max_d2;
     double damping;
     float kappa[16];
 };
 
 hipError_t launch_loc(lw_ctx *c, const LocLaunch &L);
+// The compiled kernel of a tile (block 64 / 128 / 256 / 512, chunk 7 / 14): registers, scratch, LDS.
+hipError_t loc_tile_attributes(int32_t block, int32_t chunk, hipFuncAttributes *fa);
 
 }  // namespace lw

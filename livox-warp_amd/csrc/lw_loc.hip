@@ -10,7 +10,10 @@
 // voxels around each (bitmap words of a chunk loaded together, then its slots), forms the float32 terms of
 // H and b as the reference rounds them and adds them, widened, to float64 running sums; the workgroup
 // reduces (wave shuffles, then LDS) and one lane solves the 6x6 step in float64. Units never wait on each
-// other, so the grid needs no co-residency; the grid size and block size are tuning parameters.
+// other, so the grid needs no co-residency; the grid size, the block size and the chunk (neighbour keys a lane
+// resolves together: 14, two chunks of the 27, or 7, four) are tuning parameters. A chunk's keys, hashes and
+// slot records are live together with the 29 float64 running sums: 14 fills the register file at one wave
+// per SIMD (and spills at 512 threads, where two are resident), 7 leaves room for two waves per SIMD.
 #include "lw_internal.h"
 #include "lw_loc.h"
 #include "lw_search.h"
@@ -19,7 +22,6 @@ namespace lw {
 
 namespace {
 
-constexpr int LOC_C = 14;  // 27 neighbours in two chunks
 constexpr int LOC_ACC = 29;  // H (21), b (6), count, weight
 
 struct LocArgs {
@@ -48,7 +50,7 @@ struct LocArgs {
     X(4, 5, 19) X(5, 5, 20)
 
 // localize.py:531-547 for one hypothesis: returns whether the pose moved by less than the batch tolerance.
-__device__ __noinline__ bool loc_step(const double *v, double damping, int32_t min_count, double *T, double *moved) {
+__device__ inline bool loc_step(const double *v, double damping, int32_t min_count, double *T, double *moved) {
     *moved = 0.0;
     if (v[21 + 6] < (double)min_count) return false;  // too few matches: keep the pose (localize.py:535-536)
     double Hm[36], A[36], rhs[6], d[6];
@@ -67,7 +69,7 @@ __device__ __noinline__ bool loc_step(const double *v, double damping, int32_t m
     return true;
 }
 
-template <int B>
+template <int B, int C>
 __global__ __launch_bounds__(B) void k_loc(LocArgs a) {
     constexpr int W = B / 64;
     __shared__ double red[W][LOC_ACC + 1];
@@ -107,7 +109,7 @@ __global__ __launch_bounds__(B) void k_loc(LocArgs a) {
                 float wx, wy, wz;
                 transform_point(Tf, pp[0], pp[1], pp[2], wx, wy, wz);
                 Cand best{a.max_d2, -1, 27, 0.0f, 0.0f, 0.0f};
-                search_part<1, LOC_C, true>(Table{a.slots, a.occ, a.mask}, nbr, a.inv_v, wx, wy, wz, 0, 27, best, hits);
+                search_part<1, C, true>(Table{a.slots, a.occ, a.mask}, nbr, a.inv_v, wx, wy, wz, 0, 27, best, hits);
                 const int32_t s = best.s;
                 if (s < 0) continue;
                 const float mx = best.mx, my = best.my, mz = best.mz;
@@ -191,6 +193,22 @@ __global__ void k_queue_reset(uint32_t *q) { *q = 0u; }
 
 }  // namespace
 
+hipError_t loc_tile_attributes(int32_t block, int32_t chunk, hipFuncAttributes *fa) {
+    const void *f = nullptr;
+    switch (block * 100 + chunk) {
+    case 6414: f = reinterpret_cast<const void *>(&k_loc<64, 14>); break;
+    case 12814: f = reinterpret_cast<const void *>(&k_loc<128, 14>); break;
+    case 25614: f = reinterpret_cast<const void *>(&k_loc<256, 14>); break;
+    case 51214: f = reinterpret_cast<const void *>(&k_loc<512, 14>); break;
+    case 6407: f = reinterpret_cast<const void *>(&k_loc<64, 7>); break;
+    case 12807: f = reinterpret_cast<const void *>(&k_loc<128, 7>); break;
+    case 25607: f = reinterpret_cast<const void *>(&k_loc<256, 7>); break;
+    case 51207: f = reinterpret_cast<const void *>(&k_loc<512, 7>); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipFuncGetAttributes(fa, f);
+}
+
 hipError_t launch_loc(lw_ctx *c, const LocLaunch &L) {
     LocArgs a{};
     a.scans = L.scans;
@@ -217,11 +235,15 @@ hipError_t launch_loc(lw_ctx *c, const LocLaunch &L) {
     for (int k = 0; k < 16; ++k) a.kappa[k] = L.kappa[k];
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(1), 0, c->stream, c->loc_queue);
     const dim3 grid((unsigned)(L.grid > 0 ? L.grid : 1));
-    switch (L.block) {
-    case 64: hipLaunchKernelGGL(k_loc<64>, grid, dim3(64), 0, c->stream, a); break;
-    case 128: hipLaunchKernelGGL(k_loc<128>, grid, dim3(128), 0, c->stream, a); break;
-    case 256: hipLaunchKernelGGL(k_loc<256>, grid, dim3(256), 0, c->stream, a); break;
-    case 512: hipLaunchKernelGGL(k_loc<512>, grid, dim3(512), 0, c->stream, a); break;
+    switch (L.block * 100 + L.chunk) {
+    case 6414: hipLaunchKernelGGL((k_loc<64, 14>), grid, dim3(64), 0, c->stream, a); break;
+    case 12814: hipLaunchKernelGGL((k_loc<128, 14>), grid, dim3(128), 0, c->stream, a); break;
+    case 25614: hipLaunchKernelGGL((k_loc<256, 14>), grid, dim3(256), 0, c->stream, a); break;
+    case 51214: hipLaunchKernelGGL((k_loc<512, 14>), grid, dim3(512), 0, c->stream, a); break;
+    case 6407: hipLaunchKernelGGL((k_loc<64, 7>), grid, dim3(64), 0, c->stream, a); break;
+    case 12807: hipLaunchKernelGGL((k_loc<128, 7>), grid, dim3(128), 0, c->stream, a); break;
+    case 25607: hipLaunchKernelGGL((k_loc<256, 7>), grid, dim3(256), 0, c->stream, a); break;
+    case 51207: hipLaunchKernelGGL((k_loc<512, 7>), grid, dim3(512), 0, c->stream, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

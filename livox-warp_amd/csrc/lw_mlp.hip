@@ -13,7 +13,13 @@
 //
 // One GEMM kernel serves every product: C (M x N) = A (M x K) B (K x N), operands addressed by two strides
 // each (a transpose is a stride swap), a 64 x 64 tile per workgroup of four waves, each wave 2 x 2 blocks of
-// 16 x 16, K in steps of 32 staged through LDS as bf16 with k contiguous, and a fused epilogue.
+// 16 x 16, K in steps of 32 staged through LDS as bf16 with k contiguous, and a fused epilogue. Each thread
+// moves one 16-byte vector of each operand a step (8 bf16 along the operand's contiguous axis, written to LDS
+// as one vector when that axis is k, else transposed element by element), the next step's vectors in flight
+// while the matrix cores work on the current one, with two LDS buffers and one barrier a step. The weight
+// gradients (K = the batch, an output of only 160-256 x 256) split K over the grid's z axis: each split writes
+// its partial tile, and a second kernel adds the splits in split order (the same sum on every run).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -61,51 +67,122 @@ struct Gemm {
     float *cf;          // fp32 output, row stride ldc
     uint16_t *cb;       // bf16 output, row stride ldh
     int64_t ldz, ldc, ldh;
+    int32_t kc;         // K of one split (a multiple of GK); gridDim.z splits
+    float *ws;          // the splits' partial tiles, [split][M][N] (EPI_STORE with more than one split)
+    int32_t a_vec, b_vec;  // 16-byte operand loads allowed (aligned base, strides and K multiples of 8)
 };
+
+// One operand's 8 bf16 of a step, for a tile of GT rows (i or j) x GK k. Thread t takes, along k (k_contig:
+// the operand's k stride is 1), row t / 4 and k (t % 4) * 8 .. + 7, or across rows, k t / 8 and rows
+// (t % 8) * 8 .. + 7. Outside [0, rows) x [0, kend) the values are zero.
+struct Frag {
+    uint4 v;
+};
+
+__device__ inline uint16_t half_of(const uint4 &v, int q) {
+    const uint32_t w = q < 2 ? v.x : (q < 4 ? v.y : (q < 6 ? v.z : v.w));
+    return (uint16_t)((q & 1) ? (w >> 16) : (w & 0xFFFFu));
+}
+
+// rows r0 .. (the tile's rows), p(row, k) = base[row * s_row + k * s_k]
+__device__ inline Frag fetch_frag(const uint16_t *base, int64_t s_row, int64_t s_k, int32_t r0, int32_t rows,
+                                  int32_t k0, int32_t kend, bool vec) {
+    const int t = threadIdx.x;
+    Frag f;
+    f.v = make_uint4(0u, 0u, 0u, 0u);
+    uint16_t h[8];
+    if (s_k == 1) {  // along k
+        const int32_t r = r0 + (t >> 2), k = k0 + (t & 3) * 8;
+        if (r < rows && k < kend) {
+            const uint16_t *p = base + (int64_t)r * s_row + k;
+            if (vec) {
+                f.v = *reinterpret_cast<const uint4 *>(p);
+                return f;
+            }
+            LW_UNROLL for (int q = 0; q < 8; ++q) h[q] = k + q < kend ? p[q] : (uint16_t)0;
+        } else {
+            return f;
+        }
+    } else {  // across rows, one k
+        const int32_t k = k0 + (t >> 3), r = r0 + (t & 7) * 8;
+        if (k < kend && r < rows) {
+            const uint16_t *p = base + (int64_t)k * s_k + (int64_t)r * s_row;
+            if (vec && r + 8 <= rows) {
+                f.v = *reinterpret_cast<const uint4 *>(p);
+                return f;
+            }
+            LW_UNROLL for (int q = 0; q < 8; ++q) h[q] = r + q < rows ? p[(int64_t)q * s_row] : (uint16_t)0;
+        } else {
+            return f;
+        }
+    }
+    f.v = make_uint4((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16),
+                     (uint32_t)h[4] | ((uint32_t)h[5] << 16), (uint32_t)h[6] | ((uint32_t)h[7] << 16));
+    return f;
+}
+
+// The fragment into the LDS tile [row][k] (row stride LROW).
+__device__ inline void stash_frag(uint16_t *tile, bool k_contig, const Frag &f) {
+    const int t = threadIdx.x;
+    if (k_contig) {
+        *reinterpret_cast<uint4 *>(&tile[(t >> 2) * LROW + (t & 3) * 8]) = f.v;
+    } else {
+        const int k = t >> 3, r = (t & 7) * 8;
+        LW_UNROLL for (int q = 0; q < 8; ++q) tile[(r + q) * LROW + k] = half_of(f.v, q);
+    }
+}
 
 template <int EPI>
 __global__ __launch_bounds__(256) void k_gemm(Gemm g) {
-    __shared__ __align__(16) uint16_t As[GT * LROW];  // [i][k]
-    __shared__ __align__(16) uint16_t Bs[GT * LROW];  // [j][k]
+    __shared__ __align__(16) uint16_t As[2][GT * LROW];  // [i][k]
+    __shared__ __align__(16) uint16_t Bs[2][GT * LROW];  // [j][k]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int i0 = blockIdx.y * GT, j0 = blockIdx.x * GT;
     const int wi = (wave >> 1) * 32, wj = (wave & 1) * 32;
-    const bool a_krow = g.sa1 == 1;  // k contiguous in A: neighbouring threads take neighbouring k
-    const bool b_jrow = g.sb1 == 1;  // j contiguous in B: neighbouring threads take neighbouring j
+    const int32_t kbeg = blockIdx.z * g.kc, kend = min(g.K, kbeg + g.kc);
+    const bool a_k = g.sa1 == 1, b_k = g.sb0 == 1;  // the operand's k axis is its contiguous one
+    // B(k, j): rows are j with stride sb1, k with stride sb0
     f32x4 acc[2][2];
     LW_UNROLL for (int a = 0; a < 2; ++a) LW_UNROLL for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < g.K; k0 += GK) {
-        LW_UNROLL for (int u = 0; u < (GT * GK) / 256; ++u) {
-            const int e = tid + 256 * u;
-            const int r = a_krow ? e / GK : e % GT, c = a_krow ? e % GK : e / GT;
-            const int i = i0 + r, k = k0 + c;
-            As[r * LROW + c] = (i < g.M && k < g.K) ? g.A[(int64_t)i * g.sa0 + (int64_t)k * g.sa1] : (uint16_t)0;
+    Frag fa = fetch_frag(g.A, g.sa0, g.sa1, i0, g.M, kbeg, kend, g.a_vec != 0);
+    Frag fb = fetch_frag(g.B, g.sb1, g.sb0, j0, g.N, kbeg, kend, g.b_vec != 0);
+    stash_frag(As[0], a_k, fa);
+    stash_frag(Bs[0], b_k, fb);
+    __syncthreads();
+    int buf = 0;
+    const int q = 8 * (lane >> 4), rl = lane & 15;
+    for (int32_t k0 = kbeg; k0 < kend; k0 += GK) {
+        const bool more = k0 + GK < kend;
+        if (more) {  // the next step's operands, in flight during this step's products
+            fa = fetch_frag(g.A, g.sa0, g.sa1, i0, g.M, k0 + GK, kend, g.a_vec != 0);
+            fb = fetch_frag(g.B, g.sb1, g.sb0, j0, g.N, k0 + GK, kend, g.b_vec != 0);
         }
-        LW_UNROLL for (int u = 0; u < (GT * GK) / 256; ++u) {
-            const int e = tid + 256 * u;
-            const int c = b_jrow ? e / GT : e % GK, r = b_jrow ? e % GT : e / GK;  // c: k, r: j
-            const int j = j0 + r, k = k0 + c;
-            Bs[r * LROW + c] = (j < g.N && k < g.K) ? g.B[(int64_t)k * g.sb0 + (int64_t)j * g.sb1] : (uint16_t)0;
-        }
-        __syncthreads();
-        const int q = 8 * (lane >> 4), rl = lane & 15;
         bf16x8 af[2], bfr[2];
         LW_UNROLL for (int a = 0; a < 2; ++a)
-            af[a] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&As[(wi + 16 * a + rl) * LROW + q]));
+            af[a] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&As[buf][(wi + 16 * a + rl) * LROW + q]));
         LW_UNROLL for (int b = 0; b < 2; ++b)
-            bfr[b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&Bs[(wj + 16 * b + rl) * LROW + q]));
+            bfr[b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&Bs[buf][(wj + 16 * b + rl) * LROW + q]));
         LW_UNROLL for (int a = 0; a < 2; ++a) LW_UNROLL for (int b = 0; b < 2; ++b)
             acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+        if (more) {
+            stash_frag(As[buf ^ 1], a_k, fa);
+            stash_frag(Bs[buf ^ 1], b_k, fb);
+        }
         __syncthreads();
+        buf ^= 1;
     }
     // lane holds rows 4 * (lane / 16) + r, column lane % 16 of each 16 x 16 block
+    const bool split = gridDim.z > 1;
     LW_UNROLL for (int a = 0; a < 2; ++a) LW_UNROLL for (int b = 0; b < 2; ++b) LW_UNROLL for (int r = 0; r < 4; ++r) {
         const int i = i0 + wi + 16 * a + 4 * (lane >> 4) + r;
         const int j = j0 + wj + 16 * b + (lane & 15);
         if (i >= g.M || j >= g.N) continue;
         const float v = acc[a][b][r];
         if constexpr (EPI == EPI_STORE) {
-            g.cf[(int64_t)i * g.ldc + j] = v;
+            if (split)
+                g.ws[((int64_t)blockIdx.z * g.M + i) * g.N + j] = v;
+            else
+                g.cf[(int64_t)i * g.ldc + j] = v;
         } else if constexpr (EPI == EPI_BIAS) {
             g.cf[(int64_t)i * g.ldc + j] = v + g.bias[j];
         } else if constexpr (EPI == EPI_BIAS_GELU) {
@@ -116,6 +193,17 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g) {
             g.cb[(int64_t)i * g.ldh + j] = to_bf16(v * gelu_grad(g.zin[(int64_t)i * g.ldz + j]));
         }
     }
+}
+
+// The splits' partial tiles added in split order: C(i, j) = sum_z ws[z][i][j].
+__global__ __launch_bounds__(256) void k_split_sum(const float *ws, int32_t splits, int32_t M, int32_t N, float *cf,
+                                                   int64_t ldc) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t mn = (int64_t)M * N;
+    if (e >= mn) return;
+    float s = 0.0f;
+    for (int32_t z = 0; z < splits; ++z) s += ws[z * mn + e];
+    cf[(e / N) * ldc + e % N] = s;
 }
 
 // x (n x 152 fp32, row stride ldx) -> X (n x 160 bf16, columns 152..159 zero)
@@ -206,11 +294,38 @@ __global__ __launch_bounds__(1024) void k_features(const float *xyz, int32_t m, 
     }
 }
 
+constexpr int SPLIT_MAX = 32;                            // splits of one product
+constexpr int64_t SPLIT_WS = (int64_t)SPLIT_MAX * MLP_HID * MLP_HID;  // floats of the partial tiles
+
+// 16-byte loads of an operand: its base 16-byte aligned, its row stride a multiple of 8 (its k stride is
+// then 1, or its k stride a multiple of 8 and its row stride 1), K a multiple of 8.
+inline bool vec_ok(const uint16_t *p, int64_t s_row, int64_t s_k, int32_t K) {
+    if (((uintptr_t)p & 15) != 0 || (K & 7) != 0) return false;
+    if (s_k == 1) return (s_row & 7) == 0;
+    return s_row == 1 && (s_k & 7) == 0;
+}
+
+// ws: room for SPLIT_WS floats, or null (no split)
 template <int EPI>
-hipError_t gemm(hipStream_t s, const Gemm &g) {
+hipError_t gemm(hipStream_t s, Gemm g, float *ws = nullptr) {
     if (g.M <= 0 || g.N <= 0 || g.K <= 0) return hipSuccess;
-    const dim3 grid((unsigned)((g.N + GT - 1) / GT), (unsigned)((g.M + GT - 1) / GT));
+    g.a_vec = vec_ok(g.A, g.sa0, g.sa1, g.K) ? 1 : 0;
+    g.b_vec = vec_ok(g.B, g.sb1, g.sb0, g.K) ? 1 : 0;
+    const int32_t tiles = ((g.N + GT - 1) / GT) * ((g.M + GT - 1) / GT), steps = (g.K + GK - 1) / GK;
+    int32_t splits = 1;
+    if (EPI == EPI_STORE && ws && (int64_t)g.M * g.N * SPLIT_MAX <= SPLIT_WS)  // fill the device: >= 256 tiles
+        splits = std::min(std::min(steps, SPLIT_MAX), std::max(1, (256 + tiles - 1) / tiles));
+    const int32_t per = (steps + splits - 1) / splits;
+    g.kc = per * GK;
+    splits = (steps + per - 1) / per;
+    g.ws = ws;
+    const dim3 grid((unsigned)((g.N + GT - 1) / GT), (unsigned)((g.M + GT - 1) / GT), (unsigned)splits);
     hipLaunchKernelGGL(k_gemm<EPI>, grid, dim3(256), 0, s, g);
+    if (splits > 1) {
+        const int64_t mn = (int64_t)g.M * g.N;
+        hipLaunchKernelGGL(k_split_sum, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, s, (const float *)ws, splits,
+                           g.M, g.N, g.cf, g.ldc);
+    }
     return hipGetLastError();
 }
 
@@ -235,6 +350,17 @@ struct lw_mlp {
     uint16_t *X = nullptr, *H[3] = {nullptr, nullptr, nullptr}, *dZ[2] = {nullptr, nullptr}, *dY = nullptr;
     uint16_t *ones = nullptr;
     float *Z[3] = {nullptr, nullptr, nullptr};
+    float *ws = nullptr;  // split-K partial tiles of the weight gradients
+    float *ystep = nullptr;  // the training step's outputs (n x 6)
+    // the captured training step (lw_mlp_train_step): replayed while its pointers and row count stay the same,
+    // with the Adam node's scalars updated before each replay
+    hipGraph_t step_graph = nullptr;
+    hipGraphExec_t step_exec = nullptr;
+    hipGraphNode_t adam_node = nullptr;
+    hipKernelNodeParams adam_params{};
+    const void *step_key[4] = {nullptr, nullptr, nullptr, nullptr};
+    int64_t step_ldx = 0;
+    int32_t step_n = 0;
     int32_t rows = 0;  // rows of the last forward
 };
 
@@ -250,9 +376,18 @@ constexpr int32_t OFF_W1 = 0, OFF_B1 = OFF_W1 + MLP_IN_PAD * MLP_HID, OFF_W2 = O
         if (e_ != hipSuccess) return -(int)e_; \
     } while (0)
 
+void drop_step_graph(lw_mlp *p) {
+    if (p->step_exec) (void)hipGraphExecDestroy(p->step_exec);
+    if (p->step_graph) (void)hipGraphDestroy(p->step_graph);
+    p->step_exec = nullptr;
+    p->step_graph = nullptr;
+    p->adam_node = nullptr;
+}
+
 void mlp_free(lw_mlp *p) {
+    drop_step_graph(p);
     void *bufs[] = {p->w, p->g, p->m, p->v, p->wb, p->X, p->H[0], p->H[1], p->H[2], p->dZ[0], p->dZ[1], p->dY,
-                    p->ones, p->Z[0], p->Z[1], p->Z[2]};
+                    p->ones, p->Z[0], p->Z[1], p->Z[2], p->ws, p->ystep};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
 }
@@ -305,6 +440,8 @@ int lw_mlp_create(lw_mlp **out, int32_t batch_cap, void *stream) {
     for (int l = 0; l < 2 && e == hipSuccess; ++l) e = alloc(&p->dZ[l], n * MLP_HID);
     if (e == hipSuccess) e = alloc(&p->dY, n * MLP_OUT_PAD);
     if (e == hipSuccess) e = alloc(&p->ones, n);
+    if (e == hipSuccess) e = alloc(&p->ws, (size_t)SPLIT_WS);
+    if (e == hipSuccess) e = alloc(&p->ystep, n * MLP_OUT);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_fill_u16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream, p->ones,
                            (int32_t)n, (uint16_t)0x3F80u);  // bf16 1.0
@@ -404,7 +541,7 @@ int lw_mlp_backward(lw_mlp *p, int32_t n) {
         g.K = n;
         g.cf = p->g + w_off[l];
         g.ldc = wcols;
-        MLP_CK(gemm<EPI_STORE>(p->stream, g));
+        MLP_CK(gemm<EPI_STORE>(p->stream, g, p->ws));
         Gemm gb{};  // db_l = 1^T dout
         gb.A = p->ones;
         gb.sa0 = 0;
@@ -417,7 +554,7 @@ int lw_mlp_backward(lw_mlp *p, int32_t n) {
         gb.K = n;
         gb.cf = p->g + b_off[l];
         gb.ldc = wcols;
-        MLP_CK(gemm<EPI_STORE>(p->stream, gb));
+        MLP_CK(gemm<EPI_STORE>(p->stream, gb, p->ws));
         if (l == 0) break;
         Gemm gd{};  // dZ_{l-1} = (dout W_l^T) * gelu'(Z_{l-1})
         gd.A = dout;
@@ -455,7 +592,78 @@ int lw_mlp_adam(lw_mlp *p, float lr, float beta1, float beta2, float eps, int32_
 
 int lw_mlp_set_stream(lw_mlp *p, void *stream) {
     if (!p) return LW_EINVAL;
+    if (p->stream != (hipStream_t)stream) drop_step_graph(p);
     p->stream = (hipStream_t)stream;
+    return 0;
+}
+
+int lw_mlp_train_step(lw_mlp *p, const float *x, int64_t ldx, const float *target, int32_t n, float lr, float beta1,
+                      float beta2, float eps, int32_t step, float *loss, int32_t use_graph) {
+    if (!p || !x || !target || !loss || n <= 0 || n > p->cap || ldx < MLP_IN || step < 1 || !(lr >= 0.0f) ||
+        !(beta1 >= 0.0f && beta1 < 1.0f) || !(beta2 >= 0.0f && beta2 < 1.0f))
+        return LW_EINVAL;
+    float *y = p->ystep;
+    if (!use_graph || p->stream == nullptr) {  // a captured graph needs a stream other than the null stream
+        int rc = lw_mlp_forward(p, x, ldx, n, y);
+        if (!rc) rc = lw_mlp_loss(p, y, target, n, loss);
+        if (!rc) rc = lw_mlp_backward(p, n);
+        if (!rc) rc = lw_mlp_adam(p, lr, beta1, beta2, eps, step);
+        return rc;
+    }
+    const void *key[4] = {x, target, loss, y};
+    if (!p->step_exec || std::memcmp(key, p->step_key, sizeof(key)) != 0 || p->step_ldx != ldx || p->step_n != n) {
+        drop_step_graph(p);
+        MLP_CK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+        int rc = lw_mlp_forward(p, x, ldx, n, y);
+        if (!rc) rc = lw_mlp_loss(p, y, target, n, loss);
+        if (!rc) rc = lw_mlp_backward(p, n);
+        if (!rc) rc = lw_mlp_adam(p, lr, beta1, beta2, eps, step);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(p->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        MLP_CK(e);
+        p->step_graph = g;
+        // the Adam node: the one kernel node whose arguments change from step to step
+        size_t count = 0;
+        MLP_CK(hipGraphGetNodes(g, nullptr, &count));
+        hipGraphNode_t nodes[64];
+        if (count > 64) return LW_EINVAL;
+        MLP_CK(hipGraphGetNodes(g, nodes, &count));
+        for (size_t k = 0; k < count; ++k) {
+            hipGraphNodeType t;
+            MLP_CK(hipGraphNodeGetType(nodes[k], &t));
+            if (t != hipGraphNodeTypeKernel) continue;
+            hipKernelNodeParams kp{};
+            MLP_CK(hipGraphKernelNodeGetParams(nodes[k], &kp));
+            if (kp.func == reinterpret_cast<const void *>(&k_adam)) {
+                p->adam_node = nodes[k];
+                p->adam_params = kp;
+            }
+        }
+        if (!p->adam_node) {
+            drop_step_graph(p);
+            return LW_EINVAL;
+        }
+        MLP_CK(hipGraphInstantiate(&p->step_exec, g, nullptr, nullptr, 0));
+        std::memcpy(p->step_key, key, sizeof(key));
+        p->step_ldx = ldx;
+        p->step_n = n;
+    }
+    // this step's Adam scalars (the node keeps the ones of the step it was captured at)
+    float *w = p->w, *m = p->m, *v = p->v, *gr = p->g;
+    uint16_t *wb = p->wb;
+    int32_t P = PARAMS;
+    float c1 = 1.0f - (float)std::pow((double)beta1, (double)step);
+    float c2 = 1.0f - (float)std::pow((double)beta2, (double)step);
+    void *args[] = {&w, &m, &v, &gr, &wb, &P, &lr, &beta1, &beta2, &eps, &c1, &c2};
+    hipKernelNodeParams kp = p->adam_params;
+    kp.kernelParams = args;
+    kp.extra = nullptr;
+    MLP_CK(hipGraphExecKernelNodeSetParams(p->step_exec, p->adam_node, &kp));
+    MLP_CK(hipGraphLaunch(p->step_exec, p->stream));
     return 0;
 }
 

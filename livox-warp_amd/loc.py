@@ -1,6 +1,7 @@
 """Batched global localisation on MI355X: thousands of pose hypotheses x query scans registered against one
 prior map in one persistent launch (lw_loc_batch), with the tile (threads per workgroup, persistent
-workgroups) picked by an autotuner and the launch optionally replayed from a captured hipGraph.
+workgroups per compute unit, neighbour keys a lane resolves together) picked by an autotuner that skips any
+tile whose compiled kernel spills to scratch, and the launch optionally replayed from a captured hipGraph.
 
 The reference's counterpart is GlobalLocalizer._icp (python/livox_warp/localize.py:517-553), which refines up
 to 16 hypotheses with one host solve per hypothesis and iteration.
@@ -29,7 +30,16 @@ def _torch():
 LocParams = _abi.LocParams
 
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "livox-warp_amd", "loc_tiles.json")
-TILE_CANDIDATES = [(b, g) for b in (64, 128, 256, 512) for g in (1, 2, 4, 8)]  # (threads, workgroups per CU)
+# (threads per workgroup, workgroups per compute unit, neighbour keys per chunk)
+TILE_CANDIDATES = [(b, g, c) for c in (14, 7) for b in (64, 128, 256, 512) for g in (1, 2, 4, 8)]
+DEFAULT_TILE = (256, 2, 7)
+
+
+def tile_info(lib, block: int, chunk: int) -> dict:
+    """The compiled kernel of a tile: scratch bytes per lane, registers per lane, static LDS bytes."""
+    info = (ctypes.c_int32 * 4)()
+    _abi.check(lib.lw_loc_tile_info(block, chunk, info), "lw_loc_tile_info")
+    return {"scratch": int(info[0]), "regs": int(info[1]), "lds": int(info[2]), "max_threads": int(info[3])}
 
 
 class BatchLocalizer:
@@ -100,9 +110,9 @@ class BatchLocalizer:
                           torch.empty(Q * H, dtype=torch.int32, device=d),
                           torch.empty(Q * H, dtype=torch.int32, device=d))
         oT, oS, oC, oI = self._bufs
-        block, per_cu = tile if tile is not None else self.tile_for(scans)
+        block, per_cu, chunk = tile if tile is not None else self.tile_for(scans)
         p = LocParams.from_buffer_copy(self.params)
-        p.block, p.grid, p.use_graph = block, per_cu * self.cus, 1 if use_graph else 0
+        p.block, p.grid, p.chunk, p.use_graph = block, per_cu * self.cus, chunk, 1 if use_graph else 0
         _abi.check(self.eng.lib.lw_loc_batch(self.eng.ctx, S.data_ptr(), O.data_ptr(), R.data_ptr(), Q, Hy.data_ptr(),
                                              H, ctypes.byref(p), oT.data_ptr(), oS.data_ptr(), oC.data_ptr(),
                                              oI.data_ptr()), "lw_loc_batch")
@@ -121,15 +131,18 @@ class BatchLocalizer:
         key = self._bucket(scans)
         if key not in self.tiles:
             cached = self._load_cache().get(key)
-            self.tiles[key] = tuple(cached) if cached else (128, 4)
+            self.tiles[key] = tuple(cached) if cached and len(cached) == 3 else DEFAULT_TILE
         return self.tiles[key]
 
     def autotune(self, scans, hyps, repeats: int = 3):
-        """Time every candidate tile on (scans, hyps) and keep the fastest for this scan-size bucket."""
-        torch = _torch()
+        """Time every candidate tile whose kernel does not spill on (scans, hyps) and keep the fastest for this
+        scan-size bucket. Returns (tile, {tile: best ms}, [tiles skipped for spilling])."""
         staged = self._stage(scans, hyps)
-        times = {}
+        times, spilled = {}, []
         for tile in TILE_CANDIDATES:
+            if tile_info(self.eng.lib, tile[0], tile[2])["scratch"] > 0:
+                spilled.append(tile)
+                continue
             self.run(scans, hyps, tile=tile, staged=staged)
             self.stream.synchronize()
             best = float("inf")
@@ -150,7 +163,7 @@ class BatchLocalizer:
                 json.dump(cache, f)
         except OSError:
             pass
-        return tile, {f"{b}x{g}": round(t * 1e3, 3) for (b, g), t in times.items()}
+        return tile, {f"{b}x{g}x{c}": round(t * 1e3, 3) for (b, g, c), t in times.items()}, spilled
 
     @staticmethod
     def _load_cache() -> dict:

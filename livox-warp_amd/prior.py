@@ -160,12 +160,19 @@ class MLP:
         self.steps += 1
         _abi.check(self.lib.lw_mlp_adam(self.h, lr, beta1, beta2, eps, self.steps), "lw_mlp_adam")
 
-    def train_step(self, x, t, lr: float):
-        y = self.forward(x)
-        loss = self.loss(y, t)
-        self.backward(int(x.shape[0]))
-        self.adam(lr)
-        return loss
+    def train_step(self, x, t, lr: float, graph: bool = True, beta1: float = 0.9, beta2: float = 0.999,
+                   eps: float = 1e-8):
+        """Forward, loss, backward and Adam on (x, t) in one call (lw_mlp_train_step): with graph, replayed from
+        a captured hipGraph while x, t and the row count stay the same (on a stream other than the null
+        stream). Returns the loss (a one-element device tensor, reused)."""
+        n = int(x.shape[0])
+        assert x.dtype == _torch().float32 and x.stride(1) == 1 and 0 < n <= self.cap
+        assert t.shape == (n, OUT) and t.is_contiguous()
+        self.steps += 1
+        _abi.check(self.lib.lw_mlp_train_step(self.h, x.data_ptr(), x.stride(0), t.data_ptr(), n, lr, beta1, beta2,
+                                              eps, self.steps, self._loss.data_ptr(), 1 if graph else 0),
+                   "lw_mlp_train_step")
+        return self._loss
 
 
 class Normalizer:
@@ -339,7 +346,9 @@ def fit(Xn, Yn, epochs: int = 30, batch: int = 1024, lr: float = 1e-3, seed: int
     With val = (Xv, Yv), the validation loss is measured after every epoch and the best epoch's parameters are
     returned. Returns (params, per-epoch training loss, per-epoch validation loss or [])."""
     torch = _torch()
-    mlp = MLP(batch_cap=batch, device=device or Xn.device)
+    dev = device or Xn.device
+    stream = torch.cuda.Stream(device=dev)  # the captured training step replays on a stream of its own
+    mlp = MLP(batch_cap=batch, device=dev, stream=stream)
     rng = np.random.default_rng(seed)
     mlp.set_params(params if params is not None else init_params(rng))
     n = int(Xn.shape[0])
@@ -347,22 +356,30 @@ def fit(Xn, Yn, epochs: int = 30, batch: int = 1024, lr: float = 1e-3, seed: int
     best, best_params = float("inf"), None
     gen = torch.Generator(device=Xn.device)
     gen.manual_seed(seed)
-    for ep in range(epochs):
-        perm = torch.randperm(n, device=Xn.device, generator=gen)
-        tot = torch.zeros(1, dtype=torch.float64, device=Xn.device)
-        nb = 0
-        decay = 0.5 * (1.0 + math.cos(math.pi * ep / max(epochs, 1)))
-        for a in range(0, n - batch + 1, batch):
-            idx = perm[a:a + batch]
-            tot += mlp.train_step(Xn[idx].contiguous(), Yn[idx].contiguous(), lr * (0.1 + 0.9 * decay)).double()
-            nb += 1
-        hist.append(float(tot.item()) / max(nb, 1))
-        if val is not None:
-            v = evaluate_loss(mlp, *val)
-            vhist.append(v)
-            if v < best:
-                best, best_params = v, mlp.params()
-    return (best_params if best_params is not None else mlp.params()), hist, vhist
+    xb = torch.empty((batch, Xn.shape[1]), dtype=torch.float32, device=dev)  # fixed batch buffers: one graph
+    tb = torch.empty((batch, Yn.shape[1]), dtype=torch.float32, device=dev)
+    stream.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(stream):
+        for ep in range(epochs):
+            perm = torch.randperm(n, device=Xn.device, generator=gen)
+            tot = torch.zeros(1, dtype=torch.float64, device=Xn.device)
+            nb = 0
+            decay = 0.5 * (1.0 + math.cos(math.pi * ep / max(epochs, 1)))
+            for a in range(0, n - batch + 1, batch):
+                idx = perm[a:a + batch]
+                torch.index_select(Xn, 0, idx, out=xb)
+                torch.index_select(Yn, 0, idx, out=tb)
+                tot += mlp.train_step(xb, tb, lr * (0.1 + 0.9 * decay)).double()
+                nb += 1
+            hist.append(float(tot.item()) / max(nb, 1))
+            if val is not None:
+                v = evaluate_loss(mlp, *val)
+                vhist.append(v)
+                if v < best:
+                    best, best_params = v, mlp.params()
+        out = best_params if best_params is not None else mlp.params()
+    mlp.close()
+    return out, hist, vhist
 
 
 def evaluate_loss(mlp: MLP, X, Y) -> float:

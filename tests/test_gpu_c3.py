@@ -43,7 +43,7 @@ def c3(lw, orc):
                      for q in range(len(W.scans))])
     print(f"C3: {len(W.map_xyz)} map points, scans {[len(s) for s in W.scans]}, {HYPS} hypotheses each", flush=True)
     loc = lw.loc.BatchLocalizer(torch.from_numpy(W.map_xyz).cuda(), voxel=VOXEL, reg_slots=SLOTS, chunk=1 << 22)
-    tile, _ = loc.autotune([W.scans[0]], hyps[:1, :256])
+    tile, _, _ = loc.autotune([W.scans[0]], hyps[:1, :256])
     staged = loc._stage(W.scans, hyps)
     eager = [x.clone() for x in loc.run(W.scans, hyps, tile=tile, staged=staged)]
     loc.stream.synchronize()

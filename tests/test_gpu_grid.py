@@ -33,6 +33,18 @@ def test_build_matches_oracle(small):
     assert np.array_equal(grid.xyz.cpu().numpy(), prep["xyz"]) and np.array_equal(grid.start.cpu().numpy(), prep["start"])
 
 
+@pytest.mark.parametrize("steps", [(32, 16, 5, 3, 1), (7, 2, 1)])
+def test_build_with_other_step_schedules(lw, small, steps):
+    """Jump-flood schedules beyond the default: steps above the LDS slab's 16 (the direct kernel), odd steps
+    whose lattices leave partial tiles in every direction; the seeds and the field equal the oracle's."""
+    prep, _ = small
+    grid = lw.PriorGrid(prep, "cuda:0")
+    grid.build(steps=steps)
+    nn, df = O.grid_build(prep["xyz"], prep["origin"], prep["cell"], prep["dims"], steps=steps)
+    assert np.array_equal(grid.nn.cpu().numpy(), nn)
+    assert np.array_equal(grid.df.cpu().numpy(), df)
+
+
 def test_distances_match_oracle(small):
     prep, grid = small
     rng = np.random.default_rng(8)

@@ -105,6 +105,49 @@ def test_insert_matches_oracle(lw, orc, gpu_ok):
     assert mp.map_stats() == (0, 0) and np.all(mp.records()["key"] == -1)
 
 
+def test_insert_runs_and_the_count_cap(lw, orc, gpu_ok):
+    """Scan-ordered points: runs of one voxel longer than a wave, runs of every length 1..70 and a voxel hit
+    9,000 times, the count cap (4096) crossed inside a run. Counts, times and occupancy equal the oracle's; the
+    capped voxel's sums hold exactly 4096 of its (equal) points."""
+    rng = np.random.default_rng(11)
+    parts, lens = [], list(range(1, 71)) * 3
+    rng.shuffle(lens)
+    for k, ln in enumerate(lens):
+        c = np.array([0.5 + 0.2 * (k % 30), 0.5 + 0.2 * (k // 30), 1.1], np.float32)
+        parts.append(np.repeat(c[None], ln, 0) + rng.uniform(-0.05, 0.05, (ln, 3)).astype(np.float32))
+    hot = np.repeat(np.array([[-2.05, 1.05, 0.55]], np.float32), 9000, 0)
+    xyz = np.concatenate(parts[:100] + [hot] + parts[100:]).astype(np.float32)
+    n = len(xyz)
+    attr = np.full(n, 40, np.uint32)
+    t = np.linspace(0.0, 1.0, n).astype(np.float32)
+    fid = np.zeros(n, np.int32)
+    pt = lw.PoseTable()
+    pt.set_pose(0, np.eye(4, dtype=np.float32))
+    view = lw.imap.View(min_range=0.0, max_range=100.0, min_refl=0, noise_mask=0b1111, ret_mask=0b1111)
+    cap = 1 << 16
+    mp = lw.IntegrationMap(pt, "cuda:0", map_capacity=cap)
+    mp.map_insert(_dev(xyz, torch.float32), _dev(attr.view(np.int32), torch.int32), _dev(t, torch.float32),
+                  _dev(fid, torch.int32), n, view, 0.2)
+    recs = mp.records()
+    ref = O.IntegrationMap(cap)
+    O.map_insert(ref, xyz, attr, t, fid, O.identity_poses(), 0.2, min_range=0.0, max_range=100.0, min_refl=0,
+                 noise_mask=0b1111, ret_mask=0b1111)
+    assert mp.map_stats() == tuple(ref.stats)
+    g = {int(k): i for i, k in enumerate(recs["key"]) if k >= 0}
+    r = {int(k): i for i, k in enumerate(ref.keys) if k >= 0}
+    assert set(g) == set(r)
+    gi = np.array([g[k] for k in r]), np.array(list(r.values()))
+    assert np.array_equal(recs["cnt"][gi[0]], ref.cnt[gi[1]]) and ref.cnt.max() == 9000
+    assert np.array_equal(recs["first"][gi[0]], ref.first[gi[1]])
+    assert np.array_equal(recs["last"][gi[0]], ref.last[gi[1]])
+    capped = ref.cnt[gi[1]] > 4096
+    tol = 1e-6 * np.abs(ref.acc[gi[1]]).max(1, keepdims=True) * np.minimum(ref.cnt[gi[1], None], 4096) + 1e-6
+    assert np.all(np.abs(recs["acc"][gi[0]] - ref.acc[gi[1]]) <= tol)
+    assert capped.sum() == 1
+    hot_acc = recs["acc"][gi[0][capped]][0]
+    assert np.allclose(hot_acc[:3], 4096 * hot[0], rtol=2e-5) and hot_acc[3] == 4096 * 40
+
+
 @pytest.mark.parametrize("every", [1, 2])
 def test_carve_matches_oracle(lw, orc, every, gpu_ok):
     rng = np.random.default_rng(11)

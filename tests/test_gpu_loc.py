@@ -47,7 +47,8 @@ def _check(got, want, ang_deg=1e-3, pos_m=1e-4):
             assert I[q, h] == iw, (q, h, I[q, h], iw)
 
 
-@pytest.mark.parametrize("tile", [(64, 2), (128, 4), (256, 8), (512, 1)])
+@pytest.mark.parametrize("tile", [(64, 2, 14), (128, 4, 14), (256, 8, 14), (512, 1, 14), (64, 8, 7), (128, 2, 7),
+                                  (256, 2, 7), (512, 1, 7)])
 def test_batch_matches_oracle(problem, tile, gpu_ok):
     loc, scans, hyps, want = problem
     print(f"lw_loc_batch tile {tile}, queries {[len(s) for s in scans]}, hypotheses {hyps.shape[1]}", flush=True)
@@ -59,11 +60,11 @@ def test_batch_matches_oracle(problem, tile, gpu_ok):
 def test_graph_replay_matches_eager(problem, gpu_ok):
     loc, scans, hyps, want = problem
     staged = loc._stage(scans, hyps)
-    eager = [x.clone() for x in loc.run(scans, hyps, tile=(128, 4), staged=staged)]
+    eager = [x.clone() for x in loc.run(scans, hyps, tile=(128, 4, 14), staged=staged)]
     loc.stream.synchronize()
     for rep in range(3):
         print(f"graph replay {rep}", flush=True)
-        got = loc.run(scans, hyps, tile=(128, 4), use_graph=True, staged=staged)
+        got = loc.run(scans, hyps, tile=(128, 4, 14), use_graph=True, staged=staged)
         loc.stream.synchronize()
         for a, b in zip(got, eager):
             assert torch.equal(a, b)
@@ -74,7 +75,21 @@ def test_graph_replay_matches_eager(problem, gpu_ok):
 def test_truth_hypothesis_scores_highest(problem, gpu_ok):
     """The hypothesis at the true pose stays there and outscores the one with no overlap."""
     loc, scans, hyps, want = problem
-    T, S, C, _ = (x.cpu().numpy() for x in loc.run(scans, hyps, tile=(128, 4)))
+    T, S, C, _ = (x.cpu().numpy() for x in loc.run(scans, hyps, tile=(128, 4, 14)))
     for q in range(len(scans)):
         assert np.linalg.norm(T[q, 0, :3, 3] - hyps[q, 0, :3, 3]) < 0.05
         assert S[q, 0] > S[q, 5] and C[q, 5] < 30
+
+
+def test_chunk7_tiles_do_not_spill_and_the_autotuner_skips_spilling_ones(problem, lw, gpu_ok):
+    """Every 7-key tile fits its registers (no scratch); the autotuner times only tiles that do not spill."""
+    loc, scans, hyps, want = problem
+    for b in (64, 128, 256, 512):
+        info = lw.loc.tile_info(loc.eng.lib, b, 7)
+        assert info["scratch"] == 0 and 0 < info["regs"] <= 256 and info["max_threads"] >= b, (b, info)
+    tile, table, spilled = loc.autotune(scans[:1], hyps[:1])
+    for b, g, c in spilled:
+        assert lw.loc.tile_info(loc.eng.lib, b, c)["scratch"] > 0
+    assert tile not in spilled and all(f"{b}x{g}x{c}" not in table for b, g, c in spilled)
+    assert len(table) + len(spilled) == len(lw.loc.TILE_CANDIDATES)
+    print(f"tiles: chose {tile}, skipped for spilling {sorted(set((b, c) for b, _, c in spilled))}", flush=True)

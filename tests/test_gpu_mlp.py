@@ -57,13 +57,14 @@ def test_strided_input(lw, om, gpu_ok):
     assert np.abs(y - yo).max() <= 2e-3 * np.abs(yo).max() + 1e-5
 
 
-@pytest.mark.parametrize("n", [37, 1000])
+@pytest.mark.parametrize("n", [37, 1000, 4096])
 def test_loss_and_gradients_match_oracle(lw, om, n, gpu_ok):
+    """Every gradient block; at 1000 and 4096 rows the weight gradients run split over K (the batch)."""
     p = _params(om, 3)
     rng = np.random.default_rng(4)
     x = rng.normal(0, 1, (n, om.IN)).astype(np.float32)
     t = rng.normal(0, 1, (n, om.OUT)).astype(np.float32)
-    net = lw.prior.MLP(batch_cap=1024)
+    net = lw.prior.MLP(batch_cap=max(1024, n))
     net.set_params(p)
     print(f"lw_mlp_loss/backward n={n}", flush=True)
     y = net.forward(torch.from_numpy(x).cuda())
@@ -101,6 +102,34 @@ def test_adam_step_matches_oracle(lw, om, gpu_ok):
         got = net.params()
         assert np.abs(got - w).max() <= 1e-6, step
     assert np.all(om.unpack(got)["W1"][om.IN:] == 0) and np.all(om.unpack(got)["W4"][:, om.OUT:] == 0)
+
+
+def test_graph_training_step_equals_the_eager_steps(lw, om, gpu_ok):
+    """lw_mlp_train_step replayed from its captured graph against the same step run launch by launch, four
+    steps (Adam's bias correction changes every step), with other work and synchronisations between the
+    replays: parameters and loss equal bit for bit after every step."""
+    p = _params(om, 8)
+    rng = np.random.default_rng(9)
+    n = 2048
+    x = torch.from_numpy(rng.normal(0, 1, (n, om.IN)).astype(np.float32)).cuda()
+    t = torch.from_numpy(rng.normal(0, 1, (n, om.OUT)).astype(np.float32)).cuda()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    eager, graph = lw.prior.MLP(batch_cap=n, stream=s), lw.prior.MLP(batch_cap=n, stream=s)
+    try:
+        eager.set_params(p)
+        graph.set_params(p)
+        for step in range(4):
+            le_t = eager.train_step(x, t, 1e-3, graph=False)
+            lg_t = graph.train_step(x, t, 1e-3, graph=True)
+            s.synchronize()  # the losses are written on the networks' stream
+            le, lg = float(le_t.item()), float(lg_t.item())
+            float((x * 2.0).sum().item())  # the program's own work between replays
+            torch.cuda.synchronize()
+            assert le == lg and np.array_equal(eager.params(), graph.params()), step
+    finally:
+        eager.close()
+        graph.close()
 
 
 def test_frame_features_match_oracle(lw, om, gpu_ok):
