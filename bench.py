@@ -186,10 +186,41 @@ def localisation(args, torch, lw, dev):
                       "points_per_s": round(n_map / t_map, 1), "algorithmic_GBps": round(map_bytes / t_map / 1e9, 2)},
         "gen_seconds": round(t_gen, 2),
     }
+    res["bulk_voxelisation"] = bulk_voxelisation(torch, lw, dev, X)
     res["global_search"] = global_search(torch, lw, dev, W, loc)
     if args.cpu_frames > 0:
         res["cpu_baseline"] = loc_cpu_baseline(W, hyps, args.loc_cpu_hyps)
     return res
+
+
+def bulk_voxelisation(torch, lw, dev, X, voxel: float = 0.05, slots_log2: int = 25, reps: int = 3):
+    """prior_map.voxel_downsample's device work on the floor's map points (already on the device): the insert
+    into a cleared 2^25-slot table and the slot-order compaction, timed with HIP events on the table's stream;
+    algorithmic bytes: 12 per point read, 40 per voxel record (key, count, sums) and 32 per voxel emitted."""
+    st = torch.cuda.Stream(device=dev)
+    st.wait_stream(torch.cuda.current_stream(dev))
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ins, emi = [], []
+    with lw.VoxTable(slots_log2, dev, st) as t:
+        for _ in range(reps):
+            t.clear()
+            e[0].record(st)
+            t.insert(X, voxel)
+            e[1].record(st)
+            out = t.emit(voxel)
+            e[2].record(st)
+            st.synchronize()
+            ins.append(e[0].elapsed_time(e[1]) * 1e-3)
+            emi.append(e[1].elapsed_time(e[2]) * 1e-3)
+        V, dropped = t.stats()
+        del out
+    n = int(X.shape[0])
+    t_ins, t_emi = min(ins), min(emi)
+    return {"points": n, "voxel_m": voxel, "slots": 1 << slots_log2, "voxels": V, "dropped": dropped,
+            "insert_ms": round(t_ins * 1e3, 3), "emit_ms": round(t_emi * 1e3, 3),
+            "points_per_s": round(n / t_ins, 1),
+            "insert_algorithmic_GBps": round((12.0 * n + 40.0 * V) / t_ins / 1e9, 2),
+            "emit_algorithmic_GBps": round((n / 8.0 + 40.0 * V + 32.0 * V) / t_emi / 1e9, 2)}
 
 
 def _thin(pts: np.ndarray, voxel: float) -> np.ndarray:

@@ -371,6 +371,30 @@ int lw_lvxr_decode(const uint8_t *bytes, const int64_t *pk_off, const int32_t *p
                    int32_t returns, double *t, float *xyz, uint8_t *refl, uint8_t *tag, uint8_t *valid, float *dir,
                    void *stream);
 
+/* ---- Bulk voxelisation of a prior map (python/livox_warp/prior_map.py:55-123, voxel_downsample :169-203) ----
+ * A table of 2^slots_log2 voxels: each point's offset from its voxel's centre, colour and intensity summed
+ * into the slot claimed for its key (gpu.py's key and hash, 128 probes: k_vox_insert), then the claimed
+ * voxels compacted in slot order into dense means (k_vox_flag, array_scan, k_vox_emit). Work is queued on
+ * the table's stream. */
+typedef struct lw_vox lw_vox;
+
+int lw_vox_create(lw_vox **out, int32_t slots_log2, void *stream); /* 10 <= slots_log2 <= 30; cleared */
+int lw_vox_destroy(lw_vox *v);
+int lw_vox_clear(lw_vox *v); /* every slot empty, the stats zero */
+/* k_vox_insert over n points: xyz float32 x 3, intensity float32 and rgb float32 x 3 (either may be NULL:
+ * zeros); DEVICE pointers. voxel: the edge in metres (the kernel gets 1 / voxel and voxel as float32). */
+int lw_vox_insert(lw_vox *v, const float *xyz, const float *intensity, const float *rgb, int64_t n, double voxel);
+/* out[0] voxels claimed, out[1] points dropped (no slot within 128 probes); waits for the table's stream. */
+int lw_vox_stats(lw_vox *v, int32_t *out);
+/* The claimed voxels in slot order into out_xyz (mean position, float32 x 3), out_col (mean r, g, b,
+ * intensity) and out_cnt (points): DEVICE arrays of out_cap voxels; *count = the voxels (LW_ERANGE, nothing
+ * written, when more than out_cap). */
+int lw_vox_emit(lw_vox *v, double voxel, float *out_xyz, float *out_col, int32_t *out_cnt, int64_t out_cap,
+                int64_t *count);
+/* The table as plain DEVICE arrays of its 2^slots_log2 slots: keys (-1 empty), offset sums (x 3), colour and
+ * intensity sums (x 4), counts. For checks; no counterpart in the reference. */
+int lw_vox_read(lw_vox *v, int64_t *keys, float *rsum, float *csum, int32_t *cnt);
+
 /* ---- The prior map's dense grid and coarse scoring (python/livox_warp/mapgrid.py, localize.py:55-123) -----
  * Geometry, host pointers read at the call: geo = {origin x, y, z, cell, 1 / cell} (float32), dims = {nx, ny,
  * nz} with nx ny nz < 2^31; cell c = (iz ny + iy) nx + ix. Map points (xyz, n < 2^23) are sorted by cell and

@@ -104,6 +104,13 @@ SIGNATURES = {
     "lw_odom_frame": (_i32, [_vp, ctypes.POINTER(RegParams), ctypes.POINTER(FrameIn), ctypes.POINTER(FrameOut)]),
     "lw_loc_batch": (_i32, [_vp, _vp, _vp, _vp, _i32, _vp, _i32, ctypes.POINTER(LocParams), _vp, _vp, _vp, _vp]),
     "lw_loc_tile_info": (_i32, [_i32, _i32, _i32p]),
+    "lw_vox_create": (_i32, [ctypes.POINTER(_vp), _i32, _vp]),
+    "lw_vox_destroy": (_i32, [_vp]),
+    "lw_vox_clear": (_i32, [_vp]),
+    "lw_vox_insert": (_i32, [_vp, _vp, _vp, _vp, _i64, _f64]),
+    "lw_vox_stats": (_i32, [_vp, _i32p]),
+    "lw_vox_emit": (_i32, [_vp, _f64, _vp, _vp, _vp, _i64, ctypes.POINTER(_i64)]),
+    "lw_vox_read": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "lw_mlp_param_count": (_i32, []),
     "lw_mlp_create": (_i32, [ctypes.POINTER(_vp), _i32, _vp]),
     "lw_mlp_destroy": (_i32, [_vp]),

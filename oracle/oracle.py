@@ -87,6 +87,9 @@ _SIGS = {
                              _f32p, ctypes.c_float, ctypes.c_float, ctypes.c_float, _f64p]),
     "lwo_plane_eval": (None, [_f32p, ctypes.c_int32, _f32p, ctypes.c_int32, _f32p, _i32p, _i32p, _i32p, _f32p, _f32p,
                               _i32p]),
+    "lwo_vox_insert": (None, [_f32p, _f32p, _f32p, _L, ctypes.c_float, ctypes.c_float, _L, _i64p, _f32p, _f32p,
+                              _i32p, _i32p]),
+    "lwo_vox_emit": (_L, [_i64p, _f32p, _f32p, _i32p, _L, ctypes.c_float, _f32p, _f32p, _i32p]),
 }
 
 
@@ -832,3 +835,40 @@ def plane_eval(pts, Ts, grid: dict, nn, radii=EVAL_RADII):
                          _p(np.ascontiguousarray(nn, np.int32), _i32p), _p(_f32(grid["xyz"]), _f32p),
                          _p(np.array(radii, np.float32), _f32p), _p(counts, _i32p))
     return counts
+
+
+# ---- bulk voxelisation of a prior map (prior_map.py:55-123, 169-203) ----------------------------------------
+VOX_PROBES = 128  # prior_map.py:36
+
+
+class VoxTable:
+    """k_vox_insert's table (prior_map.py:55-91) filled in index order by the C restatement: keys (-1 empty),
+    offset sums from the voxel centres, colour + intensity sums, counts, stats [voxels, dropped points]."""
+
+    def __init__(self, slots: int):
+        assert slots & (slots - 1) == 0
+        self.cap = slots
+        self.keys = np.full(slots, -1, np.int64)
+        self.rsum = np.zeros((slots, 3), np.float32)
+        self.csum = np.zeros((slots, 4), np.float32)
+        self.cnt = np.zeros(slots, np.int32)
+        self.stats = np.zeros(2, np.int32)
+
+    def insert(self, xyz, voxel: float, intensity=None, rgb=None):
+        xyz = _f32(xyz).reshape(-1, 3)
+        it = None if intensity is None else _f32(intensity)
+        c = None if rgb is None else _f32(rgb).reshape(-1, 3)
+        inv = np.float32(1.0 / voxel)  # the launch argument 1.0 / voxel rounded to float32 (prior_map.py:187)
+        lib().lwo_vox_insert(_p(xyz, _f32p), None if it is None else _p(it, _f32p), None if c is None else _p(c, _f32p),
+                             len(xyz), float(inv), float(voxel), self.cap - 1, _p(self.keys, _i64p),
+                             _p(self.rsum, _f32p), _p(self.csum, _f32p), _p(self.cnt, _i32p), _p(self.stats, _i32p))
+
+    def emit(self, voxel: float):
+        """k_vox_flag / array_scan / k_vox_emit (prior_map.py:192-201): (xyz, colour + intensity, count) of the
+        claimed slots in slot order."""
+        v = int(self.stats[0])
+        xyz, col, cnt = np.empty((v, 3), np.float32), np.empty((v, 4), np.float32), np.empty(v, np.int32)
+        got = lib().lwo_vox_emit(_p(self.keys, _i64p), _p(self.rsum, _f32p), _p(self.csum, _f32p), _p(self.cnt, _i32p),
+                                 self.cap, float(voxel), _p(xyz, _f32p), _p(col, _f32p), _p(cnt, _i32p))
+        assert got == v
+        return xyz, col, cnt
