@@ -601,10 +601,20 @@ def learned_prior(args, torch, lw, dev):
     params, norm = lw.prior.load(os.path.join(ROOT, "livox-warp_amd", "prior_weights.npz"))
     rows = lw.prior.evaluate(lw, params, norm, range(5000, 5000 + args.prior_seqs), args.prior_frames, dev)
     res = lw.prior.summarise(rows)
+    # the learned prior's targets: odometry frames/s at least 1.3x the constant-velocity guess's, and at least
+    # 30% fewer Gauss-Newton steps
+    res["targets"] = {"speedup": 1.3, "gn_reduction": 0.30, "speedup_met": bool(res["speedup"] >= 1.3),
+                      "gn_reduction_met": bool(res["gn_reduction"] >= 0.30)}
     res["workload"] = (f"{args.prior_seqs} held-out office walks (seeds 5000..) x {args.prior_frames} frames, "
                        f"10k points/frame, both guesses on the same frames")
-    # training step at batch B: forward, loss, backward (2x the forward's products), Adam
-    B = 4096
+    res["training_step"] = training_step(torch, lw, dev, 4096)
+    res["training_step_16k"] = training_step(torch, lw, dev, 16384)
+    return res
+
+
+def training_step(torch, lw, dev, B: int):
+    """The prior's training step at batch B (forward, loss, backward with 2x the forward's products, Adam),
+    replayed from its captured graph, HIP events over 50 steps; the same steps launch by launch beside it."""
     net = lw.prior.MLP(batch_cap=B, device=dev, stream=torch.cuda.Stream(device=dev))
     net.set_params(lw.prior.init_params(np.random.default_rng(0)))
     g = torch.Generator(device=dev)
@@ -631,12 +641,12 @@ def learned_prior(args, torch, lw, dev):
     eager_s = e0.elapsed_time(e1) * 1e-3 / steps
     net.close()
     tflops = 3.0 * MLP_FLOPS_PER_ROW * B / step_s / 1e12
-    res["training_step"] = {"batch": B, "ms": round(step_s * 1e3, 4), "samples_per_s": round(B / step_s, 1),
-                            "eager_ms": round(eager_s * 1e3, 4),
-                            "roofline": {"bound": "mfma", "achieved": round(tflops, 3), "peak": 2500.0,
-                                         "unit": "TFLOP/s", "frac": round(tflops / 2500.0, 6),
-                                         "kernel": "k_gemm (bf16 16x16x32 MFMA, fp32 accumulate), the step "
-                                                   "replayed from its captured hipGraph"}}
+    res = {"batch": B, "ms": round(step_s * 1e3, 4), "samples_per_s": round(B / step_s, 1),
+           "eager_ms": round(eager_s * 1e3, 4),
+           "roofline": {"bound": "mfma", "achieved": round(tflops, 3), "peak": 2500.0, "unit": "TFLOP/s",
+                        "frac": round(tflops / 2500.0, 6),
+                        "kernel": "k_mlp_fwd + k_gemm_pair + k_gemm + k_grad_sum (bf16 16x16x32 MFMA, fp32 "
+                                  "accumulate), the step replayed from its captured hipGraph"}}
     return res
 
 
@@ -867,29 +877,42 @@ def main():
     spans = [(float(f[2][0]), float(f[2][-1])) for f in frames]
     torch.cuda.synchronize()
 
+    t_run = time.perf_counter()
+
+    def leg(name, fn):
+        """One leg, with a progress line on stderr (stdout carries only the result line)."""
+        t0 = time.perf_counter()
+        r = fn()
+        print(f"[bench] {name}: {time.perf_counter() - t0:.1f} s (at {time.perf_counter() - t_run:.1f} s)",
+              file=sys.stderr, flush=True)
+        return r
+
     odo = lw.Odometry(lw.PoseTable(), dev, reg_slots=1 << 22, frame_cap=1 << 15)
     try:
-        out = odometry_headline(args, torch, lw, dev, odo, X, A, Tt, spans, seq, n_frames, per, t_gen)
+        out = leg("odometry", lambda: odometry_headline(args, torch, lw, dev, odo, X, A, Tt, spans, seq, n_frames,
+                                                          per, t_gen))
     finally:
         odo.close()
     if args.batch_seqs > 0:
-        out["multi_sequence"] = multi_sequence(args, torch, lw, dev, seq, out["value"])
+        out["multi_sequence"] = leg("multi_sequence", lambda: multi_sequence(args, torch, lw, dev, seq, out["value"]))
     if not args.no_c1:
-        out["single_registration"] = single_registration(args, torch, lw, dev)
+        out["single_registration"] = leg("single_registration", lambda: single_registration(args, torch, lw, dev))
     if not args.no_imap:
-        out["integration_map"] = integration_map(args, torch, lw, dev, X, A, Tt, seq, n_frames, per)
+        out["integration_map"] = leg("integration_map", lambda: integration_map(args, torch, lw, dev, X, A, Tt, seq,
+                                                                                n_frames, per))
     if not args.no_grid:
-        out["prior_grid"] = prior_grid(args, torch, lw, dev)
+        out["prior_grid"] = leg("prior_grid", lambda: prior_grid(args, torch, lw, dev))
     if not args.no_lvxr:
-        out["lvxr_replay"] = lvxr_replay(args, torch, lw, dev, frames)
+        out["lvxr_replay"] = leg("lvxr_replay", lambda: lvxr_replay(args, torch, lw, dev, frames))
     if not args.no_worker:
-        out["odom_worker"] = odom_worker(args, torch, lw, dev, X, A, Tt, spans, seq, n_frames, per)
+        out["odom_worker"] = leg("odom_worker", lambda: odom_worker(args, torch, lw, dev, X, A, Tt, spans, seq,
+                                                                    n_frames, per))
     if not args.no_loc:
-        out["localisation"] = localisation(args, torch, lw, dev)
+        out["localisation"] = leg("localisation", lambda: localisation(args, torch, lw, dev))
     if not args.no_prior:
-        out["learned_prior"] = learned_prior(args, torch, lw, dev)
+        out["learned_prior"] = leg("learned_prior", lambda: learned_prior(args, torch, lw, dev))
     if args.cpu_frames > 0:
-        out["cpu_baseline"] = cpu_baseline(frames, seq, SEQ_PREFIX, args.cpu_frames)
+        out["cpu_baseline"] = leg("cpu_baseline", lambda: cpu_baseline(frames, seq, SEQ_PREFIX, args.cpu_frames))
     else:
         out["cpu_baseline"] = None
     if args.copy_probe > 0:

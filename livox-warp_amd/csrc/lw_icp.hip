@@ -951,6 +951,9 @@ __global__ __launch_bounds__(64 * SEQ_WAVES) void k_register_seq(const SeqReg *D
         const Pass p{n, reach == 2 ? L.max_d2_r2 : L.max_d2_r1, st->kappa[it], it};
         double sum, comp;
         int32_t hits = 0, matched = 0;
+        // the home records directly, as the single-stream loop reads them: reading the occupancy bitmap first
+        // (k_loc's form) measured 3% slower here -- a workgroup's passes are bound by the latency of its
+        // dependent loads, not by their bytes
         if (reach == 2)
             wave_pass<MODE, 2>(a, p, Tf, nbr, wave, SEQ_WAVES, lds[wave], sum, comp, hits, matched, {}, false, false,
                                t_found, t_terms);

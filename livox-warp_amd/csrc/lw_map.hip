@@ -181,29 +181,62 @@ __device__ inline void quota_blocks(const MapRefs &M, int32_t n, int b, int nb) 
 }
 
 // 4: accumulate (odom.py:217, 225-228): r relative to the voxel centre, float32 atomics.
+// The moments of the taken points. Points arrive in scan order, so neighbouring lanes often add to one voxel:
+// each wave first combines the runs of consecutive lanes with the same voxel id (a segmented shuffle
+// reduction of the nine sums and the count), and the run's first lane makes one set of atomics for it. The
+// float32 sums come out in another order than one atomic a point (the moments are order-dependent at the
+// float32 level either way); the counts are exact.
 __device__ inline void accum_blocks(const MapRefs &M, const float *xyz, const uint32_t *attr, int32_t cnt,
                                     const Mat4f &T, float inv_v, float voxel, float max_range, int32_t ret_mask,
                                     int b, int nb) {
-    for (int32_t i = b * BLOCK + threadIdx.x; i < cnt; i += nb * BLOCK) {
-        int32_t vid = M.pt_vid[i];
-        if (vid < 0) continue;
-        int32_t q = M.vox_quota[vid];
-        if (q != INT_MAX && !(q > 0 && M.pt_take[i])) continue;
-        int32_t ix, iy, iz;
-        float wx, wy, wz;
-        insert_point(xyz, attr, i, T, inv_v, max_range, ret_mask, ix, iy, iz, wx, wy, wz);
-        float r0 = wx - center_of(ix, voxel), r1 = wy - center_of(iy, voxel), r2 = wz - center_of(iz, voxel);
+    const int lane = lane_id();
+    for (int32_t base = b * BLOCK; base < cnt; base += nb * BLOCK) {  // the same trip count for a whole wave
+        const int32_t i = base + (int32_t)threadIdx.x;
+        int32_t vid = i < cnt ? M.pt_vid[i] : -1;
+        if (vid >= 0) {
+            const int32_t q = M.vox_quota[vid];
+            if (q != INT_MAX && !(q > 0 && M.pt_take[i])) vid = -1;
+        }
+        float v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (vid >= 0) {
+            int32_t ix, iy, iz;
+            float wx, wy, wz;
+            insert_point(xyz, attr, i, T, inv_v, max_range, ret_mask, ix, iy, iz, wx, wy, wz);
+            const float r0 = wx - center_of(ix, voxel), r1 = wy - center_of(iy, voxel), r2 = wz - center_of(iz, voxel);
+            v[0] = r0;
+            v[1] = r1;
+            v[2] = r2;
+            v[3] = r0 * r0;
+            v[4] = r0 * r1;
+            v[5] = r0 * r2;
+            v[6] = r1 * r1;
+            v[7] = r1 * r2;
+            v[8] = r2 * r2;
+        }
+        // runs of one voxel: a run starts where the id differs from the lane below's (a lane adding nothing
+        // has an id of its own)
+        const int32_t key = vid >= 0 ? vid : -2 - lane;
+        const int32_t below = __shfl_up(key, 1, 64);
+        const uint64_t heads = __ballot(lane == 0 || below != key);
+        const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+        const uint64_t after = heads & ~upto;
+        const int last = after ? __ffsll((long long)after) - 2 : 63;
+        int32_t c = vid >= 0 ? 1 : 0;
+        LW_UNROLL for (int off = 1; off < 64; off <<= 1) {  // the run's totals end up at its first lane
+            float o[9];
+            LW_UNROLL for (int k = 0; k < 9; ++k) o[k] = __shfl_down(v[k], off, 64);
+            const int32_t oc = __shfl_down(c, off, 64);
+            if (lane + off <= last) {
+                LW_UNROLL for (int k = 0; k < 9; ++k) v[k] = v[k] + o[k];
+                c += oc;
+            }
+        }
+        const bool head = (heads >> lane) & 1ull;
+        if (vid < 0 || !head) continue;
         VoxMom *m = &M.vox_mom[vid];
-        unsafeAtomicAdd(&m->s[0], r0);
-        unsafeAtomicAdd(&m->s[1], r1);
-        unsafeAtomicAdd(&m->s[2], r2);
-        unsafeAtomicAdd(&m->o[0], r0 * r0);
-        unsafeAtomicAdd(&m->o[1], r0 * r1);
-        unsafeAtomicAdd(&m->o[2], r0 * r2);
-        unsafeAtomicAdd(&m->o[3], r1 * r1);
-        unsafeAtomicAdd(&m->o[4], r1 * r2);
-        unsafeAtomicAdd(&m->o[5], r2 * r2);
-        atomicAdd(&m->cnt, 1);
+        LW_UNROLL for (int k = 0; k < 3; ++k) unsafeAtomicAdd(&m->s[k], v[k]);
+        LW_UNROLL for (int k = 0; k < 6; ++k) unsafeAtomicAdd(&m->o[k], v[3 + k]);
+        atomicAdd(&m->cnt, c);
     }
 }
 
