@@ -180,7 +180,8 @@ def localisation(args, torch, lw, dev):
         "closest_final_m": [round(c, 3) for c in closest],
         "mean_iters": round(float(I.mean()), 2), "point_iterations": int(prof[2]),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "k_loc",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic("k_loc"),
+                     "kernel": f"k_loc<{tile[0]}, {tile[2]}>",
                      "bytes_per_launch": round(bytes_per_launch, 1)},
         "map_build": {"points": n_map, "voxels": V, "seconds": round(t_map, 4),
                       "points_per_s": round(n_map / t_map, 1), "algorithmic_GBps": round(map_bytes / t_map / 1e9, 2)},
@@ -216,11 +217,15 @@ def bulk_voxelisation(torch, lw, dev, X, voxel: float = 0.05, slots_log2: int = 
         del out
     n = int(X.shape[0])
     t_ins, t_emi = min(ins), min(emi)
+    achieved = (12.0 * n + 40.0 * V) / t_ins / 1e9
     return {"points": n, "voxel_m": voxel, "slots": 1 << slots_log2, "voxels": V, "dropped": dropped,
             "insert_ms": round(t_ins * 1e3, 3), "emit_ms": round(t_emi * 1e3, 3),
             "points_per_s": round(n / t_ins, 1),
-            "insert_algorithmic_GBps": round((12.0 * n + 40.0 * V) / t_ins / 1e9, 2),
-            "emit_algorithmic_GBps": round((n / 8.0 + 40.0 * V + 32.0 * V) / t_emi / 1e9, 2)}
+            "emit_algorithmic_GBps": round(((1 << slots_log2) / 8.0 + 40.0 * V + 32.0 * V) / t_emi / 1e9, 2),
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic("k_vox_insert"),
+                         "kernel": "k_vox_insert (one launch over every map point)",
+                         "bytes_per_launch": 12.0 * n + 40.0 * V}}
 
 
 def _thin(pts: np.ndarray, voxel: float) -> np.ndarray:

@@ -21,7 +21,7 @@
 // once (the latency of a step, not its bandwidth, bounds these small products). A layer's weight gradient
 // (K = the batch, an output of only 161-257 x 256) splits K over many workgroups and shares its launch with
 // the gradient flowing into the layer below; the last kernel adds every layer's splits in split order (the
-// same sum on every run) and, in a training step, takes the Adam step and the loss. A step is ten launches,
+// same sum on every run) and, in a training step, takes the Adam step and the loss. A step is six launches,
 // which a caller can replay from a captured hipGraph.
 #include <algorithm>
 #include <climits>
