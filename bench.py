@@ -188,7 +188,10 @@ def localisation(args, torch, lw, dev):
         "gen_seconds": round(t_gen, 2),
     }
     res["bulk_voxelisation"] = bulk_voxelisation(torch, lw, dev, X)
-    res["global_search"] = global_search(torch, lw, dev, W, loc)
+    try:
+        res["global_search"] = global_search(torch, lw, dev, W, loc)
+    finally:
+        loc.close()  # the map's context and its captured graph, before the interpreter's teardown
     if args.cpu_frames > 0:
         res["cpu_baseline"] = loc_cpu_baseline(W, hyps, args.loc_cpu_hyps)
     return res

@@ -199,6 +199,9 @@ class Odometry:
         """Wait for the instance's queued work and free its context (map, frame work buffers, pinned readback
         memory). Idempotent; the instance is unusable afterwards. Call it before the interpreter exits rather than
         leaving the context to the garbage collector during shutdown."""
+        prior = getattr(self, "prior", None)
+        if prior is not None and hasattr(prior, "close"):
+            prior.close()  # its network shares the instance's stream
         ctx = getattr(self, "_ctx", None)
         self._ctx = None
         if ctx is not None and ctx.value:

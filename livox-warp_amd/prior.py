@@ -213,6 +213,10 @@ class LearnedPrior:
         self.last = None
         self.guess_err = {"learned": [], "cv": []}
 
+    def close(self):
+        """Free the network's device state (idempotent; Odometry.close calls it)."""
+        self.mlp.close()
+
     def guess(self, odo, m: int):
         if len(self.hist) < 4:
             return None
@@ -307,6 +311,7 @@ def collect(lw, seeds, n_frames: int, device, seconds_margin: float = 1.0, progr
         odo, _ = run_sequence(lw, seq, n_frames, device, prior_factory=lambda o: rec.setdefault("r", Recorder(o, n_frames)))
         r = rec["r"]
         traj = [T for _, _, T in odo.traj]
+        odo.close()
         rows, targets = [], []
         for row, f in enumerate(r.frames):
             if f < 2 or f >= len(traj):
@@ -444,6 +449,7 @@ def evaluate(lw, params, norm: Normalizer, seeds, n_frames: int, device, fallbac
                     e = np.array(ge[key]) if ge[key] else np.zeros((1, 2))
                     r[f"guess_{key}_mm"] = float(e[:, 0].mean() * 1e3)
                     r[f"guess_{key}_mdeg"] = float(np.degrees(e[:, 1].mean()) * 1e3)
+            odo.close()
             out[name] = r
         rows.append(out)
         if progress is not None:

@@ -118,3 +118,31 @@ def test_float32_residual_cost(lw, orc, translated, gpu_ok):
     _, errs = _per_registration(lw, orc, translated, "fp64")
     _report("fp64", errs, "all-float64 oracle")
     assert errs[:, 0].max() < 0.1 and errs[:, 1].max() < 1.0
+
+
+def test_default_chain_path_at_1250_m_matches_the_oracle(lw, orc, gpu_ok):
+    """The same translated walk through the default path (one device chain per frame, float64 accumulation):
+    every registration the chain runs, replayed by the oracle on the device's own map at that point, at the
+    parity bar."""
+    import seqcheck
+
+    n = 100
+    seq = lw.synth.Sequence(seed=21, seconds=n / 10 + 1, offset=OFFSET)
+    X, A, Tt = seq.frames_device(n, "cuda:0")
+    per = X.shape[1]
+    odo = seqcheck.odometry(lw, reg_slots=SLOTS)
+    assert odo.chain
+    rec = seqcheck.Recorder(lw, odo)
+    try:
+        odo.reset(seq.pose(0.05))
+        for k in range(n):
+            odo.push(X[k], A[k], Tt[k], per, float(Tt[k, 0]), float(Tt[k, -1]), rec.on_frame)
+        odo.synchronize()
+        P = seqcheck.params(orc, odo)
+    finally:
+        odo.close()
+    out, _ = seqcheck.replay(orc, rec.events, P, SLOTS)
+    errs = np.array([e for _, e in out.values()])
+    print(f"1250 m, default chain: {len(out)} registrations vs the oracle: max {errs[:, 0].max() * 1e3:.4f} mm / "
+          f"{errs[:, 1].max():.5f} deg", flush=True)
+    assert len(out) >= 2 * (n - 15) and errs[:, 0].max() < 1e-3 and errs[:, 1].max() < 0.01
