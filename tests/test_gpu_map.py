@@ -98,3 +98,20 @@ def test_reset_empties_the_map(lw, orc, gpu_ok):
     om = orc.OracleMap(1 << 12)
     om.insert(pts, np.zeros(len(pts), np.uint32), np.eye(4), 0.3, max_range=1e30)
     _compare_maps(orc, eng, om, 0.3)
+
+
+def test_one_chunk_larger_than_the_launch_grid_cap(lw, orc, gpu_ok):
+    """17.3M points in one chunk (frame_cap 2^25): more than 65535 workgroups of 256 threads would cover, so the
+    insert kernels' grid-stride loops must reach the tail. Counts, occupancy and drops equal the oracle's."""
+    rng = np.random.default_rng(21)
+    n = 65535 * 256 + 500_000
+    xyz = rng.uniform(-40.0, 40.0, (n, 3)).astype(np.float32)
+    with lw.Engine(1 << 22, 1 << 25) as eng:
+        eng.insert(xyz, None, np.eye(4), 1.0, max_range=1e30)  # 512k voxels of about 34 points
+        stats = eng.stats()
+        keys, _, _, rcnt = eng.read_map()
+    om = orc.OracleMap(1 << 22)
+    om.insert(xyz, np.zeros(n, np.uint32), np.eye(4), 1.0, max_range=1e30)
+    assert stats == (int(om.stats[0]), int(om.stats[1])) and stats[1] == 0
+    assert np.array_equal(keys != -1, om.keys != -1)
+    assert int(rcnt.sum()) == int(om.rcnt.sum()) == n  # every point counted, the tail included
