@@ -48,7 +48,7 @@ void free_all(lw_ctx *c) {
     void *ptrs[] = {c->slots, c->occ, c->vox_key, c->vox_mom, c->vox_slot, c->vox_prec, c->vox_nin, c->vox_quota,
                     c->vox_dirty, c->dirty_list, c->touched, c->ctr, c->pt_slot, c->pt_vid, c->pt_take,
                     c->ds_keys, c->ds_idx, c->ds_xyz, c->blk_cnt, c->blk_off, c->chain, c->partials, c->best_slot,
-                    c->dtmp, c->loop_flags};
+                    c->dtmp, c->loop_epoch, c->loop_rows};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->prof_dev) (void)hipFree(c->prof_dev);
@@ -175,7 +175,8 @@ int lw_create(lw_ctx **out, int64_t reg_slots, int64_t frame_cap, void *stream) 
     c->fctl = reinterpret_cast<lw::FrameCtl *>(c->chain + CH_FCTL);
     c->chain_counts = reinterpret_cast<int32_t *>(c->chain + CH_COUNTS);
     if ((e = dalloc(&c->partials, (size_t)lw::icp_max_groups(frame_cap) * 128)) != hipSuccess) return fail(e);
-    if ((e = dalloc(&c->loop_flags, lw::MAX_LOOP_ROWS)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->loop_epoch, 1)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->loop_rows, (size_t)lw::LOOP_TAGGED_ROWS * lw::LOOP_ROW_WORDS)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->best_slot, frame_cap)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->dtmp, frame_cap + 64)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->loc_queue, 16)) != hipSuccess) return fail(e);
@@ -185,6 +186,7 @@ int lw_create(lw_ctx **out, int64_t reg_slots, int64_t frame_cap, void *stream) 
     hipLaunchKernelGGL(k_fill_i64, dim3(1024), dim3(256), 0, c->stream, c->ds_keys, c->ds_cap, (int64_t)-1);
     if ((e = lw::launch_fill_i32(c, c->ds_idx, c->ds_cap, INT_MAX)) != hipSuccess) return fail(e);
     if ((e = lw::launch_fill_i32(c, c->ctr, 64, 0)) != hipSuccess) return fail(e);
+    c->reg_starts = lw::LOOP_EPOCH_LIMIT;  // the first registration start clears the rows and the epoch
     if ((e = lw::launch_map_reset(c)) != hipSuccess) return fail(e);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return fail(e);
     (void)rc;
@@ -487,6 +489,7 @@ int run_registration(lw_ctx *c, const int32_t *n_dev, int32_t n_bound, int32_t n
     std::memcpy(pin_up(c) + 16 * sizeof(double), L.kappa, 64 * sizeof(float));
     LW_CK(hipMemcpyAsync(c->dtmp, pin_up(c), 16 * sizeof(double) + 64 * sizeof(float), hipMemcpyHostToDevice,
                          c->stream));
+    LW_CK(lw::loop_epoch_guard(c, 1));
     LW_CK(lw::launch_reg_begin(c, (const double *)c->dtmp));
     hipError_t e = hipErrorNotSupported;
     if (c->loop_ok) {
@@ -716,6 +719,7 @@ int lw_odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, lw_f
     // a frame one workgroup can prepare (up to frame_prep_max() points, with a thinning table of at least 4096
     // slots): deskew, thinning and the registration's start in one launch each time
     const bool one = c->frame_bound <= lw::frame_prep_max() && c->ds_cap >= 4096;
+    LW_CK(lw::loop_epoch_guard(c, 2));  // the two registrations' starts below
     // odom.py:671-681: deskew with the model's twist, register from the prediction
     if (one) {
         LW_CK(lw::launch_frame_prep(c, c->chain_tp, in->t_mid, in->inv_dt, in->xi, inv_v_thin, p->min_range,
@@ -1115,7 +1119,7 @@ int lw_batch_frame(lw_batch *b, const lw_reg_params *p, const lw_frame_in *in, c
         A.n_out = &c->ctr[5];
         A.st = at<lw::RegState>(b, b->o_st) + s;
         A.Tp = tp_dev;
-        A.flags = c->loop_flags;
+        A.epoch = nullptr;  // the batched registration has no persistent loop
         A.st1 = nullptr;
         A.fc = fc_dev;
         prep1[s] = A;

@@ -337,6 +337,7 @@ __device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf
 constexpr int RED_COLS = ACC_USED + 2;
 constexpr int RED_BATCH = 16;
 constexpr int MAX_BATCHES = MAX_LOOP_ROWS / RED_BATCH;
+static_assert(LOOP_TAGGED_ROWS == MAX_LOOP_ROWS + MAX_BATCHES, "the tagged rows, then one batch row per batch");
 static_assert(MAX_BATCHES <= 32, "one bit per batch");
 
 struct BSum {
@@ -417,92 +418,161 @@ __device__ inline double reduce_rows(const double *partials, int G) {
     return entry_of<MODE>(acc);
 }
 
+// ---- the persistent loop's tagged rows ---------------------------------------------------------------------
+//
+// The one-pass launch's rows are plain doubles behind a release and a ticket. The persistent loop's rows are
+// tagged instead: every value as two 64-bit words, (tag << 32) | one half of its bits, each stored and loaded
+// as one single-copy-atomic access. A word is therefore current exactly when it carries the pass's tag, so a
+// writer needs no release and no flag, and a reader takes a row as soon as every word it reads carries the
+// tag. Row g: words
+// 2k and 2k + 1 hold entry k (the 53 sums, then the two measurement counts), and in the compensated form words
+// 128 + 2k and 129 + 2k the compensation of entry k. The tag is (epoch << 7) | (pass + 1): the epoch advances
+// at every registration start (reg_state_begin) and is cleared with the rows before it could wrap, so no word
+// left from an earlier pass or registration carries a current tag.
+__device__ inline uint32_t row_tag(uint32_t epoch, int it) { return (epoch << 7) | (uint32_t)(it + 1); }
+
+__device__ inline void put_tagged(uint64_t *w, uint32_t tag, double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v), t = (uint64_t)tag << 32;
+    put(&w[0], (uint64_t)(t | (b & 0xffffffffu)));
+    put(&w[1], (uint64_t)(t | (b >> 32)));
+}
+
+__device__ inline bool has_tag(uint64_t w, uint32_t tag) { return (uint32_t)(w >> 32) == tag; }
+
+__device__ inline double join_tagged(uint64_t lo, uint64_t hi) {
+    return __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffu)));
+}
+
+template <int MODE>
+__device__ inline void write_row_tagged(uint64_t *rows, int g, uint32_t tag, double sum, double comp, int32_t hits,
+                                        int32_t matched) {
+    const int lane = threadIdx.x;
+    uint64_t *row = rows + (size_t)g * LOOP_ROW_WORDS;
+    LW_UNROLL for (int off = 32; off > 0; off >>= 1) {
+        hits += __shfl_xor(hits, off, 64);
+        matched += __shfl_xor(matched, off, 64);
+    }
+    if (lane < RED_COLS) {
+        const double v = lane < ACC_USED ? sum : (lane == ACC_USED ? (double)hits : (double)matched);
+        put_tagged(&row[2 * lane], tag, v);
+        if constexpr (MODE == 1)
+            if (lane < ACC_USED) put_tagged(&row[128 + 2 * lane], tag, comp);
+    }
+}
+
+// The sums of batches b[0..NB) of the tagged rows, formed exactly as sum_batches forms them (row order from
+// zero, rows at or past G adding nothing), and per batch whether every word it needs carried the tag (if not,
+// a row is still in flight and the batch is read again later). Every load is issued before any is used.
+template <int MODE, int NB>
+__device__ inline void sum_tagged(const uint64_t *rows, int G, uint32_t tag, const int (&b)[NB], BSum (&out)[NB],
+                                  bool (&ok)[NB]) {
+    constexpr int W = MODE == 1 ? 4 : 2;
+    const int lane = threadIdx.x;
+    const int cx = lane < RED_COLS ? lane : 0, cy = lane < ACC_USED ? lane : 0;
+    uint64_t w[NB][RED_BATCH][W];
+    LW_UNROLL for (int k = 0; k < NB; ++k)
+        LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) {
+            const uint64_t *row = rows + (size_t)min(b[k] * RED_BATCH + u, G - 1) * LOOP_ROW_WORDS;
+            w[k][u][0] = get(row + 2 * cx);
+            w[k][u][1] = get(row + 2 * cx + 1);
+            if constexpr (MODE == 1) {
+                w[k][u][2] = get(row + 128 + 2 * cy);
+                w[k][u][3] = get(row + 129 + 2 * cy);
+            }
+        }
+    LW_UNROLL for (int k = 0; k < NB; ++k) {
+        BSum r{0.0, 0.0f, 0.0f};
+        bool good = true;
+        LW_UNROLL for (int u = 0; u < RED_BATCH; ++u) {
+            const bool in = b[k] * RED_BATCH + u < G;
+            bool cur = has_tag(w[k][u][0], tag) && has_tag(w[k][u][1], tag);
+            double y = 0.0;
+            if constexpr (MODE == 1) {
+                cur = cur && has_tag(w[k][u][2], tag) && has_tag(w[k][u][3], tag);
+                y = join_tagged(w[k][u][2], w[k][u][3]);
+            }
+            good = good && (cur || !in);
+            add_row<MODE>(r, in, join_tagged(w[k][u][0], w[k][u][1]), y);
+        }
+        ok[k] = __ballot(!good) == 0ull;
+        out[k] = r;
+    }
+}
+
 constexpr uint32_t SPIN_LIMIT = 1u << 24;  // ~ seconds of polling: a wait that long means the loop is broken
 constexpr int32_t STATUS_SPIN_TIMEOUT = -1;
 
-// A poll of the G row flags: lane l reads rows l, l + 64, ... (rows past G read row G - 1). Issuing and
-// evaluating are separate so that several polls can be in flight at once.
-struct Poll {
-    uint32_t f[MAX_LOOP_ROWS / 64];
-};
+// ---- the persistent loop's two-level reduction ------------------------------------------------------
+//
+// Batch b of a pass is rows [16b, 16b + 16) of [0, G). Its head, searcher 16b, waits for the batch's tagged
+// rows, sums them in row order from zero (sum_tagged: sum_batches' arithmetic) and writes the sum as tagged
+// batch row b; the leader waits for the batch rows and combines them in batch order. That is reduce_rows'
+// order, so the one-pass launch and the persistent loop give the same sums bit for bit. Waiting on 16 rows
+// at a time (the heads, side by side) and then on at most 16 batch rows (the leader, in one poll) takes the
+// last row about two round trips after it is written.
 
-__device__ inline void poll_issue(const uint32_t *flags, int G, Poll &p) {
-    const int lane = threadIdx.x;
-    LW_UNROLL for (int k = 0; k < MAX_LOOP_ROWS / 64; ++k) p.f[k] = get(&flags[min(64 * k + lane, G - 1)]);
-}
-
-// Bit b set when every row of batch b carries the tag (rows past G count as set).
-__device__ inline uint32_t poll_eval(const Poll &p, int G, uint32_t tag) {
-    const int lane = threadIdx.x;
-    uint32_t ready = 0u;
-    LW_UNROLL for (int k = 0; k < MAX_LOOP_ROWS / 64; ++k) {
-        const unsigned long long m = __ballot(p.f[k] >= tag || 64 * k + lane >= G);
-        LW_UNROLL for (int q = 0; q < 64 / RED_BATCH; ++q)
-            if (((m >> (q * RED_BATCH)) & ((1ull << RED_BATCH) - 1ull)) == ((1ull << RED_BATCH) - 1ull))
-                ready |= 1u << (k * (64 / RED_BATCH) + q);
-    }
-    return ready;
-}
-
-__device__ inline uint32_t poll_flags(const uint32_t *flags, int G, uint32_t tag) {
-    Poll p;
-    poll_issue(flags, G, p);
-    return poll_eval(p, G, tag);
-}
-
-// The persistent loop's leader: sums each batch whose rows all carry this pass's tag into bs (LDS,
-// MAX_BATCHES x 64), up to NB batches at a time with all their loads in flight and the next poll of the flags
-// beside them, in whatever order the batches complete, and combines the batch sums in batch order once every
-// row is in. t_in: the device clock when the last rows were seen (measurement). False when rows are still
-// missing after SPIN_LIMIT polls that found nothing new.
+// Batch row words: entry k (lane k) at words 4k..4k+3: the float64 sum's two halves, then (forms 1 and 2)
+// the float32 sum and (form 1) its compensation.
 template <int MODE>
-__device__ inline bool leader_reduce(const double *partials, const uint32_t *flags, int G, uint32_t tag, BSum *bs,
-                                     double &v, unsigned long long &t_in, bool stamp) {
-    constexpr int NB = MODE == 1 ? 2 : 4;
+__device__ inline void write_batch_tagged(uint64_t *row, uint32_t tag, const BSum &r) {
     const int lane = threadIdx.x;
-    const int nb = (G + RED_BATCH - 1) / RED_BATCH;
-    uint32_t pend = (nb >= 32) ? ~0u : ((1u << nb) - 1u);
-    uint32_t spins = 0;
-    uint32_t ready = poll_flags(flags, G, tag) & pend;
-    while (pend) {
-        if (ready == 0u) {
-            // nothing new: poll with two polls in flight, so a flag is seen about half a round trip after it lands
-            Poll pa, pb;
-            poll_issue(flags, G, pa);
-            while (true) {
-                poll_issue(flags, G, pb);
-                ready = poll_eval(pa, G, tag) & pend;
-                if (ready != 0u) break;
-                if (++spins > SPIN_LIMIT) return false;
-                poll_issue(flags, G, pa);
-                ready = poll_eval(pb, G, tag) & pend;
-                if (ready != 0u) break;
-                if (++spins > SPIN_LIMIT) return false;
-            }
-            continue;
+    if (lane >= RED_COLS) return;
+    put_tagged(&row[4 * lane], tag, r.e);
+    const uint64_t t = (uint64_t)tag << 32;
+    if constexpr (MODE != 0) put(&row[4 * lane + 2], (uint64_t)(t | __float_as_uint(r.s)));
+    if constexpr (MODE == 1) put(&row[4 * lane + 3], (uint64_t)(t | __float_as_uint(r.c)));
+}
+
+// Batch b's head: waits until every word of the batch's rows carries the tag, then writes their sum as batch
+// row b (brows + b * LOOP_ROW_WORDS). False after SPIN_LIMIT polls that found a row missing.
+template <int MODE>
+__device__ inline bool batch_head(const uint64_t *rows, uint64_t *brows, int G, int b, uint32_t tag) {
+    const int bb[1] = {b};
+    for (uint32_t spins = 0; spins <= SPIN_LIMIT; ++spins) {
+        BSum out[1];
+        bool ok[1];
+        sum_tagged<MODE, 1>(rows, G, tag, bb, out, ok);
+        if (ok[0]) {
+            write_batch_tagged<MODE>(brows + (size_t)b * LOOP_ROW_WORDS, tag, out[0]);
+            return true;
         }
-        if (stamp && ready == pend) t_in = wall_clock64();
-        int b[NB];
-        int n_valid = 0;
-        LW_UNROLL for (int k = 0; k < NB; ++k) {
-            b[k] = ready ? __builtin_ctz(ready) : b[0];
-            if (ready) {
-                ready &= ready - 1u;
-                pend &= ~(1u << b[k]);
-                ++n_valid;
-            }
-        }
-        const uint32_t next = pend & ~ready ? poll_flags(flags, G, tag) : 0u;  // in flight with the rows
-        BSum out[NB];
-        sum_batches<MODE, NB>(partials, G, b, out);
-        LW_UNROLL for (int k = 0; k < NB; ++k)
-            if (k < n_valid && lane < RED_COLS) bs[b[k] * ICP_BLOCK + lane] = out[k];
-        ready = (ready | next) & pend;
     }
-    BSum acc{0.0, 0.0f, 0.0f};
-    for (int b = 0; b < nb; ++b) combine<MODE>(acc, bs[b * ICP_BLOCK + lane]);  // this lane's own entries
-    v = entry_of<MODE>(acc);
-    return true;
+    return false;
+}
+
+// The leader: polls the ceil(G / 16) batch rows (every load in flight at once) until every word it needs
+// carries the tag, then combines the batch sums in batch order into this lane's entry v. t_in: the device
+// clock when the last batch row was seen (measurement). False after SPIN_LIMIT polls that found a batch
+// missing.
+template <int MODE>
+__device__ inline bool leader_reduce(const uint64_t *brows, int G, uint32_t tag, double &v, unsigned long long &t_in,
+                                     bool stamp) {
+    constexpr int W = MODE == 0 ? 2 : MODE == 2 ? 3 : 4;
+    const int lane = threadIdx.x;
+    const int cx = lane < RED_COLS ? lane : 0;
+    const int nb = (G + RED_BATCH - 1) / RED_BATCH;
+    for (uint32_t spins = 0; spins <= SPIN_LIMIT; ++spins) {
+        uint64_t w[MAX_BATCHES][W];
+        LW_UNROLL for (int b = 0; b < MAX_BATCHES; ++b) {
+            const uint64_t *row = brows + (size_t)min(b, nb - 1) * LOOP_ROW_WORDS + 4 * cx;
+            LW_UNROLL for (int q = 0; q < W; ++q) w[b][q] = get(row + q);
+        }
+        bool good = true;
+        LW_UNROLL for (int b = 0; b < MAX_BATCHES; ++b)
+            LW_UNROLL for (int q = 0; q < W; ++q) good = good && (b >= nb || has_tag(w[b][q], tag));
+        if (__ballot(!good) != 0ull) continue;
+        if (stamp) t_in = wall_clock64();
+        BSum acc{0.0, 0.0f, 0.0f};
+        LW_UNROLL for (int b = 0; b < MAX_BATCHES; ++b) {
+            BSum r{join_tagged(w[b][0], w[b][1]), 0.0f, 0.0f};
+            if constexpr (W > 2) r.s = __uint_as_float((uint32_t)w[b][W > 2 ? 2 : 0]);
+            if constexpr (W > 3) r.c = __uint_as_float((uint32_t)w[b][W > 3 ? 3 : 0]);
+            if (b < nb) combine<MODE>(acc, r);
+        }
+        v = entry_of<MODE>(acc);
+        return true;
+    }
+    return false;
 }
 
 // ---- the float64 step ------------------------------------------------------------------------------
@@ -694,9 +764,10 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_icp(IcpArgs a) {
 // ---- the persistent registration ------------------------------------------------------------------
 //
 // Workgroup 0 is the leader, the others search. Per pass, searcher g (= blockIdx.x - 1) takes the point
-// groups g, g + G, ... (G = the searchers the pass can use), writes its partial row and then its row flag
-// (the pass's tag); the leader, which searches nothing, sums row batches as their flags come in, runs the
-// step, and publishes the new pose (float32, tagged; pub in RegState), which the searchers poll for. Only
+// groups g, g + G, ... (G = the searchers the pass can use) and writes its tagged partial row; every 16th
+// searcher also sums its batch's rows into a batch row; the leader, which searches nothing, combines the batch
+// rows, runs the step, and publishes the new pose (float32, tagged; pub in RegState), which the searchers poll
+// for. Only
 // the leader keeps the float64 pose; the step's code and the reduction run in one workgroup that always
 // has them in its caches, and the twist to the prediction is computed while the searchers work. Every wait
 // is bounded (SPIN_LIMIT polls, then status STATUS_SPIN_TIMEOUT); co-residency is what the cooperative
@@ -706,7 +777,8 @@ struct LoopArgs {
     IcpArgs a;                // pointers and the fixed scalars; n, max_d2, kappa, it are set per pass
     const int32_t *n_dev;     // thinned count on the device
     const int32_t *gate;      // when given, a device flag that must be nonzero for the loop to run
-    uint32_t *flags;          // MAX_LOOP_ROWS row flags
+    const uint32_t *epoch;    // the row-tag epoch (advanced by the registration's start)
+    uint64_t *rows;           // MAX_LOOP_ROWS tagged rows, then MAX_BATCHES batch rows, of LOOP_ROW_WORDS words
     int32_t max_iter, wide_iters, min_corr;
     float max_d2_r1, max_d2_r2;
 };
@@ -718,16 +790,14 @@ __device__ inline int searchers_for(int64_t n, int S) {
 }
 
 template <int MODE>
-__device__ inline void leader(const LoopArgs &L, int32_t n, unsigned char *lds) {
+__device__ inline void leader(const LoopArgs &L, int32_t n, uint32_t epoch, unsigned char *lds) {
     const IcpArgs &a = L.a;
     RegState *st = a.st;
     const int lane = threadIdx.x;
     __shared__ double sT[16], sTp[16], vs[64], xi[6];
     __shared__ int32_t s_flag;
-    static_assert(MAX_BATCHES * ICP_BLOCK * sizeof(BSum) + gnw::W_DOUBLES * sizeof(double) <= RED_BYTES,
-                  "the batch sums and the step's scratch share the searchers' LDS rows");
-    BSum *bs = reinterpret_cast<BSum *>(lds);
-    double *sh = reinterpret_cast<double *>(lds + MAX_BATCHES * ICP_BLOCK * sizeof(BSum));
+    static_assert(gnw::W_DOUBLES * sizeof(double) <= RED_BYTES, "the step's scratch fits the searchers' LDS rows");
+    double *sh = reinterpret_cast<double *>(lds);
     if (lane < 16) {
         sT[lane] = st->T[lane];
         sTp[lane] = st->T_pred[lane];
@@ -742,7 +812,8 @@ __device__ inline void leader(const LoopArgs &L, int32_t n, unsigned char *lds) 
         if (lane == 0 && a.sp.prior_info > 0.0) gn::twist_between(sT, sTp, xi);
         double v = 0.0;
         unsigned long long t_in = 0ull;
-        if (!leader_reduce<MODE>(a.partials, L.flags, G, (uint32_t)(it + 1), bs, v, t_in, a.phase != nullptr)) {
+        if (!leader_reduce<MODE>(L.rows + (size_t)MAX_LOOP_ROWS * LOOP_ROW_WORDS, G, row_tag(epoch, it), v, t_in,
+                                 a.phase != nullptr)) {
             if (lane == 0) st->status = STATUS_SPIN_TIMEOUT;
             if (lane < 16) put(&st->pub[lane], (unsigned long long)(TAG_STOP | (uint32_t)(it + 1)) << 32);
             return;
@@ -794,7 +865,7 @@ __device__ inline void leader(const LoopArgs &L, int32_t n, unsigned char *lds) 
 }
 
 template <int MODE>
-__device__ inline void searcher(const LoopArgs &L, int32_t n, unsigned char *lds) {
+__device__ inline void searcher(const LoopArgs &L, int32_t n, uint32_t epoch, unsigned char *lds) {
     const IcpArgs &a = L.a;
     RegState *st = a.st;
     const int lane = threadIdx.x;
@@ -812,7 +883,8 @@ __device__ inline void searcher(const LoopArgs &L, int32_t n, unsigned char *lds
     __syncthreads();
     const bool stamp = a.phase && g == 0;  // the first searcher's view of the passes
     unsigned long long t_wake = stamp ? wall_clock64() : 0ull;
-    // every searcher's view (passes after the first): wake-up and row flag after the pose's publication
+    // every searcher's view (passes after the first): wake-up and row (with its batch row, for a head) after the
+    // pose's publication
     unsigned long long t_pub = 0ull, t_woke = 0ull, sum_wake = 0ull, sum_done = 0ull, n_timed = 0ull;
     for (int it = 0; it < L.max_iter; ++it) {
         const int reach = it < L.wide_iters ? 2 : 1;
@@ -829,10 +901,15 @@ __device__ inline void searcher(const LoopArgs &L, int32_t n, unsigned char *lds
                 wave_pass<MODE, 1>(a, p, Tf, nbr, g, G, lds, sum, comp, hits, matched, pt[1], true, stamp, t_found,
                                    t_terms);
             const unsigned long long t_summed = stamp ? wall_clock64() : 0ull;
-            if (a.phase && it > 0 && lane == 0)  // the pass's latest row so far (ordered before the flag)
+            if (a.phase && it > 0 && lane == 0)  // the pass's latest row so far
                 atomicMax(&a.phase[27], wall_clock64() - t_pub);
-            write_row<MODE>(a, g, sum, comp, hits, matched);
-            if (lane == 0) put(&L.flags[g], (uint32_t)(it + 1));
+            const uint32_t tag = row_tag(epoch, it);
+            write_row_tagged<MODE>(L.rows, g, tag, sum, comp, hits, matched);
+            if (g % RED_BATCH == 0 &&
+                !batch_head<MODE>(L.rows, L.rows + (size_t)MAX_LOOP_ROWS * LOOP_ROW_WORDS, G, g / RED_BATCH, tag)) {
+                if (lane == 0) st->status = STATUS_SPIN_TIMEOUT;
+                return;
+            }
             if (a.phase && it > 0) {
                 sum_wake += t_woke - t_pub;
                 sum_done += wall_clock64() - t_pub;
@@ -891,13 +968,14 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_register(LoopArgs L) {
     const int32_t n = *L.n_dev;
     if (blockIdx.x == 0 && threadIdx.x == 0) st->n = n;
     if (n < L.min_corr) return;  // odom.py:756-757: the host sees n and keeps the prediction
+    const uint32_t epoch = *L.epoch;  // advanced by this registration's start (an earlier launch)
     if (blockIdx.x == 0) {
-        leader<MODE>(L, n, lds);
+        leader<MODE>(L, n, epoch, lds);
         return;
     }
     // searchers that no pass can use (the widest pass needs ceil(n * 4 / 64)) leave now
     if ((int64_t)(blockIdx.x - 1) * ICP_BLOCK >= (int64_t)n * Split<2>::S) return;
-    searcher<MODE>(L, n, lds);
+    searcher<MODE>(L, n, epoch, lds);
 }
 
 // ---- a whole registration in one workgroup (the batched odometry) ---------------------------------------
@@ -1000,11 +1078,19 @@ __global__ __launch_bounds__(64 * SEQ_WAVES) void k_register_seq(const SeqReg *D
 }
 
 // Tp: 16 doubles (the prediction) followed by 64 floats (the per-iteration robust-kernel scale); T_dev, when
-// given, holds the prediction instead. Also clears the persistent loop's row flags and published pose. Does
+// given, holds the prediction instead. Also clears the persistent loop's published pose and advances its row-tag
+// epoch. Does
 // nothing when gate is given and holds 0.
-__global__ void k_reg_begin(RegState *st, const double *Tp, uint32_t *flags, const int32_t *gate, const double *T_dev) {
+__global__ void k_reg_begin(RegState *st, const double *Tp, uint32_t *epoch, const int32_t *gate, const double *T_dev) {
     if (gated_off(gate)) return;
-    reg_state_begin(st, T_dev ? T_dev : Tp, reinterpret_cast<const float *>(Tp + 16), flags, threadIdx.x, blockDim.x);
+    reg_state_begin(st, T_dev ? T_dev : Tp, reinterpret_cast<const float *>(Tp + 16), epoch, threadIdx.x, blockDim.x);
+}
+
+// The persistent loop's tagged rows cleared and its row-tag epoch reset (loop_epoch_guard).
+__global__ void k_loop_clear(uint64_t *rows, int64_t n, uint32_t *epoch) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        rows[i] = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *epoch = 0u;
 }
 
 template <int MODE>
@@ -1044,7 +1130,18 @@ const void *loop_kernel() {
 }  // namespace
 
 hipError_t launch_reg_begin(lw_ctx *c, const double *T_pred_dev, const int32_t *gate, const double *T_dev) {
-    hipLaunchKernelGGL(k_reg_begin, dim3(1), dim3(64), 0, c->stream, c->reg, T_pred_dev, c->loop_flags, gate, T_dev);
+    hipLaunchKernelGGL(k_reg_begin, dim3(1), dim3(64), 0, c->stream, c->reg, T_pred_dev, c->loop_epoch, gate, T_dev);
+    return hipGetLastError();
+}
+
+hipError_t loop_epoch_guard(lw_ctx *c, uint32_t starts) {
+    if (c->reg_starts + starts <= LOOP_EPOCH_LIMIT) {
+        c->reg_starts += starts;
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(k_loop_clear, dim3(256), dim3(256), 0, c->stream, c->loop_rows,
+                       (int64_t)LOOP_TAGGED_ROWS * LOOP_ROW_WORDS, c->loop_epoch);
+    c->reg_starts = starts;
     return hipGetLastError();
 }
 
@@ -1122,7 +1219,8 @@ hipError_t launch_register_loop(lw_ctx *c, const int32_t *n_dev, int32_t n_bound
     L.a.sp = R.sp;
     L.n_dev = n_dev;
     L.gate = gate;
-    L.flags = c->loop_flags;
+    L.epoch = c->loop_epoch;
+    L.rows = c->loop_rows;
     L.max_iter = R.max_iter;
     L.wide_iters = R.wide_iters;
     L.min_corr = R.min_corr;

@@ -15,6 +15,11 @@ constexpr int ICP_BLOCK = 64;        // one wave per accumulation workgroup
 constexpr int ICP_MAX_GROUPS = 2048; // partial rows kept for the cross-workgroup reduction
 constexpr int LW_PROF_COUNTERS = 32; // device measurement counters (lw_ctx::prof_dev)
 constexpr int MAX_LOOP_ROWS = 256;   // searching workgroups (partial rows) of the persistent registration
+constexpr int LOOP_ROW_WORDS = 256;  // 64-bit words of one tagged partial row of the persistent registration
+constexpr int LOOP_TAGGED_ROWS = MAX_LOOP_ROWS + MAX_LOOP_ROWS / 16;  // its rows, then one batch row per 16 rows
+// Registration starts between two clears of the persistent registration's row tags: the tag's epoch has 25
+// bits and advances at most once a start, so it never wraps between clears.
+constexpr uint32_t LOOP_EPOCH_LIMIT = 1u << 24;
 constexpr int SCAN_THREADS = 1024;
 
 // Device-resident state of one registration (one Gauss-Newton loop).
@@ -83,7 +88,7 @@ struct PrepArgs {
     int32_t *n_out;        // their count
     struct RegState *st;   // the registration state to set up
     const double *Tp;      // the prediction (16 doubles) and the robust-kernel scales (64 floats)
-    uint32_t *flags;       // the persistent loop's row flags
+    uint32_t *epoch;       // the persistent loop's row-tag epoch (null: the batched chain, which has no loop)
     struct RegState *st1;  // given: the frame check runs first, and the rest only when it asks for a re-deskew
     struct FrameCtl *fc;
 };
@@ -157,7 +162,9 @@ struct lw_ctx {
     bool count_reset = false;  // lw_frame_reset's zeroing of ctr[6], left to the next append or reader of it
     hipEvent_t chain_ev = nullptr;  // lw_odom_frame's readback (the host waits for it, not for the insert's tail)
     double *partials = nullptr;
-    uint32_t *loop_flags = nullptr;  // MAX_LOOP_ROWS: the pass each searching workgroup's row belongs to
+    uint32_t *loop_epoch = nullptr;  // the persistent registration's row-tag epoch (one word)
+    uint64_t *loop_rows = nullptr;   // LOOP_TAGGED_ROWS x LOOP_ROW_WORDS: the persistent registration's tagged rows
+    uint32_t reg_starts = 0;         // launches that may start a registration since the rows were last cleared
     int32_t *best_slot = nullptr;
     int64_t *dtmp = nullptr;  // small uploads: poses, keys (frame_cap + 8 entries)
     // pinned host staging
@@ -211,12 +218,12 @@ __device__ inline Mat4f load_mat4f(const float *p) {
 }
 
 // A registration's initial state (odom.py:758-763): the pose P (16 doubles) as current pose and prediction,
-// the per-iteration robust-kernel scale, H = I, counters cleared; with the persistent loop's row flags and
-// published pose cleared. Thread tid of nth takes part.
-__device__ inline void reg_state_begin(RegState *st, const double *P, const float *kappa, uint32_t *flags, int tid,
+// the per-iteration robust-kernel scale, H = I, counters cleared; with the persistent loop's published pose
+// cleared and (epoch given) its row-tag epoch advanced. Thread tid of nth takes part.
+__device__ inline void reg_state_begin(RegState *st, const double *P, const float *kappa, uint32_t *epoch, int tid,
                                        int nth) {
     if (tid < 64) st->kappa[tid] = kappa[tid];
-    for (int k = tid; k < MAX_LOOP_ROWS; k += nth) flags[k] = 0u;
+    if (epoch && tid == 0) *epoch += 1u;
     if (tid < 16) st->pub[tid] = 0ull;
     if (tid != 0) return;
     for (int k = 0; k < 16; ++k) {
@@ -354,6 +361,9 @@ hipError_t launch_register_seq(hipStream_t s, const SeqReg *D, int32_t S, const 
 int32_t icp_max_groups(int64_t frame_cap);  // partial rows one pass over frame_cap points can write
 hipError_t launch_reg_begin(lw_ctx *c, const double *T_pred_dev, const int32_t *gate = nullptr,
                             const double *T_dev = nullptr);
+// Before launches that may start up to `starts` registrations of c (each advances the row-tag epoch once at
+// most): clears the tagged rows and the epoch on c's stream once LOOP_EPOCH_LIMIT starts have accumulated.
+hipError_t loop_epoch_guard(lw_ctx *c, uint32_t starts);
 // lw_odom.hip: after a frame's first registration, and after its second
 hipError_t launch_frame_check(lw_ctx *c);
 hipError_t launch_keyframe_check(lw_ctx *c);

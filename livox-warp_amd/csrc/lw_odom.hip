@@ -256,7 +256,7 @@ __device__ __forceinline__ void frame_prep(const PrepArgs &A) {
     if (tid == 0) s_over = 0;
     __syncthreads();
     if (!s_go) return;
-    reg_state_begin(A.st, s_T, reinterpret_cast<const float *>(A.Tp + 16), A.flags, tid, PREP_THREADS);
+    reg_state_begin(A.st, s_T, reinterpret_cast<const float *>(A.Tp + 16), A.epoch, tid, PREP_THREADS);
     const int32_t m = count_of(A.frame_cap, A.m_dev);
     if (m > PREP_THREADS * PREP_ROUNDS) {  // more than the caller's bound: thin nothing, and say so
         if (tid == 0) {
@@ -419,7 +419,7 @@ hipError_t launch_frame_prep(lw_ctx *c, const double *Tp, float t_mid, float inv
     A.n_out = &c->ctr[5];
     A.st = c->reg;
     A.Tp = Tp;
-    A.flags = c->loop_flags;
+    A.epoch = c->loop_epoch;
     A.st1 = check ? c->reg1 : nullptr;
     A.fc = c->fctl;
     hipLaunchKernelGGL(k_frame_prep, dim3(1), dim3(PREP_THREADS), 0, c->stream, A);
