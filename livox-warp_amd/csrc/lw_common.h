@@ -25,6 +25,16 @@ constexpr int ACC_USED = 53;                          // 21 + 6 + 5 + 21
 // The calling thread's lane in its wave (64 lanes).
 __device__ inline int lane_id() { return (int)(threadIdx.x & 63u); }
 
+// A device pointer p read from a descriptor in device memory (the batched kernels' per-sequence tables),
+// re-derived from a kernel argument (base): the compiler then knows it points into global memory and accesses
+// it with global instructions (a scalar base, 32-bit lane offsets) instead of flat ones, which also count
+// against the LDS counter. The value is p itself (null stays null).
+template <class T>
+__device__ inline T *via_arg(const void *base, T *p) {
+    const int64_t off = (int64_t)(reinterpret_cast<const char *>(p) - reinterpret_cast<const char *>(base));
+    return p ? reinterpret_cast<T *>(const_cast<char *>(reinterpret_cast<const char *>(base)) + off) : nullptr;
+}
+
 // A barrier among the lanes of one wave, with LDS written before it visible to every lane after it (the
 // workgroup barrier would also wait for the workgroup's other waves, which may be at another barrier).
 __device__ inline void wave_sync() {

@@ -76,14 +76,16 @@ struct Acc<2> {  // naive float32
 };
 
 constexpr int RED_STRIDE = ACC_USED + 1;  // LDS row stride (odd, so column reads do not share banks)
-constexpr int RED_BYTES = ICP_BLOCK * RED_STRIDE * 8;
+constexpr int RED_ROWS = ICP_BLOCK / 2;   // one row per point of a lane group: 64 / S with S >= 2 lanes a point
+constexpr int RED_BYTES = RED_ROWS * RED_STRIDE * 8;
 
-// Each lane owns one LDS row of 53 running sums (and compensations for the compensated form): a point's
-// terms are added to its first lane's row as soon as they are formed, so no accumulator stays in registers
-// across the neighbour search.
-template <int MODE>
+// Each point of a lane group owns one LDS row of 53 running sums (and compensations for the compensated
+// form): row lane / S, written by the point's first lane (sub-lane 0), to which a point's terms are added as
+// soon as they are formed, so no accumulator stays in registers across the neighbour search.
+template <int MODE, int S>
 __device__ inline void row_zero(unsigned char *lds) {
     const int lane = lane_id();
+    if (lane >= ICP_BLOCK / S) return;
     if constexpr (MODE == 0) {
         double *red = reinterpret_cast<double *>(lds);
         LW_UNROLL for (int k = 0; k < ACC_USED; ++k) red[lane * RED_STRIDE + k] = 0.0;
@@ -91,35 +93,36 @@ __device__ inline void row_zero(unsigned char *lds) {
         float *rs = reinterpret_cast<float *>(lds);
         LW_UNROLL for (int k = 0; k < ACC_USED; ++k) rs[lane * RED_STRIDE + k] = 0.0f;
         if constexpr (MODE == 1) {
-            float *rc = rs + ICP_BLOCK * RED_STRIDE;
+            float *rc = rs + RED_ROWS * RED_STRIDE;
             LW_UNROLL for (int k = 0; k < ACC_USED; ++k) rc[lane * RED_STRIDE + k] = 0.0f;
         }
     }
 }
 
-template <int MODE>
+// Called by a point's first lane: the point's terms into its row.
+template <int MODE, int S>
 __device__ inline void row_add(unsigned char *lds, const Acc<MODE> &acc) {
-    const int lane = lane_id();
+    const int row = lane_id() / S;
     if constexpr (MODE == 0) {
         double *red = reinterpret_cast<double *>(lds);
-        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) red[lane * RED_STRIDE + k] += acc.s[k];
+        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) red[row * RED_STRIDE + k] += acc.s[k];
     } else if constexpr (MODE == 2) {
         float *rs = reinterpret_cast<float *>(lds);
-        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) rs[lane * RED_STRIDE + k] += acc.s[k];
+        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) rs[row * RED_STRIDE + k] += acc.s[k];
     } else {
         float *rs = reinterpret_cast<float *>(lds);
-        float *rc = rs + ICP_BLOCK * RED_STRIDE;
+        float *rc = rs + RED_ROWS * RED_STRIDE;
         LW_UNROLL for (int k = 0; k < ACC_USED; ++k) {
             float t, e;
-            two_sum(rs[lane * RED_STRIDE + k], acc.s[k], t, e);
-            rs[lane * RED_STRIDE + k] = t;
-            rc[lane * RED_STRIDE + k] += acc.c[k] + e;
+            two_sum(rs[row * RED_STRIDE + k], acc.s[k], t, e);
+            rs[row * RED_STRIDE + k] = t;
+            rc[row * RED_STRIDE + k] += acc.c[k] + e;
         }
     }
 }
 
-// Sum the wave's rows per entry, in lane order. Lane k < 53 returns entry k (sum, and compensation for the
-// compensated form).
+// Sum the wave's rows per entry, in row (= lane) order. Lane k < 53 returns entry k (sum, and compensation for
+// the compensated form).
 template <int MODE, int S>
 __device__ inline void wave_sum(unsigned char *lds, double &sum, double &comp) {
     const int lane = lane_id();
@@ -130,22 +133,22 @@ __device__ inline void wave_sum(unsigned char *lds, double &sum, double &comp) {
         double *red = reinterpret_cast<double *>(lds);
         if (lane < ACC_USED) {
             double t = 0.0;
-            for (int r = 0; r < ICP_BLOCK; r += S) t += red[r * RED_STRIDE + lane];
+            for (int r = 0; r < ICP_BLOCK / S; ++r) t += red[r * RED_STRIDE + lane];
             sum = t;
         }
     } else if constexpr (MODE == 2) {
         float *red = reinterpret_cast<float *>(lds);
         if (lane < ACC_USED) {
             float t = 0.0f;
-            for (int r = 0; r < ICP_BLOCK; r += S) t += red[r * RED_STRIDE + lane];
+            for (int r = 0; r < ICP_BLOCK / S; ++r) t += red[r * RED_STRIDE + lane];
             sum = (double)t;
         }
     } else {
         float *rs = reinterpret_cast<float *>(lds);
-        float *rc = rs + ICP_BLOCK * RED_STRIDE;
+        float *rc = rs + RED_ROWS * RED_STRIDE;
         if (lane < ACC_USED) {
             float s = 0.0f, c = 0.0f;
-            for (int r = 0; r < ICP_BLOCK; r += S) {
+            for (int r = 0; r < ICP_BLOCK / S; ++r) {
                 float t, e;
                 two_sum(s, rs[r * RED_STRIDE + lane], t, e);
                 s = t;
@@ -219,6 +222,7 @@ template <>
 struct Split<2> {
     static constexpr int S = 4, C = 16;
 };
+static_assert(Split<1>::S >= 2 && Split<2>::S >= 2, "at most RED_ROWS points a lane group");
 
 // The reference's terms of one matched point (odom.py:335-375) into acc.
 template <int MODE>
@@ -281,7 +285,7 @@ __device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf
     constexpr int D = 2 * R + 1, K = D * D * D, KQ = (K + S - 1) / S;
     const int lane = lane_id();
     const int sub = lane % S;
-    row_zero<MODE>(lds);
+    row_zero<MODE, S>(lds);
     for (int64_t base = (int64_t)g * ICP_BLOCK; base < (int64_t)p.n * S; base += (int64_t)G * ICP_BLOCK) {
         const int32_t i = (int32_t)((base + lane) / S);
         float px = 0.0f, py = 0.0f, pz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
@@ -320,7 +324,7 @@ __device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf
                 Acc<MODE> acc;
                 acc.zero();
                 point_terms<MODE>(a, p.kappa, px, py, pz, wx, wy, wz, best, acc);
-                row_add<MODE>(lds, acc);
+                row_add<MODE, S>(lds, acc);
             }
         }
         if (stamp) t_terms = wall_clock64();
@@ -998,7 +1002,7 @@ __global__ __launch_bounds__(64 * SEQ_WAVES) void k_register_seq(const SeqReg *D
     __shared__ float Tf[16];
     __shared__ double sT[16], sTp[16], vs[64], xi[6], rows[SEQ_WAVES][2][64];
     __shared__ int32_t s_flag;
-    const SeqReg d = D[blockIdx.x];
+    const SeqReg d = rebased(D, D[blockIdx.x]);
     if (gated_off(d.gate)) return;  // the state stays as the frame preparation left it
     const int lane = lane_id(), wave = (int)(threadIdx.x >> 6);
     RegState *st = d.st;

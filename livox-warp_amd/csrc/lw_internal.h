@@ -101,6 +101,32 @@ struct KeySeq {
     int32_t *counts;
 };
 
+__device__ inline PrepArgs rebased(const void *b, PrepArgs A) {
+    A.raw = via_arg(b, A.raw);
+    A.t = via_arg(b, A.t);
+    A.attr = via_arg(b, A.attr);
+    A.xyz = via_arg(b, A.xyz);
+    A.m_dev = via_arg(b, A.m_dev);
+    A.ds_keys = via_arg(b, A.ds_keys);
+    A.ds_idx = via_arg(b, A.ds_idx);
+    A.ds_xyz = via_arg(b, A.ds_xyz);
+    A.n_out = via_arg(b, A.n_out);
+    A.st = via_arg(b, A.st);
+    A.Tp = via_arg(b, A.Tp);
+    A.epoch = via_arg(b, A.epoch);
+    A.st1 = via_arg(b, A.st1);
+    A.fc = via_arg(b, A.fc);
+    return A;
+}
+
+__device__ inline KeySeq rebased(const void *b, KeySeq d) {
+    d.st = via_arg(b, d.st);
+    d.fc = via_arg(b, d.fc);
+    d.ctr = via_arg(b, d.ctr);
+    d.counts = via_arg(b, d.counts);
+    return d;
+}
+
 // The parameters the per-voxel precision is computed with (odom.py:336-368).
 struct PrecParams {
     float voxel, noise2, tight2, alpha;
@@ -284,6 +310,25 @@ struct MapRefs {
     int32_t mask;
 };
 MapRefs map_refs(lw_ctx *c);
+// Every pointer of a descriptor re-derived from the descriptor array b (a kernel argument) with via_arg.
+__device__ inline MapRefs rebased(const void *b, MapRefs M) {
+    M.slots = via_arg(b, M.slots);
+    M.occ = via_arg(b, M.occ);
+    M.vox_key = via_arg(b, M.vox_key);
+    M.vox_slot = via_arg(b, M.vox_slot);
+    M.vox_mom = via_arg(b, M.vox_mom);
+    M.vox_prec = via_arg(b, M.vox_prec);
+    M.ctr = via_arg(b, M.ctr);
+    M.pt_slot = via_arg(b, M.pt_slot);
+    M.pt_vid = via_arg(b, M.pt_vid);
+    M.pt_take = via_arg(b, M.pt_take);
+    M.vox_nin = via_arg(b, M.vox_nin);
+    M.touched = via_arg(b, M.touched);
+    M.vox_quota = via_arg(b, M.vox_quota);
+    M.vox_dirty = via_arg(b, M.vox_dirty);
+    M.dirty_list = via_arg(b, M.dirty_list);
+    return M;
+}
 // One sequence's insert in a batch: its map, its points (n, or the count at n_dev capped at n), its float32
 // pose on the device, a gate that must be nonzero, and where the map's counts go once the claims are done.
 struct InsSeq {
@@ -296,6 +341,16 @@ struct InsSeq {
     const int32_t *gate;
     int32_t *counts;
 };
+__device__ inline InsSeq rebased(const void *b, InsSeq d) {
+    d.M = rebased(b, d.M);
+    d.xyz = via_arg(b, d.xyz);
+    d.attr = via_arg(b, d.attr);
+    d.n_dev = via_arg(b, d.n_dev);
+    d.T_dev = via_arg(b, d.T_dev);
+    d.gate = via_arg(b, d.gate);
+    d.counts = via_arg(b, d.counts);
+    return d;
+}
 // Every sequence's insert (D: S descriptors on the device), touched voxels refreshed with q at once; n_bound
 // bounds every sequence's points.
 hipError_t launch_insert_seq(hipStream_t s, const InsSeq *D, int32_t S, int32_t n_bound, float inv_v, float voxel,
@@ -354,6 +409,16 @@ struct SeqReg {
     int32_t mask;
     int32_t pad;
 };
+__device__ inline SeqReg rebased(const void *b, SeqReg d) {
+    d.ds = via_arg(b, d.ds);
+    d.n_dev = via_arg(b, d.n_dev);
+    d.slots = via_arg(b, d.slots);
+    d.occ = via_arg(b, d.occ);
+    d.vox_prec = via_arg(b, d.vox_prec);
+    d.st = via_arg(b, d.st);
+    d.gate = via_arg(b, d.gate);
+    return d;
+}
 // S registrations (D: S descriptors on the device), one workgroup each, every Gauss-Newton pass and step inside
 // the workgroup. prof: the measurement counters, or null.
 hipError_t launch_register_seq(hipStream_t s, const SeqReg *D, int32_t S, const RegLoop &L, int acc_mode,

@@ -313,34 +313,34 @@ __global__ __launch_bounds__(BLOCK) void k_ins_finish(MapRefs M, const int32_t *
 // The batched forms: one insert per sequence (blockIdx.y), each with its own map, points, pose and gate.
 __global__ __launch_bounds__(BLOCK) void k_ins_claim_seq(const InsSeq *D, float inv_v, float max_range,
                                                          int32_t ret_mask) {
-    const InsSeq d = D[blockIdx.y];
+    const InsSeq d = rebased(D, D[blockIdx.y]);
     if (gated_off(d.gate)) return;
     claim_blocks(d.M, d.xyz, d.attr, count_of(d.n, d.n_dev), load_mat4f(d.T_dev), inv_v, max_range, ret_mask,
                  blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_ins_count_seq(const InsSeq *D) {
-    const InsSeq d = D[blockIdx.y];
+    const InsSeq d = rebased(D, D[blockIdx.y]);
     if (gated_off(d.gate)) return;
     count_blocks(d.M, count_of(d.n, d.n_dev), d.counts, blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_ins_quota_seq(const InsSeq *D) {
-    const InsSeq d = D[blockIdx.y];
+    const InsSeq d = rebased(D, D[blockIdx.y]);
     if (gated_off(d.gate)) return;
     quota_blocks(d.M, count_of(d.n, d.n_dev), blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_ins_accum_seq(const InsSeq *D, float inv_v, float voxel, float max_range,
                                                          int32_t ret_mask) {
-    const InsSeq d = D[blockIdx.y];
+    const InsSeq d = rebased(D, D[blockIdx.y]);
     if (gated_off(d.gate)) return;
     accum_blocks(d.M, d.xyz, d.attr, count_of(d.n, d.n_dev), load_mat4f(d.T_dev), inv_v, voxel, max_range, ret_mask,
                  blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_ins_finish_seq(const InsSeq *D, PrecParams q) {
-    const InsSeq d = D[blockIdx.y];
+    const InsSeq d = rebased(D, D[blockIdx.y]);
     if (gated_off(d.gate)) return;
     finish_blocks(d.M, 1, q, blockIdx.x, gridDim.x);
 }

@@ -159,7 +159,7 @@ __global__ void k_keyframe_check(const RegState *st, FrameCtl *fc, const int32_t
 __global__ void k_keyframe_check_seq(const KeySeq *D, int32_t S) {
     const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
-    const KeySeq d = D[s];
+    const KeySeq d = rebased(D, D[s]);
     keyframe_decide(d.st, d.fc, d.ctr, d.counts);
 }
 
@@ -376,7 +376,7 @@ __global__ __launch_bounds__(PREP_THREADS) void k_frame_prep(PrepArgs A) { frame
 
 // One frame per workgroup: frame blockIdx.x of a batch.
 __global__ __launch_bounds__(PREP_THREADS) void k_frame_prep_seq(const PrepArgs *D) {
-    const PrepArgs A = D[blockIdx.x];
+    const PrepArgs A = rebased(D, D[blockIdx.x]);
     frame_prep(A);
 }
 
