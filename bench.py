@@ -951,7 +951,7 @@ def odometry_headline(args, torch, lw, dev, odo, X, A, Tt, spans, seq, n_frames:
         odo.push(X[k], A[k], Tt[k], per, spans[k][0], spans[k][1], _no_frame)
 
     # the warm-up passes run with the device phase stamps on (where a pass's time goes)
-    phases = np.zeros(22)
+    phases = np.zeros(24)
     if args.warmup > 0:
         lw._abi.check(L.lw_prof_enable(odo._ctx, 2), "lw_prof_enable")
         lw._abi.check(L.lw_prof_reset(odo._ctx), "lw_prof_reset")
@@ -1021,6 +1021,9 @@ def odometry_headline(args, torch, lw, dev, odo, X, A, Tt, spans, seq, n_frames:
                                          "searcher_neighbours": round(phases[15] / max(phases[7], 1), 2),
                                          "searcher_terms": round(phases[16] / max(phases[7], 1), 2),
                                          "searcher_row_sum": round(phases[17] / max(phases[7], 1), 2),
+                                         "searcher_neighbours_narrow": round(phases[22] / max(phases[23], 1), 2),
+                                         "searcher_neighbours_wide": round((phases[15] - phases[22]) /
+                                                                           max(phases[7] - phases[23], 1), 2),
                                          "all_searchers_wake": round(phases[18] / max(phases[20], 1), 2),
                                          "all_searchers_flag": round(phases[19] / max(phases[20], 1), 2),
                                          "slowest_searcher_row": round(phases[21], 2)},

@@ -301,8 +301,9 @@ int lw_prof_read(lw_ctx *ctx, double *out);
  * search and terms (out[5]) in parts: out[15] point to closest means (neighbour search), out[16] the terms
  * into its LDS rows, out[17] the sum of its rows; counted with out[7]. Every searching workgroup's passes
  * after the first: out[18] from the pose's publication to its wake-up, out[19] to its row flag; out[20] the
- * searcher-passes timed; out[21] the slowest searcher's row per pass, on average, from the publication.
- * out holds 22 doubles. No counterpart in the reference. */
+ * searcher-passes timed; out[21] the slowest searcher's row per pass, on average, from the publication;
+ * out[22] the first searcher's neighbour phase summed over the narrow (27-voxel) passes alone, out[23] those
+ * passes. out holds 24 doubles. No counterpart in the reference. */
 int lw_prof_phases(lw_ctx *ctx, double *out);
 
 /* ---- the integration map (the reference's Pipeline map, python/livox_warp/gpu.py) ----------------------

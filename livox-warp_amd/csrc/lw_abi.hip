@@ -864,6 +864,8 @@ int lw_prof_phases(lw_ctx *c, double *out) {
     for (int i = 0; i < 2; ++i) out[18 + i] = (double)v[24 + i] * us_per_tick;
     out[20] = (double)v[26];
     out[21] = v[29] ? (double)v[28] * us_per_tick / (double)v[29] : 0.0;
+    out[22] = (double)v[30] * us_per_tick;
+    out[23] = (double)v[31];
     return 0;
 }
 

@@ -224,13 +224,11 @@ struct Split<2> {
 };
 static_assert(Split<1>::S >= 2 && Split<2>::S >= 2, "at most RED_ROWS points a lane group");
 
-// The reference's terms of one matched point (odom.py:335-375) into acc.
+// The reference's terms of one matched point (odom.py:335-375) into acc; P: the matched voxel's precision.
 template <int MODE>
-__device__ inline void point_terms(const IcpArgs &a, float kappa, float px, float py, float pz, float wx, float wy,
+__device__ inline void point_terms(const VoxPrec &P, float kappa, float px, float py, float pz, float wx, float wy,
                                    float wz, const Cand &b, Acc<MODE> &acc) {
-    const int32_t vid = b.vid;
     const float r0 = wx - b.mx, r1 = wy - b.my, r2 = wz - b.mz;
-    const VoxPrec P = a.vox_prec[vid];
     const float M[3][3] = {{P.M[0], P.M[1], P.M[2]}, {P.M[1], P.M[3], P.M[4]}, {P.M[2], P.M[4], P.M[5]}};
     const float G[3][3] = {{P.Mg[0], P.Mg[1], P.Mg[2]}, {P.Mg[1], P.Mg[3], P.Mg[4]}, {P.Mg[2], P.Mg[4], P.Mg[5]}};
     float rn;
@@ -271,27 +269,141 @@ __device__ inline void point_terms(const IcpArgs &a, float kappa, float px, floa
     acc.add(ACC_RANGE, length3(px, py, pz));
 }
 
+// ---- the persistent loop's neighbourhood record --------------------------------------------------------
+//
+// Within a registration the map does not change. In the persistent loop a searcher keeps the points of its
+// first lane group in registers from pass to pass, and both reaches give a point four lanes (Split<2>::S, and
+// the loop's narrow split below), so the searcher serves the same points in every pass. Each wide pass (the
+// 5x5x5 search) records, for every key of each point's 125, what the search found there -- the mean and the
+// voxel id when the key holds a positive count -- at the key's position in the 5x5x5 order, and the point's
+// voxel beside it. A narrow pass whose point lies within one voxel of that centre on every axis has its 27
+// neighbours inside the recorded 125: each lane reads the entries of its part of the 27 at their positions,
+// all in flight together, and takes the closest with the search's float32 distance and (distance, position)
+// rule, so the closest mean and the hit count are the search's; only the round trips to the map go. A point
+// further away (or a pass with no record) searches. The loop writes no per-point slots, so a taken entry marks
+// its match with the voxel id in place of the slot (any non-negative value marks a match). Each point's first
+// lane also keeps the precision record of the voxel it matched last, for the terms.
+template <int R>
+struct LoopSplit;  // the persistent loop's lanes per point and chunk size: four lanes at either reach
+template <>
+struct LoopSplit<1> {
+    static constexpr int S = 4, C = 7;  // 27 as 7 + 7 + 7 + 6
+};
+template <>
+struct LoopSplit<2> {
+    static constexpr int S = Split<2>::S, C = Split<2>::C;
+};
+static_assert(LoopSplit<1>::S == LoopSplit<2>::S, "both reaches serve the same points");
+constexpr int LS = LoopSplit<2>::S;
+constexpr int CROWS = ICP_BLOCK / LS;  // points of a lane group
+constexpr int REC_NONE = -1;           // entry w: not in the map
+constexpr int REC_EMPTY = -2;          // entry w: in the map with no positive count (a hit, never a match)
+struct NbrCache {
+    float4 e[CROWS][128];         // [point][position in the 5x5x5 order]: mean x, y, z; w (voxel id or REC_*)
+    int32_t cx[CROWS], cy[CROWS], cz[CROWS];  // the voxel the record was made around
+    int32_t valid[CROWS];         // 1 once a wide pass has recorded the point
+    float4 P[4][CROWS];           // a point's last matched precision record (VoxPrec, 64 B)
+    int32_t pvid[CROWS];          // its voxel id; -1: none
+};
+
+// Every lane: no record, no precision record (a new registration).
+__device__ inline void cache_clear(NbrCache *c) {
+    const int lane = lane_id();
+    if (lane < CROWS) {
+        c->valid[lane] = 0;
+        c->pvid[lane] = -1;
+    }
+}
+
+// The wide search's recorder for the point of lane-group row `row`: every key at its position, what the search
+// found there (a key not found at its home is written as absent, and again if a probe finds it).
+struct CacheRec {
+    NbrCache *c;
+    int row;
+    __device__ void add(bool ok, int32_t, int32_t j, const int4 &lo, const float4 &hi) const {
+        const int w = ok ? (__int_as_float(lo.w) > 0.0f ? lo.z : REC_EMPTY) : REC_NONE;
+        c->e[row][j] = make_float4(hi.x, hi.y, hi.z, __int_as_float(w));
+    }
+};
+
+// This lane's part of a wide pass: the search, recording every key of its part; the point's voxel beside it.
+template <int C>
+__device__ inline void search_recorded(NbrCache *c, const Table &t, const NbrOff *tab, float inv_v, float wx,
+                                       float wy, float wz, int j_lo, int j_hi, Cand &best, int32_t &hits) {
+    const int lane = lane_id(), row = lane / LS;
+    search_part<2, C, false, CacheRec>(t, tab, inv_v, wx, wy, wz, j_lo, j_hi, best, hits, CacheRec{c, row});
+    if (lane % LS == 0) {
+        c->cx[row] = ifloor(wx * inv_v);
+        c->cy[row] = ifloor(wy * inv_v);
+        c->cz[row] = ifloor(wz * inv_v);
+        c->valid[row] = 1;
+    }
+}
+
+// This lane's part [j_lo, j_hi) of a narrow pass (at most C keys): from the record when the point's 27
+// neighbours lie inside the recorded 125, otherwise by the search.
+template <int C>
+__device__ inline void search_narrow(const NbrCache *c, const Table &t, const NbrOff *tab, float inv_v, float wx,
+                                     float wy, float wz, int j_lo, int j_hi, Cand &best, int32_t &hits) {
+    const int row = lane_id() / LS;
+    const int32_t dx = ifloor(wx * inv_v) - c->cx[row], dy = ifloor(wy * inv_v) - c->cy[row],
+                  dz = ifloor(wz * inv_v) - c->cz[row];
+    if (c->valid[row] && dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) {
+        float4 m[C];
+        LW_UNROLL for (int k = 0; k < C; ++k) {
+            const int j = min(j_lo + k, 26);  // a padded key reads a valid entry and is ignored below
+            const int ox = j / 9 - 1 + dx, oy = (j / 3) % 3 - 1 + dy, oz = j % 3 - 1 + dz;
+            m[k] = c->e[row][(ox + 2) * 25 + (oy + 2) * 5 + (oz + 2)];
+        }
+        LW_UNROLL for (int k = 0; k < C; ++k) {
+            const int w = __float_as_int(m[k].w);
+            const bool in = j_lo + k < j_hi;
+            hits += in && w != REC_NONE ? 1 : 0;
+            take_if_closer(in && w >= 0, dist2(wx, wy, wz, m[k].x, m[k].y, m[k].z), w, j_lo + k, m[k].x, m[k].y,
+                           m[k].z, w, best);
+        }
+        return;
+    }
+    search_part<1, C, false>(t, tab, inv_v, wx, wy, wz, j_lo, j_hi, best, hits);
+}
+
+// The precision record of voxel vid for the point of lane-group row `row`: the kept one when it is that voxel's.
+__device__ inline VoxPrec cached_prec(NbrCache *c, const VoxPrec *vox_prec, int row, int32_t vid) {
+    VoxPrec P;
+    float4 *q = reinterpret_cast<float4 *>(&P);
+    if (c->pvid[row] == vid) {
+        LW_UNROLL for (int k = 0; k < 4; ++k) q[k] = c->P[k][row];
+    } else {
+        P = vox_prec[vid];
+        LW_UNROLL for (int k = 0; k < 4; ++k) c->P[k][row] = q[k];
+        c->pvid[row] = vid;
+    }
+    return P;
+}
+
 // One workgroup's share of a pass: the points of lane groups g, g + G, ... (one group per workgroup whenever
 // the grid is large enough). Lane k < 53 returns the workgroup's entry k. pre: when has_pre, this lane's point
-// of group g, already loaded (the persistent loop keeps it in registers from pass to pass). With stamp set,
-// t_found and t_terms receive the device clock after the last lane group's neighbour search and after its
-// terms (measurement).
-template <int MODE, int R>
+// of group g, already loaded (the persistent loop keeps it in registers from pass to pass). cache: when given
+// (reach 1 only), group g's candidate lists. With stamp set, t_found and t_terms receive the device clock after
+// the last lane group's neighbour search and after its terms (measurement).
+template <int MODE, int R, int S = Split<R>::S, int C = Split<R>::C>
 __device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf, const NbrOff *nbr, int g, int G,
                                  unsigned char *lds, double &sum, double &comp, int32_t &hits, int32_t &matched, float3 pre,
                                  bool has_pre, bool stamp, unsigned long long &t_found,
-                                 unsigned long long &t_terms) {
-    constexpr int S = Split<R>::S, C = Split<R>::C;
+                                 unsigned long long &t_terms, NbrCache *cache = nullptr) {
+    static_assert(S >= 2 && S <= 64 && 64 % S == 0, "whole lane groups of at most RED_ROWS points");
     constexpr int D = 2 * R + 1, K = D * D * D, KQ = (K + S - 1) / S;
     const int lane = lane_id();
     const int sub = lane % S;
     row_zero<MODE, S>(lds);
     for (int64_t base = (int64_t)g * ICP_BLOCK; base < (int64_t)p.n * S; base += (int64_t)G * ICP_BLOCK) {
         const int32_t i = (int32_t)((base + lane) / S);
+        const bool first = base == (int64_t)g * ICP_BLOCK;
+        const bool cached = S == LS && cache != nullptr && has_pre && first;
         float px = 0.0f, py = 0.0f, pz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
         Cand best{p.max_d2, -1, K, 0.0f, 0.0f, 0.0f};
         if (i < p.n) {
-            if (has_pre && base == (int64_t)g * ICP_BLOCK) {
+            if (has_pre && first) {
                 px = pre.x;
                 py = pre.y;
                 pz = pre.z;
@@ -301,8 +413,14 @@ __device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf
                 pz = a.ds[3 * (int64_t)i + 2];
             }
             transform_point(Tf, px, py, pz, wx, wy, wz);
-            search_part<R, C, false>(Table{a.slots, a.occ, a.mask}, nbr, a.inv_v, wx, wy, wz, sub * KQ, min(K, (sub + 1) * KQ),
-                               best, hits);
+            const Table t{a.slots, a.occ, a.mask};
+            const int j_lo = sub * KQ, j_hi = min(K, (sub + 1) * KQ);
+            if (!cached)
+                search_part<R, C, false>(t, nbr, a.inv_v, wx, wy, wz, j_lo, j_hi, best, hits);
+            else if constexpr (R == 2)
+                search_recorded<C>(cache, t, nbr, a.inv_v, wx, wy, wz, j_lo, j_hi, best, hits);
+            else
+                search_narrow<C>(cache, t, nbr, a.inv_v, wx, wy, wz, j_lo, j_hi, best, hits);
         }
         if (stamp) t_found = wall_clock64();
         // merge the S parts: smaller distance wins, a tie goes to the earlier neighbour (the serial strict <)
@@ -323,7 +441,8 @@ __device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf
                 ++matched;
                 Acc<MODE> acc;
                 acc.zero();
-                point_terms<MODE>(a, p.kappa, px, py, pz, wx, wy, wz, best, acc);
+                const VoxPrec P = cached ? cached_prec(cache, a.vox_prec, lane / S, best.vid) : a.vox_prec[best.vid];
+                point_terms<MODE>(P, p.kappa, px, py, pz, wx, wy, wz, best, acc);
                 row_add<MODE, S>(lds, acc);
             }
         }
@@ -811,7 +930,7 @@ __device__ inline void leader(const LoopArgs &L, int32_t n, uint32_t epoch, unsi
     __syncthreads();
     for (int it = 0; it < L.max_iter; ++it) {
         const int reach = it < L.wide_iters ? 2 : 1;
-        const int G = searchers_for(n, reach == 2 ? Split<2>::S : Split<1>::S);
+        const int G = searchers_for(n, LS);
         // the twist to the prediction, while the pass's search runs (odom.py:775-781)
         if (lane == 0 && a.sp.prior_info > 0.0) gn::twist_between(sT, sTp, xi);
         double v = 0.0;
@@ -876,13 +995,15 @@ __device__ inline void searcher(const LoopArgs &L, int32_t n, uint32_t epoch, un
     const int g = (int)blockIdx.x - 1;
     __shared__ float Tf[16];
     __shared__ NbrOff nbr[NBR_ROWS];
+    __shared__ NbrCache cache;
     if (lane < 16) Tf[lane] = st->Tf[lane];
     nbr_fill(nbr);
-    // this lane's point in its first lane group, for either reach: the same every pass
-    float3 pt[2] = {};
-    LW_UNROLL for (int k = 0; k < 2; ++k) {
-        const int64_t i = ((int64_t)g * ICP_BLOCK + lane) / (k == 0 ? Split<2>::S : Split<1>::S);
-        if (i < n) pt[k] = make_float3(a.ds[3 * i], a.ds[3 * i + 1], a.ds[3 * i + 2]);
+    cache_clear(&cache);
+    // this lane's point in its first lane group: the same every pass, at either reach
+    float3 pt = {};
+    {
+        const int64_t i = ((int64_t)g * ICP_BLOCK + lane) / LS;
+        if (i < n) pt = make_float3(a.ds[3 * i], a.ds[3 * i + 1], a.ds[3 * i + 2]);
     }
     __syncthreads();
     const bool stamp = a.phase && g == 0;  // the first searcher's view of the passes
@@ -893,17 +1014,17 @@ __device__ inline void searcher(const LoopArgs &L, int32_t n, uint32_t epoch, un
     for (int it = 0; it < L.max_iter; ++it) {
         const int reach = it < L.wide_iters ? 2 : 1;
         const Pass p{n, reach == 2 ? L.max_d2_r2 : L.max_d2_r1, st->kappa[it], it};
-        const int G = searchers_for(n, reach == 2 ? Split<2>::S : Split<1>::S);
+        const int G = searchers_for(n, LS);
         if (g < G) {
             double sum, comp;
             int32_t hits = 0, matched = 0;
             unsigned long long t_found = 0ull, t_terms = 0ull;
             if (reach == 2)
-                wave_pass<MODE, 2>(a, p, Tf, nbr, g, G, lds, sum, comp, hits, matched, pt[0], true, stamp, t_found,
-                                   t_terms);
+                wave_pass<MODE, 2, LoopSplit<2>::S, LoopSplit<2>::C>(a, p, Tf, nbr, g, G, lds, sum, comp, hits, matched,
+                                                                     pt, true, stamp, t_found, t_terms, &cache);
             else
-                wave_pass<MODE, 1>(a, p, Tf, nbr, g, G, lds, sum, comp, hits, matched, pt[1], true, stamp, t_found,
-                                   t_terms);
+                wave_pass<MODE, 1, LoopSplit<1>::S, LoopSplit<1>::C>(a, p, Tf, nbr, g, G, lds, sum, comp, hits, matched,
+                                                                     pt, true, stamp, t_found, t_terms, &cache);
             const unsigned long long t_summed = stamp ? wall_clock64() : 0ull;
             if (a.phase && it > 0 && lane == 0)  // the pass's latest row so far
                 atomicMax(&a.phase[27], wall_clock64() - t_pub);
@@ -926,6 +1047,10 @@ __device__ inline void searcher(const LoopArgs &L, int32_t n, uint32_t epoch, un
                 atomicAdd(&a.phase[21], t_found - t_wake);
                 atomicAdd(&a.phase[22], t_terms - t_found);
                 atomicAdd(&a.phase[23], t_summed - t_terms);
+                if (reach == 1) {  // the narrow passes' neighbour phase on its own
+                    atomicAdd(&a.phase[30], t_found - t_wake);
+                    atomicAdd(&a.phase[31], 1ull);
+                }
             }
         }
         // wait for the leader's pose of pass it + 1: 16 tagged words, each read with its own tag

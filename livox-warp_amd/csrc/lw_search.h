@@ -57,25 +57,43 @@ __device__ inline int64_t key_of(const int4 &lo) {
 
 __device__ inline bool bit_of(uint32_t word, int32_t s) { return ((word >> (s & 31)) & 1u) != 0u; }
 
-// One candidate, without branches: taken when ok (the key was found), its count is positive (odom.py:324) and
-// it is closer, or as close and earlier in the order, than the best so far (the serial loop starts from
-// best_d2 = max_d2 and takes strictly smaller distances, odom.py:326-330).
-__device__ inline void consider(bool ok, float wx, float wy, float wz, int32_t s, int32_t j, const int4 &lo,
-                                const float4 &hi, Cand &best) {
-    const float d0 = wx - hi.x, d1 = wy - hi.y, d2 = wz - hi.z;
+// The squared distance from the point to a mean, in the reference's order of float32 operations.
+__device__ inline float dist2(float wx, float wy, float wz, float mx, float my, float mz) {
+    const float d0 = wx - mx, d1 = wy - my, d2 = wz - mz;
     float dd = d0 * d0;
     dd = dd + d1 * d1;
     dd = dd + d2 * d2;
+    return dd;
+}
+
+// A candidate of positive count (odom.py:324) at squared distance dd is taken when it is closer, or as close
+// and earlier in the order, than the best so far (the serial loop starts from best_d2 = max_d2 and takes
+// strictly smaller distances, odom.py:326-330). Without branches.
+__device__ inline void take_if_closer(bool ok, float dd, int32_t s, int32_t j, float mx, float my, float mz,
+                                      int32_t vid, Cand &best) {
     const bool closer = dd < best.d2 || (best.s >= 0 && dd == best.d2 && j < best.j);
-    const bool take = ok && __int_as_float(lo.w) > 0.0f && closer;
+    const bool take = ok && closer;
     best.d2 = take ? dd : best.d2;
     best.s = take ? s : best.s;
     best.j = take ? j : best.j;
-    best.mx = take ? hi.x : best.mx;
-    best.my = take ? hi.y : best.my;
-    best.mz = take ? hi.z : best.mz;
-    best.vid = take ? lo.z : best.vid;
+    best.mx = take ? mx : best.mx;
+    best.my = take ? my : best.my;
+    best.mz = take ? mz : best.mz;
+    best.vid = take ? vid : best.vid;
 }
+
+// One candidate record: taken when ok (the key was found), its count is positive and it is the closer one.
+__device__ inline void consider(bool ok, float wx, float wy, float wz, int32_t s, int32_t j, const int4 &lo,
+                                const float4 &hi, Cand &best) {
+    take_if_closer(ok && __int_as_float(lo.w) > 0.0f, dist2(wx, wy, wz, hi.x, hi.y, hi.z), s, j, hi.x, hi.y, hi.z,
+                   lo.z, best);
+}
+
+// What a search can hand to a recorder: every key of round 0, found (ok) or not, and then each key a probe finds,
+// as slot s at position j with its record.
+struct NoRec {
+    __device__ void add(bool, int32_t, int32_t, const int4 &, const float4 &) {}
+};
 
 // Key and hash-product offsets of the neighbours of either reach (dx outer, dz inner), in LDS: rows
 // [0, NBR_ROWS_1) for reach 1, then [NBR_ROWS_1, NBR_ROWS_1 + NBR_ROWS_2) for reach 2; past (2R+1)^3 the rows
@@ -99,9 +117,9 @@ __device__ inline void nbr_fill(NbrOff *tab) {
 
 // The closest mean over neighbours [j_lo, j_hi) of the (2R+1)^3 block around the point's voxel (dx outer, dz
 // inner), in chunks of C. hits counts the occupied neighbours found. tab: the LDS offset table (nbr_fill).
-template <int R, int C, bool BITMAP>
+template <int R, int C, bool BITMAP, class Rec = NoRec>
 __device__ inline void search_part(const Table &t, const NbrOff *tab, float inv_v, float wx, float wy, float wz,
-                                   int j_lo, int j_hi, Cand &best, int32_t &hits) {
+                                   int j_lo, int j_hi, Cand &best, int32_t &hits, Rec rec = Rec{}) {
     constexpr int D = 2 * R + 1;
     constexpr int ROW0 = R == 1 ? 0 : NBR_ROWS_1;
     static_assert(C <= 32, "one bit per key of a chunk");
@@ -153,6 +171,7 @@ __device__ inline void search_part(const Table &t, const NbrOff *tab, float inv_
                 need |= (in && !same && dmax[c] > 0 ? 1u : 0u) << c;
                 hits += in && same ? 1 : 0;
                 consider(in && same, wx, wy, wz, h[c], j0 + c, lo[c], hi[c], best);
+                rec.add(in && same, h[c], j0 + c, lo[c], hi[c]);
             }
         }
         // the displaced keys: every unresolved key of the chunk one slot further per round
@@ -165,6 +184,7 @@ __device__ inline void search_part(const Table &t, const NbrOff *tab, float inv_
                 const bool same = key_of(lo[c]) == key[c];
                 hits += in && same ? 1 : 0;
                 consider(in && same, wx, wy, wz, (h[c] + probe) & t.mask, j0 + c, lo[c], hi[c], best);
+                if (in && same) rec.add(true, (h[c] + probe) & t.mask, j0 + c, lo[c], hi[c]);
                 // found, or past the bound: not in the map
                 need &= ~((in && (same || probe >= dmax[c]) ? 1u : 0u) << c);
             }
