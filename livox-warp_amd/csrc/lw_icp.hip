@@ -1,15 +1,15 @@
 // Scan-to-map accumulation and the Gauss-Newton loop on the device: k_icp + k_reduce
 // (python/livox_warp/odom.py:275-384) and the host step of Odometry._register (odom.py:752-803).
 //
-// One pass: each thinned point gets S lanes (S = 2 for the 27-voxel search, 4 for the 125-voxel one); the
-// lanes split the point's neighbours, in the reference's order (dx outer, dz inner; odom.py:315-332), into
-// contiguous parts. A lane searches its part in chunks (lw_search.h): the bitmap words of a whole chunk are
-// loaded before any is tested, then the chunk's slots (32 B records that carry key, voxel id, count and
-// mean), and the keys whose home slot holds another key continue their probes together, so a chunk costs two
-// dependent memory round trips plus two per probe depth, and a lane one or two chunks. The lanes then merge
-// their closest means by (distance, position in the order), which is exactly the serial loop's choice. The
-// first lane of the point forms the 53 float32 terms of the ACC layout as the reference rounds them
-// (odom.py:246-259, 269-271, 372-375).
+// One pass: each thinned point gets S lanes (S = 2 for the 27-voxel search, 4 for the 125-voxel one; the
+// persistent loop gives four at both); the lanes split the point's neighbours, in the reference's order (dx
+// outer, dz inner; odom.py:315-332), into contiguous parts. A lane searches its part in chunks (lw_search.h):
+// the chunk's home slots (32 B records that carry key, voxel id, count and mean) are loaded together, and the
+// keys whose home slot holds another key continue their probes together, so a chunk costs one dependent
+// memory round trip plus one per probe depth, and a lane one or two chunks. The lanes then merge their
+// closest means by (distance, position in the order), which is exactly the serial loop's choice. The point's
+// first two lanes form the 53 float32 terms of the ACC layout as the reference rounds them (odom.py:246-259,
+// 269-271, 372-375).
 //
 // Reduction: the wave's terms are transposed through LDS and summed per entry in lane order; each
 // workgroup writes one partial row; the rows are summed in batches of 32 (each batch in row order, then the
@@ -40,48 +40,13 @@ __device__ inline void two_sum(float a, float b, float &s, float &e) {
     e = (a - (s - bb)) + (b - bb);
 }
 
-template <int MODE>
-struct Acc;
-
-template <>
-struct Acc<0> {  // float64
-    double s[ACC_USED];
-    __device__ void zero() {
-        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) s[k] = 0.0;
-    }
-    __device__ void add(int k, float x) { s[k] += (double)x; }
-};
-
-template <>
-struct Acc<1> {  // compensated float32
-    float s[ACC_USED], c[ACC_USED];
-    __device__ void zero() {
-        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) s[k] = c[k] = 0.0f;
-    }
-    __device__ void add(int k, float x) {
-        float t, e;
-        two_sum(s[k], x, t, e);
-        s[k] = t;
-        c[k] += e;
-    }
-};
-
-template <>
-struct Acc<2> {  // naive float32
-    float s[ACC_USED];
-    __device__ void zero() {
-        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) s[k] = 0.0f;
-    }
-    __device__ void add(int k, float x) { s[k] += x; }
-};
-
 constexpr int RED_STRIDE = ACC_USED + 1;  // LDS row stride (odd, so column reads do not share banks)
 constexpr int RED_ROWS = ICP_BLOCK / 2;   // one row per point of a lane group: 64 / S with S >= 2 lanes a point
 constexpr int RED_BYTES = RED_ROWS * RED_STRIDE * 8;
 
 // Each point of a lane group owns one LDS row of 53 running sums (and compensations for the compensated
-// form): row lane / S, written by the point's first lane (sub-lane 0), to which a point's terms are added as
-// soon as they are formed, so no accumulator stays in registers across the neighbour search.
+// form): row lane / S, written by the point's first two lanes, to which a point's terms are added as soon as
+// they are formed, so no accumulator stays in registers across the neighbour search.
 template <int MODE, int S>
 __device__ inline void row_zero(unsigned char *lds) {
     const int lane = lane_id();
@@ -99,25 +64,23 @@ __device__ inline void row_zero(unsigned char *lds) {
     }
 }
 
-// Called by a point's first lane: the point's terms into its row.
-template <int MODE, int S>
-__device__ inline void row_add(unsigned char *lds, const Acc<MODE> &acc) {
-    const int row = lane_id() / S;
+// One term x of a point into entry k of its row: a float64 sum of the float32 terms (the reference's
+// accumulation), or a compensated / naive float32 sum.
+template <int MODE>
+__device__ inline void row_put(unsigned char *lds, int row, int k, float x) {
     if constexpr (MODE == 0) {
         double *red = reinterpret_cast<double *>(lds);
-        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) red[row * RED_STRIDE + k] += acc.s[k];
+        red[row * RED_STRIDE + k] += (double)x;
     } else if constexpr (MODE == 2) {
         float *rs = reinterpret_cast<float *>(lds);
-        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) rs[row * RED_STRIDE + k] += acc.s[k];
+        rs[row * RED_STRIDE + k] += x;
     } else {
         float *rs = reinterpret_cast<float *>(lds);
         float *rc = rs + RED_ROWS * RED_STRIDE;
-        LW_UNROLL for (int k = 0; k < ACC_USED; ++k) {
-            float t, e;
-            two_sum(rs[row * RED_STRIDE + k], acc.s[k], t, e);
-            rs[row * RED_STRIDE + k] = t;
-            rc[row * RED_STRIDE + k] += acc.c[k] + e;
-        }
+        float t, e;
+        two_sum(rs[row * RED_STRIDE + k], x, t, e);
+        rs[row * RED_STRIDE + k] = t;
+        rc[row * RED_STRIDE + k] += e;
     }
 }
 
@@ -224,21 +187,18 @@ struct Split<2> {
 };
 static_assert(Split<1>::S >= 2 && Split<2>::S >= 2, "at most RED_ROWS points a lane group");
 
-// The reference's terms of one matched point (odom.py:335-375) into acc; P: the matched voxel's precision.
-template <int MODE>
+// The reference's terms of one matched point (odom.py:335-375) into its row (row = lane / S), formed by the
+// point's lanes together with the same code on different data: sub-lane 0 adds w J^T M J, sub-lane 1 w J^T Mg J
+// (the same products on the other matrix), and sub-lane 0 then the vector and scalar entries. Each entry's
+// value is the reference's float32 expression, and each row entry receives the points' terms in point order.
+// P: the matched voxel's precision.
+template <int MODE, int S>
 __device__ inline void point_terms(const VoxPrec &P, float kappa, float px, float py, float pz, float wx, float wy,
-                                   float wz, const Cand &b, Acc<MODE> &acc) {
+                                   float wz, const Cand &b, unsigned char *lds) {
+    static_assert(S >= 2, "two lanes a point at least");
+    const int lane = lane_id(), row = lane / S, sub = lane % S;
     const float r0 = wx - b.mx, r1 = wy - b.my, r2 = wz - b.mz;
     const float M[3][3] = {{P.M[0], P.M[1], P.M[2]}, {P.M[1], P.M[3], P.M[4]}, {P.M[2], P.M[4], P.M[5]}};
-    const float G[3][3] = {{P.Mg[0], P.Mg[1], P.Mg[2]}, {P.Mg[1], P.Mg[3], P.Mg[4]}, {P.Mg[2], P.Mg[4], P.Mg[5]}};
-    float rn;
-    if (P.shape > 0.5f) {
-        rn = r0 * P.q[0];
-        rn = rn + r1 * P.q[1];
-        rn = rn + r2 * P.q[2];
-    } else {
-        rn = length3(r0, r1, r2);
-    }
     float Mr[3];
     LW_UNROLL for (int k = 0; k < 3; ++k) {
         float t = M[k][0] * r0;
@@ -250,23 +210,37 @@ __device__ inline void point_terms(const VoxPrec &P, float kappa, float px, floa
     d2 = d2 + r2 * Mr[2];
     float wgt = 1.0f / (1.0f + d2 / (kappa * kappa));
     wgt = wgt * wgt;  // Geman-McClure (odom.py:370-371)
-    acc.add(ACC_RMS, wgt * rn * rn);
-#define LW_ADD_H(p, q, k) acc.add(ACC_H + k, wgt * hval<p, q>(M, wx, wy, wz));
-    LW_H_PAIRS(LW_ADD_H)
-#undef LW_ADD_H
-    acc.add(ACC_B + 0, wgt * bval<0>(Mr, wx, wy, wz));
-    acc.add(ACC_B + 1, wgt * bval<1>(Mr, wx, wy, wz));
-    acc.add(ACC_B + 2, wgt * bval<2>(Mr, wx, wy, wz));
-    acc.add(ACC_B + 3, wgt * bval<3>(Mr, wx, wy, wz));
-    acc.add(ACC_B + 4, wgt * bval<4>(Mr, wx, wy, wz));
-    acc.add(ACC_B + 5, wgt * bval<5>(Mr, wx, wy, wz));
-    acc.add(ACC_COUNT, 1.0f);
-    acc.add(ACC_ERR, wgt * d2);
-    acc.add(ACC_WEIGHT, wgt);
-#define LW_ADD_G(p, q, k) acc.add(ACC_HG + k, wgt * hval<p, q>(G, wx, wy, wz));
-    LW_H_PAIRS(LW_ADD_G)
-#undef LW_ADD_G
-    acc.add(ACC_RANGE, length3(px, py, pz));
+    if (sub < 2) {
+        const bool g = sub == 1;
+        const float X[3][3] = {{g ? P.Mg[0] : P.M[0], g ? P.Mg[1] : P.M[1], g ? P.Mg[2] : P.M[2]},
+                               {g ? P.Mg[1] : P.M[1], g ? P.Mg[3] : P.M[3], g ? P.Mg[4] : P.M[4]},
+                               {g ? P.Mg[2] : P.M[2], g ? P.Mg[4] : P.M[4], g ? P.Mg[5] : P.M[5]}};
+        const int base = g ? ACC_HG : ACC_H;
+#define LW_PUT_H(p, q, k) row_put<MODE>(lds, row, base + k, wgt * hval<p, q>(X, wx, wy, wz));
+        LW_H_PAIRS(LW_PUT_H)
+#undef LW_PUT_H
+    }
+    if (sub == 0) {
+        float rn;
+        if (P.shape > 0.5f) {
+            rn = r0 * P.q[0];
+            rn = rn + r1 * P.q[1];
+            rn = rn + r2 * P.q[2];
+        } else {
+            rn = length3(r0, r1, r2);
+        }
+        row_put<MODE>(lds, row, ACC_RMS, wgt * rn * rn);
+        row_put<MODE>(lds, row, ACC_B + 0, wgt * bval<0>(Mr, wx, wy, wz));
+        row_put<MODE>(lds, row, ACC_B + 1, wgt * bval<1>(Mr, wx, wy, wz));
+        row_put<MODE>(lds, row, ACC_B + 2, wgt * bval<2>(Mr, wx, wy, wz));
+        row_put<MODE>(lds, row, ACC_B + 3, wgt * bval<3>(Mr, wx, wy, wz));
+        row_put<MODE>(lds, row, ACC_B + 4, wgt * bval<4>(Mr, wx, wy, wz));
+        row_put<MODE>(lds, row, ACC_B + 5, wgt * bval<5>(Mr, wx, wy, wz));
+        row_put<MODE>(lds, row, ACC_COUNT, 1.0f);
+        row_put<MODE>(lds, row, ACC_ERR, wgt * d2);
+        row_put<MODE>(lds, row, ACC_WEIGHT, wgt);
+        row_put<MODE>(lds, row, ACC_RANGE, length3(px, py, pz));
+    }
 }
 
 // ---- the persistent loop's neighbourhood record --------------------------------------------------------
@@ -375,8 +349,10 @@ __device__ inline VoxPrec cached_prec(NbrCache *c, const VoxPrec *vox_prec, int 
         LW_UNROLL for (int k = 0; k < 4; ++k) q[k] = c->P[k][row];
     } else {
         P = vox_prec[vid];
-        LW_UNROLL for (int k = 0; k < 4; ++k) c->P[k][row] = q[k];
-        c->pvid[row] = vid;
+        if (lane_id() % LS == 0) {
+            LW_UNROLL for (int k = 0; k < 4; ++k) c->P[k][row] = q[k];
+            c->pvid[row] = vid;
+        }
     }
     return P;
 }
@@ -435,15 +411,12 @@ __device__ inline void wave_pass(const IcpArgs &a, const Pass p, const float *Tf
             o.vid = __shfl_xor(best.vid, off, 64);
             if (o.s >= 0 && (best.s < 0 || o.d2 < best.d2 || (o.d2 == best.d2 && o.j < best.j))) best = o;
         }
-        if (i < p.n && sub == 0) {
-            if (a.best_slot) a.best_slot[i] = best.s;
+        if (i < p.n) {  // every lane of the point holds the merged choice
+            if (sub == 0 && a.best_slot) a.best_slot[i] = best.s;
             if (best.s >= 0) {
-                ++matched;
-                Acc<MODE> acc;
-                acc.zero();
+                matched += sub == 0 ? 1 : 0;
                 const VoxPrec P = cached ? cached_prec(cache, a.vox_prec, lane / S, best.vid) : a.vox_prec[best.vid];
-                point_terms<MODE>(P, p.kappa, px, py, pz, wx, wy, wz, best, acc);
-                row_add<MODE, S>(lds, acc);
+                point_terms<MODE, S>(P, p.kappa, px, py, pz, wx, wy, wz, best, lds);
             }
         }
         if (stamp) t_terms = wall_clock64();
