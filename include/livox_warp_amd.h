@@ -464,6 +464,15 @@ int lw_host_voxel_precision(const float *rsum, const float *router6, int32_t c, 
 int lw_host_apply_twist(double *T, const double *delta);
 int lw_host_twist_between(const double *T_from, const double *T_to, double *xi);
 double lw_host_cond(const double *H);
+/* The device's two forms of the float64 step on one reduced accumulator v (64 doubles, the ACC layout) at pose
+ * T with prediction T_pred (host arrays): the wave form (the persistent registration's common case) and the
+ * one-lane scalar form, each followed by its pose update, in one launch on the current device. out (48
+ * doubles): [0, 16) the wave form's new pose, [16, 32) the scalar form's, 32 the wave form's result (1
+ * handled, 0 left to the scalar form, -1 too few correspondences), 33 its rms, [34, 40) its step, 40 the scalar
+ * form's return code (0 ok), 41 its rms, [42, 48) its step. No counterpart in the reference: a check that the
+ * two device forms agree bit for bit. */
+int lw_gn_step_check(const double *v, const double *T, const double *T_pred, double damping, double prior_info,
+                     double degen_thr, int32_t min_corr, double *out);
 
 #ifdef __cplusplus
 }

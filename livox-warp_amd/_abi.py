@@ -130,6 +130,7 @@ SIGNATURES = {
     "lw_prof_phases": (_i32, [_vp, _f64p]),
     "lw_host_gn_step": (_i32, [_f64p, _f64p, _f64p, _f64, _f64, _f64, _i32, _f64p, _f64p, _i32p]),
     "lw_host_voxel_precision": (_i32, [_f32p, _f32p, _i32, _f32, _f32, _f32, _f32, _f32p, _f32p, _f32p]),
+    "lw_gn_step_check": (_i32, [_f64p, _f64p, _f64p, _f64, _f64, _f64, _i32, _f64p]),
     "lw_host_apply_twist": (_i32, [_f64p, _f64p]),
     "lw_host_twist_between": (_i32, [_f64p, _f64p, _f64p]),
     "lw_host_cond": (_f64, [_f64p]),

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/livox_warp_amd.h"
+#include "lw_gn_wave.h"
 #include "lw_internal.h"
 #include "lw_loc.h"
 #include "lw_mlp.h"
@@ -902,6 +903,61 @@ int lw_host_twist_between(const double *T_from, const double *T_to, double *xi) 
 }
 
 double lw_host_cond(const double *H) { return H ? lw::gn::cond_of(H) : -1.0; }
+
+namespace {
+
+// One wave: the wave form of the step and its pose update, then the scalar form and its update in lane 0, on
+// the same accumulator (lw_gn_step_check's out layout).
+__global__ __launch_bounds__(64) void k_gn_step_check(const double *v, const double *T, const double *Tp,
+                                                      lw::gn::StepParams P, double *out) {
+    __shared__ double sh[lw::gnw::W_DOUBLES], sT[16], xi[6], H[36];
+    const int lane = threadIdx.x;
+    if (lane < 16) sT[lane] = T[lane];
+    if (lane == 0) lw::gn::twist_between(T, Tp, xi);
+    __syncthreads();
+    lw::gnw::WaveOut o;
+    unsigned long long t1 = 0ull, t2 = 0ull, t3 = 0ull;
+    const int r = lw::gnw::wave_step(v, sT, xi, P, sh, o, H, false, t1, t2, t3);
+    if (r > 0 && lane == 0) lw::gn::apply_twist(sT, o.delta);  // as the loop's accept_step moves the pose
+    __syncthreads();
+    if (lane < 16) out[lane] = sT[lane];
+    if (lane == 0) {
+        out[32] = (double)r;
+        out[33] = r > 0 ? o.rms : 0.0;
+        for (int k = 0; k < 6; ++k) out[34 + k] = r > 0 ? o.delta[k] : 0.0;
+        lw::gn::StepOut so;
+        double Ts[16];
+        for (int k = 0; k < 16; ++k) Ts[k] = T[k];
+        const int rc = lw::gn::step(v, T, Tp, P, so);
+        if (rc == lw::gn::STEP_OK) lw::gn::apply_twist(Ts, so.delta);
+        for (int k = 0; k < 16; ++k) out[16 + k] = Ts[k];
+        out[40] = (double)rc;
+        out[41] = so.rms;
+        for (int k = 0; k < 6; ++k) out[42 + k] = so.delta[k];
+    }
+}
+
+}  // namespace
+
+int lw_gn_step_check(const double *v, const double *T, const double *T_pred, double damping, double prior_info,
+                     double degen_thr, int32_t min_corr, double *out) {
+    if (!v || !T || !T_pred || !out) return LW_EINVAL;
+    double *d = nullptr;
+    LW_CK(hipMalloc((void **)&d, (64 + 16 + 16 + 48) * sizeof(double)));
+    hipError_t e = hipMemcpy(d, v, 64 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + 64, T, 16 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + 80, T_pred, 16 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        lw::gn::StepParams P{damping, prior_info, degen_thr, min_corr, 0};
+        hipLaunchKernelGGL(k_gn_step_check, dim3(1), dim3(64), 0, nullptr, d, d + 64, d + 80, P, d + 96);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, d + 96, 48 * sizeof(double), hipMemcpyDeviceToHost);
+    const hipError_t ef = hipFree(d);
+    LW_CK(e);
+    LW_CK(ef);
+    return 0;
+}
 
 }  // extern "C"
 

@@ -1,9 +1,10 @@
 // The float64 Gauss-Newton step (lw_gn.h's gn::step) spread over the 64 lanes of one wave, for the common
 // case: every direction constrained (J^T Hg J - thr I positive definite) and the damped system positive
-// definite. Matrices live in LDS; each lane forms matrix entries with the scalar routine's exact operation
-// order (the same products summed in the same order, the same divisions and square roots), so the result
-// equals gn::step's bit for bit; the Cholesky factorisations go column by column with the column's entries
-// in parallel. Anything else (a degenerate direction to hold, or an indefinite system needing the LU
+// definite. Matrices live in LDS, one entry a lane, and the Cholesky factorisations go column by column with
+// the column's entries in parallel; the substitutions keep one row of the factor a lane in registers and pass
+// the unknowns by readlane. Each lane forms its values with the scalar routine's exact operation order (the
+// same products summed in the same order, the same divisions and square roots), so the result equals
+// gn::step's bit for bit. Anything else (a degenerate direction to hold, or an indefinite system needing the LU
 // fallback) reports "not handled" and the caller runs gn::step in one lane.
 //
 // The scalar routine compiles to tens of kilobytes of straight-line code executed by one lane; this one is
@@ -20,7 +21,7 @@ namespace gnw {
 
 // LDS layout (doubles) of the step's scratch
 constexpr int W_H = 0, W_HG = 36, W_A = 72, W_J = 108, W_JI = 144, W_M1 = 180, W_M2 = 216, W_M3 = 252, W_L = 288,
-              W_L2 = 324, W_RHS = 360, W_Y = 366, W_OUT = 372, W_T = 378, W_DOUBLES = 394;
+              W_L2 = 324, W_RHS = 360, W_T = 366, W_DOUBLES = 382;
 
 // Index of (p, q), p <= q, in a row-major upper triangle of a 6x6 matrix (odom.py:387, np.triu_indices(6)).
 __device__ inline int triu(int p, int q) { return p * 6 - (p * (p - 1)) / 2 + (q - p); }
@@ -39,6 +40,14 @@ __device__ inline double tmul_entry(const double *A, const double *B, int l) {
     double s = 0.0;
     for (int k = 0; k < 6; ++k) s += A[6 * k + i] * B[6 * k + j];
     return s;
+}
+
+// A double held by lane src (the same lane for every lane) in every lane.
+__device__ inline double bcast(double x, int src) {
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), src);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | (uint64_t)lo));
 }
 
 // Two Cholesky factorisations side by side, column by column: lanes 0-31 factor A0 - shift0 I into L0, lanes
@@ -67,6 +76,23 @@ __device__ inline void wave_cholesky2(const double *A0, double shift0, double *L
     }
     pd0 = __shfl(pd ? 1 : 0, 0, 64) != 0;
     pd1 = __shfl(pd ? 1 : 0, 32, 64) != 0;
+}
+
+// Solve L L^T x = rhs with lane r < 6 holding row r of L in a[0..r] and rhs[r]; every lane returns x. Forward: y_k = t_k / L_kk, then each row below subtracts L_rk y_k (so row r subtracts k = 0, 1, ...
+// in order); back: x_i = (y_i - sum_{k > i} L_ki x_k) / L_ii with k ascending, on every lane. The operations
+// and their order are gn::solve6's.
+__device__ inline void wave_chol_solve(const double (&a)[6], double rhs_r, double (&x)[6]) {
+    const int lane = lane_id();
+    double t = rhs_r, y[6];
+    LW_UNROLL for (int k = 0; k < 6; ++k) {
+        y[k] = bcast(t, k) / bcast(a[k], k);
+        if (lane < 6 && lane > k) t = t - a[k] * y[k];
+    }
+    LW_UNROLL for (int i = 5; i >= 0; --i) {
+        double u = y[i];
+        LW_UNROLL for (int k = i + 1; k < 6; ++k) u = u - bcast(a[i], k) * x[k];
+        x[i] = u / bcast(a[i], i);
+    }
 }
 
 // Entry (r, c) of K = [t]x for the pose T (row-major 4x4 in LDS), as gn::sensor_basis lays it out:
@@ -108,7 +134,7 @@ __device__ inline void basis_entry(const double *Ts, double L, int l, double &j_
 }
 
 struct WaveOut {
-    double delta[6];
+    double delta[6];  // in every lane
     double rms;
     int32_t corr;
     int32_t n_degen;
@@ -129,8 +155,7 @@ __device__ inline int wave_step(const double *v, const double *T, const double *
     o.n_degen = 0;
     if (o.corr < P.min_corr) return -1;
     double *H = sh + W_H, *Hg = sh + W_HG, *A = sh + W_A, *J = sh + W_J, *Ji = sh + W_JI, *M1 = sh + W_M1,
-           *M2 = sh + W_M2, *M3 = sh + W_M3, *L = sh + W_L, *L2 = sh + W_L2, *rhs = sh + W_RHS, *y = sh + W_Y,
-           *out = sh + W_OUT, *Ts = sh + W_T;
+           *M2 = sh + W_M2, *M3 = sh + W_M3, *L = sh + W_L, *L2 = sh + W_L2, *rhs = sh + W_RHS, *Ts = sh + W_T;
     const bool prior = P.prior_info > 0.0, degen = P.degen_thr > 0.0;
     if (lane < 16) Ts[lane] = T[lane];
     wave_sync();
@@ -181,20 +206,11 @@ __device__ inline int wave_step(const double *v, const double *T, const double *
     if (stamp) t_factor = wall_clock64();
     if (degen && !pd_g) return 0;  // a direction to hold: the scalar routine does it
     if (!pd_a) return 0;           // indefinite: the scalar routine's LU fallback
-    if (lane == 0) {
-        for (int i = 0; i < 6; ++i) {
-            double t = rhs[i];
-            for (int k = 0; k < i; ++k) t -= L[6 * i + k] * y[k];
-            y[i] = t / L[6 * i + i];
-        }
-        for (int i = 5; i >= 0; --i) {
-            double t = y[i];
-            for (int k = i + 1; k < 6; ++k) t -= L[6 * k + i] * out[k];
-            out[i] = t / L[6 * i + i];
-        }
-        LW_UNROLL for (int i = 0; i < 6; ++i) o.delta[i] = out[i];
-        if (stamp) t_solve = wall_clock64();
-    }
+    double lrow[6];
+    const int r6 = lane < 6 ? lane : 0;
+    LW_UNROLL for (int m = 0; m < 6; ++m) lrow[m] = m <= r6 ? L[6 * r6 + m] : 0.0;
+    wave_chol_solve(lrow, rhs[r6], o.delta);
+    if (stamp) t_solve = wall_clock64();
     if (lane < 36) H_out[lane] = H[lane];
     return 1;
 }

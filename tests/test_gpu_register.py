@@ -117,3 +117,54 @@ def test_device_frame_count(lw, gpu_ok):
     assert a[0] == b[0]
     assert np.array_equal(a[1], b[1]) and np.array_equal(a[4], b[4])  # keys and counts
     np.testing.assert_allclose(a[2], b[2], rtol=1e-5, atol=1e-6)  # float atomics: order-dependent rounding
+
+
+def _step_check(lw, v, T, Tp, P):
+    A = lw._abi
+    out = np.zeros(48)
+    A.check(A.lib().lw_gn_step_check(A.ptr(np.ascontiguousarray(v, np.float64), A._f64p),
+                                     A.ptr(np.ascontiguousarray(T, np.float64), A._f64p),
+                                     A.ptr(np.ascontiguousarray(Tp, np.float64), A._f64p), P.damping, P.prior_info,
+                                     P.degen_thr, P.min_corr, A.ptr(out, A._f64p)), "lw_gn_step_check")
+    return out
+
+
+def test_wave_step_equals_scalar_step(lw, orc, gpu_ok):
+    """The wave form of the float64 step (the persistent registration's common case: the set-up and the
+    factorisations an entry a lane, the substitutions a row a lane) and its pose update give the one-lane scalar
+    form's step, rms and new pose bit for bit, on the accumulators of real registrations with the prior and the degeneracy test on and
+    off; a floor seen from above, with held directions, is left to the scalar form."""
+    w = lw.synth.single_scan(seed=14, map_points=300_000)
+    m = orc.OracleMap(1 << 20)
+    m.insert(w.map_xyz, np.zeros(len(w.map_xyz), np.uint32), np.eye(4), 0.25, max_range=1e30)
+    ds = orc.ds_insert(w.scan_xyz, w.scan_attr, 8.0, 0.3, 60.0, 15, 1 << 15)
+    handled = 0
+    for prior, degen in ((2000.0, 3000.0), (0.0, 0.0), (2000.0, 0.0), (0.0, 3000.0)):
+        P = orc.Params(reg_voxel=0.25, max_iter=12, prior_info=prior, degen_thr=degen)
+        r = orc.register(m, ds, w.T_init, P, keep_accs=True)
+        for T, _, _, _, v in r.accs:
+            out = _step_check(lw, v, T, w.T_init, P)
+            assert out[40] == 0 and out[32] in (0, 1)
+            if out[32] == 0:  # a held direction: the scalar form's case
+                continue
+            assert np.array_equal(out[34:40], out[42:48]) and out[33] == out[41]
+            assert np.array_equal(out[0:16], out[16:32])
+            handled += 1
+    assert handled >= 20
+    # held directions: not the wave form's case
+    rng = np.random.default_rng(9)
+    g = rng.uniform(-6, 6, (150000, 2))
+    floor = np.column_stack([g, np.zeros(len(g))]).astype(np.float32)
+    mf = orc.OracleMap(1 << 18)
+    mf.insert(floor, np.zeros(len(floor), np.uint32), np.eye(4), 0.25, max_range=1e30)
+    T = np.eye(4)
+    T[:3, 3] = [0.0, 0.0, 1.5]
+    T[:3, :3] = np.array([[0, 0, 1], [0, 1, 0], [-1, 0, 0]], float)
+    d = rng.normal(size=(3000, 3)) * [0, 0.3, 0.3] + [1, 0, 0]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    scan = (d * (1.5 / d[:, 0:1])).astype(np.float32)
+    dsf = orc.ds_insert(scan, np.zeros(len(scan), np.uint32), 8.0, 0.3, 60.0, 15, 1 << 13)
+    P = orc.Params(reg_voxel=0.25, max_iter=4)
+    r = orc.register(mf, dsf, T, P, keep_accs=True)
+    for Ti, _, _, _, v in r.accs:
+        assert _step_check(lw, v, Ti, T, P)[32] == 0
