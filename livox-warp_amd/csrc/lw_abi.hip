@@ -49,7 +49,7 @@ void free_all(lw_ctx *c) {
     void *ptrs[] = {c->slots, c->occ, c->vox_key, c->vox_mom, c->vox_slot, c->vox_prec, c->vox_nin, c->vox_quota,
                     c->vox_dirty, c->dirty_list, c->touched, c->ctr, c->pt_slot, c->pt_vid, c->pt_take,
                     c->ds_keys, c->ds_idx, c->ds_xyz, c->blk_cnt, c->blk_off, c->chain, c->partials, c->best_slot,
-                    c->dtmp, c->loop_epoch, c->loop_rows};
+                    c->dtmp, c->loop_epoch, c->loop_rows, c->prep_sync};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->prof_dev) (void)hipFree(c->prof_dev);
@@ -177,6 +177,10 @@ int lw_create(lw_ctx **out, int64_t reg_slots, int64_t frame_cap, void *stream) 
     c->chain_counts = reinterpret_cast<int32_t *>(c->chain + CH_COUNTS);
     if ((e = dalloc(&c->partials, (size_t)lw::icp_max_groups(frame_cap) * 128)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->loop_epoch, 1)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&c->prep_sync, 1)) != hipSuccess) return fail(e);
+    if ((e = lw::launch_fill_i32(c, reinterpret_cast<int32_t *>(c->prep_sync), sizeof(lw::PrepSync) / 4, 0)) !=
+        hipSuccess)
+        return fail(e);
     if ((e = dalloc(&c->loop_rows, (size_t)lw::LOOP_TAGGED_ROWS * lw::LOOP_ROW_WORDS)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->best_slot, frame_cap)) != hipSuccess) return fail(e);
     if ((e = dalloc(&c->dtmp, frame_cap + 64)) != hipSuccess) return fail(e);
@@ -772,6 +776,11 @@ int lw_odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, lw_f
     std::memcpy(&st2, h, sizeof(st2));
     std::memcpy(&st1, h + CH_REG1, sizeof(st1));
     std::memcpy(&fc, h + CH_FCTL, sizeof(fc));
+    if (one && fc.err == 2) {  // a grid barrier's wait ran out: its arrival count is reset for the next frame
+        LW_CK(lw::launch_fill_i32(c, reinterpret_cast<int32_t *>(c->prep_sync), sizeof(lw::PrepSync) / 4, 0));
+        LW_CK(hipStreamSynchronize(c->stream));
+        return LW_ELOOP;
+    }
     if (one && fc.err) return LW_EINVAL;
     if (st1.status < 0 || (fc.redo && st2.status < 0)) return LW_ELOOP;
     out->m = fc.m;

@@ -66,7 +66,7 @@ struct FrameCtl {
     float xi1[6];           // the twist the frame ends deskewed with
     float T32[16];          // the pose a keyframe insert uses
     int32_t m, skipped, ok1, redo, ok, keyframe;
-    int32_t err;            // 1: the frame held more points than the single-workgroup preparation takes
+    int32_t err;            // 1: the frame held more points than the preparation takes; 2: a grid barrier's wait ran out
 };
 constexpr size_t FRAME_CTL_INPUT = offsetof(FrameCtl, T1);  // bytes the host uploads
 
@@ -91,6 +91,14 @@ struct PrepArgs {
     uint32_t *epoch;       // the persistent loop's row-tag epoch (null: the batched chain, which has no loop)
     struct RegState *st1;  // given: the frame check runs first, and the rest only when it asks for a re-deskew
     struct FrameCtl *fc;
+};
+
+// The many-workgroup frame preparation's grid barrier (k_frame_prep_grid, lw_odom.hip): arrivals counted on
+// one counter that never resets, the base the next launch starts from, and each workgroup's thinned points.
+constexpr int PG_MAX_G = 64;
+struct PrepSync {
+    unsigned long long base, count;
+    int32_t total[PG_MAX_G];
 };
 
 // One sequence's keyframe decision in a batch (k_keyframe_check's inputs).
@@ -189,6 +197,7 @@ struct lw_ctx {
     hipEvent_t chain_ev = nullptr;  // lw_odom_frame's readback (the host waits for it, not for the insert's tail)
     double *partials = nullptr;
     uint32_t *loop_epoch = nullptr;  // the persistent registration's row-tag epoch (one word)
+    lw::PrepSync *prep_sync = nullptr;  // the many-workgroup frame preparation's barrier and counts
     uint64_t *loop_rows = nullptr;   // LOOP_TAGGED_ROWS x LOOP_ROW_WORDS: the persistent registration's tagged rows
     uint32_t reg_starts = 0;         // launches that may start a registration since the rows were last cleared
     int32_t *best_slot = nullptr;
@@ -437,6 +446,8 @@ hipError_t launch_keyframe_check(lw_ctx *c);
 // frame_prep_max() points. check: after the first registration -- the frame check first, then the rest
 // (deskew with the check's twist, the registration's start from its pose) only when it asks for a re-deskew.
 int32_t frame_prep_max();
+int32_t frame_prep_grid_max();  // the many-workgroup form's bound (launch_frame_prep takes it when c->prep_sync is set
+                                // and the frame's bound is within it)
 hipError_t launch_frame_prep(lw_ctx *c, const double *Tp, float t_mid, float inv_dt, const float *xi, float inv_v,
                              float min_range, float max_range, int32_t ret_mask, bool check);
 

@@ -3,6 +3,7 @@
 // the keyframe test and insert -- is queued at once and read back once (lw_odom_frame in lw_abi.hip). Each
 // decision is one lane of a tiny launch that reads the registration state and writes a FrameCtl flag the
 // next kernels are gated on.
+#include <algorithm>
 #include <climits>
 
 #include "lw_frame.h"
@@ -65,16 +66,24 @@ __device__ inline bool reg_outcome(const RegState *st, const double *T_pred, int
     return has && st->corr >= min_corr;
 }
 
-// After the first registration (one lane): the frame's count and skip test (odom.py:676-678) and the re-deskew
-// test (odom.py:683-690) on xi_new = body twist of T_prev^-1 T. cnt: the frame's appended points.
-__device__ inline void frame_check(const RegState *st, FrameCtl *fc, int32_t cnt) {
-    const int32_t m = cnt < fc->frame_cap ? cnt : fc->frame_cap;
-    fc->m = m;
-    fc->skipped = m < fc->min_points ? 1 : 0;
+// What the frame check decides (FrameCtl's fields of the same names).
+struct Check {
     double T1[16];
-    const bool ok1 = !fc->skipped && reg_outcome(st, fc->T_pred, fc->min_corr, T1);
-    for (int k = 0; k < 16; ++k) fc->T1[k] = T1[k];
-    fc->ok1 = ok1 ? 1 : 0;
+    float xi1[6];
+    int32_t m, skipped, ok1, redo;
+};
+
+// After the first registration (one lane): the frame's count and skip test (odom.py:676-678) and the re-deskew
+// test (odom.py:683-690) on xi_new = body twist of T_prev^-1 T, from the registration state and the frame's
+// inputs alone. cnt: the frame's appended points.
+__device__ inline void frame_check_of(const RegState *st, const FrameCtl *fc, int32_t cnt, Check &c) {
+    const int32_t m = cnt < fc->frame_cap ? cnt : fc->frame_cap;
+    c.m = m;
+    c.skipped = m < fc->min_points ? 1 : 0;
+    double T1[16];
+    const bool ok1 = !c.skipped && reg_outcome(st, fc->T_pred, fc->min_corr, T1);
+    for (int k = 0; k < 16; ++k) c.T1[k] = T1[k];
+    c.ok1 = ok1 ? 1 : 0;
     bool redo = false;
     float xn[6];
     if (ok1 && fc->redeskew && fc->has_prev) {
@@ -91,8 +100,23 @@ __device__ inline void frame_check(const RegState *st, FrameCtl *fc, int32_t cnt
         const float nt = sqrtf(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
         redo = (double)nr > fc->redeskew_rot || (double)nt > fc->redeskew_trans;
     }
-    fc->redo = redo ? 1 : 0;
-    for (int k = 0; k < 6; ++k) fc->xi1[k] = redo ? xn[k] : fc->xi0[k];
+    c.redo = redo ? 1 : 0;
+    for (int k = 0; k < 6; ++k) c.xi1[k] = redo ? xn[k] : fc->xi0[k];
+}
+
+__device__ inline void check_store(const Check &c, FrameCtl *fc) {
+    fc->m = c.m;
+    fc->skipped = c.skipped;
+    for (int k = 0; k < 16; ++k) fc->T1[k] = c.T1[k];
+    fc->ok1 = c.ok1;
+    fc->redo = c.redo;
+    for (int k = 0; k < 6; ++k) fc->xi1[k] = c.xi1[k];
+}
+
+__device__ inline void frame_check(const RegState *st, FrameCtl *fc, int32_t cnt) {
+    Check c;
+    frame_check_of(st, fc, cnt, c);
+    check_store(c, fc);
 }
 
 // Keep the first registration's state in st1 (the second registration reuses st); every thread of the block
@@ -380,9 +404,179 @@ __global__ __launch_bounds__(PREP_THREADS) void k_frame_prep_seq(const PrepArgs 
     frame_prep(A);
 }
 
+// ---- many workgroups: deskew, thinning and the registration's start --------------------------------------
+//
+// The same preparation over G workgroups of PG_THREADS threads (one launch, two grid barriers), for the
+// single-stream chain, where one workgroup leaves the rest of the device idle. Workgroup g takes the contiguous
+// indices [g chunk, (g + 1) chunk), chunk a multiple of PG_THREADS, in rounds of PG_THREADS, so index order is
+// (workgroup, round, wave, lane) order. Every workgroup takes the frame check itself (the same reads, the same
+// decision; workgroup 0 records it and keeps the first registration's state), then deskews its points and
+// claims their keys in the global thinning table (gpu.py's probe, each slot keeping the lowest index, exactly as
+// the grid kernels do). After the first barrier every claim is in: each point learns whether it holds its slot's
+// lowest index, each workgroup counts its winners per (round, wave) and publishes its total, and workgroup 0
+// starts the registration state (nothing reads the old one any more). After the second barrier each workgroup
+// writes its winners at its offset (the totals of the workgroups before it) in index order and clears their
+// slots. The result is k_frame_prep's: one point per thinning voxel, the lowest index, in index order.
+//
+// The barrier counts arrivals on one 64-bit counter that never resets: a launch reads the base it started
+// from, waits for base + G and base + 2 G, and workgroup 0 moves the base on after the second. Workgroups are
+// few (at most PG_MAX_G) and small; every wait is bounded (then the frame reports err 2). Data crossing
+// workgroups (the claims, the totals) moves by device-coherent atomics; a workgroup's deskewed points are read
+// back only by itself.
+constexpr int PG_THREADS = 256, PG_WAVES = PG_THREADS / 64, PG_ROUNDS = 8;
+static_assert(PG_ROUNDS * PG_WAVES <= 64, "one wave scans the per-(round, wave) counts");
+constexpr uint32_t PG_SPIN_LIMIT = 1u << 24;
+
+// Arrive at the grid barrier, then wait until target arrivals are in. Every thread of the workgroup calls it;
+// false when the wait ran out.
+__device__ inline bool grid_barrier(PrepSync *ps, unsigned long long target) {
+    __shared__ int32_t s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this workgroup's claims, totals and stores are done
+        __hip_atomic_fetch_add(&ps->count, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int32_t ok = 0;
+        for (uint32_t spins = 0; spins <= PG_SPIN_LIMIT; ++spins) {
+            if (__hip_atomic_load(&ps->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) {
+                ok = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+__global__ __launch_bounds__(PG_THREADS) void k_frame_prep_grid(PrepArgs A, PrepSync *ps) {
+    __shared__ double s_T[16];
+    __shared__ float s_xi[6];
+    __shared__ int32_t s_go, s_off;
+    __shared__ unsigned long long s_base;
+    __shared__ int32_t cnt[PG_ROUNDS * PG_WAVES];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = (int)blockIdx.x, G = (int)gridDim.x;
+    if (A.st1 && g == 0) keep_state(A.st, A.st1);  // the first registration's state, kept for the host
+    if (tid == 0) {
+        s_base = __hip_atomic_load(&ps->base, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (A.st1) {  // after the first registration: the skip and re-deskew tests
+            Check c;
+            frame_check_of(A.st, A.fc, *A.m_dev, c);
+            if (g == 0) check_store(c, A.fc);
+            s_go = c.redo;
+            for (int k = 0; k < 16; ++k) s_T[k] = c.T1[k];
+            for (int k = 0; k < 6; ++k) s_xi[k] = c.xi1[k];
+        } else {
+            s_go = 1;
+            if (g == 0) A.fc->err = 0;
+            for (int k = 0; k < 16; ++k) s_T[k] = A.Tp[k];
+            for (int k = 0; k < 6; ++k) s_xi[k] = A.xi[k];
+        }
+    }
+    __syncthreads();
+    if (!s_go) return;  // every workgroup decides alike, before any barrier
+    const int32_t m = count_of(A.frame_cap, A.m_dev);
+    const int32_t chunk = (((m + G - 1) / G) + PG_THREADS - 1) / PG_THREADS * PG_THREADS;
+    const int rounds = chunk / PG_THREADS;
+    if (rounds > PG_ROUNDS) {  // more than the launch's bound (every workgroup alike): thin nothing, and say so
+        if (g == 0 && tid == 0) {
+            A.fc->err = 1;
+            *A.n_out = 0;
+        }
+        return;
+    }
+    const int32_t lo = g * chunk;
+    float xi[6];
+    for (int k = 0; k < 6; ++k) xi[k] = s_xi[k];
+    int32_t slot[PG_ROUNDS];
+    float qx[PG_ROUNDS], qy[PG_ROUNDS], qz[PG_ROUNDS];
+    LW_UNROLL for (int r = 0; r < PG_ROUNDS; ++r) slot[r] = -1;
+    LW_UNROLL for (int r = 0; r < PG_ROUNDS; ++r) {
+        if (r >= rounds) break;
+        const int32_t i = lo + r * PG_THREADS + tid;
+        if (i < m) {
+            deskew_point(A.raw[3 * i], A.raw[3 * i + 1], A.raw[3 * i + 2], A.t[i], A.t_mid, A.inv_dt, xi, qx[r],
+                         qy[r], qz[r]);
+            A.xyz[3 * i] = qx[r];
+            A.xyz[3 * i + 1] = qy[r];
+            A.xyz[3 * i + 2] = qz[r];
+            int64_t key;
+            if (thin_key(qx[r], qy[r], qz[r], A.attr[i], A.inv_v, A.min_range, A.max_range, A.ret_mask, key))
+                slot[r] = global_claim(A.ds_keys, A.ds_idx, A.ds_mask, key, i);
+        }
+    }
+    const bool ok1 = grid_barrier(ps, s_base + (unsigned long long)G);
+    if (g == 0) reg_state_begin(A.st, s_T, reinterpret_cast<const float *>(A.Tp + 16), A.epoch, tid, PG_THREADS);
+    // winners, counted per (round, wave)
+    uint32_t win = 0u;
+    LW_UNROLL for (int r = 0; r < PG_ROUNDS; ++r) {
+        if (r >= rounds) break;
+        const int32_t i = lo + r * PG_THREADS + tid;
+        const int32_t s = slot[r];
+        const bool w = s >= 0 && __hip_atomic_load(&A.ds_idx[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == i;
+        win |= (w ? 1u : 0u) << r;
+        const unsigned long long bw = __ballot(w);
+        if (lane == 0) cnt[r * PG_WAVES + wave] = __popcll(bw);
+    }
+    __syncthreads();
+    if (wave == 0) {  // exclusive offsets over (round, wave) in that order; the workgroup's total published
+        const int n = rounds * PG_WAVES;
+        const int32_t v = lane < n ? cnt[lane] : 0;
+        int32_t incl = v;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int32_t o = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += o;
+        }
+        if (lane < n) cnt[lane] = incl - v;
+        const int32_t total = __shfl(incl, 63, 64);
+        if (lane == 0) __hip_atomic_store(&ps->total[g], total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const bool ok2 = grid_barrier(ps, s_base + 2ull * (unsigned long long)G);
+    if (!ok1 || !ok2) {
+        if (tid == 0) A.fc->err = 2;
+        return;
+    }
+    if (wave == 0) {  // this workgroup's offset; workgroup 0 also the frame's count and the next launch's base
+        int32_t before = 0, all = 0;
+        for (int k = lane; k < G; k += 64) {
+            const int32_t t = __hip_atomic_load(&ps->total[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            all += t;
+            before += k < g ? t : 0;
+        }
+        for (int d = 32; d > 0; d >>= 1) {
+            before += __shfl_xor(before, d, 64);
+            all += __shfl_xor(all, d, 64);
+        }
+        if (lane == 0) {
+            s_off = before;
+            if (g == 0) {
+                *A.n_out = all;
+                __hip_atomic_store(&ps->base, s_base + 2ull * (unsigned long long)G, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+    __syncthreads();
+    LW_UNROLL for (int r = 0; r < PG_ROUNDS; ++r) {
+        if (r >= rounds) break;
+        const bool w = ((win >> r) & 1u) != 0u;
+        const unsigned long long bw = __ballot(w);
+        if (w) {
+            const int32_t j = s_off + cnt[r * PG_WAVES + wave] + __popcll(bw & ((1ull << lane) - 1ull));
+            A.ds_xyz[3 * j] = qx[r];
+            A.ds_xyz[3 * j + 1] = qy[r];
+            A.ds_xyz[3 * j + 2] = qz[r];
+            A.ds_keys[slot[r]] = -1;  // the table left clean, as the grid kernels leave it
+            A.ds_idx[slot[r]] = INT_MAX;
+        }
+    }
+}
+
 }  // namespace
 
 int32_t frame_prep_max() { return PREP_THREADS * PREP_ROUNDS; }
+
+int32_t frame_prep_grid_max() { return PG_MAX_G * PG_THREADS * PG_ROUNDS; }
 
 hipError_t launch_frame_prep_seq(hipStream_t s, const PrepArgs *D, int32_t S) {
     if (S <= 0) return hipSuccess;
@@ -396,8 +590,10 @@ hipError_t launch_keyframe_check_seq(hipStream_t s, const KeySeq *D, int32_t S) 
     return hipGetLastError();
 }
 
-hipError_t launch_frame_prep(lw_ctx *c, const double *Tp, float t_mid, float inv_dt, const float *xi, float inv_v,
-                             float min_range, float max_range, int32_t ret_mask, bool check) {
+namespace {
+
+PrepArgs prep_args(lw_ctx *c, const double *Tp, float t_mid, float inv_dt, const float *xi, float inv_v,
+                   float min_range, float max_range, int32_t ret_mask, bool check) {
     PrepArgs A{};
     A.raw = c->f_raw;
     A.t = c->f_t;
@@ -422,7 +618,21 @@ hipError_t launch_frame_prep(lw_ctx *c, const double *Tp, float t_mid, float inv
     A.epoch = c->loop_epoch;
     A.st1 = check ? c->reg1 : nullptr;
     A.fc = c->fctl;
-    hipLaunchKernelGGL(k_frame_prep, dim3(1), dim3(PREP_THREADS), 0, c->stream, A);
+    return A;
+}
+
+}  // namespace
+
+hipError_t launch_frame_prep(lw_ctx *c, const double *Tp, float t_mid, float inv_dt, const float *xi, float inv_v,
+                             float min_range, float max_range, int32_t ret_mask, bool check) {
+    const PrepArgs A = prep_args(c, Tp, t_mid, inv_dt, xi, inv_v, min_range, max_range, ret_mask, check);
+    if (c->prep_sync && c->frame_bound > 0 && c->frame_bound <= frame_prep_grid_max()) {
+        // one workgroup per PG_THREADS points the frame can hold
+        const int G = (int)std::min<int64_t>(PG_MAX_G, (c->frame_bound + PG_THREADS - 1) / PG_THREADS);
+        hipLaunchKernelGGL(k_frame_prep_grid, dim3(G), dim3(PG_THREADS), 0, c->stream, A, c->prep_sync);
+    } else {
+        hipLaunchKernelGGL(k_frame_prep, dim3(1), dim3(PREP_THREADS), 0, c->stream, A);
+    }
     return hipGetLastError();
 }
 
