@@ -161,13 +161,38 @@ typedef struct {
 } lw_frame_in;
 
 typedef struct {
-    lw_reg_result reg[2];  /* the frame's registrations in order (the first `registrations` are valid) */
+    lw_reg_result reg[3];  /* the frame's registrations in order (the first `registrations` are valid) */
     double T[16];          /* the frame's pose */
+    double d[6];           /* lw_odom_frame_prior: the learned increment the guess used (when guess) */
     float xi[6];           /* the twist the frame ended deskewed with */
-    int32_t m, skipped, ok, registrations, keyframe, map_voxels, map_overflow, pad;
+    int32_t m, skipped, ok, registrations, keyframe, map_voxels, map_overflow;
+    int32_t guess;         /* lw_odom_frame_prior: the first registration started from the learned guess */
+    int32_t fallback;      /* lw_odom_frame_prior: its first residual failed the gate; the frame went on from the
+                              constant-velocity prediction */
+    int32_t pad;
 } lw_frame_out;
 
 int lw_odom_frame(lw_ctx *ctx, const lw_reg_params *p, const lw_frame_in *in, lw_frame_out *out);
+
+/* The same chain with the learned pose-increment prior in place of the constant-velocity guess (Odometry.prior,
+ * LearnedPrior.guess / accept): in->T_pred and in->xi hold the constant-velocity prediction and twist. When the
+ * frame holds at least min_points points, the first registration starts from prior_T exp(d), d = hist0 +
+ * y sd_t + mu_t (y: the network's 6 outputs, on the device: queue the forward pass on the context's stream
+ * first), with the points deskewed by d; if its first residual (rms0) exceeds rms_gate the frame is deskewed with
+ * the constant-velocity twist and registered from the constant-velocity prediction instead; the re-deskew test,
+ * the keyframe test and the insert follow as in lw_odom_frame. Up to three registrations (out->reg in order);
+ * out->guess, out->fallback and out->d report the prior's part. No counterpart in the reference (the learned
+ * prior is this build's extension). */
+typedef struct {
+    const float *y;        /* device: the network's output for this frame (6 float32) */
+    double T[16];          /* the previous frame's pose */
+    double hist0[6];       /* the last registered increment (body twist) */
+    double sd_t[6], mu_t[6]; /* the output's scale and offset */
+    double rms_gate;       /* fallback x the running median of earlier first residuals (HUGE_VAL: none yet) */
+} lw_frame_prior;
+
+int lw_odom_frame_prior(lw_ctx *ctx, const lw_reg_params *p, const lw_frame_in *in, const lw_frame_prior *prior,
+                        lw_frame_out *out);
 
 /* ---- The batched odometry: one frame of each of S independent sequences per call. No counterpart in the
  * reference (it runs one Odometry per stream); each sequence's frame goes through lw_odom_frame's chain --
@@ -275,7 +300,7 @@ int lw_mlp_train_step(lw_mlp *m, const float *x, int64_t ldx, const float *targe
 
 /* The 152 inputs of the open frame's m points (the bound f_raw buffer, sensor frame) into out (device):
  * motion[24] as given, then 64 range bins over [0, r_max) and 64 elevation bins over [-e_max, e_max), each
- * count * 64 / m. */
+ * count * 64 / m. m < 0: the open frame's count, read on the device (up to frame_cap). */
 int lw_frame_features(lw_ctx *ctx, int32_t m, float r_max, float e_max, const float *motion, float *out);
 
 /* Live measurement of the accumulation launches (for throughput and roofline reports): lw_prof_enable(ctx, 1)

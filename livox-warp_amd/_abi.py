@@ -49,9 +49,14 @@ class FrameIn(ctypes.Structure):
 
 
 class FrameOut(ctypes.Structure):
-    _fields_ = [("reg", RegResult * 2), ("T", _f64 * 16), ("xi", _f32 * 6), ("m", _i32), ("skipped", _i32),
-                ("ok", _i32), ("registrations", _i32), ("keyframe", _i32), ("map_voxels", _i32),
-                ("map_overflow", _i32), ("pad", _i32)]
+    _fields_ = [("reg", RegResult * 3), ("T", _f64 * 16), ("d", _f64 * 6), ("xi", _f32 * 6), ("m", _i32),
+                ("skipped", _i32), ("ok", _i32), ("registrations", _i32), ("keyframe", _i32), ("map_voxels", _i32),
+                ("map_overflow", _i32), ("guess", _i32), ("fallback", _i32), ("pad", _i32)]
+
+
+class FramePrior(ctypes.Structure):
+    _fields_ = [("y", _vp), ("T", _f64 * 16), ("hist0", _f64 * 6), ("sd_t", _f64 * 6), ("mu_t", _f64 * 6),
+                ("rms_gate", _f64)]
 
 
 class BatchSrc(ctypes.Structure):
@@ -102,6 +107,8 @@ SIGNATURES = {
     "lw_register_thinned": (_i32, [_vp, _i32, ctypes.POINTER(RegParams), _f64p, ctypes.POINTER(RegResult)]),
     "lw_set_persistent": (_i32, [_vp, _i32]),
     "lw_odom_frame": (_i32, [_vp, ctypes.POINTER(RegParams), ctypes.POINTER(FrameIn), ctypes.POINTER(FrameOut)]),
+    "lw_odom_frame_prior": (_i32, [_vp, ctypes.POINTER(RegParams), ctypes.POINTER(FrameIn),
+                                   ctypes.POINTER(FramePrior), ctypes.POINTER(FrameOut)]),
     "lw_loc_batch": (_i32, [_vp, _vp, _vp, _vp, _i32, _vp, _i32, ctypes.POINTER(LocParams), _vp, _vp, _vp, _vp]),
     "lw_loc_tile_info": (_i32, [_i32, _i32, _i32p]),
     "lw_vox_create": (_i32, [ctypes.POINTER(_vp), _i32, _vp]),

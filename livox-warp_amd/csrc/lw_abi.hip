@@ -80,7 +80,8 @@ hipError_t prof_harvest(lw_ctx *c) {
 // The chain block (lw_ctx::chain): [reg | reg1 | chain_tp | fctl | chain_counts], each part 256-byte aligned.
 constexpr size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 constexpr size_t CH_REG1 = align256(sizeof(lw::RegState));
-constexpr size_t CH_TP = CH_REG1 + align256(sizeof(lw::RegState));
+constexpr size_t CH_REGFB = CH_REG1 + align256(sizeof(lw::RegState));
+constexpr size_t CH_TP = CH_REGFB + align256(sizeof(lw::RegState));
 constexpr size_t CH_TP_BYTES = 16 * sizeof(double) + 64 * sizeof(float);
 constexpr size_t CH_FCTL = CH_TP + align256(CH_TP_BYTES);
 constexpr size_t CH_COUNTS = CH_FCTL + align256(sizeof(lw::FrameCtl));
@@ -172,6 +173,7 @@ int lw_create(lw_ctx **out, int64_t reg_slots, int64_t frame_cap, void *stream) 
     if ((e = dalloc(&c->chain, CH_BYTES)) != hipSuccess) return fail(e);
     c->reg = reinterpret_cast<lw::RegState *>(c->chain);
     c->reg1 = reinterpret_cast<lw::RegState *>(c->chain + CH_REG1);
+    c->reg_fb = reinterpret_cast<lw::RegState *>(c->chain + CH_REGFB);
     c->chain_tp = reinterpret_cast<double *>(c->chain + CH_TP);
     c->fctl = reinterpret_cast<lw::FrameCtl *>(c->chain + CH_FCTL);
     c->chain_counts = reinterpret_cast<int32_t *>(c->chain + CH_COUNTS);
@@ -661,13 +663,14 @@ int lw_loc_tile_info(int32_t block, int32_t chunk, int32_t *info) {
 }
 
 int lw_frame_features(lw_ctx *c, int32_t m, float r_max, float e_max, const float *motion, float *out) {
-    if (!c || !motion || !out || m < 0 || m > c->frame_cap || !(r_max > 0.0f) || !(e_max > 0.0f)) return LW_EINVAL;
+    if (!c || !motion || !out || m > c->frame_cap || !(r_max > 0.0f) || !(e_max > 0.0f)) return LW_EINVAL;
     if (!c->f_raw) return LW_ENOTBOUND;
+    if (m < 0) LW_CK(frame_count_ready(c));  // the open frame's count, read on the device
     lw::FeatArgs f{};
     f.r_max = r_max;
     f.e_max = e_max;
     std::memcpy(f.motion, motion, sizeof(f.motion));
-    LW_CK(lw::launch_features(c->stream, c->f_raw, m, f, out));
+    LW_CK(lw::launch_features(c->stream, c->f_raw, m, m < 0 ? &c->ctr[6] : nullptr, (int32_t)c->frame_cap, f, out));
     return 0;
 }
 
@@ -689,8 +692,13 @@ int lw_register(lw_ctx *c, int32_t m, const lw_reg_params *p, const double *T_pr
     return run_registration(c, &c->ctr[5], bound, -1, m < 0 ? -1 : m, p, T_pred, res);  // thinned count on the device
 }
 
-int lw_odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, lw_frame_out *out) {
+namespace {
+
+// lw_odom_frame's chain, with the learned prior's steps when pr is given (lw_odom_frame_prior).
+int odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, const lw_frame_prior *pr,
+               lw_frame_out *out) {
     if (!c || bad_params(p) || !in || !out) return LW_EINVAL;
+    if (pr && !pr->y) return LW_EINVAL;
     if (!c->f_raw) return LW_ENOTBOUND;
     if (!c->loop_ok) return LW_ENOLOOP;
     LW_CK(frame_count_ready(c));
@@ -714,6 +722,14 @@ int lw_odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, lw_f
     f.has_key = in->has_key;
     f.redeskew = in->redeskew;
     f.frame_cap = (int32_t)c->frame_cap;
+    if (pr) {
+        f.prior_on = 1;
+        std::memcpy(f.prior_T, pr->T, sizeof(f.prior_T));
+        std::memcpy(f.hist0, pr->hist0, sizeof(f.hist0));
+        std::memcpy(f.sd_t, pr->sd_t, sizeof(f.sd_t));
+        std::memcpy(f.mu_t, pr->mu_t, sizeof(f.mu_t));
+        f.rms_gate = pr->rms_gate;
+    }
     constexpr size_t UP_CTL = CH_FCTL - CH_TP;
     std::memcpy(pin_up(c), in->T_pred, 16 * sizeof(double));
     std::memcpy(pin_up(c) + 16 * sizeof(double), L.kappa, 64 * sizeof(float));
@@ -724,11 +740,15 @@ int lw_odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, lw_f
     // a frame one workgroup can prepare (up to frame_prep_max() points, with a thinning table of at least 4096
     // slots): deskew, thinning and the registration's start in one launch each time
     const bool one = c->frame_bound <= lw::frame_prep_max() && c->ds_cap >= 4096;
-    LW_CK(lw::loop_epoch_guard(c, 2));  // the two registrations' starts below
+    LW_CK(lw::loop_epoch_guard(c, pr ? 3 : 2));  // the registrations' starts below
+    if (pr) {
+        if (!one) return LW_EINVAL;  // the prior's chain prepares the frame in one launch
+        LW_CK(lw::launch_prior_guess(c, pr->y));  // the learned guess into the prediction and the twist
+    }
     // odom.py:671-681: deskew with the model's twist, register from the prediction
     if (one) {
         LW_CK(lw::launch_frame_prep(c, c->chain_tp, in->t_mid, in->inv_dt, in->xi, inv_v_thin, p->min_range,
-                                    p->max_range, p->ret_mask, false));
+                                    p->max_range, p->ret_mask, false, pr ? c->fctl->xi0 : nullptr));
     } else {
         LW_CK(lw::launch_deskew(c, -1, in->t_mid, in->inv_dt, in->xi));
         LW_CK(lw::launch_thin(c, -1, inv_v_thin, p->min_range, p->max_range, p->ret_mask));
@@ -742,6 +762,12 @@ int lw_odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, lw_f
         return LW_ENOLOOP;
     }
     if (e != hipSuccess) return -(int)e;
+    if (pr) {  // LearnedPrior.accept: a first residual above the gate hands the frame to the fallback
+        LW_CK(lw::launch_prior_fallback(c));
+        LW_CK(lw::launch_frame_prep_fallback(c, in->t_mid, in->inv_dt, inv_v_thin, p->min_range, p->max_range,
+                                             p->ret_mask));
+        LW_CK(lw::launch_register_loop(c, &c->ctr[5], cap, L, p->acc_mode, &c->fctl->fb));
+    }
     // odom.py:683-690: the re-deskew test, then (gated on it) deskew and register again from the first pose
     const int32_t *redo = &c->fctl->redo;
     if (one) {
@@ -771,10 +797,11 @@ int lw_odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, lw_f
     LW_CK(lw::launch_insert_tail(c, ins));
     LW_CK(hipEventSynchronize(c->chain_ev));
     LW_CK(prof_harvest(c));
-    lw::RegState st2, st1;
+    lw::RegState st2, st1, stf;
     lw::FrameCtl fc;
     std::memcpy(&st2, h, sizeof(st2));
     std::memcpy(&st1, h + CH_REG1, sizeof(st1));
+    std::memcpy(&stf, h + CH_REGFB, sizeof(stf));
     std::memcpy(&fc, h + CH_FCTL, sizeof(fc));
     if (one && fc.err == 2) {  // a grid barrier's wait ran out: its arrival count is reset for the next frame
         LW_CK(lw::launch_fill_i32(c, reinterpret_cast<int32_t *>(c->prep_sync), sizeof(lw::PrepSync) / 4, 0));
@@ -782,25 +809,39 @@ int lw_odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, lw_f
         return LW_ELOOP;
     }
     if (one && fc.err) return LW_EINVAL;
-    if (st1.status < 0 || (fc.redo && st2.status < 0)) return LW_ELOOP;
+    const bool fb = pr && fc.fb;
+    if (st1.status < 0 || (fb && stf.status < 0) || (fc.redo && st2.status < 0)) return LW_ELOOP;
     out->m = fc.m;
     out->skipped = fc.skipped;
     out->ok = fc.ok;
     out->keyframe = fc.keyframe;
-    out->registrations = fc.skipped ? 0 : 1 + fc.redo;
+    out->registrations = fc.skipped ? 0 : 1 + (fb ? 1 : 0) + fc.redo;
+    out->guess = pr ? fc.guess : 0;
+    out->fallback = fb ? 1 : 0;
+    if (pr && fc.guess) std::memcpy(out->d, fc.d, sizeof(out->d));
     std::memcpy(out->T, fc.T, sizeof(out->T));
     std::memcpy(out->xi, fc.xi1, sizeof(out->xi));
     std::memcpy(&out->map_voxels, h + CH_COUNTS, sizeof(int32_t));
     std::memcpy(&out->map_overflow, h + CH_COUNTS + sizeof(int32_t), sizeof(int32_t));
-    if (out->registrations >= 1) {
-        out->reg[0].m = fc.m;
-        fill_result(st1, p, &out->reg[0]);
-    }
-    if (out->registrations >= 2) {
-        out->reg[1].m = fc.m;
-        fill_result(st2, p, &out->reg[1]);
+    // the registrations in order: the guess's (kept aside when the fallback ran), the fallback's, the second
+    const lw::RegState *order[3] = {fb ? &stf : &st1, fb ? &st1 : &st2, &st2};
+    for (int k = 0; k < out->registrations && k < 3; ++k) {
+        out->reg[k].m = fc.m;
+        fill_result(*order[k], p, &out->reg[k]);
     }
     return 0;
+}
+
+}  // namespace
+
+int lw_odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, lw_frame_out *out) {
+    return odom_frame(c, p, in, nullptr, out);
+}
+
+int lw_odom_frame_prior(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, const lw_frame_prior *prior,
+                        lw_frame_out *out) {
+    if (!prior) return LW_EINVAL;
+    return odom_frame(c, p, in, prior, out);
 }
 
 int lw_prof_enable(lw_ctx *c, int32_t on) {

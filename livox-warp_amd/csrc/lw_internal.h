@@ -60,7 +60,18 @@ struct FrameCtl {
     // the batched chain's (lw_batch_frame): a warm-up frame (inserted at T_pred unregistered), the first
     // registration's gate (1 unless warm), the frame's point count
     int32_t warm, run1, m_in, pad_in;
+    // the learned prior's frame (lw_odom_frame_prior): T_pred and xi0 above hold the constant-velocity
+    // prediction and twist; the guess is prior_T exp(d), d = hist0 + y sd_t + mu_t (y: the network's output);
+    // a first residual (rms0) above rms_gate hands the frame back to the constant-velocity prediction
+    double prior_T[16];
+    double hist0[6], sd_t[6], mu_t[6];
+    double rms_gate;
+    int32_t prior_on, pad_p;
     // written on the device
+    double T_cv[16];        // the constant-velocity prediction, kept while T_pred holds the guess
+    float xi_cv[6];
+    int32_t guess, fb;      // the first registration started from the learned guess; the fallback ran
+    double d[6];            // the learned increment
     double T1[16];          // the first registration's pose (the second's prediction)
     double T[16];           // the frame's pose
     float xi1[6];           // the twist the frame ends deskewed with
@@ -68,11 +79,13 @@ struct FrameCtl {
     int32_t m, skipped, ok1, redo, ok, keyframe;
     int32_t err;            // 1: the frame held more points than the preparation takes; 2: a grid barrier's wait ran out
 };
-constexpr size_t FRAME_CTL_INPUT = offsetof(FrameCtl, T1);  // bytes the host uploads
+constexpr size_t FRAME_CTL_INPUT = offsetof(FrameCtl, T_cv);  // bytes the host uploads
 
 // One frame's deskew, thinning and registration start in one workgroup (k_frame_prep, lw_odom.hip).
 struct PrepArgs {
     const float *raw, *t;  // the frame: raw points and times
+    const float *xi_dev;   // given: the deskew twist on the device (in place of xi)
+    const int32_t *gate;   // given: the preparation runs only when *gate is nonzero
     const uint32_t *attr;
     float *xyz;            // the deskewed points
     const int32_t *m_dev;  // the frame's appended points (capped at frame_cap)
@@ -198,6 +211,7 @@ struct lw_ctx {
     double *partials = nullptr;
     uint32_t *loop_epoch = nullptr;  // the persistent registration's row-tag epoch (one word)
     lw::PrepSync *prep_sync = nullptr;  // the many-workgroup frame preparation's barrier and counts
+    lw::RegState *reg_fb = nullptr;     // lw_odom_frame_prior: the guess's registration, kept when the fallback runs
     uint64_t *loop_rows = nullptr;   // LOOP_TAGGED_ROWS x LOOP_ROW_WORDS: the persistent registration's tagged rows
     uint32_t reg_starts = 0;         // launches that may start a registration since the rows were last cleared
     int32_t *best_slot = nullptr;
@@ -446,9 +460,18 @@ hipError_t launch_keyframe_check(lw_ctx *c);
 // frame_prep_max() points. check: after the first registration -- the frame check first, then the rest
 // (deskew with the check's twist, the registration's start from its pose) only when it asks for a re-deskew.
 int32_t frame_prep_max();
+// lw_odom.hip: the learned prior in the frame chain: its guess into the chain's prediction (y: the network's
+// output on the device), the residual gate after the first registration, and the preparation gated on it
+// (deskew with the constant-velocity twist, thinning, the registration's start from the constant-velocity
+// prediction).
+hipError_t launch_prior_guess(lw_ctx *c, const float *y);
+hipError_t launch_prior_fallback(lw_ctx *c);
+hipError_t launch_frame_prep_fallback(lw_ctx *c, float t_mid, float inv_dt, float inv_v, float min_range,
+                                      float max_range, int32_t ret_mask);
 int32_t frame_prep_grid_max();  // the many-workgroup form's bound (launch_frame_prep takes it when c->prep_sync is set
                                 // and the frame's bound is within it)
 hipError_t launch_frame_prep(lw_ctx *c, const double *Tp, float t_mid, float inv_dt, const float *xi, float inv_v,
-                             float min_range, float max_range, int32_t ret_mask, bool check);
+                             float min_range, float max_range, int32_t ret_mask, bool check,
+                             const float *xi_dev = nullptr);
 
 }  // namespace lw

@@ -19,7 +19,9 @@ struct FeatArgs {
     float motion[MLP_MOTION];  // the motion inputs, already scaled
 };
 
-// The 152 inputs of one frame into out (device): motion from the arguments, histograms of xyz[0..m).
-hipError_t launch_features(hipStream_t s, const float *xyz, int32_t m, const FeatArgs &f, float *out);
+// The 152 inputs of one frame into out (device): motion from the arguments, histograms of xyz[0..m). m_dev,
+// when given, holds the frame's count on the device (taken up to cap) in place of m.
+hipError_t launch_features(hipStream_t s, const float *xyz, int32_t m, const int32_t *m_dev, int32_t cap,
+                           const FeatArgs &f, float *out);
 
 }  // namespace lw

@@ -482,8 +482,10 @@ __global__ void k_params_bf16(const float *w, uint16_t *wb, int32_t P) {
 // The 128 histogram inputs of one frame (points in the sensor frame): 64 range bins over [0, r_max) and 64
 // elevation bins over [-e_max, e_max), each as count * 64 / m (a flat histogram reads 1); out-of-range values
 // go to the end bins. The 24 motion inputs come with the arguments.
-__global__ __launch_bounds__(1024) void k_features(const float *xyz, int32_t m, FeatArgs f, float *out) {
+__global__ __launch_bounds__(1024) void k_features(const float *xyz, int32_t m, const int32_t *m_dev, int32_t cap,
+                                                   FeatArgs f, float *out) {
     __shared__ int32_t hr[64], he[64];
+    if (m_dev) m = min(*m_dev, cap);
     if (threadIdx.x < 64) hr[threadIdx.x] = he[threadIdx.x] = 0;
     __syncthreads();
     const float rs = 64.0f / f.r_max, es = 32.0f / f.e_max;
@@ -549,8 +551,9 @@ hipError_t gemm_pair(hipStream_t s, const Gemm &g1, const Gemm &g2) {
 
 }  // namespace
 
-hipError_t launch_features(hipStream_t s, const float *xyz, int32_t m, const FeatArgs &f, float *out) {
-    hipLaunchKernelGGL(k_features, dim3(1), dim3(1024), 0, s, xyz, m, f, out);
+hipError_t launch_features(hipStream_t s, const float *xyz, int32_t m, const int32_t *m_dev, int32_t cap,
+                           const FeatArgs &f, float *out) {
+    hipLaunchKernelGGL(k_features, dim3(1), dim3(1024), 0, s, xyz, m, m_dev, cap, f, out);
     return hipGetLastError();
 }
 

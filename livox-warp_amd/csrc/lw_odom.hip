@@ -187,6 +187,60 @@ __global__ void k_keyframe_check_seq(const KeySeq *D, int32_t S) {
     keyframe_decide(d.st, d.fc, d.ctr, d.counts);
 }
 
+// ---- the learned prior in the frame chain (lw_odom_frame_prior) --------------------------------------------
+//
+// LearnedPrior.guess and .accept, between the chain's steps. Before the first preparation (one lane): the
+// constant-velocity prediction and twist the host uploaded are kept (T_cv, xi_cv); when the frame holds at least
+// min_points points (guess() is asked only then), the network's output y becomes the increment
+// d = hist0 + y sd_t + mu_t and the first registration starts from prior_T exp(d) -- written into the chain's
+// prediction (tp) and T_pred -- with the points deskewed by d. After the first registration: a first residual
+// above the gate (fallback x the running median of earlier first residuals, taken by the host) keeps that
+// registration's state for the host and turns the frame back to the constant-velocity prediction and twist,
+// and the preparation and registration gated on fb run from there (odom.py's prediction, as _estimate's
+// fallback does).
+__global__ void k_prior_guess(FrameCtl *fc, const float *y, double *tp, const int32_t *m_dev, int32_t cap) {
+    for (int k = 0; k < 16; ++k) fc->T_cv[k] = fc->T_pred[k];
+    for (int k = 0; k < 6; ++k) fc->xi_cv[k] = fc->xi0[k];
+    fc->fb = 0;
+    const int32_t m = count_of(cap, m_dev);
+    fc->guess = fc->prior_on && m >= fc->min_points ? 1 : 0;
+    if (!fc->guess) return;
+    double d[6], R[9], Tw[16], Tg[16];
+    for (int k = 0; k < 6; ++k) d[k] = fc->hist0[k] + (double)y[k] * fc->sd_t[k] + fc->mu_t[k];
+    gn::rot_from_rotvec(d + 3, R);  // twist_matrix(d): the rotation of d's rotation vector, d's translation
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) Tw[4 * i + j] = R[3 * i + j];
+        Tw[4 * i + 3] = d[i];
+        Tw[12 + i] = 0.0;
+    }
+    Tw[15] = 1.0;
+    mat4_mul(fc->prior_T, Tw, Tg);
+    for (int k = 0; k < 16; ++k) {
+        tp[k] = Tg[k];
+        fc->T_pred[k] = Tg[k];
+    }
+    for (int k = 0; k < 6; ++k) {
+        fc->d[k] = d[k];
+        fc->xi0[k] = (float)d[k];
+    }
+}
+
+__global__ void k_prior_fallback(FrameCtl *fc, const RegState *st, RegState *st_fb, double *tp) {
+    __shared__ int32_t s_fb;
+    if (threadIdx.x == 0) s_fb = fc->guess && !(st->rms0 <= fc->rms_gate) ? 1 : 0;
+    __syncthreads();
+    if (!s_fb) return;
+    keep_state(st, st_fb);
+    if (threadIdx.x == 0) {
+        fc->fb = 1;
+        for (int k = 0; k < 16; ++k) {
+            tp[k] = fc->T_cv[k];
+            fc->T_pred[k] = fc->T_cv[k];
+        }
+        for (int k = 0; k < 6; ++k) fc->xi0[k] = fc->xi_cv[k];
+    }
+}
+
 // ---- one workgroup: deskew, thinning and the registration's start ------------------------------------
 //
 // A frame of at most PREP_MAX points is deskewed and thinned by one workgroup of PREP_THREADS threads in one
@@ -267,10 +321,10 @@ __device__ __forceinline__ void frame_prep(const PrepArgs &A) {
             for (int k = 0; k < 6; ++k) s_xi[k] = A.fc->xi1[k];
         }
     } else if (tid == 0) {
-        s_go = 1;
-        A.fc->err = 0;
+        s_go = A.gate ? (*A.gate != 0 ? 1 : 0) : 1;
+        if (!A.gate) A.fc->err = 0;
         for (int k = 0; k < 16; ++k) s_T[k] = A.Tp[k];
-        for (int k = 0; k < 6; ++k) s_xi[k] = A.xi[k];
+        for (int k = 0; k < 6; ++k) s_xi[k] = A.xi_dev ? A.xi_dev[k] : A.xi[k];
     }
     for (int s = tid; s < PREP_SLOTS; s += PREP_THREADS) {
         tkey[s] = -1;
@@ -467,10 +521,10 @@ __global__ __launch_bounds__(PG_THREADS) void k_frame_prep_grid(PrepArgs A, Prep
             for (int k = 0; k < 16; ++k) s_T[k] = c.T1[k];
             for (int k = 0; k < 6; ++k) s_xi[k] = c.xi1[k];
         } else {
-            s_go = 1;
-            if (g == 0) A.fc->err = 0;
+            s_go = A.gate ? (*A.gate != 0 ? 1 : 0) : 1;
+            if (g == 0 && !A.gate) A.fc->err = 0;
             for (int k = 0; k < 16; ++k) s_T[k] = A.Tp[k];
-            for (int k = 0; k < 6; ++k) s_xi[k] = A.xi[k];
+            for (int k = 0; k < 6; ++k) s_xi[k] = A.xi_dev ? A.xi_dev[k] : A.xi[k];
         }
     }
     __syncthreads();
@@ -624,10 +678,36 @@ PrepArgs prep_args(lw_ctx *c, const double *Tp, float t_mid, float inv_dt, const
 }  // namespace
 
 hipError_t launch_frame_prep(lw_ctx *c, const double *Tp, float t_mid, float inv_dt, const float *xi, float inv_v,
-                             float min_range, float max_range, int32_t ret_mask, bool check) {
-    const PrepArgs A = prep_args(c, Tp, t_mid, inv_dt, xi, inv_v, min_range, max_range, ret_mask, check);
+                             float min_range, float max_range, int32_t ret_mask, bool check, const float *xi_dev) {
+    PrepArgs A = prep_args(c, Tp, t_mid, inv_dt, xi, inv_v, min_range, max_range, ret_mask, check);
+    A.xi_dev = xi_dev;
     if (c->prep_sync && c->frame_bound > 0 && c->frame_bound <= frame_prep_grid_max()) {
         // one workgroup per PG_THREADS points the frame can hold
+        const int G = (int)std::min<int64_t>(PG_MAX_G, (c->frame_bound + PG_THREADS - 1) / PG_THREADS);
+        hipLaunchKernelGGL(k_frame_prep_grid, dim3(G), dim3(PG_THREADS), 0, c->stream, A, c->prep_sync);
+    } else {
+        hipLaunchKernelGGL(k_frame_prep, dim3(1), dim3(PREP_THREADS), 0, c->stream, A);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_prior_guess(lw_ctx *c, const float *y) {
+    hipLaunchKernelGGL(k_prior_guess, dim3(1), dim3(1), 0, c->stream, c->fctl, y, c->chain_tp, &c->ctr[6],
+                       (int32_t)c->frame_cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_prior_fallback(lw_ctx *c) {
+    hipLaunchKernelGGL(k_prior_fallback, dim3(1), dim3(256), 0, c->stream, c->fctl, c->reg, c->reg_fb, c->chain_tp);
+    return hipGetLastError();
+}
+
+hipError_t launch_frame_prep_fallback(lw_ctx *c, float t_mid, float inv_dt, float inv_v, float min_range,
+                                      float max_range, int32_t ret_mask) {
+    PrepArgs A = prep_args(c, c->chain_tp, t_mid, inv_dt, nullptr, inv_v, min_range, max_range, ret_mask, false);
+    A.xi_dev = c->fctl->xi0;
+    A.gate = &c->fctl->fb;
+    if (c->prep_sync && c->frame_bound > 0 && c->frame_bound <= frame_prep_grid_max()) {
         const int G = (int)std::min<int64_t>(PG_MAX_G, (c->frame_bound + PG_THREADS - 1) / PG_THREADS);
         hipLaunchKernelGGL(k_frame_prep_grid, dim3(G), dim3(PG_THREADS), 0, c->stream, A, c->prep_sync);
     } else {

@@ -338,17 +338,19 @@ class Odometry:
         warm = self.map_frames < self.warmup
         t_mid = (k + 0.5) * self.frame_dt
         chained = False
-        if warm or self.prior is not None:
-            m = min(self._frame_count(), self.frame_cap)
-            T, xi, ok = self._estimate(m, t_mid, warm)
-        elif self.chain:  # deskew, registrations, keyframe test and insert queued at once, one readback
+        # warm-up frames, and a prior that cannot take the device chain (one that only records), go step by step
+        stepwise = warm or (self.prior is not None and not hasattr(self.prior, "chain_inputs"))
+        if not stepwise and self.chain:  # deskew, registrations, keyframe test and insert queued at once
             res = self._frame_chain(t_mid)
             if res is not None:
                 T, xi, ok, m = res
                 chained = True
-        if not chained and not warm and self.prior is None:
-            # the frame's point count comes back with the registration: no read of it before
-            T, xi, ok, m = self._estimate_counted(t_mid)
+        if not chained:
+            if stepwise or self.prior is not None:
+                m = min(self._frame_count(), self.frame_cap)
+                T, xi, ok = self._estimate(m, t_mid, warm)
+            else:  # the frame's point count comes back with the registration: no read of it before
+                T, xi, ok, m = self._estimate_counted(t_mid)
         if not chained and ok and (warm or self._is_keyframe(T)):
             self._insert(m, T)
         if self.T_prev is not None and not warm:
@@ -397,10 +399,11 @@ class Odometry:
         return T, xi, ok
 
     def _frame_chain(self, t_mid: float):
-        """_estimate and the keyframe insert of a frame after the warm-up (no learned prior) as one device chain
-        (lw_odom_frame): the same decisions -- skip under min_points, the re-deskew test and second registration,
-        the keyframe test -- taken on the device between the steps. Returns (T, xi, ok, m), or None when the
-        device has no cooperative launch (the step-by-step path then runs this and every later frame)."""
+        """_estimate and the keyframe insert of a frame after the warm-up as one device chain (lw_odom_frame, or
+        lw_odom_frame_prior with the learned prior): the same decisions -- skip under min_points, the learned
+        guess and its residual test, the re-deskew test and second registration, the keyframe test -- taken on
+        the device between the steps. Returns (T, xi, ok, m), or None when the device has no persistent launch
+        (the step-by-step path then runs this and every later frame)."""
         fi, fo = self._fin, self._fout
         xi = self.xi if self.deskew else np.zeros(6, np.float32)
         T_pred = self.predict()
@@ -417,12 +420,19 @@ class Odometry:
         fi.t_mid, fi.inv_dt = float(t_mid), 1.0 / self.frame_dt
         fi.min_points = int(self.min_points)
         fi.redeskew = 1 if (self.deskew and self.redeskew) else 0
-        rc = self._lib.lw_odom_frame(self._ctx, _abi.ctypes.byref(self.reg_params()), _abi.ctypes.byref(fi),
-                                     _abi.ctypes.byref(fo))
+        fp = self.prior.chain_inputs(self) if self.prior is not None else None
+        if fp is None:
+            rc = self._lib.lw_odom_frame(self._ctx, _abi.ctypes.byref(self.reg_params()), _abi.ctypes.byref(fi),
+                                         _abi.ctypes.byref(fo))
+        else:  # the learned guess, its residual test and the fallback, in the chain
+            rc = self._lib.lw_odom_frame_prior(self._ctx, _abi.ctypes.byref(self.reg_params()),
+                                               _abi.ctypes.byref(fi), _abi.ctypes.byref(fp), _abi.ctypes.byref(fo))
         if rc == _abi.LW_ENOLOOP:
             self.chain = False
             return None
         _abi.check(rc, "lw_odom_frame")
+        if fp is not None:
+            self.prior.chain_result(self, fo)
         m = int(fo.m)
         if fo.skipped:
             self.stats["skipped"] += 1

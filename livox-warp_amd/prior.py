@@ -210,6 +210,11 @@ class LearnedPrior:
         self.hist = collections.deque(maxlen=4)   # increments, most recent first
         self.res = collections.deque(maxlen=window)
         self.feat = torch.zeros((1, IN), dtype=torch.float32, device=odo.device)
+        self._fp = _abi.FramePrior()  # the frame chain's prior block, and numpy views onto it
+        self._fp_T = np.ctypeslib.as_array(self._fp.T)
+        self._fp_h = np.ctypeslib.as_array(self._fp.hist0)
+        self._fp_sd = np.ctypeslib.as_array(self._fp.sd_t)
+        self._fp_mu = np.ctypeslib.as_array(self._fp.mu_t)
         self.last = None
         self.guess_err = {"learned": [], "cv": []}
 
@@ -233,6 +238,38 @@ class LearnedPrior:
     def _read(odo, y):
         odo.synchronize()
         return y.cpu().numpy()[0]
+
+    def chain_inputs(self, odo):
+        """guess() for the device frame chain (lw_odom_frame_prior): the frame's inputs and the forward pass are
+        queued on the odometry's stream (the frame's count stays on the device), and the chain forms the guess,
+        and takes the residual test, itself. Returns the chain's prior block, or None while fewer than four
+        increments are known (the frame then takes the constant-velocity chain)."""
+        if len(self.hist) < 4:
+            return None
+        mot = self.norm.motion(self.hist)
+        _abi.check(odo._lib.lw_frame_features(odo._ctx, -1, R_MAX, E_MAX, _abi.ptr(mot, _abi._f32p),
+                                              self.feat.data_ptr()), "lw_frame_features")
+        y = self.mlp.forward(self.feat)  # a reused buffer on the odometry's stream, read by the chain after it
+        fp = self._fp
+        fp.y = y.data_ptr()
+        self._fp_T[:] = odo.T.ravel()
+        self._fp_h[:] = self.hist[0]
+        self._fp_sd[:] = self.norm.sd_t
+        self._fp_mu[:] = self.norm.mu_t
+        # accept()'s test, with the window as it stands before this frame's residual joins it
+        fp.rms_gate = (self.fallback * float(np.median(self.res)) if len(self.res) >= self.min_history
+                       else math.inf)
+        return fp
+
+    def chain_result(self, odo, fo):
+        """The bookkeeping guess() and accept() do, from the chain's output fo."""
+        if not fo.guess:
+            return
+        self.last = (np.array(fo.d, dtype=np.float64), body_twist(odo.delta))
+        odo.totals["prior_used"] += 1
+        self.accept(float(fo.reg[0].rms0))  # the first residual joins the window
+        if fo.fallback:
+            odo.totals["prior_fallbacks"] += 1
 
     def accept(self, rms0: float) -> bool:
         ok = len(self.res) < self.min_history or rms0 <= self.fallback * float(np.median(self.res))

@@ -84,10 +84,10 @@ def test_thinning_matches_serial(lw, orc, xi, gpu_ok):
 @pytest.mark.parametrize("case", ["scan", "scattered", "large"])
 def test_frame_chain_preparation(lw, orc, case, gpu_ok):
     """The deskew and thinning lw_odom_frame runs before its first registration: for a frame of up to 32768
-    points one workgroup does both (its thinning table in LDS; "scattered" holds more thinning voxels than that
-    table, so the workgroup takes the global table instead), for a larger frame ("large") the grid kernels.
-    The deskewed frame is the grid deskew kernel's bit for bit, the thinned points are the oracle's serial
-    thinning (winners and order) bit for bit, and the global thinning table is left clean."""
+    points one launch over many workgroups does both (claims in the global thinning table, two grid barriers;
+    "scattered" holds more thinning voxels than a one-workgroup LDS table), for a larger frame ("large") the
+    grid kernels. The deskewed frame is the grid deskew kernel's bit for bit, the thinned points are the
+    oracle's serial thinning (winners and order) bit for bit, and the global thinning table is left clean."""
     import ctypes
 
     rng = np.random.default_rng({"scan": 1, "scattered": 2, "large": 3}[case])
