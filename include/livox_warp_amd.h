@@ -181,8 +181,9 @@ int lw_odom_frame(lw_ctx *ctx, const lw_reg_params *p, const lw_frame_in *in, lw
  * first), with the points deskewed by d; if its first residual (rms0) exceeds rms_gate the frame is deskewed with
  * the constant-velocity twist and registered from the constant-velocity prediction instead; the re-deskew test,
  * the keyframe test and the insert follow as in lw_odom_frame. Up to three registrations (out->reg in order);
- * out->guess, out->fallback and out->d report the prior's part. No counterpart in the reference (the learned
- * prior is this build's extension). */
+ * out->guess, out->fallback and out->d report the prior's part. A frame beyond the one-launch preparation
+ * (more than 32768 points appended, or a thinning table under 4096 slots) returns LW_ERANGE with nothing queued.
+ * No counterpart in the reference (the learned prior is this build's extension). */
 typedef struct {
     const float *y;        /* device: the network's output for this frame (6 float32) */
     double T[16];          /* the previous frame's pose */

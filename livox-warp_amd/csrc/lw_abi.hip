@@ -699,6 +699,8 @@ int odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, const l
                lw_frame_out *out) {
     if (!c || bad_params(p) || !in || !out) return LW_EINVAL;
     if (pr && !pr->y) return LW_EINVAL;
+    // the prior's chain prepares the frame in one launch: a larger frame is refused before anything is queued
+    if (pr && !(c->frame_bound <= lw::frame_prep_max() && c->ds_cap >= 4096)) return LW_ERANGE;
     if (!c->f_raw) return LW_ENOTBOUND;
     if (!c->loop_ok) return LW_ENOLOOP;
     LW_CK(frame_count_ready(c));
@@ -741,10 +743,7 @@ int odom_frame(lw_ctx *c, const lw_reg_params *p, const lw_frame_in *in, const l
     // slots): deskew, thinning and the registration's start in one launch each time
     const bool one = c->frame_bound <= lw::frame_prep_max() && c->ds_cap >= 4096;
     LW_CK(lw::loop_epoch_guard(c, pr ? 3 : 2));  // the registrations' starts below
-    if (pr) {
-        if (!one) return LW_EINVAL;  // the prior's chain prepares the frame in one launch
-        LW_CK(lw::launch_prior_guess(c, pr->y));  // the learned guess into the prediction and the twist
-    }
+    if (pr) LW_CK(lw::launch_prior_guess(c, pr->y));  // the learned guess into the prediction and the twist
     // odom.py:671-681: deskew with the model's twist, register from the prediction
     if (one) {
         LW_CK(lw::launch_frame_prep(c, c->chain_tp, in->t_mid, in->inv_dt, in->xi, inv_v_thin, p->min_range,

@@ -427,6 +427,8 @@ class Odometry:
         else:  # the learned guess, its residual test and the fallback, in the chain
             rc = self._lib.lw_odom_frame_prior(self._ctx, _abi.ctypes.byref(self.reg_params()),
                                                _abi.ctypes.byref(fi), _abi.ctypes.byref(fp), _abi.ctypes.byref(fo))
+            if rc == _abi.LW_ERANGE:  # a frame beyond the one-launch preparation: this one goes step by step
+                return None
         if rc == _abi.LW_ENOLOOP:
             self.chain = False
             return None
