@@ -216,7 +216,11 @@ def test_lifecycle_gap_sparse_and_jump_match_the_oracle(lw, orc, gpu_ok):
         rd, ro = rows["dev"][i], rows["ora"][i]
         assert rd[0] == ro[0] and rd[1] == ro[1] and rd[4] == ro[4]
         assert rd[2]["skipped"] == ro[2]["skipped"] and rd[2]["keyframes"] == ro[2]["keyframes"]
-        assert rd[2]["ds"] == ro[2]["ds"] and rd[2]["iters"] == ro[2]["iters"]
+        # the two maps are built apart, and the device's moments are float atomics whose order varies from run to
+        # run (two device runs of this sequence end a frame with residuals 2e-4 apart): a step's size can fall on
+        # either side of the convergence test, so the counts may part by one step; the replay below holds every
+        # device registration to the oracle's on the device's own map
+        assert rd[2]["ds"] == ro[2]["ds"] and abs(rd[2]["iters"] - ro[2]["iters"]) <= 1
         assert np.abs(rd[3] - ro[3]).max() < 1e-6
     pairs = [_err(traj_d[i][2], traj_o[i][2]) for i in before]
     worst = (max(e[0] for e in pairs), max(e[1] for e in pairs))
