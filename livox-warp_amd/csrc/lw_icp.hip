@@ -1086,9 +1086,13 @@ __global__ __launch_bounds__(ICP_BLOCK) void k_register(LoopArgs L) {
 // sequence's whole Gauss-Newton loop. Per pass the waves take the point groups wave, wave + SEQ_WAVES, ...
 // (wave_pass), their rows meet in LDS and are added in wave order (the first RED_BATCH rows of the
 // persistent loop's reduction, in the same arithmetic), and wave 0 takes the float64 step while the others
-// wait at the workgroup barrier. The rows of the four waves use most of the CU's LDS, so a CU runs one
-// sequence at a time and the grid is the batch.
-constexpr int SEQ_WAVES = 4;
+// wait at the workgroup barrier. A pass is bound by the latency of each lane group's dependent gathers, so
+// the workgroup runs two waves a SIMD: eight waves of at most 256 registers, each lane taking its keys in
+// chunks of 8 at the 5x5x5 reach and 7 at the 3x3x3 (chunks of 16 need more registers than a wave of the eight
+// has). The rows of the eight waves use most of the CU's LDS, so a CU runs one sequence at a time and the grid
+// is the batch.
+constexpr int SEQ_WAVES = 8;
+constexpr int SEQ_C2 = 8, SEQ_C1 = 7;  // keys a chunk at the 5x5x5 and 3x3x3 reaches (lanes: Split<R>::S)
 static_assert(SEQ_WAVES <= RED_BATCH, "the wave rows are one batch of the reduction");
 static_assert(gnw::W_DOUBLES * sizeof(double) <= RED_BYTES, "the step's scratch fits one wave's rows");
 
@@ -1135,11 +1139,11 @@ __global__ __launch_bounds__(64 * SEQ_WAVES) void k_register_seq(const SeqReg *D
         // (k_loc's form) measured 3% slower here -- a workgroup's passes are bound by the latency of its
         // dependent loads, not by their bytes
         if (reach == 2)
-            wave_pass<MODE, 2>(a, p, Tf, nbr, wave, SEQ_WAVES, lds[wave], sum, comp, hits, matched, {}, false, false,
-                               t_found, t_terms);
+            wave_pass<MODE, 2, Split<2>::S, SEQ_C2>(a, p, Tf, nbr, wave, SEQ_WAVES, lds[wave], sum, comp, hits,
+                                                    matched, {}, false, false, t_found, t_terms);
         else
-            wave_pass<MODE, 1>(a, p, Tf, nbr, wave, SEQ_WAVES, lds[wave], sum, comp, hits, matched, {}, false, false,
-                               t_found, t_terms);
+            wave_pass<MODE, 1, Split<1>::S, SEQ_C1>(a, p, Tf, nbr, wave, SEQ_WAVES, lds[wave], sum, comp, hits,
+                                                    matched, {}, false, false, t_found, t_terms);
         LW_UNROLL for (int off = 32; off > 0; off >>= 1) {
             hits += __shfl_xor(hits, off, 64);
             matched += __shfl_xor(matched, off, 64);
