@@ -49,7 +49,7 @@ void free_all(lw_ctx *c) {
     void *ptrs[] = {c->slots, c->occ, c->vox_key, c->vox_mom, c->vox_slot, c->vox_prec, c->vox_nin, c->vox_quota,
                     c->vox_dirty, c->dirty_list, c->touched, c->ctr, c->pt_slot, c->pt_vid, c->pt_take,
                     c->ds_keys, c->ds_idx, c->ds_xyz, c->blk_cnt, c->blk_off, c->chain, c->partials, c->best_slot,
-                    c->dtmp, c->loop_epoch, c->loop_rows, c->prep_sync};
+                    c->dtmp, c->loop_epoch, c->loop_rows, c->prep_sync, c->nb_occ};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->prof_dev) (void)hipFree(c->prof_dev);
@@ -616,6 +616,16 @@ int lw_loc_batch(lw_ctx *c, const float *scans, const int32_t *scan_off, const i
     for (int it = 0; it < 16; ++it) {  // the odometry's robust-kernel schedule (odom.py:767)
         const double kap = 6.0 * std::pow(0.7, it);
         L.kappa[it] = (float)(kap > 2.0 ? kap : 2.0);
+    }
+    // the neighbourhood bitmap the kernel tests before each point's search: built on first use and again after
+    // any change of the map, before the launch (and outside a captured graph)
+    if (!c->nb_occ) {
+        LW_CK(dalloc(&c->nb_occ, c->occ_words));
+        c->nb_stale = true;
+    }
+    if (c->nb_stale) {
+        LW_CK(lw::launch_nb_build(c));
+        c->nb_stale = false;
     }
     const bool timed = c->prof_on && c->prof_used < c->prof_cap;
     if (timed) LW_CK(hipEventRecord(c->prof_ev[2 * c->prof_used], c->stream));

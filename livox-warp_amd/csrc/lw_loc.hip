@@ -35,6 +35,7 @@ struct LocArgs {
     int32_t *out_iters;
     const Slot *slots;
     const uint32_t *occ;
+    const uint32_t *nb_occ;  // the neighbourhood bitmap (launch_nb_build)
     const VoxPrec *vox_prec;
     uint32_t *queue;
     unsigned long long *prof;
@@ -108,6 +109,11 @@ __global__ __launch_bounds__(B) void k_loc(LocArgs a) {
                 const float *pp = a.scans + 3 * (int64_t)(p0 + i);
                 float wx, wy, wz;
                 transform_point(Tf, pp[0], pp[1], pp[2], wx, wy, wz);
+                // a clear bit in the neighbourhood bitmap: no voxel of the map among the 27, so the search would
+                // find nothing (no hit, no match) and the point adds nothing
+                const int32_t h0 =
+                    slot_hash(voxel_key_ijk(ifloor(wx * a.inv_v), ifloor(wy * a.inv_v), ifloor(wz * a.inv_v))) & a.mask;
+                if (((a.nb_occ[h0 >> 5] >> (h0 & 31)) & 1u) == 0u) continue;
                 Cand best{a.max_d2, -1, 27, 0.0f, 0.0f, 0.0f};
                 search_part<1, C, true>(Table{a.slots, a.occ, a.mask}, nbr, a.inv_v, wx, wy, wz, 0, 27, best, hits);
                 const int32_t s = best.s;
@@ -221,6 +227,7 @@ hipError_t launch_loc(lw_ctx *c, const LocLaunch &L) {
     a.out_iters = L.out_iters;
     a.slots = c->slots;
     a.occ = c->occ;
+    a.nb_occ = c->nb_occ;
     a.vox_prec = c->vox_prec;
     a.queue = c->loc_queue;
     a.prof = L.prof;

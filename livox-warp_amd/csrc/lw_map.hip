@@ -402,6 +402,25 @@ MapRefs map_refs(lw_ctx *c) {
     return M;
 }
 
+// The localisation's neighbourhood bitmap: each voxel id handed out (ctr[2] of them, at most vox_cap) sets the
+// bits of its 27 neighbours' keys, formed as voxel_key_ijk forms them (the axis indices decoded from the key,
+// so the 21-bit wrap is the search's). A key whose bit is clear has no voxel of the map among its own 27
+// neighbours; a set bit may be another key's (then the search runs as it would without the bitmap).
+__global__ __launch_bounds__(BLOCK) void k_nb_build(const int64_t *vox_key, const int32_t *ctr, int32_t vox_cap,
+                                                   int32_t mask, uint32_t *nb) {
+    const int32_t V = min(ctr[2], vox_cap);
+    for (int32_t v = blockIdx.x * BLOCK + threadIdx.x; v < V; v += gridDim.x * BLOCK) {
+        int32_t ix, iy, iz;
+        voxel_key_unpack(vox_key[v], ix, iy, iz);
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const int32_t h = slot_hash(voxel_key_ijk(ix + dx, iy + dy, iz + dz)) & mask;
+                    atomicOr(&nb[h >> 5], 1u << (h & 31));
+                }
+    }
+}
+
 namespace {
 
 // Workgroups for n items, capped at max_grid: every kernel launched with it strides over its items by the grid.
@@ -420,6 +439,7 @@ hipError_t launch_fill_i32(lw_ctx *c, int32_t *p, int64_t n, int32_t v) {
 }
 
 hipError_t launch_map_reset(lw_ctx *c) {
+    c->nb_stale = true;
     hipLaunchKernelGGL(k_clear_slots, dim3(grid_for(c->cap, BLOCK, 8192)), dim3(BLOCK), 0, c->stream, c->slots,
                        c->cap);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(c->occ_words, BLOCK, 4096)), dim3(BLOCK), 0, c->stream, c->occ,
@@ -435,8 +455,17 @@ hipError_t launch_map_reset(lw_ctx *c) {
     return hipGetLastError();
 }
 
+hipError_t launch_nb_build(lw_ctx *c) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(c->occ_words, BLOCK, 4096)), dim3(BLOCK), 0, c->stream, c->nb_occ,
+                       c->occ_words, 0u);
+    hipLaunchKernelGGL(k_nb_build, dim3(grid_for(c->vox_cap, BLOCK, 4096)), dim3(BLOCK), 0, c->stream, c->vox_key,
+                       c->ctr, (int32_t)c->vox_cap, c->mask, c->nb_occ);
+    return hipGetLastError();
+}
+
 hipError_t launch_insert_head(lw_ctx *c, const InsertArgs &a) {
     if (a.n <= 0) return hipSuccess;
+    c->nb_stale = true;
     const int g = grid_for(a.n);
     const MapRefs M = map_refs(c);
     hipLaunchKernelGGL(k_ins_claim, dim3(g), dim3(BLOCK), 0, c->stream, M, a.xyz, a.attr, a.n, a.n_dev, a.T, a.inv_v,

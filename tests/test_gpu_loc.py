@@ -93,3 +93,30 @@ def test_chunk7_tiles_do_not_spill_and_the_autotuner_skips_spilling_ones(problem
     assert tile not in spilled and all(f"{b}x{g}x{c}" not in table for b, g, c in spilled)
     assert len(table) + len(spilled) == len(lw.loc.TILE_CANDIDATES)
     print(f"tiles: chose {tile}, skipped for spilling {sorted(set((b, c) for b, _, c in spilled))}", flush=True)
+
+
+def test_map_change_rebuilds_the_neighbourhood_bitmap(problem, lw, gpu_ok):
+    """A point with no voxel of the map among its 27 skips the search on a bitmap of the map's neighbourhoods,
+    and the bitmap is rebuilt when the map changes: a localiser built on the half of the map with x below the
+    median, run from its captured graph, then given the other half and run again from the same graph, equals the
+    oracle on the whole map (the first run does not: its scans reach voxels the half-map lacks)."""
+    loc_full, scans, hyps, want = problem
+    ss = lw.synth.single_scan(seed=5, n=6000, map_points=150_000)
+    x = np.asarray(ss.map_xyz, np.float32)
+    west = x[:, 0] < np.median(x[:, 0])
+    loc = lw.loc.BatchLocalizer(x[west], voxel=VOXEL, reg_slots=SLOTS, chunk=1 << 18)
+    try:
+        staged = loc._stage(scans, hyps)
+        first = [t.clone() for t in loc.run(scans, hyps, tile=(128, 4, 14), use_graph=True, staged=staged)]
+        loc.stream.synchronize()
+        with torch.cuda.stream(loc.stream):
+            loc.eng.insert(torch.from_numpy(np.ascontiguousarray(x[~west])).cuda(), None, np.eye(4), VOXEL,
+                           max_range=1e30)
+        loc.eng.sync()
+        assert loc.map_stats() == loc_full.map_stats()
+        got = loc.run(scans, hyps, tile=(128, 4, 14), use_graph=True, staged=staged)
+        loc.stream.synchronize()
+        _check(got, want)
+        assert not all(torch.equal(a, b) for a, b in zip(first, got))
+    finally:
+        loc.close()
