@@ -70,6 +70,14 @@ __device__ inline bool loc_step(const double *v, double damping, int32_t min_cou
     return true;
 }
 
+// The sum of the first k entries of c (k <= W), in order.
+template <int W>
+__device__ inline int32_t wave_prefix(const int32_t *c, int k) {
+    int32_t t = 0;
+    for (int w = 0; w < k; ++w) t += c[w];
+    return t;
+}
+
 template <int B, int C>
 __global__ __launch_bounds__(B) void k_loc(LocArgs a) {
     constexpr int W = B / 64;
@@ -78,6 +86,8 @@ __global__ __launch_bounds__(B) void k_loc(LocArgs a) {
     __shared__ double Td[16];
     __shared__ float Tf[16];
     __shared__ int32_t s_unit, s_stop;
+    __shared__ int32_t q_idx[2 * B];  // points waiting for their search (fewer than 2B at any time)
+    __shared__ int32_t wcnt[W];
     __shared__ NbrOff nbr[NBR_ROWS];
     nbr_fill(nbr);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -105,15 +115,47 @@ __global__ __launch_bounds__(B) void k_loc(LocArgs a) {
             const float kappa = a.kappa[it < 16 ? it : 15];
             double acc[LOC_ACC];
             LW_UNROLL for (int k = 0; k < LOC_ACC; ++k) acc[k] = 0.0;
-            for (int32_t i = threadIdx.x; i < n; i += B) {
+            // The workgroup's points that the neighbourhood bitmap passes are queued in LDS, B at a time in
+            // point order (wave by wave, lanes in order), so every lane of a round searches a point that can
+            // match. A round tests the next B points and appends; once B are queued (or every point is tested)
+            // the queue's first B (or all) are searched, one a lane, and the rest move to the front.
+            int32_t base = 0, qn = 0;
+            for (;;) {
+                if (base < n) {
+                    const int32_t i = base + (int32_t)threadIdx.x;
+                    bool act = false;
+                    if (i < n) {
+                        const float *pp = a.scans + 3 * (int64_t)(p0 + i);
+                        float wx, wy, wz;
+                        transform_point(Tf, pp[0], pp[1], pp[2], wx, wy, wz);
+                        // a clear bit: no voxel of the map among the 27, so the search would find nothing
+                        const int32_t h0 = slot_hash(voxel_key_ijk(ifloor(wx * a.inv_v), ifloor(wy * a.inv_v),
+                                                                   ifloor(wz * a.inv_v))) &
+                                           a.mask;
+                        act = ((a.nb_occ[h0 >> 5] >> (h0 & 31)) & 1u) != 0u;
+                    }
+                    const uint64_t bal = __ballot(act);
+                    if (lane == 0) wcnt[wave] = __popcll(bal);
+                    __syncthreads();
+                    if (act) q_idx[qn + wave_prefix<W>(wcnt, wave) + __popcll(bal & ((1ull << lane) - 1ull))] = i;
+                    qn += wave_prefix<W>(wcnt, W);
+                    base += B;
+                    __syncthreads();
+                }
+                const bool flush = base >= n;
+                if (qn == 0 && flush) break;
+                if (qn < B && !flush) continue;
+                const int32_t take = qn < B ? qn : B, rest = qn - take;
+                const int32_t i = (int32_t)threadIdx.x < take ? q_idx[threadIdx.x] : -1;
+                const int32_t moved = (int32_t)threadIdx.x < rest ? q_idx[take + threadIdx.x] : 0;
+                __syncthreads();
+                if ((int32_t)threadIdx.x < rest) q_idx[threadIdx.x] = moved;
+                qn = rest;
+                __syncthreads();  // the moved entries in place before the next round reads or appends
+                if (i < 0) continue;
                 const float *pp = a.scans + 3 * (int64_t)(p0 + i);
                 float wx, wy, wz;
                 transform_point(Tf, pp[0], pp[1], pp[2], wx, wy, wz);
-                // a clear bit in the neighbourhood bitmap: no voxel of the map among the 27, so the search would
-                // find nothing (no hit, no match) and the point adds nothing
-                const int32_t h0 =
-                    slot_hash(voxel_key_ijk(ifloor(wx * a.inv_v), ifloor(wy * a.inv_v), ifloor(wz * a.inv_v))) & a.mask;
-                if (((a.nb_occ[h0 >> 5] >> (h0 & 31)) & 1u) == 0u) continue;
                 Cand best{a.max_d2, -1, 27, 0.0f, 0.0f, 0.0f};
                 search_part<1, C, true>(Table{a.slots, a.occ, a.mask}, nbr, a.inv_v, wx, wy, wz, 0, 27, best, hits);
                 const int32_t s = best.s;
