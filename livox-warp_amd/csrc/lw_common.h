@@ -96,6 +96,12 @@ __host__ __device__ inline int32_t slot_hash(int64_t k) {
 
 __device__ inline int32_t ifloor(float x) { return (int32_t)floorf(x); }
 
+// A second hash of a voxel key, independent of slot_hash (the high bits of another odd multiplier): the
+// neighbourhood bitmap sets and tests two bits a key.
+__host__ __device__ inline int32_t nb_hash2(int64_t k) {
+    return (int32_t)(((uint64_t)k * 0xD6E8FEB86659FD93ull) >> 33);
+}
+
 // A launch whose work a device flag decides: it runs unless gate is given and holds 0.
 __device__ inline bool gated_off(const int32_t *gate) { return gate && *gate == 0; }
 

@@ -129,10 +129,10 @@ __global__ __launch_bounds__(B) void k_loc(LocArgs a) {
                         float wx, wy, wz;
                         transform_point(Tf, pp[0], pp[1], pp[2], wx, wy, wz);
                         // a clear bit: no voxel of the map among the 27, so the search would find nothing
-                        const int32_t h0 = slot_hash(voxel_key_ijk(ifloor(wx * a.inv_v), ifloor(wy * a.inv_v),
-                                                                   ifloor(wz * a.inv_v))) &
-                                           a.mask;
-                        act = ((a.nb_occ[h0 >> 5] >> (h0 & 31)) & 1u) != 0u;
+                        const int64_t k0 =
+                            voxel_key_ijk(ifloor(wx * a.inv_v), ifloor(wy * a.inv_v), ifloor(wz * a.inv_v));
+                        const int32_t h0 = slot_hash(k0) & a.mask, h1 = nb_hash2(k0) & a.mask;
+                        act = (((a.nb_occ[h0 >> 5] >> (h0 & 31)) & (a.nb_occ[h1 >> 5] >> (h1 & 31))) & 1u) != 0u;
                     }
                     const uint64_t bal = __ballot(act);
                     if (lane == 0) wcnt[wave] = __popcll(bal);

@@ -402,10 +402,11 @@ MapRefs map_refs(lw_ctx *c) {
     return M;
 }
 
-// The localisation's neighbourhood bitmap: each voxel id handed out (ctr[2] of them, at most vox_cap) sets the
-// bits of its 27 neighbours' keys, formed as voxel_key_ijk forms them (the axis indices decoded from the key,
-// so the 21-bit wrap is the search's). A key whose bit is clear has no voxel of the map among its own 27
-// neighbours; a set bit may be another key's (then the search runs as it would without the bitmap).
+// The localisation's neighbourhood bitmap: each voxel id handed out (ctr[2] of them, at most vox_cap) sets two
+// bits (slot_hash and nb_hash2, each & mask) for each of its 27 neighbours' keys, formed as voxel_key_ijk forms
+// them (the axis indices decoded from the key, so the 21-bit wrap is the search's). A key with either bit clear
+// has no voxel of the map among its own 27 neighbours; both bits set may be other keys' (then the search runs as
+// it would without the bitmap).
 __global__ __launch_bounds__(BLOCK) void k_nb_build(const int64_t *vox_key, const int32_t *ctr, int32_t vox_cap,
                                                    int32_t mask, uint32_t *nb) {
     const int32_t V = min(ctr[2], vox_cap);
@@ -415,8 +416,10 @@ __global__ __launch_bounds__(BLOCK) void k_nb_build(const int64_t *vox_key, cons
         for (int dx = -1; dx <= 1; ++dx)
             for (int dy = -1; dy <= 1; ++dy)
                 for (int dz = -1; dz <= 1; ++dz) {
-                    const int32_t h = slot_hash(voxel_key_ijk(ix + dx, iy + dy, iz + dz)) & mask;
+                    const int64_t k = voxel_key_ijk(ix + dx, iy + dy, iz + dz);
+                    const int32_t h = slot_hash(k) & mask, h2 = nb_hash2(k) & mask;
                     atomicOr(&nb[h >> 5], 1u << (h & 31));
+                    atomicOr(&nb[h2 >> 5], 1u << (h2 & 31));
                 }
     }
 }
