@@ -6,8 +6,9 @@
 // (A = H + diag(1e-3 diag H + 1e-9), solve, left twist; skipped below 30 matches; localize.py:531-547).
 //
 // Work: (query, hypothesis) units, queries longest first; a workgroup takes a unit from an atomic queue and
-// runs all its iterations alone: every lane takes points of the query scan (grid-stride), searches the 27
-// voxels around each (bitmap words of a chunk loaded together, then its slots), forms the float32 terms of
+// runs all its iterations alone: the lanes test the query scan's points against the map's neighbourhood bitmap
+// (launch_nb_build) and queue those that can match; every lane then takes a queued point, searches the 27
+// voxels around it (bitmap words of a chunk loaded together, then its slots), forms the float32 terms of
 // H and b as the reference rounds them and adds them, widened, to float64 running sums; the workgroup
 // reduces (wave shuffles, then LDS) and one lane solves the 6x6 step in float64. Units never wait on each
 // other, so the grid needs no co-residency; the grid size, the block size and the chunk (neighbour keys a lane
