@@ -258,7 +258,7 @@ typedef struct lw_loc_params {
  * (longest first balances the work); hyp_T / out_T: n_queries * n_hyp row-major 4x4 float64 (query-major);
  * out_score: sum of robust weights of the last pass; out_count: its matched points; out_iters: steps taken.
  * Every array is a DEVICE pointer. Queued on the context's stream; it does not wait. The first call allocates
- * the map's neighbourhood bitmap (one bit per slot), and a call after any change of the map rebuilds it on the
+ * the map's neighbourhood bitmap (two bits per slot), and a call after any change of the map rebuilds it on the
  * stream before the launch (outside a captured graph). */
 int lw_loc_batch(lw_ctx *ctx, const float *scans, const int32_t *scan_off, const int32_t *order, int32_t n_queries,
                  const double *hyp_T, int32_t n_hyp, const lw_loc_params *p, double *out_T, double *out_score,

@@ -620,7 +620,7 @@ int lw_loc_batch(lw_ctx *c, const float *scans, const int32_t *scan_off, const i
     // the neighbourhood bitmap the kernel tests before each point's search: built on first use and again after
     // any change of the map, before the launch (and outside a captured graph)
     if (!c->nb_occ) {
-        LW_CK(dalloc(&c->nb_occ, c->occ_words));
+        LW_CK(dalloc(&c->nb_occ, lw::NB_SCALE * c->occ_words));
         c->nb_stale = true;
     }
     if (c->nb_stale) {

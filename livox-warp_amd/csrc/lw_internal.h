@@ -241,8 +241,8 @@ struct lw_ctx {
     hipGraph_t loc_graph = nullptr;
     hipGraphExec_t loc_exec = nullptr;
     unsigned char loc_key[512] = {0};
-    // the localisation's neighbourhood bitmap (launch_nb_build), allocated by the first localisation launch and
-    // rebuilt before one whenever the map changed since the last build
+    // the localisation's neighbourhood bitmap (launch_nb_build, NB_SCALE bits a slot), allocated by the first
+    // localisation launch and rebuilt before one whenever the map changed since the last build
     uint32_t *nb_occ = nullptr;
     bool nb_stale = true;
     // the persistent registration: co-resident workgroups per accumulation form (0 = not yet queried), and
@@ -298,8 +298,9 @@ __device__ inline void reg_state_begin(RegState *st, const double *P, const floa
 }
 
 hipError_t launch_map_reset(lw_ctx *c);
-// The neighbourhood bitmap c->nb_occ (occ_words words): cleared, then bit slot_hash(k) & mask set for every key k
-// within one voxel on each axis of a voxel of the map.
+constexpr int64_t NB_SCALE = 2;  // bits of the neighbourhood bitmap per map slot
+// The neighbourhood bitmap c->nb_occ (NB_SCALE * occ_words words): cleared, then, for every key k within one voxel
+// on each axis of a voxel of the map, the bits slot_hash(k) and nb_hash2(k), each & (NB_SCALE * cap - 1), set.
 hipError_t launch_nb_build(lw_ctx *c);
 // n_dev: when given, the points to insert are its count capped at n (n sizes the grids). gate: when given, a
 // device flag that must be nonzero for the insert to happen. T_dev: the pose on the device (replaces T).

@@ -37,6 +37,7 @@ struct LocArgs {
     const Slot *slots;
     const uint32_t *occ;
     const uint32_t *nb_occ;  // the neighbourhood bitmap (launch_nb_build)
+    int32_t nb_mask;         // its bits - 1
     const VoxPrec *vox_prec;
     uint32_t *queue;
     unsigned long long *prof;
@@ -132,7 +133,7 @@ __global__ __launch_bounds__(B) void k_loc(LocArgs a) {
                         // a clear bit: no voxel of the map among the 27, so the search would find nothing
                         const int64_t k0 =
                             voxel_key_ijk(ifloor(wx * a.inv_v), ifloor(wy * a.inv_v), ifloor(wz * a.inv_v));
-                        const int32_t h0 = slot_hash(k0) & a.mask, h1 = nb_hash2(k0) & a.mask;
+                        const int32_t h0 = slot_hash(k0) & a.nb_mask, h1 = nb_hash2(k0) & a.nb_mask;
                         act = (((a.nb_occ[h0 >> 5] >> (h0 & 31)) & (a.nb_occ[h1 >> 5] >> (h1 & 31))) & 1u) != 0u;
                     }
                     const uint64_t bal = __ballot(act);
@@ -271,6 +272,7 @@ hipError_t launch_loc(lw_ctx *c, const LocLaunch &L) {
     a.slots = c->slots;
     a.occ = c->occ;
     a.nb_occ = c->nb_occ;
+    a.nb_mask = (int32_t)(NB_SCALE * c->cap - 1);
     a.vox_prec = c->vox_prec;
     a.queue = c->loc_queue;
     a.prof = L.prof;

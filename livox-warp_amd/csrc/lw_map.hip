@@ -402,11 +402,11 @@ MapRefs map_refs(lw_ctx *c) {
     return M;
 }
 
-// The localisation's neighbourhood bitmap: each voxel id handed out (ctr[2] of them, at most vox_cap) sets two
-// bits (slot_hash and nb_hash2, each & mask) for each of its 27 neighbours' keys, formed as voxel_key_ijk forms
-// them (the axis indices decoded from the key, so the 21-bit wrap is the search's). A key with either bit clear
-// has no voxel of the map among its own 27 neighbours; both bits set may be other keys' (then the search runs as
-// it would without the bitmap).
+// The localisation's neighbourhood bitmap (NB_SCALE bits a map slot; mask = its bits - 1): each voxel id handed
+// out (ctr[2] of them, at most vox_cap) sets two bits, slot_hash and nb_hash2 (each & mask), for each of its 27
+// neighbours' keys, formed as voxel_key_ijk forms them (the axis indices decoded from the key, so the 21-bit wrap
+// is the search's). A key with either bit clear has no voxel of the map among its own 27 neighbours; both bits
+// set may be other keys' (then the search runs as it would without the bitmap).
 __global__ __launch_bounds__(BLOCK) void k_nb_build(const int64_t *vox_key, const int32_t *ctr, int32_t vox_cap,
                                                    int32_t mask, uint32_t *nb) {
     const int32_t V = min(ctr[2], vox_cap);
@@ -459,10 +459,10 @@ hipError_t launch_map_reset(lw_ctx *c) {
 }
 
 hipError_t launch_nb_build(lw_ctx *c) {
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(c->occ_words, BLOCK, 4096)), dim3(BLOCK), 0, c->stream, c->nb_occ,
-                       c->occ_words, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(NB_SCALE * c->occ_words, BLOCK, 4096)), dim3(BLOCK), 0, c->stream,
+                       c->nb_occ, NB_SCALE * c->occ_words, 0u);
     hipLaunchKernelGGL(k_nb_build, dim3(grid_for(c->vox_cap, BLOCK, 4096)), dim3(BLOCK), 0, c->stream, c->vox_key,
-                       c->ctr, (int32_t)c->vox_cap, c->mask, c->nb_occ);
+                       c->ctr, (int32_t)c->vox_cap, (int32_t)(NB_SCALE * c->cap - 1), c->nb_occ);
     return hipGetLastError();
 }
 
